@@ -1,0 +1,150 @@
+"""Triangle meshes for the BEM engine: seeded icospheres, ellipsoids, OFF files.
+
+A mesh is a pair of arrays, ``nodes`` (n_nodes x 3, float64) and ``tris``
+(n_tri x 3, int32), elements numbered by row (NiHu numbers elements by
+insertion order, nihu/core/mesh.hpp:137-193).  Triangles are oriented so that
+(X1 - X0) x (X2 - X0) points outwards.
+
+Icosphere: the 20 faces of an icosahedron are subdivided ``level`` times
+depth first (each parent's four children are consecutive, so consecutive
+element indices form compact patches), with edge midpoints projected to the
+unit sphere.  Levels 3/4/5 give 1280/5120/20480 triangles and 642/2562/10242
+vertices.  Radial jitter: every vertex is scaled by ``1 + U(-jitter, jitter)``
+drawn from ``numpy.random.default_rng(seed)`` (PCG64) in vertex order.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+NIHU_TRIA_1_ID = 32303  # element type id of tria_1_elem, nihu/core/element.hpp:112-118
+
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([
+        [-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+        [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+        [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+         [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+         [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+         [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    faces = []
+    for a, b, c in f:
+        n = np.cross(v[b] - v[a], v[c] - v[a])
+        if np.dot(n, v[a] + v[b] + v[c]) < 0:
+            b, c = c, b
+        faces.append((a, b, c))
+    return v, faces
+
+
+def icosphere(level: int, radius: float = 1.0, jitter: float = 0.0, seed: int = 0):
+    """Return (nodes, tris) of a subdivided icosahedron on a sphere of ``radius``."""
+    if level < 0:
+        raise ValueError("level must be >= 0")
+    v0, faces = _icosahedron()
+    verts = [tuple(p) for p in v0]
+    mid_cache: dict[tuple[int, int], int] = {}
+
+    def midpoint(a, b):
+        key = (a, b) if a < b else (b, a)
+        idx = mid_cache.get(key)
+        if idx is None:
+            p = np.add(verts[a], verts[b]) * 0.5
+            p = p / np.linalg.norm(p)
+            idx = len(verts)
+            verts.append(tuple(p))
+            mid_cache[key] = idx
+        return idx
+
+    out = []
+
+    def subdivide(a, b, c, lev):
+        if lev == 0:
+            out.append((a, b, c))
+            return
+        ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+        subdivide(a, ab, ca, lev - 1)
+        subdivide(ab, b, bc, lev - 1)
+        subdivide(ca, bc, c, lev - 1)
+        subdivide(ab, bc, ca, lev - 1)
+
+    for a, b, c in faces:
+        subdivide(a, b, c, level)
+    nodes = np.asarray(verts, dtype=np.float64) * radius
+    if jitter:
+        rng = np.random.default_rng(seed)
+        u = rng.uniform(-jitter, jitter, size=nodes.shape[0])
+        nodes = nodes * (1.0 + u)[:, None]
+    return np.ascontiguousarray(nodes), np.ascontiguousarray(np.asarray(out, dtype=np.int32))
+
+
+def jittered_icosphere(level: int, seed: int = 0):
+    """The configuration meshes: unit icosphere, +-1% seeded radial jitter."""
+    return icosphere(level, 1.0, 0.01, seed)
+
+
+def ellipsoid(level: int = 5, seed: int = 0, jitter: float = 0.01):
+    """Icosphere scaled by seeded semi-axes U[0.5, 1.5] (drawn first from the
+    same generator), then radially jittered by 1 + U(-jitter, jitter)."""
+    nodes, tris = icosphere(level, 1.0, 0.0)
+    rng = np.random.default_rng(seed)
+    axes = rng.uniform(0.5, 1.5, size=3)
+    u = rng.uniform(-jitter, jitter, size=nodes.shape[0])
+    nodes = nodes * axes[None, :] * (1.0 + u)[:, None]
+    return np.ascontiguousarray(nodes), tris
+
+
+def read_off(path):
+    """Read an OFF file of triangles (nihu/interface/read_off_mesh.hpp:75-105).
+
+    Raises ValueError for a bad header, truncated data or non-triangle faces
+    (NiHu raises std::runtime_error in the same places)."""
+    with open(path, "r") as fh:
+        tok = fh.read().split()
+    if not tok or tok[0] != "OFF":
+        raise ValueError("Possibly invalid off file")
+    try:
+        n_nodes, n_elem = int(tok[1]), int(tok[2])
+    except (IndexError, ValueError):
+        raise ValueError("Error reading number of mesh entries")
+    pos = 4
+    try:
+        nodes = np.array(tok[pos:pos + 3 * n_nodes], dtype=np.float64).reshape(n_nodes, 3)
+    except ValueError:
+        raise ValueError("Error reading mesh nodes")
+    pos += 3 * n_nodes
+    tris = np.empty((n_elem, 3), dtype=np.int32)
+    for e in range(n_elem):
+        if pos >= len(tok):
+            raise ValueError("Error reading mesh elements")
+        nv = int(tok[pos])
+        if nv != 3:
+            raise ValueError("Invalid number of vertices for current element tag vector")
+        tris[e] = [int(t) for t in tok[pos + 1:pos + 4]]
+        pos += 1 + nv
+    return nodes, tris
+
+
+def write_off(path, nodes, tris):
+    with open(path, "w") as fh:
+        fh.write("OFF\n%d %d 0\n" % (len(nodes), len(tris)))
+        for p in nodes:
+            fh.write("%.17g %.17g %.17g\n" % tuple(p))
+        for t in tris:
+            fh.write("3 %d %d %d\n" % tuple(t))
+
+
+def to_nihu_elements(tris):
+    """NiHu's element matrix rows [type_id, n0, n1, n2] (nihu/core/mesh.hpp:137-193)."""
+    tris = np.asarray(tris)
+    return np.hstack([np.full((len(tris), 1), NIHU_TRIA_1_ID, dtype=np.int64), tris.astype(np.int64)])
+
+
+def from_nihu_elements(elements):
+    elements = np.asarray(elements)
+    if elements.ndim != 2 or elements.shape[1] < 4 or np.any(elements[:, 0] != NIHU_TRIA_1_ID):
+        raise ValueError("only tria_1 elements (id 32303) are supported")
+    return np.ascontiguousarray(elements[:, 1:4].astype(np.int32))

@@ -1,0 +1,832 @@
+// nbem_oracle.cpp -- FP64 CPU restatement of NiHu's dense BEM assembly for
+// flat triangles (collocation with constant fields, Galerkin with linear
+// isoparametric fields), 3D Laplace and Helmholtz SLP / DLP / DLPt / HSP.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker of the HIP engine in
+// nihu_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load the library built from it, and only as the checker or as the
+// timed CPU baseline -- never as the thing measured or shipped.
+//
+// Every routine cites the NiHu source it restates (paths relative to the
+// NiHu repository root).  NiHu itself cannot be built here (it needs Eigen and
+// Boost), so parity is pinned by:
+//   * the golden element integrals of nihu/gtest (near-singular SLP, Laplace
+//     and Helmholtz),
+//   * the finite-difference kernel check of nihu/gtest,
+//   * closed forms and brute-force quadrature (diagonal integrals, near
+//     integrals, Gauss-Legendre nodes, Dunavant exactness),
+//   * known-answer systems (transparent point sources, pulsating sphere).
+// Deliberate, documented departures from NiHu's arithmetic (all at the level
+// of one rounding):
+//   * the 3x3 plane rotation is applied as T^T (T is orthonormal) where NiHu
+//     uses Eigen's partialPivLu().solve;
+//   * Gauss-Legendre nodes come from Newton iteration on P_n where NiHu uses
+//     Eigen's Golub-Welsch eigen-solve;
+//   * squared distances for pair classification are fma(dz,dz,fma(dy,dy,dx*dx)),
+//     the convention the GPU engine shares bit for bit.
+// The file is compiled with -ffp-contract=off so that no other fusion happens.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+typedef std::complex<double> cplx;
+static const double PI = 3.141592653589793238462643383279502884;   // boost double_constants::pi
+static const double TWO_PI = 6.283185307179586476925286766559005768;
+
+struct V3 { double x, y, z; };
+static inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 cross(V3 a, V3 b)
+{
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+static inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+static inline V3 normalized(V3 a) { return a / norm(a); }
+
+// sgn and sinc: nihu/util/math_functions.hpp:42-61
+static inline int sgn(double v) { return (0.0 < v) - (v < 0.0); }
+static inline double sinc(double x)
+{
+	if (std::abs(x) > 1e-3)
+		return std::sin(x) / x;
+	return 1. - x * x / 6. * (1. - x * x / 20.);
+}
+
+// ---------------------------------------------------------------------------
+// Quadrature
+// ---------------------------------------------------------------------------
+struct QP { double xi, eta, w; };
+
+// Dunavant rules on the reference triangle (0,0),(1,0),(0,1), weights summing
+// to 1/2; nihu/core/gaussian_quadrature.hpp:209-512 (point counts :209-225).
+// Written out as explicit point lists in NiHu's order.
+static std::vector<QP> dunavant(int degree)
+{
+	const double t3 = 1. / 3.;
+	const double c = 0.333333333333333;
+	switch (degree) {
+	case 0: case 1:
+		return {{t3, t3, 0.5}};
+	case 2:
+		return {{1. / 6., 4. / 6., 1. / 6.}, {1. / 6., 1. / 6., 1. / 6.}, {4. / 6., 1. / 6., 1. / 6.}};
+	case 3:
+		return {{t3, t3, -0.281250000000000}, {1. / 5., 3. / 5., 0.260416666666667},
+			{1. / 5., 1. / 5., 0.260416666666667}, {3. / 5., 1. / 5., 0.260416666666667}};
+	default: break;
+	}
+	// higher orders: orbits {a, b, w} (two-coordinate orbit a,b,a with b=1-2a
+	// listed as in NiHu) and {a, b, c, w} (six permutations)
+	std::vector<QP> q;
+	auto s21 = [&](double a, double b, double w) {
+		q.push_back({a, b, w}); q.push_back({a, a, w}); q.push_back({b, a, w});
+	};
+	auto s21r = [&](double a, double b, double w) {   // NiHu's alternative listing (b,a),(a,a),(a,b)
+		q.push_back({b, a, w}); q.push_back({a, a, w}); q.push_back({a, b, w});
+	};
+	auto s111 = [&](double a, double b, double c3, double w) {
+		q.push_back({a, b, w}); q.push_back({c3, a, w}); q.push_back({b, c3, w});
+		q.push_back({b, a, w}); q.push_back({a, c3, w}); q.push_back({c3, b, w});
+	};
+	auto s111b = [&](double a, double b, double c3, double w) {   // (a,b),(b,c),(c,a),(b,a),(c,b),(a,c)
+		q.push_back({a, b, w}); q.push_back({b, c3, w}); q.push_back({c3, a, w});
+		q.push_back({b, a, w}); q.push_back({c3, b, w}); q.push_back({a, c3, w});
+	};
+	switch (degree) {
+	case 4:
+		s21(0.445948490915965, 0.108103018168070, 0.111690794839006);
+		s21(0.091576213509771, 0.816847572980459, 0.054975871827661);
+		break;
+	case 5:
+		q.push_back({c, c, 0.112500000000000});
+		s21(0.470142064105115, 0.059715871789770, 0.066197076394253);
+		s21(0.101286507323456, 0.797426985353087, 0.062969590272414);
+		break;
+	case 6:
+		s21(0.249286745170910, 0.501426509658179, 0.058393137863189);
+		s21(0.063089014491502, 0.873821971016996, 0.025422453185104);
+		s111(0.310352451033784, 0.053145049844817, 0.636502499121399, 0.041425537809187);
+		break;
+	case 7:
+		q.push_back({c, c, -0.074785022233841});
+		s21(0.260345966079040, 0.479308067841920, 0.087807628716604);
+		s21(0.065130102902216, 0.869739794195568, 0.026673617804419);
+		s111(0.312865496004874, 0.048690315425316, 0.638444188569810, 0.038556880445128);
+		break;
+	case 8:
+		q.push_back({c, c, 0.072157803838894});
+		s21(0.459292588292723, 0.081414823414554, 0.047545817133642);
+		s21(0.170569307751760, 0.658861384496480, 0.051608685267359);
+		s21(0.050547228317031, 0.898905543365938, 0.016229248811599);
+		s111(0.263112829634638, 0.008394777409958, 0.728492392955404, 0.013615157087218);
+		break;
+	case 9:
+		q.push_back({c, c, 0.048567898141400});
+		s21(0.489682519198738, 0.020634961602525, 0.015667350113570);
+		s21(0.437089591492937, 0.125820817014127, 0.038913770502387);
+		s21(0.188203535619033, 0.623592928761935, 0.039823869463605);
+		s21(0.044729513394453, 0.910540973211095, 0.012788837829349);
+		s111(0.221962989160766, 0.036838412054736, 0.741198598784498, 0.021641769688645);
+		break;
+	case 10:
+		q.push_back({c, c, 0.045408995191377});
+		s21r(0.485577633383657, 0.028844733232685, 0.018362978878233);
+		s21r(0.109481575485037, 0.781036849029926, 0.022660529717764);
+		s111b(0.141707219414880, 0.307939838764121, 0.550352941820999, 0.036378958422710);
+		s111b(0.025003534762686, 0.246672560639903, 0.728323904597411, 0.014163621265528);
+		s111b(0.009540815400299, 0.066803251012200, 0.923655933587500, 0.004710833481867);
+		break;
+	case 11: case 12:   // NiHu uses the degree-12 rule for 11
+		s21r(0.488217389773805, 0.023565220452390, 0.012865533220227);
+		s21r(0.439724392294460, 0.120551215411079, 0.021846272269019);
+		s21r(0.271210385012116, 0.457579229975768, 0.031429112108943);
+		s21r(0.127576145541586, 0.744847708916828, 0.017398056465355);
+		s21r(0.021317350453210, 0.957365299093579, 0.003083130525780);
+		s111b(0.115343494534698, 0.275713269685514, 0.608943235779788, 0.020185778883191);
+		s111b(0.022838332222257, 0.281325580989940, 0.695836086787803, 0.011178386601152);
+		s111b(0.025734050548330, 0.116251915907597, 0.858014033544073, 0.008658115554329);
+		break;
+	case 13:
+		q.push_back({c, c, 0.026260461700401});
+		s21r(0.495048184939705, 0.009903630120591, 0.005640072604665);
+		s21r(0.468716635109574, 0.062566729780852, 0.015711759181227);
+		s21r(0.414521336801277, 0.170957326397447, 0.023536251252097);
+		s21r(0.229399572042831, 0.541200855914337, 0.023681793268178);
+		s21r(0.114424495196330, 0.771151009607340, 0.015583764522897);
+		s21r(0.024811391363459, 0.950377217273082, 0.003987885732537);
+		s111b(0.094853828379579, 0.268794997058761, 0.636351174561660, 0.018424201364366);
+		s111b(0.018100773278807, 0.291730066734288, 0.690169159986905, 0.008700731651911);
+		s111b(0.022233076674090, 0.126357385491669, 0.851409537834241, 0.007760893419522);
+		break;
+	case 14:
+		s21r(0.488963910362179, 0.022072179275643, 0.010941790684715);
+		s21r(0.417644719340454, 0.164710561319092, 0.016394176772063);
+		s21r(0.273477528308839, 0.453044943382323, 0.025887052253646);
+		s21r(0.177205532412543, 0.645588935174913, 0.021081294368497);
+		s21r(0.061799883090873, 0.876400233818255, 0.007216849834889);
+		s21r(0.019390961248701, 0.961218077502598, 0.002461701801200);
+		s111b(0.057124757403648, 0.172266687821356, 0.770608554774996, 0.012332876606282);
+		s111b(0.092916249356972, 0.336861459796345, 0.570222290846683, 0.019285755393531);
+		s111b(0.014646950055654, 0.298372882136258, 0.686980167808088, 0.007218154056767);
+		s111b(0.001268330932872, 0.118974497696957, 0.879757171370171, 0.002505114419250);
+		break;
+	default:
+		// nihu/core/gaussian_quadrature.hpp:508-510
+		throw std::out_of_range("unsupported dunavant degree");
+	}
+	return q;
+}
+
+// Gauss-Legendre rule with N = degree/2 + 1 points on [-1, 1], ascending nodes
+// (nihu/core/gaussian_quadrature.hpp:64-88,136-150).  Nodes are the roots of
+// P_N found by Newton iteration; weights 2 / ((1-x^2) P_N'(x)^2).
+static void legendre_eval(int n, double x, double &p, double &dp)
+{
+	double p0 = 1.0, p1 = x;
+	if (n == 0) { p = 1.0; dp = 0.0; return; }
+	for (int k = 2; k <= n; ++k) {
+		double pk = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+		p0 = p1; p1 = pk;
+	}
+	p = p1;
+	dp = n * (x * p1 - p0) / (x * x - 1.0);
+}
+
+static std::vector<std::pair<double, double> > gauss_legendre_n(int N)
+{
+	std::vector<std::pair<double, double> > r(N);
+	for (int i = 0; i < N; ++i) {
+		double x = std::cos(PI * (i + 0.75) / (N + 0.5));
+		for (int it = 0; it < 100; ++it) {
+			double p, dp;
+			legendre_eval(N, x, p, dp);
+			double dx = p / dp;
+			x -= dx;
+			if (std::abs(dx) < 1e-17) break;
+		}
+		double p, dp;
+		legendre_eval(N, x, p, dp);
+		r[N - 1 - i] = {x, 2.0 / ((1.0 - x * x) * dp * dp)};
+	}
+	return r;
+}
+
+static const std::vector<std::pair<double, double> > &gauss_line(int degree)
+{
+	// static stores of the line rules NiHu uses (orders 7, 10, 40) plus any
+	// order a test asks for
+	static std::vector<std::pair<double, double> > pool[128];
+	static bool filled[128] = {false};
+	if (degree < 0 || degree >= 128)
+		throw std::out_of_range("gauss_line: unsupported degree");
+	#pragma omp critical(orc_gauss_line)
+	if (!filled[degree]) { pool[degree] = gauss_legendre_n(degree / 2 + 1); filled[degree] = true; }
+	return pool[degree];
+}
+
+static const std::vector<QP> &dunavant_store(int degree)
+{
+	// function-local static pool, as nihu/util/store_pattern.hpp:34-43
+	static std::vector<QP> pool[15];
+	static bool init = [] { for (int d = 0; d < 15; ++d) pool[d] = dunavant(d); return true; }();
+	(void)init;
+	if (degree < 0 || degree > 14)
+		throw std::out_of_range("Too high quadrature degree selected for collocational integration");
+	return pool[degree];
+}
+
+// ---------------------------------------------------------------------------
+// Geometry of a flat 3-node triangle (tria_1_elem)
+// nihu/core/element.hpp:272-292 (centre, linear size estimate), :503-516
+// (normal = dx/dxi x dx/deta); nihu/library/lib_shape.hpp:134-199 (shape set);
+// nihu/library/lib_domain.cpp:59-71 (corners, edges, centre 1/3).
+// ---------------------------------------------------------------------------
+struct Elem {
+	V3 X[3];
+	V3 c;        // centre = x(1/3,1/3)
+	V3 n;        // jacobian vector (X1-X0)x(X2-X0)
+	double J;    // |n|
+	V3 nu;       // unit normal
+	double R;    // mean edge length
+	double diam; // longest edge
+	int id;
+};
+
+static inline V3 elem_x(Elem const &e, double xi, double eta)
+{
+	double N0 = 1.0 - xi - eta, N1 = xi, N2 = eta;
+	return {e.X[0].x * N0 + e.X[1].x * N1 + e.X[2].x * N2,
+		e.X[0].y * N0 + e.X[1].y * N1 + e.X[2].y * N2,
+		e.X[0].z * N0 + e.X[1].z * N1 + e.X[2].z * N2};
+}
+
+static Elem make_elem(V3 X0, V3 X1, V3 X2, int id)
+{
+	Elem e;
+	e.X[0] = X0; e.X[1] = X1; e.X[2] = X2; e.id = id;
+	e.c = elem_x(e, 1. / 3., 1. / 3.);
+	e.n = cross(X1 - X0, X2 - X0);
+	e.J = norm(e.n);
+	e.nu = e.n / e.J;
+	double l0 = norm(X1 - X0), l1 = norm(X2 - X1), l2 = norm(X0 - X2);
+	double d = 0.0;
+	d += l0; d += l1; d += l2;
+	e.R = d / 3.0;
+	e.diam = std::max(l0, std::max(l1, l2));
+	return e;
+}
+
+static std::vector<Elem> make_mesh(const double *xyz, int n_nodes, const int *tri, int n_tri)
+{
+	std::vector<Elem> m(n_tri);
+	for (int e = 0; e < n_tri; ++e) {
+		V3 X[3];
+		for (int a = 0; a < 3; ++a) {
+			int v = tri[3 * e + a];
+			if (v < 0 || v >= n_nodes) throw std::out_of_range("node index out of range");
+			X[a] = {xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
+		}
+		m[e] = make_elem(X[0], X[1], X[2], e);
+	}
+	return m;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: nihu/library/helmholtz_kernel.hpp:169-228,
+// nihu/library/laplace_kernel.hpp:133-171,
+// nihu/library/normal_derivative_kernel.hpp:114-257.  r = y - x.
+// family 0 = Laplace, 1 = Helmholtz.  Laplace results have zero imaginary part.
+// ---------------------------------------------------------------------------
+static cplx kernel_eval(int family, int nx, int ny, double k, V3 x, V3 y, V3 nxv, V3 nyv)
+{
+	V3 rvec = y - x;
+	double r = norm(rvec);
+	const cplx I(0.0, 1.0);
+	if (family == 0) {
+		if (nx == 0 && ny == 0)
+			return 1. / r / (4. * PI);
+		if (nx == 0 && ny == 1) {
+			double rdny = dot(rvec, nyv) / r;
+			return (-1. / (r * r) / (4. * PI)) * rdny;
+		}
+		if (nx == 1 && ny == 0) {
+			double rdnx = -dot(rvec, nxv) / r;
+			return (-1. / (r * r) / (4. * PI)) * rdnx;
+		}
+		double rdny = dot(rvec, nyv) / r;
+		double rdnx = -dot(rvec, nxv) / r;
+		double g = 1. / (r * r * r) / (4. * PI);
+		double f1 = -1. * g, f0 = 3. * g;
+		return f0 * rdnx * rdny - f1 * dot(nxv, nyv);
+	}
+	if (nx == 0 && ny == 0) {
+		cplx ikr = I * (k * r);
+		return std::exp(-ikr) / r / (4. * PI);
+	}
+	if (nx + ny == 1) {
+		cplx ikr = I * (k * r);
+		cplx f = std::exp(-ikr) / (r * r) / (4. * PI) * (-ikr - 1.);
+		double rd = (ny == 1) ? dot(rvec, nyv) / r : -dot(rvec, nxv) / r;
+		return f * rd;
+	}
+	double rdny = dot(rvec, nyv) / r;
+	double rdnx = -dot(rvec, nxv) / r;
+	cplx ikr = I * (k * r);
+	cplx g = std::exp(-ikr) / (r * r * r) / (4. * PI);
+	cplx f1 = -g * (1. + ikr);
+	cplx f0 = g * (3. + ikr * (3. + ikr));
+	return f0 * rdnx * rdny - f1 * dot(nxv, nyv);
+}
+
+// ---------------------------------------------------------------------------
+// Quadrature degree policy
+// nihu/core/complexity_estimator.hpp:69-75, nihu/library/interval_estimator.hpp:51-62,
+// nihu/tmp/interval.hpp:173-185 (first breakpoint with r < x),
+// nihu/library/distance_kernel_intervals.hpp:116-155 (accuracy 3 tables),
+// far-field behaviour inverse<1+Nx+Ny> (nihu/library/helmholtz_kernel.hpp,
+// laplace_kernel.hpp kernel_traits_ns).
+// ---------------------------------------------------------------------------
+struct BP { double x; int deg; };
+static const std::vector<BP> &interval_table(int order)
+{
+	static const std::vector<BP> inv1 = {{11. / 10., 6}, {18. / 10., 4}, {91. / 10., 2}, {1e9 / 1., 0}};
+	static const std::vector<BP> inv2 = {{11. / 10., 8}, {15. / 10., 6}, {28. / 10., 4}, {158. / 10., 2}, {1e9 / 1., 0}};
+	static const std::vector<BP> inv3 = {{10. / 10., 9}, {11. / 10., 9}, {14. / 10., 8}, {19. / 10., 6},
+		{36. / 10., 4}, {223. / 10., 2}, {1e9 / 1., 0}};
+	if (order == 1) return inv1;
+	if (order == 2) return inv2;
+	return inv3;
+}
+
+static int eval_interval(std::vector<BP> const &t, double r)
+{
+	for (auto const &bp : t)
+		if (r < bp.x) return bp.deg;
+	// nihu/library/lib_tmp.cpp:8-11
+	throw std::out_of_range("cannot determine interval value");
+}
+
+// pair-distance convention shared with the GPU engine
+static inline double dist2(V3 a, V3 b)
+{
+	double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+	return std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+}
+
+struct Policy {
+	int rule;          // 0: NiHu adaptive degree (accuracy 3), 1: fixed Dunavant degree
+	int fixed_degree;  // Dunavant degree for rule 1
+	int near_mode;     // 0: d / R_mean_edge < limit (NiHu), 1: d / diameter < limit, 2: no near pass
+	double near_limit; // 1.5 in NiHu
+};
+
+// nearly_singular_integral<...>::needed,
+// nihu/library/helmholtz_nearly_singular_integrals.hpp:234-246 and the Laplace
+// counterparts in laplace_nearly_singular_integrals.hpp.
+static bool near_needed(Policy const &p, double d2, Elem const &trial)
+{
+	if (p.near_mode == 2) return false;
+	double d = std::sqrt(d2);
+	double S = p.near_mode == 0 ? trial.R : trial.diam;
+	return d / S < p.near_limit;
+}
+
+static int regular_degree(Policy const &p, int field_complexity, int nx, int ny, double d2, Elem const &trial)
+{
+	if (p.rule == 1) return p.fixed_degree;
+	double distance = std::sqrt(d2);
+	return field_complexity + eval_interval(interval_table(1 + nx + ny), distance / trial.R);
+}
+
+// ---------------------------------------------------------------------------
+// Regular collocational integral: sum_q K(x, y_q) * (J * w_q)
+// nihu/core/double_integral.hpp:445-467 (eval_on_accelerator), :543-589
+// ---------------------------------------------------------------------------
+static cplx regular_integral(int family, int nx, int ny, double k, V3 x, V3 nxv, Elem const &trial, int degree)
+{
+	if (degree > 14)
+		throw std::out_of_range("Too high quadrature degree selected for collocational integration");
+	cplx res = 0.0;
+	for (auto const &q : dunavant_store(degree)) {
+		V3 y = elem_x(trial, q.xi, q.eta);
+		res += kernel_eval(family, nx, ny, k, x, y, nxv, trial.nu) * (1.0 * (trial.J * q.w));
+	}
+	return res;
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal (singular) integrals over a constant flat triangle, x0 = centre
+// plane_element_helper: nihu/library/plane_element_helper.hpp:42-74
+// Helmholtz SLP / HSP: nihu/library/helmholtz_singular_integrals.hpp:703-839
+// Laplace SLP / HSP: nihu/library/laplace_singular_integrals.hpp:456-505
+// DLP / DLPt: zero, nihu/library/normal_derivative_singular_integrals.hpp:13-58
+// ---------------------------------------------------------------------------
+static void plane_element_helper(Elem const &e, V3 x0, double r[3], double theta[3], double alpha[3])
+{
+	V3 R[3], C[3];
+	for (int i = 0; i < 3; ++i) {
+		R[i] = e.X[i] - x0;
+		r[i] = norm(R[i]);
+		R[i] = R[i] / r[i];
+		C[i] = e.X[i] - e.X[(i + 1) % 3];
+		C[i] = C[i] / norm(C[i]);
+	}
+	for (int i = 0; i < 3; ++i) {
+		double cs = dot(R[i], R[(i + 1) % 3]);
+		theta[i] = cs >= 1. ? 0. : std::acos(cs);
+		alpha[i] = std::acos(dot(R[i], C[i]));
+	}
+}
+
+static cplx singular_integral(int family, int nx, int ny, double k, Elem const &e)
+{
+	if (nx + ny == 1) return 0.0;
+	V3 x0 = e.c;
+	double r[3], theta[3], alpha[3];
+	plane_element_helper(e, x0, r, theta, alpha);
+	const cplx I(0.0, 1.0);
+	if (family == 0) {
+		double res = 0.0;
+		if (nx == 0) {
+			for (int i = 0; i < 3; ++i)
+				res += r[i] * std::sin(alpha[i]) *
+					std::log(std::tan((alpha[i] + theta[i]) / 2.) / std::tan(alpha[i] / 2.));
+		} else {
+			for (int i = 0; i < 3; ++i)
+				res += (std::cos(alpha[i] + theta[i]) - std::cos(alpha[i])) / (r[i] * std::sin(alpha[i]));
+		}
+		return res / (4. * PI);
+	}
+	auto const &q7 = dunavant_store(7);
+	if (nx == 0) {
+		double I_stat = 0.0;
+		for (int i = 0; i < 3; ++i)
+			I_stat += r[i] * std::sin(alpha[i]) *
+				std::log(std::tan((alpha[i] + theta[i]) / 2.0) / std::tan(alpha[i] / 2.0));
+		cplx I_dyn = 0.0;
+		for (auto const &q : q7) {
+			double rq = norm(elem_x(e, q.xi, q.eta) - x0);
+			double jac = e.J;
+			cplx dyn = -I * k * std::exp(-I * k * rq / 2.0) * sinc(k * rq / 2.0);
+			I_dyn += dyn * q.w * jac;
+		}
+		return (I_stat + I_dyn) / (4.0 * PI);
+	}
+	double IG0 = 0.0, IddG0 = 0.0;
+	for (int i = 0; i < 3; ++i) {
+		if (theta[i] < 1e-3) continue;
+		IG0 += r[i] * std::sin(alpha[i]) * std::log(std::tan((alpha[i] + theta[i]) / 2.0) / std::tan(alpha[i] / 2.0));
+		IddG0 += (std::cos(alpha[i] + theta[i]) - std::cos(alpha[i])) / (r[i] * std::sin(alpha[i]));
+	}
+	cplx I_acc = 0.0;
+	for (auto const &q : q7) {
+		double rq = norm(elem_x(e, q.xi, q.eta) - x0);
+		double jac = e.J;
+		cplx ikr = I * k * rq;
+		cplx dyn;
+		if (std::abs(rq) > 1e-3)
+			dyn = (std::exp(-ikr) * (1.0 + ikr) - 1.0 + ikr * ikr / 2.0) / rq / rq / rq;
+		else
+			dyn = -I * k * k * k * (1.0 / 3.0 - ikr * (1.0 / 8.0 - ikr * (1.0 / 30.0 - ikr * (1.0 / 144.0 -
+				ikr * (1.0 / 840.0 - ikr / 5760.0)))));
+		I_acc += dyn * q.w * jac;
+	}
+	return (IddG0 + k * k / 2.0 * IG0 + I_acc) / (4.0 * PI);
+}
+
+// ---------------------------------------------------------------------------
+// Near-singular integrals over a constant flat triangle
+// Laplace static part (planar polar, radial integral in closed form):
+//   nihu/library/laplace_nearly_singular_integrals.hpp:62-389
+// plane_elem_transform, plane_elem_helper_mid:
+//   nihu/library/plane_element_helper.hpp:76-128
+// Helmholtz = Laplace static + Dunavant-14 (42 points) of (K_helm - K_lap):
+//   nihu/library/helmholtz_nearly_singular_integrals.hpp:48-219
+// ---------------------------------------------------------------------------
+static double laplace_near_static(int nx, int ny, Elem const &e, V3 x0_in, V3 nx_in, int line_degree = 0)
+{
+	// T = [t1, t3 x t1, t3] with t1 = v1/|v1|, t3 = (v1 x v2)/|v1 x v2|
+	V3 v1 = e.X[1] - e.X[0], v2 = e.X[2] - e.X[0];
+	V3 T0 = normalized(v1);
+	V3 T2 = normalized(cross(v1, v2));
+	V3 T1 = normalized(cross(T2, T0));
+	auto to_plane = [&](V3 p) -> V3 { return {dot(T0, p), dot(T1, p), dot(T2, p)}; };
+	V3 C[3];
+	for (int i = 0; i < 3; ++i) C[i] = to_plane(e.X[i]);
+	V3 x0 = to_plane(x0_in);
+	V3 nxl = to_plane(nx_in);
+
+	double theta_lim[3], ref_distance[3], theta0[3];
+	for (int n = 0; n < 3; ++n) {
+		double c1x = C[n].x, c1y = C[n].y;
+		double c2x = C[(n + 1) % 3].x, c2y = C[(n + 1) % 3].y;
+		double lx = c2x - c1x, ly = c2y - c1y;
+		double ln = std::sqrt(lx * lx + ly * ly);
+		lx /= ln; ly /= ln;
+		double d1x = c1x - x0.x, d1y = c1y - x0.y;
+		double dl = d1x * lx + d1y * ly;
+		double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		theta_lim[n] = std::atan2(d1y, d1x);
+		theta0[n] = std::atan2(d0y, d0x);
+		ref_distance[n] = std::sqrt(d0x * d0x + d0y * d0y);
+	}
+	double z = x0.z - C[0].z;
+
+	bool hsp = (nx == 1 && ny == 1);
+	// line quadrature order 10 (6 points) for SLP/DLP/DLPt, 40 (21 points) for HSP
+	auto const &gl = gauss_line(line_degree > 0 ? line_degree : (hsp ? 40 : 10));
+	double result = 0.0;
+	for (int i = 0; i < 3; ++i) {
+		double th1 = theta_lim[i], th2 = theta_lim[(i + 1) % 3];
+		if (std::abs(th2 - th1) > PI) {
+			if (th2 > th1) th1 += TWO_PI;
+			else th2 += TWO_PI;
+		}
+		double jac = (th2 - th1) / 2.;
+		for (auto const &g : gl) {
+			double xi = g.first, w = g.second;
+			double theta = th1 * (1. - xi) / 2. + th2 * (1. + xi) / 2.;
+			double R = ref_distance[i] / std::cos(theta - theta0[i]);
+			double r = std::sqrt(R * R + z * z);
+			if (nx == 0 && ny == 0) {
+				result += (r - std::abs(z)) * jac * w;
+			} else if (nx == 0 && ny == 1) {
+				result += (1 - std::abs(z) / r) * jac * w;
+			} else if (nx == 1 && ny == 0) {
+				V3 rvec = {R * std::cos(theta), R * std::sin(theta), -z};
+				result += (-dot(nxl, rvec) / r + (nxl.x * std::cos(theta) + nxl.y * std::sin(theta)) * std::log(R + r)
+					- nxl.z * sgn(z)) * jac * w;
+			} else {
+				V3 rvec = {R * std::cos(theta), R * std::sin(theta), -z};
+				double rdnx = -dot(rvec, nxl) / r;
+				double integrand;
+				if (std::abs(z) > 1e-12)
+					integrand = -R * R / (r * r) / z * rdnx;
+				else
+					integrand = -nxl.z / R;
+				result += integrand * jac * w;
+			}
+		}
+	}
+	if (nx == 0 && ny == 1) return sgn(z) * result / (4. * PI);
+	return result / (4. * PI);
+}
+
+static cplx near_integral(int family, int nx, int ny, double k, Elem const &e, V3 x0, V3 nxv, int line_degree = 0)
+{
+	double res_static = laplace_near_static(nx, ny, e, x0, nxv, line_degree);
+	if (family == 0) return res_static;
+	cplx res_dynamic = 0.0;
+	for (auto const &q : dunavant_store(14)) {
+		V3 y = elem_x(e, q.xi, q.eta);
+		double jac = e.J;
+		res_dynamic += (kernel_eval(1, nx, ny, k, x0, y, nxv, e.nu) - kernel_eval(0, nx, ny, k, x0, y, nxv, e.nu))
+			* q.w * jac;
+	}
+	return res_static + res_dynamic;
+}
+
+// ---------------------------------------------------------------------------
+// Pair integral with classification
+// nihu/core/double_integral.hpp:599-673 (WITH / WITHOUT singularity check),
+// nihu/core/element_match.hpp:84-112 (collocation: face match = same id)
+// ---------------------------------------------------------------------------
+static cplx pair_integral(int family, int nx, int ny, double k, Policy const &p, Elem const &test, Elem const &trial,
+	bool same_mesh)
+{
+	if (same_mesh && test.id == trial.id)
+		return singular_integral(family, nx, ny, k, trial);
+	double d2 = dist2(test.c, trial.c);
+	if (near_needed(p, d2, trial))
+		return near_integral(family, nx, ny, k, trial, test.c, test.nu);
+	int degree = regular_degree(p, 0, nx, ny, d2, trial);
+	return regular_integral(family, nx, ny, k, test.c, test.nu, trial, degree);
+}
+
+} // namespace orc
+
+// ===========================================================================
+// C interface (ctypes)
+// ===========================================================================
+using namespace orc;
+
+static thread_local std::string g_err;
+
+#define ORC_TRY try {
+#define ORC_CATCH } catch (std::out_of_range const &ex) { g_err = ex.what(); return 2; } \
+	catch (std::exception const &ex) { g_err = ex.what(); return 1; } return 0;
+
+extern "C" {
+
+typedef struct { int rule; int fixed_degree; int near_mode; double near_limit; } orc_policy;
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+	return omp_get_max_threads();
+#else
+	return 1;
+#endif
+}
+
+int orc_dunavant(int degree, double *xi, double *eta, double *w, int cap, int *n_out)
+{
+	ORC_TRY
+	auto q = dunavant(degree);
+	if ((int)q.size() > cap) { g_err = "capacity"; return 1; }
+	for (size_t i = 0; i < q.size(); ++i) { xi[i] = q[i].xi; eta[i] = q[i].eta; w[i] = q[i].w; }
+	*n_out = (int)q.size();
+	ORC_CATCH
+}
+
+int orc_gauss_legendre(int n, double *x, double *w)
+{
+	auto g = gauss_legendre_n(n);
+	for (int i = 0; i < n; ++i) { x[i] = g[i].first; w[i] = g[i].second; }
+	return 0;
+}
+
+int orc_geometry(const double *xyz, int n_nodes, const int *tri, int n_tri,
+	double *centers, double *unit_normals, double *jac, double *R, double *diam)
+{
+	ORC_TRY
+	auto m = make_mesh(xyz, n_nodes, tri, n_tri);
+	for (int e = 0; e < n_tri; ++e) {
+		centers[3 * e] = m[e].c.x; centers[3 * e + 1] = m[e].c.y; centers[3 * e + 2] = m[e].c.z;
+		unit_normals[3 * e] = m[e].nu.x; unit_normals[3 * e + 1] = m[e].nu.y; unit_normals[3 * e + 2] = m[e].nu.z;
+		jac[e] = m[e].J; R[e] = m[e].R; diam[e] = m[e].diam;
+	}
+	ORC_CATCH
+}
+
+int orc_kernel(int family, int nx, int ny, double k, const double *x, const double *y,
+	const double *nxv, const double *nyv, double *out)
+{
+	ORC_TRY
+	cplx v = kernel_eval(family, nx, ny, k, {x[0], x[1], x[2]}, {y[0], y[1], y[2]},
+		{nxv[0], nxv[1], nxv[2]}, {nyv[0], nyv[1], nyv[2]});
+	out[0] = v.real(); out[1] = v.imag();
+	ORC_CATCH
+}
+
+// kind: 0 regular (Dunavant degree), 1 near-singular, 2 singular (x0 = centre).
+// For kind 1, degree > 0 replaces NiHu's line quadrature order of the static
+// part (used by tests to converge the formula to reference golden values).
+int orc_elem_integral(int kind, int family, int nx, int ny, double k, const double *coords9,
+	const double *x0, const double *nx0, int degree, double *out)
+{
+	ORC_TRY
+	Elem e = make_elem({coords9[0], coords9[1], coords9[2]}, {coords9[3], coords9[4], coords9[5]},
+		{coords9[6], coords9[7], coords9[8]}, 0);
+	V3 x = {x0[0], x0[1], x0[2]}, n = {nx0[0], nx0[1], nx0[2]};
+	cplx v;
+	if (kind == 0) v = regular_integral(family, nx, ny, k, x, n, e, degree);
+	else if (kind == 1) v = near_integral(family, nx, ny, k, e, x, n, degree);
+	else v = singular_integral(family, nx, ny, k, e);
+	out[0] = v.real(); out[1] = v.imag();
+	ORC_CATCH
+}
+
+// Pair classes for a block of rows: -1 diagonal, -2 near, otherwise the regular
+// Dunavant degree.  cls is row-major [n_rows][n_trial].
+int orc_classify(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh,
+	int nx, int ny, const orc_policy *pol, const int *rows, int n_rows, int8_t *cls)
+{
+	ORC_TRY
+	auto test = make_mesh(txyz, tn, ttri, tnt);
+	auto trial = same_mesh ? test : make_mesh(sxyz, sn, stri, snt);
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	int nr = rows ? n_rows : tnt;
+	#pragma omp parallel for schedule(dynamic, 8)
+	for (int r = 0; r < nr; ++r) {
+		int i = rows ? rows[r] : r;
+		for (int j = 0; j < snt; ++j) {
+			int8_t c;
+			if (same_mesh && i == j) c = -1;
+			else {
+				double d2 = dist2(test[i].c, trial[j].c);
+				if (near_needed(p, d2, trial[j])) c = -2;
+				else c = (int8_t)regular_degree(p, 0, nx, ny, d2, trial[j]);
+			}
+			cls[(int64_t)r * snt + j] = c;
+		}
+	}
+	ORC_CATCH
+}
+
+// result(r, j) (+)= alpha * integral(test row rows[r], trial j); complex
+// interleaved, column-major with leading dimension ld (ld >= n_rows).
+// nihu/core/assembly.hpp:61-97 (the pair loop, accumulating through
+// nihu/util/matrix_block.hpp:59-65); the scalar multiplies each pair result
+// (nihu/core/integral_operator.hpp:159-167).  OpenMP over rows is the oracle's
+// own extension; with n_threads = 1 it is NiHu's serial loop.
+int orc_assemble(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh,
+	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	ORC_TRY
+	auto test = make_mesh(txyz, tn, ttri, tnt);
+	auto trial = same_mesh ? test : make_mesh(sxyz, sn, stri, snt);
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	int nr = rows ? n_rows : tnt;
+	cplx alpha(alpha_re, alpha_im);
+	if (n_threads <= 0) n_threads = orc_num_threads();
+	std::string err;
+	#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
+	for (int r = 0; r < nr; ++r) {
+		try {
+			int i = rows ? rows[r] : r;
+			for (int j = 0; j < snt; ++j) {
+				cplx v = alpha * pair_integral(family, nx, ny, k, p, test[i], trial[j], same_mesh != 0);
+				double *o = out + 2 * ((int64_t)j * ld + r);
+				if (accumulate) { o[0] += v.real(); o[1] += v.imag(); }
+				else { o[0] = v.real(); o[1] = v.imag(); }
+			}
+		} catch (std::exception const &ex) {
+			#pragma omp critical
+			err = ex.what();
+		}
+	}
+	if (!err.empty()) throw std::out_of_range(err);
+	ORC_CATCH
+}
+
+// identity operator: result(i,i) += alpha on the same mesh (local branch of
+// nihu/core/assembly.hpp:78-96; value N_trial(centre) = 1,
+// nihu/core/single_integral.hpp:146-178)
+int orc_identity(int n, double alpha_re, double alpha_im, const int *rows, int n_rows, double *out, int64_t ld)
+{
+	int nr = rows ? n_rows : n;
+	for (int r = 0; r < nr; ++r) {
+		int i = rows ? rows[r] : r;
+		double *o = out + 2 * ((int64_t)i * ld + r);
+		o[0] += alpha_re; o[1] += alpha_im;
+	}
+	return 0;
+}
+
+// Burton-Miller collocation matrices in NiHu's composition order
+// (nihu/applications/helmholtz/helmholtz_3d_coll.cpp:41-96):
+//   B = Gs: G, then alpha*Ht, then alpha/2*I;   A = Hs: alpha*D, then -1/2*I, then H
+// alpha = -i/k there.  Both outputs are overwritten (setZero first).
+int orc_assemble_bm(const double *xyz, int nn, const int *tri, int nt, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *A, double *B, int64_t ld, int n_threads)
+{
+	int nr = rows ? n_rows : nt;
+	for (int j = 0; j < nt; ++j)
+		for (int r = 0; r < nr; ++r) {
+			double *a = A + 2 * ((int64_t)j * ld + r), *b = B + 2 * ((int64_t)j * ld + r);
+			a[0] = a[1] = b[0] = b[1] = 0.0;
+		}
+	cplx al(alpha_re, alpha_im);
+	int s;
+	if ((s = orc_assemble(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, 0, 0, k, 1.0, 0.0, pol, rows, n_rows, B, ld, 1, n_threads))) return s;
+	if ((s = orc_assemble(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, 1, 0, k, al.real(), al.imag(), pol, rows, n_rows, B, ld, 1, n_threads))) return s;
+	orc_identity(nt, (al / 2.).real(), (al / 2.).imag(), rows, n_rows, B, ld);
+	if ((s = orc_assemble(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, 1, 1, k, al.real(), al.imag(), pol, rows, n_rows, A, ld, 1, n_threads))) return s;
+	orc_identity(nt, -.5, 0.0, rows, n_rows, A, ld);
+	if ((s = orc_assemble(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, 0, 1, k, 1.0, 0.0, pol, rows, n_rows, A, ld, 1, n_threads))) return s;
+	return 0;
+}
+
+// NiHu's own multithreaded pattern: one whole Burton-Miller assembly per
+// wavenumber per thread (benchmark/EAA/sphere/homogeneous/bem/bem.cpp:91).
+// Each frequency writes its own A_f, B_f (n_rows x nt blocks, ld = n_rows),
+// the output arrays are [n_k][nt][n_rows] complex.
+int orc_sweep_bm(const double *xyz, int nn, const int *tri, int nt, const double *ks, int n_k,
+	const orc_policy *pol, const int *rows, int n_rows, double *A, double *B, int n_threads)
+{
+	int nr = rows ? n_rows : nt;
+	int64_t blk = 2 * (int64_t)nr * nt;
+	if (n_threads <= 0) n_threads = orc_num_threads();
+	int status = 0;
+	#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+	for (int f = 0; f < n_k; ++f) {
+		double k = ks[f];
+		int s = orc_assemble_bm(xyz, nn, tri, nt, k, 0.0, -1.0 / k, pol, rows, n_rows, A + f * blk, B + f * blk, nr, 1);
+		if (s) {
+			#pragma omp critical
+			status = s;
+		}
+	}
+	return status;
+}
+
+} // extern "C"

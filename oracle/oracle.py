@@ -1,0 +1,189 @@
+"""ctypes front end of the FP64 oracle (liboracle.so built from nbem_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+engine in nihu_amd/ never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class Policy(C.Structure):
+    _fields_ = [("rule", C.c_int), ("fixed_degree", C.c_int), ("near_mode", C.c_int), ("near_limit", C.c_double)]
+
+
+NIHU = dict(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5)
+
+
+def policy(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5):
+    return Policy(rule, fixed_degree, near_mode, near_limit)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        l = C.CDLL(_LIB)
+        D = C.POINTER(C.c_double)
+        I = C.POINTER(C.c_int)
+        P = C.POINTER(Policy)
+        l.orc_last_error.restype = C.c_char_p
+        l.orc_dunavant.argtypes = [C.c_int, D, D, D, C.c_int, I]
+        l.orc_gauss_legendre.argtypes = [C.c_int, D, D]
+        l.orc_geometry.argtypes = [D, C.c_int, I, C.c_int, D, D, D, D, D]
+        l.orc_kernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, D, D]
+        l.orc_elem_integral.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, C.c_int, D]
+        l.orc_classify.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   P, I, C.c_int, C.POINTER(C.c_int8)]
+        l.orc_assemble.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_identity.argtypes = [C.c_int, C.c_double, C.c_double, I, C.c_int, D, C.c_int64]
+        l.orc_assemble_bm.argtypes = [D, C.c_int, I, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      P, I, C.c_int, D, D, C.c_int64, C.c_int]
+        l.orc_sweep_bm.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, P, I, C.c_int, D, D, C.c_int]
+        _lib = l
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().orc_last_error().decode()
+        if status == 2:
+            raise IndexError(msg)
+        raise RuntimeError(msg)
+
+
+def _mesh(nodes, tris):
+    return (np.ascontiguousarray(nodes, dtype=np.float64), np.ascontiguousarray(tris, dtype=np.int32))
+
+
+def dunavant(degree):
+    xi, eta, w = (np.zeros(64) for _ in range(3))
+    n = C.c_int(0)
+    _check(lib().orc_dunavant(degree, _d(xi), _d(eta), _d(w), 64, C.byref(n)))
+    return xi[:n.value].copy(), eta[:n.value].copy(), w[:n.value].copy()
+
+
+def gauss_legendre(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_gauss_legendre(n, _d(x), _d(w))
+    return x, w
+
+
+def geometry(nodes, tris):
+    nodes, tris = _mesh(nodes, tris)
+    n = len(tris)
+    c, nu, J, R, diam = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(lib().orc_geometry(_d(nodes), len(nodes), _i(tris), n, _d(c), _d(nu), _d(J), _d(R), _d(diam)))
+    return dict(center=c, normal=nu, jac=J, R=R, diam=diam)
+
+
+def kernel(family, nx, ny, k, x, y, nxv, nyv):
+    out = np.zeros(2)
+    arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, nxv, nyv)]
+    _check(lib().orc_kernel(family, nx, ny, k, *[_d(a) for a in arr], _d(out)))
+    return complex(out[0], out[1])
+
+
+def elem_integral(kind, family, nx, ny, k, coords, x0=(0, 0, 0), nx0=(0, 0, 1), degree=0):
+    """kind 0 regular (Dunavant degree), 1 near-singular, 2 singular (x0 = centre).
+    coords: 3 x 3, one vertex per row."""
+    out = np.zeros(2)
+    cs = np.ascontiguousarray(coords, dtype=np.float64).reshape(9)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    nx0 = np.ascontiguousarray(nx0, dtype=np.float64)
+    _check(lib().orc_elem_integral(kind, family, nx, ny, k, _d(cs), _d(x0), _d(nx0), degree, _d(out)))
+    return complex(out[0], out[1])
+
+
+def classify(nodes, tris, nx, ny, pol=None, rows=None, trial=None):
+    """Pair classes (-1 diagonal, -2 near, else regular degree), [n_rows, n_trial] int8."""
+    pol = pol or policy()
+    t_nodes, t_tris = _mesh(nodes, tris)
+    same = trial is None
+    s_nodes, s_tris = (t_nodes, t_tris) if same else _mesh(*trial)
+    rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nr = len(t_tris) if rows_a is None else len(rows_a)
+    cls = np.zeros((nr, len(s_tris)), dtype=np.int8)
+    _check(lib().orc_classify(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                              _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same), nx, ny,
+                              C.byref(pol), _i(rows_a), nr, cls.ctypes.data_as(C.POINTER(C.c_int8))))
+    return cls
+
+
+def assemble(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, rows=None, trial=None,
+             out=None, accumulate=False, threads=0):
+    """alpha * operator block [rows, all trial columns], complex128 (Fortran order)."""
+    pol = pol or policy()
+    t_nodes, t_tris = _mesh(nodes, tris)
+    same = trial is None
+    s_nodes, s_tris = (t_nodes, t_tris) if same else _mesh(*trial)
+    rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nr = len(t_tris) if rows_a is None else len(rows_a)
+    if out is None:
+        out = np.zeros((nr, len(s_tris)), dtype=np.complex128, order="F")
+        accumulate = False
+    alpha = complex(alpha)
+    _check(lib().orc_assemble(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                              _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
+                              family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol), _i(rows_a), nr,
+                              out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0], int(accumulate), threads))
+    return out
+
+
+def assemble_bm(nodes, tris, k, alpha=None, pol=None, rows=None, threads=0):
+    """Burton-Miller (A, B) in NiHu's composition order; alpha defaults to -i/k."""
+    pol = pol or policy()
+    nodes, tris = _mesh(nodes, tris)
+    alpha = complex(0.0, -1.0 / k) if alpha is None else complex(alpha)
+    rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nr = len(tris) if rows_a is None else len(rows_a)
+    A = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
+    B = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
+    _check(lib().orc_assemble_bm(_d(nodes), len(nodes), _i(tris), len(tris), k, alpha.real, alpha.imag,
+                                 C.byref(pol), _i(rows_a), nr, A.ctypes.data_as(C.POINTER(C.c_double)),
+                                 B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
+    return A, B
+
+
+def sweep_bm(nodes, tris, ks, pol=None, rows=None, threads=0):
+    """NiHu's OpenMP-over-frequencies pattern; returns A, B of shape [n_k, n_rows, n_tri]."""
+    pol = pol or policy()
+    nodes, tris = _mesh(nodes, tris)
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nr = len(tris) if rows_a is None else len(rows_a)
+    A = np.zeros((len(ks), len(tris), nr), dtype=np.complex128)
+    B = np.zeros_like(A)
+    _check(lib().orc_sweep_bm(_d(nodes), len(nodes), _i(tris), len(tris), _d(ks), len(ks), C.byref(pol),
+                              _i(rows_a), nr, A.ctypes.data_as(C.POINTER(C.c_double)),
+                              B.ctypes.data_as(C.POINTER(C.c_double)), threads))
+    return A.transpose(0, 2, 1), B.transpose(0, 2, 1)
+
+
+def num_threads():
+    return lib().orc_num_threads()
