@@ -1,0 +1,157 @@
+/*
+ * nbem.h -- C ABI of the MI355X (gfx950) dense BEM assembly engine.
+ *
+ * The engine replaces the pair loop behind NiHu's expression
+ *     result << test_space * op[trial_space]
+ * for 3D Laplace / Helmholtz SLP, DLP, DLPt and HSP kernels on flat 3-node
+ * triangles: collocation (dirac(constant_view) x constant_view) and Galerkin
+ * (isoparametric_view x isoparametric_view).  Every entry point returns an
+ * nb_status; on failure nb_last_error() returns a message for the calling
+ * thread.  Plain pointers and sizes only.
+ *
+ * Reference interfaces (paths relative to the NiHu repository root):
+ *   nb_mesh_create   <- NiHu::create_mesh / mesh(nodes, elements)
+ *                       nihu/core/mesh.hpp:137-193, nihu/library/lib_mesh.hpp
+ *   nb_assemble      <- assembly<TestSpace, TrialSpace, OnSameMesh>::eval_into
+ *                       nihu/core/assembly.hpp:136-157, reached from
+ *                       integral_transform::test_on_into (integral_transform.hpp:150-192)
+ *                       and weighted_residual's operator<< (weighted_residual.hpp:197-206)
+ *   nb_assemble_bm   <- the Burton-Miller composition of six eval_into calls in
+ *                       nihu/applications/helmholtz/helmholtz_3d_coll.cpp:63-96
+ *   nb_assemble_sweep<- the per-frequency loop of
+ *                       benchmark/EAA/sphere/homogeneous/bem/bem.cpp:91-115
+ */
+#ifndef NBEM_H_INCLUDED
+#define NBEM_H_INCLUDED
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the C++ facade maps them to NiHu's exception types */
+enum nb_status {
+	NB_OK = 0,
+	NB_ERR_ARG = 1,          /* std::invalid_argument */
+	NB_ERR_UNSUPPORTED = 2,  /* std::logic_error: combination NiHu rejects or this engine lacks */
+	NB_ERR_RANGE = 3,        /* std::out_of_range: e.g. quadrature degree > 14 */
+	NB_ERR_OOM = 4,          /* std::bad_alloc: device memory */
+	NB_ERR_DEVICE = 5,       /* std::runtime_error: HIP error */
+	NB_ERR_RUNTIME = 6       /* std::runtime_error: other */
+};
+
+/* kernel families: nihu/library/laplace_kernel.hpp, helmholtz_kernel.hpp */
+enum nb_family { NB_LAPLACE = 0, NB_HELMHOLTZ = 1, NB_IDENTITY = 2 };
+
+/* regular-pair quadrature policy */
+enum nb_rule {
+	NB_RULE_NIHU = 0,   /* NiHu default: per-pair Dunavant degree from the accuracy-3 distance
+	                       tables (nihu/library/distance_kernel_intervals.hpp:116-155) */
+	NB_RULE_FIXED = 1   /* one Dunavant degree for every regular pair (5 -> 7 points) */
+};
+
+/* near-singular classification of a pair (test centre x, trial element j) */
+enum nb_near {
+	NB_NEAR_MEAN_EDGE = 0, /* |x - c_j| / R_j < limit, R_j mean edge (NiHu, limit 1.5) */
+	NB_NEAR_DIAMETER = 1,  /* |x - c_j| / diam_j < limit, diam_j longest edge */
+	NB_NEAR_NONE = 2       /* no near-singular pass */
+};
+
+enum nb_dtype { NB_C64 = 0, NB_C128 = 1, NB_F32 = 2, NB_F64 = 3 };
+enum nb_location { NB_HOST = 0, NB_DEVICE = 1 };
+enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1 };
+
+typedef struct nb_mesh nb_mesh;
+
+/* kernel: family, normal-derivative orders (nx, ny) in {0,1}^2, real wavenumber */
+typedef struct {
+	int32_t family;
+	int32_t nx;
+	int32_t ny;
+	double k;
+} nb_kernel;
+
+/* scalar multiplier of scaled_integral_operator (nihu/core/integral_operator.hpp:131-194) */
+typedef struct {
+	double re;
+	double im;
+} nb_scalar;
+
+typedef struct {
+	int32_t rule;          /* nb_rule */
+	int32_t fixed_degree;  /* Dunavant degree for NB_RULE_FIXED, 0..14 */
+	int32_t near_mode;     /* nb_near */
+	double near_limit;     /* 1.5 for NiHu */
+} nb_policy;
+
+/* Caller-owned result matrix, column-major: entry (i, j) at ptr[j * ld + i],
+ * i = test dof, j = trial dof.  accumulate = 1 adds into it as NiHu's
+ * matrix_block::operator+= does (nihu/util/matrix_block.hpp:59-65);
+ * accumulate = 0 overwrites every entry. */
+typedef struct {
+	void *ptr;
+	int64_t rows;
+	int64_t cols;
+	int64_t ld;
+	int32_t dtype;       /* nb_dtype */
+	int32_t location;    /* nb_location */
+	int32_t accumulate;
+} nb_out;
+
+/* Launch shape of the regular-pair kernels (0 = tuned default). */
+typedef struct {
+	int32_t tile_cols;     /* trial columns per workgroup */
+	int32_t k_batch;       /* wavenumbers evaluated per geometry pass (sweep) */
+	int32_t persistent;    /* sweep: workgroups per CU of the persistent grid */
+} nb_launch;
+
+const char *nb_last_error(void);
+int nb_version(void);
+int nb_device_info(char *name, int32_t name_len, int32_t *n_cu, int64_t *hbm_bytes);
+
+/* Upload a triangle mesh: xyz[n_nodes][3], tri[n_tri][3] (0-based node ids,
+ * element ids = row numbers).  Geometry (centres, normals, Jacobians, sizes)
+ * is computed in FP64 and kept resident on the current device. */
+int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32_t n_tri, nb_mesh **out);
+int nb_mesh_destroy(nb_mesh *mesh);
+int nb_mesh_info(const nb_mesh *mesh, int32_t *n_nodes, int32_t *n_tri, int64_t *device_bytes);
+
+/* result (+)= alpha * <test, K trial>.  same_mesh = 1 when test and trial are
+ * the same mesh object (NiHu decides this by pointer equality,
+ * integral_transform.hpp:150-162): then the singular (diagonal) pass runs and
+ * the identity family is defined.  stream: a hipStream_t or NULL. */
+int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, int32_t formalism,
+                nb_kernel kernel, nb_scalar alpha, const nb_policy *policy, const nb_out *out, void *stream);
+
+/* Burton-Miller collocation matrices in one fused pass over the pairs:
+ *   A = alpha*D - I/2 + H,   B = G + alpha*Ht + (alpha/2) I
+ * with G, H, Ht, D the Helmholtz SLP, DLP, DLPt, HSP operators. */
+int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_policy *policy,
+                   const nb_out *A, const nb_out *B, void *stream);
+
+/* Burton-Miller matrices for n_k wavenumbers on one mesh in one persistent
+ * launch (alpha = -i/k each).  A and B are device c64 arrays holding n_k
+ * matrices of n_tri x n_tri with leading dimension ld, matrix f at
+ * A + f * k_stride elements.  launch may be NULL (autotuned). */
+int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy,
+                      void *A, void *B, int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream);
+
+/* Pair classes as the engine sees them: cls[i * n_trial + j] = -1 diagonal,
+ * -2 near-singular, otherwise the regular Dunavant degree of kernel (nx, ny).
+ * cls is a device int8 array of n_test * n_trial. */
+int nb_classify(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, int32_t nx, int32_t ny,
+                const nb_policy *policy, int8_t *cls, void *stream);
+
+/* Choose the launch shape of nb_assemble_bm / nb_assemble_sweep for this mesh
+ * by timing candidates on the device; the result is cached per mesh size. */
+int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_launch *best);
+int nb_set_launch(const nb_launch *launch);
+
+int nb_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBEM_H_INCLUDED */

@@ -1,0 +1,53 @@
+"""Build libnbem.so (HIP, gfx950) in-tree.
+
+    python -m nihu_amd.build        # or __graft_entry__.build()
+
+The host geometry file is compiled with g++ -ffp-contract=off (pair
+classification must round exactly as written); the HIP translation unit with
+hipcc for gfx950 only.  Packed FP32 instructions are disabled for all device
+code (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(CSRC, "_build")
+LIB = os.path.join(HERE, "libnbem.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    geom_o = os.path.join(BUILD, "nbem_geom.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+          "-c", os.path.join(CSRC, "nbem_geom.cpp"), "-o", geom_o])
+    api_o = os.path.join(BUILD, "nbem_api.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "nbem_api.hip"), "-o", api_o])
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", api_o, geom_o, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,676 @@
+// nbem_api.hip -- C ABI of the engine (include/nbem.h): mesh residency,
+// per-mesh caches (classification thresholds, quadrature tables, near-pair
+// lists) and the launch sequence of one assembly:
+//   1. regular pairs, FP32 (regular_kernel)
+//   2. near-singular pairs, FP64 (fp64_pass_kernel<..., false>)
+//   3. diagonal pairs on the same mesh, FP64 (fp64_pass_kernel<..., true>)
+// The three passes write disjoint entries, so no atomics are used and every
+// entry is computed by one lane in a fixed order (results do not depend on
+// what else runs on the device).
+
+#include "../../include/nbem.h"
+#include "nbem_kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+extern "C" {
+int nbg_dunavant(int degree, double *xi, double *eta, double *w, int cap);
+void nbg_gauss_legendre(int n, double *x, double *w);
+int nbg_geometry(const double *xyz, int n_nodes, const int32_t *tri, int n, double *c, double *nu,
+	double *J, double *R, double *diam, double *X);
+double nbg_threshold(double scale, double limit);
+int nbg_quad_table(int degree, int n, const double *c, const double *J, const double *X, float *out);
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+	g_err = msg;
+	return code;
+}
+
+#define NB_HIP(expr)                                                                               \
+	do {                                                                                           \
+		hipError_t e_ = (expr);                                                                    \
+		if (e_ != hipSuccess) {                                                                    \
+			if (e_ == hipErrorOutOfMemory) return fail(NB_ERR_OOM, std::string(#expr) + ": out of device memory"); \
+			return fail(NB_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+		}                                                                                          \
+	} while (0)
+
+struct HipFree {
+	void operator()(void *p) const { if (p) (void)hipFree(p); }
+};
+typedef std::unique_ptr<void, HipFree> DevPtr;
+
+int dev_alloc(DevPtr &p, size_t bytes)
+{
+	void *q = nullptr;
+	if (bytes == 0) bytes = 16;
+	hipError_t e = hipMalloc(&q, bytes);
+	if (e == hipErrorOutOfMemory) return fail(NB_ERR_OOM, "hipMalloc: out of device memory");
+	if (e != hipSuccess) return fail(NB_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+	p.reset(q);
+	return NB_OK;
+}
+
+template <class T>
+int upload(DevPtr &p, const std::vector<T> &v)
+{
+	int s = dev_alloc(p, v.size() * sizeof(T));
+	if (s) return s;
+	NB_HIP(hipMemcpy(p.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+	return NB_OK;
+}
+
+struct NearList {
+	uint64_t test_uid;
+	int same, mode;
+	double limit;
+	int n;
+	DevPtr pi, pj;
+};
+
+struct LaunchCfg {
+	int tile_cols = 32;
+	int k_batch = 4;
+	int persistent = 4;
+};
+std::mutex g_cfg_mu;
+LaunchCfg g_cfg;
+
+// per-device rule tables of the FP64 pass
+std::mutex g_rules_mu;
+std::map<int, DevPtr> g_rules;
+
+int device_rules(const nbk::Rules **out)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_rules_mu);
+	auto it = g_rules.find(dev);
+	if (it == g_rules.end()) {
+		nbk::Rules r;
+		nbg_gauss_legendre(6, r.gl6x, r.gl6w);
+		nbg_gauss_legendre(21, r.gl21x, r.gl21w);
+		if (nbg_dunavant(7, r.d7xi, r.d7eta, r.d7w, 13) != 13) return fail(NB_ERR_RUNTIME, "Dunavant 7 table");
+		if (nbg_dunavant(14, r.d14xi, r.d14eta, r.d14w, 42) != 42) return fail(NB_ERR_RUNTIME, "Dunavant 14 table");
+		DevPtr p;
+		int s = dev_alloc(p, sizeof(r));
+		if (s) return s;
+		NB_HIP(hipMemcpy(p.get(), &r, sizeof(r), hipMemcpyHostToDevice));
+		it = g_rules.emplace(dev, std::move(p)).first;
+	}
+	*out = (const nbk::Rules *)it->second.get();
+	return NB_OK;
+}
+
+} // namespace
+
+struct nb_mesh {
+	uint64_t uid = 0;   // unique over the process: cache key that survives address reuse
+	int device = 0;
+	int n_nodes = 0, n = 0;
+	std::vector<double> c, nu, J, R, diam, X;   // SoA host copies (FP64)
+	DevPtr d_c, d_nuf, d_nu64, d_X, d_J;
+	int thr_mode = -1;
+	double thr_limit = 0.0;
+	DevPtr d_thr;
+	DevPtr d_qt[nbk::NSLOT];
+	int qn[nbk::NSLOT] = {0};
+	std::vector<std::unique_ptr<NearList> > near_cache;
+	std::mutex mu;
+	int64_t bytes = 0;
+};
+
+namespace {
+
+// squared-distance thresholds of the trial mesh for a near-singular policy
+int ensure_thresholds(nb_mesh *m, int mode, double limit)
+{
+	if (m->d_thr && m->thr_mode == mode && m->thr_limit == limit) return NB_OK;
+	const int n = m->n;
+	// breakpoints x = num / den of the accuracy-3 tables, evaluated as NiHu's
+	// tmp::break_point::x_value does
+	static const double bp[nbk::NTHR - 1] = {10. / 10., 11. / 10., 14. / 10., 15. / 10., 18. / 10., 19. / 10.,
+		28. / 10., 36. / 10., 91. / 10., 158. / 10., 223. / 10.};
+	std::vector<double> thr((size_t)nbk::NTHR * n);
+	for (int e = 0; e < n; ++e) {
+		double scale = mode == NB_NEAR_DIAMETER ? m->diam[e] : m->R[e];
+		thr[e] = mode == NB_NEAR_NONE ? 0.0 : nbg_threshold(scale, limit);
+		for (int t = 1; t < nbk::NTHR; ++t) thr[(size_t)t * n + e] = nbg_threshold(m->R[e], bp[t - 1]);
+	}
+	int s = upload(m->d_thr, thr);
+	if (s) return s;
+	m->thr_mode = mode;
+	m->thr_limit = limit;
+	return NB_OK;
+}
+
+int ensure_quad(nb_mesh *m, int degree)
+{
+	if (degree < 0 || degree > 14) return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
+	if (m->d_qt[degree]) return NB_OK;
+	double xi[64], eta[64], w[64];
+	int Q = nbg_dunavant(degree, xi, eta, w, 64);
+	std::vector<float> tab((size_t)m->n * Q * 8);
+	if (nbg_quad_table(degree, m->n, m->c.data(), m->J.data(), m->X.data(), tab.data()) != Q)
+		return fail(NB_ERR_RUNTIME, "quadrature table");
+	int s = upload(m->d_qt[degree], tab);
+	if (s) return s;
+	m->qn[degree] = Q;
+	m->bytes += (int64_t)tab.size() * 4;
+	return NB_OK;
+}
+
+// degrees the regular pass may select under a policy (kernel orders in km)
+void policy_degrees(const nb_policy &p, int km, int field_complexity, std::vector<int> &deg)
+{
+	deg.clear();
+	if (p.rule == NB_RULE_FIXED) { deg.push_back(p.fixed_degree); return; }
+	static const int d1[] = {6, 4, 2, 0}, d2[] = {8, 6, 4, 2, 0}, d3[] = {9, 8, 6, 4, 2, 0};
+	auto add = [&](const int *d, int n) { for (int t = 0; t < n; ++t) deg.push_back(field_complexity + d[t]); };
+	if (km & nbk::SLP) add(d1, 4);
+	if (km & (nbk::DLP | nbk::DLPT)) add(d2, 5);
+	if (km & nbk::HSP) add(d3, 6);
+	std::sort(deg.begin(), deg.end());
+	deg.erase(std::unique(deg.begin(), deg.end()), deg.end());
+}
+
+int ensure_near_list(nb_mesh *trial, const nb_mesh *test, int same, const nb_policy &pol, hipStream_t st,
+	NearList **out)
+{
+	for (auto &nl : trial->near_cache)
+		if (nl->test_uid == test->uid && nl->same == same && nl->mode == pol.near_mode && nl->limit == pol.near_limit) {
+			*out = nl.get();
+			return NB_OK;
+		}
+	std::unique_ptr<NearList> nl(new NearList());
+	nl->test_uid = test->uid; nl->same = same; nl->mode = pol.near_mode; nl->limit = pol.near_limit; nl->n = 0;
+	if (pol.near_mode != NB_NEAR_NONE) {
+		nbk::ListArgs a;
+		a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = same;
+		a.row_tiles = (test->n + 255) / 256;
+		a.chunk = 1024;
+		a.n_chunks = (trial->n + a.chunk - 1) / a.chunk;
+		a.tn = test->n; a.sn = trial->n;
+		a.tc = (const double *)test->d_c.get();
+		a.sc = (const double *)trial->d_c.get();
+		a.thr_near = (const double *)trial->d_thr.get();
+		const size_t ncnt = (size_t)a.n_rows * a.n_chunks;
+		DevPtr counts, offsets, tmp;
+		int s;
+		if ((s = dev_alloc(counts, ncnt * sizeof(int)))) return s;
+		if ((s = dev_alloc(offsets, ncnt * sizeof(int)))) return s;
+		a.counts = (int *)counts.get();
+		a.offsets = (const int *)offsets.get();
+		a.pi = a.pj = nullptr;
+		const int grid = a.row_tiles * a.n_chunks;
+		hipLaunchKernelGGL(nbk::near_list_kernel<false>, dim3(grid), dim3(256), 0, st, a);
+		NB_HIP(hipGetLastError());
+		size_t tmp_bytes = 0;
+		NB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int *)counts.get(), (int *)offsets.get(), (int)ncnt, st));
+		if ((s = dev_alloc(tmp, tmp_bytes))) return s;
+		NB_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tmp_bytes, (int *)counts.get(), (int *)offsets.get(), (int)ncnt, st));
+		int last[2];
+		NB_HIP(hipMemcpyAsync(&last[0], (int *)offsets.get() + ncnt - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+		NB_HIP(hipMemcpyAsync(&last[1], (int *)counts.get() + ncnt - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+		NB_HIP(hipStreamSynchronize(st));
+		nl->n = last[0] + last[1];
+		if ((s = dev_alloc(nl->pi, (size_t)nl->n * sizeof(int)))) return s;
+		if ((s = dev_alloc(nl->pj, (size_t)nl->n * sizeof(int)))) return s;
+		a.pi = (int *)nl->pi.get();
+		a.pj = (int *)nl->pj.get();
+		hipLaunchKernelGGL(nbk::near_list_kernel<true>, dim3(grid), dim3(256), 0, st, a);
+		NB_HIP(hipGetLastError());
+		NB_HIP(hipStreamSynchronize(st));   // counts/offsets are freed on return
+		trial->bytes += (int64_t)nl->n * 8;
+	}
+	if (trial->near_cache.size() >= 8) trial->near_cache.erase(trial->near_cache.begin());
+	trial->near_cache.push_back(std::move(nl));
+	*out = trial->near_cache.back().get();
+	return NB_OK;
+}
+
+int dtype_bytes(int dt)
+{
+	switch (dt) {
+	case NB_C64: return 8;
+	case NB_C128: return 16;
+	case NB_F32: return 4;
+	case NB_F64: return 8;
+	default: return 0;
+	}
+}
+
+int check_out(const nb_out *o, int64_t rows, int64_t cols, bool complex_needed)
+{
+	if (!o) return fail(NB_ERR_ARG, "null output descriptor");
+	if (!o->ptr) return fail(NB_ERR_ARG, "null output pointer");
+	if (dtype_bytes(o->dtype) == 0) return fail(NB_ERR_ARG, "unknown output dtype");
+	if (o->location != NB_HOST && o->location != NB_DEVICE) return fail(NB_ERR_ARG, "unknown output location");
+	if (o->rows != rows || o->cols != cols)
+		return fail(NB_ERR_ARG, "output size must be n_test x n_trial");
+	if (o->ld < rows) return fail(NB_ERR_ARG, "leading dimension smaller than the number of rows");
+	if (complex_needed && (o->dtype == NB_F32 || o->dtype == NB_F64))
+		return fail(NB_ERR_ARG, "complex result into a real output");
+	return NB_OK;
+}
+
+struct Job {
+	const nb_mesh *test;
+	nb_mesh *trial;
+	int same, family, km, mode;
+	double k;
+	nb_scalar alpha;
+	nb_policy pol;
+	int tile_cols;
+};
+
+// one assembly into device buffers (dtype, ld, accumulate taken from outs)
+int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	nb_mesh *trial = jb.trial;
+	const nb_mesh *test = jb.test;
+	int s;
+	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
+	std::vector<int> degs;
+	policy_degrees(jb.pol, jb.km, 0, degs);
+	for (int d : degs)
+		if ((s = ensure_quad(trial, d))) return s;
+	NearList *nl = nullptr;
+	if ((s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
+
+	// 1. regular pairs
+	nbk::RegArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = jb.same;
+	a.tile_cols = jb.tile_cols > 0 ? jb.tile_cols : 32;
+	a.row_tiles = (test->n + 255) / 256;
+	a.tn = test->n; a.sn = trial->n;
+	a.tc = (const double *)test->d_c.get();
+	a.tnu = (const float *)test->d_nuf.get();
+	a.sc = (const double *)trial->d_c.get();
+	a.snu = (const float *)trial->d_nuf.get();
+	a.thr = (const double *)trial->d_thr.get();
+	for (int d = 0; d < nbk::NSLOT; ++d) { a.qt[d] = (const nbk::QPoint *)trial->d_qt[d].get(); a.qn[d] = trial->qn[d]; }
+	a.fixed_degree = jb.pol.rule == NB_RULE_FIXED ? jb.pol.fixed_degree : -1;
+	a.field_complexity = 0;
+	const double two_pi = 6.283185307179586476925286766559005768;
+	a.k = (float)jb.k;
+	a.k_rev = jb.k / two_pi;
+	a.k_revf = (float)a.k_rev;
+	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
+	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
+	const long long col_tiles = (trial->n + a.tile_cols - 1) / a.tile_cols;
+	const long long grid = (long long)a.row_tiles * col_tiles;
+	if (grid > 0) {
+		using namespace nbk;
+		if (jb.mode == OUT_BM) {
+			hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM>), dim3(grid), dim3(256), 0, st, a);
+		} else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: hipLaunchKernelGGL((regular_kernel<HELM, SLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLP: hipLaunchKernelGGL((regular_kernel<HELM, DLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLPT: hipLaunchKernelGGL((regular_kernel<HELM, DLPT, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			default: hipLaunchKernelGGL((regular_kernel<HELM, HSP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: hipLaunchKernelGGL((regular_kernel<LAP, SLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLP: hipLaunchKernelGGL((regular_kernel<LAP, DLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLPT: hipLaunchKernelGGL((regular_kernel<LAP, DLPT, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			default: hipLaunchKernelGGL((regular_kernel<LAP, HSP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			}
+		}
+		NB_HIP(hipGetLastError());
+	}
+
+	// 2./3. near-singular and diagonal pairs in FP64
+	const nbk::Rules *rules = nullptr;
+	if ((s = device_rules(&rules))) return s;
+	nbk::NearArgs na;
+	std::memset(&na, 0, sizeof(na));
+	na.tn = test->n; na.sn = trial->n;
+	na.tc = (const double *)test->d_c.get();
+	na.tnu = (const double *)test->d_nu64.get();
+	na.X = (const double *)trial->d_X.get();
+	na.sJ = (const double *)trial->d_J.get();
+	na.snu = (const double *)trial->d_nu64.get();
+	na.k = jb.k;
+	na.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+	na.out0 = out0; na.out1 = out1; na.ld = ld; na.dtype = dtype; na.accumulate = accumulate;
+	na.rules = rules;
+	for (int pass = 0; pass < 2; ++pass) {
+		const bool diag = pass == 1;
+		if (diag && !jb.same) continue;
+		na.n_pairs = diag ? test->n : nl->n;
+		na.pi = diag ? nullptr : (const int *)nl->pi.get();
+		na.pj = diag ? nullptr : (const int *)nl->pj.get();
+		if (na.n_pairs == 0) continue;
+		const int g = (na.n_pairs + 63) / 64;
+		using namespace nbk;
+#define NB_FP64(F, K, M) do { \
+		if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true>), dim3(g), dim3(64), 0, st, na); \
+		else hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, false>), dim3(g), dim3(64), 0, st, na); } while (0)
+		if (jb.mode == OUT_BM) NB_FP64(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_FP64(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_FP64(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_FP64(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_FP64(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_FP64(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_FP64(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_FP64(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_FP64(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_FP64
+		NB_HIP(hipGetLastError());
+	}
+	return NB_OK;
+}
+
+void host_add(void *dst, int64_t ld, const void *src, int64_t rows, int64_t cols, int dtype, bool accumulate)
+{
+	const int eb = dtype_bytes(dtype);
+	unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+	std::vector<std::thread> th;
+	for (unsigned t = 0; t < nt; ++t) {
+		th.emplace_back([=]() {
+			for (int64_t j = t; j < cols; j += nt) {
+				char *d = (char *)dst + j * ld * eb;
+				const char *sp = (const char *)src + j * rows * eb;
+				if (!accumulate) { std::memcpy(d, sp, rows * eb); continue; }
+				switch (dtype) {
+				case NB_C64: case NB_F32: {
+					float *df = (float *)d; const float *sf = (const float *)sp;
+					for (int64_t q = 0; q < rows * eb / 4; ++q) df[q] += sf[q];
+					break;
+				}
+				default: {
+					double *dd = (double *)d; const double *sd = (const double *)sp;
+					for (int64_t q = 0; q < rows * eb / 8; ++q) dd[q] += sd[q];
+					break;
+				}
+				}
+			}
+		});
+	}
+	for (auto &t : th) t.join();
+}
+
+// run a job into up to two outputs (both device, or both host)
+int run_job(const Job &jb, const nb_out *o0, const nb_out *o1, hipStream_t st)
+{
+	if (o1 && (o0->location != o1->location || o0->dtype != o1->dtype || o0->accumulate != o1->accumulate))
+		return fail(NB_ERR_ARG, "A and B must share location, dtype and accumulate");
+	const int64_t rows = o0->rows, cols = o0->cols;
+	if (o0->location == NB_DEVICE) {
+		if (o1 && o0->ld != o1->ld) return fail(NB_ERR_ARG, "A and B must share the leading dimension");
+		return run_device(jb, o0->ptr, o1 ? o1->ptr : nullptr, o0->ld, o0->dtype, o0->accumulate, st);
+	}
+	const size_t bytes = (size_t)rows * cols * dtype_bytes(o0->dtype);
+	DevPtr t0, t1;
+	int s;
+	if ((s = dev_alloc(t0, bytes))) return s;
+	if (o1 && (s = dev_alloc(t1, bytes))) return s;
+	if ((s = run_device(jb, t0.get(), o1 ? t1.get() : nullptr, rows, o0->dtype, 0, st))) return s;
+	std::vector<char> h(bytes);
+	const nb_out *outs[2] = {o0, o1};
+	void *tmps[2] = {t0.get(), o1 ? t1.get() : nullptr};
+	for (int q = 0; q < 2; ++q) {
+		if (!outs[q]) continue;
+		NB_HIP(hipMemcpyAsync(h.data(), tmps[q], bytes, hipMemcpyDeviceToHost, st));
+		NB_HIP(hipStreamSynchronize(st));
+		host_add(outs[q]->ptr, outs[q]->ld, h.data(), rows, cols, outs[q]->dtype, outs[q]->accumulate != 0);
+	}
+	return NB_OK;
+}
+
+int check_policy(const nb_policy *p)
+{
+	if (!p) return fail(NB_ERR_ARG, "null policy");
+	if (p->rule != NB_RULE_NIHU && p->rule != NB_RULE_FIXED) return fail(NB_ERR_ARG, "unknown quadrature rule");
+	if (p->rule == NB_RULE_FIXED && (p->fixed_degree < 0 || p->fixed_degree > 14))
+		return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
+	if (p->near_mode < NB_NEAR_MEAN_EDGE || p->near_mode > NB_NEAR_NONE) return fail(NB_ERR_ARG, "unknown near-singular mode");
+	if (p->near_mode != NB_NEAR_NONE && !(p->near_limit > 0.0)) return fail(NB_ERR_ARG, "near limit must be positive");
+	return NB_OK;
+}
+
+int kernel_mask(int nx, int ny)
+{
+	if (nx == 0 && ny == 0) return nbk::SLP;
+	if (nx == 0 && ny == 1) return nbk::DLP;
+	if (nx == 1 && ny == 0) return nbk::DLPT;
+	return nbk::HSP;
+}
+
+int current_device_matches(const nb_mesh *m)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	if (dev != m->device) return fail(NB_ERR_ARG, "mesh lives on another device");
+	return NB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *nb_last_error(void) { return g_err.c_str(); }
+
+int nb_version(void) { return 10000; }
+
+int nb_device_info(char *name, int32_t name_len, int32_t *n_cu, int64_t *hbm_bytes)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	hipDeviceProp_t pr;
+	NB_HIP(hipGetDeviceProperties(&pr, dev));
+	if (name && name_len > 0) {
+		std::strncpy(name, pr.gcnArchName, name_len - 1);
+		name[name_len - 1] = 0;
+	}
+	if (n_cu) *n_cu = pr.multiProcessorCount;
+	if (hbm_bytes) *hbm_bytes = (int64_t)pr.totalGlobalMem;
+	return NB_OK;
+}
+
+int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32_t n_tri, nb_mesh **out)
+{
+	if (!out) return fail(NB_ERR_ARG, "null output handle");
+	*out = nullptr;
+	if (!xyz || !tri || n_nodes <= 0 || n_tri <= 0) return fail(NB_ERR_ARG, "empty mesh");
+	static std::atomic<uint64_t> next_uid(1);
+	std::unique_ptr<nb_mesh> m(new nb_mesh());
+	m->uid = next_uid++;
+	const int n = n_tri;
+	m->n_nodes = n_nodes;
+	m->n = n;
+	m->c.resize(3 * (size_t)n); m->nu.resize(3 * (size_t)n); m->J.resize(n); m->R.resize(n);
+	m->diam.resize(n); m->X.resize(9 * (size_t)n);
+	int g = nbg_geometry(xyz, n_nodes, tri, n, m->c.data(), m->nu.data(), m->J.data(), m->R.data(),
+		m->diam.data(), m->X.data());
+	if (g == 1) return fail(NB_ERR_ARG, "node index out of range");
+	if (g == 2) return fail(NB_ERR_ARG, "degenerate element (zero area)");
+	NB_HIP(hipGetDevice(&m->device));
+	std::vector<float> nuf(m->nu.begin(), m->nu.end());
+	int s;
+	if ((s = upload(m->d_c, m->c)) || (s = upload(m->d_nu64, m->nu)) || (s = upload(m->d_nuf, nuf)) ||
+		(s = upload(m->d_X, m->X)) || (s = upload(m->d_J, m->J)))
+		return s;
+	m->bytes = (int64_t)n * (3 * 8 + 3 * 8 + 3 * 4 + 9 * 8 + 8);
+	*out = m.release();
+	return NB_OK;
+}
+
+int nb_mesh_destroy(nb_mesh *mesh)
+{
+	delete mesh;
+	return NB_OK;
+}
+
+int nb_mesh_info(const nb_mesh *mesh, int32_t *n_nodes, int32_t *n_tri, int64_t *device_bytes)
+{
+	if (!mesh) return fail(NB_ERR_ARG, "null mesh");
+	if (n_nodes) *n_nodes = mesh->n_nodes;
+	if (n_tri) *n_tri = mesh->n;
+	if (device_bytes) *device_bytes = mesh->bytes;
+	return NB_OK;
+}
+
+int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, int32_t formalism,
+	nb_kernel kernel, nb_scalar alpha, const nb_policy *policy, const nb_out *out, void *stream)
+{
+	if (!test || !trial) return fail(NB_ERR_ARG, "null mesh");
+	if (same_mesh && test != trial) return fail(NB_ERR_ARG, "same_mesh requires test == trial");
+	if (formalism != NB_COLLOCATION)
+		return fail(NB_ERR_UNSUPPORTED, "only the collocational formalism is implemented");
+	int s;
+	if ((s = current_device_matches(test)) || (s = current_device_matches(trial))) return s;
+	hipStream_t st = (hipStream_t)stream;
+	if (kernel.family == NB_IDENTITY) {
+		if ((s = check_out(out, test->n, trial->n, alpha.im != 0.0))) return s;
+		// local operator: nothing on different meshes (nihu/core/assembly.hpp:141-142)
+		if (!same_mesh) return NB_OK;
+		if (out->location == NB_HOST) {
+			const int eb = dtype_bytes(out->dtype);
+			for (int64_t j = 0; j < out->cols; ++j) {
+				char *col = (char *)out->ptr + j * out->ld * eb;
+				if (!out->accumulate) std::memset(col, 0, out->rows * eb);
+				char *e = col + j * eb;
+				switch (out->dtype) {
+				case NB_C64: ((float *)e)[0] += (float)alpha.re; ((float *)e)[1] += (float)alpha.im; break;
+				case NB_C128: ((double *)e)[0] += alpha.re; ((double *)e)[1] += alpha.im; break;
+				case NB_F32: *(float *)e += (float)alpha.re; break;
+				default: *(double *)e += alpha.re; break;
+				}
+			}
+			return NB_OK;
+		}
+		if (!out->accumulate) {
+			const int eb = dtype_bytes(out->dtype);
+			const long long words = out->rows * eb / 4 * out->cols;
+			hipLaunchKernelGGL(nbk::zero_kernel, dim3((words + 255) / 256), dim3(256), 0, st, out->ptr, (long long)out->ld,
+				(long long)out->rows, (long long)out->cols, eb);
+		}
+		hipLaunchKernelGGL(nbk::identity_kernel, dim3((test->n + 255) / 256), dim3(256), 0, st, out->ptr,
+			(long long)out->ld, test->n, out->dtype, alpha.re, alpha.im);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	}
+	if (kernel.family != NB_LAPLACE && kernel.family != NB_HELMHOLTZ) return fail(NB_ERR_ARG, "unknown kernel family");
+	if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
+		return fail(NB_ERR_UNSUPPORTED, "normal derivative orders must be 0 or 1");
+	if (kernel.family == NB_HELMHOLTZ && !(kernel.k > 0.0)) return fail(NB_ERR_ARG, "wavenumber must be positive");
+	if ((s = check_policy(policy))) return s;
+	bool cplx = kernel.family == NB_HELMHOLTZ || alpha.im != 0.0;
+	if ((s = check_out(out, test->n, trial->n, cplx))) return s;
+	nb_mesh *tr = const_cast<nb_mesh *>(trial);
+	std::lock_guard<std::mutex> lk(tr->mu);
+	Job jb;
+	jb.test = test; jb.trial = tr; jb.same = same_mesh ? 1 : 0; jb.family = kernel.family;
+	jb.km = kernel_mask(kernel.nx, kernel.ny); jb.mode = nbk::OUT_SINGLE;
+	jb.k = kernel.family == NB_HELMHOLTZ ? kernel.k : 0.0;
+	jb.alpha = alpha; jb.pol = *policy;
+	{
+		std::lock_guard<std::mutex> lc(g_cfg_mu);
+		jb.tile_cols = g_cfg.tile_cols;
+	}
+	return run_job(jb, out, nullptr, st);
+}
+
+int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_policy *policy,
+	const nb_out *A, const nb_out *B, void *stream)
+{
+	if (!mesh) return fail(NB_ERR_ARG, "null mesh");
+	if (!(k > 0.0)) return fail(NB_ERR_ARG, "wavenumber must be positive");
+	int s;
+	if ((s = current_device_matches(mesh))) return s;
+	if ((s = check_policy(policy))) return s;
+	if ((s = check_out(A, mesh->n, mesh->n, true)) || (s = check_out(B, mesh->n, mesh->n, true))) return s;
+	nb_mesh *m = const_cast<nb_mesh *>(mesh);
+	std::lock_guard<std::mutex> lk(m->mu);
+	Job jb;
+	jb.test = m; jb.trial = m; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
+	jb.k = k; jb.alpha = alpha; jb.pol = *policy;
+	{
+		std::lock_guard<std::mutex> lc(g_cfg_mu);
+		jb.tile_cols = g_cfg.tile_cols;
+	}
+	return run_job(jb, A, B, (hipStream_t)stream);
+}
+
+int nb_classify(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, int32_t nx, int32_t ny,
+	const nb_policy *policy, int8_t *cls, void *stream)
+{
+	if (!test || !trial || !cls) return fail(NB_ERR_ARG, "null argument");
+	if (same_mesh && test != trial) return fail(NB_ERR_ARG, "same_mesh requires test == trial");
+	int s;
+	if ((s = check_policy(policy))) return s;
+	nb_mesh *tr = const_cast<nb_mesh *>(trial);
+	std::lock_guard<std::mutex> lk(tr->mu);
+	if ((s = ensure_thresholds(tr, policy->near_mode, policy->near_limit))) return s;
+	nbk::ClassArgs a;
+	a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = same_mesh ? 1 : 0;
+	a.tn = test->n; a.sn = trial->n; a.order = 1 + nx + ny;
+	a.fixed_degree = policy->rule == NB_RULE_FIXED ? policy->fixed_degree : -1;
+	a.field_complexity = 0;
+	a.tc = (const double *)test->d_c.get(); a.sc = (const double *)trial->d_c.get();
+	a.thr = (const double *)trial->d_thr.get();
+	a.cls = cls;
+	const long long n = (long long)a.n_rows * a.n_cols;
+	hipLaunchKernelGGL(nbk::classify_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
+	NB_HIP(hipGetLastError());
+	return NB_OK;
+}
+
+int nb_assemble_sweep(const nb_mesh *, const double *, int32_t, const nb_policy *, void *, void *, int64_t, int64_t,
+	const nb_launch *, void *)
+{
+	return fail(NB_ERR_UNSUPPORTED, "sweep not built yet");
+}
+
+int nb_autotune(const nb_mesh *, int32_t, const nb_policy *, nb_launch *)
+{
+	return fail(NB_ERR_UNSUPPORTED, "autotuner not built yet");
+}
+
+int nb_set_launch(const nb_launch *launch)
+{
+	if (!launch) return fail(NB_ERR_ARG, "null launch");
+	if (launch->tile_cols < 0 || launch->tile_cols > 4096) return fail(NB_ERR_ARG, "tile_cols out of range");
+	std::lock_guard<std::mutex> lc(g_cfg_mu);
+	if (launch->tile_cols) g_cfg.tile_cols = launch->tile_cols;
+	if (launch->k_batch) g_cfg.k_batch = launch->k_batch;
+	if (launch->persistent) g_cfg.persistent = launch->persistent;
+	return NB_OK;
+}
+
+int nb_synchronize(void *stream)
+{
+	NB_HIP(hipStreamSynchronize((hipStream_t)stream));
+	return NB_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,658 @@
+// nbem_kernels.hpp -- HIP kernels of the dense BEM assembly engine (gfx950).
+//
+// Data layout in HBM (per mesh, structure of arrays, n = number of triangles):
+//   centres   c[3][n]  FP64        unit normals nu[3][n] FP32 and FP64
+//   vertices  X[9][n]  FP64        Jacobians J[n] FP64
+//   thresholds thr[NTHR][n] FP64: squared-distance limits per trial element
+//              (slot 0 near-singular, slots 1.. the distance-table breakpoints)
+//   quadrature tables per Dunavant degree: QPoint[n][Q] FP32 offsets of the
+//              points from the element centre, weight*J/(4 pi), |offset|^2
+// Result matrices are column-major: entry (i, j) at out[j * ld + i].
+//
+// Regular pairs: one lane per test row i, a workgroup of 256 lanes sweeps a
+// tile of trial columns; the column index is wave-uniform so trial data come
+// through scalar loads and a column's 64 results leave as one 512-byte store.
+// Near-singular and diagonal pairs: one lane per pair in FP64.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbk {
+
+enum { SLP = 1, DLP = 2, DLPT = 4, HSP = 8, ALL4 = 15 };
+enum { LAP = 0, HELM = 1 };
+enum { OUT_SINGLE = 0, OUT_BM = 1 };
+enum { DT_C64 = 0, DT_C128 = 1, DT_F32 = 2, DT_F64 = 3 };
+
+// squared-distance threshold slots; the breakpoint values are those of the
+// accuracy-3 distance tables (nihu/library/distance_kernel_intervals.hpp:116-155)
+enum { THR_NEAR = 0, THR_1_0, THR_1_1, THR_1_4, THR_1_5, THR_1_8, THR_1_9, THR_2_8, THR_3_6, THR_9_1,
+	THR_15_8, THR_22_3, NTHR };
+
+constexpr int NSLOT = 16;   // quadrature tables indexed by Dunavant degree 0..14 (+1 spare)
+
+struct QPoint { float ox, oy, oz, w, oo, pad0, pad1, pad2; };
+
+struct RegArgs {
+	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	int tn, sn;                  // element counts of the test / trial mesh (SoA strides)
+	const double *tc;            // test centres [3][tn]
+	const float *tnu;            // test unit normals [3][tn]
+	const double *sc;            // trial centres [3][sn]
+	const float *snu;            // trial unit normals [3][sn]
+	const double *thr;           // [NTHR][sn]
+	const QPoint *qt[NSLOT];     // per degree
+	int qn[NSLOT];
+	int fixed_degree;            // >= 0: fixed rule
+	int field_complexity;        // added to the table degree (0 for constant fields)
+	float k, k_revf;             // wavenumber, k / (2 pi)
+	double k_rev;
+	float2 alpha;
+	void *out0, *out1;
+	long long ld;
+	int dtype, accumulate;
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double d2_pair(double ax, double ay, double az, double bx, double by, double bz)
+{
+	double dx = ax - bx, dy = ay - by, dz = az - bz;
+	return __fma_rn(dz, dz, __fma_rn(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ void store_entry(void *out, long long idx, int dtype, int accumulate, float re, float im)
+{
+	switch (dtype) {
+	case DT_C64: {
+		float2 *p = (float2 *)out + idx;
+		if (accumulate) { float2 v = *p; v.x += re; v.y += im; *p = v; }
+		else *p = make_float2(re, im);
+		break;
+	}
+	case DT_C128: {
+		double2 *p = (double2 *)out + idx;
+		if (accumulate) { double2 v = *p; v.x += re; v.y += im; *p = v; }
+		else *p = make_double2(re, im);
+		break;
+	}
+	case DT_F32: {
+		float *p = (float *)out + idx;
+		if (accumulate) *p += re; else *p = re;
+		break;
+	}
+	default: {
+		double *p = (double *)out + idx;
+		if (accumulate) *p += re; else *p = re;
+		break;
+	}
+	}
+}
+
+__device__ __forceinline__ void store_entry64(void *out, long long idx, int dtype, int accumulate, double re, double im)
+{
+	switch (dtype) {
+	case DT_C64: {
+		float2 *p = (float2 *)out + idx;
+		if (accumulate) { float2 v = *p; *p = make_float2((float)(v.x + re), (float)(v.y + im)); }
+		else *p = make_float2((float)re, (float)im);
+		break;
+	}
+	case DT_C128: {
+		double2 *p = (double2 *)out + idx;
+		if (accumulate) { double2 v = *p; v.x += re; v.y += im; *p = v; }
+		else *p = make_double2(re, im);
+		break;
+	}
+	case DT_F32: {
+		float *p = (float *)out + idx;
+		if (accumulate) *p = (float)(*p + re); else *p = (float)re;
+		break;
+	}
+	default: {
+		double *p = (double *)out + idx;
+		if (accumulate) *p += re; else *p = re;
+		break;
+	}
+	}
+}
+
+// Regular Dunavant degree of one pair from the squared-distance thresholds of
+// the trial element: the first breakpoint x with d/R < x (nihu/tmp/interval.hpp:173-185).
+template <int ORDER>
+__device__ __forceinline__ int table_degree(double d2, const double *thr, int sn, int j)
+{
+	if (ORDER == 1) {          // inverse<1>: 1.1 -> 6, 1.8 -> 4, 9.1 -> 2, else 0
+		if (d2 < thr[THR_1_1 * sn + j]) return 6;
+		if (d2 < thr[THR_1_8 * sn + j]) return 4;
+		if (d2 < thr[THR_9_1 * sn + j]) return 2;
+		return 0;
+	} else if (ORDER == 2) {   // inverse<2>: 1.1 -> 8, 1.5 -> 6, 2.8 -> 4, 15.8 -> 2, else 0
+		if (d2 < thr[THR_1_1 * sn + j]) return 8;
+		if (d2 < thr[THR_1_5 * sn + j]) return 6;
+		if (d2 < thr[THR_2_8 * sn + j]) return 4;
+		if (d2 < thr[THR_15_8 * sn + j]) return 2;
+		return 0;
+	} else {                   // inverse<3>: 1.0, 1.1 -> 9, 1.4 -> 8, 1.9 -> 6, 3.6 -> 4, 22.3 -> 2
+		if (d2 < thr[THR_1_1 * sn + j]) return 9;
+		if (d2 < thr[THR_1_4 * sn + j]) return 8;
+		if (d2 < thr[THR_1_9 * sn + j]) return 6;
+		if (d2 < thr[THR_3_6 * sn + j]) return 4;
+		if (d2 < thr[THR_22_3 * sn + j]) return 2;
+		return 0;
+	}
+}
+
+struct PairCtx {
+	float dcx, dcy, dcz;       // c_j - c_i (FP32)
+	float dcx2, dcy2, dcz2;    // 2 (c_j - c_i)
+	float df, t0;              // |c_j - c_i|, fract(k |c_j - c_i| / 2pi) in [-1/2, 1/2]
+	float nxx, nxy, nxz;       // test unit normal
+	float nyx, nyy, nyz;       // trial unit normal
+	float nxny;
+};
+
+// sum over the points of one quadrature table of the kernels in MASK:
+//   SLP  e/(4 pi r)                         DLP  e(-1-ikr)/(4 pi r^2) (r.ny)/r
+//   DLPt e(-1-ikr)/(4 pi r^2) (-r.nx)/r     HSP  e/(4 pi r^3)[(3+3ikr-(kr)^2) rdnx rdny + (1+ikr) nx.ny]
+// with e = exp(-ikr), r = y - x (nihu/library/helmholtz_kernel.hpp:169-228,
+// normal_derivative_kernel.hpp:114-257); weights carry J w / (4 pi).
+template <int FAM, int MASK>
+__device__ __forceinline__ void accumulate_points(const QPoint *__restrict__ qp, int nq, const PairCtx &p,
+	float k_revf, float k, float2 (&s)[4])
+{
+	for (int q = 0; q < nq; ++q) {
+		const QPoint P = qp[q];
+		const float rx = p.dcx + P.ox, ry = p.dcy + P.oy, rz = p.dcz + P.oz;
+		const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+		const float rinv = __builtin_amdgcn_rsqf(r2);
+		const float g = P.w * rinv;
+		float ec, es, th = 0.0f;
+		if (FAM == HELM) {
+			const float r = r2 * rinv;
+			// r - d = (2 dc.off + |off|^2) / (r + d): accurate phase correction
+			const float u = __builtin_fmaf(p.dcx2, P.ox, __builtin_fmaf(p.dcy2, P.oy, __builtin_fmaf(p.dcz2, P.oz, P.oo)));
+			const float delta = u * __builtin_amdgcn_rcpf(r + p.df);
+			const float t = __builtin_fmaf(k_revf, delta, p.t0);
+			float sn, cs;
+			sincospif(2.0f * t, &sn, &cs);
+			ec = g * cs;
+			es = -g * sn;
+			th = k * r;
+		} else {
+			ec = g;
+			es = 0.0f;
+		}
+		if (MASK & SLP) { s[0].x += ec; s[0].y += es; }
+		if (MASK & (DLP | DLPT | HSP)) {
+			float rdny = 0.0f, rdnx = 0.0f;
+			if (MASK & (DLP | HSP)) rdny = (rx * p.nyx + ry * p.nyy + rz * p.nyz) * rinv;
+			if (MASK & (DLPT | HSP)) rdnx = -(rx * p.nxx + ry * p.nxy + rz * p.nxz) * rinv;
+			if (MASK & (DLP | DLPT)) {
+				// E (-1 - i th) / r
+				const float f1r = __builtin_fmaf(th, es, -ec) * rinv;
+				const float f1i = __builtin_fmaf(-th, ec, -es) * rinv;
+				if (MASK & DLP) { s[1].x += f1r * rdny; s[1].y += f1i * rdny; }
+				if (MASK & DLPT) { s[2].x += f1r * rdnx; s[2].y += f1i * rdnx; }
+			}
+			if (MASK & HSP) {
+				const float cc = rdnx * rdny;
+				const float ar = __builtin_fmaf(3.0f - th * th, cc, p.nxny);
+				const float ai = th * __builtin_fmaf(3.0f, cc, p.nxny);
+				const float ri2 = rinv * rinv;
+				s[3].x += (ec * ar - es * ai) * ri2;
+				s[3].y += (ec * ai + es * ar) * ri2;
+			}
+		}
+	}
+}
+
+// Evaluate the kernels of MASK for the active lanes, each lane at its own
+// quadrature degree; lanes are served one distinct degree at a time so that
+// the table pointer stays wave-uniform.
+template <int FAM, int MASK, int ORDER>
+__device__ __forceinline__ void eval_group(const RegArgs &a, int j, double d2, bool act, const PairCtx &p,
+	float2 (&s)[4])
+{
+	int deg = a.fixed_degree >= 0 ? a.fixed_degree : a.field_complexity + table_degree<ORDER>(d2, a.thr, a.sn, j);
+	deg = act ? deg : -1;
+	for (;;) {
+		const unsigned long long m = __ballot(deg >= 0);
+		if (m == 0) break;
+		const int lane = __ffsll((long long)m) - 1;
+		const int dsel = __builtin_amdgcn_readlane(deg, lane);
+		if (deg == dsel) {
+			accumulate_points<FAM, MASK>(a.qt[dsel] + (size_t)j * a.qn[dsel], a.qn[dsel], p, a.k_revf, a.k, s);
+			deg = -1;
+		}
+	}
+}
+
+template <int FAM, int KM, int MODE>
+__global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
+{
+	const int tile_r = blockIdx.x % a.row_tiles;
+	const int tile_c = blockIdx.x / a.row_tiles;
+	const int i = tile_r * 256 + threadIdx.x;
+	const bool row_ok = i < a.n_rows;
+	const int ii = row_ok ? i : a.n_rows - 1;
+	const double cix = a.tc[ii], ciy = a.tc[a.tn + ii], ciz = a.tc[2 * a.tn + ii];
+	PairCtx p;
+	p.nxx = p.nxy = p.nxz = 0.0f;
+	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
+	p.nyx = p.nyy = p.nyz = p.nxny = 0.0f;
+	const int j0 = tile_c * a.tile_cols;
+	const int j1 = min(j0 + a.tile_cols, a.n_cols);
+	for (int j = j0; j < j1; ++j) {
+		const double cjx = a.sc[j], cjy = a.sc[a.sn + j], cjz = a.sc[2 * a.sn + j];
+		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
+		const bool act = row_ok && !(d2 < a.thr[THR_NEAR * a.sn + j]) && !(a.same_mesh && i == j);
+		if (__ballot(act) == 0) continue;
+		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
+		p.dcx2 = 2.0f * p.dcx; p.dcy2 = 2.0f * p.dcy; p.dcz2 = 2.0f * p.dcz;
+		if (FAM == HELM) {
+			const double d = sqrt(d2);
+			double t = d * a.k_rev;
+			t -= rint(t);
+			p.df = (float)d;
+			p.t0 = (float)t;
+		} else {
+			p.df = 0.0f; p.t0 = 0.0f;
+		}
+		if (KM & (DLP | HSP)) { p.nyx = a.snu[j]; p.nyy = a.snu[a.sn + j]; p.nyz = a.snu[2 * a.sn + j]; }
+		if (KM & HSP) p.nxny = p.nxx * p.nyx + p.nxy * p.nyy + p.nxz * p.nyz;
+		float2 s[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+		if (a.fixed_degree >= 0) {
+			eval_group<FAM, KM, 1>(a, j, d2, act, p, s);
+		} else {
+			if (KM & SLP) eval_group<FAM, KM & SLP, 1>(a, j, d2, act, p, s);
+			if (KM & (DLP | DLPT)) eval_group<FAM, KM & (DLP | DLPT), 2>(a, j, d2, act, p, s);
+			if (KM & HSP) eval_group<FAM, KM & HSP, 3>(a, j, d2, act, p, s);
+		}
+		if (act) {
+			const long long idx = (long long)j * a.ld + i;
+			if (MODE == OUT_SINGLE) {
+				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+				const float re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
+				const float im = a.alpha.x * s[kk].y + a.alpha.y * s[kk].x;
+				store_entry(a.out0, idx, a.dtype, a.accumulate, re, im);
+			} else {
+				// A = alpha D + H, B = G + alpha Ht
+				const float ar = a.alpha.x * s[3].x - a.alpha.y * s[3].y + s[1].x;
+				const float ai = a.alpha.x * s[3].y + a.alpha.y * s[3].x + s[1].y;
+				const float br = s[0].x + a.alpha.x * s[2].x - a.alpha.y * s[2].y;
+				const float bi = s[0].y + a.alpha.x * s[2].y + a.alpha.y * s[2].x;
+				store_entry(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+				store_entry(a.out1, idx, a.dtype, a.accumulate, br, bi);
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------- FP64 pass
+
+// line and triangle rules of the near-singular and singular formulas
+struct Rules {
+	double gl6x[6], gl6w[6];          // Gauss-Legendre order 10 (6 points)
+	double gl21x[21], gl21w[21];      // Gauss-Legendre order 40 (21 points)
+	double d7xi[13], d7eta[13], d7w[13];      // Dunavant degree 7
+	double d14xi[42], d14eta[42], d14w[42];   // Dunavant degree 14
+};
+
+struct NearArgs {
+	int n_pairs;
+	const int *pi, *pj;          // pair list (near pass); diagonal pass uses (p, p)
+	int tn, sn;
+	const double *tc;            // test centres [3][tn]
+	const double *tnu;           // test unit normals FP64 [3][tn]
+	const double *X;             // trial vertices [9][sn]
+	const double *sJ;            // trial Jacobians
+	const double *snu;           // trial unit normals FP64 [3][sn]
+	double k;
+	double2 alpha;
+	void *out0, *out1;
+	long long ld;
+	int dtype, accumulate;
+	const Rules *rules;
+};
+
+struct D3 { double x, y, z; };
+__device__ __forceinline__ D3 d3sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double d3dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 d3cross(D3 a, D3 b)
+{
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double d3norm(D3 a) { return sqrt(d3dot(a, a)); }
+__device__ __forceinline__ D3 d3scale(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+
+__device__ __forceinline__ D3 tri_x(const D3 V[3], double xi, double eta)
+{
+	const double N0 = 1.0 - xi - eta;
+	return {V[0].x * N0 + V[1].x * xi + V[2].x * eta, V[0].y * N0 + V[1].y * xi + V[2].y * eta,
+		V[0].z * N0 + V[1].z * xi + V[2].z * eta};
+}
+
+constexpr double PI_D = 3.141592653589793238462643383279502884;
+
+// K_helm - K_lap for the kernels of MASK at one point (FP64), r = y - x
+template <int MASK>
+__device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4])
+{
+	const D3 rv = d3sub(y, x);
+	const double r = d3norm(rv);
+	const double kr = k * r;
+	double sn, cs;
+	sincos(kr, &sn, &cs);
+	// e^{-ikr} = cs - i sn
+	const double inv4pir = 1.0 / (4.0 * PI_D * r);
+	if (MASK & SLP) {
+		s[0].x += (cs - 1.0) * inv4pir * wj;
+		s[0].y += -sn * inv4pir * wj;
+	}
+	if (MASK & (DLP | DLPT | HSP)) {
+		const double rdny = d3dot(rv, ny) / r;
+		const double rdnx = -d3dot(rv, nx) / r;
+		const double g1 = inv4pir / r;
+		// e(-1 - ikr) - (-1)
+		const double er = -cs - kr * sn + 1.0;
+		const double ei = -kr * cs + sn;
+		if (MASK & DLP) { s[1].x += er * g1 * rdny * wj; s[1].y += ei * g1 * rdny * wj; }
+		if (MASK & DLPT) { s[2].x += er * g1 * rdnx * wj; s[2].y += ei * g1 * rdnx * wj; }
+		if (MASK & HSP) {
+			const double g3 = g1 / r;
+			const double cc = rdnx * rdny;
+			const double nn = d3dot(nx, ny);
+			// e [(3 - kr^2 + 3ikr) cc + (1 + ikr) nn] - [3 cc + nn]
+			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
+			s[3].x += ((cs * ar + sn * ai) - (3.0 * cc + nn)) * g3 * wj;
+			s[3].y += (cs * ai - sn * ar) * g3 * wj;
+		}
+	}
+}
+
+// Laplace static part of the near-singular integral over a flat triangle in
+// its own plane (nihu/library/laplace_nearly_singular_integrals.hpp:62-389,
+// nihu/library/plane_element_helper.hpp:76-128), all kernels of MASK.
+template <int MASK>
+__device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, double (&st)[4])
+{
+	const D3 v1 = d3sub(V[1], V[0]), v2 = d3sub(V[2], V[0]);
+	const D3 T0 = d3scale(v1, 1.0 / d3norm(v1));
+	D3 T2 = d3cross(v1, v2);
+	T2 = d3scale(T2, 1.0 / d3norm(T2));
+	D3 T1 = d3cross(T2, T0);
+	T1 = d3scale(T1, 1.0 / d3norm(T1));
+	D3 C[3];
+	for (int a = 0; a < 3; ++a) C[a] = {d3dot(T0, V[a]), d3dot(T1, V[a]), d3dot(T2, V[a])};
+	const D3 x0 = {d3dot(T0, x0g), d3dot(T1, x0g), d3dot(T2, x0g)};
+	const D3 nx = {d3dot(T0, nxg), d3dot(T1, nxg), d3dot(T2, nxg)};
+	double th_lim[3], th0[3], dist[3];
+	for (int n = 0; n < 3; ++n) {
+		const int m = n == 2 ? 0 : n + 1;
+		double lx = C[m].x - C[n].x, ly = C[m].y - C[n].y;
+		const double ln = sqrt(lx * lx + ly * ly);
+		lx /= ln; ly /= ln;
+		const double d1x = C[n].x - x0.x, d1y = C[n].y - x0.y;
+		const double dl = d1x * lx + d1y * ly;
+		const double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		th_lim[n] = atan2(d1y, d1x);
+		th0[n] = atan2(d0y, d0x);
+		dist[n] = sqrt(d0x * d0x + d0y * d0y);
+	}
+	const double z = x0.z - C[0].z;
+	const double az = fabs(z);
+	const double sgz = (0.0 < z) - (z < 0.0);
+	double rs = 0.0, rd = 0.0, rt = 0.0, rh = 0.0;
+	for (int n = 0; n < 3; ++n) {
+		double t1 = th_lim[n], t2 = th_lim[n == 2 ? 0 : n + 1];
+		if (fabs(t2 - t1) > PI_D) {
+			if (t2 > t1) t1 += 2.0 * PI_D;
+			else t2 += 2.0 * PI_D;
+		}
+		const double jac = (t2 - t1) / 2.0;
+		if (MASK & (SLP | DLP | DLPT)) {
+			for (int g = 0; g < 6; ++g) {
+				const double xi = R.gl6x[g], w = R.gl6w[g];
+				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
+				const double P = dist[n] / cos(th - th0[n]);
+				const double r = sqrt(P * P + z * z);
+				if (MASK & SLP) rs += (r - az) * jac * w;
+				if (MASK & DLP) rd += (1.0 - az / r) * jac * w;
+				if (MASK & DLPT) {
+					double sth, cth;
+					sincos(th, &sth, &cth);
+					const double rdot = nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z);
+					rt += (-rdot / r + (nx.x * cth + nx.y * sth) * log(P + r) - nx.z * sgz) * jac * w;
+				}
+			}
+		}
+		if (MASK & HSP) {
+			for (int g = 0; g < 21; ++g) {
+				const double xi = R.gl21x[g], w = R.gl21w[g];
+				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
+				const double P = dist[n] / cos(th - th0[n]);
+				const double r = sqrt(P * P + z * z);
+				double sth, cth;
+				sincos(th, &sth, &cth);
+				const double rdnx = -(nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z)) / r;
+				const double integrand = az > 1e-12 ? -P * P / (r * r) / z * rdnx : -nx.z / P;
+				rh += integrand * jac * w;
+			}
+		}
+	}
+	const double f = 1.0 / (4.0 * PI_D);
+	st[0] = rs * f;
+	st[1] = sgz * rd * f;
+	st[2] = rt * f;
+	st[3] = rh * f;
+}
+
+// Diagonal integrals at the centre of a flat triangle
+// (nihu/library/helmholtz_singular_integrals.hpp:703-839,
+//  nihu/library/laplace_singular_integrals.hpp:456-505, plane_element_helper.hpp:42-74);
+// DLP and DLPt vanish (normal_derivative_singular_integrals.hpp:13-58).
+template <int FAM, int MASK>
+__device__ void diag_values(const D3 V[3], D3 x0, double J, double k, const Rules &Rr, double2 (&s)[4])
+{
+	double r[3], theta[3], alpha[3];
+	D3 Rv[3], Cv[3];
+	for (int a = 0; a < 3; ++a) {
+		Rv[a] = d3sub(V[a], x0);
+		r[a] = d3norm(Rv[a]);
+		Rv[a] = d3scale(Rv[a], 1.0 / r[a]);
+		Cv[a] = d3sub(V[a], V[a == 2 ? 0 : a + 1]);
+		Cv[a] = d3scale(Cv[a], 1.0 / d3norm(Cv[a]));
+	}
+	for (int a = 0; a < 3; ++a) {
+		const double cs = d3dot(Rv[a], Rv[a == 2 ? 0 : a + 1]);
+		theta[a] = cs >= 1.0 ? 0.0 : acos(cs);
+		alpha[a] = acos(d3dot(Rv[a], Cv[a]));
+	}
+	const double f = 1.0 / (4.0 * PI_D);
+	if (MASK & SLP) {
+		double st = 0.0;
+		for (int a = 0; a < 3; ++a)
+			st += r[a] * sin(alpha[a]) * log(tan((alpha[a] + theta[a]) / 2.0) / tan(alpha[a] / 2.0));
+		double dr = 0.0, di = 0.0;
+		if (FAM == HELM) {
+			for (int q = 0; q < 13; ++q) {
+				const double rq = d3norm(d3sub(tri_x(V, Rr.d7xi[q], Rr.d7eta[q]), x0));
+				// -ik e^{-ik rq/2} sinc(k rq/2)
+				const double h = k * rq / 2.0;
+				const double sc = fabs(h) > 1e-3 ? sin(h) / h : 1.0 - h * h / 6.0 * (1.0 - h * h / 20.0);
+				double sh, ch;
+				sincos(h, &sh, &ch);
+				dr += (-k * sh * sc) * Rr.d7w[q] * J;
+				di += (-k * ch * sc) * Rr.d7w[q] * J;
+			}
+		}
+		s[0] = make_double2((st + dr) * f, di * f);
+	}
+	if (MASK & HSP) {
+		double IG0 = 0.0, IddG0 = 0.0;
+		for (int a = 0; a < 3; ++a) {
+			if (FAM == HELM && theta[a] < 1e-3) continue;
+			if (FAM == HELM)
+				IG0 += r[a] * sin(alpha[a]) * log(tan((alpha[a] + theta[a]) / 2.0) / tan(alpha[a] / 2.0));
+			IddG0 += (cos(alpha[a] + theta[a]) - cos(alpha[a])) / (r[a] * sin(alpha[a]));
+		}
+		double ar = 0.0, ai = 0.0;
+		if (FAM == HELM) {
+			for (int q = 0; q < 13; ++q) {
+				const double rq = d3norm(d3sub(tri_x(V, Rr.d7xi[q], Rr.d7eta[q]), x0));
+				double vr, vi;
+				if (fabs(rq) > 1e-3) {
+					// (e^{-ikr}(1 + ikr) - 1 + (ikr)^2/2) / r^3
+					const double kr = k * rq;
+					double sn, cs;
+					sincos(kr, &sn, &cs);
+					const double er = cs + kr * sn, ei = kr * cs - sn;
+					vr = (er - 1.0 - kr * kr / 2.0) / (rq * rq * rq);
+					vi = ei / (rq * rq * rq);
+				} else {
+					// -i k^3 (1/3 - ikr(1/8 - ikr(1/30 - ikr(1/144 - ikr(1/840 - ikr/5760)))))
+					const double kr = k * rq;
+					double pr = 1.0 / 840.0, pi = -kr / 5760.0;   // 1/840 - ikr/5760
+					double c2[4] = {1.0 / 144.0, 1.0 / 30.0, 1.0 / 8.0, 1.0 / 3.0};
+					for (int t = 0; t < 4; ++t) {
+						// c - ikr * p
+						const double nr = c2[t] + kr * pi;
+						const double ni = -kr * pr;
+						pr = nr; pi = ni;
+					}
+					const double k3 = k * k * k;
+					vr = k3 * pi;        // -i k^3 (pr + i pi) = k^3 pi - i k^3 pr
+					vi = -k3 * pr;
+				}
+				ar += vr * Rr.d7w[q] * J;
+				ai += vi * Rr.d7w[q] * J;
+			}
+		}
+		s[3] = make_double2((IddG0 + k * k / 2.0 * IG0 + ar) * f, ai * f);
+	}
+}
+
+template <int FAM, int KM, int MODE, bool DIAG>
+__global__ void __launch_bounds__(64) fp64_pass_kernel(NearArgs a)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= a.n_pairs) return;
+	const int i = DIAG ? p : a.pi[p];
+	const int j = DIAG ? p : a.pj[p];
+	const Rules &R = *a.rules;
+	D3 V[3];
+	for (int v = 0; v < 3; ++v)
+		V[v] = {a.X[(3 * v) * a.sn + j], a.X[(3 * v + 1) * a.sn + j], a.X[(3 * v + 2) * a.sn + j]};
+	const D3 x0 = {a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i]};
+	const D3 nx = {a.tnu[i], a.tnu[a.tn + i], a.tnu[2 * a.tn + i]};
+	const D3 ny = {a.snu[j], a.snu[a.sn + j], a.snu[2 * a.sn + j]};
+	const double J = a.sJ[j];
+	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+	if (DIAG) {
+		diag_values<FAM, KM & (SLP | HSP)>(V, x0, J, a.k, R, s);
+	} else {
+		double st[4] = {0, 0, 0, 0};
+		near_static<KM>(V, x0, nx, R, st);
+		for (int t = 0; t < 4; ++t) s[t].x = st[t];
+		if (FAM == HELM) {
+			for (int q = 0; q < 42; ++q) {
+				const D3 y = tri_x(V, R.d14xi[q], R.d14eta[q]);
+				dynamic_point<KM>(x0, y, nx, ny, a.k, R.d14w[q] * J, s);
+			}
+		}
+	}
+	const long long idx = (long long)j * a.ld + i;
+	if (MODE == OUT_SINGLE) {
+		const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+		const double re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
+		const double im = a.alpha.x * s[kk].y + a.alpha.y * s[kk].x;
+		store_entry64(a.out0, idx, a.dtype, a.accumulate, re, im);
+	} else {
+		double ar = a.alpha.x * s[3].x - a.alpha.y * s[3].y + s[1].x;
+		double ai = a.alpha.x * s[3].y + a.alpha.y * s[3].x + s[1].y;
+		double br = s[0].x + a.alpha.x * s[2].x - a.alpha.y * s[2].y;
+		double bi = s[0].y + a.alpha.x * s[2].y + a.alpha.y * s[2].x;
+		if (DIAG) {
+			ar += -0.5;                      // - I/2
+			br += a.alpha.x / 2.0;           // + alpha/2 I
+			bi += a.alpha.y / 2.0;
+		}
+		store_entry64(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+		store_entry64(a.out1, idx, a.dtype, a.accumulate, br, bi);
+	}
+}
+
+// ---------------------------------------------------------------- near list, classes, identity
+
+struct ListArgs {
+	int n_rows, n_cols, same_mesh, row_tiles, chunk, n_chunks, tn, sn;
+	const double *tc, *sc, *thr_near;
+	int *counts;            // [n_rows][n_chunks]
+	const int *offsets;     // exclusive scan of counts
+	int *pi, *pj;
+};
+
+template <bool FILL>
+__global__ void __launch_bounds__(256) near_list_kernel(ListArgs a)
+{
+	const int tile_r = blockIdx.x % a.row_tiles;
+	const int ch = blockIdx.x / a.row_tiles;
+	const int i = tile_r * 256 + threadIdx.x;
+	if (i >= a.n_rows) return;
+	const double cix = a.tc[i], ciy = a.tc[a.tn + i], ciz = a.tc[2 * a.tn + i];
+	const int j0 = ch * a.chunk, j1 = min(j0 + a.chunk, a.n_cols);
+	int cnt = 0;
+	int base = FILL ? a.offsets[(long long)i * a.n_chunks + ch] : 0;
+	for (int j = j0; j < j1; ++j) {
+		const double d2 = d2_pair(cix, ciy, ciz, a.sc[j], a.sc[a.sn + j], a.sc[2 * a.sn + j]);
+		if (d2 < a.thr_near[j] && !(a.same_mesh && i == j)) {
+			if (FILL) { a.pi[base + cnt] = i; a.pj[base + cnt] = j; }
+			++cnt;
+		}
+	}
+	if (!FILL) a.counts[(long long)i * a.n_chunks + ch] = cnt;
+}
+
+struct ClassArgs {
+	int n_rows, n_cols, same_mesh, tn, sn, order, fixed_degree, field_complexity;
+	const double *tc, *sc, *thr;
+	int8_t *cls;
+};
+
+__global__ void __launch_bounds__(256) classify_kernel(ClassArgs a)
+{
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long long)a.n_rows * a.n_cols) return;
+	const int i = (int)(t / a.n_cols), j = (int)(t % a.n_cols);
+	const double d2 = d2_pair(a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i], a.sc[j], a.sc[a.sn + j], a.sc[2 * a.sn + j]);
+	int c;
+	if (a.same_mesh && i == j) c = -1;
+	else if (d2 < a.thr[THR_NEAR * a.sn + j]) c = -2;
+	else if (a.fixed_degree >= 0) c = a.fixed_degree;
+	else if (a.order == 1) c = a.field_complexity + table_degree<1>(d2, a.thr, a.sn, j);
+	else if (a.order == 2) c = a.field_complexity + table_degree<2>(d2, a.thr, a.sn, j);
+	else c = a.field_complexity + table_degree<3>(d2, a.thr, a.sn, j);
+	a.cls[t] = (int8_t)c;
+}
+
+__global__ void identity_kernel(void *out, long long ld, int n, int dtype, double are, double aim)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) store_entry64(out, (long long)i * ld + i, dtype, 1, are, aim);
+}
+
+// zero a column-major rows x cols block with leading dimension ld (a fill
+// kernel rather than a memset so that callers may capture it in a graph)
+__global__ void zero_kernel(void *out, long long ld, long long rows, long long cols, int elem_bytes)
+{
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long words_per_col = rows * elem_bytes / 4;
+	if (t >= words_per_col * cols) return;
+	const long long c = t / words_per_col, w = t % words_per_col;
+	((uint32_t *)out)[c * (ld * elem_bytes / 4) + w] = 0u;
+}
+
+} // namespace nbk

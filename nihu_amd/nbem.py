@@ -1,0 +1,272 @@
+"""ctypes binding of libnbem.so (include/nbem.h).
+
+Device buffers are torch tensors on the GPU (torch supplies memory and
+streams; all arithmetic happens in the library's HIP kernels).  Host buffers
+are numpy arrays.  There is no fallback: if the library is missing, loading
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnbem.so")
+
+NB_OK, NB_ERR_ARG, NB_ERR_UNSUPPORTED, NB_ERR_RANGE, NB_ERR_OOM, NB_ERR_DEVICE, NB_ERR_RUNTIME = range(7)
+NB_LAPLACE, NB_HELMHOLTZ, NB_IDENTITY = 0, 1, 2
+NB_RULE_NIHU, NB_RULE_FIXED = 0, 1
+NB_NEAR_MEAN_EDGE, NB_NEAR_DIAMETER, NB_NEAR_NONE = 0, 1, 2
+NB_C64, NB_C128, NB_F32, NB_F64 = 0, 1, 2, 3
+NB_HOST, NB_DEVICE = 0, 1
+NB_COLLOCATION, NB_GALERKIN = 0, 1
+
+# symbols declared in include/nbem.h
+EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_destroy", "nb_mesh_info",
+           "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
+           "nb_synchronize"]
+
+
+class NbKernel(C.Structure):
+    _fields_ = [("family", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("k", C.c_double)]
+
+
+class NbScalar(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class NbPolicy(C.Structure):
+    _fields_ = [("rule", C.c_int32), ("fixed_degree", C.c_int32), ("near_mode", C.c_int32), ("near_limit", C.c_double)]
+
+
+class NbOut(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("rows", C.c_int64), ("cols", C.c_int64), ("ld", C.c_int64),
+                ("dtype", C.c_int32), ("location", C.c_int32), ("accumulate", C.c_int32)]
+
+
+class NbLaunch(C.Structure):
+    _fields_ = [("tile_cols", C.c_int32), ("k_batch", C.c_int32), ("persistent", C.c_int32)]
+
+
+class NbemError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libnbem.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libnbem.so not built: run `python -m nihu_amd.build` (HIP engine missing)")
+        l = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        l.nb_last_error.restype = C.c_char_p
+        l.nb_device_info.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        l.nb_mesh_create.argtypes = [P, C.c_int32, P, C.c_int32, C.POINTER(P)]
+        l.nb_mesh_destroy.argtypes = [P]
+        l.nb_mesh_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        l.nb_assemble.argtypes = [P, P, C.c_int32, C.c_int32, NbKernel, NbScalar, C.POINTER(NbPolicy),
+                                  C.POINTER(NbOut), P]
+        l.nb_assemble_bm.argtypes = [P, C.c_double, NbScalar, C.POINTER(NbPolicy), C.POINTER(NbOut),
+                                     C.POINTER(NbOut), P]
+        l.nb_assemble_sweep.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
+                                        C.POINTER(NbLaunch), P]
+        l.nb_classify.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(NbPolicy), P, P]
+        l.nb_autotune.argtypes = [P, C.c_int32, C.POINTER(NbPolicy), C.POINTER(NbLaunch)]
+        l.nb_set_launch.argtypes = [C.POINTER(NbLaunch)]
+        l.nb_synchronize.argtypes = [P]
+        _lib = l
+    return _lib
+
+
+def check(status: int):
+    if status == NB_OK:
+        return
+    msg = lib().nb_last_error().decode()
+    if status == NB_ERR_ARG:
+        raise ValueError(msg)
+    if status == NB_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if status == NB_ERR_RANGE:
+        raise IndexError(msg)
+    if status == NB_ERR_OOM:
+        raise MemoryError(msg)
+    raise NbemError(msg)
+
+
+def policy(rule: str | int = "nihu", degree: int = 5, near: str | int = "mean_edge", limit: float = 1.5) -> NbPolicy:
+    """'nihu' = NiHu's per-pair degree tables; 'fixed' = one Dunavant degree.
+    near: 'mean_edge' (NiHu, limit 1.5), 'diameter' (limit in element diameters) or 'none'."""
+    r = {"nihu": NB_RULE_NIHU, "fixed": NB_RULE_FIXED}.get(rule, rule)
+    n = {"mean_edge": NB_NEAR_MEAN_EDGE, "diameter": NB_NEAR_DIAMETER, "none": NB_NEAR_NONE}.get(near, near)
+    return NbPolicy(int(r), int(degree), int(n), float(limit))
+
+
+NIHU_POLICY = dict(rule="nihu", near="mean_edge", limit=1.5)
+# the headline workload's policy: 7-point rule far, FP64 within 2 element diameters
+CONFIG_POLICY = dict(rule="fixed", degree=5, near="diameter", limit=2.0)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available() and torch.cuda.is_initialized():
+                return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except Exception:
+            pass
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+class Mesh:
+    """A triangle mesh resident on the current GPU (nb_mesh)."""
+
+    def __init__(self, nodes, tris):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self.tris = np.ascontiguousarray(tris, dtype=np.int32)
+        if self.nodes.ndim != 2 or self.nodes.shape[1] != 3 or self.tris.ndim != 2 or self.tris.shape[1] != 3:
+            raise ValueError("nodes must be n x 3 and tris m x 3")
+        h = C.c_void_p()
+        check(lib().nb_mesh_create(self.nodes.ctypes.data, len(self.nodes), self.tris.ctypes.data, len(self.tris),
+                                   C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n(self):
+        return len(self.tris)
+
+    def device_bytes(self):
+        b = C.c_int64()
+        check(lib().nb_mesh_info(self._h, None, None, C.byref(b)))
+        return b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nb_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_TORCH_DT = {"complex64": NB_C64, "complex128": NB_C128, "float32": NB_F32, "float64": NB_F64}
+_NP_DT = {np.dtype(np.complex64): NB_C64, np.dtype(np.complex128): NB_C128, np.dtype(np.float32): NB_F32,
+          np.dtype(np.float64): NB_F64}
+
+
+def out_desc(arr, accumulate=True) -> NbOut:
+    """Describe a column-major result: a torch GPU tensor (2-D, column-major,
+    i.e. arr.t() contiguous or a transposed view) or a Fortran-ordered numpy array."""
+    if isinstance(arr, np.ndarray):
+        if arr.ndim != 2:
+            raise ValueError("result must be 2-D")
+        if arr.dtype not in _NP_DT:
+            raise ValueError("unsupported result dtype %s" % arr.dtype)
+        isz = arr.itemsize
+        if arr.strides[0] != isz or arr.strides[1] % isz:
+            raise ValueError("host result must be column-major (Fortran order)")
+        ld = arr.strides[1] // isz if arr.shape[1] > 1 else arr.shape[0]
+        return NbOut(arr.ctypes.data, arr.shape[0], arr.shape[1], ld, _NP_DT[arr.dtype], NB_HOST, int(accumulate))
+    import torch
+    if not isinstance(arr, torch.Tensor) or not arr.is_cuda:
+        raise ValueError("device result must be a CUDA/HIP torch tensor")
+    if arr.dim() != 2:
+        raise ValueError("result must be 2-D")
+    dt = _TORCH_DT.get(str(arr.dtype).replace("torch.", ""))
+    if dt is None:
+        raise ValueError("unsupported result dtype %s" % arr.dtype)
+    if arr.stride(0) != 1:
+        raise ValueError("device result must be column-major (use a transposed view of a row-major tensor)")
+    ld = arr.stride(1) if arr.shape[1] > 1 else arr.shape[0]
+    return NbOut(arr.data_ptr(), arr.shape[0], arr.shape[1], ld, dt, NB_DEVICE, int(accumulate))
+
+
+def colmajor(rows, cols, dtype="complex64", device="cuda"):
+    """A zeroed column-major torch matrix (rows x cols), as a transposed view."""
+    import torch
+    return torch.zeros((cols, rows), dtype=getattr(torch, dtype), device=device).t()
+
+
+def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.0, alpha=1.0, pol=None,
+             same_mesh=None, accumulate=True, stream=None):
+    pol = pol or policy(**NIHU_POLICY)
+    same = (test is trial) if same_mesh is None else bool(same_mesh)
+    a = complex(alpha)
+    o = out_desc(out, accumulate)
+    check(lib().nb_assemble(test.handle, trial.handle, int(same), NB_COLLOCATION, NbKernel(family, nx, ny, float(k)),
+                            NbScalar(a.real, a.imag), C.byref(pol), C.byref(o), _stream_ptr(stream)))
+    return out
+
+
+def assemble_bm(mesh: Mesh, k, A, B, alpha=None, pol=None, accumulate=False, stream=None):
+    pol = pol or policy(**NIHU_POLICY)
+    a = complex(0.0, -1.0 / k) if alpha is None else complex(alpha)
+    oa, ob = out_desc(A, accumulate), out_desc(B, accumulate)
+    check(lib().nb_assemble_bm(mesh.handle, float(k), NbScalar(a.real, a.imag), C.byref(pol), C.byref(oa),
+                               C.byref(ob), _stream_ptr(stream)))
+    return A, B
+
+
+def assemble_sweep(mesh: Mesh, ks, A, B, pol=None, launch=None, stream=None):
+    """A, B: torch complex64 tensors of shape (n_k, n, n) holding column-major
+    matrices (e.g. a transposed view per frequency); alpha = -i/k each."""
+    pol = pol or policy(**NIHU_POLICY)
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    n = mesh.n
+    if A.shape != (len(ks), n, n) or B.shape != A.shape:
+        raise ValueError("A and B must be (n_k, n, n)")
+    if A.stride(1) != 1 or B.stride(1) != 1 or A.stride() != B.stride():
+        raise ValueError("A and B must hold column-major matrices with equal strides")
+    ld, kst = A.stride(2), A.stride(0)
+    lp = C.byref(launch) if launch is not None else None
+    check(lib().nb_assemble_sweep(mesh.handle, ks.ctypes.data, len(ks), C.byref(pol), A.data_ptr(), B.data_ptr(),
+                                  ld, kst, lp, _stream_ptr(stream)))
+    return A, B
+
+
+def classify(test: Mesh, trial: Mesh, nx, ny, pol=None, same_mesh=None, stream=None):
+    import torch
+    pol = pol or policy(**NIHU_POLICY)
+    same = (test is trial) if same_mesh is None else bool(same_mesh)
+    cls = torch.empty((test.n, trial.n), dtype=torch.int8, device="cuda")
+    check(lib().nb_classify(test.handle, trial.handle, int(same), nx, ny, C.byref(pol), C.c_void_p(cls.data_ptr()),
+                            _stream_ptr(stream)))
+    return cls
+
+
+def autotune(mesh: Mesh, n_k=1, pol=None):
+    pol = pol or policy(**NIHU_POLICY)
+    best = NbLaunch()
+    check(lib().nb_autotune(mesh.handle, int(n_k), C.byref(pol), C.byref(best)))
+    return best
+
+
+def set_launch(tile_cols=0, k_batch=0, persistent=0):
+    check(lib().nb_set_launch(C.byref(NbLaunch(tile_cols, k_batch, persistent))))
+
+
+def synchronize(stream=None):
+    check(lib().nb_synchronize(_stream_ptr(stream)))
+
+
+def device_info():
+    name = C.create_string_buffer(64)
+    ncu, hbm = C.c_int32(), C.c_int64()
+    check(lib().nb_device_info(name, 64, C.byref(ncu), C.byref(hbm)))
+    return name.value.decode(), ncu.value, hbm.value
