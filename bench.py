@@ -1,0 +1,197 @@
+"""Headline benchmark: dense Burton-Miller collocation assembly on one MI355X.
+
+Workload (one step): the Helmholtz 3D collocation matrices A = alpha*D - I/2 + H
+and B = G + alpha*Ht + alpha/2*I (alpha = -i/k) on the 20480-triangle unit
+icosphere (level 5, +-1% seeded radial jitter, seed 0), ka = 10: far pairs with
+the 7-point Dunavant rule in FP32 (complex64 results), pairs whose centroid
+distance is under 2 element diameters and the diagonal in FP64.  Inputs stay
+resident on the device; a step is one nb_assemble_bm call (regular, near and
+diagonal passes) writing both 20480^2 complex64 matrices.
+
+Metric: matrix entries/s = (number of kernel operators: SLP, DLP, DLPt, HSP =
+4) x 20480^2 per step / wall seconds.
+
+Also reported: the dominant kernel's roofline from its HIP-event time, the
+relative Frobenius error of sampled rows against the FP64 oracle, and the
+oracle (NiHu's algorithm restated, FP64, OpenMP over rows) timed on the host
+on the same sampled rows as the CPU baseline.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_POINT_BM = 65        # SLP+DLP+DLPt+HSP fused, per (collocation point, quadrature point)
+PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 peak (packed); non-packed VALU issue is half
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(path):
+    """HBM bytes per launch of the dominant kernel from a committed rocprofv3
+    --pmc summary (line 'traffic_bytes_per_launch: <float>'), else None."""
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--ka", type=float, default=10.0)
+    ap.add_argument("--policy", choices=["config", "nihu"], default="config")
+    ap.add_argument("--cpu-rows", type=int, default=512, help="rows of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; the multi-GPU path (row blocks per GPU) is design only. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import jittered_icosphere
+
+    torch.cuda.set_device(0)
+    nodes, tris = jittered_icosphere(args.level, seed=0)
+    n = len(tris)
+    k = args.ka  # unit radius: ka = k
+    alpha = complex(0.0, -1.0 / k)
+    pol_kw = nbem.CONFIG_POLICY if args.policy == "config" else nbem.NIHU_POLICY
+    pol = nbem.policy(**pol_kw)
+    name, ncu, hbm = nbem.device_info()
+    log("device %s, %d CUs, %.0f GiB; mesh %d triangles; policy %s" % (name, ncu, hbm / 2 ** 30, n, pol_kw))
+
+    mesh = nbem.Mesh(nodes, tris)
+    A, B = nbem.colmajor(n, n), nbem.colmajor(n, n)
+    stream = torch.cuda.current_stream()
+
+    # pair census (for the algorithmic work of the regular pass)
+    cls = nbem.classify(mesh, mesh, 0, 0, pol)
+    n_near = int((cls == -2).sum().item())
+    n_diag = n
+    n_reg = n * n - n_near - n_diag
+    if args.policy == "config":
+        pts = 7 * n_reg
+    else:
+        pts = None
+    del cls
+
+    for w in range(args.warmup):
+        nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+        torch.cuda.synchronize()
+        log("warmup step %d done" % w)
+
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    wall = t1 - t0
+    ms_per_step = wall / args.steps * 1e3
+    entries = 4 * n * n
+    value = entries * args.steps / wall
+    log("timed %d steps: %.3f ms/step, %.4g entries/s" % (args.steps, ms_per_step, value))
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    log("per-pass device ms: %s" % pass_ms)
+
+    reg_ms = pass_ms["regular"]
+    write_bytes = 2 * 8 * n_reg
+    roofline = None
+    if reg_ms > 0:
+        if pts is not None:
+            flops = pts * FLOPS_PER_POINT_BM
+            ach = flops / (reg_ms * 1e-3) / 1e12
+            roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic(args.traffic_file),
+                        "engine": "VALU fp32 (no MFMA; packed fp32 disabled at build, so half the peak is the "
+                                  "issue limit)",
+                        "kernel": "regular_kernel<HELM,ALL4,OUT_BM>", "kernel_ms": round(reg_ms, 4),
+                        "algorithmic_flops": flops,
+                        "hbm_write_GBs": round(write_bytes / (reg_ms * 1e-3) / 1e9, 1)}
+        else:
+            gbs = write_bytes / (reg_ms * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": read_traffic(args.traffic_file),
+                        "kernel": "regular_kernel<HELM,ALL4,OUT_BM>", "kernel_ms": round(reg_ms, 4)}
+
+    # accuracy on sampled rows and CPU baseline on the same rows
+    cpu = None
+    err = None
+    if args.cpu_rows > 0:
+        from oracle import oracle
+        rows = np.sort(np.random.default_rng(123).choice(n, min(args.cpu_rows, n), replace=False)).astype(np.int32)
+        opol = oracle.policy(rule=pol.rule, fixed_degree=pol.fixed_degree, near_mode=pol.near_mode,
+                             near_limit=pol.near_limit)
+        threads = oracle.num_threads()
+        tc = time.perf_counter()
+        Ar, Br = oracle.assemble_bm(nodes, tris, k, alpha=alpha, pol=opol, rows=rows, threads=threads)
+        dt = time.perf_counter() - tc
+        cpu_value = 4 * len(rows) * n / dt
+        cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+               "sample": "%d of %d rows of A and B (4 operators, all %d columns), FP64 oracle restating NiHu's "
+                         "pair algorithm, OpenMP over rows, %.1f s wall" % (len(rows), n, n, dt)}
+        ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+        Ag, Bg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy()
+        ea = float(np.linalg.norm(Ag - Ar) / np.linalg.norm(Ar))
+        eb = float(np.linalg.norm(Bg - Br) / np.linalg.norm(Br))
+        err = {"rows": len(rows), "A_rel_fro": ea, "B_rel_fro": eb,
+               "A_max_abs": float(np.abs(Ag - Ar).max()), "B_max_abs": float(np.abs(Bg - Br).max())}
+        log("sampled-row error vs oracle: %s" % err)
+        log("cpu baseline: %.4g entries/s on %d threads (%.1fx faster on GPU)" % (cpu_value, threads, value / cpu_value))
+
+    out = {
+        "metric": "matrix entries/s, Helmholtz 3D coll. 20480 tris",
+        "value": value,
+        "unit": "matrix entries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64) far pairs, fp64 near/diagonal",
+        "data": "synthetic: seeded jittered icosphere",
+        "config": {"workload": "Helmholtz 3D collocation, Burton-Miller A and B (SLP, DLP, DLPt, HSP)",
+                   "n_tri": n, "icosphere_level": args.level, "ka": k, "mesh_seed": 0,
+                   "far_rule": "Dunavant degree 5 (7 points)" if args.policy == "config" else "NiHu degree tables",
+                   "near": "centroid distance < 2 element diameters, FP64" if args.policy == "config"
+                   else "d/R < 1.5, FP64",
+                   "pairs": {"regular": n_reg, "near": n_near, "diagonal": n_diag}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "accuracy": err,
+        "pass_ms": pass_ms,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -150,6 +150,13 @@ int nb_set_launch(const nb_launch *launch);
 
 int nb_synchronize(void *stream);
 
+/* Per-pass device timing with HIP events recorded on the launch stream
+ * (pass 0 regular pairs, 1 near-singular pairs, 2 diagonal, 3 sweep).
+ * nb_profile(1) starts recording, nb_profile_read sums the elapsed times
+ * since the last reset (it waits for the recorded events). */
+int nb_profile(int32_t enable);
+int nb_profile_read(double *ms, int64_t *launches, int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,46 @@ struct LaunchCfg {
 std::mutex g_cfg_mu;
 LaunchCfg g_cfg;
 
+// per-pass device timing (nb_profile)
+constexpr int NPASS = 4;
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+struct ProfEv { int pass; hipEvent_t a, b; };
+std::vector<ProfEv> g_prof_pending;
+double g_prof_ms[NPASS] = {0, 0, 0, 0};
+int64_t g_prof_n[NPASS] = {0, 0, 0, 0};
+
+bool prof_enabled()
+{
+	std::lock_guard<std::mutex> lk(g_prof_mu);
+	return g_prof_on;
+}
+
+// record the start of a pass; returns the event pair index or -1
+int prof_begin(int pass, hipStream_t st)
+{
+	if (!prof_enabled()) return -1;
+	ProfEv e;
+	e.pass = pass;
+	if (hipEventCreate(&e.a) != hipSuccess) return -1;
+	if (hipEventCreate(&e.b) != hipSuccess) { (void)hipEventDestroy(e.a); return -1; }
+	(void)hipEventRecord(e.a, st);
+	std::lock_guard<std::mutex> lk(g_prof_mu);
+	g_prof_pending.push_back(e);
+	return (int)g_prof_pending.size() - 1;
+}
+
+void prof_end(int idx, hipStream_t st)
+{
+	if (idx < 0) return;
+	hipEvent_t b;
+	{
+		std::lock_guard<std::mutex> lk(g_prof_mu);
+		b = g_prof_pending[idx].b;
+	}
+	(void)hipEventRecord(b, st);
+}
+
 // per-device rule tables of the FP64 pass
 std::mutex g_rules_mu;
 std::map<int, DevPtr> g_rules;
@@ -320,6 +360,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	const long long grid = (long long)a.row_tiles * col_tiles;
 	if (grid > 0) {
 		using namespace nbk;
+		const int pe = prof_begin(0, st);
 		if (jb.mode == OUT_BM) {
 			hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM>), dim3(grid), dim3(256), 0, st, a);
 		} else if (jb.family == NB_HELMHOLTZ) {
@@ -338,6 +379,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 			}
 		}
 		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
 	}
 
 	// 2./3. near-singular and diagonal pairs in FP64
@@ -363,6 +405,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		na.pj = diag ? nullptr : (const int *)nl->pj.get();
 		if (na.n_pairs == 0) continue;
 		const int g = (na.n_pairs + 63) / 64;
+		const int pe = prof_begin(diag ? 2 : 1, st);
 		using namespace nbk;
 #define NB_FP64(F, K, M) do { \
 		if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true>), dim3(g), dim3(64), 0, st, na); \
@@ -385,6 +428,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		}
 #undef NB_FP64
 		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
 	}
 	return NB_OK;
 }
@@ -664,6 +708,39 @@ int nb_set_launch(const nb_launch *launch)
 	if (launch->tile_cols) g_cfg.tile_cols = launch->tile_cols;
 	if (launch->k_batch) g_cfg.k_batch = launch->k_batch;
 	if (launch->persistent) g_cfg.persistent = launch->persistent;
+	return NB_OK;
+}
+
+int nb_profile(int32_t enable)
+{
+	std::lock_guard<std::mutex> lk(g_prof_mu);
+	g_prof_on = enable != 0;
+	return NB_OK;
+}
+
+int nb_profile_read(double *ms, int64_t *launches, int32_t reset)
+{
+	std::vector<ProfEv> pend;
+	{
+		std::lock_guard<std::mutex> lk(g_prof_mu);
+		pend.swap(g_prof_pending);
+	}
+	for (auto &e : pend) {
+		NB_HIP(hipEventSynchronize(e.b));
+		float t = 0.0f;
+		NB_HIP(hipEventElapsedTime(&t, e.a, e.b));
+		std::lock_guard<std::mutex> lk(g_prof_mu);
+		g_prof_ms[e.pass] += t;
+		g_prof_n[e.pass] += 1;
+		(void)hipEventDestroy(e.a);
+		(void)hipEventDestroy(e.b);
+	}
+	std::lock_guard<std::mutex> lk(g_prof_mu);
+	for (int p = 0; p < NPASS; ++p) {
+		if (ms) ms[p] = g_prof_ms[p];
+		if (launches) launches[p] = g_prof_n[p];
+		if (reset) { g_prof_ms[p] = 0.0; g_prof_n[p] = 0; }
+	}
 	return NB_OK;
 }
 

@@ -27,7 +27,7 @@ NB_COLLOCATION, NB_GALERKIN = 0, 1
 # symbols declared in include/nbem.h
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
-           "nb_synchronize"]
+           "nb_synchronize", "nb_profile", "nb_profile_read"]
 
 
 class NbKernel(C.Structure):
@@ -81,6 +81,8 @@ def lib():
         l.nb_autotune.argtypes = [P, C.c_int32, C.POINTER(NbPolicy), C.POINTER(NbLaunch)]
         l.nb_set_launch.argtypes = [C.POINTER(NbLaunch)]
         l.nb_synchronize.argtypes = [P]
+        l.nb_profile.argtypes = [C.c_int32]
+        l.nb_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib = l
     return _lib
 
@@ -263,6 +265,22 @@ def set_launch(tile_cols=0, k_batch=0, persistent=0):
 
 def synchronize(stream=None):
     check(lib().nb_synchronize(_stream_ptr(stream)))
+
+
+PASS_NAMES = ("regular", "near", "diagonal", "sweep")
+
+
+def profile(enable=True):
+    """Record per-pass device times with HIP events on the launch stream."""
+    check(lib().nb_profile(int(bool(enable))))
+
+
+def profile_read(reset=True):
+    """{pass: (total_ms, launches)} since the last reset (waits for the events)."""
+    ms = (C.c_double * 4)()
+    n = (C.c_int64 * 4)()
+    check(lib().nb_profile_read(ms, n, int(bool(reset))))
+    return {PASS_NAMES[p]: (ms[p], n[p]) for p in range(4)}
 
 
 def device_info():
