@@ -361,21 +361,23 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	if (grid > 0) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
+		const bool fix7 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 7;
 		if (jb.mode == OUT_BM) {
-			hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM>), dim3(grid), dim3(256), 0, st, a);
+			if (fix7) hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM, 7>), dim3(grid), dim3(256), 0, st, a);
+			else hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM, 0>), dim3(grid), dim3(256), 0, st, a);
 		} else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
-			case SLP: hipLaunchKernelGGL((regular_kernel<HELM, SLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLP: hipLaunchKernelGGL((regular_kernel<HELM, DLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLPT: hipLaunchKernelGGL((regular_kernel<HELM, DLPT, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
-			default: hipLaunchKernelGGL((regular_kernel<HELM, HSP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			case SLP: hipLaunchKernelGGL((regular_kernel<HELM, SLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLP: hipLaunchKernelGGL((regular_kernel<HELM, DLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLPT: hipLaunchKernelGGL((regular_kernel<HELM, DLPT, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			default: hipLaunchKernelGGL((regular_kernel<HELM, HSP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
 			}
 		} else {
 			switch (jb.km) {
-			case SLP: hipLaunchKernelGGL((regular_kernel<LAP, SLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLP: hipLaunchKernelGGL((regular_kernel<LAP, DLP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLPT: hipLaunchKernelGGL((regular_kernel<LAP, DLPT, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
-			default: hipLaunchKernelGGL((regular_kernel<LAP, HSP, OUT_SINGLE>), dim3(grid), dim3(256), 0, st, a); break;
+			case SLP: hipLaunchKernelGGL((regular_kernel<LAP, SLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLP: hipLaunchKernelGGL((regular_kernel<LAP, DLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			case DLPT: hipLaunchKernelGGL((regular_kernel<LAP, DLPT, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			default: hipLaunchKernelGGL((regular_kernel<LAP, HSP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
 			}
 		}
 		NB_HIP(hipGetLastError());

@@ -57,6 +57,14 @@ struct RegArgs {
 
 // ---------------------------------------------------------------- helpers
 
+// View of read-only global memory in the constant address space: loads from a
+// wave-uniform address through it become scalar loads (SGPRs, scalar cache).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *uni(const T *p)
+{
+	return (const __attribute__((address_space(4))) T *)(size_t)p;
+}
+
 __device__ __forceinline__ double d2_pair(double ax, double ay, double az, double bx, double by, double bz)
 {
 	double dx = ax - bx, dy = ay - by, dz = az - bz;
@@ -146,40 +154,50 @@ __device__ __forceinline__ int table_degree(double d2, const double *thr, int sn
 }
 
 struct PairCtx {
-	float dcx, dcy, dcz;       // c_j - c_i (FP32)
-	float dcx2, dcy2, dcz2;    // 2 (c_j - c_i)
-	float df, t0;              // |c_j - c_i|, fract(k |c_j - c_i| / 2pi) in [-1/2, 1/2]
+	float dcx2, dcy2, dcz2;    // 2 (c_j - c_i) (FP32)
+	float d2f, df, t0;         // |c_j - c_i|^2, |c_j - c_i|, fract(k |c_j - c_i| / 2pi) in [-1/2, 1/2]
 	float nxx, nxy, nxz;       // test unit normal
-	float nyx, nyy, nyz;       // trial unit normal
-	float nxny;
+	float b0;                  // (c_j - c_i) . n_x
 };
 
-// sum over the points of one quadrature table of the kernels in MASK:
-//   SLP  e/(4 pi r)                         DLP  e(-1-ikr)/(4 pi r^2) (r.ny)/r
-//   DLPt e(-1-ikr)/(4 pi r^2) (-r.nx)/r     HSP  e/(4 pi r^3)[(3+3ikr-(kr)^2) rdnx rdny + (1+ikr) nx.ny]
-// with e = exp(-ikr), r = y - x (nihu/library/helmholtz_kernel.hpp:169-228,
-// normal_derivative_kernel.hpp:114-257); weights carry J w / (4 pi).
-template <int FAM, int MASK>
-__device__ __forceinline__ void accumulate_points(const QPoint *__restrict__ qp, int nq, const PairCtx &p,
-	float k_revf, float k, float2 (&s)[4])
+// Per-pair accumulators of the regular pass (complex):
+//   s[0] = sum E                    SLP   = s[0]
+//   s[1] = sum Q   (DLP group)      DLP   = -a s[1]
+//   s[2] = sum Q b                  DLPt  = s[2]
+//   s[3] = sum (3Q - th^2 G) b/r^2  HSP   = nx.ny sum Q - a s[3]
+//   s[4] = sum Q   (HSP group, when it is evaluated apart from DLP)
+// with, per quadrature point y = c_j + off and r = y - x (|r| = r):
+//   E = w e^{-i th} / r,  G = E / r^2,  Q = G (1 + i th),  th = k r,
+//   a = r.n_y = (c_j - x).n_y (off lies in the trial plane), b = r.n_x,
+// which restates SLP e/(4 pi r), DLP e(-1-ikr)/(4 pi r^2) (r.ny)/r,
+// DLPt e(-1-ikr)/(4 pi r^2) (-r.nx)/r and
+// HSP e/(4 pi r^3)[(3+3ikr-(kr)^2) rdnx rdny + (1+ikr) nx.ny]
+// (nihu/library/helmholtz_kernel.hpp:169-228, normal_derivative_kernel.hpp:114-257);
+// the weights w carry J w_q / (4 pi).
+// r^2 = d^2 + u with u = 2 (c_j - x).off + |off|^2, and the phase is
+// t0 + (k/2pi) u / (r + d) revolutions: t0 is the pair's FP64-reduced phase of
+// k d and u / (r + d) = r - d keeps full FP32 relative accuracy; sin and cos
+// are the hardware instructions, which take revolutions.
+// NQ > 0: compile-time point count (fully unrolled).
+template <int FAM, int MASK, int NQ>
+__device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt, const PairCtx &p,
+	float k_revf, float k, float2 (&s)[5])
 {
+	const int nq = NQ > 0 ? NQ : nq_rt;
+	const auto qf = uni((const float *)qp_in);
+#pragma unroll (NQ > 0 ? NQ : 1)
 	for (int q = 0; q < nq; ++q) {
-		const QPoint P = qp[q];
-		const float rx = p.dcx + P.ox, ry = p.dcy + P.oy, rz = p.dcz + P.oz;
-		const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3], oo = qf[8 * q + 4];
+		const float u = __builtin_fmaf(p.dcx2, ox, __builtin_fmaf(p.dcy2, oy, __builtin_fmaf(p.dcz2, oz, oo)));
+		const float r2 = p.d2f + u;
 		const float rinv = __builtin_amdgcn_rsqf(r2);
-		const float g = P.w * rinv;
+		const float g = w * rinv;
 		float ec, es, th = 0.0f;
 		if (FAM == HELM) {
 			const float r = r2 * rinv;
-			// r - d = (2 dc.off + |off|^2) / (r + d): accurate phase correction
-			const float u = __builtin_fmaf(p.dcx2, P.ox, __builtin_fmaf(p.dcy2, P.oy, __builtin_fmaf(p.dcz2, P.oz, P.oo)));
-			const float delta = u * __builtin_amdgcn_rcpf(r + p.df);
-			const float t = __builtin_fmaf(k_revf, delta, p.t0);
-			float sn, cs;
-			sincospif(2.0f * t, &sn, &cs);
-			ec = g * cs;
-			es = -g * sn;
+			const float t = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(r + p.df), p.t0);
+			ec = g * __builtin_amdgcn_cosf(t);
+			es = -g * __builtin_amdgcn_sinf(t);
 			th = k * r;
 		} else {
 			ec = g;
@@ -187,23 +205,23 @@ __device__ __forceinline__ void accumulate_points(const QPoint *__restrict__ qp,
 		}
 		if (MASK & SLP) { s[0].x += ec; s[0].y += es; }
 		if (MASK & (DLP | DLPT | HSP)) {
-			float rdny = 0.0f, rdnx = 0.0f;
-			if (MASK & (DLP | HSP)) rdny = (rx * p.nyx + ry * p.nyy + rz * p.nyz) * rinv;
-			if (MASK & (DLPT | HSP)) rdnx = -(rx * p.nxx + ry * p.nxy + rz * p.nxz) * rinv;
-			if (MASK & (DLP | DLPT)) {
-				// E (-1 - i th) / r
-				const float f1r = __builtin_fmaf(th, es, -ec) * rinv;
-				const float f1i = __builtin_fmaf(-th, ec, -es) * rinv;
-				if (MASK & DLP) { s[1].x += f1r * rdny; s[1].y += f1i * rdny; }
-				if (MASK & DLPT) { s[2].x += f1r * rdnx; s[2].y += f1i * rdnx; }
-			}
-			if (MASK & HSP) {
-				const float cc = rdnx * rdny;
-				const float ar = __builtin_fmaf(3.0f - th * th, cc, p.nxny);
-				const float ai = th * __builtin_fmaf(3.0f, cc, p.nxny);
-				const float ri2 = rinv * rinv;
-				s[3].x += (ec * ar - es * ai) * ri2;
-				s[3].y += (ec * ai + es * ar) * ri2;
+			const float ri2 = rinv * rinv;
+			const float Gr = ec * ri2, Gi = es * ri2;
+			const float Qr = __builtin_fmaf(-th, Gi, Gr);
+			const float Qi = __builtin_fmaf(th, Gr, Gi);
+			if (MASK & DLP) { s[1].x += Qr; s[1].y += Qi; }
+			if ((MASK & HSP) && !(MASK & DLP)) { s[4].x += Qr; s[4].y += Qi; }
+			if (MASK & (DLPT | HSP)) {
+				const float b = __builtin_fmaf(oz, p.nxz, __builtin_fmaf(oy, p.nxy, __builtin_fmaf(ox, p.nxx, p.b0)));
+				if (MASK & DLPT) { s[2].x = __builtin_fmaf(Qr, b, s[2].x); s[2].y = __builtin_fmaf(Qi, b, s[2].y); }
+				if (MASK & HSP) {
+					const float w2 = b * ri2;
+					const float th2 = th * th;
+					const float hr = __builtin_fmaf(3.0f, Qr, -th2 * Gr);
+					const float hi = __builtin_fmaf(3.0f, Qi, -th2 * Gi);
+					s[3].x = __builtin_fmaf(hr, w2, s[3].x);
+					s[3].y = __builtin_fmaf(hi, w2, s[3].y);
+				}
 			}
 		}
 	}
@@ -212,9 +230,9 @@ __device__ __forceinline__ void accumulate_points(const QPoint *__restrict__ qp,
 // Evaluate the kernels of MASK for the active lanes, each lane at its own
 // quadrature degree; lanes are served one distinct degree at a time so that
 // the table pointer stays wave-uniform.
-template <int FAM, int MASK, int ORDER>
+template <int FAM, int MASK, int ORDER, int NQ>
 __device__ __forceinline__ void eval_group(const RegArgs &a, int j, double d2, bool act, const PairCtx &p,
-	float2 (&s)[4])
+	float2 (&s)[5])
 {
 	int deg = a.fixed_degree >= 0 ? a.fixed_degree : a.field_complexity + table_degree<ORDER>(d2, a.thr, a.sn, j);
 	deg = act ? deg : -1;
@@ -224,13 +242,15 @@ __device__ __forceinline__ void eval_group(const RegArgs &a, int j, double d2, b
 		const int lane = __ffsll((long long)m) - 1;
 		const int dsel = __builtin_amdgcn_readlane(deg, lane);
 		if (deg == dsel) {
-			accumulate_points<FAM, MASK>(a.qt[dsel] + (size_t)j * a.qn[dsel], a.qn[dsel], p, a.k_revf, a.k, s);
+			accumulate_points<FAM, MASK, NQ>(a.qt[dsel] + (size_t)j * a.qn[dsel], a.qn[dsel], p, a.k_revf, a.k, s);
 			deg = -1;
 		}
 	}
 }
 
-template <int FAM, int KM, int MODE>
+// FIXQ > 0: fixed rule with FIXQ points (the caller guarantees a.fixed_degree
+// has that many); 0: any policy.
+template <int FAM, int KM, int MODE, int FIXQ>
 __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 {
 	const int tile_r = blockIdx.x % a.row_tiles;
@@ -242,18 +262,24 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 	PairCtx p;
 	p.nxx = p.nxy = p.nxz = 0.0f;
 	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
-	p.nyx = p.nyy = p.nyz = p.nxny = 0.0f;
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
+	const auto sc = uni(a.sc);
+	const auto snu = uni(a.snu);
+	const auto thr = uni(a.thr);
 	for (int j = j0; j < j1; ++j) {
-		const double cjx = a.sc[j], cjy = a.sc[a.sn + j], cjz = a.sc[2 * a.sn + j];
+		const double cjx = sc[j], cjy = sc[a.sn + j], cjz = sc[2 * a.sn + j];
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
-		const bool act = row_ok && !(d2 < a.thr[THR_NEAR * a.sn + j]) && !(a.same_mesh && i == j);
+		const bool act = row_ok && !(d2 < thr[THR_NEAR * a.sn + j]) && !(a.same_mesh && i == j);
 		if (__ballot(act) == 0) continue;
-		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
-		p.dcx2 = 2.0f * p.dcx; p.dcy2 = 2.0f * p.dcy; p.dcz2 = 2.0f * p.dcz;
+		const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);
+		p.dcx2 = 2.0f * dcx; p.dcy2 = 2.0f * dcy; p.dcz2 = 2.0f * dcz;
+		p.d2f = (float)d2;
 		if (FAM == HELM) {
-			const double d = sqrt(d2);
+			// d in FP64 from the hardware reciprocal square root and one Newton step
+			double y = __builtin_amdgcn_rsq(d2);
+			y = y * (1.5 - 0.5 * d2 * y * y);
+			const double d = d2 * y;
 			double t = d * a.k_rev;
 			t -= rint(t);
 			p.df = (float)d;
@@ -261,29 +287,46 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 		} else {
 			p.df = 0.0f; p.t0 = 0.0f;
 		}
-		if (KM & (DLP | HSP)) { p.nyx = a.snu[j]; p.nyy = a.snu[a.sn + j]; p.nyz = a.snu[2 * a.sn + j]; }
-		if (KM & HSP) p.nxny = p.nxx * p.nyx + p.nxy * p.nyy + p.nxz * p.nyz;
-		float2 s[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-		if (a.fixed_degree >= 0) {
-			eval_group<FAM, KM, 1>(a, j, d2, act, p, s);
+		float av = 0.0f, nn = 0.0f;
+		if (KM & (DLP | HSP)) {
+			const float nyx = snu[j], nyy = snu[a.sn + j], nyz = snu[2 * a.sn + j];
+			av = dcx * nyx + dcy * nyy + dcz * nyz;
+			if (KM & HSP) nn = p.nxx * nyx + p.nxy * nyy + p.nxz * nyz;
+		}
+		if (KM & (DLPT | HSP)) p.b0 = dcx * p.nxx + dcy * p.nxy + dcz * p.nxz;
+		float2 s[5] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
+			make_float2(0.f, 0.f)};
+		bool hsp_apart = false;
+		if (FIXQ > 0) {
+			if (act) accumulate_points<FAM, KM, FIXQ>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s);
+		} else if (a.fixed_degree >= 0) {
+			eval_group<FAM, KM, 1, 0>(a, j, d2, act, p, s);
 		} else {
-			if (KM & SLP) eval_group<FAM, KM & SLP, 1>(a, j, d2, act, p, s);
-			if (KM & (DLP | DLPT)) eval_group<FAM, KM & (DLP | DLPT), 2>(a, j, d2, act, p, s);
-			if (KM & HSP) eval_group<FAM, KM & HSP, 3>(a, j, d2, act, p, s);
+			if (KM & SLP) eval_group<FAM, KM & SLP, 1, 0>(a, j, d2, act, p, s);
+			if (KM & (DLP | DLPT)) eval_group<FAM, KM & (DLP | DLPT), 2, 0>(a, j, d2, act, p, s);
+			if (KM & HSP) eval_group<FAM, KM & HSP, 3, 0>(a, j, d2, act, p, s);
+			hsp_apart = true;
 		}
 		if (act) {
+			// kernel values from the accumulators
+			float2 K[4];
+			K[0] = s[0];
+			K[1] = make_float2(-av * s[1].x, -av * s[1].y);
+			K[2] = s[2];
+			const float2 sq = (hsp_apart || !(KM & DLP)) ? s[4] : s[1];
+			K[3] = make_float2(nn * sq.x - av * s[3].x, nn * sq.y - av * s[3].y);
 			const long long idx = (long long)j * a.ld + i;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
-				const float re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
-				const float im = a.alpha.x * s[kk].y + a.alpha.y * s[kk].x;
+				const float re = a.alpha.x * K[kk].x - a.alpha.y * K[kk].y;
+				const float im = a.alpha.x * K[kk].y + a.alpha.y * K[kk].x;
 				store_entry(a.out0, idx, a.dtype, a.accumulate, re, im);
 			} else {
 				// A = alpha D + H, B = G + alpha Ht
-				const float ar = a.alpha.x * s[3].x - a.alpha.y * s[3].y + s[1].x;
-				const float ai = a.alpha.x * s[3].y + a.alpha.y * s[3].x + s[1].y;
-				const float br = s[0].x + a.alpha.x * s[2].x - a.alpha.y * s[2].y;
-				const float bi = s[0].y + a.alpha.x * s[2].y + a.alpha.y * s[2].x;
+				const float ar = a.alpha.x * K[3].x - a.alpha.y * K[3].y + K[1].x;
+				const float ai = a.alpha.x * K[3].y + a.alpha.y * K[3].x + K[1].y;
+				const float br = K[0].x + a.alpha.x * K[2].x - a.alpha.y * K[2].y;
+				const float bi = K[0].y + a.alpha.x * K[2].y + a.alpha.y * K[2].x;
 				store_entry(a.out0, idx, a.dtype, a.accumulate, ar, ai);
 				store_entry(a.out1, idx, a.dtype, a.accumulate, br, bi);
 			}
