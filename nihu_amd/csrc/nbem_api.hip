@@ -133,6 +133,27 @@ void prof_end(int idx, hipStream_t st)
 	(void)hipEventRecord(b, st);
 }
 
+// per-device side stream: the FP64 passes run on it, forked from and joined
+// back into the caller's stream with events, so that they overlap the regular
+// pass (all passes write disjoint entries)
+std::mutex g_side_mu;
+std::map<int, hipStream_t> g_side;
+
+int side_stream(hipStream_t *out)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_side_mu);
+	auto it = g_side.find(dev);
+	if (it == g_side.end()) {
+		hipStream_t s;
+		NB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		it = g_side.emplace(dev, s).first;
+	}
+	*out = it->second;
+	return NB_OK;
+}
+
 // per-device rule tables of the FP64 pass
 std::mutex g_rules_mu;
 std::map<int, DevPtr> g_rules;
@@ -358,6 +379,18 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
 	const long long col_tiles = (trial->n + a.tile_cols - 1) / a.tile_cols;
 	const long long grid = (long long)a.row_tiles * col_tiles;
+
+	// fork the FP64 passes onto the side stream
+	hipStream_t side = st;
+	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+	const bool has_fp64 = nl->n > 0 || jb.same;
+	if (has_fp64) {
+		if ((s = side_stream(&side))) return s;
+		NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+		NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+		NB_HIP(hipEventRecord(ev_fork, st));
+		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+	}
 	if (grid > 0) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
@@ -407,11 +440,11 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		na.pj = diag ? nullptr : (const int *)nl->pj.get();
 		if (na.n_pairs == 0) continue;
 		const int g = (na.n_pairs + 63) / 64;
-		const int pe = prof_begin(diag ? 2 : 1, st);
+		const int pe = prof_begin(diag ? 2 : 1, side);
 		using namespace nbk;
 #define NB_FP64(F, K, M) do { \
-		if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true>), dim3(g), dim3(64), 0, st, na); \
-		else hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, false>), dim3(g), dim3(64), 0, st, na); } while (0)
+		if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true>), dim3(g), dim3(64), 0, side, na); \
+		else hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, false>), dim3(g), dim3(64), 0, side, na); } while (0)
 		if (jb.mode == OUT_BM) NB_FP64(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
@@ -430,7 +463,14 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		}
 #undef NB_FP64
 		NB_HIP(hipGetLastError());
-		prof_end(pe, st);
+		prof_end(pe, side);
+	}
+	if (has_fp64) {
+		// join: the caller's stream waits for the FP64 passes
+		NB_HIP(hipEventRecord(ev_join, side));
+		NB_HIP(hipStreamWaitEvent(st, ev_join, 0));
+		NB_HIP(hipEventDestroy(ev_fork));
+		NB_HIP(hipEventDestroy(ev_join));
 	}
 	return NB_OK;
 }
