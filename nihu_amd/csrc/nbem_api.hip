@@ -10,12 +10,14 @@
 
 #include "../../include/nbem.h"
 #include "nbem_kernels.hpp"
+#include "nbem_sweep.hpp"
 
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -141,6 +143,15 @@ std::map<int, hipStream_t> g_side;
 
 int side_stream(hipStream_t *out)
 {
+	// the FP64 passes run on the caller's stream after the regular pass unless
+	// NBEM_OVERLAP=1 puts them on a side stream that overlaps it (same result;
+	// measured on MI355X the overlap gains nothing once the FP64 pass is cheap,
+	// since both passes are VALU-bound)
+	static const bool serial = [] {
+		const char *e = std::getenv("NBEM_OVERLAP");
+		return !(e && e[0] == '1');
+	}();
+	if (serial) { *out = nullptr; return NB_OK; }
 	int dev = 0;
 	NB_HIP(hipGetDevice(&dev));
 	std::lock_guard<std::mutex> lk(g_side_mu);
@@ -386,6 +397,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	const bool has_fp64 = nl->n > 0 || jb.same;
 	if (has_fp64) {
 		if ((s = side_stream(&side))) return s;
+		if (!side) side = st;
 		NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
 		NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 		NB_HIP(hipEventRecord(ev_fork, st));
@@ -556,6 +568,161 @@ int current_device_matches(const nb_mesh *m)
 	int dev = 0;
 	NB_HIP(hipGetDevice(&dev));
 	if (dev != m->device) return fail(NB_ERR_ARG, "mesh lives on another device");
+	return NB_OK;
+}
+
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, nb_launch> g_tuned;
+
+// Burton-Miller sweep.  tune_block > 0: only the regular pass on the leading
+// tune_block x tune_block block (autotuning); otherwise the full matrices.
+int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy, void *A, void *B,
+	int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream, int tune_block, int regular_only)
+{
+	if (!mesh || !k || !A || !B) return fail(NB_ERR_ARG, "null argument");
+	if (n_k <= 0) return fail(NB_ERR_ARG, "n_k must be positive");
+	int s;
+	if ((s = check_policy(policy))) return s;
+	if ((s = current_device_matches(mesh))) return s;
+	const int n = mesh->n;
+	const int nblk = tune_block > 0 ? tune_block : n;
+	if (ld < nblk || k_stride < ld * (int64_t)nblk) return fail(NB_ERR_ARG, "ld / k_stride too small for the matrices");
+	for (int f = 0; f < n_k; ++f)
+		if (!(k[f] > 0.0)) return fail(NB_ERR_ARG, "wavenumbers must be positive");
+	hipStream_t st = (hipStream_t)stream;
+	nb_mesh *m = const_cast<nb_mesh *>(mesh);
+	std::lock_guard<std::mutex> lk(m->mu);
+	const double two_pi = 6.283185307179586476925286766559005768;
+
+	const int Q = policy->rule == NB_RULE_FIXED ? (ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
+	if (Q < 0) return NB_ERR_RANGE;
+	if (policy->rule != NB_RULE_FIXED || (Q != 7 && Q != 1)) {
+		// other policies: one Burton-Miller assembly per wavenumber
+		for (int f = 0; f < n_k; ++f) {
+			Job jb;
+			jb.test = m; jb.trial = m; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
+			jb.k = k[f]; jb.alpha = {0.0, -1.0 / k[f]}; jb.pol = *policy; jb.tile_cols = 32;
+			if ((s = run_device(jb, (float2 *)A + f * k_stride, (float2 *)B + f * k_stride, ld, NB_C64, 0, st))) return s;
+		}
+		return NB_OK;
+	}
+	if ((s = ensure_thresholds(m, policy->near_mode, policy->near_limit))) return s;
+	NearList *nl = nullptr;
+	if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
+
+	nb_launch lc = {32, 4, 4};
+	if (launch && launch->tile_cols > 0) lc = *launch;
+	else {
+		bool found = false;
+		{
+			std::lock_guard<std::mutex> lt(g_tune_mu);
+			auto it = g_tuned.find({n, std::min<int>(n_k, 64)});
+			if (it != g_tuned.end()) { lc = it->second; found = true; }
+		}
+		if (!found && tune_block == 0) {
+			m->mu.unlock();
+			nb_launch best;
+			s = nb_autotune(mesh, n_k, policy, &best);
+			m->mu.lock();
+			if (s) return s;
+			lc = best;
+		}
+	}
+	if (lc.k_batch != 2 && lc.k_batch != 4 && lc.k_batch != 8) return fail(NB_ERR_ARG, "k_batch must be 2, 4 or 8");
+	if (lc.persistent <= 0 || lc.persistent > 32) return fail(NB_ERR_ARG, "persistent workgroups per CU must be 1..32");
+
+	// per-wavenumber constants
+	std::vector<double> krev(n_k), kd(n_k);
+	std::vector<float> kf(n_k), krf(n_k);
+	std::vector<float2> alf(n_k);
+	std::vector<double2> ald(n_k);
+	for (int f = 0; f < n_k; ++f) {
+		kd[f] = k[f];
+		krev[f] = k[f] / two_pi;
+		kf[f] = (float)k[f];
+		krf[f] = (float)krev[f];
+		alf[f] = make_float2(0.0f, (float)(-1.0 / k[f]));
+		ald[f] = make_double2(0.0, -1.0 / k[f]);
+	}
+	DevPtr d_krev, d_kd, d_kf, d_krf, d_alf, d_ald;
+	if ((s = upload(d_krev, krev)) || (s = upload(d_kd, kd)) || (s = upload(d_kf, kf)) || (s = upload(d_krf, krf)) ||
+		(s = upload(d_alf, alf)) || (s = upload(d_ald, ald)))
+		return s;
+
+	int n_cu = 256;
+	{
+		int dev = 0;
+		NB_HIP(hipGetDevice(&dev));
+		NB_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+	}
+
+	// FP64 passes on the side stream (static once per pair, dynamic per wavenumber)
+	hipStream_t side = st;
+	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+	DevPtr d_static;
+	const nbk::Rules *rules = nullptr;
+	if (!regular_only) {
+		if ((s = device_rules(&rules))) return s;
+		if ((s = side_stream(&side))) return s;
+		if (!side) side = st;
+		const int np = nl->n + n;
+		if ((s = dev_alloc(d_static, (size_t)np * 4 * sizeof(double)))) return s;
+		NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+		NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+		NB_HIP(hipEventRecord(ev_fork, st));
+		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+		nbk::StaticArgs sa;
+		sa.n = n; sa.n_near = nl->n; sa.pi = (const int *)nl->pi.get(); sa.pj = (const int *)nl->pj.get();
+		sa.c = (const double *)m->d_c.get(); sa.nu64 = (const double *)m->d_nu64.get();
+		sa.X = (const double *)m->d_X.get(); sa.rules = rules; sa.st = (double *)d_static.get();
+		const int pe1 = prof_begin(1, side);
+		hipLaunchKernelGGL(nbk::fp64_static_kernel, dim3((np + 63) / 64), dim3(64), 0, side, sa);
+		NB_HIP(hipGetLastError());
+		nbk::DynArgs da;
+		da.n = n; da.n_near = nl->n; da.n_k = n_k; da.pi = sa.pi; da.pj = sa.pj; da.c = sa.c; da.nu64 = sa.nu64;
+		da.X = sa.X; da.J = (const double *)m->d_J.get(); da.rules = rules; da.st = sa.st;
+		da.k = (const double *)d_kd.get(); da.alpha = (const double2 *)d_ald.get();
+		da.A = (float2 *)A; da.B = (float2 *)B; da.ld = ld; da.kstride = k_stride;
+		const long long nt = (long long)np * n_k;
+		hipLaunchKernelGGL(nbk::fp64_dynamic_kernel, dim3((nt + 63) / 64), dim3(64), 0, side, da);
+		NB_HIP(hipGetLastError());
+		prof_end(pe1, side);
+	}
+
+	nbk::SweepArgs sw;
+	sw.n = n; sw.nr = nblk; sw.nc = nblk; sw.tile_cols = lc.tile_cols; sw.n_k = n_k;
+	sw.row_tiles = (nblk + 255) / 256;
+	sw.col_tiles = (nblk + lc.tile_cols - 1) / lc.tile_cols;
+	const long long kbt = (n_k + lc.k_batch - 1) / lc.k_batch;
+	const long long items = (long long)sw.row_tiles * sw.col_tiles * kbt;
+	if (items > 0x7fffffffLL) return fail(NB_ERR_RANGE, "too many work items");
+	sw.n_items = (int)items;
+	sw.c = (const double *)m->d_c.get(); sw.nu = (const float *)m->d_nuf.get(); sw.thr = (const double *)m->d_thr.get();
+	sw.qt = (const nbk::QPoint *)m->d_qt[policy->fixed_degree].get();
+	sw.k_rev = (const double *)d_krev.get(); sw.kf = (const float *)d_kf.get(); sw.krevf = (const float *)d_krf.get();
+	sw.alpha = (const float2 *)d_alf.get();
+	sw.A = (float2 *)A; sw.B = (float2 *)B; sw.ld = ld; sw.kstride = k_stride;
+	const long long grid = std::min<long long>(items, (long long)lc.persistent * n_cu);
+	const int pe = prof_begin(3, st);
+	using namespace nbk;
+#define NB_SW(KB, QQ) hipLaunchKernelGGL((sweep_regular_kernel<KB, QQ>), dim3(grid), dim3(256), 0, st, sw)
+	if (Q == 7) {
+		if (lc.k_batch == 2) NB_SW(2, 7); else if (lc.k_batch == 4) NB_SW(4, 7); else NB_SW(8, 7);
+	} else {
+		if (lc.k_batch == 2) NB_SW(2, 1); else if (lc.k_batch == 4) NB_SW(4, 1); else NB_SW(8, 1);
+	}
+#undef NB_SW
+	NB_HIP(hipGetLastError());
+	prof_end(pe, st);
+	if (!regular_only) {
+		NB_HIP(hipEventRecord(ev_join, side));
+		NB_HIP(hipStreamWaitEvent(st, ev_join, 0));
+		NB_HIP(hipEventDestroy(ev_fork));
+		NB_HIP(hipEventDestroy(ev_join));
+	}
+	// the per-call buffers (wavenumber constants, static parts) are freed on
+	// return: wait for the kernels that read them
+	NB_HIP(hipStreamSynchronize(st));
 	return NB_OK;
 }
 
@@ -731,15 +898,61 @@ int nb_classify(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 	return NB_OK;
 }
 
-int nb_assemble_sweep(const nb_mesh *, const double *, int32_t, const nb_policy *, void *, void *, int64_t, int64_t,
-	const nb_launch *, void *)
+int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy, void *A, void *B,
+	int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream)
 {
-	return fail(NB_ERR_UNSUPPORTED, "sweep not built yet");
+	return sweep_impl(mesh, k, n_k, policy, A, B, ld, k_stride, launch, stream, 0, 0);
 }
 
-int nb_autotune(const nb_mesh *, int32_t, const nb_policy *, nb_launch *)
+int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_launch *best)
 {
-	return fail(NB_ERR_UNSUPPORTED, "autotuner not built yet");
+	if (!mesh || !policy || !best) return fail(NB_ERR_ARG, "null argument");
+	if (n_k <= 0) return fail(NB_ERR_ARG, "n_k must be positive");
+	int s;
+	if ((s = check_policy(policy))) return s;
+	if (policy->rule != NB_RULE_FIXED) {
+		// the NiHu-degree policy runs the per-wavenumber kernel: only the tile matters
+		*best = {32, 1, 0};
+		return NB_OK;
+	}
+	const int n = mesh->n;
+	const int nb = std::min(n, 4096);          // tune on a leading block of the matrix
+	const int nk = std::min<int>(n_k, 8);
+	DevPtr A, B;
+	const size_t bytes = (size_t)nb * nb * nk * sizeof(float2);
+	if ((s = dev_alloc(A, bytes)) || (s = dev_alloc(B, bytes))) return s;
+	std::vector<double> ks(nk);
+	for (int f = 0; f < nk; ++f) ks[f] = 1.0 + f;
+	hipStream_t st = nullptr;
+	hipEvent_t e0, e1;
+	NB_HIP(hipEventCreate(&e0));
+	NB_HIP(hipEventCreate(&e1));
+	double best_ms = 1e30;
+	nb_launch bl = {32, 4, 4};
+	for (int tc : {16, 32, 64})
+		for (int kb : {2, 4, 8})
+			for (int pc : {2, 4, 8}) {
+				nb_launch c = {tc, kb, pc};
+				if ((s = sweep_impl(mesh, ks.data(), nk, policy, A.get(), B.get(), nb, (int64_t)nb * nb, &c, st, nb, 1)))
+					return s;   // warm-up
+				NB_HIP(hipEventRecord(e0, st));
+				for (int rep = 0; rep < 3; ++rep)
+					if ((s = sweep_impl(mesh, ks.data(), nk, policy, A.get(), B.get(), nb, (int64_t)nb * nb, &c, st, nb, 1)))
+						return s;
+				NB_HIP(hipEventRecord(e1, st));
+				NB_HIP(hipEventSynchronize(e1));
+				float ms = 0.0f;
+				NB_HIP(hipEventElapsedTime(&ms, e0, e1));
+				if (ms < best_ms) { best_ms = ms; bl = c; }
+			}
+	NB_HIP(hipEventDestroy(e0));
+	NB_HIP(hipEventDestroy(e1));
+	{
+		std::lock_guard<std::mutex> lk(g_tune_mu);
+		g_tuned[{n, std::min<int>(n_k, 64)}] = bl;
+	}
+	*best = bl;
+	return NB_OK;
 }
 
 int nb_set_launch(const nb_launch *launch)

@@ -380,38 +380,49 @@ __device__ __forceinline__ D3 tri_x(const D3 V[3], double xi, double eta)
 
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
+// 1/sqrt(x) in FP64: hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rsqrt64(double x)
+{
+	double y = __builtin_amdgcn_rsq(x);
+	y = y * (1.5 - 0.5 * x * y * y);
+	const double e = __fma_rn(-x * y, y, 1.0);      // 1 - x y^2
+	return __fma_rn(0.5 * y, e, y);
+}
+
 // K_helm - K_lap for the kernels of MASK at one point (FP64), r = y - x
 template <int MASK>
 __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4])
 {
 	const D3 rv = d3sub(y, x);
-	const double r = d3norm(rv);
+	const double r2 = d3dot(rv, rv);
+	const double rinv = rsqrt64(r2);
+	const double r = r2 * rinv;
 	const double kr = k * r;
 	double sn, cs;
 	sincos(kr, &sn, &cs);
 	// e^{-ikr} = cs - i sn
-	const double inv4pir = 1.0 / (4.0 * PI_D * r);
+	const double g0 = rinv * (wj / (4.0 * PI_D));
 	if (MASK & SLP) {
-		s[0].x += (cs - 1.0) * inv4pir * wj;
-		s[0].y += -sn * inv4pir * wj;
+		s[0].x += (cs - 1.0) * g0;
+		s[0].y += -sn * g0;
 	}
 	if (MASK & (DLP | DLPT | HSP)) {
-		const double rdny = d3dot(rv, ny) / r;
-		const double rdnx = -d3dot(rv, nx) / r;
-		const double g1 = inv4pir / r;
+		const double rdny = d3dot(rv, ny) * rinv;
+		const double rdnx = -d3dot(rv, nx) * rinv;
+		const double g1 = g0 * rinv;
 		// e(-1 - ikr) - (-1)
 		const double er = -cs - kr * sn + 1.0;
 		const double ei = -kr * cs + sn;
-		if (MASK & DLP) { s[1].x += er * g1 * rdny * wj; s[1].y += ei * g1 * rdny * wj; }
-		if (MASK & DLPT) { s[2].x += er * g1 * rdnx * wj; s[2].y += ei * g1 * rdnx * wj; }
+		if (MASK & DLP) { s[1].x += er * g1 * rdny; s[1].y += ei * g1 * rdny; }
+		if (MASK & DLPT) { s[2].x += er * g1 * rdnx; s[2].y += ei * g1 * rdnx; }
 		if (MASK & HSP) {
-			const double g3 = g1 / r;
+			const double g3 = g1 * rinv;
 			const double cc = rdnx * rdny;
 			const double nn = d3dot(nx, ny);
 			// e [(3 - kr^2 + 3ikr) cc + (1 + ikr) nn] - [3 cc + nn]
 			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
-			s[3].x += ((cs * ar + sn * ai) - (3.0 * cc + nn)) * g3 * wj;
-			s[3].y += (cs * ai - sn * ar) * g3 * wj;
+			s[3].x += ((cs * ar + sn * ai) - (3.0 * cc + nn)) * g3;
+			s[3].y += (cs * ai - sn * ar) * g3;
 		}
 	}
 }
@@ -419,6 +430,10 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 // Laplace static part of the near-singular integral over a flat triangle in
 // its own plane (nihu/library/laplace_nearly_singular_integrals.hpp:62-389,
 // nihu/library/plane_element_helper.hpp:76-128), all kernels of MASK.
+// Along a ray at angle th from the projected point, the in-plane distance to
+// side n is P = dist_n / cos(th - th0_n), evaluated here as
+// dist_n / (cos th cos th0_n + sin th sin th0_n) with the sin and cos of th
+// that DLPt and HSP need anyway.
 template <int MASK>
 __device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, double (&st)[4])
 {
@@ -432,21 +447,33 @@ __device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, doubl
 	for (int a = 0; a < 3; ++a) C[a] = {d3dot(T0, V[a]), d3dot(T1, V[a]), d3dot(T2, V[a])};
 	const D3 x0 = {d3dot(T0, x0g), d3dot(T1, x0g), d3dot(T2, x0g)};
 	const D3 nx = {d3dot(T0, nxg), d3dot(T1, nxg), d3dot(T2, nxg)};
-	double th_lim[3], th0[3], dist[3];
+	double th_lim[3], c0[3], s0[3], dist[3];
 	for (int n = 0; n < 3; ++n) {
 		const int m = n == 2 ? 0 : n + 1;
 		double lx = C[m].x - C[n].x, ly = C[m].y - C[n].y;
-		const double ln = sqrt(lx * lx + ly * ly);
-		lx /= ln; ly /= ln;
+		const double lninv = rsqrt64(lx * lx + ly * ly);
+		lx *= lninv; ly *= lninv;
 		const double d1x = C[n].x - x0.x, d1y = C[n].y - x0.y;
 		const double dl = d1x * lx + d1y * ly;
 		const double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
 		th_lim[n] = atan2(d1y, d1x);
-		th0[n] = atan2(d0y, d0x);
-		dist[n] = sqrt(d0x * d0x + d0y * d0y);
+		const double d02 = d0x * d0x + d0y * d0y;
+		// direction of the foot of the perpendicular: (cos th0, sin th0);
+		// th0 = atan2(0, 0) = 0 when the projected point lies on the side's line
+		if (d02 > 0.0) {
+			const double d0inv = rsqrt64(d02);
+			dist[n] = d02 * d0inv;
+			c0[n] = d0x * d0inv;
+			s0[n] = d0y * d0inv;
+		} else {
+			dist[n] = 0.0;
+			c0[n] = 1.0;
+			s0[n] = 0.0;
+		}
 	}
 	const double z = x0.z - C[0].z;
 	const double az = fabs(z);
+	const double z2 = z * z;
 	const double sgz = (0.0 < z) - (z < 0.0);
 	double rs = 0.0, rd = 0.0, rt = 0.0, rh = 0.0;
 	for (int n = 0; n < 3; ++n) {
@@ -458,31 +485,35 @@ __device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, doubl
 		const double jac = (t2 - t1) / 2.0;
 		if (MASK & (SLP | DLP | DLPT)) {
 			for (int g = 0; g < 6; ++g) {
-				const double xi = R.gl6x[g], w = R.gl6w[g];
+				const double xi = R.gl6x[g], w = R.gl6w[g] * jac;
 				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
-				const double P = dist[n] / cos(th - th0[n]);
-				const double r = sqrt(P * P + z * z);
-				if (MASK & SLP) rs += (r - az) * jac * w;
-				if (MASK & DLP) rd += (1.0 - az / r) * jac * w;
+				double sth, cth;
+				sincos(th, &sth, &cth);
+				const double P = dist[n] / (cth * c0[n] + sth * s0[n]);
+				const double r2 = P * P + z2;
+				const double rinv = rsqrt64(r2);
+				const double r = r2 * rinv;
+				if (MASK & SLP) rs += (r - az) * w;
+				if (MASK & DLP) rd += (1.0 - az * rinv) * w;
 				if (MASK & DLPT) {
-					double sth, cth;
-					sincos(th, &sth, &cth);
 					const double rdot = nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z);
-					rt += (-rdot / r + (nx.x * cth + nx.y * sth) * log(P + r) - nx.z * sgz) * jac * w;
+					rt += (-rdot * rinv + (nx.x * cth + nx.y * sth) * log(P + r) - nx.z * sgz) * w;
 				}
 			}
 		}
 		if (MASK & HSP) {
 			for (int g = 0; g < 21; ++g) {
-				const double xi = R.gl21x[g], w = R.gl21w[g];
+				const double xi = R.gl21x[g], w = R.gl21w[g] * jac;
 				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
-				const double P = dist[n] / cos(th - th0[n]);
-				const double r = sqrt(P * P + z * z);
 				double sth, cth;
 				sincos(th, &sth, &cth);
-				const double rdnx = -(nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z)) / r;
-				const double integrand = az > 1e-12 ? -P * P / (r * r) / z * rdnx : -nx.z / P;
-				rh += integrand * jac * w;
+				const double Pinv = (cth * c0[n] + sth * s0[n]) / dist[n];
+				const double P = 1.0 / Pinv;
+				const double r2 = P * P + z2;
+				const double rdot = nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z);
+				// -P^2 / r^2 / z * (-r.n / r) with r.n = rdot, or -n_z / P in the plane
+				const double integrand = az > 1e-12 ? P * P * rdot * rsqrt64(r2) / (r2 * z) : -nx.z * Pinv;
+				rh += integrand * w;
 			}
 		}
 	}
@@ -497,8 +528,11 @@ __device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, doubl
 // (nihu/library/helmholtz_singular_integrals.hpp:703-839,
 //  nihu/library/laplace_singular_integrals.hpp:456-505, plane_element_helper.hpp:42-74);
 // DLP and DLPt vanish (normal_derivative_singular_integrals.hpp:13-58).
-template <int FAM, int MASK>
-__device__ void diag_values(const D3 V[3], D3 x0, double J, double k, const Rules &Rr, double2 (&s)[4])
+// Static (wavenumber-independent) part: st[0] = sum r sin(a) ln(tan((a+t)/2)/tan(a/2))
+// over all sides (SLP), st[1] = the same over sides with theta >= 1e-3 (IG0 of
+// the Helmholtz HSP), st[2] = sum (cos(a+t) - cos a)/(r sin a) over sides with
+// theta >= 1e-3 (IddG0), st[3] = the same over all sides (Laplace HSP).
+__device__ void diag_static(const D3 V[3], D3 x0, double (&st)[4])
 {
 	double r[3], theta[3], alpha[3];
 	D3 Rv[3], Cv[3];
@@ -514,67 +548,89 @@ __device__ void diag_values(const D3 V[3], D3 x0, double J, double k, const Rule
 		theta[a] = cs >= 1.0 ? 0.0 : acos(cs);
 		alpha[a] = acos(d3dot(Rv[a], Cv[a]));
 	}
-	const double f = 1.0 / (4.0 * PI_D);
-	if (MASK & SLP) {
-		double st = 0.0;
-		for (int a = 0; a < 3; ++a)
-			st += r[a] * sin(alpha[a]) * log(tan((alpha[a] + theta[a]) / 2.0) / tan(alpha[a] / 2.0));
-		double dr = 0.0, di = 0.0;
-		if (FAM == HELM) {
-			for (int q = 0; q < 13; ++q) {
-				const double rq = d3norm(d3sub(tri_x(V, Rr.d7xi[q], Rr.d7eta[q]), x0));
-				// -ik e^{-ik rq/2} sinc(k rq/2)
-				const double h = k * rq / 2.0;
-				const double sc = fabs(h) > 1e-3 ? sin(h) / h : 1.0 - h * h / 6.0 * (1.0 - h * h / 20.0);
-				double sh, ch;
-				sincos(h, &sh, &ch);
-				dr += (-k * sh * sc) * Rr.d7w[q] * J;
-				di += (-k * ch * sc) * Rr.d7w[q] * J;
-			}
-		}
-		s[0] = make_double2((st + dr) * f, di * f);
+	st[0] = st[1] = st[2] = st[3] = 0.0;
+	for (int a = 0; a < 3; ++a) {
+		const double sa = sin(alpha[a]);
+		const double lg = r[a] * sa * log(tan((alpha[a] + theta[a]) / 2.0) / tan(alpha[a] / 2.0));
+		const double dd = (cos(alpha[a] + theta[a]) - cos(alpha[a])) / (r[a] * sa);
+		st[0] += lg;
+		st[3] += dd;
+		if (theta[a] < 1e-3) continue;
+		st[1] += lg;
+		st[2] += dd;
 	}
-	if (MASK & HSP) {
-		double IG0 = 0.0, IddG0 = 0.0;
-		for (int a = 0; a < 3; ++a) {
-			if (FAM == HELM && theta[a] < 1e-3) continue;
-			if (FAM == HELM)
-				IG0 += r[a] * sin(alpha[a]) * log(tan((alpha[a] + theta[a]) / 2.0) / tan(alpha[a] / 2.0));
-			IddG0 += (cos(alpha[a] + theta[a]) - cos(alpha[a])) / (r[a] * sin(alpha[a]));
+}
+
+// Wavenumber-dependent (13-point) part of the Helmholtz diagonal integrals:
+// dyn[0] for SLP, dyn[1] for HSP, not yet divided by 4 pi.
+template <int MASK>
+__device__ void diag_dynamic(const D3 V[3], D3 x0, double J, double k, const Rules &Rr, double2 (&dyn)[2])
+{
+	double dr = 0.0, di = 0.0, ar = 0.0, ai = 0.0;
+	for (int q = 0; q < 13; ++q) {
+		const double rq = d3norm(d3sub(tri_x(V, Rr.d7xi[q], Rr.d7eta[q]), x0));
+		if (MASK & SLP) {
+			// -ik e^{-ik rq/2} sinc(k rq/2)
+			const double h = k * rq / 2.0;
+			const double sc = fabs(h) > 1e-3 ? sin(h) / h : 1.0 - h * h / 6.0 * (1.0 - h * h / 20.0);
+			double sh, ch;
+			sincos(h, &sh, &ch);
+			dr += (-k * sh * sc) * Rr.d7w[q] * J;
+			di += (-k * ch * sc) * Rr.d7w[q] * J;
 		}
-		double ar = 0.0, ai = 0.0;
-		if (FAM == HELM) {
-			for (int q = 0; q < 13; ++q) {
-				const double rq = d3norm(d3sub(tri_x(V, Rr.d7xi[q], Rr.d7eta[q]), x0));
-				double vr, vi;
-				if (fabs(rq) > 1e-3) {
-					// (e^{-ikr}(1 + ikr) - 1 + (ikr)^2/2) / r^3
-					const double kr = k * rq;
-					double sn, cs;
-					sincos(kr, &sn, &cs);
-					const double er = cs + kr * sn, ei = kr * cs - sn;
-					vr = (er - 1.0 - kr * kr / 2.0) / (rq * rq * rq);
-					vi = ei / (rq * rq * rq);
-				} else {
-					// -i k^3 (1/3 - ikr(1/8 - ikr(1/30 - ikr(1/144 - ikr(1/840 - ikr/5760)))))
-					const double kr = k * rq;
-					double pr = 1.0 / 840.0, pi = -kr / 5760.0;   // 1/840 - ikr/5760
-					double c2[4] = {1.0 / 144.0, 1.0 / 30.0, 1.0 / 8.0, 1.0 / 3.0};
-					for (int t = 0; t < 4; ++t) {
-						// c - ikr * p
-						const double nr = c2[t] + kr * pi;
-						const double ni = -kr * pr;
-						pr = nr; pi = ni;
-					}
-					const double k3 = k * k * k;
-					vr = k3 * pi;        // -i k^3 (pr + i pi) = k^3 pi - i k^3 pr
-					vi = -k3 * pr;
+		if (MASK & HSP) {
+			double vr, vi;
+			if (fabs(rq) > 1e-3) {
+				// (e^{-ikr}(1 + ikr) - 1 + (ikr)^2/2) / r^3
+				const double kr = k * rq;
+				double sn, cs;
+				sincos(kr, &sn, &cs);
+				const double er = cs + kr * sn, ei = kr * cs - sn;
+				vr = (er - 1.0 - kr * kr / 2.0) / (rq * rq * rq);
+				vi = ei / (rq * rq * rq);
+			} else {
+				// -i k^3 (1/3 - ikr(1/8 - ikr(1/30 - ikr(1/144 - ikr(1/840 - ikr/5760)))))
+				const double kr = k * rq;
+				double pr = 1.0 / 840.0, pi = -kr / 5760.0;
+				const double c2[4] = {1.0 / 144.0, 1.0 / 30.0, 1.0 / 8.0, 1.0 / 3.0};
+				for (int t = 0; t < 4; ++t) {
+					const double nr = c2[t] + kr * pi;
+					const double ni = -kr * pr;
+					pr = nr; pi = ni;
 				}
-				ar += vr * Rr.d7w[q] * J;
-				ai += vi * Rr.d7w[q] * J;
+				const double k3 = k * k * k;
+				vr = k3 * pi;
+				vi = -k3 * pr;
 			}
+			ar += vr * Rr.d7w[q] * J;
+			ai += vi * Rr.d7w[q] * J;
 		}
-		s[3] = make_double2((IddG0 + k * k / 2.0 * IG0 + ar) * f, ai * f);
+	}
+	dyn[0] = make_double2(dr, di);
+	dyn[1] = make_double2(ar, ai);
+}
+
+// combine the static and dynamic diagonal parts into kernel values
+template <int FAM, int MASK>
+__device__ __forceinline__ void diag_combine(const double (&st)[4], const double2 (&dyn)[2], double k, double2 (&s)[4])
+{
+	const double f = 1.0 / (4.0 * PI_D);
+	if (MASK & SLP) s[0] = FAM == HELM ? make_double2((st[0] + dyn[0].x) * f, dyn[0].y * f) : make_double2(st[0] * f, 0.0);
+	if (MASK & HSP)
+		s[3] = FAM == HELM ? make_double2((st[2] + k * k / 2.0 * st[1] + dyn[1].x) * f, dyn[1].y * f)
+		                   : make_double2(st[3] * f, 0.0);
+}
+
+// dynamic part of the Helmholtz near-singular integral: sum over the 42
+// Dunavant-14 points of (K_helm - K_lap) J w
+// (nihu/library/helmholtz_nearly_singular_integrals.hpp:48-219)
+template <int MASK>
+__device__ __forceinline__ void near_dynamic(const D3 V[3], D3 x0, D3 nx, D3 ny, double J, double k, const Rules &R,
+	double2 (&s)[4])
+{
+	for (int q = 0; q < 42; ++q) {
+		const D3 y = tri_x(V, R.d14xi[q], R.d14eta[q]);
+		dynamic_point<MASK>(x0, y, nx, ny, k, R.d14w[q] * J, s);
 	}
 }
 
@@ -595,17 +651,16 @@ __global__ void __launch_bounds__(64) fp64_pass_kernel(NearArgs a)
 	const double J = a.sJ[j];
 	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
 	if (DIAG) {
-		diag_values<FAM, KM & (SLP | HSP)>(V, x0, J, a.k, R, s);
+		double st[4];
+		double2 dyn[2] = {make_double2(0, 0), make_double2(0, 0)};
+		diag_static(V, x0, st);
+		if (FAM == HELM) diag_dynamic<KM & (SLP | HSP)>(V, x0, J, a.k, R, dyn);
+		diag_combine<FAM, KM & (SLP | HSP)>(st, dyn, a.k, s);
 	} else {
 		double st[4] = {0, 0, 0, 0};
 		near_static<KM>(V, x0, nx, R, st);
 		for (int t = 0; t < 4; ++t) s[t].x = st[t];
-		if (FAM == HELM) {
-			for (int q = 0; q < 42; ++q) {
-				const D3 y = tri_x(V, R.d14xi[q], R.d14eta[q]);
-				dynamic_point<KM>(x0, y, nx, ny, a.k, R.d14w[q] * J, s);
-			}
-		}
+		if (FAM == HELM) near_dynamic<KM>(V, x0, nx, ny, J, a.k, R, s);
 	}
 	const long long idx = (long long)j * a.ld + i;
 	if (MODE == OUT_SINGLE) {

@@ -1,0 +1,225 @@
+// nbem_sweep.hpp -- Burton-Miller matrices for many wavenumbers on one mesh.
+//
+// Regular pairs: one persistent launch.  A workgroup of 256 lanes (one test
+// row per lane) takes work items (row tile, trial-column tile, batch of KB
+// wavenumbers) with a fixed stride until none are left.  For each pair the
+// wavenumber-independent geometry of every quadrature point (r, r - d, the
+// weight over r, 1/r^2 and the test-normal projection) is computed once and
+// kept in registers while the KB wavenumbers of the batch reuse it; per
+// wavenumber only the phase, the hardware sin/cos and the kernel algebra
+// remain.  The FP64 near-singular and diagonal passes compute their
+// wavenumber-independent (Laplace static) parts once per pair into a buffer
+// and add the dynamic part per (pair, wavenumber).
+
+#pragma once
+
+#include "nbem_kernels.hpp"
+
+namespace nbk {
+
+struct SweepArgs {
+	int n, row_tiles, col_tiles, tile_cols, n_k, n_items;
+	int nr, nc;                 // rows / columns processed (n, or a leading block when tuning)
+	const double *c;            // centres [3][n]
+	const float *nu;            // unit normals [3][n]
+	const double *thr;          // [NTHR][n]
+	const QPoint *qt;           // quadrature table of the fixed degree [n][Q]
+	const double *k_rev;        // [n_k] k / (2 pi), FP64
+	const float *kf;            // [n_k] k
+	const float *krevf;         // [n_k] k / (2 pi)
+	const float2 *alpha;        // [n_k]
+	float2 *A, *B;
+	long long ld, kstride;
+};
+
+template <int KB, int Q>
+__global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
+{
+	const auto sc = uni(a.c);
+	const auto snu = uni(a.nu);
+	const auto thr = uni(a.thr);
+	const auto k_rev = uni(a.k_rev);
+	const auto kf = uni(a.kf);
+	const auto krevf = uni(a.krevf);
+	const auto alpha = uni((const float *)a.alpha);
+	for (int item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+		const int rt = item % a.row_tiles;
+		const int rest = item / a.row_tiles;
+		const int ct = rest % a.col_tiles;
+		const int f0 = (rest / a.col_tiles) * KB;
+		const int i = rt * 256 + threadIdx.x;
+		const bool row_ok = i < a.nr;
+		const int ii = row_ok ? i : a.nr - 1;
+		const double cix = a.c[ii], ciy = a.c[a.n + ii], ciz = a.c[2 * a.n + ii];
+		const float nxx = a.nu[ii], nxy = a.nu[a.n + ii], nxz = a.nu[2 * a.n + ii];
+		const int j0 = ct * a.tile_cols;
+		const int j1 = min(j0 + a.tile_cols, a.nc);
+		for (int j = j0; j < j1; ++j) {
+			const double cjx = sc[j], cjy = sc[a.n + j], cjz = sc[2 * a.n + j];
+			const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
+			const bool act = row_ok && !(d2 < thr[THR_NEAR * a.n + j]) && i != j;
+			if (__ballot(act) == 0) continue;
+			const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);
+			const float dcx2 = 2.0f * dcx, dcy2 = 2.0f * dcy, dcz2 = 2.0f * dcz;
+			const float d2f = (float)d2;
+			double y = __builtin_amdgcn_rsq(d2);
+			y = y * (1.5 - 0.5 * d2 * y * y);
+			const double d = d2 * y;
+			const float df = (float)d;
+			const float nyx = snu[j], nyy = snu[a.n + j], nyz = snu[2 * a.n + j];
+			const float av = dcx * nyx + dcy * nyy + dcz * nyz;
+			const float nn = nxx * nyx + nxy * nyy + nxz * nyz;
+			const float b0 = dcx * nxx + dcy * nxy + dcz * nxz;
+			// wavenumber-independent geometry of the points
+			float gr[Q], gdel[Q], gg[Q], gri2[Q], gb[Q], gw2[Q];
+			const auto qf = uni((const float *)(a.qt + (size_t)j * Q));
+#pragma unroll
+			for (int q = 0; q < Q; ++q) {
+				const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3], oo = qf[8 * q + 4];
+				const float u = __builtin_fmaf(dcx2, ox, __builtin_fmaf(dcy2, oy, __builtin_fmaf(dcz2, oz, oo)));
+				const float r2 = d2f + u;
+				const float rinv = __builtin_amdgcn_rsqf(r2);
+				const float r = r2 * rinv;
+				gr[q] = r;
+				gdel[q] = u * __builtin_amdgcn_rcpf(r + df);
+				gg[q] = w * rinv;
+				gri2[q] = rinv * rinv;
+				gb[q] = __builtin_fmaf(oz, nxz, __builtin_fmaf(oy, nxy, __builtin_fmaf(ox, nxx, b0)));
+				gw2[q] = gb[q] * gri2[q];
+			}
+			for (int fb = 0; fb < KB; ++fb) {
+				const int f = f0 + fb;
+				if (f >= a.n_k) break;
+				double td = d * k_rev[f];
+				td -= rint(td);
+				const float t0 = (float)td;
+				const float k = kf[f], kr = krevf[f];
+				float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+#pragma unroll
+				for (int q = 0; q < Q; ++q) {
+					const float t = __builtin_fmaf(kr, gdel[q], t0);
+					const float ec = gg[q] * __builtin_amdgcn_cosf(t);
+					const float es = -gg[q] * __builtin_amdgcn_sinf(t);
+					const float th = k * gr[q];
+					s0.x += ec; s0.y += es;
+					const float Gr = ec * gri2[q], Gi = es * gri2[q];
+					const float Qr = __builtin_fmaf(-th, Gi, Gr);
+					const float Qi = __builtin_fmaf(th, Gr, Gi);
+					s1.x += Qr; s1.y += Qi;
+					s2.x = __builtin_fmaf(Qr, gb[q], s2.x);
+					s2.y = __builtin_fmaf(Qi, gb[q], s2.y);
+					const float th2 = th * th;
+					const float hr = __builtin_fmaf(3.0f, Qr, -th2 * Gr);
+					const float hi = __builtin_fmaf(3.0f, Qi, -th2 * Gi);
+					s3.x = __builtin_fmaf(hr, gw2[q], s3.x);
+					s3.y = __builtin_fmaf(hi, gw2[q], s3.y);
+				}
+				if (act) {
+					// SLP = s0, DLP = -a s1, DLPt = s2, HSP = nn s1 - a s3
+					const float2 al = make_float2(alpha[2 * f], alpha[2 * f + 1]);
+					const float hsr = nn * s1.x - av * s3.x, hsi = nn * s1.y - av * s3.y;
+					const float ar = al.x * hsr - al.y * hsi - av * s1.x;
+					const float ai = al.x * hsi + al.y * hsr - av * s1.y;
+					const float br = s0.x + al.x * s2.x - al.y * s2.y;
+					const float bi = s0.y + al.x * s2.y + al.y * s2.x;
+					const long long idx = (long long)f * a.kstride + (long long)j * a.ld + i;
+					a.A[idx] = make_float2(ar, ai);
+					a.B[idx] = make_float2(br, bi);
+				}
+			}
+		}
+	}
+}
+
+// Wavenumber-independent parts of the FP64 pairs: near pairs (p < n_near)
+// store the Laplace static integrals of SLP, DLP, DLPt, HSP; diagonal pairs
+// (n_near <= p < n_near + n) store diag_static's four sums.
+struct StaticArgs {
+	int n, n_near;
+	const int *pi, *pj;
+	const double *c, *nu64, *X;
+	const Rules *rules;
+	double *st;                 // [(n_near + n) * 4]
+};
+
+__global__ void __launch_bounds__(64) fp64_static_kernel(StaticArgs a)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= a.n_near + a.n) return;
+	const bool diag = p >= a.n_near;
+	const int i = diag ? p - a.n_near : a.pi[p];
+	const int j = diag ? i : a.pj[p];
+	D3 V[3];
+	for (int v = 0; v < 3; ++v)
+		V[v] = {a.X[(3 * v) * a.n + j], a.X[(3 * v + 1) * a.n + j], a.X[(3 * v + 2) * a.n + j]};
+	const D3 x0 = {a.c[i], a.c[a.n + i], a.c[2 * a.n + i]};
+	double st[4];
+	if (diag) {
+		diag_static(V, x0, st);
+	} else {
+		const D3 nx = {a.nu64[i], a.nu64[a.n + i], a.nu64[2 * a.n + i]};
+		near_static<ALL4>(V, x0, nx, *a.rules, st);
+	}
+	for (int t = 0; t < 4; ++t) a.st[(size_t)p * 4 + t] = st[t];
+}
+
+struct DynArgs {
+	int n, n_near, n_k;
+	const int *pi, *pj;
+	const double *c, *nu64, *X, *J;
+	const Rules *rules;
+	const double *st;
+	const double *k;            // [n_k]
+	const double2 *alpha;       // [n_k]
+	float2 *A, *B;
+	long long ld, kstride;
+};
+
+// one lane per (FP64 pair, wavenumber): consecutive lanes take consecutive pairs
+__global__ void __launch_bounds__(64) fp64_dynamic_kernel(DynArgs a)
+{
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long np = (long long)a.n_near + a.n;
+	if (t >= np * a.n_k) return;
+	const int f = (int)(t / np);
+	const int p = (int)(t % np);
+	const bool diag = p >= a.n_near;
+	const int i = diag ? p - a.n_near : a.pi[p];
+	const int j = diag ? i : a.pj[p];
+	const Rules &R = *a.rules;
+	D3 V[3];
+	for (int v = 0; v < 3; ++v)
+		V[v] = {a.X[(3 * v) * a.n + j], a.X[(3 * v + 1) * a.n + j], a.X[(3 * v + 2) * a.n + j]};
+	const D3 x0 = {a.c[i], a.c[a.n + i], a.c[2 * a.n + i]};
+	const double k = a.k[f];
+	const double J = a.J[j];
+	double st[4];
+	for (int q = 0; q < 4; ++q) st[q] = a.st[(size_t)p * 4 + q];
+	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+	const double2 al = a.alpha[f];
+	double ar, ai, br, bi;
+	if (diag) {
+		double2 dyn[2];
+		diag_dynamic<SLP | HSP>(V, x0, J, k, R, dyn);
+		diag_combine<HELM, SLP | HSP>(st, dyn, k, s);
+	} else {
+		const D3 nx = {a.nu64[i], a.nu64[a.n + i], a.nu64[2 * a.n + i]};
+		const D3 ny = {a.nu64[j], a.nu64[a.n + j], a.nu64[2 * a.n + j]};
+		for (int q = 0; q < 4; ++q) s[q].x = st[q];
+		near_dynamic<ALL4>(V, x0, nx, ny, J, k, R, s);
+	}
+	ar = al.x * s[3].x - al.y * s[3].y + s[1].x;
+	ai = al.x * s[3].y + al.y * s[3].x + s[1].y;
+	br = s[0].x + al.x * s[2].x - al.y * s[2].y;
+	bi = s[0].y + al.x * s[2].y + al.y * s[2].x;
+	if (diag) {
+		ar += -0.5;
+		br += al.x / 2.0;
+		bi += al.y / 2.0;
+	}
+	const long long idx = (long long)f * a.kstride + (long long)j * a.ld + i;
+	a.A[idx] = make_float2((float)ar, (float)ai);
+	a.B[idx] = make_float2((float)br, (float)bi);
+}
+
+} // namespace nbk

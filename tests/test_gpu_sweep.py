@@ -1,0 +1,88 @@
+"""Frequency sweep (one persistent launch for many wavenumbers) against the
+single-wavenumber Burton-Miller path and the FP64 oracle."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CONFIG = dict(rule="fixed", degree=5, near="diameter", limit=2.0)
+CONFIG_O = dict(rule=1, fixed_degree=5, near_mode=1, near_limit=2.0)
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available()
+    from nihu_amd import nbem
+    return nbem
+
+
+@pytest.fixture(scope="module")
+def l3():
+    from nihu_amd.mesh import jittered_icosphere
+    return jittered_icosphere(3, seed=2)
+
+
+def stack(nk, n):
+    import torch
+    return torch.zeros((nk, n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
+
+
+def relfro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kb", [2, 4, 8])
+def test_sweep_matches_single_and_oracle(nb, orc, l3, kb):
+    import torch
+    nodes, tris = l3
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    ks = np.array([0.7, 3.3, 10.0, 17.5, 31.9])
+    A, B = stack(len(ks), n), stack(len(ks), n)
+    nb.assemble_sweep(m, ks, A, B, pol=nb.policy(**CONFIG), launch=nb.NbLaunch(32, kb, 4))
+    torch.cuda.synchronize()
+    for f, k in enumerate(ks):
+        A1, B1 = nb.colmajor(n, n), nb.colmajor(n, n)
+        nb.assemble_bm(m, k, A1, B1, pol=nb.policy(**CONFIG))
+        torch.cuda.synchronize()
+        a, b = A[f].cpu().numpy(), B[f].cpu().numpy()
+        assert relfro(a, A1.cpu().numpy()) < 1e-6 and relfro(b, B1.cpu().numpy()) < 1e-6, f
+    for f in (0, 4):
+        Ar, Br = orc.assemble_bm(nodes, tris, ks[f], pol=orc.policy(**CONFIG_O))
+        ea, eb = relfro(A[f].cpu().numpy(), Ar), relfro(B[f].cpu().numpy(), Br)
+        print("k", ks[f], "A", ea, "B", eb, flush=True)
+        assert ea < 1e-5 and eb < 1e-5
+
+
+def test_sweep_nihu_policy(nb, orc, l3):
+    import torch
+    nodes, tris = l3
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    ks = np.array([2.0, 9.0])
+    A, B = stack(2, n), stack(2, n)
+    nb.assemble_sweep(m, ks, A, B, pol=nb.policy("nihu"))
+    torch.cuda.synchronize()
+    Ar, Br = orc.assemble_bm(nodes, tris, 9.0)
+    assert relfro(A[1].cpu().numpy(), Ar) < 1e-5 and relfro(B[1].cpu().numpy(), Br) < 1e-5
+
+
+def test_autotune(nb, l3):
+    nodes, tris = l3
+    m = nb.Mesh(nodes, tris)
+    best = nb.autotune(m, 8, nb.policy(**CONFIG))
+    print("tuned", best.tile_cols, best.k_batch, best.persistent, flush=True)
+    assert best.tile_cols in (16, 32, 64) and best.k_batch in (2, 4, 8) and best.persistent in (2, 4, 8)
+
+
+def test_sweep_rejects_bad_arguments(nb, l3):
+    nodes, tris = l3
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    A, B = stack(2, n), stack(2, n)
+    with pytest.raises(ValueError):
+        nb.assemble_sweep(m, np.array([1.0, -2.0]), A, B, pol=nb.policy(**CONFIG))
+    with pytest.raises(ValueError):
+        nb.assemble_sweep(m, np.array([1.0, 2.0]), A, B, pol=nb.policy(**CONFIG), launch=nb.NbLaunch(32, 3, 4))
