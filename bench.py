@@ -63,8 +63,11 @@ def main():
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep"], default="bm",
-                    help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles")
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve"], default="bm",
+                    help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
+                         "serve: Poisson stream of assembly-plus-LU-solve requests")
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -74,6 +77,8 @@ def main():
 
     if args.workload == "sweep":
         return run_sweep(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     import numpy as np
     import torch
@@ -290,6 +295,46 @@ def run_sweep(args):
                          "sample": "%d of %d rows, 64 wavenumbers, NiHu's OpenMP-over-frequencies pattern (FP64 oracle)"
                                    ", %.1f s wall" % (len(rows), n, dt)},
         "accuracy": {"max_rel_fro_over_k": float(max(errs)), "rows": len(rows)},
+    }
+    print(json.dumps(out))
+
+
+def run_serve(args):
+    """Poisson stream of Burton-Miller assembly + LU solve requests: mesh
+    uniform over the 1280/5120/20480-triangle jittered icospheres (resident on
+    the device), ka uniform in [1, 40], plane wave of seeded random direction;
+    admission by complex64 footprint, 4 slots (streams)."""
+    import numpy as np
+    import torch
+    from nihu_amd import serve
+    from nihu_amd.mesh import jittered_icosphere
+
+    torch.cuda.set_device(0)
+    meshes = {"L3": jittered_icosphere(3, seed=0), "L4": jittered_icosphere(4, seed=0),
+              "L5": jittered_icosphere(5, seed=0)}
+    srv = serve.Server(meshes, slots=4)
+    warm = [serve.Request(-1 - i, key, 5.0, (0.0, 0.0, 1.0)) for i, key in enumerate(meshes)]
+    srv.run(warm, realtime=False)
+    reqs = serve.poisson_stream(args.requests, rate=args.rate, seed=11)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    lat = [r.latency for r in res]
+    by = {}
+    for r in res:
+        by.setdefault(r.mesh, []).append(r.latency)
+    log("served %d requests in %.2f s; p50 %.3f s p99 %.3f s; %s" % (len(res), wall, serve.percentile(lat, 50),
+        serve.percentile(lat, 99), {k: (len(v), round(serve.percentile(v, 50), 4)) for k, v in sorted(by.items())}))
+    out = {
+        "metric": "solves/s and p50/p99 latency, Poisson stream of Burton-Miller assembly + LU solves",
+        "value": len(res) / wall, "unit": "solves/s", "n_gpus": 1, "steps": len(res), "warmup": len(warm),
+        "ms_per_step": wall / len(res) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64) far pairs and LU, fp64 near/diagonal", "data": "synthetic: seeded Poisson stream",
+        "config": {"workload": "serving", "meshes_n_tri": {k: len(v[1]) for k, v in meshes.items()},
+                   "ka": [1.0, 40.0], "rate_per_s": args.rate, "slots": 4},
+        "latency_s": {"p50": serve.percentile(lat, 50), "p99": serve.percentile(lat, 99),
+                      "per_mesh_p50": {k: serve.percentile(v, 50) for k, v in by.items()},
+                      "per_mesh_p99": {k: serve.percentile(v, 99) for k, v in by.items()}},
     }
     print(json.dumps(out))
 

@@ -1,0 +1,191 @@
+"""Continuous-batching server for assembly-plus-solve requests on one GPU.
+
+A request asks for the rigid-body scattered surface pressure of a plane wave
+p_inc = exp(-i k d.x) (unit amplitude, direction d) on one of the resident
+meshes: the Burton-Miller collocation system
+    A p = B q,   A = alpha*D - I/2 + H,   B = G + alpha*Ht + alpha/2 I,
+    alpha = -i/k,  q = -dp_inc/dn = i k (d.n) exp(-i k d.x)
+(the composition of nihu/applications/helmholtz/helmholtz_3d_coll.cpp:35-120,
+with a dense LU solve where that application uses GMRES) is assembled by the
+HIP engine and solved by LU on the device.
+
+Scheduling: requests arrive over time (their `arrival` offsets); the loop
+admits the oldest waiting request as soon as a slot (a HIP stream) is free
+and its HBM footprint fits the budget, launches its whole pipeline
+asynchronously on that slot's stream and goes on admitting while earlier
+requests are still on the device, so requests of very different sizes
+overlap.  Completion is detected by events; latency = completion time -
+arrival time.  Results do not depend on the mix: every kernel computes each
+matrix entry in a fixed order without atomics, and the LU of a request runs
+on its own stream and buffers.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import time
+from collections import deque
+
+import numpy as np
+
+from . import nbem
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    mesh: str              # key of a resident mesh
+    k: float               # wavenumber
+    direction: tuple       # unit propagation direction of the plane wave
+    arrival: float = 0.0   # seconds after the start of the run
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    mesh: str
+    k: float
+    n: int
+    arrival: float
+    admitted: float
+    done: float
+    p: np.ndarray          # scattered surface pressure at the element centres
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+def footprint_bytes(n: int) -> int:
+    """Device bytes a request holds while admitted: A, B and the LU factor
+    (complex64 n x n each) plus vectors and pivots."""
+    return 3 * 8 * n * n + 64 * n
+
+
+class Server:
+    def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None):
+        import torch
+        self.torch = torch
+        self.meshes = {}
+        self.geom = {}
+        for key, (nodes, tris) in meshes.items():
+            m = nbem.Mesh(nodes, tris)
+            self.meshes[key] = m
+            c, nu = _centres_normals(nodes, tris)
+            self.geom[key] = (torch.from_numpy(c).cuda(), torch.from_numpy(nu).cuda())
+        self.slots = [torch.cuda.Stream() for _ in range(slots)]
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = 0.8 * free
+        self.budget = float(hbm_budget)
+        self.policy = policy or nbem.policy(**nbem.CONFIG_POLICY)
+        for key, m in self.meshes.items():
+            if footprint_bytes(m.n) > self.budget:
+                raise ValueError("mesh %s does not fit the HBM budget" % key)
+
+    def _launch(self, req: Request, stream):
+        torch = self.torch
+        mesh = self.meshes[req.mesh]
+        n = mesh.n
+        c, nu = self.geom[req.mesh]
+        with torch.cuda.stream(stream):
+            A = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
+            B = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
+            nbem.assemble_bm(mesh, req.k, A, B, pol=self.policy, stream=stream)
+            d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")
+            phase = -req.k * (c @ d)
+            pinc = torch.polar(torch.ones_like(phase), phase)
+            q = (1j * req.k) * (nu @ d) * pinc
+            rhs = (B @ q.to(torch.complex64)).unsqueeze(1)
+            LU, piv, _ = torch.linalg.lu_factor_ex(A)
+            p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        # keep the buffers alive until the event has passed
+        return ev, (A, B, LU, piv, rhs, p)
+
+    def run(self, requests, realtime: bool = True):
+        """Serve the requests; returns the results in completion order.
+        realtime: wait for each arrival time (Poisson streams); otherwise all
+        requests are available at once (arrival offsets still set latency)."""
+        torch = self.torch
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        free_slots = list(range(len(self.slots)))
+        in_use = 0.0
+        active = []   # (req, slot, event, bufs, bytes, admitted)
+        results = []
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while pending or active:
+            # admit as many arrived requests as slots and memory allow (FIFO)
+            while pending and free_slots:
+                req = pending[0]
+                if realtime and req.arrival > now():
+                    break
+                nb = footprint_bytes(self.meshes[req.mesh].n)
+                if in_use + nb > self.budget:
+                    break
+                pending.popleft()
+                slot = free_slots.pop(0)
+                t_adm = now()
+                ev, bufs = self._launch(req, self.slots[slot])
+                in_use += nb
+                active.append((req, slot, ev, bufs, nb, t_adm))
+            # retire finished requests
+            still = []
+            for item in active:
+                req, slot, ev, bufs, nb, t_adm = item
+                if ev.query():
+                    p = bufs[-1].cpu().numpy()
+                    results.append(Result(req.rid, req.mesh, req.k, len(p), req.arrival, t_adm, now(), p))
+                    in_use -= nb
+                    free_slots.append(slot)
+                else:
+                    still.append(item)
+            if len(still) == len(active) and still:
+                time.sleep(2e-4)
+            active = still
+            if not active and pending and realtime and pending[0].arrival > now():
+                time.sleep(min(1e-3, max(0.0, pending[0].arrival - now())))
+        torch.cuda.synchronize()
+        return results
+
+
+def _centres_normals(nodes, tris):
+    X = nodes[tris]
+    c = X.mean(axis=1)
+    nv = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+    return c, nv / np.linalg.norm(nv, axis=1)[:, None]
+
+
+def poisson_stream(n_req: int, rate: float, seed: int = 0, meshes=("L3", "L4", "L5"), k_range=(1.0, 40.0)):
+    """Seeded Poisson arrivals (rate per second), mesh uniform over `meshes`,
+    k uniform in k_range, direction uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    reqs = []
+    for i in range(n_req):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        reqs.append(Request(i, str(rng.choice(list(meshes))), float(rng.uniform(*k_range)), tuple(v), float(t[i])))
+    return reqs
+
+
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values), q)) if len(values) else float("nan")
+
+
+def mie_rigid_sphere_surface(k, a, cos_theta, n_terms=None):
+    """Total surface pressure of a unit plane wave exp(-i k d.x) on a rigid
+    sphere of radius a at angle theta from d (time convention exp(+i w t)):
+    p = sum (2n+1) (-i)^n (-i / ((ka)^2 h_n'(ka))) P_n(cos theta),
+    h_n = j_n - i y_n (spherical Hankel function of the second kind)."""
+    from scipy.special import eval_legendre, spherical_jn, spherical_yn
+    ka = k * a
+    if n_terms is None:
+        n_terms = int(ka + 4 * ka ** (1 / 3) + 10)
+    p = np.zeros_like(np.asarray(cos_theta, dtype=float), dtype=complex)
+    for n in range(n_terms + 1):
+        hp = spherical_jn(n, ka, derivative=True) - 1j * spherical_yn(n, ka, derivative=True)
+        p += (2 * n + 1) * (-1j) ** n * (-1j / (ka ** 2 * hp)) * eval_legendre(n, cos_theta)
+    return p
