@@ -45,7 +45,21 @@ def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", api_o, geom_o, "-o", tmp])
     os.replace(tmp, LIB)
+    build_cpp_tests()
     return LIB
+
+
+def build_cpp_tests():
+    """The C++ front-end test program (tests/cpp), linked against libnbem.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_facade.cpp")
+    if not os.path.exists(src):
+        return
+    out_dir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "include"), src, "-o",
+          os.path.join(out_dir, "test_facade"), "-L", HERE, "-lnbem", "-Wl,-rpath," + HERE,
+          "-Wl,-rpath,$ORIGIN/../../../nihu_amd"])
 
 
 if __name__ == "__main__":

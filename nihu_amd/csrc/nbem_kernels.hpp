@@ -185,7 +185,8 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 {
 	const int nq = NQ > 0 ? NQ : nq_rt;
 	const auto qf = uni((const float *)qp_in);
-#pragma unroll (NQ > 0 ? NQ : 1)
+	constexpr int UNROLL = NQ > 0 ? NQ : 1;
+#pragma unroll UNROLL
 	for (int q = 0; q < nq; ++q) {
 		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3], oo = qf[8 * q + 4];
 		const float u = __builtin_fmaf(p.dcx2, ox, __builtin_fmaf(p.dcy2, oy, __builtin_fmaf(p.dcz2, oz, oo)));

@@ -1,0 +1,368 @@
+// nihu.hpp -- C++ front end of the MI355X BEM engine with NiHu's expression API.
+//
+// The names, argument meaning and error behaviour follow NiHu's hot-path API
+// (paths relative to the NiHu repository root):
+//   create_mesh / read_off_mesh          nihu/core/mesh.hpp:137-193, nihu/interface/read_off_mesh.hpp
+//   constant_view, dirac                  nihu/core/function_space.hpp:248-266,353-359
+//   helmholtz_3d_{SLP,DLP,DLPt,HSP}_kernel, laplace_3d_*_kernel
+//                                         nihu/library/helmholtz_kernel.hpp, laplace_kernel.hpp
+//   create_integral_operator, identity_integral_operator, scalar * operator
+//                                         nihu/core/integral_operator.hpp:131-329
+//   op[trial_space]                       nihu/core/integral_operator.hpp:79-89
+//   test_space * transform, wr + wr, result << wr
+//                                         nihu/core/weighted_residual.hpp:155-206
+// `result << wr` accumulates into the caller's matrix, as NiHu's
+// matrix_block::operator+= does.  The result may be any column-major matrix
+// type with data(), rows() and cols() (Eigen::Matrix qualifies) holding
+// std::complex<double>, std::complex<float>, double or float, on the host, or
+// a nihu_amd::device_matrix on the GPU.  Collocation (dirac(constant_view)
+// test, constant_view trial) is implemented; combinations the engine lacks
+// throw std::logic_error.  Errors map to NiHu's exception types:
+// std::out_of_range (quadrature degree), std::invalid_argument,
+// std::logic_error, std::bad_alloc and std::runtime_error.
+
+#ifndef NIHU_AMD_NIHU_HPP_INCLUDED
+#define NIHU_AMD_NIHU_HPP_INCLUDED
+
+#include "../../../include/nbem.h"
+
+#include <complex>
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <new>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+namespace nihu_amd {
+
+inline void check(int status)
+{
+	if (status == NB_OK) return;
+	const std::string msg = nb_last_error();
+	switch (status) {
+	case NB_ERR_RANGE: throw std::out_of_range(msg);
+	case NB_ERR_ARG: throw std::invalid_argument(msg);
+	case NB_ERR_UNSUPPORTED: throw std::logic_error(msg);
+	case NB_ERR_OOM: throw std::bad_alloc();
+	default: throw std::runtime_error(msg);
+	}
+}
+
+// ------------------------------------------------------------------ policy
+
+// Quadrature policy of regular and near-singular pairs.  The default is
+// NiHu's: per-pair Dunavant degree from the accuracy-3 distance tables and
+// the near-singular shortcut below 1.5 mean edge lengths.
+inline nb_policy &default_policy()
+{
+	static nb_policy p = {NB_RULE_NIHU, 5, NB_NEAR_MEAN_EDGE, 1.5};
+	return p;
+}
+inline void set_policy(nb_policy const &p) { default_policy() = p; }
+
+// ------------------------------------------------------------------ mesh
+
+struct tria_1_tag {};
+enum { tria_1_id = 32303 };   // element type id of tria_1_elem (dimension 3, tria domain, 3 nodes)
+
+// A triangle mesh resident on the current GPU.  Elements are numbered by
+// insertion order; shared ownership lets spaces and operators refer to it.
+class mesh {
+public:
+	mesh(std::vector<double> const &xyz, std::vector<int32_t> const &tri)
+		: m_xyz(xyz), m_tri(tri)
+	{
+		if (xyz.size() % 3 || tri.size() % 3) throw std::invalid_argument("node and element arrays must be n x 3");
+		nb_mesh *h = nullptr;
+		check(nb_mesh_create(m_xyz.data(), (int32_t)(m_xyz.size() / 3), m_tri.data(), (int32_t)(m_tri.size() / 3), &h));
+		m_h.reset(h, [](nb_mesh *p) { nb_mesh_destroy(p); });
+	}
+	nb_mesh *handle() const { return m_h.get(); }
+	int32_t num_elements() const { return (int32_t)(m_tri.size() / 3); }
+	int32_t num_nodes() const { return (int32_t)(m_xyz.size() / 3); }
+	std::vector<double> const &nodes() const { return m_xyz; }
+	std::vector<int32_t> const &elements() const { return m_tri; }
+
+private:
+	std::vector<double> m_xyz;
+	std::vector<int32_t> m_tri;
+	std::shared_ptr<nb_mesh> m_h;
+};
+
+// create_mesh(nodes, elements, tria_1_tag): nodes is an (n x 3) and elements
+// an (m x 4) row-major table [type_id, n0, n1, n2] with type_id = 32303, as
+// NiHu's mesh(nodes, elements) constructor reads them.
+inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> const &elements, tria_1_tag)
+{
+	if (elements.size() % 4) throw std::invalid_argument("elements must be m x 4: type id and three node indices");
+	std::vector<int32_t> tri;
+	tri.reserve(elements.size() / 4 * 3);
+	for (size_t e = 0; e < elements.size(); e += 4) {
+		if (elements[e] != (unsigned)tria_1_id) throw std::invalid_argument("only tria_1 elements are supported");
+		for (int a = 1; a <= 3; ++a) tri.push_back((int32_t)elements[e + a]);
+	}
+	return mesh(nodes, tri);
+}
+
+// read_off_mesh(filename, tria_1_tag()): OFF reader with NiHu's error messages
+inline mesh read_off_mesh(std::string const &fname, tria_1_tag)
+{
+	std::ifstream is(fname);
+	std::string header;
+	if (!(is >> header) || header != "OFF") throw std::runtime_error("Possibly invalid off file");
+	unsigned nn, ne, nedge;
+	if (!(is >> nn >> ne >> nedge)) throw std::runtime_error("Error reading number of mesh entries");
+	std::vector<double> xyz(3 * (size_t)nn);
+	for (auto &v : xyz)
+		if (!(is >> v)) throw std::runtime_error("Error reading mesh nodes");
+	std::vector<int32_t> tri;
+	tri.reserve(3 * (size_t)ne);
+	for (unsigned e = 0; e < ne; ++e) {
+		unsigned nv;
+		if (!(is >> nv)) throw std::runtime_error("Error reading mesh elements");
+		if (nv != 3) throw std::runtime_error("Invalid number of vertices for current element tag vector");
+		for (unsigned a = 0; a < 3; ++a) {
+			unsigned v;
+			if (!(is >> v)) throw std::runtime_error("Error reading mesh elements");
+			tri.push_back((int32_t)v);
+		}
+	}
+	return mesh(xyz, tri);
+}
+
+// ------------------------------------------------------------------ spaces
+
+// constant_view: one dof per element (the element id)
+class constant_space {
+public:
+	explicit constant_space(mesh const &m) : m_mesh(m) {}
+	mesh const &get_mesh() const { return m_mesh; }
+	int32_t get_num_dofs() const { return m_mesh.num_elements(); }
+
+private:
+	mesh m_mesh;
+};
+
+// dirac(space): collocation at the element centres of the underlying space
+class dirac_space {
+public:
+	explicit dirac_space(constant_space const &s) : m_space(s) {}
+	mesh const &get_mesh() const { return m_space.get_mesh(); }
+	int32_t get_num_dofs() const { return m_space.get_num_dofs(); }
+
+private:
+	constant_space m_space;
+};
+
+inline constant_space constant_view(mesh const &m) { return constant_space(m); }
+inline dirac_space dirac(constant_space const &s) { return dirac_space(s); }
+
+// ------------------------------------------------------------------ kernels
+
+template <int Family, int Nx, int Ny, class WaveNumber = double>
+struct kernel_t {
+	explicit kernel_t(WaveNumber k = WaveNumber(0)) : m_k(k) {}
+	WaveNumber get_wave_number() const { return m_k; }
+	nb_kernel abi() const { return nb_kernel{Family, Nx, Ny, (double)m_k}; }
+	WaveNumber m_k;
+};
+
+template <class WaveNumber> using helmholtz_3d_SLP_kernel = kernel_t<NB_HELMHOLTZ, 0, 0, WaveNumber>;
+template <class WaveNumber> using helmholtz_3d_DLP_kernel = kernel_t<NB_HELMHOLTZ, 0, 1, WaveNumber>;
+template <class WaveNumber> using helmholtz_3d_DLPt_kernel = kernel_t<NB_HELMHOLTZ, 1, 0, WaveNumber>;
+template <class WaveNumber> using helmholtz_3d_HSP_kernel = kernel_t<NB_HELMHOLTZ, 1, 1, WaveNumber>;
+typedef kernel_t<NB_LAPLACE, 0, 0> laplace_3d_SLP_kernel;
+typedef kernel_t<NB_LAPLACE, 0, 1> laplace_3d_DLP_kernel;
+typedef kernel_t<NB_LAPLACE, 1, 0> laplace_3d_DLPt_kernel;
+typedef kernel_t<NB_LAPLACE, 1, 1> laplace_3d_HSP_kernel;
+
+// ------------------------------------------------------------------ operators
+
+// An integral operator: a kernel (or the identity) and a scalar multiplier
+// applied to every pair result (scaled_integral_operator).
+class integral_operator {
+public:
+	integral_operator(nb_kernel k, std::complex<double> s = 1.0) : m_kernel(k), m_scalar(s) {}
+	nb_kernel const &kernel() const { return m_kernel; }
+	std::complex<double> scalar() const { return m_scalar; }
+	template <class Space> class integral_transform operator[](Space const &trial) const;
+
+private:
+	nb_kernel m_kernel;
+	std::complex<double> m_scalar;
+};
+
+template <int F, int Nx, int Ny, class W>
+integral_operator create_integral_operator(kernel_t<F, Nx, Ny, W> const &k)
+{
+	return integral_operator(k.abi());
+}
+
+inline integral_operator identity_integral_operator()
+{
+	return integral_operator(nb_kernel{NB_IDENTITY, 0, 0, 0.0});
+}
+
+template <class Scalar, class = typename std::enable_if<std::is_arithmetic<Scalar>::value ||
+	std::is_same<Scalar, std::complex<double> >::value || std::is_same<Scalar, std::complex<float> >::value>::type>
+integral_operator operator*(Scalar const &s, integral_operator const &op)
+{
+	return integral_operator(op.kernel(), std::complex<double>(s) * op.scalar());
+}
+
+class integral_transform {
+public:
+	integral_transform(integral_operator const &op, constant_space const &trial) : m_op(op), m_trial(trial) {}
+	integral_operator const &op() const { return m_op; }
+	constant_space const &trial() const { return m_trial; }
+
+private:
+	integral_operator m_op;
+	constant_space m_trial;
+};
+
+template <class Space>
+integral_transform integral_operator::operator[](Space const &trial) const
+{
+	static_assert(std::is_same<Space, constant_space>::value,
+		"the engine implements constant_view trial spaces (collocation)");
+	return integral_transform(*this, trial);
+}
+
+// test * transform: a weighted residual; sums of residuals accumulate in order
+class weighted_residual {
+public:
+	struct term {
+		integral_transform tr;
+		dirac_space test;
+	};
+	weighted_residual(dirac_space const &test, integral_transform const &tr) { m_terms.push_back(term{tr, test}); }
+	std::vector<term> const &terms() const { return m_terms; }
+	weighted_residual operator+(weighted_residual const &o) const
+	{
+		weighted_residual r(*this);
+		r.m_terms.insert(r.m_terms.end(), o.m_terms.begin(), o.m_terms.end());
+		return r;
+	}
+
+private:
+	std::vector<term> m_terms;
+};
+
+inline weighted_residual operator*(dirac_space const &test, integral_transform const &tr)
+{
+	return weighted_residual(test, tr);
+}
+
+template <class Space>
+weighted_residual operator*(Space const &, integral_transform const &)
+{
+	static_assert(std::is_same<Space, dirac_space>::value,
+		"only the collocational formalism (dirac test space) is implemented by this engine");
+	throw std::logic_error("unsupported formalism");
+}
+
+// ------------------------------------------------------------------ results
+
+// A column-major matrix in device memory (complex<float> by default)
+template <class T = std::complex<float> >
+class device_matrix {
+public:
+	device_matrix(int64_t rows, int64_t cols, void *ptr, int64_t ld) : m_rows(rows), m_cols(cols), m_ld(ld), m_ptr(ptr) {}
+	int64_t rows() const { return m_rows; }
+	int64_t cols() const { return m_cols; }
+	int64_t ld() const { return m_ld; }
+	void *data() const { return m_ptr; }
+	typedef T Scalar;
+
+private:
+	int64_t m_rows, m_cols, m_ld;
+	void *m_ptr;
+};
+
+template <class T> struct dtype_of;
+template <> struct dtype_of<std::complex<float> > { enum { value = NB_C64 }; };
+template <> struct dtype_of<std::complex<double> > { enum { value = NB_C128 }; };
+template <> struct dtype_of<float> { enum { value = NB_F32 }; };
+template <> struct dtype_of<double> { enum { value = NB_F64 }; };
+
+template <class Res>
+nb_out describe_host(Res &res)
+{
+	typedef typename std::remove_cv<typename std::remove_reference<decltype(*res.data())>::type>::type T;
+	nb_out o;
+	o.ptr = (void *)res.data();
+	o.rows = res.rows();
+	o.cols = res.cols();
+	o.ld = res.rows();
+	o.dtype = dtype_of<T>::value;
+	o.location = NB_HOST;
+	o.accumulate = 1;
+	return o;
+}
+
+template <class T>
+nb_out describe(device_matrix<T> &res)
+{
+	nb_out o;
+	o.ptr = res.data();
+	o.rows = res.rows();
+	o.cols = res.cols();
+	o.ld = res.ld();
+	o.dtype = dtype_of<T>::value;
+	o.location = NB_DEVICE;
+	o.accumulate = 1;
+	return o;
+}
+
+template <class Res>
+nb_out describe(Res &res) { return describe_host(res); }
+
+// result << wr: evaluate every term, accumulating into result
+template <class Res>
+Res &operator<<(Res &result, weighted_residual const &wr)
+{
+	nb_out o = describe(result);
+	for (auto const &t : wr.terms()) {
+		const nb_mesh *test = t.test.get_mesh().handle();
+		const nb_mesh *trial = t.tr.trial().get_mesh().handle();
+		const int same = test == trial;
+		const std::complex<double> s = t.tr.op().scalar();
+		check(nb_assemble(test, trial, same, NB_COLLOCATION, t.tr.op().kernel(), nb_scalar{s.real(), s.imag()},
+			&default_policy(), &o, nullptr));
+	}
+	if (o.location == NB_DEVICE) check(nb_synchronize(nullptr));
+	return result;
+}
+
+// ------------------------------------------------------------------ a host matrix
+
+// Minimal column-major host matrix for callers without Eigen.
+template <class T>
+class matrix {
+public:
+	matrix(int64_t r = 0, int64_t c = 0) : m_r(r), m_c(c), m_v((size_t)(r * c), T(0)) {}
+	int64_t rows() const { return m_r; }
+	int64_t cols() const { return m_c; }
+	T *data() { return m_v.data(); }
+	T const *data() const { return m_v.data(); }
+	T &operator()(int64_t i, int64_t j) { return m_v[(size_t)(j * m_r + i)]; }
+	T const &operator()(int64_t i, int64_t j) const { return m_v[(size_t)(j * m_r + i)]; }
+	void setZero() { std::fill(m_v.begin(), m_v.end(), T(0)); }
+
+private:
+	int64_t m_r, m_c;
+	std::vector<T> m_v;
+};
+
+typedef matrix<std::complex<double> > cMatrix;
+typedef matrix<double> dMatrix;
+
+} // namespace nihu_amd
+
+#endif

@@ -1,0 +1,214 @@
+// Known-answer tests of the C++ front end, written the way NiHu's own
+// tutorial tests are (nihu/tutorial/helmholtz_3d_transparent_standalone.cpp:17-92,
+// nihu/test/stallone_unit/helmholtz_3d_pulsating_sphere_test.cpp:28-73), on the
+// GPU through the C ABI.
+//
+//   test_facade <mesh.off>
+// Prints the errors and exits non-zero if one is above its bound.
+
+#include "nihu_amd/nihu.hpp"
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+using nihu_amd::cMatrix;
+typedef std::complex<double> cplx;
+
+// dense LU with partial pivoting: solves A x = b in place (A overwritten)
+static std::vector<cplx> solve(cMatrix A, std::vector<cplx> b)
+{
+	const int64_t n = A.rows();
+	for (int64_t k = 0; k < n; ++k) {
+		int64_t p = k;
+		for (int64_t i = k + 1; i < n; ++i)
+			if (std::abs(A(i, k)) > std::abs(A(p, k))) p = i;
+		if (p != k) {
+			for (int64_t j = 0; j < n; ++j) std::swap(A(k, j), A(p, j));
+			std::swap(b[k], b[p]);
+		}
+		for (int64_t i = k + 1; i < n; ++i) {
+			const cplx f = A(i, k) / A(k, k);
+			if (f == cplx(0)) continue;
+			for (int64_t j = k; j < n; ++j) A(i, j) -= f * A(k, j);
+			b[i] -= f * b[k];
+		}
+	}
+	std::vector<cplx> x(n);
+	for (int64_t i = n - 1; i >= 0; --i) {
+		cplx s = b[i];
+		for (int64_t j = i + 1; j < n; ++j) s -= A(i, j) * x[j];
+		x[i] = s / A(i, i);
+	}
+	return x;
+}
+
+struct centre_normal {
+	double c[3], n[3];
+};
+
+static std::vector<centre_normal> geometry(nihu_amd::mesh const &m)
+{
+	std::vector<centre_normal> g(m.num_elements());
+	auto const &X = m.nodes();
+	auto const &T = m.elements();
+	for (int e = 0; e < m.num_elements(); ++e) {
+		const double *a = &X[3 * T[3 * e]], *b = &X[3 * T[3 * e + 1]], *c = &X[3 * T[3 * e + 2]];
+		double u[3], v[3];
+		for (int d = 0; d < 3; ++d) {
+			g[e].c[d] = (a[d] + b[d] + c[d]) / 3.0;
+			u[d] = b[d] - a[d];
+			v[d] = c[d] - a[d];
+		}
+		double nv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+		double l = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+		for (int d = 0; d < 3; ++d) g[e].n[d] = nv[d] / l;
+	}
+	return g;
+}
+
+// transparent test: point sources inside the surface give exact p0, q0;
+// solve (M - I/2) p = L q0 and compare with p0
+template <class TestSpace, class TrialSpace>
+static double tester(TestSpace const &test_space, TrialSpace const &trial_space)
+{
+	const double pi = 3.141592653589793238462643383279502884;
+	double wn = 1.;
+	auto I_op = nihu_amd::identity_integral_operator();
+	auto L_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(wn));
+	auto M_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLP_kernel<double>(wn));
+
+	size_t N = test_space.get_num_dofs();
+	std::vector<cplx> q0(N, 0.0), p0(N, 0.0);
+	const double x0[2][3] = {{.2, .3, .1}, {.3, .2, .2}};
+	const double A[2] = {1.0, -1.0};
+	auto g = geometry(trial_space.get_mesh());
+	for (size_t k = 0; k < N; ++k) {
+		for (int s = 0; s < 2; ++s) {
+			double rv[3], r = 0, rdn = 0;
+			for (int d = 0; d < 3; ++d) { rv[d] = g[k].c[d] - x0[s][d]; r += rv[d] * rv[d]; }
+			r = std::sqrt(r);
+			for (int d = 0; d < 3; ++d) rdn += rv[d] * g[k].n[d] / r;
+			cplx ikr(0., wn * r);
+			p0[k] += A[s] * std::exp(-ikr) / r / (4. * pi);
+			q0[k] += A[s] * -(std::exp(-ikr) / r / r) * rdn / (4. * pi) * (1. + ikr);
+		}
+	}
+
+	cMatrix L(N, N), M(N, N), I(N, N);
+	I << test_space * I_op[trial_space];
+	L << test_space * L_op[trial_space];
+	M << test_space * M_op[trial_space];
+
+	cMatrix S(N, N);
+	std::vector<cplx> rhs(N, 0.0);
+	for (size_t i = 0; i < N; ++i)
+		for (size_t j = 0; j < N; ++j) {
+			S(i, j) = M(i, j) - .5 * I(i, j);
+			rhs[i] += L(i, j) * q0[j];
+		}
+	auto p = solve(S, rhs);
+	double e = 0, nrm = 0;
+	for (size_t k = 0; k < N; ++k) { e += std::norm(p[k] - p0[k]); nrm += std::norm(p0[k]); }
+	return std::sqrt(e / nrm);
+}
+
+// pulsating sphere: p = -r/(1+ikr) q for a unit normal velocity, here collocation
+static double pulsating(nihu_amd::mesh const &m)
+{
+	auto trial_space = nihu_amd::constant_view(m);
+	auto test_space = nihu_amd::dirac(trial_space);
+	const size_t N = trial_space.get_num_dofs();
+	double k = 1.0;
+	cMatrix L(N, N), M(N, N), I(N, N);
+	auto I_op = nihu_amd::identity_integral_operator();
+	auto L_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(k));
+	auto M_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLP_kernel<double>(k));
+	I << test_space * I_op[trial_space];
+	L << test_space * L_op[trial_space];
+	M << test_space * M_op[trial_space];
+	cMatrix S(N, N);
+	std::vector<cplx> rhs(N, 0.0);
+	for (size_t i = 0; i < N; ++i)
+		for (size_t j = 0; j < N; ++j) {
+			S(i, j) = M(i, j) - .5 * I(i, j);
+			rhs[i] += L(i, j);
+		}
+	auto p = solve(S, rhs);
+	const cplx panal = -1.0 / (1.0 + cplx(0, k));
+	double e = 0;
+	for (auto const &v : p) e += std::norm(v - panal);
+	return std::sqrt(e / N) / std::abs(panal);
+}
+
+// Burton-Miller composition as nihu/applications/helmholtz/helmholtz_3d_coll.cpp:63-96
+// writes it, against the fused nb_assemble_bm of the same matrices
+static double burton_miller(nihu_amd::mesh const &m)
+{
+	auto trial_space = nihu_amd::constant_view(m);
+	auto test_space = nihu_amd::dirac(trial_space);
+	const size_t N = trial_space.get_num_dofs();
+	double k = 4.0;
+	std::complex<double> alpha(0., -1. / k);
+	auto Gop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(k));
+	auto Hop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLP_kernel<double>(k));
+	auto Htop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLPt_kernel<double>(k));
+	auto Dop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_HSP_kernel<double>(k));
+	auto Iop = nihu_amd::identity_integral_operator();
+	cMatrix Gs(N, N), Hs(N, N);
+	Gs << test_space * Gop[trial_space];
+	Gs << test_space * (alpha * Htop)[trial_space];
+	Gs << test_space * (alpha / 2. * Iop)[trial_space];
+	Hs << test_space * (alpha * Dop)[trial_space];
+	Hs << test_space * (-.5 * Iop)[trial_space];
+	Hs << test_space * Hop[trial_space];
+	cMatrix A(N, N), B(N, N);
+	nb_out oa = {A.data(), (int64_t)N, (int64_t)N, (int64_t)N, NB_C128, NB_HOST, 0};
+	nb_out ob = {B.data(), (int64_t)N, (int64_t)N, (int64_t)N, NB_C128, NB_HOST, 0};
+	nihu_amd::check(nb_assemble_bm(m.handle(), k, nb_scalar{alpha.real(), alpha.imag()}, &nihu_amd::default_policy(),
+		&oa, &ob, nullptr));
+	double e = 0, nrm = 0;
+	for (size_t i = 0; i < N; ++i)
+		for (size_t j = 0; j < N; ++j) {
+			e += std::norm(A(i, j) - Hs(i, j)) + std::norm(B(i, j) - Gs(i, j));
+			nrm += std::norm(Hs(i, j)) + std::norm(Gs(i, j));
+		}
+	return std::sqrt(e / nrm);
+}
+
+int main(int argc, char *argv[])
+{
+	if (argc < 2) {
+		std::fprintf(stderr, "Usage: %s OFFname\n", argv[0]);
+		return 1;
+	}
+	auto mesh = nihu_amd::read_off_mesh(argv[1], nihu_amd::tria_1_tag());
+	auto const &trial_space = nihu_amd::constant_view(mesh);
+	int bad = 0;
+
+	double coll_error = tester(nihu_amd::dirac(trial_space), trial_space);
+	std::printf("Collocation transparent error: %g\n", coll_error);
+	bad |= !(coll_error < 0.05);
+
+	double puls = pulsating(mesh);
+	std::printf("Pulsating sphere relative error: %g\n", puls);
+	bad |= !(puls < 0.05);
+
+	double bm = burton_miller(mesh);
+	std::printf("Burton-Miller composition vs fused assembly: %g\n", bm);
+	bad |= !(bm < 1e-6);
+
+	// error behaviour: a Dunavant degree above 14 throws std::out_of_range
+	try {
+		nihu_amd::set_policy(nb_policy{NB_RULE_FIXED, 15, NB_NEAR_MEAN_EDGE, 1.5});
+		cMatrix X(trial_space.get_num_dofs(), trial_space.get_num_dofs());
+		X << nihu_amd::dirac(trial_space) * nihu_amd::create_integral_operator(
+			nihu_amd::helmholtz_3d_SLP_kernel<double>(1.0))[trial_space];
+		std::printf("no exception for degree 15\n");
+		bad = 1;
+	} catch (std::out_of_range const &ex) {
+		std::printf("out_of_range as expected: %s\n", ex.what());
+	}
+	return bad;
+}

@@ -1,0 +1,22 @@
+"""The C++ front end (nihu_amd/include/nihu_amd/nihu.hpp) on the GPU: NiHu-style
+known-answer programs built against libnbem.so, on the reference's
+sphere_1_3.off mesh fixture (218 nodes, 432 triangles)."""
+
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_cpp_facade_known_answers():
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
+    assert os.path.exists(exe), "build the C++ tests first (__graft_entry__.build())"
+    mesh = os.path.join(ROOT, "tests", "golden", "sphere_1_3.off")
+    r = subprocess.run([exe, mesh], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "out_of_range as expected" in r.stdout
