@@ -616,6 +616,166 @@ static cplx pair_integral(int family, int nx, int ny, double k, Policy const &p,
 	return regular_integral(family, nx, ny, k, test.c, test.nu, trial, degree);
 }
 
+// ---------------------------------------------------------------------------
+// Galerkin formalism, isoparametric linear triangles (nodal dofs)
+// nihu/core/double_integral.hpp:181-206 (diadic regular rule, N-set weighting),
+// :283-383 (singularity check); nihu/core/element_match.hpp:84-112 and
+// nihu/core/element.hpp:375-413 (node overlap -> face / edge / vertex match);
+// nihu/core/singular_accelerator.hpp:137-359 (canonical rules rotated to the
+// shared corners); nihu/core/singular_galerkin_quadrature.hpp:197-455 (the
+// Duffy-type 4D rules on tria x tria); order 7 (4-point Gauss-Legendre per
+// axis) from singular_quadrature_order (helmholtz_kernel.hpp:69-71).
+// ---------------------------------------------------------------------------
+struct Overlap { int num, ind1, ind2; };
+
+// element::get_overlapping, node by node in the same order
+static Overlap get_overlapping(const int a[3], const int b[3])
+{
+	int num = 0, s1 = 0, s2 = 0;
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+			if (a[i] == b[j]) {
+				if (num == 0) { s1 = i; s2 = j; }
+				else {
+					if (i < s1 || (i == 2 && s1 == 0 && num == 1)) s1 = i;
+					if (j < s2 || (j == 2 && s2 == 0 && num == 1)) s2 = j;
+				}
+				++num;
+			}
+	return {num, s1, s2};
+}
+
+struct SPt { double txi, teta, sxi, seta, w; };
+
+// canonical singular rule of a match dimension (2 face, 1 edge, 0 vertex)
+static std::vector<SPt> make_singular_rule(int match)
+{
+	auto const &g = gauss_line(7);       // 4 points on [-1, 1]
+	std::vector<std::pair<double, double> > b;
+	for (auto const &p : g) b.push_back({(1.0 + p.first) / 2.0, p.second * 0.5});   // to [0, 1]
+	std::vector<SPt> out;
+	const int ndom = match == 2 ? 3 : match == 1 ? 6 : 1;
+	const bool symmetric = match != 1;
+	for (auto const &p1 : b)
+	for (auto const &p2 : b)
+	for (auto const &p3 : b)
+	for (auto const &p4 : b)
+	for (int idx = 0; idx < ndom; ++idx) {
+		double x0 = p1.first, x1 = p2.first, x2 = p3.first, x3 = p4.first;
+		double w = p1.second * p2.second * p3.second * p4.second;
+		double mu1 = 0, mu2 = 0, xi1 = 0, xi2 = 0, J = 0;
+		if (match == 2) {
+			switch (idx) {
+			case 0: mu1 = x0; mu2 = x0 * x1; xi1 = (1 - mu1) * x2; xi2 = xi1 * x3; J = (1 - mu1) * xi1; break;
+			case 1: mu1 = x0 * x1; mu2 = x0 * (x1 - 1.0); xi1 = (1 - mu1 + mu2) * x2 - mu2; xi2 = (xi1 + mu2) * x3 - mu2;
+				J = (1.0 - mu1 + mu2) * (xi1 + mu2); break;
+			default: mu1 = x0 * x1; mu2 = x0; xi1 = (1 - mu2) * x2 + mu2 - mu1; xi2 = (xi1 - mu2 + mu1) * x3;
+				J = (1 - mu2) * (xi1 - mu2 + mu1); break;
+			}
+			w *= J * x0;
+		} else if (match == 1) {
+			switch (idx) {
+			case 0: mu1 = -x0 * x1; mu2 = -x0 * x1 * x2; xi1 = (1.0 - x0) * x3 + x0; xi2 = x0 * (1.0 - x1 + x1 * x2); break;
+			case 1: mu1 = x0 * x1; mu2 = x0 * x1 * x2; xi1 = (1.0 - x0) * x3 + x0 * (1.0 - x1); xi2 = x0 * (1.0 - x1); break;
+			case 2: mu1 = -x0 * x1 * x2; mu2 = x0 * x1 * (1.0 - x2); xi1 = (1.0 - x0) * x3 + x0; xi2 = x0 * (1.0 - x1); break;
+			case 3: mu1 = x0 * x1 * x2; mu2 = x0 * x1 * (x2 - 1.0); xi1 = (1.0 - x0) * x3 + x0 * (1.0 - x1 * x2);
+				xi2 = x0 * (1.0 - x1 * x2); break;
+			case 4: mu1 = -x0 * x1 * x2; mu2 = -x0 * x1; xi1 = (1.0 - x0) * x3 + x0; xi2 = x0; break;
+			default: mu1 = x0 * x1 * x2; mu2 = x0 * x1; xi1 = (1.0 - x0) * x3 + x0 * (1.0 - x1 * x2); xi2 = x0 * (1.0 - x1); break;
+			}
+			w *= x1 * x0 * x0 * (1.0 - x0);
+		}
+		double e1 = mu1 + xi1, e2 = mu2 + xi2;      // eta
+		if (match == 0) {
+			xi1 = x0; xi2 = xi1 * x1; e1 = xi1 * x2; e2 = e1 * x3;
+			w *= x2 * x0 * x0 * x0;
+		}
+		SPt p = {xi1 - xi2, xi2, e1 - e2, e2, w};
+		out.push_back(p);
+		if (symmetric) out.push_back({p.sxi, p.seta, p.txi, p.teta, w});
+	}
+	return out;
+}
+
+static const std::vector<SPt> &singular_rule(int match)
+{
+	static const std::vector<SPt> r0 = make_singular_rule(0), r1 = make_singular_rule(1), r2 = make_singular_rule(2);
+	return match == 0 ? r0 : match == 1 ? r1 : r2;
+}
+
+// point of the reference triangle mapped by the corner permutation c[]
+// (quadrature::transform with isoparam_shape_set and these corners)
+static inline void map_corners(double xi, double eta, const int c[3], double &oxi, double &oeta)
+{
+	static const double cx[3] = {0.0, 1.0, 0.0}, cy[3] = {0.0, 0.0, 1.0};
+	double N0 = 1.0 - xi - eta, N1 = xi, N2 = eta;
+	oxi = N0 * cx[c[0]] + N1 * cx[c[1]] + N2 * cx[c[2]];
+	oeta = N0 * cy[c[0]] + N1 * cy[c[1]] + N2 * cy[c[2]];
+}
+
+// 3x3 Galerkin block of one element pair (not yet scaled); throws like NiHu
+// linear = 1: isoparametric field (3 nodal shape functions, field complexity 1);
+// linear = 0: constant field (one shape function 1, complexity 0,
+// nihu/core/complexity_estimator.hpp:39-75)
+static void galerkin_block(int family, int nx, int ny, double k, Policy const &p, Elem const &te, Elem const &se,
+	const int *tn, const int *sn, bool same_mesh, int linear, cplx blk[3][3])
+{
+	for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) blk[a][b] = 0.0;
+	int match = -1;
+	Overlap ov = {0, 0, 0};
+	if (same_mesh) {
+		if (te.id == se.id) match = 2;
+		else {
+			ov = get_overlapping(tn, sn);
+			if (ov.num > 1) match = 1;
+			else if (ov.num == 1) match = 0;
+		}
+	}
+	if (match >= 0) {
+		// DLP / DLPt over a flat triangle vanish on the face match
+		// (nihu/library/normal_derivative_singular_integrals.hpp:13-58)
+		if (nx + ny == 1 && match == 2) return;
+		// 1/r^3 needs 4 free dimensions (nihu/core/singular_accelerator.hpp:483-503)
+		if (nx == 1 && ny == 1 && match >= 1) throw std::runtime_error("Invalid general singular quadrature");
+		int tc[3] = {0, 1, 2}, scn[3] = {0, 1, 2};
+		if (match < 2) {
+			for (int q = 0; q < 3; ++q) tc[q] = (ov.ind1 + q) % 3;
+			for (int q = 0; q < 3; ++q) scn[q] = match == 1 ? (ov.ind2 + 1 + 3 - q) % 3 : (ov.ind2 + q) % 3;
+		}
+		for (auto const &P : singular_rule(match)) {
+			double txi, teta, sxi, seta;
+			map_corners(P.txi, P.teta, tc, txi, teta);
+			map_corners(P.sxi, P.seta, scn, sxi, seta);
+			V3 x = elem_x(te, txi, teta), y = elem_x(se, sxi, seta);
+			cplx K = kernel_eval(family, nx, ny, k, x, y, te.nu, se.nu);
+			double Nt[3] = {1.0 - txi - teta, txi, teta}, Ns[3] = {1.0 - sxi - seta, sxi, seta};
+			if (!linear) { Nt[0] = 1.0; Ns[0] = 1.0; }
+			for (int a = 0; a < (linear ? 3 : 1); ++a)
+				for (int b = 0; b < (linear ? 3 : 1); ++b)
+					blk[a][b] += (Nt[a] * (te.J * P.w)) * K * (Ns[b] * (se.J * 1.0));
+		}
+		return;
+	}
+	double d2 = dist2(te.c, se.c);
+	int degree = regular_degree(p, linear ? 1 : 0, nx, ny, d2, se);
+	if (degree > 14) throw std::out_of_range("Too high quadrature degree selected for collocational integration");
+	auto const &Q = dunavant_store(degree);
+	for (auto const &qp : Q) {
+		V3 x = elem_x(te, qp.xi, qp.eta);
+		double Nt[3] = {1.0 - qp.xi - qp.eta, qp.xi, qp.eta};
+		if (!linear) Nt[0] = 1.0;
+		for (auto const &qs : Q) {
+			V3 y = elem_x(se, qs.xi, qs.eta);
+			double Ns[3] = {1.0 - qs.xi - qs.eta, qs.xi, qs.eta};
+			if (!linear) Ns[0] = 1.0;
+			cplx K = kernel_eval(family, nx, ny, k, x, y, te.nu, se.nu);
+			for (int a = 0; a < (linear ? 3 : 1); ++a)
+				for (int b = 0; b < (linear ? 3 : 1); ++b)
+					blk[a][b] += (Nt[a] * (te.J * qp.w)) * K * (Ns[b] * (se.J * qs.w));
+		}
+	}
+}
+
 } // namespace orc
 
 // ===========================================================================
@@ -827,6 +987,113 @@ int orc_sweep_bm(const double *xyz, int nn, const int *tri, int nt, const double
 		}
 	}
 	return status;
+}
+
+
+// Galerkin assembly, result(dof_a, dof_b) (+)= alpha * block (complex interleaved,
+// column-major, ld >= test dofs).  linear = 1: isoparametric_view (dofs = mesh
+// nodes, nihu/core/field.hpp:234-258); linear = 0: constant_view (dofs =
+// elements).  family 2 = identity (numerical mass matrix,
+// nihu/core/single_integral.hpp:65-140).
+int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh, int linear,
+	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	ORC_TRY
+	auto test = make_mesh(txyz, tn, ttri, tnt);
+	auto trial = same_mesh ? test : make_mesh(sxyz, sn, stri, snt);
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	cplx alpha(alpha_re, alpha_im);
+	const int nd = linear ? 3 : 1;
+	// dof of (element, local shape function): node id, or the element itself
+	auto tdof = [&](int e, int a) { return linear ? ttri[3 * e + a] : e; };
+	auto sdof = [&](int e, int b) { return linear ? stri[3 * e + b] : e; };
+	int nrows = linear ? tn : tnt;
+	int ncols = linear ? (same_mesh ? tn : sn) : (same_mesh ? tnt : snt);
+	if (!accumulate)
+		for (int j = 0; j < ncols; ++j)
+			for (int i = 0; i < nrows; ++i) { out[2 * ((int64_t)j * ld + i)] = 0; out[2 * ((int64_t)j * ld + i) + 1] = 0; }
+	if (family == 2) {
+		// identity: single integral of N_a N_b J with the Dunavant degree
+		// test order + trial order + jacobian order
+		if (!same_mesh) return 0;
+		auto const &qm = dunavant_store(linear ? 2 : 0);
+		for (int e = 0; e < tnt; ++e) {
+			for (int a = 0; a < nd; ++a)
+				for (int b = 0; b < nd; ++b) {
+					double m = 0.0;
+					for (auto const &q : qm) {
+						double N[3] = {1.0 - q.xi - q.eta, q.xi, q.eta};
+						if (!linear) N[0] = 1.0;
+						m += N[a] * N[b] * test[e].J * q.w;
+					}
+					cplx v = alpha * m;
+					double *o = out + 2 * ((int64_t)tdof(e, b) * ld + tdof(e, a));
+					o[0] += v.real(); o[1] += v.imag();
+				}
+		}
+		return 0;
+	}
+	if (n_threads <= 0) n_threads = orc_num_threads();
+	std::string err;
+	int errcode = 0;
+	#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
+	for (int i = 0; i < tnt; ++i) {
+		try {
+			for (int j = 0; j < snt; ++j) {
+				cplx blk[3][3];
+				galerkin_block(family, nx, ny, k, p, test[i], trial[j], ttri + 3 * i, stri + 3 * j, same_mesh != 0,
+					linear, blk);
+				for (int a = 0; a < nd; ++a)
+					for (int b = 0; b < nd; ++b) {
+						cplx v = alpha * blk[a][b];
+						double *o = out + 2 * ((int64_t)sdof(j, b) * ld + tdof(i, a));
+						#pragma omp atomic
+						o[0] += v.real();
+						#pragma omp atomic
+						o[1] += v.imag();
+					}
+			}
+		} catch (std::out_of_range const &ex) {
+			#pragma omp critical
+			{ err = ex.what(); errcode = 2; }
+		} catch (std::exception const &ex) {
+			#pragma omp critical
+			{ err = ex.what(); errcode = 1; }
+		}
+	}
+	if (errcode == 2) throw std::out_of_range(err);
+	if (errcode == 1) throw std::runtime_error(err);
+	ORC_CATCH
+}
+
+// canonical singular rule of a match dimension: n points (txi, teta, sxi, seta, w)
+int orc_singular_rule(int match, double *pts, int cap, int *n_out)
+{
+	auto const &r = singular_rule(match);
+	if ((int)r.size() > cap) { g_err = "capacity"; return 1; }
+	for (size_t q = 0; q < r.size(); ++q) {
+		pts[5 * q] = r[q].txi; pts[5 * q + 1] = r[q].teta; pts[5 * q + 2] = r[q].sxi;
+		pts[5 * q + 3] = r[q].seta; pts[5 * q + 4] = r[q].w;
+	}
+	*n_out = (int)r.size();
+	return 0;
+}
+
+// one Galerkin block of an element pair given by coordinates and node ids
+int orc_galerkin_block(int family, int nx, int ny, double k, const orc_policy *pol, const double *tc9, const int *tn3,
+	const double *sc9, const int *sn3, int same_mesh, int same_elem, int linear, double *blk18)
+{
+	ORC_TRY
+	Elem te = make_elem({tc9[0], tc9[1], tc9[2]}, {tc9[3], tc9[4], tc9[5]}, {tc9[6], tc9[7], tc9[8]}, 0);
+	Elem se = make_elem({sc9[0], sc9[1], sc9[2]}, {sc9[3], sc9[4], sc9[5]}, {sc9[6], sc9[7], sc9[8]}, same_elem ? 0 : 1);
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	cplx blk[3][3];
+	galerkin_block(family, nx, ny, k, p, te, se, tn3, sn3, same_mesh != 0, linear, blk);
+	for (int a = 0; a < 3; ++a)
+		for (int b = 0; b < 3; ++b) { blk18[2 * (3 * a + b)] = blk[a][b].real(); blk18[2 * (3 * a + b) + 1] = blk[a][b].imag(); }
+	ORC_CATCH
 }
 
 } // extern "C"

@@ -57,6 +57,12 @@ def lib():
         l.orc_assemble_bm.argtypes = [D, C.c_int, I, C.c_int, C.c_double, C.c_double, C.c_double,
                                       P, I, C.c_int, D, D, C.c_int64, C.c_int]
         l.orc_sweep_bm.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, P, I, C.c_int, D, D, C.c_int]
+        l.orc_assemble_galerkin.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int,
+                                            C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            P, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_singular_rule.argtypes = [C.c_int, D, C.c_int, I]
+        l.orc_galerkin_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, P, D, I, D, I, C.c_int, C.c_int,
+                                         C.c_int, D]
         _lib = l
     return _lib
 
@@ -183,6 +189,53 @@ def sweep_bm(nodes, tris, ks, pol=None, rows=None, threads=0):
                               _i(rows_a), nr, A.ctypes.data_as(C.POINTER(C.c_double)),
                               B.ctypes.data_as(C.POINTER(C.c_double)), threads))
     return A.transpose(0, 2, 1), B.transpose(0, 2, 1)
+
+
+def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, trial=None, linear=True,
+                      out=None, accumulate=False, threads=0):
+    """alpha * Galerkin matrix, complex128 (Fortran order): linear=True gives
+    isoparametric linear triangles ([test nodes, trial nodes]), linear=False
+    constant triangles ([test elements, trial elements]); family 2 = identity."""
+    pol = pol or policy()
+    t_nodes, t_tris = _mesh(nodes, tris)
+    same = trial is None
+    s_nodes, s_tris = (t_nodes, t_tris) if same else _mesh(*trial)
+    if out is None:
+        shape = (len(t_nodes), len(s_nodes)) if linear else (len(t_tris), len(s_tris))
+        out = np.zeros(shape, dtype=np.complex128, order="F")
+        accumulate = False
+    alpha = complex(alpha)
+    _check(lib().orc_assemble_galerkin(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                                       _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same), int(linear),
+                                       family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol),
+                                       out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0],
+                                       int(accumulate), threads))
+    return out
+
+
+def singular_rule(match):
+    """Canonical singular Galerkin rule of a match dimension (2 face, 1 edge, 0 vertex):
+    columns (test xi, test eta, trial xi, trial eta, weight)."""
+    cap = 4 ** 4 * 6 * 2
+    pts = np.zeros((cap, 5))
+    n = C.c_int(0)
+    _check(lib().orc_singular_rule(match, _d(pts), cap, C.byref(n)))
+    return pts[:n.value].copy()
+
+
+def galerkin_block(family, nx, ny, k, test_coords, test_nodes, trial_coords, trial_nodes,
+                   same_mesh=True, same_elem=False, pol=None, linear=True):
+    """3x3 Galerkin block (test shape a, trial shape b) of one element pair
+    (linear=False: the constant-field value in [0, 0])."""
+    pol = pol or policy()
+    tc = np.ascontiguousarray(test_coords, dtype=np.float64).reshape(9)
+    sc = np.ascontiguousarray(trial_coords, dtype=np.float64).reshape(9)
+    tn = np.ascontiguousarray(test_nodes, dtype=np.int32)
+    sn = np.ascontiguousarray(trial_nodes, dtype=np.int32)
+    out = np.zeros(18)
+    _check(lib().orc_galerkin_block(family, nx, ny, k, C.byref(pol), _d(tc), _i(tn), _d(sc), _i(sn),
+                                    int(same_mesh), int(same_elem), int(linear), _d(out)))
+    return (out[0::2] + 1j * out[1::2]).reshape(3, 3)
 
 
 def num_threads():

@@ -118,3 +118,64 @@ def dunavant_monomial_exact(a, b):
     """int over the reference triangle of xi^a eta^b = a! b! / (a+b+2)!"""
     from math import factorial
     return factorial(a) * factorial(b) / factorial(a + b + 2)
+
+
+def shape_values(coords, Y):
+    """Linear shape functions (1 - xi - eta, xi, eta) of a flat triangle at
+    points Y lying in its plane (least-squares inverse map)."""
+    X0, X1, X2 = coords
+    M = np.array([X1 - X0, X2 - X0]).T
+    sol, *_ = np.linalg.lstsq(M, (Y - X0).T, rcond=None)
+    xi, eta = sol
+    return np.stack([1 - xi - eta, xi, eta], axis=1)
+
+
+def polar_rule_projected(coords, x, n_theta=24, n_rho=24):
+    """polar_area_rule centred at the orthogonal projection of x onto the
+    triangle's plane (x may lie off the plane)."""
+    X0, X1, X2 = coords
+    nrm = np.cross(X1 - X0, X2 - X0)
+    nrm = nrm / np.linalg.norm(nrm)
+    p = x - np.dot(x - X0, nrm) * nrm
+    return polar_area_rule(coords, p, n_theta, n_rho)
+
+
+def galerkin_pair_brute(family, nx, ny, k, tc, sc, outer_points, outer_weights, n_theta=24, n_rho=24):
+    """3x3 block int_T N_a(x) int_T' K(x, y) N_b(y) dy dx: outer rule given,
+    inner polar rule around the projection of each outer point."""
+    nxv = np.cross(tc[1] - tc[0], tc[2] - tc[0])
+    nxv = nxv / np.linalg.norm(nxv)
+    nyv = np.cross(sc[1] - sc[0], sc[2] - sc[0])
+    nyv = nyv / np.linalg.norm(nyv)
+    Nt = shape_values(tc, outer_points)
+    blk = np.zeros((3, 3), complex)
+    for x, wx, nt in zip(outer_points, outer_weights, Nt):
+        Y, Wy = polar_rule_projected(sc, x, n_theta, n_rho)
+        K = np_kernel(family, nx, ny, k, x, Y, nxv, nyv)
+        blk += wx * np.outer(nt, (K * Wy) @ shape_values(sc, Y))
+    return blk
+
+
+def solid_angle(coords, X):
+    """Signed solid angle of a flat triangle seen from points X (Van Oosterom and
+    Strackee, 1983): int (y - x).n / |y - x|^3 dS_y, n = (X1-X0)x(X2-X0)/|.|."""
+    R = [c[None, :] - X for c in coords]
+    r = [np.linalg.norm(v, axis=1) for v in R]
+    num = np.einsum("ij,ij->i", R[0], np.cross(R[1], R[2]))
+    den = (r[0] * r[1] * r[2] + np.einsum("ij,ij->i", R[0], R[1]) * r[2]
+           + np.einsum("ij,ij->i", R[0], R[2]) * r[1] + np.einsum("ij,ij->i", R[1], R[2]) * r[0])
+    return 2.0 * np.arctan2(num, den)
+
+
+def square_mesh(n):
+    """[-1, 1]^2 in the z = 0 plane, n x n cells, two triangles each."""
+    g = np.linspace(-1.0, 1.0, n + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a = i * (n + 1) + j
+            b, d = a + n + 1, a + 1
+            tris += [[a, b, b + 1], [a, b + 1, d]]
+    return nodes, np.array(tris, dtype=np.int32)
