@@ -5,7 +5,8 @@
  *     result << test_space * op[trial_space]
  * for 3D Laplace / Helmholtz SLP, DLP, DLPt and HSP kernels on flat 3-node
  * triangles: collocation (dirac(constant_view) x constant_view) and Galerkin
- * (isoparametric_view x isoparametric_view).  Every entry point returns an
+ * (constant_view x constant_view, isoparametric_view x isoparametric_view).
+ * Every entry point returns an
  * nb_status; on failure nb_last_error() returns a message for the calling
  * thread.  Plain pointers and sizes only.
  *
@@ -60,7 +61,17 @@ enum nb_near {
 
 enum nb_dtype { NB_C64 = 0, NB_C128 = 1, NB_F32 = 2, NB_F64 = 3 };
 enum nb_location { NB_HOST = 0, NB_DEVICE = 1 };
-enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1 };
+/* test x trial spaces (nihu/core/formalism.hpp:58-63, function_space.hpp):
+ *   NB_COLLOCATION      dirac(constant_view(mesh)) x constant_view(mesh):
+ *                       rows and columns are elements
+ *   NB_GALERKIN         constant_view x constant_view: rows / columns are elements
+ *   NB_GALERKIN_LINEAR  isoparametric_view x isoparametric_view: rows / columns
+ *                       are mesh nodes (nihu/core/field.hpp:234-258)
+ * Galerkin pairs that share a node on the same mesh use the Duffy-type rules of
+ * nihu/core/singular_galerkin_quadrature.hpp:197-455; the hypersingular kernel
+ * has none for coincident and edge-adjacent pairs and fails with
+ * NB_ERR_RUNTIME "Invalid general singular quadrature", as NiHu throws. */
+enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1, NB_GALERKIN_LINEAR = 2 };
 
 typedef struct nb_mesh nb_mesh;
 

@@ -2,8 +2,8 @@
 
     python -m nihu_amd.build        # or __graft_entry__.build()
 
-The host geometry file is compiled with g++ -ffp-contract=off (pair
-classification must round exactly as written); the HIP translation unit with
+The host geometry and layout files are compiled with g++ -ffp-contract=off
+(pair classification must round exactly as written); the HIP translation unit with
 hipcc for gfx950 only.  Packed FP32 instructions are disabled for all device
 code (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -37,13 +37,16 @@ def _run(cmd):
 
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    geom_o = os.path.join(BUILD, "nbem_geom.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
-          "-c", os.path.join(CSRC, "nbem_geom.cpp"), "-o", geom_o])
+    host_o = []
+    for name in ("nbem_geom", "nbem_layout"):
+        o = os.path.join(BUILD, name + ".o")
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+              "-c", os.path.join(CSRC, name + ".cpp"), "-o", o])
+        host_o.append(o)
     api_o = os.path.join(BUILD, "nbem_api.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "nbem_api.hip"), "-o", api_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", api_o, geom_o, "-o", tmp])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", api_o, *host_o, "-o", tmp])
     os.replace(tmp, LIB)
     build_cpp_tests()
     return LIB

@@ -1,16 +1,26 @@
 // nbem_api.hip -- C ABI of the engine (include/nbem.h): mesh residency,
 // per-mesh caches (classification thresholds, quadrature tables, near-pair
-// lists) and the launch sequence of one assembly:
+// lists, Galerkin layouts and singular pair lists) and the launch sequence of
+// one assembly.
+// Collocation:
 //   1. regular pairs, FP32 (regular_kernel)
 //   2. near-singular pairs, FP64 (fp64_pass_kernel<..., false>)
 //   3. diagonal pairs on the same mesh, FP64 (fp64_pass_kernel<..., true>)
 // The three passes write disjoint entries, so no atomics are used and every
 // entry is computed by one lane in a fixed order (results do not depend on
 // what else runs on the device).
+// Galerkin:
+//   1. regular pairs, FP32 (galerkin_regular_kernel)
+//   2. coincident / edge / vertex pairs on the same mesh, FP64
+//      (galerkin_singular_kernel)
+// With nodal (isoparametric) dofs an entry sums the blocks of every element
+// pair around its two nodes; those sums use atomics, so their rounding order
+// (not their value beyond FP32 / FP64 rounding) depends on scheduling.
 
 #include "../../include/nbem.h"
 #include "nbem_kernels.hpp"
 #include "nbem_sweep.hpp"
+#include "nbem_galerkin.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -33,6 +43,10 @@ int nbg_geometry(const double *xyz, int n_nodes, const int32_t *tri, int n, doub
 	double *J, double *R, double *diam, double *X);
 double nbg_threshold(double scale, double limit);
 int nbg_quad_table(int degree, int n, const double *c, const double *J, const double *X, float *out);
+int nbg_singular_rule(int match, double *out, int cap);
+int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, int32_t *pf, int32_t *info);
+int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk, int32_t *order_out, int32_t *elem,
+	int32_t *loc, int32_t *rows, int32_t *col, int32_t *fin);
 }
 
 namespace {
@@ -191,6 +205,37 @@ int device_rules(const nbk::Rules **out)
 	return NB_OK;
 }
 
+// per-device canonical singular rules of the Galerkin pass (vertex, edge, face)
+struct SingRules {
+	DevPtr r[3];
+	int n[3];
+};
+std::mutex g_srules_mu;
+std::map<int, std::unique_ptr<SingRules> > g_srules;
+
+int device_sing_rules(const SingRules **out)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_srules_mu);
+	auto it = g_srules.find(dev);
+	if (it == g_srules.end()) {
+		std::unique_ptr<SingRules> sr(new SingRules());
+		std::vector<double> buf(5 * 1536);
+		for (int m = 0; m < 3; ++m) {
+			const int n = nbg_singular_rule(m, buf.data(), 1536);
+			if (n <= 0) return fail(NB_ERR_RUNTIME, "singular Galerkin rule");
+			std::vector<double> v(buf.begin(), buf.begin() + 5 * n);
+			int s = upload(sr->r[m], v);
+			if (s) return s;
+			sr->n[m] = n;
+		}
+		it = g_srules.emplace(dev, std::move(sr)).first;
+	}
+	*out = it->second.get();
+	return NB_OK;
+}
+
 } // namespace
 
 struct nb_mesh {
@@ -205,6 +250,14 @@ struct nb_mesh {
 	DevPtr d_qt[nbk::NSLOT];
 	int qn[nbk::NSLOT] = {0};
 	std::vector<std::unique_ptr<NearList> > near_cache;
+	std::vector<int32_t> tri;                  // element-major node ids (host)
+	DevPtr d_tri;                              // node ids [3][n]
+	// Galerkin, nodal dofs: element order, wave tiles, column-slot schedule
+	int g_tiles = 0, g_chunk = 0;
+	DevPtr g_order, g_elem, g_loc, g_rows, g_col, g_fin;
+	// Galerkin singular pairs against itself (coincident pairs first)
+	int64_t sp_n = -1;
+	DevPtr sp_e, sp_f, sp_info;
 	std::mutex mu;
 	int64_t bytes = 0;
 };
@@ -318,6 +371,54 @@ int ensure_near_list(nb_mesh *trial, const nb_mesh *test, int same, const nb_pol
 	return NB_OK;
 }
 
+// trial-sequence chunk per work-group: as long as possible (the slot
+// accumulators flush at every chunk end) while the grid keeps >= 2048
+// work-groups for the 256 CUs
+int gal_chunk(int n_tiles, int sn)
+{
+	int ch = 512;
+	const long long wg = (n_tiles + 3) / 4;
+	while (ch > 64 && wg * ((sn + ch - 1) / ch) < 2048) ch >>= 1;
+	return ch;
+}
+
+int ensure_gal_layout(nb_mesh *m)
+{
+	if (m->g_tiles > 0) return NB_OK;
+	const int n = m->n;
+	const int tiles = nbg_galerkin_layout(n, m->tri.data(), m->c.data(), 512, nullptr, nullptr, nullptr, nullptr,
+		nullptr, nullptr);
+	const int chunk = gal_chunk(tiles, n);
+	const size_t nl = (size_t)tiles * nbk::GAL_TILE;
+	std::vector<int32_t> order(n), elem(nl), loc(nl), rows(nl), col(4 * (size_t)n),
+		fin((size_t)((n + chunk - 1) / chunk) * nbk::GAL_CS);
+	if (nbg_galerkin_layout(n, m->tri.data(), m->c.data(), chunk, order.data(), elem.data(), loc.data(), rows.data(),
+		col.data(), fin.data()) != tiles)
+		return fail(NB_ERR_RUNTIME, "Galerkin layout");
+	int s;
+	if ((s = upload(m->g_order, order)) || (s = upload(m->g_elem, elem)) || (s = upload(m->g_loc, loc)) ||
+		(s = upload(m->g_rows, rows)) || (s = upload(m->g_col, col)) || (s = upload(m->g_fin, fin)))
+		return s;
+	m->g_tiles = tiles;
+	m->g_chunk = chunk;
+	m->bytes += (int64_t)(order.size() + 3 * nl + col.size() + fin.size()) * 4;
+	return NB_OK;
+}
+
+int ensure_sing_pairs(nb_mesh *m)
+{
+	if (m->sp_n >= 0) return NB_OK;
+	const int64_t cnt = nbg_singular_pairs(m->n_nodes, m->n, m->tri.data(), nullptr, nullptr, nullptr);
+	if (cnt > 0x7fffffff) return fail(NB_ERR_RANGE, "too many singular element pairs");
+	std::vector<int32_t> pe(cnt), pf(cnt), info(cnt);
+	nbg_singular_pairs(m->n_nodes, m->n, m->tri.data(), pe.data(), pf.data(), info.data());
+	int s;
+	if ((s = upload(m->sp_e, pe)) || (s = upload(m->sp_f, pf)) || (s = upload(m->sp_info, info))) return s;
+	m->sp_n = cnt;
+	m->bytes += cnt * 12;
+	return NB_OK;
+}
+
 int dtype_bytes(int dt)
 {
 	switch (dt) {
@@ -346,6 +447,7 @@ int check_out(const nb_out *o, int64_t rows, int64_t cols, bool complex_needed)
 struct Job {
 	const nb_mesh *test;
 	nb_mesh *trial;
+	int formalism;
 	int same, family, km, mode;
 	double k;
 	nb_scalar alpha;
@@ -353,9 +455,165 @@ struct Job {
 	int tile_cols;
 };
 
+// Galerkin assembly into a device buffer
+int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	nb_mesh *trial = jb.trial;
+	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
+	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
+	const int64_t rows = nodal ? test->n_nodes : test->n, cols = nodal ? trial->n_nodes : trial->n;
+	const int eb = dtype_bytes(dtype);
+	auto zero = [&]() -> int {
+		const long long words = rows * eb / 4 * cols;
+		if (words > 0)
+			hipLaunchKernelGGL(nbk::zero_kernel, dim3((words + 255) / 256), dim3(256), 0, st, out, (long long)ld,
+				(long long)rows, (long long)cols, eb);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	};
+	int s;
+	if (jb.family == NB_IDENTITY) {
+		if (!accumulate && (s = zero())) return s;
+		// local operator: nothing on different meshes (nihu/core/assembly.hpp:141-142)
+		if (jb.same)
+			hipLaunchKernelGGL(nbk::galerkin_identity_kernel, dim3((test->n + 255) / 256), dim3(256), 0, st, out,
+				(long long)ld, test->n, (const double *)test->d_J.get(), (const int *)test->d_tri.get(), nodal ? 1 : 0,
+				dtype, jb.alpha.re, jb.alpha.im);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	}
+	// 1/r^3 has no valid singular rule on coincident and edge-adjacent pairs
+	// (nihu/core/singular_accelerator.hpp:483-503, double_integral.hpp:255-273)
+	if (jb.same && jb.km == nbk::HSP) return fail(NB_ERR_RUNTIME, "Invalid general singular quadrature");
+	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
+	std::vector<int> degs;
+	policy_degrees(jb.pol, jb.km, nodal ? 1 : 0, degs);
+	for (int d : degs)
+		if ((s = ensure_quad(trial, d)) || (s = ensure_quad(test, d))) return s;
+	if (nodal && ((s = ensure_gal_layout(test)) || (s = ensure_gal_layout(trial)))) return s;
+	if (jb.same && (s = ensure_sing_pairs(trial))) return s;
+	if (nodal && !accumulate && (s = zero())) return s;
+
+	nbk::GalArgs a;
+	std::memset(&a, 0, sizeof(a));
+	if (nodal) {
+		a.n_tiles = test->g_tiles;
+		a.chunk = trial->g_chunk;
+		a.t_elem = (const int *)test->g_elem.get();
+		a.t_loc = (const int *)test->g_loc.get();
+		a.t_rows = (const int *)test->g_rows.get();
+		a.s_order = (const int *)trial->g_order.get();
+		a.s_col = (const int *)trial->g_col.get();
+		a.s_fin = (const int *)trial->g_fin.get();
+	} else {
+		a.n_tiles = (test->n + nbk::GAL_TILE - 1) / nbk::GAL_TILE;
+		a.chunk = gal_chunk(a.n_tiles, trial->n);
+	}
+	a.tn = test->n; a.sn = trial->n; a.same_mesh = jb.same;
+	a.t_tri = (const int *)test->d_tri.get();
+	a.s_tri = (const int *)trial->d_tri.get();
+	a.tc = (const double *)test->d_c.get();
+	a.sc = (const double *)trial->d_c.get();
+	a.tnu = (const float *)test->d_nuf.get();
+	a.snu = (const float *)trial->d_nuf.get();
+	a.thr = (const double *)trial->d_thr.get();
+	for (int d = 0; d < nbk::NSLOT; ++d) {
+		a.tq[d] = (const nbk::QPoint *)test->d_qt[d].get();
+		a.sq[d] = (const nbk::QPoint *)trial->d_qt[d].get();
+		a.qn[d] = trial->qn[d];
+	}
+	a.fixed_degree = jb.pol.rule == NB_RULE_FIXED ? jb.pol.fixed_degree : -1;
+	const double two_pi = 6.283185307179586476925286766559005768;
+	a.k = (float)jb.k;
+	a.k_rev = jb.k / two_pi;
+	a.k_revf = (float)a.k_rev;
+	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
+	a.out = out; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
+	const long long grid = (long long)((a.n_tiles + 3) / 4) * ((trial->n + a.chunk - 1) / a.chunk);
+	const bool fix6 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 6;
+	{
+		using namespace nbk;
+		const int pe = prof_begin(0, st);
+#define NB_GAL(F, K) do { \
+		if (nodal) { \
+			if (F == HELM && fix6) hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 3, (F == HELM ? 6 : 0)>), dim3(grid), dim3(256), 0, st, a); \
+			else hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 3, 0>), dim3(grid), dim3(256), 0, st, a); \
+		} else { \
+			if (F == HELM && fix6) hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 1, (F == HELM ? 6 : 0)>), dim3(grid), dim3(256), 0, st, a); \
+			else hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 1, 0>), dim3(grid), dim3(256), 0, st, a); \
+		} } while (0)
+		if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_GAL(HELM, SLP); break;
+			case DLP: NB_GAL(HELM, DLP); break;
+			case DLPT: NB_GAL(HELM, DLPT); break;
+			default: NB_GAL(HELM, HSP); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_GAL(LAP, SLP); break;
+			case DLP: NB_GAL(LAP, DLP); break;
+			case DLPT: NB_GAL(LAP, DLPT); break;
+			default: NB_GAL(LAP, HSP); break;
+			}
+		}
+#undef NB_GAL
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	if (!jb.same) return NB_OK;
+
+	// singular pairs in FP64; the double layer of a coincident flat pair is 0
+	// (nihu/library/normal_derivative_singular_integrals.hpp:13-58)
+	const SingRules *sr = nullptr;
+	if ((s = device_sing_rules(&sr))) return s;
+	nbk::GalSingArgs g;
+	std::memset(&g, 0, sizeof(g));
+	const int64_t first = (jb.km == nbk::DLP || jb.km == nbk::DLPT) ? trial->n : 0;
+	g.n_pairs = (int)(trial->sp_n - first);
+	if (g.n_pairs <= 0) return NB_OK;
+	g.pe = (const int *)trial->sp_e.get() + first;
+	g.pf = (const int *)trial->sp_f.get() + first;
+	g.pinfo = (const int *)trial->sp_info.get() + first;
+	g.n = trial->n;
+	g.X = (const double *)trial->d_X.get();
+	g.J = (const double *)trial->d_J.get();
+	g.nu = (const double *)trial->d_nu64.get();
+	g.tri = (const int *)trial->d_tri.get();
+	for (int m = 0; m < 3; ++m) { g.rule[m] = (const double *)sr->r[m].get(); g.rule_n[m] = sr->n[m]; }
+	g.k = jb.k;
+	g.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+	g.out = out; g.ld = ld; g.dtype = dtype;
+	{
+		using namespace nbk;
+		const int pe = prof_begin(2, st);
+#define NB_GSING(F, K) do { \
+		if (nodal) hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 3>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+		else hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 1>), dim3(g.n_pairs), dim3(64), 0, st, g); } while (0)
+		if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_GSING(HELM, SLP); break;
+			case DLP: NB_GSING(HELM, DLP); break;
+			default: NB_GSING(HELM, DLPT); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_GSING(LAP, SLP); break;
+			case DLP: NB_GSING(LAP, DLP); break;
+			default: NB_GSING(LAP, DLPT); break;
+			}
+		}
+#undef NB_GSING
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	return NB_OK;
+}
+
 // one assembly into device buffers (dtype, ld, accumulate taken from outs)
 int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
+	if (jb.formalism != NB_COLLOCATION) return run_galerkin(jb, out0, ld, dtype, accumulate, st);
 	nb_mesh *trial = jb.trial;
 	const nb_mesh *test = jb.test;
 	int s;
@@ -600,7 +858,8 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 		// other policies: one Burton-Miller assembly per wavenumber
 		for (int f = 0; f < n_k; ++f) {
 			Job jb;
-			jb.test = m; jb.trial = m; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
+			jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION; jb.same = 1; jb.family = NB_HELMHOLTZ;
+			jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
 			jb.k = k[f]; jb.alpha = {0.0, -1.0 / k[f]}; jb.pol = *policy; jb.tile_cols = 32;
 			if ((s = run_device(jb, (float2 *)A + f * k_stride, (float2 *)B + f * k_stride, ld, NB_C64, 0, st))) return s;
 		}
@@ -728,6 +987,44 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 
 } // namespace
 
+namespace {
+
+int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, int formalism, nb_kernel kernel,
+	nb_scalar alpha, const nb_policy *policy, const nb_out *out, hipStream_t st)
+{
+	const bool nodal = formalism == NB_GALERKIN_LINEAR;
+	const bool ident = kernel.family == NB_IDENTITY;
+	int s;
+	if (!ident) {
+		if (kernel.family != NB_LAPLACE && kernel.family != NB_HELMHOLTZ) return fail(NB_ERR_ARG, "unknown kernel family");
+		if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
+			return fail(NB_ERR_UNSUPPORTED, "normal derivative orders must be 0 or 1");
+		if (kernel.family == NB_HELMHOLTZ && !(kernel.k > 0.0)) return fail(NB_ERR_ARG, "wavenumber must be positive");
+		if ((s = check_policy(policy))) return s;
+	}
+	const bool cplx = (!ident && kernel.family == NB_HELMHOLTZ) || alpha.im != 0.0;
+	if ((s = check_out(out, nodal ? test->n_nodes : test->n, nodal ? trial->n_nodes : trial->n, cplx))) return s;
+	nb_mesh *tr = const_cast<nb_mesh *>(trial);
+	nb_mesh *te = const_cast<nb_mesh *>(test);
+	// the test mesh's tables and layout are built on first use too
+	std::unique_lock<std::mutex> l1(tr->mu, std::defer_lock), l2(te->mu, std::defer_lock);
+	if (te == tr) l1.lock();
+	else std::lock(l1, l2);
+	Job jb;
+	jb.test = test; jb.trial = tr; jb.formalism = formalism; jb.same = same_mesh ? 1 : 0;
+	jb.family = kernel.family;
+	jb.km = ident ? 0 : kernel_mask(kernel.nx, kernel.ny);
+	jb.mode = nbk::OUT_SINGLE;
+	jb.k = kernel.family == NB_HELMHOLTZ ? kernel.k : 0.0;
+	jb.alpha = alpha;
+	if (policy) jb.pol = *policy;
+	else jb.pol = nb_policy{NB_RULE_NIHU, 0, NB_NEAR_MEAN_EDGE, 1.5};
+	jb.tile_cols = 0;
+	return run_job(jb, out, nullptr, st);
+}
+
+} // namespace
+
 extern "C" {
 
 const char *nb_last_error(void) { return g_err.c_str(); }
@@ -767,12 +1064,16 @@ int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32
 	if (g == 1) return fail(NB_ERR_ARG, "node index out of range");
 	if (g == 2) return fail(NB_ERR_ARG, "degenerate element (zero area)");
 	NB_HIP(hipGetDevice(&m->device));
+	m->tri.assign(tri, tri + 3 * (size_t)n);
+	std::vector<int32_t> tri_soa(3 * (size_t)n);
+	for (int e = 0; e < n; ++e)
+		for (int a = 0; a < 3; ++a) tri_soa[(size_t)a * n + e] = tri[3 * (size_t)e + a];
 	std::vector<float> nuf(m->nu.begin(), m->nu.end());
 	int s;
 	if ((s = upload(m->d_c, m->c)) || (s = upload(m->d_nu64, m->nu)) || (s = upload(m->d_nuf, nuf)) ||
-		(s = upload(m->d_X, m->X)) || (s = upload(m->d_J, m->J)))
+		(s = upload(m->d_X, m->X)) || (s = upload(m->d_J, m->J)) || (s = upload(m->d_tri, tri_soa)))
 		return s;
-	m->bytes = (int64_t)n * (3 * 8 + 3 * 8 + 3 * 4 + 9 * 8 + 8);
+	m->bytes = (int64_t)n * (3 * 8 + 3 * 8 + 3 * 4 + 9 * 8 + 8 + 3 * 4);
 	*out = m.release();
 	return NB_OK;
 }
@@ -797,11 +1098,12 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 {
 	if (!test || !trial) return fail(NB_ERR_ARG, "null mesh");
 	if (same_mesh && test != trial) return fail(NB_ERR_ARG, "same_mesh requires test == trial");
-	if (formalism != NB_COLLOCATION)
-		return fail(NB_ERR_UNSUPPORTED, "only the collocational formalism is implemented");
+	if (formalism != NB_COLLOCATION && formalism != NB_GALERKIN && formalism != NB_GALERKIN_LINEAR)
+		return fail(NB_ERR_ARG, "unknown formalism");
 	int s;
 	if ((s = current_device_matches(test)) || (s = current_device_matches(trial))) return s;
 	hipStream_t st = (hipStream_t)stream;
+	if (formalism != NB_COLLOCATION) return assemble_galerkin(test, trial, same_mesh, formalism, kernel, alpha, policy, out, st);
 	if (kernel.family == NB_IDENTITY) {
 		if ((s = check_out(out, test->n, trial->n, alpha.im != 0.0))) return s;
 		// local operator: nothing on different meshes (nihu/core/assembly.hpp:141-142)
@@ -842,7 +1144,7 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 	nb_mesh *tr = const_cast<nb_mesh *>(trial);
 	std::lock_guard<std::mutex> lk(tr->mu);
 	Job jb;
-	jb.test = test; jb.trial = tr; jb.same = same_mesh ? 1 : 0; jb.family = kernel.family;
+	jb.test = test; jb.trial = tr; jb.formalism = NB_COLLOCATION; jb.same = same_mesh ? 1 : 0; jb.family = kernel.family;
 	jb.km = kernel_mask(kernel.nx, kernel.ny); jb.mode = nbk::OUT_SINGLE;
 	jb.k = kernel.family == NB_HELMHOLTZ ? kernel.k : 0.0;
 	jb.alpha = alpha; jb.pol = *policy;
@@ -865,7 +1167,8 @@ int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_poli
 	nb_mesh *m = const_cast<nb_mesh *>(mesh);
 	std::lock_guard<std::mutex> lk(m->mu);
 	Job jb;
-	jb.test = m; jb.trial = m; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
+	jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4;
+	jb.mode = nbk::OUT_BM;
 	jb.k = k; jb.alpha = alpha; jb.pol = *policy;
 	{
 		std::lock_guard<std::mutex> lc(g_cfg_mu);

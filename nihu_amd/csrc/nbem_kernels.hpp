@@ -33,7 +33,8 @@ enum { THR_NEAR = 0, THR_1_0, THR_1_1, THR_1_4, THR_1_5, THR_1_8, THR_1_9, THR_2
 
 constexpr int NSLOT = 16;   // quadrature tables indexed by Dunavant degree 0..14 (+1 spare)
 
-struct QPoint { float ox, oy, oz, w, oo, pad0, pad1, pad2; };
+// offset from the element centre, w J / (4 pi), |offset|^2, reference point, w J
+struct QPoint { float ox, oy, oz, w, oo, xi, eta, wj; };
 
 struct RegArgs {
 	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;

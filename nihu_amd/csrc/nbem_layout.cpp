@@ -1,0 +1,299 @@
+// nbem_layout.cpp -- host-side set-up of the Galerkin passes (FP64, g++).
+//
+// * canonical singular rules of coincident / edge / vertex triangle pairs
+//   (nihu/core/singular_galerkin_quadrature.hpp:197-455), the 4-point
+//   Gauss-Legendre rule per axis of singular_quadrature_order 7
+//   (nihu/library/helmholtz_kernel.hpp:69-71, laplace_kernel.hpp:68-71);
+// * the node-overlap scan that decides the match type and the start corners
+//   (nihu/core/element.hpp:375-413, element_match.hpp:84-112);
+// * the list of singular element pairs of a mesh;
+// * the element order, wave tiles and column-slot schedule of the regular
+//   Galerkin kernel for nodal (isoparametric) dofs.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+extern "C" void nbg_gauss_legendre(int n, double *x, double *w);
+
+namespace {
+
+constexpr int TILE = 64;   // test elements per wave
+constexpr int RMAX = 64;   // distinct nodes per wave tile
+constexpr int CS = 16;     // column-node slots per wave
+
+// Morton key of a point quantised on a 1024^3 grid
+uint64_t spread3(uint64_t v)
+{
+	v &= 0x3ff;
+	v = (v | (v << 16)) & 0x030000ffull;
+	v = (v | (v << 8)) & 0x0300f00full;
+	v = (v | (v << 4)) & 0x030c30c3ull;
+	v = (v | (v << 2)) & 0x09249249ull;
+	return v;
+}
+
+struct Tiles {
+	std::vector<int> elem, loc, rows;   // [n_tiles * TILE]
+	int n_tiles = 0;
+};
+
+// greedy wave tiles over an element order: up to TILE elements whose corners
+// use at most RMAX distinct nodes
+Tiles make_tiles(const std::vector<int> &order, const int32_t *tri)
+{
+	Tiles t;
+	const int n = (int)order.size();
+	int pos = 0;
+	std::vector<int> rows;
+	while (pos < n) {
+		rows.clear();
+		int cnt = 0;
+		std::vector<int> locs;
+		while (pos + cnt < n && cnt < TILE) {
+			const int e = order[pos + cnt];
+			int l3[3];
+			std::vector<int> add;
+			for (int a = 0; a < 3; ++a) {
+				const int v = tri[3 * e + a];
+				auto it = std::find(rows.begin(), rows.end(), v);
+				if (it != rows.end()) { l3[a] = (int)(it - rows.begin()); continue; }
+				auto jt = std::find(add.begin(), add.end(), v);
+				if (jt != add.end()) { l3[a] = (int)rows.size() + (int)(jt - add.begin()); continue; }
+				l3[a] = (int)(rows.size() + add.size());
+				add.push_back(v);
+			}
+			if ((int)(rows.size() + add.size()) > RMAX) break;
+			rows.insert(rows.end(), add.begin(), add.end());
+			locs.push_back(l3[0] | (l3[1] << 8) | (l3[2] << 16));
+			++cnt;
+		}
+		for (int l = 0; l < TILE; ++l) {
+			t.elem.push_back(l < cnt ? order[pos + l] : -1);
+			t.loc.push_back(l < cnt ? locs[l] : 0);
+			t.rows.push_back(l < (int)rows.size() ? rows[l] : -1);
+		}
+		++t.n_tiles;
+		pos += cnt;
+	}
+	return t;
+}
+
+// least-recently-used assignment of the column nodes of a trial sequence to
+// CS slots, restarted at every chunk; returns the number of misses
+int64_t schedule(const std::vector<int> &order, const int32_t *tri, int chunk, int *col, int *fin)
+{
+	const int n = (int)order.size();
+	int64_t misses = 0;
+	int node[CS], last[CS];
+	for (int c0 = 0, ch = 0; c0 < n; c0 += chunk, ++ch) {
+		for (int s = 0; s < CS; ++s) { node[s] = -1; last[s] = -1; }
+		const int c1 = std::min(n, c0 + chunk);
+		for (int i = c0; i < c1; ++i) {
+			const int e = order[i];
+			int slot[3] = {-1, -1, -1}, victim[3] = {-1, -1, -1};
+			for (int b = 0; b < 3; ++b)
+				for (int s = 0; s < CS; ++s)
+					if (node[s] == tri[3 * e + b]) { slot[b] = s; last[s] = i; }
+			for (int b = 0; b < 3; ++b) {
+				if (slot[b] >= 0) continue;
+				int best = -1;
+				for (int s = 0; s < CS; ++s) {
+					if (last[s] == i) continue;        // in use by this element
+					if (best < 0 || last[s] < last[best]) best = s;
+				}
+				slot[b] = best;
+				victim[b] = node[best];
+				node[best] = tri[3 * e + b];
+				last[best] = i;
+				++misses;
+			}
+			if (col) {
+				col[4 * i] = victim[0]; col[4 * i + 1] = victim[1]; col[4 * i + 2] = victim[2];
+				col[4 * i + 3] = slot[0] | (slot[1] << 5) | (slot[2] << 10);
+			}
+		}
+		if (fin)
+			for (int s = 0; s < CS; ++s) fin[ch * CS + s] = node[s];
+	}
+	return misses;
+}
+
+} // namespace
+
+extern "C" {
+
+// Canonical singular rule of a match dimension (2 coincident, 1 common edge,
+// 0 common vertex) on the reference triangle pair: rows (test xi, test eta,
+// trial xi, trial eta, weight); returns the point count (1536, 1536, 512) or
+// -1 if cap is too small.  The Duffy-type maps and their Jacobians are those
+// of the cited helper (Taylor's decomposition of the 4D simplex product).
+int nbg_singular_rule(int match, double *out, int cap)
+{
+	double gx[4], gw[4];
+	nbg_gauss_legendre(4, gx, gw);
+	double x[4], w[4];
+	for (int q = 0; q < 4; ++q) { x[q] = (1.0 + gx[q]) / 2.0; w[q] = gw[q] * 0.5; }   // on [0, 1]
+	const int ndom = match == 2 ? 3 : match == 1 ? 6 : 1;
+	const bool symmetric = match != 1;
+	int n = 0;
+	for (int i1 = 0; i1 < 4; ++i1)
+	for (int i2 = 0; i2 < 4; ++i2)
+	for (int i3 = 0; i3 < 4; ++i3)
+	for (int i4 = 0; i4 < 4; ++i4)
+	for (int d = 0; d < ndom; ++d) {
+		const double u0 = x[i1], u1 = x[i2], u2 = x[i3], u3 = x[i4];
+		double wt = w[i1] * w[i2] * w[i3] * w[i4];
+		double s1, s2, t1, t2;   // test (s1, s2), trial (t1, t2) before the simplex shift
+		if (match == 2) {
+			double m1, m2, jac;
+			if (d == 0) {
+				m1 = u0; m2 = u0 * u1;
+				s1 = (1 - m1) * u2; s2 = s1 * u3;
+				jac = (1 - m1) * s1;
+			} else if (d == 1) {
+				m1 = u0 * u1; m2 = u0 * (u1 - 1.0);
+				s1 = (1 - m1 + m2) * u2 - m2; s2 = (s1 + m2) * u3 - m2;
+				jac = (1.0 - m1 + m2) * (s1 + m2);
+			} else {
+				m1 = u0 * u1; m2 = u0;
+				s1 = (1 - m2) * u2 + m2 - m1; s2 = (s1 - m2 + m1) * u3;
+				jac = (1 - m2) * (s1 - m2 + m1);
+			}
+			wt *= jac * u0;
+			t1 = m1 + s1; t2 = m2 + s2;
+		} else if (match == 1) {
+			double m1, m2;
+			switch (d) {
+			case 0: m1 = -u0 * u1; m2 = -u0 * u1 * u2; s1 = (1.0 - u0) * u3 + u0; s2 = u0 * (1.0 - u1 + u1 * u2); break;
+			case 1: m1 = u0 * u1; m2 = u0 * u1 * u2; s1 = (1.0 - u0) * u3 + u0 * (1.0 - u1); s2 = u0 * (1.0 - u1); break;
+			case 2: m1 = -u0 * u1 * u2; m2 = u0 * u1 * (1.0 - u2); s1 = (1.0 - u0) * u3 + u0; s2 = u0 * (1.0 - u1); break;
+			case 3: m1 = u0 * u1 * u2; m2 = u0 * u1 * (u2 - 1.0); s1 = (1.0 - u0) * u3 + u0 * (1.0 - u1 * u2);
+				s2 = u0 * (1.0 - u1 * u2); break;
+			case 4: m1 = -u0 * u1 * u2; m2 = -u0 * u1; s1 = (1.0 - u0) * u3 + u0; s2 = u0; break;
+			default: m1 = u0 * u1 * u2; m2 = u0 * u1; s1 = (1.0 - u0) * u3 + u0 * (1.0 - u1 * u2); s2 = u0 * (1.0 - u1); break;
+			}
+			wt *= u1 * u0 * u0 * (1.0 - u0);
+			t1 = m1 + s1; t2 = m2 + s2;
+		} else {
+			s1 = u0; s2 = s1 * u1; t1 = s1 * u2; t2 = t1 * u3;
+			wt *= u2 * u0 * u0 * u0;
+		}
+		const double p[4] = {s1 - s2, s2, t1 - t2, t2};
+		for (int rep = 0; rep < (symmetric ? 2 : 1); ++rep) {
+			if (n >= cap) return -1;
+			double *o = out + 5 * n;
+			if (rep == 0) { o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = p[3]; }
+			else { o[0] = p[2]; o[1] = p[3]; o[2] = p[0]; o[3] = p[1]; }
+			o[4] = wt;
+			++n;
+		}
+	}
+	return n;
+}
+
+// Number of nodes two triangles share, with the start corners of the shared
+// edge or vertex as NiHu's scan finds them (test nodes a, trial nodes b).
+int nbg_overlap(const int32_t *a, const int32_t *b, int *ind1, int *ind2)
+{
+	int num = 0, s1 = 0, s2 = 0;
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j) {
+			if (a[i] != b[j]) continue;
+			if (num == 0) { s1 = i; s2 = j; }
+			else {
+				if (i < s1 || (i == 2 && s1 == 0 && num == 1)) s1 = i;
+				if (j < s2 || (j == 2 && s2 == 0 && num == 1)) s2 = j;
+			}
+			++num;
+		}
+	*ind1 = s1;
+	*ind2 = s2;
+	return num;
+}
+
+// Singular pairs of a mesh against itself: coincident pairs (e, e) first,
+// then every ordered pair of distinct elements sharing a node.  info =
+// match | ind1 << 4 | ind2 << 8.  Returns the pair count; with null outputs
+// only counts.
+int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, int32_t *pf, int32_t *info)
+{
+	std::vector<int> deg(n_nodes + 1, 0);
+	for (int e = 0; e < n; ++e)
+		for (int a = 0; a < 3; ++a) ++deg[tri[3 * e + a] + 1];
+	for (int v = 0; v < n_nodes; ++v) deg[v + 1] += deg[v];
+	std::vector<int> adj(deg[n_nodes]);
+	std::vector<int> fill(deg.begin(), deg.end() - 1);
+	for (int e = 0; e < n; ++e)
+		for (int a = 0; a < 3; ++a) adj[fill[tri[3 * e + a]]++] = e;
+	int64_t cnt = 0;
+	for (int e = 0; e < n; ++e) {
+		if (pe) { pe[cnt] = e; pf[cnt] = e; info[cnt] = 2; }
+		++cnt;
+	}
+	std::vector<int> cand;
+	for (int e = 0; e < n; ++e) {
+		cand.clear();
+		for (int a = 0; a < 3; ++a) {
+			const int v = tri[3 * e + a];
+			for (int q = deg[v]; q < deg[v + 1]; ++q)
+				if (adj[q] != e) cand.push_back(adj[q]);
+		}
+		std::sort(cand.begin(), cand.end());
+		cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+		for (int f : cand) {
+			int i1, i2;
+			const int num = nbg_overlap(tri + 3 * e, tri + 3 * f, &i1, &i2);
+			if (pe) { pe[cnt] = e; pf[cnt] = f; info[cnt] = (num > 1 ? 1 : 0) | (i1 << 4) | (i2 << 8); }
+			++cnt;
+		}
+	}
+	return cnt;
+}
+
+// Layout of the nodal-dof regular kernel for one mesh.  Picks the element
+// order (the mesh's own, or Morton order of the centres when that packs the
+// wave tiles better), builds the wave tiles and the column-slot schedule of
+// the trial sequence in chunks of `chunk` elements.  Call with null arrays to
+// get n_tiles; then with arrays of n_tiles * 64 (elem, loc, rows), n (order),
+// 4 n (col) and ceil(n / chunk) * 16 (fin).
+int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk,
+	int32_t *order_out, int32_t *elem, int32_t *loc, int32_t *rows, int32_t *col, int32_t *fin)
+{
+	std::vector<int> mesh_order(n), morton(n);
+	for (int e = 0; e < n; ++e) mesh_order[e] = e;
+	double lo[3], hi[3];
+	for (int d = 0; d < 3; ++d) {
+		lo[d] = *std::min_element(c + d * n, c + (d + 1) * n);
+		hi[d] = *std::max_element(c + d * n, c + (d + 1) * n);
+	}
+	std::vector<std::pair<uint64_t, int> > key(n);
+	for (int e = 0; e < n; ++e) {
+		uint64_t k = 0;
+		for (int d = 0; d < 3; ++d) {
+			const double span = hi[d] - lo[d];
+			const uint64_t q = span > 0 ? (uint64_t)std::min(1023.0, (c[d * n + e] - lo[d]) / span * 1023.0) : 0;
+			k |= spread3(q) << d;
+		}
+		key[e] = {k, e};
+	}
+	std::stable_sort(key.begin(), key.end(),
+		[](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) { return a.first < b.first; });
+	for (int e = 0; e < n; ++e) morton[e] = key[e].second;
+	Tiles t0 = make_tiles(mesh_order, tri), t1 = make_tiles(morton, tri);
+	const int64_t m0 = schedule(mesh_order, tri, chunk, nullptr, nullptr);
+	const int64_t m1 = schedule(morton, tri, chunk, nullptr, nullptr);
+	const bool use_morton = t1.n_tiles < t0.n_tiles || (t1.n_tiles == t0.n_tiles && m1 < m0);
+	const std::vector<int> &ord = use_morton ? morton : mesh_order;
+	const Tiles &t = use_morton ? t1 : t0;
+	if (!order_out) return t.n_tiles;
+	std::copy(ord.begin(), ord.end(), order_out);
+	std::copy(t.elem.begin(), t.elem.end(), elem);
+	std::copy(t.loc.begin(), t.loc.end(), loc);
+	std::copy(t.rows.begin(), t.rows.end(), rows);
+	schedule(ord, tri, chunk, col, fin);
+	return t.n_tiles;
+}
+
+} // extern "C"

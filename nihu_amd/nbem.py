@@ -22,7 +22,8 @@ NB_RULE_NIHU, NB_RULE_FIXED = 0, 1
 NB_NEAR_MEAN_EDGE, NB_NEAR_DIAMETER, NB_NEAR_NONE = 0, 1, 2
 NB_C64, NB_C128, NB_F32, NB_F64 = 0, 1, 2, 3
 NB_HOST, NB_DEVICE = 0, 1
-NB_COLLOCATION, NB_GALERKIN = 0, 1
+NB_COLLOCATION, NB_GALERKIN, NB_GALERKIN_LINEAR = 0, 1, 2
+FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_linear": NB_GALERKIN_LINEAR}
 
 # symbols declared in include/nbem.h
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_destroy", "nb_mesh_info",
@@ -150,6 +151,15 @@ class Mesh:
     def n(self):
         return len(self.tris)
 
+    @property
+    def n_nodes(self):
+        return len(self.nodes)
+
+    def dofs(self, formalism="collocation"):
+        """rows / columns of this mesh's side of a matrix: nodes for
+        isoparametric Galerkin, elements otherwise"""
+        return self.n_nodes if FORMALISMS[formalism] == NB_GALERKIN_LINEAR else self.n
+
     def device_bytes(self):
         b = C.c_int64()
         check(lib().nb_mesh_info(self._h, None, None, C.byref(b)))
@@ -206,13 +216,17 @@ def colmajor(rows, cols, dtype="complex64", device="cuda"):
 
 
 def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.0, alpha=1.0, pol=None,
-             same_mesh=None, accumulate=True, stream=None):
+             same_mesh=None, accumulate=True, stream=None, formalism="collocation"):
+    """out (+)= alpha * operator; formalism "collocation" (dirac(constant) x
+    constant), "galerkin" (constant x constant) or "galerkin_linear"
+    (isoparametric x isoparametric, rows / columns = mesh nodes)."""
     pol = pol or policy(**NIHU_POLICY)
     same = (test is trial) if same_mesh is None else bool(same_mesh)
     a = complex(alpha)
     o = out_desc(out, accumulate)
-    check(lib().nb_assemble(test.handle, trial.handle, int(same), NB_COLLOCATION, NbKernel(family, nx, ny, float(k)),
-                            NbScalar(a.real, a.imag), C.byref(pol), C.byref(o), _stream_ptr(stream)))
+    check(lib().nb_assemble(test.handle, trial.handle, int(same), FORMALISMS[formalism],
+                            NbKernel(family, nx, ny, float(k)), NbScalar(a.real, a.imag), C.byref(pol), C.byref(o),
+                            _stream_ptr(stream)))
     return out
 
 

@@ -998,7 +998,8 @@ int orc_sweep_bm(const double *xyz, int nn, const int *tri, int nt, const double
 int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
 	const double *sxyz, int sn, const int *stri, int snt, int same_mesh, int linear,
 	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
-	const orc_policy *pol, double *out, int64_t ld, int accumulate, int n_threads)
+	const orc_policy *pol, const int *test_elems, int n_test_elems, double *out, int64_t ld, int accumulate,
+	int n_threads)
 {
 	ORC_TRY
 	auto test = make_mesh(txyz, tn, ttri, tnt);
@@ -1038,8 +1039,12 @@ int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
 	if (n_threads <= 0) n_threads = orc_num_threads();
 	std::string err;
 	int errcode = 0;
+	// test_elems: only these test elements contribute (rows of dofs whose
+	// elements are all listed come out complete)
+	const int n_loop = test_elems ? n_test_elems : tnt;
 	#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
-	for (int i = 0; i < tnt; ++i) {
+	for (int ii = 0; ii < n_loop; ++ii) {
+		const int i = test_elems ? test_elems[ii] : ii;
 		try {
 			for (int j = 0; j < snt; ++j) {
 				cplx blk[3][3];
@@ -1066,6 +1071,15 @@ int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
 	if (errcode == 2) throw std::out_of_range(err);
 	if (errcode == 1) throw std::runtime_error(err);
 	ORC_CATCH
+}
+
+// node overlap of two elements as NiHu's scan finds it: returns the count
+int orc_overlap(const int *a, const int *b, int *ind1, int *ind2)
+{
+	Overlap o = get_overlapping(a, b);
+	*ind1 = o.ind1;
+	*ind2 = o.ind2;
+	return o.num;
 }
 
 // canonical singular rule of a match dimension: n points (txi, teta, sxi, seta, w)

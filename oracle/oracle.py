@@ -59,7 +59,8 @@ def lib():
         l.orc_sweep_bm.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, P, I, C.c_int, D, D, C.c_int]
         l.orc_assemble_galerkin.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int,
                                             C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
-                                            P, D, C.c_int64, C.c_int, C.c_int]
+                                            P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_overlap.argtypes = [I, I, I, I]
         l.orc_singular_rule.argtypes = [C.c_int, D, C.c_int, I]
         l.orc_galerkin_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, P, D, I, D, I, C.c_int, C.c_int,
                                          C.c_int, D]
@@ -192,10 +193,12 @@ def sweep_bm(nodes, tris, ks, pol=None, rows=None, threads=0):
 
 
 def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, trial=None, linear=True,
-                      out=None, accumulate=False, threads=0):
+                      out=None, accumulate=False, threads=0, test_elems=None):
     """alpha * Galerkin matrix, complex128 (Fortran order): linear=True gives
     isoparametric linear triangles ([test nodes, trial nodes]), linear=False
-    constant triangles ([test elements, trial elements]); family 2 = identity."""
+    constant triangles ([test elements, trial elements]); family 2 = identity.
+    test_elems: only these test elements contribute (rows whose elements are
+    all listed are complete)."""
     pol = pol or policy()
     t_nodes, t_tris = _mesh(nodes, tris)
     same = trial is None
@@ -205,10 +208,12 @@ def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, t
         out = np.zeros(shape, dtype=np.complex128, order="F")
         accumulate = False
     alpha = complex(alpha)
+    te = None if test_elems is None else np.ascontiguousarray(test_elems, dtype=np.int32)
     _check(lib().orc_assemble_galerkin(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
                                        _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same), int(linear),
-                                       family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol),
-                                       out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0],
+                                       family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol), _i(te),
+                                       0 if te is None else len(te), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                       out.shape[0],
                                        int(accumulate), threads))
     return out
 
@@ -236,6 +241,15 @@ def galerkin_block(family, nx, ny, k, test_coords, test_nodes, trial_coords, tri
     _check(lib().orc_galerkin_block(family, nx, ny, k, C.byref(pol), _d(tc), _i(tn), _d(sc), _i(sn),
                                     int(same_mesh), int(same_elem), int(linear), _d(out)))
     return (out[0::2] + 1j * out[1::2]).reshape(3, 3)
+
+
+def overlap(a, b):
+    """(count, ind1, ind2) of NiHu's node-overlap scan of two elements"""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    i1, i2 = C.c_int(0), C.c_int(0)
+    n = lib().orc_overlap(_i(a), _i(b), C.byref(i1), C.byref(i2))
+    return n, i1.value, i2.value
 
 
 def num_threads():

@@ -1,0 +1,143 @@
+"""CPU checks of the engine's host-side Galerkin set-up (no GPU calls): the
+canonical singular rules and the node-overlap scan agree with the oracle, the
+singular pair list is every node-sharing pair, and the wave tiles and the
+column-slot schedule of the nodal regular kernel are consistent (every
+element in exactly one lane, every corner's local row and slot holds its node,
+every evicted node is the slot's previous occupant)."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests import bemref
+
+I32 = C.POINTER(C.c_int32)
+D = C.POINTER(C.c_double)
+TILE, CS = 64, 16
+
+
+@pytest.fixture(scope="module")
+def raw():
+    from nihu_amd import nbem
+    l = nbem.lib()
+    l.nbg_singular_rule.argtypes = [C.c_int, D, C.c_int]
+    l.nbg_overlap.argtypes = [I32, I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    l.nbg_singular_pairs.argtypes = [C.c_int, C.c_int, I32, I32, I32, I32]
+    l.nbg_singular_pairs.restype = C.c_int64
+    l.nbg_galerkin_layout.argtypes = [C.c_int, I32, D, C.c_int, I32, I32, I32, I32, I32, I32]
+    return l
+
+
+def _i(a):
+    return a.ctypes.data_as(I32)
+
+
+@pytest.mark.parametrize("match", [0, 1, 2])
+def test_singular_rule_matches_oracle(orc, raw, match):
+    buf = np.zeros((1536, 5))
+    n = raw.nbg_singular_rule(match, buf.ctypes.data_as(D), 1536)
+    ref = orc.singular_rule(match)
+    assert n == len(ref)
+    assert np.abs(buf[:n] - ref).max() <= 1e-16
+
+
+def test_overlap_scan_matches_oracle(orc, raw):
+    rng = np.random.default_rng(5)
+    i1, i2 = C.c_int(), C.c_int()
+    for _ in range(2000):
+        a = rng.choice(6, 3, replace=False).astype(np.int32)
+        b = rng.choice(6, 3, replace=False).astype(np.int32)
+        n = raw.nbg_overlap(_i(a), _i(b), C.byref(i1), C.byref(i2))
+        assert (n, i1.value, i2.value) == orc.overlap(a, b), (a, b)
+
+
+def _mesh(level=2):
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(level, seed=1)
+    return nodes, np.ascontiguousarray(tris, dtype=np.int32)
+
+
+def test_singular_pairs_are_node_sharing_pairs(orc, raw):
+    nodes, tris = _mesh(2)
+    n = len(tris)
+    cnt = raw.nbg_singular_pairs(len(nodes), n, _i(tris), None, None, None)
+    pe, pf, info = (np.zeros(cnt, np.int32) for _ in range(3))
+    assert raw.nbg_singular_pairs(len(nodes), n, _i(tris), _i(pe), _i(pf), _i(info)) == cnt
+    assert np.all(pe[:n] == np.arange(n)) and np.all(pf[:n] == np.arange(n)) and np.all(info[:n] == 2)
+    ref = {(e, f) for e in range(n) for f in range(n) if e != f and set(tris[e]) & set(tris[f])}
+    got = set(zip(pe[n:].tolist(), pf[n:].tolist()))
+    assert got == ref and len(got) == cnt - n
+    for e, f, inf in zip(pe[n:], pf[n:], info[n:]):
+        num, i1, i2 = orc.overlap(tris[e], tris[f])
+        assert inf == (1 if num > 1 else 0) | (i1 << 4) | (i2 << 8)
+
+
+def _layout(raw, nodes, tris, chunk):
+    n = len(tris)
+    c = np.ascontiguousarray(nodes[tris].mean(1).T)   # [3][n]
+    tiles = raw.nbg_galerkin_layout(n, _i(tris), c.ctypes.data_as(D), chunk, None, None, None, None, None, None)
+    order = np.zeros(n, np.int32)
+    elem, loc, rows = (np.zeros(tiles * TILE, np.int32) for _ in range(3))
+    col = np.zeros(4 * n, np.int32)
+    fin = np.zeros(((n + chunk - 1) // chunk) * CS, np.int32)
+    assert raw.nbg_galerkin_layout(n, _i(tris), c.ctypes.data_as(D), chunk, _i(order), _i(elem), _i(loc), _i(rows),
+                                   _i(col), _i(fin)) == tiles
+    return tiles, order, elem, loc, rows, col.reshape(n, 4), fin.reshape(-1, CS)
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_wave_tiles_cover_each_element_once(raw, shuffle):
+    nodes, tris = _mesh(3)
+    if shuffle:   # a mesh without locality: the Morton order takes over
+        tris = np.ascontiguousarray(tris[np.random.default_rng(0).permutation(len(tris))])
+    tiles, order, elem, loc, rows, _, _ = _layout(raw, nodes, tris, 128)
+    assert sorted(order.tolist()) == list(range(len(tris)))
+    used = elem[elem >= 0]
+    assert sorted(used.tolist()) == list(range(len(tris)))
+    # tiles take elements in sequence order
+    assert np.all(used == order)
+    for t in range(tiles):
+        r = rows[t * TILE:(t + 1) * TILE]
+        nr = int((r >= 0).sum())
+        assert np.all(r[:nr] >= 0) and np.all(r[nr:] == -1) and len(set(r[:nr].tolist())) == nr
+        for l in range(TILE):
+            e = elem[t * TILE + l]
+            if e < 0:
+                continue
+            for a in range(3):
+                la = (loc[t * TILE + l] >> (8 * a)) & 255
+                assert la < nr and r[la] == tris[e][a]
+    # a mesh with locality fills its tiles
+    if not shuffle:
+        assert tiles == (len(tris) + TILE - 1) // TILE
+
+
+@pytest.mark.parametrize("chunk", [64, 128, 512])
+def test_column_slot_schedule_replays(raw, chunk):
+    nodes, tris = _mesh(3)
+    n = len(tris)
+    _, order, _, _, _, col, fin = _layout(raw, nodes, tris, chunk)
+    flushed = {}
+    for c0 in range(0, n, chunk):
+        slot = [-1] * CS
+        for i in range(c0, min(n, c0 + chunk)):
+            e = order[i]
+            sl = [col[i, 3] & 31, (col[i, 3] >> 5) & 31, (col[i, 3] >> 10) & 31]
+            assert len(set(sl)) == 3 and max(sl) < CS
+            for b in range(3):
+                v = col[i, b]
+                if v >= 0:
+                    assert slot[sl[b]] == v       # evict the current occupant
+                    flushed[v] = flushed.get(v, 0) + 1
+                    slot[sl[b]] = -1
+                if slot[sl[b]] == -1:
+                    slot[sl[b]] = tris[e][b]
+                assert slot[sl[b]] == tris[e][b]
+        assert slot == fin[c0 // chunk].tolist()
+        for v in slot:
+            if v >= 0:
+                flushed[v] = flushed.get(v, 0) + 1
+    # every node of the mesh is flushed at least once, and on average few times
+    assert set(flushed) == set(range(len(nodes)))
+    assert sum(flushed.values()) < 2.5 * len(nodes) * max(1, 512 // chunk)

@@ -1,0 +1,233 @@
+"""Parity of the Galerkin passes with the FP64 oracle, through the C ABI.
+
+Bar: with constant fields the entries of node-sharing pairs come from the FP64
+pass alone and agree within 1e-12 of the row's largest entry (complex128
+output); every matrix (FP32 regular pairs plus FP64 singular pairs) agrees
+within 2e-6 of the row's largest entry and within 1e-5 relative Frobenius
+error of the oracle under the same quadrature policy; the mass matrix to
+1e-14.  At the full ellipsoid size (about 10k nodes, 20k elements) whole rows
+of sampled nodes are compared with the oracle, which then integrates only the
+test elements around them.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROW_TOL = 2e-6
+FRO_TOL = 1e-5
+FORMS = {"galerkin": False, "galerkin_linear": True}
+POLICIES = {
+    "nihu": (dict(rule="nihu", near="mean_edge", limit=1.5), dict(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5)),
+    "fixed4": (dict(rule="fixed", degree=4, near="none"), dict(rule=1, fixed_degree=4, near_mode=2, near_limit=1.5)),
+}
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    from nihu_amd import nbem
+    return nbem
+
+
+@pytest.fixture(scope="module")
+def l2():
+    from nihu_amd.mesh import jittered_icosphere
+    return jittered_icosphere(2, seed=4)
+
+
+def rowscaled(got, ref):
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    scale[scale == 0] = 1.0
+    return (np.abs(got - ref) / scale).max()
+
+
+def relfro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _assemble(nb, m_test, m_trial, form, fam, nx, ny, k, alpha, gp, dtype="complex128", same=None):
+    import torch
+    out = nb.colmajor(m_test.dofs(form), m_trial.dofs(form), dtype)
+    nb.assemble(m_test, m_trial, out, fam, nx, ny, k, alpha=alpha, pol=nb.policy(**gp), accumulate=False,
+                formalism=form, same_mesh=same)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def _singular_mask(tris):
+    n = len(tris)
+    mask = np.zeros((n, n), bool)
+    for e in range(n):
+        for f in range(n):
+            mask[e, f] = bool(set(tris[e]) & set(tris[f]))
+    return mask
+
+
+@pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
+@pytest.mark.parametrize("pname", ["nihu", "fixed4"])
+@pytest.mark.parametrize("fam,k", [(0, 0.0), (1, 3.0), (1, 10.0)])
+@pytest.mark.parametrize("nxy", [(0, 0), (0, 1), (1, 0)])
+def test_galerkin_operator(nb, orc, l2, form, pname, fam, k, nxy):
+    nodes, tris = l2
+    gp, op = POLICIES[pname]
+    alpha = 0.8 + 0.25j
+    m = nb.Mesh(nodes, tris)
+    print("case", form, pname, fam, k, nxy, flush=True)
+    got = _assemble(nb, m, m, form, fam, nxy[0], nxy[1], k, alpha, gp)
+    ref = orc.assemble_galerkin(nodes, tris, fam, nxy[0], nxy[1], k=k, alpha=alpha, pol=orc.policy(**op),
+                                linear=FORMS[form])
+    rs, rf = rowscaled(got, ref), relfro(got, ref)
+    print("row-scaled %.2e  rel fro %.2e" % (rs, rf), flush=True)
+    assert rs <= ROW_TOL and rf <= FRO_TOL
+    if form == "galerkin":
+        mask = _singular_mask(tris)
+        scale = np.abs(ref).max(axis=1, keepdims=True)
+        err = (np.abs(got - ref) / scale)[mask].max()
+        print("singular entries %.2e" % err, flush=True)
+        assert err <= 1e-12
+
+
+@pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
+@pytest.mark.parametrize("nxy", [(0, 0), (1, 1)])
+def test_other_mesh(nb, orc, l2, form, nxy):
+    """Different test and trial meshes: no singular check, so the hypersingular
+    kernel assembles too."""
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = l2
+    fn, ft = jittered_icosphere(1, seed=9)
+    fn = fn * 1.6 + np.array([0.2, -0.1, 0.3])
+    gp, op = POLICIES["nihu"]
+    mt, ms = nb.Mesh(nodes, tris), nb.Mesh(fn, ft)
+    got = _assemble(nb, mt, ms, form, 1, nxy[0], nxy[1], 2.5, 1.0, gp)
+    ref = orc.assemble_galerkin(nodes, tris, 1, nxy[0], nxy[1], k=2.5, pol=orc.policy(**op), trial=(fn, ft),
+                                linear=FORMS[form])
+    assert rowscaled(got, ref) <= ROW_TOL and relfro(got, ref) <= FRO_TOL
+
+
+@pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
+def test_hypersingular_same_mesh_raises(nb, l2, form):
+    nodes, tris = l2
+    m = nb.Mesh(nodes, tris)
+    out = nb.colmajor(m.dofs(form), m.dofs(form), "complex64")
+    with pytest.raises(nb.NbemError, match="Invalid general singular quadrature"):
+        nb.assemble(m, m, out, 1, 1, 1, 1.0, formalism=form)
+
+
+@pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
+def test_mass_matrix(nb, orc, l2, form):
+    nodes, tris = l2
+    m = nb.Mesh(nodes, tris)
+    got = _assemble(nb, m, m, form, nb.NB_IDENTITY, 0, 0, 0.0, -0.5, POLICIES["nihu"][0])
+    ref = orc.assemble_galerkin(nodes, tris, 2, 0, 0, alpha=-0.5, linear=FORMS[form])
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
+def test_accumulate_host_and_complex64(nb, orc, l2, form):
+    """Host (numpy) output with accumulate: result = previous + alpha M; the
+    complex64 device output within FP32 rounding of complex128."""
+    import torch
+    nodes, tris = l2
+    gp, op = POLICIES["fixed4"]
+    m = nb.Mesh(nodes, tris)
+    rows = m.dofs(form)
+    rng = np.random.default_rng(2)
+    host = np.asfortranarray(rng.standard_normal((rows, rows)) + 1j * rng.standard_normal((rows, rows)))
+    start = host.copy()
+    nb.assemble(m, m, host, 1, 0, 1, 4.0, alpha=-1j, pol=nb.policy(**gp), accumulate=True, formalism=form)
+    ref = orc.assemble_galerkin(nodes, tris, 1, 0, 1, k=4.0, alpha=-1j, pol=orc.policy(**op), linear=FORMS[form])
+    assert rowscaled(host - start, ref) <= ROW_TOL
+    c64 = _assemble(nb, m, m, form, 1, 0, 0, 4.0, 1.0, gp, dtype="complex64")
+    c128 = _assemble(nb, m, m, form, 1, 0, 0, 4.0, 1.0, gp, dtype="complex128")
+    assert rowscaled(c64.astype(np.complex128), c128) <= 1e-6
+    torch.cuda.synchronize()
+
+
+def test_fixed_rule_level3(nb, orc):
+    """The 6 x 6-point rule of the Galerkin workload on 1280 elements, single
+    and double layer with nodal fields, whole matrices."""
+    from nihu_amd.mesh import ellipsoid
+    nodes, tris = ellipsoid(3, seed=3)
+    gp, op = POLICIES["fixed4"]
+    m = nb.Mesh(nodes, tris)
+    for nxy in ((0, 0), (0, 1)):
+        got = _assemble(nb, m, m, "galerkin_linear", 1, nxy[0], nxy[1], 5.0, 1.0, gp)
+        ref = orc.assemble_galerkin(nodes, tris, 1, nxy[0], nxy[1], k=5.0, pol=orc.policy(**op), linear=True)
+        rs, rf = rowscaled(got, ref), relfro(got, ref)
+        print("L3 ellipsoid", nxy, "row-scaled %.2e rel fro %.2e" % (rs, rf), flush=True)
+        assert rs <= ROW_TOL and rf <= FRO_TOL
+
+
+def test_transparent_sphere_fixture(nb):
+    """NiHu's transparent known answer on its sphere_1_3.off fixture with GPU
+    Galerkin matrices (constant and nodal fields)."""
+    import os
+    from nihu_amd.mesh import read_off
+    nodes, tris = read_off(os.path.join(os.path.dirname(__file__), "golden", "sphere_1_3.off"))
+    m = nb.Mesh(nodes, tris)
+    gp = POLICIES["nihu"][0]
+    k = 1.0
+    for form in ("galerkin", "galerkin_linear"):
+        L = _assemble(nb, m, m, form, 1, 0, 0, k, 1.0, gp)
+        M = _assemble(nb, m, m, form, 1, 0, 1, k, 1.0, gp)
+        I = _assemble(nb, m, m, form, nb.NB_IDENTITY, 0, 0, 0.0, 1.0, gp)
+        if form == "galerkin_linear":
+            P, nrm = nodes, nodes / np.linalg.norm(nodes, axis=1)[:, None]
+        else:
+            X = nodes[tris]
+            P = X.mean(1)
+            nv = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+            nrm = nv / np.linalg.norm(nv, axis=1)[:, None]
+        p0 = np.zeros(len(P), complex)
+        q0 = np.zeros(len(P), complex)
+        for x0, amp in (((.2, .3, .1), 1.0), ((.3, .2, .2), -1.0)):
+            rv = P - np.array(x0)
+            r = np.linalg.norm(rv, axis=1)
+            rdn = np.sum(rv * nrm, 1) / r
+            p0 += amp * np.exp(-1j * k * r) / r / (4 * np.pi)
+            q0 += amp * -(np.exp(-1j * k * r) / r / r) * rdn / (4 * np.pi) * (1 + 1j * k * r)
+        p = np.linalg.solve(M - 0.5 * I, L @ q0)
+        err = np.linalg.norm(p - p0) / np.linalg.norm(p0)
+        print(form, "transparent error", err, flush=True)
+        assert err < 0.03
+
+
+def test_full_size_rows_and_gauss_identity(nb, orc):
+    """The Galerkin workload's ellipsoid (level 5: 10242 nodes, 20480
+    elements), nodal fields, fixed 6 x 6 rule: full rows of 6 sampled nodes
+    against the oracle, and Gauss's identity on the Laplace double layer
+    (row sums = -1/2 mass row sums) over every row."""
+    import torch
+    from nihu_amd.mesh import ellipsoid
+    nodes, tris = ellipsoid(5, seed=0)
+    gp, op = POLICIES["fixed4"]
+    m = nb.Mesh(nodes, tris)
+    nn = len(nodes)
+    rng = np.random.default_rng(11)
+    sample = np.sort(rng.choice(nn, 6, replace=False))
+    elems = np.unique(np.nonzero(np.isin(tris, sample))[0]).astype(np.int32)
+    print("full size: nodes", nn, "elements", len(tris), "sampled", sample.tolist(), flush=True)
+    out = nb.colmajor(nn, nn, "complex64")
+    nb.assemble(m, m, out, 1, 0, 0, 5.0, pol=nb.policy(**gp), accumulate=False, formalism="galerkin_linear")
+    torch.cuda.synchronize()
+    got = out[sample].cpu().numpy().astype(np.complex128)
+    ref = orc.assemble_galerkin(nodes, tris, 1, 0, 0, k=5.0, pol=orc.policy(**op), linear=True,
+                                test_elems=elems)[sample]
+    rs = rowscaled(got, ref)
+    print("sampled rows row-scaled %.2e rel %.2e" % (rs, relfro(got, ref)), flush=True)
+    assert rs <= 1e-5 and relfro(got, ref) <= FRO_TOL
+    del out
+    D = nb.colmajor(nn, nn, "float64")
+    nb.assemble(m, m, D, 0, 0, 1, 0.0, pol=nb.policy(**gp), accumulate=False, formalism="galerkin_linear")
+    rs_d = D.sum(dim=1).cpu().numpy()
+    del D
+    Mm = nb.colmajor(nn, nn, "float64")
+    nb.assemble(m, m, Mm, nb.NB_IDENTITY, 0, 0, 0.0, accumulate=False, formalism="galerkin_linear")
+    ms = Mm.sum(dim=1).cpu().numpy()
+    torch.cuda.synchronize()
+    gauss = np.abs(rs_d + 0.5 * ms).max() / ms.max()
+    print("Gauss identity max deviation %.2e" % gauss, flush=True)
+    assert gauss < 2e-3
