@@ -63,9 +63,10 @@ def main():
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep", "serve"], default="bm",
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin"], default="bm",
                     help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
-                         "serve: Poisson stream of assembly-plus-LU-solve requests")
+                         "serve: Poisson stream of assembly-plus-LU-solve requests; galerkin: SLP and DLP "
+                         "Galerkin matrices with linear triangles on the 20480-element ellipsoid")
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
     args = ap.parse_args()
@@ -79,6 +80,8 @@ def main():
         return run_sweep(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "galerkin":
+        return run_galerkin(args)
 
     import numpy as np
     import torch
@@ -210,6 +213,124 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": err,
         "pass_ms": pass_ms,
+    }
+    print(json.dumps(out))
+
+
+# FP32 operations per (test point, trial point) of the Galerkin regular pass
+# with linear fields (kernel value, phase, three shape-weighted sums, and the
+# per-test-point block update spread over the 6 trial points)
+FLOPS_PER_POINTPAIR_GAL = {"SLP": 40, "DLP": 51}
+
+
+def run_galerkin(args):
+    """Galerkin workload: Helmholtz single- and double-layer matrices with
+    isoparametric linear triangles on the seeded ellipsoid (level 5: 20480
+    elements, 10242 nodes, semi-axes U[0.5, 1.5], +-1% jitter), k = 5 (ka = 5
+    for the unit parent sphere), 6 x 6-point regular rule (Dunavant degree 4
+    on both elements) in FP32, coincident / edge / vertex pairs with the
+    Duffy-type rules in FP64, complex64 results.  A step assembles both
+    10242^2 matrices."""
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import ellipsoid
+
+    torch.cuda.set_device(0)
+    nodes, tris = ellipsoid(args.level, seed=0)
+    ne, nn = len(tris), len(nodes)
+    k = 5.0
+    pol = nbem.policy(rule="fixed", degree=4, near="none")
+    name, ncu, hbm = nbem.device_info()
+    log("device %s, %d CUs; ellipsoid %d elements, %d nodes; k = %g" % (name, ncu, ne, nn, k))
+    mesh = nbem.Mesh(nodes, tris)
+    L, M = nbem.colmajor(nn, nn), nbem.colmajor(nn, nn)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        nbem.assemble(mesh, mesh, L, 1, 0, 0, k, pol=pol, accumulate=False, stream=stream,
+                      formalism="galerkin_linear")
+        nbem.assemble(mesh, mesh, M, 1, 0, 1, k, pol=pol, accumulate=False, stream=stream,
+                      formalism="galerkin_linear")
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log("warmup step %d done" % w)
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    ms_per_step = wall / args.steps * 1e3
+    value = 2 * nn * nn * args.steps / wall
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    log("timed %d steps: %.3f ms/step, %.4g entries/s; per-launch device ms %s" % (args.steps, ms_per_step, value,
+                                                                                 pass_ms))
+
+    # node-sharing pairs go to the FP64 pass; the rest are regular
+    node_elems = [[] for _ in range(nn)]
+    for e, t in enumerate(tris):
+        for v in t:
+            node_elems[v].append(e)
+    n_sing = 0
+    for e, t in enumerate(tris):
+        n_sing += len(set(node_elems[t[0]]) | set(node_elems[t[1]]) | set(node_elems[t[2]]))
+    n_reg = ne * ne - n_sing
+    reg_ms_pair = 2 * pass_ms["regular"]       # SLP and DLP launches per step
+    flops = n_reg * 36 * (FLOPS_PER_POINTPAIR_GAL["SLP"] + FLOPS_PER_POINTPAIR_GAL["DLP"])
+    ach = flops / (reg_ms_pair * 1e-3) / 1e12 if reg_ms_pair > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "engine": "VALU fp32 (no MFMA; packed fp32 disabled at build)",
+                "kernel": "galerkin_regular_kernel<HELM,{SLP,DLP},3,6>", "kernel_ms": round(reg_ms_pair, 4),
+                "algorithmic_flops": flops}
+
+    cpu = err = None
+    if args.cpu_rows > 0:
+        from oracle import oracle
+        rng = np.random.default_rng(123)
+        sample = np.sort(rng.choice(nn, max(1, args.cpu_rows // 16), replace=False))
+        elems = np.unique(np.nonzero(np.isin(tris, sample))[0]).astype(np.int32)
+        opol = oracle.policy(rule=1, fixed_degree=4, near_mode=2, near_limit=1.5)
+        threads = oracle.num_threads()
+        tc = time.perf_counter()
+        Lr = oracle.assemble_galerkin(nodes, tris, 1, 0, 0, k=k, pol=opol, linear=True, test_elems=elems,
+                                      threads=threads)[sample]
+        Mr = oracle.assemble_galerkin(nodes, tris, 1, 0, 1, k=k, pol=opol, linear=True, test_elems=elems,
+                                      threads=threads)[sample]
+        dt = time.perf_counter() - tc
+        # the oracle integrated len(elems) of ne test elements against every trial element
+        cpu_value = 2 * nn * nn / (dt * ne / len(elems))
+        cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+               "sample": "test elements around %d sampled nodes (%d of %d elements) against all %d trial "
+                         "elements, SLP and DLP, FP64 oracle, %.1f s wall; rate scaled by %d/%d to the whole "
+                         "matrices" % (len(sample), len(elems), ne, ne, dt, len(elems), ne)}
+        si = torch.from_numpy(sample.astype(np.int64)).cuda()
+        Lg = L[si, :].cpu().numpy().astype(np.complex128)
+        Mg = M[si, :].cpu().numpy().astype(np.complex128)
+        err = {"rows": int(len(sample)),
+               "L_rel_fro": float(np.linalg.norm(Lg - Lr) / np.linalg.norm(Lr)),
+               "M_rel_fro": float(np.linalg.norm(Mg - Mr) / np.linalg.norm(Mr))}
+        log("sampled-row error vs oracle: %s; cpu %.4g entries/s on %d threads" % (err, cpu_value, threads))
+
+    out = {
+        "metric": "matrix entries/s, Helmholtz 3D Galerkin linear tris, 20480 elems",
+        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64) regular pairs, fp64 singular pairs",
+        "data": "synthetic: seeded ellipsoid",
+        "config": {"workload": "Helmholtz 3D Galerkin, isoparametric linear triangles, SLP and DLP matrices",
+                   "n_elements": ne, "n_nodes": nn, "k": k, "mesh_seed": 0,
+                   "regular_rule": "Dunavant degree 4 x degree 4 (6 x 6 points)",
+                   "singular": "coincident/edge/vertex Duffy-type rules, 4-point Gauss per axis, FP64",
+                   "pairs": {"regular": n_reg, "singular": n_sing}},
+        "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
     print(json.dumps(out))
 

@@ -46,7 +46,7 @@ int nbg_quad_table(int degree, int n, const double *c, const double *J, const do
 int nbg_singular_rule(int match, double *out, int cap);
 int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, int32_t *pf, int32_t *info);
 int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk, int32_t *order_out, int32_t *elem,
-	int32_t *loc, int32_t *rows, int32_t *col, int32_t *fin);
+	uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin);
 }
 
 namespace {
@@ -254,7 +254,7 @@ struct nb_mesh {
 	DevPtr d_tri;                              // node ids [3][n]
 	// Galerkin, nodal dofs: element order, wave tiles, column-slot schedule
 	int g_tiles = 0, g_chunk = 0;
-	DevPtr g_order, g_elem, g_loc, g_rows, g_col, g_fin;
+	DevPtr g_order, g_elem, g_contrib, g_rows, g_col, g_fin;
 	// Galerkin singular pairs against itself (coincident pairs first)
 	int64_t sp_n = -1;
 	DevPtr sp_e, sp_f, sp_info;
@@ -390,18 +390,19 @@ int ensure_gal_layout(nb_mesh *m)
 		nullptr, nullptr);
 	const int chunk = gal_chunk(tiles, n);
 	const size_t nl = (size_t)tiles * nbk::GAL_TILE;
-	std::vector<int32_t> order(n), elem(nl), loc(nl), rows(nl), col(4 * (size_t)n),
+	std::vector<int32_t> order(n), elem(nl), rows(nl), col(4 * (size_t)n),
 		fin((size_t)((n + chunk - 1) / chunk) * nbk::GAL_CS);
-	if (nbg_galerkin_layout(n, m->tri.data(), m->c.data(), chunk, order.data(), elem.data(), loc.data(), rows.data(),
-		col.data(), fin.data()) != tiles)
+	std::vector<uint32_t> contrib(2 * nl);
+	if (nbg_galerkin_layout(n, m->tri.data(), m->c.data(), chunk, order.data(), elem.data(), contrib.data(),
+		rows.data(), col.data(), fin.data()) != tiles)
 		return fail(NB_ERR_RUNTIME, "Galerkin layout");
 	int s;
-	if ((s = upload(m->g_order, order)) || (s = upload(m->g_elem, elem)) || (s = upload(m->g_loc, loc)) ||
+	if ((s = upload(m->g_order, order)) || (s = upload(m->g_elem, elem)) || (s = upload(m->g_contrib, contrib)) ||
 		(s = upload(m->g_rows, rows)) || (s = upload(m->g_col, col)) || (s = upload(m->g_fin, fin)))
 		return s;
 	m->g_tiles = tiles;
 	m->g_chunk = chunk;
-	m->bytes += (int64_t)(order.size() + 3 * nl + col.size() + fin.size()) * 4;
+	m->bytes += (int64_t)(order.size() + 4 * nl + col.size() + fin.size()) * 4;
 	return NB_OK;
 }
 
@@ -500,7 +501,7 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 		a.n_tiles = test->g_tiles;
 		a.chunk = trial->g_chunk;
 		a.t_elem = (const int *)test->g_elem.get();
-		a.t_loc = (const int *)test->g_loc.get();
+		a.t_contrib = (const uint2 *)test->g_contrib.get();
 		a.t_rows = (const int *)test->g_rows.get();
 		a.s_order = (const int *)trial->g_order.get();
 		a.s_col = (const int *)trial->g_col.get();

@@ -13,12 +13,15 @@
 // distance interval, or a fixed degree.
 //   * constant fields: the pair's entry is one store, as in collocation;
 //   * nodal fields: the 3x3 block of a pair belongs to nodes that other pairs
-//     share.  Each wave keeps an LDS accumulator of GAL_RMAX row nodes (the
-//     distinct corners of its tile) x GAL_CS column-node slots; a host-built
+//     share.  Each lane keeps, in LDS private to it, its element's 3 corners x
+//     GAL_CS column-node slots (plain read-add-write: shared accumulators with
+//     LDS atomics serialise on bank and address conflicts); a host-built
 //     schedule assigns the trial corners to slots (least recently used) and
-//     names the node a slot held before, whose column is then added to the
-//     result with global atomics.  Every column node of the chunk ends in one
-//     or a few such flushes instead of one atomic per pair and entry.
+//     names the node a slot held before.  That column is then reduced over the
+//     lanes whose elements share a row node (a per-tile contributor table,
+//     lane l owning local row l) and added to the result with one global
+//     atomic per row.  Every column node of the chunk ends in one or a few
+//     such flushes instead of one atomic per pair and entry.
 // Singular pairs (FP64, same mesh only): coincident, edge- and vertex-adjacent
 // element pairs, one wave per pair over the 512 / 1536 points of the
 // Duffy-type rule, rotated to the shared corners as
@@ -32,14 +35,15 @@ namespace nbk {
 
 constexpr int GAL_TILE = 64;    // test elements per wave tile
 constexpr int GAL_RMAX = 64;    // distinct nodes per wave tile (nodal fields)
-constexpr int GAL_CS = 16;      // column-node slots per wave (nodal fields)
+constexpr int GAL_CS = 8;       // column-node slots per lane (nodal fields)
 
 struct GalArgs {
 	int n_tiles, chunk;          // test wave tiles; trial sequence positions per chunk
 	int tn, sn;                  // element counts of test / trial mesh (SoA strides)
 	int same_mesh;
 	const int *t_elem;           // nodal: [n_tiles][64] test element of each lane (-1: idle)
-	const int *t_loc;            // nodal: local row of corners 0, 1, 2 (8 bits each)
+	const uint2 *t_contrib;      // nodal: [n_tiles][64] contributors lane * 4 + corner of each
+	                             //        local row, 8 bits each, 0xff: none
 	const int *t_rows;           // nodal: node of each local row (-1: unused)
 	const int *t_tri;            // test element nodes [3][tn]
 	const int *s_order;          // nodal: trial element at each sequence position
@@ -235,13 +239,29 @@ __device__ __forceinline__ void gal_eval_dynamic(const GalArgs &a, int e, int f,
 	}
 }
 
-// add one column slot of the wave's accumulator to the result and clear it;
-// lane l owns local row l
-__device__ __forceinline__ void gal_flush(float2 *acc, int lane, int slot, int col, int rnode, const GalArgs &a)
+// per-lane accumulator of corner c in column slot s: acc[(s * 3 + c) * 64 + lane]
+__device__ __forceinline__ float2 *gal_acc(float2 *acc, int slot, int corner, int lane)
 {
-	float2 *p = acc + slot * GAL_RMAX + lane;
-	const float2 v = *p;
-	*p = make_float2(0.0f, 0.0f);
+	return acc + (slot * 3 + corner) * GAL_TILE + lane;
+}
+
+// add one column slot to the result and clear it: lane l sums local row l
+// over its contributors, then every lane clears its own three entries
+__device__ __forceinline__ void gal_flush(float2 *acc, int lane, int slot, int col, int rnode, uint2 con,
+	const GalArgs &a)
+{
+	float2 v = make_float2(0.0f, 0.0f);
+#pragma unroll
+	for (int q = 0; q < 8; ++q) {
+		const unsigned c = ((q < 4 ? con.x : con.y) >> (8 * (q & 3))) & 255u;
+		if (c != 255u) {
+			const float2 t = *gal_acc(acc, slot, c & 3u, c >> 2);
+			v.x += t.x;
+			v.y += t.y;
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < 3; ++c) *gal_acc(acc, slot, c, lane) = make_float2(0.0f, 0.0f);
 	if (rnode >= 0 && (v.x != 0.0f || v.y != 0.0f)) {
 		const float re = a.alpha.x * v.x - a.alpha.y * v.y;
 		const float im = a.alpha.x * v.y + a.alpha.y * v.x;
@@ -254,13 +274,13 @@ __device__ __forceinline__ void gal_flush(float2 *acc, int lane, int slot, int c
 template <int FAM, int KM, int NDOF, int FIXQ>
 __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 {
-	__shared__ float2 acc_all[NDOF == 3 ? 4 * GAL_CS * GAL_RMAX : 1];
+	__shared__ float2 acc_all[NDOF == 3 ? 4 * GAL_CS * 3 * GAL_TILE : 1];
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	const int wg_tiles = (a.n_tiles + 3) >> 2;
 	const int tile = (int)(blockIdx.x % wg_tiles) * 4 + wave;
 	const int chunk = (int)(blockIdx.x / wg_tiles);
 	if (tile >= a.n_tiles) return;     // no block-wide barriers below
-	float2 *acc = acc_all + (NDOF == 3 ? wave * GAL_CS * GAL_RMAX : 0);
+	float2 *acc = acc_all + (NDOF == 3 ? wave * GAL_CS * 3 * GAL_TILE : 0);
 
 	int e;
 	if (NDOF == 3) e = a.t_elem[tile * GAL_TILE + lane];
@@ -273,12 +293,15 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 	if (KM & (DLPT | HSP)) { P.nxx = a.tnu[ee]; P.nxy = a.tnu[a.tn + ee]; P.nxz = a.tnu[2 * a.tn + ee]; }
 	int tv0 = -1, tv1 = -1, tv2 = -1;
 	if (a.same_mesh) { tv0 = a.t_tri[ee]; tv1 = a.t_tri[a.tn + ee]; tv2 = a.t_tri[2 * a.tn + ee]; }
-	int loc = 0, rnode = -1;
+	int rnode = -1;
+	uint2 con = make_uint2(0xffffffffu, 0xffffffffu);
 	if (NDOF == 3) {
-		loc = a.t_loc[tile * GAL_TILE + lane];
+		con = a.t_contrib[tile * GAL_TILE + lane];
 		rnode = a.t_rows[tile * GAL_TILE + lane];
 #pragma unroll
-		for (int s = 0; s < GAL_CS; ++s) acc[s * GAL_RMAX + lane] = make_float2(0.0f, 0.0f);
+		for (int s = 0; s < GAL_CS; ++s)
+#pragma unroll
+			for (int c = 0; c < 3; ++c) *gal_acc(acc, s, c, lane) = make_float2(0.0f, 0.0f);
 	}
 	GTest tpr[FIXQ > 0 ? FIXQ : 1];
 	if (FIXQ > 0) {
@@ -299,9 +322,9 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 		if (NDOF == 3) {
 			const int v0 = scol[4 * i], v1 = scol[4 * i + 1], v2 = scol[4 * i + 2], sl = scol[4 * i + 3];
 			sl0 = sl & 31; sl1 = (sl >> 5) & 31; sl2 = (sl >> 10) & 31;
-			if (v0 >= 0) gal_flush(acc, lane, sl0, v0, rnode, a);
-			if (v1 >= 0) gal_flush(acc, lane, sl1, v1, rnode, a);
-			if (v2 >= 0) gal_flush(acc, lane, sl2, v2, rnode, a);
+			if (v0 >= 0) gal_flush(acc, lane, sl0, v0, rnode, con, a);
+			if (v1 >= 0) gal_flush(acc, lane, sl1, v1, rnode, con, a);
+			if (v2 >= 0) gal_flush(acc, lane, sl2, v2, rnode, con, a);
 		}
 		const double cjx = sc[f], cjy = sc[a.sn + f], cjz = sc[2 * a.sn + f];
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
@@ -356,15 +379,16 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 				store_entry(a.out, (long long)f * a.ld + e, a.dtype, a.accumulate, re, im);
 			}
 		} else if (act) {
-			const int rl[3] = {loc & 255, (loc >> 8) & 255, (loc >> 16) & 255};
 			const int cl[3] = {sl0, sl1, sl2};
 #pragma unroll
 			for (int x = 0; x < 3; ++x)
 #pragma unroll
 				for (int y = 0; y < 3; ++y) {
-					float *p = (float *)(acc + cl[y] * GAL_RMAX + rl[x]);
-					__hip_atomic_fetch_add(p, blk[x][y].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-					__hip_atomic_fetch_add(p + 1, blk[x][y].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+					float2 *p = gal_acc(acc, cl[y], x, lane);
+					float2 v = *p;
+					v.x += blk[x][y].x;
+					v.y += blk[x][y].y;
+					*p = v;
 				}
 		}
 	}
@@ -372,7 +396,7 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 		const auto sfin = uni(a.s_fin);
 		for (int s = 0; s < GAL_CS; ++s) {
 			const int v = sfin[chunk * GAL_CS + s];
-			if (v >= 0) gal_flush(acc, lane, s, v, rnode, a);
+			if (v >= 0) gal_flush(acc, lane, s, v, rnode, con, a);
 		}
 	}
 }

@@ -21,7 +21,8 @@ namespace {
 
 constexpr int TILE = 64;   // test elements per wave
 constexpr int RMAX = 64;   // distinct nodes per wave tile
-constexpr int CS = 16;     // column-node slots per wave
+constexpr int KMAX = 8;    // elements of a wave tile sharing one node
+constexpr int CS = 8;      // column-node slots per wave
 
 // Morton key of a point quantised on a 1024^3 grid
 uint64_t spread3(uint64_t v)
@@ -35,44 +36,59 @@ uint64_t spread3(uint64_t v)
 }
 
 struct Tiles {
-	std::vector<int> elem, loc, rows;   // [n_tiles * TILE]
+	std::vector<int> elem, rows;        // [n_tiles * TILE]
+	std::vector<uint32_t> contrib;      // [n_tiles * TILE * 2]
 	int n_tiles = 0;
 };
 
 // greedy wave tiles over an element order: up to TILE elements whose corners
-// use at most RMAX distinct nodes
+// use at most RMAX distinct nodes, no node in more than KMAX of them.  Row r
+// of a tile lists its contributors (lane * 4 + corner) in 8 bytes, 0xff = none.
 Tiles make_tiles(const std::vector<int> &order, const int32_t *tri)
 {
 	Tiles t;
 	const int n = (int)order.size();
 	int pos = 0;
 	std::vector<int> rows;
+	std::vector<std::vector<int> > con;
 	while (pos < n) {
 		rows.clear();
+		con.clear();
 		int cnt = 0;
-		std::vector<int> locs;
 		while (pos + cnt < n && cnt < TILE) {
 			const int e = order[pos + cnt];
 			int l3[3];
-			std::vector<int> add;
+			int extra = 0;
+			bool full = false;
 			for (int a = 0; a < 3; ++a) {
 				const int v = tri[3 * e + a];
 				auto it = std::find(rows.begin(), rows.end(), v);
-				if (it != rows.end()) { l3[a] = (int)(it - rows.begin()); continue; }
-				auto jt = std::find(add.begin(), add.end(), v);
-				if (jt != add.end()) { l3[a] = (int)rows.size() + (int)(jt - add.begin()); continue; }
-				l3[a] = (int)(rows.size() + add.size());
-				add.push_back(v);
+				if (it == rows.end()) { l3[a] = -1; ++extra; continue; }
+				l3[a] = (int)(it - rows.begin());
+				if ((int)con[l3[a]].size() >= KMAX) full = true;
 			}
-			if ((int)(rows.size() + add.size()) > RMAX) break;
-			rows.insert(rows.end(), add.begin(), add.end());
-			locs.push_back(l3[0] | (l3[1] << 8) | (l3[2] << 16));
+			if (full || (int)rows.size() + extra > RMAX) break;
+			for (int a = 0; a < 3; ++a) {
+				if (l3[a] < 0) {
+					l3[a] = (int)rows.size();
+					rows.push_back(tri[3 * e + a]);
+					con.emplace_back();
+				}
+				con[l3[a]].push_back(cnt * 4 + a);
+			}
 			++cnt;
 		}
 		for (int l = 0; l < TILE; ++l) {
 			t.elem.push_back(l < cnt ? order[pos + l] : -1);
-			t.loc.push_back(l < cnt ? locs[l] : 0);
 			t.rows.push_back(l < (int)rows.size() ? rows[l] : -1);
+			uint32_t w[2] = {0xffffffffu, 0xffffffffu};
+			if (l < (int)rows.size())
+				for (int q = 0; q < (int)con[l].size(); ++q) {
+					w[q >> 2] &= ~(0xffu << (8 * (q & 3)));
+					w[q >> 2] |= (uint32_t)con[l][q] << (8 * (q & 3));
+				}
+			t.contrib.push_back(w[0]);
+			t.contrib.push_back(w[1]);
 		}
 		++t.n_tiles;
 		pos += cnt;
@@ -256,10 +272,10 @@ int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, 
 // order (the mesh's own, or Morton order of the centres when that packs the
 // wave tiles better), builds the wave tiles and the column-slot schedule of
 // the trial sequence in chunks of `chunk` elements.  Call with null arrays to
-// get n_tiles; then with arrays of n_tiles * 64 (elem, loc, rows), n (order),
-// 4 n (col) and ceil(n / chunk) * 16 (fin).
+// get n_tiles; then with arrays of n_tiles * 64 (elem, rows), n_tiles * 128
+// (contrib), n (order), 4 n (col) and ceil(n / chunk) * 8 (fin).
 int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk,
-	int32_t *order_out, int32_t *elem, int32_t *loc, int32_t *rows, int32_t *col, int32_t *fin)
+	int32_t *order_out, int32_t *elem, uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin)
 {
 	std::vector<int> mesh_order(n), morton(n);
 	for (int e = 0; e < n; ++e) mesh_order[e] = e;
@@ -290,7 +306,7 @@ int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk,
 	if (!order_out) return t.n_tiles;
 	std::copy(ord.begin(), ord.end(), order_out);
 	std::copy(t.elem.begin(), t.elem.end(), elem);
-	std::copy(t.loc.begin(), t.loc.end(), loc);
+	std::copy(t.contrib.begin(), t.contrib.end(), contrib);
 	std::copy(t.rows.begin(), t.rows.end(), rows);
 	schedule(ord, tri, chunk, col, fin);
 	return t.n_tiles;

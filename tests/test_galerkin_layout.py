@@ -14,7 +14,7 @@ from tests import bemref
 
 I32 = C.POINTER(C.c_int32)
 D = C.POINTER(C.c_double)
-TILE, CS = 64, 16
+TILE, CS, KMAX = 64, 8, 8
 
 
 @pytest.fixture(scope="module")
@@ -78,12 +78,13 @@ def _layout(raw, nodes, tris, chunk):
     c = np.ascontiguousarray(nodes[tris].mean(1).T)   # [3][n]
     tiles = raw.nbg_galerkin_layout(n, _i(tris), c.ctypes.data_as(D), chunk, None, None, None, None, None, None)
     order = np.zeros(n, np.int32)
-    elem, loc, rows = (np.zeros(tiles * TILE, np.int32) for _ in range(3))
+    elem, rows = (np.zeros(tiles * TILE, np.int32) for _ in range(2))
+    contrib = np.zeros(tiles * TILE * 2, np.uint32)
     col = np.zeros(4 * n, np.int32)
     fin = np.zeros(((n + chunk - 1) // chunk) * CS, np.int32)
-    assert raw.nbg_galerkin_layout(n, _i(tris), c.ctypes.data_as(D), chunk, _i(order), _i(elem), _i(loc), _i(rows),
-                                   _i(col), _i(fin)) == tiles
-    return tiles, order, elem, loc, rows, col.reshape(n, 4), fin.reshape(-1, CS)
+    assert raw.nbg_galerkin_layout(n, _i(tris), c.ctypes.data_as(D), chunk, _i(order), _i(elem),
+                                   contrib.ctypes.data_as(I32), _i(rows), _i(col), _i(fin)) == tiles
+    return tiles, order, elem, contrib.reshape(-1, 2), rows, col.reshape(n, 4), fin.reshape(-1, CS)
 
 
 @pytest.mark.parametrize("shuffle", [False, True])
@@ -91,7 +92,7 @@ def test_wave_tiles_cover_each_element_once(raw, shuffle):
     nodes, tris = _mesh(3)
     if shuffle:   # a mesh without locality: the Morton order takes over
         tris = np.ascontiguousarray(tris[np.random.default_rng(0).permutation(len(tris))])
-    tiles, order, elem, loc, rows, _, _ = _layout(raw, nodes, tris, 128)
+    tiles, order, elem, contrib, rows, _, _ = _layout(raw, nodes, tris, 128)
     assert sorted(order.tolist()) == list(range(len(tris)))
     used = elem[elem >= 0]
     assert sorted(used.tolist()) == list(range(len(tris)))
@@ -101,13 +102,20 @@ def test_wave_tiles_cover_each_element_once(raw, shuffle):
         r = rows[t * TILE:(t + 1) * TILE]
         nr = int((r >= 0).sum())
         assert np.all(r[:nr] >= 0) and np.all(r[nr:] == -1) and len(set(r[:nr].tolist())) == nr
+        # each (lane, corner) of the tile contributes to exactly the row of its node
+        seen = set()
         for l in range(TILE):
-            e = elem[t * TILE + l]
-            if e < 0:
-                continue
-            for a in range(3):
-                la = (loc[t * TILE + l] >> (8 * a)) & 255
-                assert la < nr and r[la] == tris[e][a]
+            bytes8 = [(int(contrib[t * TILE + l][q >> 2]) >> (8 * (q & 3))) & 255 for q in range(8)]
+            cs = [b for b in bytes8 if b != 255]
+            assert bytes8[len(cs):] == [255] * (8 - len(cs)) and len(cs) <= KMAX
+            if l >= nr:
+                assert not cs
+            for b in cs:
+                lane, a = b >> 2, b & 3
+                e = elem[t * TILE + lane]
+                assert a < 3 and e >= 0 and tris[e][a] == r[l]
+                seen.add((lane, a))
+        assert seen == {(l, a) for l in range(TILE) if elem[t * TILE + l] >= 0 for a in range(3)}
     # a mesh with locality fills its tiles
     if not shuffle:
         assert tiles == (len(tris) + TILE - 1) // TILE

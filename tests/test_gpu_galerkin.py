@@ -198,8 +198,10 @@ def test_transparent_sphere_fixture(nb):
 def test_full_size_rows_and_gauss_identity(nb, orc):
     """The Galerkin workload's ellipsoid (level 5: 10242 nodes, 20480
     elements), nodal fields, fixed 6 x 6 rule: full rows of 6 sampled nodes
-    against the oracle, and Gauss's identity on the Laplace double layer
-    (row sums = -1/2 mass row sums) over every row."""
+    against the oracle; Gauss's identity on the Laplace double layer (row sums
+    = -1/2 mass row sums) over every row, to the rule's accuracy on adjacent
+    pairs (2.4e-3 at worst here), and the row sums of the sampled rows equal
+    to the oracle's."""
     import torch
     from nihu_amd.mesh import ellipsoid
     nodes, tris = ellipsoid(5, seed=0)
@@ -224,10 +226,14 @@ def test_full_size_rows_and_gauss_identity(nb, orc):
     nb.assemble(m, m, D, 0, 0, 1, 0.0, pol=nb.policy(**gp), accumulate=False, formalism="galerkin_linear")
     rs_d = D.sum(dim=1).cpu().numpy()
     del D
+    ref_d = orc.assemble_galerkin(nodes, tris, 0, 0, 1, pol=orc.policy(**op), linear=True,
+                                  test_elems=elems)[sample].real.sum(1)
+    print("sampled DLP row sums", rs_d[sample], ref_d, flush=True)
+    assert np.abs(rs_d[sample] - ref_d).max() <= 1e-5 * np.abs(ref_d).max()
     Mm = nb.colmajor(nn, nn, "float64")
     nb.assemble(m, m, Mm, nb.NB_IDENTITY, 0, 0, 0.0, accumulate=False, formalism="galerkin_linear")
     ms = Mm.sum(dim=1).cpu().numpy()
     torch.cuda.synchronize()
     gauss = np.abs(rs_d + 0.5 * ms).max() / ms.max()
     print("Gauss identity max deviation %.2e" % gauss, flush=True)
-    assert gauss < 2e-3
+    assert gauss < 5e-3
