@@ -3,7 +3,8 @@
 // The names, argument meaning and error behaviour follow NiHu's hot-path API
 // (paths relative to the NiHu repository root):
 //   create_mesh / read_off_mesh          nihu/core/mesh.hpp:137-193, nihu/interface/read_off_mesh.hpp
-//   constant_view, dirac                  nihu/core/function_space.hpp:248-266,353-359
+//   constant_view, isoparametric_view, dirac
+//                                         nihu/core/function_space.hpp:248-266,353-359
 //   helmholtz_3d_{SLP,DLP,DLPt,HSP}_kernel, laplace_3d_*_kernel
 //                                         nihu/library/helmholtz_kernel.hpp, laplace_kernel.hpp
 //   create_integral_operator, identity_integral_operator, scalar * operator
@@ -15,9 +16,11 @@
 // matrix_block::operator+= does.  The result may be any column-major matrix
 // type with data(), rows() and cols() (Eigen::Matrix qualifies) holding
 // std::complex<double>, std::complex<float>, double or float, on the host, or
-// a nihu_amd::device_matrix on the GPU.  Collocation (dirac(constant_view)
-// test, constant_view trial) is implemented; combinations the engine lacks
-// throw std::logic_error.  Errors map to NiHu's exception types:
+// a nihu_amd::device_matrix on the GPU.  Implemented formalisms: collocation
+// (dirac(constant_view) test, constant_view trial) and Galerkin
+// (constant_view x constant_view, isoparametric_view x isoparametric_view,
+// dofs = elements or mesh nodes); other test / trial combinations throw
+// std::logic_error.  Errors map to NiHu's exception types:
 // std::out_of_range (quadrature degree), std::invalid_argument,
 // std::logic_error, std::bad_alloc and std::runtime_error.
 
@@ -137,9 +140,12 @@ inline mesh read_off_mesh(std::string const &fname, tria_1_tag)
 
 // ------------------------------------------------------------------ spaces
 
+enum space_kind { constant_kind = 0, isoparametric_kind = 1, dirac_constant_kind = 2 };
+
 // constant_view: one dof per element (the element id)
 class constant_space {
 public:
+	static const int kind = constant_kind;
 	explicit constant_space(mesh const &m) : m_mesh(m) {}
 	mesh const &get_mesh() const { return m_mesh; }
 	int32_t get_num_dofs() const { return m_mesh.num_elements(); }
@@ -148,9 +154,23 @@ private:
 	mesh m_mesh;
 };
 
+// isoparametric_view: linear shape functions, one dof per mesh node
+// (get_num_points, nihu/core/function_space.hpp:156-165)
+class isoparametric_space {
+public:
+	static const int kind = isoparametric_kind;
+	explicit isoparametric_space(mesh const &m) : m_mesh(m) {}
+	mesh const &get_mesh() const { return m_mesh; }
+	int32_t get_num_dofs() const { return m_mesh.num_nodes(); }
+
+private:
+	mesh m_mesh;
+};
+
 // dirac(space): collocation at the element centres of the underlying space
 class dirac_space {
 public:
+	static const int kind = dirac_constant_kind;
 	explicit dirac_space(constant_space const &s) : m_space(s) {}
 	mesh const &get_mesh() const { return m_space.get_mesh(); }
 	int32_t get_num_dofs() const { return m_space.get_num_dofs(); }
@@ -160,7 +180,17 @@ private:
 };
 
 inline constant_space constant_view(mesh const &m) { return constant_space(m); }
+inline isoparametric_space isoparametric_view(mesh const &m) { return isoparametric_space(m); }
 inline dirac_space dirac(constant_space const &s) { return dirac_space(s); }
+
+// a space as the assembly sees it: mesh and kind
+struct space_ref {
+	mesh m;
+	int kind;
+};
+
+template <class Space>
+space_ref make_space_ref(Space const &s) { return space_ref{s.get_mesh(), Space::kind}; }
 
 // ------------------------------------------------------------------ kernels
 
@@ -217,21 +247,30 @@ integral_operator operator*(Scalar const &s, integral_operator const &op)
 
 class integral_transform {
 public:
-	integral_transform(integral_operator const &op, constant_space const &trial) : m_op(op), m_trial(trial) {}
+	integral_transform(integral_operator const &op, space_ref const &trial) : m_op(op), m_trial(trial) {}
 	integral_operator const &op() const { return m_op; }
-	constant_space const &trial() const { return m_trial; }
+	space_ref const &trial() const { return m_trial; }
 
 private:
 	integral_operator m_op;
-	constant_space m_trial;
+	space_ref m_trial;
 };
 
 template <class Space>
 integral_transform integral_operator::operator[](Space const &trial) const
 {
-	static_assert(std::is_same<Space, constant_space>::value,
-		"the engine implements constant_view trial spaces (collocation)");
-	return integral_transform(*this, trial);
+	static_assert(std::is_same<Space, constant_space>::value || std::is_same<Space, isoparametric_space>::value,
+		"trial spaces: constant_view or isoparametric_view");
+	return integral_transform(*this, make_space_ref(trial));
+}
+
+// the ABI formalism of a test / trial pair of spaces, or -1
+inline int formalism_of(int test_kind, int trial_kind)
+{
+	if (test_kind == dirac_constant_kind && trial_kind == constant_kind) return NB_COLLOCATION;
+	if (test_kind == constant_kind && trial_kind == constant_kind) return NB_GALERKIN;
+	if (test_kind == isoparametric_kind && trial_kind == isoparametric_kind) return NB_GALERKIN_LINEAR;
+	return -1;
 }
 
 // test * transform: a weighted residual; sums of residuals accumulate in order
@@ -239,9 +278,15 @@ class weighted_residual {
 public:
 	struct term {
 		integral_transform tr;
-		dirac_space test;
+		space_ref test;
 	};
-	weighted_residual(dirac_space const &test, integral_transform const &tr) { m_terms.push_back(term{tr, test}); }
+	weighted_residual(space_ref const &test, integral_transform const &tr)
+	{
+		if (formalism_of(test.kind, tr.trial().kind) < 0)
+			throw std::logic_error("test and trial spaces: dirac(constant) x constant, constant x constant or "
+				"isoparametric x isoparametric");
+		m_terms.push_back(term{tr, test});
+	}
 	std::vector<term> const &terms() const { return m_terms; }
 	weighted_residual operator+(weighted_residual const &o) const
 	{
@@ -254,17 +299,13 @@ private:
 	std::vector<term> m_terms;
 };
 
-inline weighted_residual operator*(dirac_space const &test, integral_transform const &tr)
-{
-	return weighted_residual(test, tr);
-}
-
 template <class Space>
-weighted_residual operator*(Space const &, integral_transform const &)
+weighted_residual operator*(Space const &test, integral_transform const &tr)
 {
-	static_assert(std::is_same<Space, dirac_space>::value,
-		"only the collocational formalism (dirac test space) is implemented by this engine");
-	throw std::logic_error("unsupported formalism");
+	static_assert(std::is_same<Space, dirac_space>::value || std::is_same<Space, constant_space>::value ||
+		std::is_same<Space, isoparametric_space>::value, "test spaces: dirac(constant_view), constant_view, "
+		"isoparametric_view");
+	return weighted_residual(make_space_ref(test), tr);
 }
 
 // ------------------------------------------------------------------ results
@@ -329,12 +370,12 @@ Res &operator<<(Res &result, weighted_residual const &wr)
 {
 	nb_out o = describe(result);
 	for (auto const &t : wr.terms()) {
-		const nb_mesh *test = t.test.get_mesh().handle();
-		const nb_mesh *trial = t.tr.trial().get_mesh().handle();
+		const nb_mesh *test = t.test.m.handle();
+		const nb_mesh *trial = t.tr.trial().m.handle();
 		const int same = test == trial;
 		const std::complex<double> s = t.tr.op().scalar();
-		check(nb_assemble(test, trial, same, NB_COLLOCATION, t.tr.op().kernel(), nb_scalar{s.real(), s.imag()},
-			&default_policy(), &o, nullptr));
+		check(nb_assemble(test, trial, same, formalism_of(t.test.kind, t.tr.trial().kind), t.tr.op().kernel(),
+			nb_scalar{s.real(), s.imag()}, &default_policy(), &o, nullptr));
 	}
 	if (o.location == NB_DEVICE) check(nb_synchronize(nullptr));
 	return result;

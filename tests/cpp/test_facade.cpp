@@ -1,7 +1,8 @@
 // Known-answer tests of the C++ front end, written the way NiHu's own
 // tutorial tests are (nihu/tutorial/helmholtz_3d_transparent_standalone.cpp:17-92,
-// nihu/test/stallone_unit/helmholtz_3d_pulsating_sphere_test.cpp:28-73), on the
-// GPU through the C ABI.
+// collocation and Galerkin; nihu/test/stallone_unit/helmholtz_3d_pulsating_sphere_test.cpp:28-73,
+// Galerkin with constant fields), on the GPU through the C ABI, plus the same
+// answers with isoparametric (nodal) fields.
 //
 //   test_facade <mesh.off>
 // Prints the errors and exits non-zero if one is above its bound.
@@ -48,8 +49,23 @@ struct centre_normal {
 	double c[3], n[3];
 };
 
-static std::vector<centre_normal> geometry(nihu_amd::mesh const &m)
+// sample points of a space's dofs: element centres and normals, or for
+// nodal dofs the nodes with the sphere's radial normal
+static std::vector<centre_normal> geometry(nihu_amd::isoparametric_space const &s)
 {
+	auto const &X = s.get_mesh().nodes();
+	std::vector<centre_normal> g(s.get_num_dofs());
+	for (size_t v = 0; v < g.size(); ++v) {
+		double r = std::sqrt(X[3 * v] * X[3 * v] + X[3 * v + 1] * X[3 * v + 1] + X[3 * v + 2] * X[3 * v + 2]);
+		for (int d = 0; d < 3; ++d) { g[v].c[d] = X[3 * v + d]; g[v].n[d] = X[3 * v + d] / r; }
+	}
+	return g;
+}
+
+template <class Space>
+static std::vector<centre_normal> geometry(Space const &s)
+{
+	nihu_amd::mesh const &m = s.get_mesh();
 	std::vector<centre_normal> g(m.num_elements());
 	auto const &X = m.nodes();
 	auto const &T = m.elements();
@@ -83,7 +99,7 @@ static double tester(TestSpace const &test_space, TrialSpace const &trial_space)
 	std::vector<cplx> q0(N, 0.0), p0(N, 0.0);
 	const double x0[2][3] = {{.2, .3, .1}, {.3, .2, .2}};
 	const double A[2] = {1.0, -1.0};
-	auto g = geometry(trial_space.get_mesh());
+	auto g = geometry(trial_space);
 	for (size_t k = 0; k < N; ++k) {
 		for (int s = 0; s < 2; ++s) {
 			double rv[3], r = 0, rdn = 0;
@@ -114,11 +130,10 @@ static double tester(TestSpace const &test_space, TrialSpace const &trial_space)
 	return std::sqrt(e / nrm);
 }
 
-// pulsating sphere: p = -r/(1+ikr) q for a unit normal velocity, here collocation
-static double pulsating(nihu_amd::mesh const &m)
+// pulsating sphere: p = -r/(1+ikr) q for a unit normal velocity
+template <class TestSpace, class TrialSpace>
+static double pulsating(TestSpace const &test_space, TrialSpace const &trial_space)
 {
-	auto trial_space = nihu_amd::constant_view(m);
-	auto test_space = nihu_amd::dirac(trial_space);
 	const size_t N = trial_space.get_num_dofs();
 	double k = 1.0;
 	cMatrix L(N, N), M(N, N), I(N, N);
@@ -191,9 +206,46 @@ int main(int argc, char *argv[])
 	std::printf("Collocation transparent error: %g\n", coll_error);
 	bad |= !(coll_error < 0.05);
 
-	double puls = pulsating(mesh);
-	std::printf("Pulsating sphere relative error: %g\n", puls);
+	double puls = pulsating(nihu_amd::dirac(trial_space), trial_space);
+	std::printf("Pulsating sphere relative error (collocation): %g\n", puls);
 	bad |= !(puls < 0.05);
+
+	double gal_error = tester(trial_space, trial_space);
+	std::printf("Galerkin transparent error: %g\n", gal_error);
+	bad |= !(gal_error < 0.05);
+
+	double gal_puls = pulsating(trial_space, trial_space);
+	std::printf("Pulsating sphere relative error (Galerkin): %g\n", gal_puls);
+	bad |= !(gal_puls < 0.05);
+
+	auto const &iso_space = nihu_amd::isoparametric_view(mesh);
+	double iso_error = tester(iso_space, iso_space);
+	std::printf("Galerkin isoparametric transparent error: %g\n", iso_error);
+	bad |= !(iso_error < 0.05);
+
+	double iso_puls = pulsating(iso_space, iso_space);
+	std::printf("Pulsating sphere relative error (Galerkin isoparametric): %g\n", iso_puls);
+	bad |= !(iso_puls < 0.05);
+
+	// a hypersingular Galerkin operator on one mesh has no valid singular rule
+	try {
+		nihu_amd::cMatrix D(iso_space.get_num_dofs(), iso_space.get_num_dofs());
+		D << iso_space * nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_HSP_kernel<double>(1.0))[iso_space];
+		std::printf("no exception for Galerkin HSP\n");
+		bad = 1;
+	} catch (std::runtime_error const &ex) {
+		std::printf("runtime_error as expected: %s\n", ex.what());
+	}
+
+	// mixed spaces are not a formalism of the engine
+	try {
+		nihu_amd::cMatrix X(trial_space.get_num_dofs(), iso_space.get_num_dofs());
+		X << trial_space * nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(1.0))[iso_space];
+		std::printf("no exception for constant x isoparametric\n");
+		bad = 1;
+	} catch (std::logic_error const &ex) {
+		std::printf("logic_error as expected: %s\n", ex.what());
+	}
 
 	double bm = burton_miller(mesh);
 	std::printf("Burton-Miller composition vs fused assembly: %g\n", bm);

@@ -1,6 +1,8 @@
 """The C++ front end (nihu_amd/include/nihu_amd/nihu.hpp) on the GPU: NiHu-style
 known-answer programs built against libnbem.so, on the reference's
-sphere_1_3.off mesh fixture (218 nodes, 432 triangles)."""
+sphere_1_3.off mesh fixture (218 nodes, 432 triangles): transparent and
+pulsating-sphere answers in collocation, constant Galerkin and isoparametric
+Galerkin, and NiHu's exceptions."""
 
 import os
 import subprocess
@@ -20,3 +22,8 @@ def test_cpp_facade_known_answers():
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "out_of_range as expected" in r.stdout
+    assert "runtime_error as expected: Invalid general singular quadrature" in r.stdout
+    assert "logic_error as expected" in r.stdout
+    for line in ("Galerkin transparent error", "Pulsating sphere relative error (Galerkin)",
+                 "Galerkin isoparametric transparent error"):
+        assert line in r.stdout
