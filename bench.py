@@ -51,6 +51,18 @@ def read_traffic(path):
     return float(m.group(1)) if m else None
 
 
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of nbytes: the kernel whose HBM counters
+    calibrate FETCH_SIZE / WRITE_SIZE (it reads and writes nbytes)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy kernel: %d bytes read, %d bytes written" % (nbytes, nbytes))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,13 +126,7 @@ def main():
     del cls
 
     if args.profile_copy_bytes > 0:
-        src = torch.ones(args.profile_copy_bytes // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy kernel: %d bytes read, %d bytes written" % (args.profile_copy_bytes, args.profile_copy_bytes))
-        del src, dst
+        copy_calibration(args.profile_copy_bytes)
 
     for w in range(args.warmup):
         nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
@@ -246,6 +252,8 @@ def run_galerkin(args):
     mesh = nbem.Mesh(nodes, tris)
     L, M = nbem.colmajor(nn, nn), nbem.colmajor(nn, nn)
     stream = torch.cuda.current_stream()
+    if args.profile_copy_bytes > 0:
+        copy_calibration(args.profile_copy_bytes)
 
     def step():
         nbem.assemble(mesh, mesh, L, 1, 0, 0, k, pol=pol, accumulate=False, stream=stream,
