@@ -165,8 +165,9 @@ struct PairCtx {
 //   s[0] = sum E                    SLP   = s[0]
 //   s[1] = sum Q   (DLP group)      DLP   = -a s[1]
 //   s[2] = sum Q b                  DLPt  = s[2]
-//   s[3] = sum (3Q - th^2 G) b/r^2  HSP   = nx.ny sum Q - a s[3]
+//   s[3] = sum Q b/r^2              HSP   = nx.ny sum Q - a (3 s[3] - k^2 s[5])
 //   s[4] = sum Q   (HSP group, when it is evaluated apart from DLP)
+//   s[5] = sum E b/r^2
 // with, per quadrature point y = c_j + off and r = y - x (|r| = r):
 //   E = w e^{-i th} / r,  G = E / r^2,  Q = G (1 + i th),  th = k r,
 //   a = r.n_y = (c_j - x).n_y (off lies in the trial plane), b = r.n_x,
@@ -174,6 +175,7 @@ struct PairCtx {
 // DLPt e(-1-ikr)/(4 pi r^2) (-r.nx)/r and
 // HSP e/(4 pi r^3)[(3+3ikr-(kr)^2) rdnx rdny + (1+ikr) nx.ny]
 // (nihu/library/helmholtz_kernel.hpp:169-228, normal_derivative_kernel.hpp:114-257);
+// since th^2 G = k^2 E, the (kr)^2 term is k^2 times a sum of E;
 // the weights w carry J w_q / (4 pi).
 // r^2 = d^2 + u with u = 2 (c_j - x).off + |off|^2, and the phase is
 // t0 + (k/2pi) u / (r + d) revolutions: t0 is the pair's FP64-reduced phase of
@@ -182,7 +184,7 @@ struct PairCtx {
 // NQ > 0: compile-time point count (fully unrolled).
 template <int FAM, int MASK, int NQ>
 __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt, const PairCtx &p,
-	float k_revf, float k, float2 (&s)[5])
+	float k_revf, float k, float2 (&s)[6])
 {
 	const int nq = NQ > 0 ? NQ : nq_rt;
 	const auto qf = uni((const float *)qp_in);
@@ -218,11 +220,12 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 				if (MASK & DLPT) { s[2].x = __builtin_fmaf(Qr, b, s[2].x); s[2].y = __builtin_fmaf(Qi, b, s[2].y); }
 				if (MASK & HSP) {
 					const float w2 = b * ri2;
-					const float th2 = th * th;
-					const float hr = __builtin_fmaf(3.0f, Qr, -th2 * Gr);
-					const float hi = __builtin_fmaf(3.0f, Qi, -th2 * Gi);
-					s[3].x = __builtin_fmaf(hr, w2, s[3].x);
-					s[3].y = __builtin_fmaf(hi, w2, s[3].y);
+					s[3].x = __builtin_fmaf(Qr, w2, s[3].x);
+					s[3].y = __builtin_fmaf(Qi, w2, s[3].y);
+					if (FAM == HELM) {
+						s[5].x = __builtin_fmaf(ec, w2, s[5].x);
+						s[5].y = __builtin_fmaf(es, w2, s[5].y);
+					}
 				}
 			}
 		}
@@ -234,7 +237,7 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 // the table pointer stays wave-uniform.
 template <int FAM, int MASK, int ORDER, int NQ>
 __device__ __forceinline__ void eval_group(const RegArgs &a, int j, double d2, bool act, const PairCtx &p,
-	float2 (&s)[5])
+	float2 (&s)[6])
 {
 	int deg = a.fixed_degree >= 0 ? a.fixed_degree : a.field_complexity + table_degree<ORDER>(d2, a.thr, a.sn, j);
 	deg = act ? deg : -1;
@@ -296,8 +299,8 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			if (KM & HSP) nn = p.nxx * nyx + p.nxy * nyy + p.nxz * nyz;
 		}
 		if (KM & (DLPT | HSP)) p.b0 = dcx * p.nxx + dcy * p.nxy + dcz * p.nxz;
-		float2 s[5] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
-			make_float2(0.f, 0.f)};
+		float2 s[6] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
+			make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
 		bool hsp_apart = false;
 		if (FIXQ > 0) {
 			if (act) accumulate_points<FAM, KM, FIXQ>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s);
@@ -316,7 +319,9 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			K[1] = make_float2(-av * s[1].x, -av * s[1].y);
 			K[2] = s[2];
 			const float2 sq = (hsp_apart || !(KM & DLP)) ? s[4] : s[1];
-			K[3] = make_float2(nn * sq.x - av * s[3].x, nn * sq.y - av * s[3].y);
+			const float kk = FAM == HELM ? a.k * a.k : 0.0f;
+			const float hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x), hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
+			K[3] = make_float2(nn * sq.x - av * hr, nn * sq.y - av * hi);
 			const long long idx = (long long)j * a.ld + i;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;

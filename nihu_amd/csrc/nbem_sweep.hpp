@@ -70,8 +70,11 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 			const float av = dcx * nyx + dcy * nyy + dcz * nyz;
 			const float nn = nxx * nyx + nxy * nyy + nxz * nyz;
 			const float b0 = dcx * nxx + dcy * nxy + dcz * nxz;
-			// wavenumber-independent geometry of the points
-			float gr[Q], gdel[Q], gg[Q], gri2[Q], gb[Q], gw2[Q];
+			// wavenumber-independent geometry of the points: the phase
+			// correction and, per kernel sum, the weight W that multiplies
+			// e^{-i th} (g = w/r, G-type sums carry 1/r^2, Q = G (1 + i k r)
+			// splits into a G sum and a k r G sum, b = r.n_x, w2 = b / r^2)
+			float gdel[Q], W[8][Q];
 			const auto qf = uni((const float *)(a.qt + (size_t)j * Q));
 #pragma unroll
 			for (int q = 0; q < Q; ++q) {
@@ -80,12 +83,19 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 				const float r2 = d2f + u;
 				const float rinv = __builtin_amdgcn_rsqf(r2);
 				const float r = r2 * rinv;
-				gr[q] = r;
 				gdel[q] = u * __builtin_amdgcn_rcpf(r + df);
-				gg[q] = w * rinv;
-				gri2[q] = rinv * rinv;
-				gb[q] = __builtin_fmaf(oz, nxz, __builtin_fmaf(oy, nxy, __builtin_fmaf(ox, nxx, b0)));
-				gw2[q] = gb[q] * gri2[q];
+				const float g = w * rinv;
+				const float gi2 = g * (rinv * rinv);
+				const float b = __builtin_fmaf(oz, nxz, __builtin_fmaf(oy, nxy, __builtin_fmaf(ox, nxx, b0)));
+				const float w2 = b * (rinv * rinv);
+				W[0][q] = g;              // SLP:  sum E
+				W[1][q] = gi2;            // sum G
+				W[2][q] = gi2 * r;        // sum r G
+				W[3][q] = gi2 * b;        // sum G b
+				W[4][q] = W[3][q] * r;    // sum r G b
+				W[5][q] = gi2 * w2;       // sum G w2
+				W[6][q] = W[5][q] * r;    // sum r G w2
+				W[7][q] = g * w2;         // sum E w2
 			}
 			for (int fb = 0; fb < KB; ++fb) {
 				const int f = f0 + fb;
@@ -94,26 +104,27 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 				td -= rint(td);
 				const float t0 = (float)td;
 				const float k = kf[f], kr = krevf[f];
-				float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+				// A[m] = sum_q W[m][q] (cos t_q, sin t_q); e^{-i th} = (cos, -sin)
+				float2 A[8];
+#pragma unroll
+				for (int m = 0; m < 8; ++m) A[m] = make_float2(0.f, 0.f);
 #pragma unroll
 				for (int q = 0; q < Q; ++q) {
 					const float t = __builtin_fmaf(kr, gdel[q], t0);
-					const float ec = gg[q] * __builtin_amdgcn_cosf(t);
-					const float es = -gg[q] * __builtin_amdgcn_sinf(t);
-					const float th = k * gr[q];
-					s0.x += ec; s0.y += es;
-					const float Gr = ec * gri2[q], Gi = es * gri2[q];
-					const float Qr = __builtin_fmaf(-th, Gi, Gr);
-					const float Qi = __builtin_fmaf(th, Gr, Gi);
-					s1.x += Qr; s1.y += Qi;
-					s2.x = __builtin_fmaf(Qr, gb[q], s2.x);
-					s2.y = __builtin_fmaf(Qi, gb[q], s2.y);
-					const float th2 = th * th;
-					const float hr = __builtin_fmaf(3.0f, Qr, -th2 * Gr);
-					const float hi = __builtin_fmaf(3.0f, Qi, -th2 * Gi);
-					s3.x = __builtin_fmaf(hr, gw2[q], s3.x);
-					s3.y = __builtin_fmaf(hi, gw2[q], s3.y);
+					const float c = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
+#pragma unroll
+					for (int m = 0; m < 8; ++m) {
+						A[m].x = __builtin_fmaf(W[m][q], c, A[m].x);
+						A[m].y = __builtin_fmaf(W[m][q], sn, A[m].y);
+					}
 				}
+				// complex sums: S_m = (A_m.x, -A_m.y); sum Q = S1 + i k S2 etc.
+				const float2 s0 = make_float2(A[0].x, -A[0].y);
+				const float2 s1 = make_float2(A[1].x + k * A[2].y, -A[1].y + k * A[2].x);
+				const float2 s2 = make_float2(A[3].x + k * A[4].y, -A[3].y + k * A[4].x);
+				const float2 sqw = make_float2(A[5].x + k * A[6].y, -A[5].y + k * A[6].x);
+				const float kk = k * k;
+				const float2 s3 = make_float2(3.0f * sqw.x - kk * A[7].x, 3.0f * sqw.y + kk * A[7].y);
 				if (act) {
 					// SLP = s0, DLP = -a s1, DLPt = s2, HSP = nn s1 - a s3
 					const float2 al = make_float2(alpha[2 * f], alpha[2 * f + 1]);
