@@ -51,6 +51,35 @@ def read_traffic(path):
     return float(m.group(1)) if m else None
 
 
+def surface_field_error(nodes, tris, k, A, B, Ar, Br):
+    """Plane-wave scattering from the rigid surface, A p = B q with
+    q = i k (d.n) e^{-ik d.x} at the element centres (seeded direction d),
+    solved in complex128 on the GPU once with the engine's matrices and once
+    with the oracle's; returns the relative 2-norm and max error of p."""
+    import numpy as np
+    import torch
+    X = nodes[tris]
+    c = X.mean(1)
+    nv = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+    nu = nv / np.linalg.norm(nv, axis=1)[:, None]
+    d = np.random.default_rng(7).standard_normal(3)
+    d /= np.linalg.norm(d)
+    q = torch.from_numpy((1j * k) * (nu @ d) * np.exp(-1j * k * (c @ d))).cuda()
+
+    def solve(Am, Bm):
+        Am = Am.to(torch.complex128)
+        rhs = Bm.to(torch.complex128) @ q
+        return torch.linalg.solve(Am, rhs.unsqueeze(1)).squeeze(1)
+
+    pg = solve(A, B).cpu().numpy()
+    torch.cuda.empty_cache()
+    pr = solve(torch.from_numpy(Ar).cuda(), torch.from_numpy(Br).cuda()).cpu().numpy()
+    torch.cuda.empty_cache()
+    return {"surface_field_rel": float(np.linalg.norm(pg - pr) / np.linalg.norm(pr)),
+            "surface_field_max_abs": float(np.abs(pg - pr).max()),
+            "surface_field_solve": "A p = B q, plane wave, complex128 LU on the GPU for both matrix sets"}
+
+
 def copy_calibration(nbytes):
     """One plain device-to-device copy of nbytes: the kernel whose HBM counters
     calibrate FETCH_SIZE / WRITE_SIZE (it reads and writes nbytes)."""
@@ -71,7 +100,9 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--ka", type=float, default=10.0)
     ap.add_argument("--policy", choices=["config", "nihu"], default="config")
-    ap.add_argument("--cpu-rows", type=int, default=512, help="rows of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=-1,
+                    help="CPU baseline: -1 the whole matrices (default; also gives full-matrix and solved-field "
+                         "errors), N > 0 a sample of N rows, 0 skip")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
@@ -171,30 +202,43 @@ def main():
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": read_traffic(args.traffic_file),
                         "kernel": "regular_kernel<HELM,ALL4,OUT_BM>", "kernel_ms": round(reg_ms, 4)}
 
-    # accuracy on sampled rows and CPU baseline on the same rows
+    # CPU baseline (the FP64 oracle) and accuracy against it
     cpu = None
     err = None
-    if args.cpu_rows > 0:
+    if args.cpu_rows != 0:
         from oracle import oracle
-        rows = np.sort(np.random.default_rng(123).choice(n, min(args.cpu_rows, n), replace=False)).astype(np.int32)
+        full = args.cpu_rows < 0 or args.cpu_rows >= n
+        rows = None if full else np.sort(np.random.default_rng(123).choice(n, args.cpu_rows, replace=False)).astype(
+            np.int32)
+        nr = n if full else len(rows)
         opol = oracle.policy(rule=pol.rule, fixed_degree=pol.fixed_degree, near_mode=pol.near_mode,
                              near_limit=pol.near_limit)
         threads = oracle.num_threads()
+        log("cpu baseline: oracle on %d rows, %d threads" % (nr, threads))
         tc = time.perf_counter()
         Ar, Br = oracle.assemble_bm(nodes, tris, k, alpha=alpha, pol=opol, rows=rows, threads=threads)
         dt = time.perf_counter() - tc
-        cpu_value = 4 * len(rows) * n / dt
+        cpu_value = 4 * nr * n / dt
         cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
-               "sample": "%d of %d rows of A and B (4 operators, all %d columns), FP64 oracle restating NiHu's "
-                         "pair algorithm, OpenMP over rows, %.1f s wall" % (len(rows), n, n, dt)}
-        ri = torch.from_numpy(rows.astype(np.int64)).cuda()
-        Ag, Bg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy()
+               "sample": ("the whole workload (A and B, all %d rows; 4 operators)" % n if full else
+                          "%d of %d rows of A and B (4 operators, all %d columns)" % (nr, n, n)) +
+                         ", FP64 oracle restating NiHu's pair algorithm, OpenMP over rows, %.1f s wall" % dt}
+        log("cpu baseline: %.4g entries/s on %d threads (%.1fx faster on GPU)" % (cpu_value, threads,
+                                                                                 value / cpu_value))
+        if full:
+            Ag, Bg = A.cpu().numpy(), B.cpu().numpy()
+        else:
+            ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+            Ag, Bg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy()
         ea = float(np.linalg.norm(Ag - Ar) / np.linalg.norm(Ar))
         eb = float(np.linalg.norm(Bg - Br) / np.linalg.norm(Br))
-        err = {"rows": len(rows), "A_rel_fro": ea, "B_rel_fro": eb,
+        err = {"rows": nr, "A_rel_fro": ea, "B_rel_fro": eb,
                "A_max_abs": float(np.abs(Ag - Ar).max()), "B_max_abs": float(np.abs(Bg - Br).max())}
-        log("sampled-row error vs oracle: %s" % err)
-        log("cpu baseline: %.4g entries/s on %d threads (%.1fx faster on GPU)" % (cpu_value, threads, value / cpu_value))
+        del Ag, Bg
+        if full:
+            err.update(surface_field_error(nodes, tris, k, A, B, Ar, Br))
+        del Ar, Br
+        log("error vs oracle: %s" % err)
 
     out = {
         "metric": "matrix entries/s, Helmholtz 3D coll. 20480 tris",
@@ -300,10 +344,11 @@ def run_galerkin(args):
                 "algorithmic_flops": flops}
 
     cpu = err = None
-    if args.cpu_rows > 0:
+    if args.cpu_rows != 0:
         from oracle import oracle
         rng = np.random.default_rng(123)
-        sample = np.sort(rng.choice(nn, max(1, args.cpu_rows // 16), replace=False))
+        n_sample = 32 if args.cpu_rows < 0 else max(1, args.cpu_rows // 16)
+        sample = np.sort(rng.choice(nn, n_sample, replace=False))
         elems = np.unique(np.nonzero(np.isin(tris, sample))[0]).astype(np.int32)
         opol = oracle.policy(rule=1, fixed_degree=4, near_mode=2, near_limit=1.5)
         threads = oracle.num_threads()
