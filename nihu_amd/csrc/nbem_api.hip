@@ -246,7 +246,7 @@ struct nb_mesh {
 	DevPtr d_c, d_nuf, d_nu64, d_X, d_J;
 	int thr_mode = -1;
 	double thr_limit = 0.0;
-	DevPtr d_thr;
+	DevPtr d_thr, d_rec;
 	DevPtr d_qt[nbk::NSLOT];
 	int qn[nbk::NSLOT] = {0};
 	std::vector<std::unique_ptr<NearList> > near_cache;
@@ -281,6 +281,24 @@ int ensure_thresholds(nb_mesh *m, int mode, double limit)
 	}
 	int s = upload(m->d_thr, thr);
 	if (s) return s;
+	// the same thresholds, the centre, normal and one-point weight per element
+	// in one record (the regular pass reads it through the scalar cache); the
+	// one-point weight assumes Dunavant degrees 0 and 1 are the centroid rule
+	double xi[4], eta[4], w[4];
+	for (int deg = 0; deg <= 1; ++deg)
+		if (nbg_dunavant(deg, xi, eta, w, 4) != 1 || xi[0] != 1. / 3. || eta[0] != 1. / 3. || w[0] != 0.5)
+			return fail(NB_ERR_RUNTIME, "Dunavant degrees 0 and 1 must be the centroid rule");
+	const double four_pi = 4.0 * 3.141592653589793238462643383279502884;
+	std::vector<nbk::TrialRec> rec(n);
+	for (int e = 0; e < n; ++e) {
+		nbk::TrialRec &r = rec[e];
+		std::memset(&r, 0, sizeof(r));
+		for (int t = 0; t < nbk::NTHR; ++t) r.thr[t] = thr[(size_t)t * n + e];
+		for (int d = 0; d < 3; ++d) { r.c[d] = m->c[(size_t)d * n + e]; r.nu[d] = (float)m->nu[(size_t)d * n + e]; }
+		r.w1 = (float)(0.5 * m->J[e] / four_pi);
+	}
+	if ((s = upload(m->d_rec, rec))) return s;
+	if (m->thr_mode < 0) m->bytes += (int64_t)n * (nbk::NTHR * 8 + sizeof(nbk::TrialRec));
 	m->thr_mode = mode;
 	m->thr_limit = limit;
 	return NB_OK;
@@ -638,6 +656,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	a.sc = (const double *)trial->d_c.get();
 	a.snu = (const float *)trial->d_nuf.get();
 	a.thr = (const double *)trial->d_thr.get();
+	a.rec = (const nbk::TrialRec *)trial->d_rec.get();
 	for (int d = 0; d < nbk::NSLOT; ++d) { a.qt[d] = (const nbk::QPoint *)trial->d_qt[d].get(); a.qn[d] = trial->qn[d]; }
 	a.fixed_degree = jb.pol.rule == NB_RULE_FIXED ? jb.pol.fixed_degree : -1;
 	a.field_complexity = 0;

@@ -36,6 +36,18 @@ constexpr int NSLOT = 16;   // quadrature tables indexed by Dunavant degree 0..1
 // offset from the element centre, w J / (4 pi), |offset|^2, reference point, w J
 struct QPoint { float ox, oy, oz, w, oo, xi, eta, wj; };
 
+// Per trial element, read as one 160-byte block through the scalar cache by
+// the regular pass: distance thresholds, centre, unit normal and the weight
+// 0.5 J / (4 pi) of the one-point (centroid) Dunavant rule of degrees 0 and 1.
+struct TrialRec {
+	double thr[NTHR];
+	double c[3];
+	float nu[3];
+	float w1;
+	float pad[6];
+};
+static_assert(sizeof(TrialRec) == 160, "TrialRec layout");
+
 struct RegArgs {
 	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
 	int tn, sn;                  // element counts of the test / trial mesh (SoA strides)
@@ -44,6 +56,7 @@ struct RegArgs {
 	const double *sc;            // trial centres [3][sn]
 	const float *snu;            // trial unit normals [3][sn]
 	const double *thr;           // [NTHR][sn]
+	const TrialRec *rec;         // [sn]
 	const QPoint *qt[NSLOT];     // per degree
 	int qn[NSLOT];
 	int fixed_degree;            // >= 0: fixed rule
@@ -181,6 +194,31 @@ struct PairCtx {
 // t0 + (k/2pi) u / (r + d) revolutions: t0 is the pair's FP64-reduced phase of
 // k d and u / (r + d) = r - d keeps full FP32 relative accuracy; sin and cos
 // are the hardware instructions, which take revolutions.
+// Add one point's E = (ec, es), th = k r, 1/r and b = r.n_x to the sums.
+template <int FAM, int MASK>
+__device__ __forceinline__ void add_point(float ec, float es, float th, float rinv, float b, float2 (&s)[6])
+{
+	if (MASK & SLP) { s[0].x += ec; s[0].y += es; }
+	if (MASK & (DLP | DLPT | HSP)) {
+		const float ri2 = rinv * rinv;
+		const float Gr = ec * ri2, Gi = es * ri2;
+		const float Qr = __builtin_fmaf(-th, Gi, Gr);
+		const float Qi = __builtin_fmaf(th, Gr, Gi);
+		if (MASK & DLP) { s[1].x += Qr; s[1].y += Qi; }
+		if ((MASK & HSP) && !(MASK & DLP)) { s[4].x += Qr; s[4].y += Qi; }
+		if (MASK & DLPT) { s[2].x = __builtin_fmaf(Qr, b, s[2].x); s[2].y = __builtin_fmaf(Qi, b, s[2].y); }
+		if (MASK & HSP) {
+			const float w2 = b * ri2;
+			s[3].x = __builtin_fmaf(Qr, w2, s[3].x);
+			s[3].y = __builtin_fmaf(Qi, w2, s[3].y);
+			if (FAM == HELM) {
+				s[5].x = __builtin_fmaf(ec, w2, s[5].x);
+				s[5].y = __builtin_fmaf(es, w2, s[5].y);
+			}
+		}
+	}
+}
+
 // NQ > 0: compile-time point count (fully unrolled).
 template <int FAM, int MASK, int NQ>
 __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt, const PairCtx &p,
@@ -207,50 +245,66 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 			ec = g;
 			es = 0.0f;
 		}
-		if (MASK & SLP) { s[0].x += ec; s[0].y += es; }
-		if (MASK & (DLP | DLPT | HSP)) {
-			const float ri2 = rinv * rinv;
-			const float Gr = ec * ri2, Gi = es * ri2;
-			const float Qr = __builtin_fmaf(-th, Gi, Gr);
-			const float Qi = __builtin_fmaf(th, Gr, Gi);
-			if (MASK & DLP) { s[1].x += Qr; s[1].y += Qi; }
-			if ((MASK & HSP) && !(MASK & DLP)) { s[4].x += Qr; s[4].y += Qi; }
-			if (MASK & (DLPT | HSP)) {
-				const float b = __builtin_fmaf(oz, p.nxz, __builtin_fmaf(oy, p.nxy, __builtin_fmaf(ox, p.nxx, p.b0)));
-				if (MASK & DLPT) { s[2].x = __builtin_fmaf(Qr, b, s[2].x); s[2].y = __builtin_fmaf(Qi, b, s[2].y); }
-				if (MASK & HSP) {
-					const float w2 = b * ri2;
-					s[3].x = __builtin_fmaf(Qr, w2, s[3].x);
-					s[3].y = __builtin_fmaf(Qi, w2, s[3].y);
-					if (FAM == HELM) {
-						s[5].x = __builtin_fmaf(ec, w2, s[5].x);
-						s[5].y = __builtin_fmaf(es, w2, s[5].y);
-					}
-				}
-			}
-		}
+		float b = 0.0f;
+		if (MASK & (DLPT | HSP))
+			b = __builtin_fmaf(oz, p.nxz, __builtin_fmaf(oy, p.nxy, __builtin_fmaf(ox, p.nxx, p.b0)));
+		add_point<FAM, MASK>(ec, es, th, rinv, b, s);
 	}
 }
 
-// Evaluate the kernels of MASK for the active lanes, each lane at its own
-// quadrature degree; lanes are served one distinct degree at a time so that
-// the table pointer stays wave-uniform.
-template <int FAM, int MASK, int ORDER, int NQ>
-__device__ __forceinline__ void eval_group(const RegArgs &a, int j, double d2, bool act, const PairCtx &p,
+// One-point (centroid) rule: y = c_j, so r = d, the phase is the pair's
+// FP64-reduced k d and b = (c_j - x).n_x; rinv_d = 1/d, w1 = 0.5 J / (4 pi).
+template <int FAM, int MASK>
+__device__ __forceinline__ void accumulate_centroid(const PairCtx &p, float w1, float rinv_d, float k,
 	float2 (&s)[6])
 {
-	int deg = a.fixed_degree >= 0 ? a.fixed_degree : a.field_complexity + table_degree<ORDER>(d2, a.thr, a.sn, j);
-	deg = act ? deg : -1;
+	const float g = w1 * rinv_d;
+	float ec = g, es = 0.0f, th = 0.0f;
+	if (FAM == HELM) {
+		ec = g * __builtin_amdgcn_cosf(p.t0);
+		es = -g * __builtin_amdgcn_sinf(p.t0);
+		th = k * p.df;
+	}
+	add_point<FAM, MASK>(ec, es, th, rinv_d, p.b0, s);
+}
+
+// Evaluate the kernels of MASK for the active lanes, each lane at its own
+// quadrature degree (deg < 0: lane idle); lanes are served one distinct
+// degree at a time so that the table pointer stays wave-uniform.  One-point
+// rules (degrees 0 and 1) take an unrolled path.
+// Degrees 0 and 1 are the one-point centroid rule (Dunavant; the host checks
+// its tables for this) and take accumulate_centroid: no table load.
+template <int FAM, int MASK>
+__device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, const PairCtx &p, float w1,
+	float rinv_d, float2 (&s)[6])
+{
+	if (__ballot(deg >= 0 && deg <= 1) != 0) {
+		if (deg >= 0 && deg <= 1) accumulate_centroid<FAM, MASK>(p, w1, rinv_d, a.k, s);
+		deg = deg <= 1 ? -1 : deg;
+	}
 	for (;;) {
 		const unsigned long long m = __ballot(deg >= 0);
 		if (m == 0) break;
 		const int lane = __ffsll((long long)m) - 1;
 		const int dsel = __builtin_amdgcn_readlane(deg, lane);
 		if (deg == dsel) {
-			accumulate_points<FAM, MASK, NQ>(a.qt[dsel] + (size_t)j * a.qn[dsel], a.qn[dsel], p, a.k_revf, a.k, s);
+			const int nq = a.qn[dsel];
+			accumulate_points<FAM, MASK, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, s);
 			deg = -1;
 		}
 	}
+}
+
+// table_degree from the thresholds of a trial record, as selects
+template <int ORDER>
+__device__ __forceinline__ int table_degree_rec(double d2, const double (&t)[NTHR])
+{
+	if (ORDER == 1)
+		return d2 < t[THR_1_1] ? 6 : d2 < t[THR_1_8] ? 4 : d2 < t[THR_9_1] ? 2 : 0;
+	if (ORDER == 2)
+		return d2 < t[THR_1_1] ? 8 : d2 < t[THR_1_5] ? 6 : d2 < t[THR_2_8] ? 4 : d2 < t[THR_15_8] ? 2 : 0;
+	return d2 < t[THR_1_1] ? 9 : d2 < t[THR_1_4] ? 8 : d2 < t[THR_1_9] ? 6 : d2 < t[THR_3_6] ? 4 :
+		d2 < t[THR_22_3] ? 2 : 0;
 }
 
 // FIXQ > 0: fixed rule with FIXQ points (the caller guarantees a.fixed_degree
@@ -269,48 +323,73 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
-	const auto sc = uni(a.sc);
-	const auto snu = uni(a.snu);
-	const auto thr = uni(a.thr);
+	const auto rec = uni(a.rec);
 	for (int j = j0; j < j1; ++j) {
-		const double cjx = sc[j], cjy = sc[a.sn + j], cjz = sc[2 * a.sn + j];
+		// the whole trial record in SGPRs (a few wide scalar loads)
+		double t[NTHR];
+#pragma unroll
+		for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
+		const double cjx = rec[j].c[0], cjy = rec[j].c[1], cjz = rec[j].c[2];
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
-		const bool act = row_ok && !(d2 < thr[THR_NEAR * a.sn + j]) && !(a.same_mesh && i == j);
+		const bool act = row_ok && !(d2 < t[THR_NEAR]) && !(a.same_mesh && i == j);
 		if (__ballot(act) == 0) continue;
 		const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);
 		p.dcx2 = 2.0f * dcx; p.dcy2 = 2.0f * dcy; p.dcz2 = 2.0f * dcz;
 		p.d2f = (float)d2;
+		// d and 1/d in FP64 from the hardware reciprocal square root and one Newton step
+		double y = __builtin_amdgcn_rsq(d2);
+		y = y * (1.5 - 0.5 * d2 * y * y);
+		const double d = d2 * y;
+		const float rinv_d = (float)y;
 		if (FAM == HELM) {
-			// d in FP64 from the hardware reciprocal square root and one Newton step
-			double y = __builtin_amdgcn_rsq(d2);
-			y = y * (1.5 - 0.5 * d2 * y * y);
-			const double d = d2 * y;
-			double t = d * a.k_rev;
-			t -= rint(t);
+			double tt = d * a.k_rev;
+			tt -= rint(tt);
 			p.df = (float)d;
-			p.t0 = (float)t;
+			p.t0 = (float)tt;
 		} else {
 			p.df = 0.0f; p.t0 = 0.0f;
 		}
 		float av = 0.0f, nn = 0.0f;
 		if (KM & (DLP | HSP)) {
-			const float nyx = snu[j], nyy = snu[a.sn + j], nyz = snu[2 * a.sn + j];
+			const float nyx = rec[j].nu[0], nyy = rec[j].nu[1], nyz = rec[j].nu[2];
 			av = dcx * nyx + dcy * nyy + dcz * nyz;
 			if (KM & HSP) nn = p.nxx * nyx + p.nxy * nyy + p.nxz * nyz;
 		}
 		if (KM & (DLPT | HSP)) p.b0 = dcx * p.nxx + dcy * p.nxy + dcz * p.nxz;
 		float2 s[6] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
 			make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+		// hsp_apart: the hypersingular sum of Q went to s[4] (its own degree)
 		bool hsp_apart = false;
 		if (FIXQ > 0) {
 			if (act) accumulate_points<FAM, KM, FIXQ>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s);
 		} else if (a.fixed_degree >= 0) {
-			eval_group<FAM, KM, 1, 0>(a, j, d2, act, p, s);
+			eval_degrees<FAM, KM>(a, j, act ? a.fixed_degree : -1, p, rec[j].w1, rinv_d, s);
 		} else {
-			if (KM & SLP) eval_group<FAM, KM & SLP, 1, 0>(a, j, d2, act, p, s);
-			if (KM & (DLP | DLPT)) eval_group<FAM, KM & (DLP | DLPT), 2, 0>(a, j, d2, act, p, s);
-			if (KM & HSP) eval_group<FAM, KM & HSP, 3, 0>(a, j, d2, act, p, s);
-			hsp_apart = true;
+			// per-kernel degrees from the distance tables; where they coincide
+			// (all far pairs: one point each) all kernels share one pass
+			const float w1 = rec[j].w1;
+			const int fc = a.field_complexity;
+			// beyond the last breakpoint (22.3 R, the largest of all tables)
+			// every degree is the table's 0; a wave of such pairs skips the selects
+			int dS = (KM & SLP) ? fc : -1, dD = (KM & (DLP | DLPT)) ? fc : -1, dH = (KM & HSP) ? fc : -1;
+			if (__ballot(act && d2 < t[THR_22_3]) != 0) {
+				if (KM & SLP) dS = fc + table_degree_rec<1>(d2, t);
+				if (KM & (DLP | DLPT)) dD = fc + table_degree_rec<2>(d2, t);
+				if (KM & HSP) dH = fc + table_degree_rec<3>(d2, t);
+			}
+			constexpr int NG = ((KM & SLP) ? 1 : 0) + ((KM & (DLP | DLPT)) ? 1 : 0) + ((KM & HSP) ? 1 : 0);
+			bool joint = false;
+			if (NG > 1) {
+				const int d0 = (KM & SLP) ? dS : (KM & (DLP | DLPT)) ? dD : dH;
+				joint = act && (!(KM & SLP) || dS == d0) && (!(KM & (DLP | DLPT)) || dD == d0) &&
+					(!(KM & HSP) || dH == d0);
+				eval_degrees<FAM, KM>(a, j, joint ? d0 : -1, p, w1, rinv_d, s);
+			}
+			const bool sep = act && !joint;
+			if (KM & SLP) eval_degrees<FAM, KM & SLP>(a, j, sep ? dS : -1, p, w1, rinv_d, s);
+			if (KM & (DLP | DLPT)) eval_degrees<FAM, KM & (DLP | DLPT)>(a, j, sep ? dD : -1, p, w1, rinv_d, s);
+			if (KM & HSP) eval_degrees<FAM, KM & HSP>(a, j, sep ? dH : -1, p, w1, rinv_d, s);
+			hsp_apart = !joint;
 		}
 		if (act) {
 			// kernel values from the accumulators
