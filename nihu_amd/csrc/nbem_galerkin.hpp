@@ -228,7 +228,9 @@ __device__ __forceinline__ void gal_eval_dynamic(const GalArgs &a, int e, int f,
 		const unsigned long long m = __ballot(deg >= 0);
 		if (m == 0) break;
 		const int dsel = __builtin_amdgcn_readlane(deg, __ffsll((long long)m) - 1);
-		if (deg == dsel) {
+		// lanes picked by a ballot mask (not deg == dsel): dsel stays wave-uniform
+		const unsigned long long msel = __ballot(deg == dsel);
+		if ((msel >> __lane_id()) & 1ull) {
 			const int nq = a.qn[dsel];
 			const float *tq = (const float *)(a.tq[dsel] + (size_t)e * nq);
 			gal_points<FAM, KM, NDOF, 0>(

@@ -282,12 +282,18 @@ __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, c
 		if (deg >= 0 && deg <= 1) accumulate_centroid<FAM, MASK>(p, w1, rinv_d, a.k, s);
 		deg = deg <= 1 ? -1 : deg;
 	}
+	// wave-uniform guard: waves without a multi-point lane skip the loop and
+	// the accumulator moves it costs
+	if (__ballot(deg >= 0) == 0) return;
 	for (;;) {
 		const unsigned long long m = __ballot(deg >= 0);
 		if (m == 0) break;
 		const int lane = __ffsll((long long)m) - 1;
 		const int dsel = __builtin_amdgcn_readlane(deg, lane);
-		if (deg == dsel) {
+		// lanes picked by a ballot mask rather than by deg == dsel, so that the
+		// compiler keeps dsel (and the table address) wave-uniform
+		const unsigned long long msel = __ballot(deg == dsel);
+		if ((msel >> __lane_id()) & 1ull) {
 			const int nq = a.qn[dsel];
 			accumulate_points<FAM, MASK, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, s);
 			deg = -1;
@@ -386,9 +392,12 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				eval_degrees<FAM, KM>(a, j, joint ? d0 : -1, p, w1, rinv_d, s);
 			}
 			const bool sep = act && !joint;
-			if (KM & SLP) eval_degrees<FAM, KM & SLP>(a, j, sep ? dS : -1, p, w1, rinv_d, s);
-			if (KM & (DLP | DLPT)) eval_degrees<FAM, KM & (DLP | DLPT)>(a, j, sep ? dD : -1, p, w1, rinv_d, s);
-			if (KM & HSP) eval_degrees<FAM, KM & HSP>(a, j, sep ? dH : -1, p, w1, rinv_d, s);
+			if (__ballot(sep) != 0) {
+				if (KM & SLP) eval_degrees<FAM, KM & SLP>(a, j, sep ? dS : -1, p, w1, rinv_d, s);
+				if (KM & (DLP | DLPT)) eval_degrees<FAM, KM & (DLP | DLPT)>(a, j, sep ? dD : -1, p, w1, rinv_d, s);
+				if (KM & HSP) eval_degrees<FAM, KM & HSP>(a, j, sep ? dH : -1, p, w1, rinv_d, s);
+			}
+			// the hypersingular sum of Q went to s[4] unless all kernels shared a pass
 			hsp_apart = !joint;
 		}
 		if (act) {
