@@ -1,6 +1,7 @@
 // nbem_oracle.cpp -- FP64 CPU restatement of NiHu's dense BEM assembly for
-// flat triangles (collocation with constant fields, Galerkin with linear
-// isoparametric fields), 3D Laplace and Helmholtz SLP / DLP / DLPt / HSP.
+// flat triangles (collocation with constant fields, Galerkin with constant
+// and linear isoparametric fields), 3D Laplace and Helmholtz SLP / DLP / DLPt
+// / HSP.
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the checker of the HIP engine in
 // nihu_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
@@ -15,7 +16,15 @@
 //   * the finite-difference kernel check of nihu/gtest,
 //   * closed forms and brute-force quadrature (diagonal integrals, near
 //     integrals, Gauss-Legendre nodes, Dunavant exactness),
-//   * known-answer systems (transparent point sources, pulsating sphere).
+//   * known-answer systems (transparent point sources, pulsating sphere),
+//   * for the Galerkin part: exact integrals of polynomials over two
+//     triangles (the Duffy-type rules), brute-force integration of coincident,
+//     edge- and vertex-adjacent pairs under every labelling, the closed-form
+//     solid angle, the analytic self-potential of a square (NiHu's
+//     lenoir_salles_2012 test), Gauss's identity, the mass-matrix closed form.
+// The node-overlap scan keeps NiHu's start-corner rule, including the
+// labelling in which it misses an edge's orientation (documented in the
+// Galerkin tests).
 // Deliberate, documented departures from NiHu's arithmetic (all at the level
 // of one rounding):
 //   * the 3x3 plane rotation is applied as T^T (T is orthonormal) where NiHu
