@@ -11,10 +11,12 @@ diagonal passes) writing both 20480^2 complex64 matrices.
 Metric: matrix entries/s = (number of kernel operators: SLP, DLP, DLPt, HSP =
 4) x 20480^2 per step / wall seconds.
 
-Also reported: the dominant kernel's roofline from its HIP-event time, the
-relative Frobenius error of sampled rows against the FP64 oracle, and the
-oracle (NiHu's algorithm restated, FP64, OpenMP over rows) timed on the host
-on the same sampled rows as the CPU baseline.
+Also reported: the dominant kernel's roofline from its HIP-event time; the
+relative Frobenius and max errors of both matrices against the FP64 oracle
+(NiHu's algorithm restated), which assembles the same matrices on the host
+with OpenMP over rows and is timed as the CPU baseline (beside it the serial
+pair loop on a row sample, and the host's CPU model); the solved surface field
+against the oracle's, and the total field against the Mie series.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
 """
@@ -75,9 +77,34 @@ def surface_field_error(nodes, tris, k, A, B, Ar, Br):
     torch.cuda.empty_cache()
     pr = solve(torch.from_numpy(Ar).cuda(), torch.from_numpy(Br).cuda()).cpu().numpy()
     torch.cuda.empty_cache()
+    # discretisation sanity check: total field p_inc + p against the Mie
+    # series of the rigid unit sphere (the mesh is the sphere to +-1%)
+    from nihu_amd.serve import mie_rigid_sphere_surface
+    pinc = np.exp(-1j * k * (c @ d))
+    mie = mie_rigid_sphere_surface(k, 1.0, (c @ d) / np.linalg.norm(c, axis=1))
     return {"surface_field_rel": float(np.linalg.norm(pg - pr) / np.linalg.norm(pr)),
             "surface_field_max_abs": float(np.abs(pg - pr).max()),
-            "surface_field_solve": "A p = B q, plane wave, complex128 LU on the GPU for both matrix sets"}
+            "surface_field_solve": "A p = B q, plane wave, complex128 LU on the GPU for both matrix sets",
+            "mie_rel": float(np.linalg.norm(pinc + pg - mie) / np.linalg.norm(mie)),
+            "mie_rel_oracle": float(np.linalg.norm(pinc + pr - mie) / np.linalg.norm(mie))}
+
+
+def host_cpu():
+    """CPU model (from /proc/cpuinfo) and the number of cores this process may
+    run on, stated beside every CPU number."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "cores_available": avail, "logical_cpus": os.cpu_count()}
 
 
 def copy_calibration(nbytes):
@@ -225,6 +252,16 @@ def main():
                          ", FP64 oracle restating NiHu's pair algorithm, OpenMP over rows, %.1f s wall" % dt}
         log("cpu baseline: %.4g entries/s on %d threads (%.1fx faster on GPU)" % (cpu_value, threads,
                                                                                  value / cpu_value))
+        # NiHu's own single-matrix path is the serial pair loop: the same
+        # oracle on one thread, on a bounded row sample
+        srows = np.sort(np.random.default_rng(321).choice(n, 24, replace=False)).astype(np.int32)
+        tc = time.perf_counter()
+        oracle.assemble_bm(nodes, tris, k, alpha=alpha, pol=opol, rows=srows, threads=1)
+        dts = time.perf_counter() - tc
+        cpu["serial"] = {"value": 4 * len(srows) * n / dts, "cores": 1,
+                         "sample": "%d of %d rows of A and B, one thread, %.1f s wall" % (len(srows), n, dts)}
+        cpu["host"] = host_cpu()
+        log("cpu serial: %.4g entries/s; host %s" % (cpu["serial"]["value"], cpu["host"]))
         if full:
             Ag, Bg = A.cpu().numpy(), B.cpu().numpy()
         else:
@@ -363,7 +400,8 @@ def run_galerkin(args):
         cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
                "sample": "test elements around %d sampled nodes (%d of %d elements) against all %d trial "
                          "elements, SLP and DLP, FP64 oracle, %.1f s wall; rate scaled by %d/%d to the whole "
-                         "matrices" % (len(sample), len(elems), ne, ne, dt, len(elems), ne)}
+                         "matrices" % (len(sample), len(elems), ne, ne, dt, len(elems), ne),
+               "host": host_cpu()}
         si = torch.from_numpy(sample.astype(np.int64)).cuda()
         Lg = L[si, :].cpu().numpy().astype(np.complex128)
         Mg = M[si, :].cpu().numpy().astype(np.complex128)
@@ -467,7 +505,7 @@ def run_sweep(args):
         "regular_sweep_kernel_ms": reg_ms,
         "cpu_baseline": {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
                          "sample": "%d of %d rows, 64 wavenumbers, NiHu's OpenMP-over-frequencies pattern (FP64 oracle)"
-                                   ", %.1f s wall" % (len(rows), n, dt)},
+                                   ", %.1f s wall" % (len(rows), n, dt), "host": host_cpu()},
         "accuracy": {"max_rel_fro_over_k": float(max(errs)), "rows": len(rows)},
     }
     print(json.dumps(out))
