@@ -1,4 +1,5 @@
-"""Triangle meshes for the BEM engine: seeded icospheres, ellipsoids, OFF files.
+"""Meshes for the BEM engine: seeded icospheres, ellipsoids, cubed spheres
+(quadrilaterals), OFF files.
 
 A mesh is a pair of arrays, ``nodes`` (n_nodes x 3, float64) and ``tris``
 (n_tri x 3, int32), elements numbered by row (NiHu numbers elements by
@@ -18,6 +19,7 @@ from __future__ import annotations
 import numpy as np
 
 NIHU_TRIA_1_ID = 32303  # element type id of tria_1_elem, nihu/core/element.hpp:112-118
+NIHU_QUAD_1_ID = 32404  # quad_1_elem (space dimension 3, quad domain, 4 nodes)
 
 
 def _icosahedron():
@@ -97,11 +99,61 @@ def ellipsoid(level: int = 5, seed: int = 0, jitter: float = 0.01):
     return np.ascontiguousarray(nodes), tris
 
 
-def read_off(path):
-    """Read an OFF file of triangles (nihu/interface/read_off_mesh.hpp:75-105).
+def cubed_sphere(n: int, radius: float = 1.0, jitter: float = 0.0, seed: int = 0):
+    """Quadrilateral mesh of a sphere: each face of the cube [-1, 1]^3 cut
+    into n x n squares, nodes projected radially to the sphere (so the
+    quadrilaterals are warped bilinear elements), 6 n^2 elements and
+    6 n^2 + 2 nodes.  Corners ordered so that (X1 - X0) x (X3 - X0) points
+    outwards; elements numbered face by face, row by row.  Radial jitter as
+    for the icosphere: 1 + U(-jitter, jitter) per node in node order."""
+    index = {}
+    nodes = []
 
-    Raises ValueError for a bad header, truncated data or non-triangle faces
-    (NiHu raises std::runtime_error in the same places)."""
+    def node(p):
+        key = tuple(int(round(v)) for v in p)
+        if key not in index:
+            index[key] = len(nodes)
+            nodes.append(key)
+        return index[key]
+
+    quads = []
+    # (normal axis, sign): the two in-plane axes (u, v) with u x v along the outward normal
+    for axis, sign in ((0, 1), (0, -1), (1, 1), (1, -1), (2, 1), (2, -1)):
+        u_ax, v_ax = [(1, 2), (2, 0), (0, 1)][axis]
+        if sign < 0:
+            u_ax, v_ax = v_ax, u_ax
+        for j in range(n):
+            for i in range(n):
+                corner = []
+                for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                    p = [0, 0, 0]
+                    p[axis] = sign * n
+                    p[u_ax] = 2 * (i + di) - n
+                    p[v_ax] = 2 * (j + dj) - n
+                    corner.append(node(p))
+                quads.append(corner)
+    X = np.array(nodes, dtype=np.float64)
+    X /= np.linalg.norm(X, axis=1)[:, None]
+    X *= radius
+    if jitter > 0.0:
+        u = np.random.default_rng(seed).uniform(-jitter, jitter, size=len(X))
+        X = X * (1.0 + u)[:, None]
+    return np.ascontiguousarray(X), np.ascontiguousarray(np.array(quads, dtype=np.int32))
+
+
+def jittered_cubed_sphere(n: int, seed: int = 0):
+    """Unit cubed sphere with +-1% seeded radial jitter."""
+    return cubed_sphere(n, 1.0, 0.01, seed)
+
+
+def read_off(path):
+    """Read an OFF file of triangles or of quadrilaterals
+    (nihu/interface/read_off_mesh.hpp:75-105); returns (nodes, elements) with
+    3 or 4 node indices per element.
+
+    Raises ValueError for a bad header, truncated data, faces that are
+    neither triangles nor quadrilaterals (NiHu raises std::runtime_error in the
+    same places) and for meshes that mix the two (not supported here)."""
     with open(path, "r") as fh:
         tok = fh.read().split()
     if not tok or tok[0] != "OFF":
@@ -116,35 +168,51 @@ def read_off(path):
     except ValueError:
         raise ValueError("Error reading mesh nodes")
     pos += 3 * n_nodes
-    tris = np.empty((n_elem, 3), dtype=np.int32)
+    elems = []
     for e in range(n_elem):
         if pos >= len(tok):
             raise ValueError("Error reading mesh elements")
         nv = int(tok[pos])
-        if nv != 3:
+        if nv not in (3, 4):
             raise ValueError("Invalid number of vertices for current element tag vector")
-        tris[e] = [int(t) for t in tok[pos + 1:pos + 4]]
+        if pos + 1 + nv > len(tok):
+            raise ValueError("Error reading mesh elements")
+        elems.append([int(t) for t in tok[pos + 1:pos + 1 + nv]])
         pos += 1 + nv
-    return nodes, tris
+    sizes = {len(e) for e in elems}
+    if len(sizes) > 1:
+        raise ValueError("meshes mixing triangles and quadrilaterals are not supported")
+    npe = sizes.pop() if sizes else 3
+    return nodes, np.array(elems, dtype=np.int32).reshape(n_elem, npe)
 
 
-def write_off(path, nodes, tris):
+def write_off(path, nodes, elems):
+    elems = np.asarray(elems)
     with open(path, "w") as fh:
-        fh.write("OFF\n%d %d 0\n" % (len(nodes), len(tris)))
+        fh.write("OFF\n%d %d 0\n" % (len(nodes), len(elems)))
         for p in nodes:
             fh.write("%.17g %.17g %.17g\n" % tuple(p))
-        for t in tris:
-            fh.write("3 %d %d %d\n" % tuple(t))
+        for t in elems:
+            fh.write(("%d" + " %d" * len(t)) % ((len(t),) + tuple(t)) + "\n")
 
 
-def to_nihu_elements(tris):
-    """NiHu's element matrix rows [type_id, n0, n1, n2] (nihu/core/mesh.hpp:137-193)."""
-    tris = np.asarray(tris)
-    return np.hstack([np.full((len(tris), 1), NIHU_TRIA_1_ID, dtype=np.int64), tris.astype(np.int64)])
+def to_nihu_elements(elems):
+    """NiHu's element matrix rows [type_id, n0, n1, n2(, n3)]
+    (nihu/core/mesh.hpp:137-193): 32303 for triangles, 32404 for
+    quadrilaterals."""
+    elems = np.asarray(elems)
+    tid = NIHU_TRIA_1_ID if elems.shape[1] == 3 else NIHU_QUAD_1_ID
+    return np.hstack([np.full((len(elems), 1), tid, dtype=np.int64), elems.astype(np.int64)])
 
 
 def from_nihu_elements(elements):
+    """Node indices from NiHu's element matrix: all tria_1 (32303) or all
+    quad_1 (32404) rows."""
     elements = np.asarray(elements)
-    if elements.ndim != 2 or elements.shape[1] < 4 or np.any(elements[:, 0] != NIHU_TRIA_1_ID):
-        raise ValueError("only tria_1 elements (id 32303) are supported")
-    return np.ascontiguousarray(elements[:, 1:4].astype(np.int32))
+    if elements.ndim != 2 or elements.shape[1] < 4 or len(elements) == 0:
+        raise ValueError("only tria_1 (32303) or quad_1 (32404) elements are supported")
+    if np.all(elements[:, 0] == NIHU_TRIA_1_ID):
+        return np.ascontiguousarray(elements[:, 1:4].astype(np.int32))
+    if np.all(elements[:, 0] == NIHU_QUAD_1_ID) and elements.shape[1] >= 5:
+        return np.ascontiguousarray(elements[:, 1:5].astype(np.int32))
+    raise ValueError("only tria_1 (32303) or quad_1 (32404) elements are supported, not mixed")

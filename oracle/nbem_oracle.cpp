@@ -525,23 +525,26 @@ static cplx singular_integral(int family, int nx, int ny, double k, Elem const &
 // Helmholtz = Laplace static + Dunavant-14 (42 points) of (K_helm - K_lap):
 //   nihu/library/helmholtz_nearly_singular_integrals.hpp:48-219
 // ---------------------------------------------------------------------------
-static double laplace_near_static(int nx, int ny, Elem const &e, V3 x0_in, V3 nx_in, int line_degree = 0)
+// The planar polygon with corners X[0..NC-1] (triangles: NC = 3; the
+// linearised quadrilaterals of the near-singular quadrilateral integrals:
+// NC = 4); the plane is spanned by the corners 0, 1, 2.
+static double planar_near_static(int nx, int ny, const V3 *X, int NC, V3 x0_in, V3 nx_in, int line_degree)
 {
 	// T = [t1, t3 x t1, t3] with t1 = v1/|v1|, t3 = (v1 x v2)/|v1 x v2|
-	V3 v1 = e.X[1] - e.X[0], v2 = e.X[2] - e.X[0];
+	V3 v1 = X[1] - X[0], v2 = X[2] - X[0];
 	V3 T0 = normalized(v1);
 	V3 T2 = normalized(cross(v1, v2));
 	V3 T1 = normalized(cross(T2, T0));
 	auto to_plane = [&](V3 p) -> V3 { return {dot(T0, p), dot(T1, p), dot(T2, p)}; };
-	V3 C[3];
-	for (int i = 0; i < 3; ++i) C[i] = to_plane(e.X[i]);
+	V3 C[4];
+	for (int i = 0; i < NC; ++i) C[i] = to_plane(X[i]);
 	V3 x0 = to_plane(x0_in);
 	V3 nxl = to_plane(nx_in);
 
-	double theta_lim[3], ref_distance[3], theta0[3];
-	for (int n = 0; n < 3; ++n) {
+	double theta_lim[4], ref_distance[4], theta0[4];
+	for (int n = 0; n < NC; ++n) {
 		double c1x = C[n].x, c1y = C[n].y;
-		double c2x = C[(n + 1) % 3].x, c2y = C[(n + 1) % 3].y;
+		double c2x = C[(n + 1) % NC].x, c2y = C[(n + 1) % NC].y;
 		double lx = c2x - c1x, ly = c2y - c1y;
 		double ln = std::sqrt(lx * lx + ly * ly);
 		lx /= ln; ly /= ln;
@@ -558,8 +561,8 @@ static double laplace_near_static(int nx, int ny, Elem const &e, V3 x0_in, V3 nx
 	// line quadrature order 10 (6 points) for SLP/DLP/DLPt, 40 (21 points) for HSP
 	auto const &gl = gauss_line(line_degree > 0 ? line_degree : (hsp ? 40 : 10));
 	double result = 0.0;
-	for (int i = 0; i < 3; ++i) {
-		double th1 = theta_lim[i], th2 = theta_lim[(i + 1) % 3];
+	for (int i = 0; i < NC; ++i) {
+		double th1 = theta_lim[i], th2 = theta_lim[(i + 1) % NC];
 		if (std::abs(th2 - th1) > PI) {
 			if (th2 > th1) th1 += TWO_PI;
 			else th2 += TWO_PI;
@@ -592,6 +595,11 @@ static double laplace_near_static(int nx, int ny, Elem const &e, V3 x0_in, V3 nx
 	}
 	if (nx == 0 && ny == 1) return sgn(z) * result / (4. * PI);
 	return result / (4. * PI);
+}
+
+static double laplace_near_static(int nx, int ny, Elem const &e, V3 x0_in, V3 nx_in, int line_degree = 0)
+{
+	return planar_near_static(nx, ny, e.X, 3, x0_in, nx_in, line_degree);
 }
 
 static cplx near_integral(int family, int nx, int ny, double k, Elem const &e, V3 x0, V3 nxv, int line_degree = 0)
@@ -783,6 +791,500 @@ static void galerkin_block(int family, int nx, int ny, double k, Policy const &p
 					blk[a][b] += (Nt[a] * (te.J * qp.w)) * K * (Ns[b] * (se.J * qs.w));
 		}
 	}
+}
+
+// ---------------------------------------------------------------------------
+// Quadrilateral elements with constant fields, collocation at the centre
+// quad_1_elem: 4-node bilinear, possibly warped.
+// nihu/library/lib_shape.hpp:215-310 (shape functions, first and second
+// derivatives; jacobian order 1), nihu/library/lib_domain.cpp:74-94 (corners
+// (-1,-1), (1,-1), (1,1), (-1,1), edges 01 12 23 30, centre (0,0)),
+// nihu/core/element.hpp:272-292 (centre, mean edge length), :613-616 (normal =
+// dx/dxi x dx/deta).
+// ---------------------------------------------------------------------------
+struct QElem {
+	V3 X[4];
+	V3 c;        // x(0, 0)
+	V3 nu;       // unit normal at the centre
+	double R;    // mean edge length
+	double diam; // largest corner-to-corner distance
+	int id;
+};
+
+static const double QCX[4] = {-1.0, 1.0, 1.0, -1.0}, QCY[4] = {-1.0, -1.0, 1.0, 1.0};
+
+static inline V3 q_x(QElem const &e, double xi, double eta)
+{
+	double N[4] = {(1.0 - xi) * (1.0 - eta) / 4.0, (1.0 + xi) * (1.0 - eta) / 4.0,
+		(1.0 + xi) * (1.0 + eta) / 4.0, (1.0 - xi) * (1.0 + eta) / 4.0};
+	V3 x = {0.0, 0.0, 0.0};
+	for (int a = 0; a < 4; ++a) x = x + e.X[a] * N[a];
+	return x;
+}
+
+static inline void q_dx(QElem const &e, double xi, double eta, V3 &dxi, V3 &deta)
+{
+	double Dxi[4] = {-(1.0 - eta) / 4.0, (1.0 - eta) / 4.0, (1.0 + eta) / 4.0, -(1.0 + eta) / 4.0};
+	double Deta[4] = {-(1.0 - xi) / 4.0, -(1.0 + xi) / 4.0, (1.0 + xi) / 4.0, (1.0 - xi) / 4.0};
+	dxi = {0.0, 0.0, 0.0}; deta = {0.0, 0.0, 0.0};
+	for (int a = 0; a < 4; ++a) { dxi = dxi + e.X[a] * Dxi[a]; deta = deta + e.X[a] * Deta[a]; }
+}
+
+// d2x / dxi deta (the other second derivatives of the bilinear map vanish)
+static inline V3 q_dxe(QElem const &e)
+{
+	return e.X[0] * 0.25 + e.X[1] * (-0.25) + e.X[2] * 0.25 + e.X[3] * (-0.25);
+}
+
+static inline V3 q_normal(QElem const &e, double xi, double eta)
+{
+	V3 a, b;
+	q_dx(e, xi, eta, a, b);
+	return cross(a, b);
+}
+
+static QElem make_qelem(const V3 X[4], int id)
+{
+	QElem e;
+	for (int a = 0; a < 4; ++a) e.X[a] = X[a];
+	e.id = id;
+	e.c = q_x(e, 0.0, 0.0);
+	e.nu = normalized(q_normal(e, 0.0, 0.0));
+	double d = 0.0;
+	for (int a = 0; a < 4; ++a) d += norm(q_x(e, QCX[(a + 1) % 4], QCY[(a + 1) % 4]) - q_x(e, QCX[a], QCY[a]));
+	e.R = d / 4.0;
+	e.diam = 0.0;
+	for (int a = 0; a < 4; ++a)
+		for (int b = a + 1; b < 4; ++b) e.diam = std::max(e.diam, norm(X[b] - X[a]));
+	return e;
+}
+
+// A test point of either element type: the collocation point (centre) and
+// the unit normal there.
+struct TestPt { V3 x, nu; int id; };
+
+// the elements of a mesh given as npe (3 or 4) node indices per element
+static void read_elems(const double *xyz, int n_nodes, const int *el, int n_el, int npe,
+	std::vector<Elem> *tris, std::vector<QElem> *quads, std::vector<TestPt> *pts)
+{
+	if (npe != 3 && npe != 4) throw std::invalid_argument("elements must have 3 or 4 nodes");
+	for (int e = 0; e < n_el; ++e) {
+		V3 X[4];
+		for (int a = 0; a < npe; ++a) {
+			int v = el[npe * e + a];
+			if (v < 0 || v >= n_nodes) throw std::out_of_range("node index out of range");
+			X[a] = {xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
+		}
+		if (npe == 3) {
+			Elem t = make_elem(X[0], X[1], X[2], e);
+			if (tris) tris->push_back(t);
+			if (pts) pts->push_back({t.c, t.nu, e});
+		} else {
+			QElem q = make_qelem(X, e);
+			if (quads) quads->push_back(q);
+			if (pts) pts->push_back({q.c, q.nu, e});
+		}
+	}
+}
+
+// Gauss-Legendre tensor rule on [-1, 1]^2 of a degree: N = degree/2 + 1 points
+// per axis, xi outer, eta inner (nihu/core/gaussian_quadrature.hpp:152-207).
+static std::vector<QP> gauss_quad(int degree)
+{
+	auto const &g = gauss_line(degree);
+	std::vector<QP> q;
+	for (auto const &a : g)
+		for (auto const &b : g) q.push_back({a.first, b.first, a.second * b.second});
+	return q;
+}
+
+static const std::vector<QP> &gauss_quad_store(int degree)
+{
+	static std::vector<QP> pool[128];
+	static bool filled[128] = {false};
+	if (degree < 0 || degree >= 128) throw std::out_of_range("quadrilateral rule degree out of range");
+	#pragma omp critical(orc_gauss_quad)
+	if (!filled[degree]) { pool[degree] = gauss_quad(degree); filled[degree] = true; }
+	return pool[degree];
+}
+
+// kernel at a trial point of a quadrilateral times the Jacobian |n(xi)|
+static inline cplx q_kernel_jac(int family, int nx, int ny, double k, V3 x, V3 nxv, QElem const &e, double xi,
+	double eta)
+{
+	V3 y = q_x(e, xi, eta);
+	V3 n = q_normal(e, xi, eta);
+	double J = norm(n);
+	return kernel_eval(family, nx, ny, k, x, y, nxv, n / J) * J;
+}
+
+// Regular pair: sum_q K(x, y_q) J(xi_q) w_q.  The degree is the field
+// complexity (jacobian order 1 of quad_1: nihu/core/complexity_estimator.hpp:
+// 48-62) plus the distance interval of the kernel (NiHu policy), or fixed.
+static cplx quad_regular(int family, int nx, int ny, double k, V3 x, V3 nxv, QElem const &e, int degree)
+{
+	if (degree > 14)
+		throw std::out_of_range("Too high quadrature degree selected for collocational integration");
+	cplx res = 0.0;
+	for (auto const &q : gauss_quad_store(degree))
+		res += q_kernel_jac(family, nx, ny, k, x, nxv, e, q.xi, q.eta) * q.w;
+	return res;
+}
+
+// Duffy-type blind rule around a point of the reference square: the square is
+// cut into the four triangles (xi0, corner s, corner s+1), each the image of
+// the Gauss square rule of the degree under the bilinear map of the
+// degenerate quadrilateral (xi0, xi0, corner s, corner s+1), weights times the
+// signed Jacobian.  nihu/core/duffy_quadrature.hpp (on_face, duffy_impl),
+// nihu/core/quadrature.hpp:105-127 (transform_inplace).
+static std::vector<QP> quad_duffy_face(int degree, double x0, double y0)
+{
+	std::vector<QP> out;
+	auto const &src = gauss_quad_store(degree);
+	for (int s = 0; s < 4; ++s) {
+		double cx[4] = {x0, x0, QCX[s], QCX[(s + 1) % 4]};
+		double cy[4] = {y0, y0, QCY[s], QCY[(s + 1) % 4]};
+		for (auto const &q : src) {
+			double u = q.xi, v = q.eta;
+			double N[4] = {(1.0 - u) * (1.0 - v) / 4.0, (1.0 + u) * (1.0 - v) / 4.0,
+				(1.0 + u) * (1.0 + v) / 4.0, (1.0 - u) * (1.0 + v) / 4.0};
+			double Du[4] = {-(1.0 - v) / 4.0, (1.0 - v) / 4.0, (1.0 + v) / 4.0, -(1.0 + v) / 4.0};
+			double Dv[4] = {-(1.0 - u) / 4.0, -(1.0 + u) / 4.0, (1.0 + u) / 4.0, (1.0 - u) / 4.0};
+			double px = 0, py = 0, a = 0, b = 0, c = 0, d = 0;
+			for (int i = 0; i < 4; ++i) {
+				px += N[i] * cx[i]; py += N[i] * cy[i];
+				a += Du[i] * cx[i]; b += Du[i] * cy[i];
+				c += Dv[i] * cx[i]; d += Dv[i] * cy[i];
+			}
+			out.push_back({px, py, q.w * (a * d - c * b)});
+		}
+	}
+	return out;
+}
+
+// Singular (same element) SLP / DLP / DLPt: the blind Duffy rule of order 7
+// (singular_quadrature_order, nihu/library/helmholtz_kernel.hpp:69-71) around
+// the centre; nihu/core/singular_accelerator.hpp:363-456 and
+// nihu/core/double_integral.hpp:470-510 (eval_singular_on_accelerator).
+static cplx quad_singular_blind(int family, int nx, int ny, double k, QElem const &e)
+{
+	static const std::vector<QP> rule = quad_duffy_face(7, 0.0, 0.0);
+	cplx res = 0.0;
+	for (auto const &q : rule)
+		res += q_kernel_jac(family, nx, ny, k, e.c, e.nu, e, q.xi, q.eta) * q.w;
+	return res;
+}
+
+// Guiggiani's method for the hypersingular collocational integral (Guiggiani,
+// Krishnasamy, Rudolphi, Rizzo, J. Appl. Mech. 59 (1992) 604-614) as NiHu
+// applies it: nihu/library/guiggiani_1992.hpp:138-330 (the sides of the
+// reference domain seen from the singular point in the distorted plane
+// eta = T xi, where T makes the first-order distance expansion unit length;
+// the Laurent terms subtracted under the radial integral and added back in
+// closed form), with the Laurent coefficients of the Laplace HSP core,
+// nihu/library/laplace_kernel.hpp:266-295 (also used for Helmholtz HSP, whose
+// singular core is Laplace's).  Radial rule of degree rad_order, angular rule
+// of degree tan_order (5 and 9 in NiHu's HSP shortcuts,
+// nihu/library/helmholtz_singular_integrals.hpp:1190-1236).
+static cplx quad_guiggiani(int family, double k, QElem const &e, double xi0x, double xi0y, int rad_order,
+	int tan_order)
+{
+	V3 dxi, deta;
+	q_dx(e, xi0x, xi0y, dxi, deta);
+	V3 dxe = q_dxe(e);
+	V3 nJ = cross(dxi, deta);
+	V3 n0 = normalized(nJ);
+	V3 x0 = q_x(e, xi0x, xi0y);
+
+	double la = norm(dxi), lb = norm(deta);
+	double cosg = dot(dxi, deta) / la / lb;
+	double u = lb / la;
+	double T00 = la, T01 = la * cosg * u, T10 = 0.0, T11 = la * std::sqrt(1.0 - cosg * cosg) * u;
+	double detT = T00 * T11 - T10 * T01;
+	double Ti00 = T11 / detT, Ti01 = -T01 / detT, Ti10 = -T10 / detT, Ti11 = T00 / detT;
+	double e0x = T00 * xi0x + T01 * xi0y, e0y = T10 * xi0x + T11 * xi0y;
+	V3 J0 = nJ / detT;
+
+	double theta_lim[4], theta0[4], ref_distance[4];
+	for (int n = 0; n < 4; ++n) {
+		double c1x = T00 * QCX[n] + T01 * QCY[n], c1y = T10 * QCX[n] + T11 * QCY[n];
+		int m = (n + 1) % 4;
+		double c2x = T00 * QCX[m] + T01 * QCY[m], c2y = T10 * QCX[m] + T11 * QCY[m];
+		double lx = 1.0, ly = 0.0;
+		double ln = std::sqrt((c2x - c1x) * (c2x - c1x) + (c2y - c1y) * (c2y - c1y));
+		if (ln > 1e-3) { lx = (c2x - c1x) / ln; ly = (c2y - c1y) / ln; }
+		double d1x = c1x - e0x, d1y = c1y - e0y;
+		double dl = d1x * lx + d1y * ly;
+		double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		theta_lim[n] = std::atan2(d1y, d1x);
+		theta0[n] = std::atan2(d0y, d0x);
+		ref_distance[n] = std::sqrt(d0x * d0x + d0y * d0y);
+	}
+
+	auto const &gt = gauss_line(tan_order);
+	auto const &gr = gauss_line(rad_order);
+	cplx I = 0.0;
+	for (int n = 0; n < 4; ++n) {
+		double t1 = theta_lim[n], t2 = theta_lim[(n + 1) % 4];
+		if (std::abs(t2 - t1) < 1e-3) continue;
+		if (std::abs(t2 - t1) > PI) t2 += TWO_PI;
+		for (auto const &qt : gt) {
+			double theta = ((1.0 - qt.first) * t1 + (1.0 + qt.first) * t2) / 2.0;
+			double w_theta = qt.second * (t2 - t1) / 2.0;
+			// direction in the reference square and the Taylor terms of the
+			// distance (r1, r2) and of the Jacobian vector (J0, J1)
+			double ct = std::cos(theta), st = std::sin(theta);
+			double vx = Ti00 * ct + Ti01 * st, vy = Ti10 * ct + Ti11 * st;
+			V3 r1 = dxi * vx + deta * vy;
+			V3 r2 = dxe * (vx * vy);
+			V3 J1 = (cross(dxi, dxe) * vx + cross(dxe, deta) * vy) / detT;
+			// Laurent coefficients of the Laplace HSP core (constant field: N0 = 1, N1 = 0)
+			V3 g1 = r1 * (dot(r2, J0) + dot(r1, J1));
+			V3 b0 = J0 * -1.0;
+			V3 b1 = g1 * 3.0 - J1;
+			double a0 = dot(b0, n0);
+			double a1 = dot(b1, n0);
+			double Sm2 = -3.0 * dot(r1, r2);
+			double Fm1 = -(Sm2 * a0 + a1) / (4.0 * PI);
+			double Fm2 = -a0 / (4.0 * PI);
+
+			double rho_lim = ref_distance[n] / std::cos(theta - theta0[n]);
+			I += (Fm1 * std::log(rho_lim) - Fm2 / rho_lim) * w_theta;
+			for (auto const &qr : gr) {
+				double rho = (1.0 + qr.first) * rho_lim / 2.0;
+				double w_rho = qr.second * rho_lim / 2.0;
+				double ex = rho * ct, ey = rho * st;
+				double xi = Ti00 * ex + Ti01 * ey + xi0x, eta = Ti10 * ex + Ti11 * ey + xi0y;
+				V3 y = q_x(e, xi, eta);
+				V3 nv = q_normal(e, xi, eta);
+				double J = norm(nv);
+				cplx F = kernel_eval(family, 1, 1, k, x0, y, n0, nv / J) * (J / detT * rho);
+				double sing = (Fm1 + Fm2 / rho) / rho;
+				I += (F - sing) * (w_theta * w_rho);
+			}
+		}
+	}
+	return I;
+}
+
+// Image of a point on the element by Newton-Raphson iteration on
+// x(xi) + zeta n(xi) / sqrt|n(xi)| = x0 from (0, 0, 0); false if not
+// converged.  nihu/core/inverse_mapping.hpp:37-118.
+static bool quad_inverse_map(QElem const &e, V3 x0, double tol, int max_iter, double &xi, double &eta, double &zeta)
+{
+	double r[3] = {0.0, 0.0, 0.0};
+	V3 dxe = q_dxe(e);
+	for (int it = 0; it < max_iter; ++it) {
+		V3 y = q_x(e, r[0], r[1]);
+		V3 n = q_normal(e, r[0], r[1]);
+		double sj = std::sqrt(norm(n));
+		n = n / sj;
+		V3 f = y + n * r[2] - x0;
+		if (norm(f) < tol) { xi = r[0]; eta = r[1]; zeta = r[2]; return true; }
+		V3 a, b;
+		q_dx(e, r[0], r[1], a, b);
+		V3 dnxi = cross(a, dxe) / sj;
+		V3 dneta = cross(dxe, b) / sj;
+		V3 c0 = a + dnxi * r[2], c1 = b + dneta * r[2], c2 = n;
+		// solve [c0 c1 c2] d = f by Gaussian elimination with partial pivoting
+		double M[3][4] = {{c0.x, c1.x, c2.x, f.x}, {c0.y, c1.y, c2.y, f.y}, {c0.z, c1.z, c2.z, f.z}};
+		for (int col = 0; col < 3; ++col) {
+			int piv = col;
+			for (int i = col + 1; i < 3; ++i)
+				if (std::abs(M[i][col]) > std::abs(M[piv][col])) piv = i;
+			if (piv != col) for (int j = 0; j < 4; ++j) std::swap(M[piv][j], M[col][j]);
+			for (int i = col + 1; i < 3; ++i) {
+				double fct = M[i][col] / M[col][col];
+				for (int j = col; j < 4; ++j) M[i][j] -= fct * M[col][j];
+			}
+		}
+		double d[3];
+		for (int i = 2; i >= 0; --i) {
+			double sacc = M[i][3];
+			for (int j = i + 1; j < 3; ++j) sacc -= M[i][j] * d[j];
+			d[i] = sacc / M[i][i];
+		}
+		for (int i = 0; i < 3; ++i) r[i] -= d[i];
+	}
+	return false;
+}
+
+// the parallelogram tangent to the element at xi0 (element::get_linearized_elem,
+// nihu/core/element.hpp:418-437)
+static QElem quad_linearized(QElem const &e, double xi0, double eta0)
+{
+	V3 x0 = q_x(e, xi0, eta0);
+	V3 a, b;
+	q_dx(e, xi0, eta0, a, b);
+	V3 X[4];
+	for (int i = 0; i < 4; ++i) X[i] = x0 + a * (QCX[i] - xi0) + b * (QCY[i] - eta0);
+	return make_qelem(X, e.id);
+}
+
+// Planar constant collocation integral over a planar quadrilateral (the
+// linearised element): Laplace static part in the plane plus, for Helmholtz,
+// the 8 x 8-point Gauss rule (degree 14) of K_helm - K_lap;
+// nihu/library/helmholtz_nearly_singular_integrals.hpp:40-219 with
+// elem_t = quad_1_elem, nihu/library/laplace_nearly_singular_integrals.hpp:62-389.
+static cplx quad_planar_near(int family, int nx, int ny, double k, QElem const &lin, V3 x0, V3 nxv)
+{
+	V3 C[4];
+	for (int i = 0; i < 4; ++i) C[i] = q_x(lin, QCX[i], QCY[i]);
+	cplx res = planar_near_static(nx, ny, C, 4, x0, nxv, 0);
+	if (family == 0) return res;
+	V3 nyv = normalized(q_normal(lin, 0.0, 0.0));
+	cplx dyn = 0.0;
+	for (auto const &q : gauss_quad_store(14)) {
+		V3 y = q_x(lin, q.xi, q.eta);
+		double jac = norm(q_normal(lin, q.xi, q.eta));
+		dyn += (kernel_eval(1, nx, ny, k, x0, y, nxv, nyv) - kernel_eval(0, nx, ny, k, x0, y, nxv, nyv)) * q.w * jac;
+	}
+	return res + dyn;
+}
+
+// Near-singular integral by polar integration around the image xi0 of the
+// collocation point of K J - K_lin J_lin (the linearised element at xi0), plus
+// the planar integral over the linearised element;
+// nihu/library/nearly_singular_collocational.hpp:71-203 (inverse mapping
+// tolerance 1e-5, 100 iterations).
+static cplx quad_near_polar(int family, int nx, int ny, double k, QElem const &e, V3 x0, V3 nxv, int rad_order,
+	int tan_order)
+{
+	double xi0, eta0, zeta;
+	if (!quad_inverse_map(e, x0, 1e-5, 100, xi0, eta0, zeta))
+		throw std::runtime_error("Could not perform inverse mapping");
+	QElem lin = quad_linearized(e, xi0, eta0);
+	double theta_lim[4], theta0[4], ref_distance[4];
+	for (int n = 0; n < 4; ++n) {
+		int m = (n + 1) % 4;
+		double c1x = QCX[n], c1y = QCY[n], c2x = QCX[m], c2y = QCY[m];
+		double lx = 1.0, ly = 0.0;
+		double ln = std::sqrt((c2x - c1x) * (c2x - c1x) + (c2y - c1y) * (c2y - c1y));
+		if (ln > 1e-3) { lx = (c2x - c1x) / ln; ly = (c2y - c1y) / ln; }
+		double d1x = c1x - xi0, d1y = c1y - eta0;
+		double dl = d1x * lx + d1y * ly;
+		double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		theta_lim[n] = std::atan2(d1y, d1x);
+		theta0[n] = std::atan2(d0y, d0x);
+		ref_distance[n] = std::sqrt(d0x * d0x + d0y * d0y);
+	}
+	auto const &gt = gauss_line(tan_order);
+	auto const &gr = gauss_line(rad_order);
+	cplx I = 0.0;
+	for (int n = 0; n < 4; ++n) {
+		double t1 = theta_lim[n], t2 = theta_lim[(n + 1) % 4];
+		if (std::abs(t2 - t1) > PI) {
+			if (t2 > t1) t1 += TWO_PI;
+			else t2 += TWO_PI;
+		}
+		if (std::abs(t2 - t1) < 1e-3) continue;
+		for (auto const &qt : gt) {
+			double theta = ((1.0 - qt.first) * t1 + (1.0 + qt.first) * t2) / 2.0;
+			double w_theta = qt.second * (t2 - t1) / 2.0;
+			double rho_lim = ref_distance[n] / std::cos(theta - theta0[n]);
+			for (auto const &qr : gr) {
+				double rho = (1.0 + qr.first) * rho_lim / 2.0;
+				double w_rho = qr.second * rho_lim / 2.0;
+				double xi = rho * std::cos(theta) + xi0, eta = rho * std::sin(theta) + eta0;
+				cplx F = q_kernel_jac(family, nx, ny, k, x0, nxv, e, xi, eta);
+				cplx Fl = q_kernel_jac(family, nx, ny, k, x0, nxv, lin, xi, eta);
+				I += (F - Fl) * (w_theta * w_rho * rho);
+			}
+		}
+	}
+	return I + quad_planar_near(family, nx, ny, k, lin, x0, nxv);
+}
+
+// Telles' cubic transformation of a Gauss rule towards xi0 (J. C. F. Telles,
+// Int. J. Numer. Meth. Eng. 24 (1987) 959-973, eqs. (21) and (27)), applied
+// per axis with the distance parameter d; nihu/library/telles_1987.hpp:30-96.
+static std::vector<QP> telles_rule(int degree, double xi0, double eta0, double d)
+{
+	std::vector<QP> q = gauss_quad_store(degree);
+	double r_bar;
+	if (d < 0.05) r_bar = 0.0;
+	else if (d <= 1.3) r_bar = 0.85 + 0.24 * std::log(d);
+	else if (d <= 3.618) r_bar = 0.893 + 0.0832 * std::log(d);
+	else r_bar = 1.0;
+	for (int dim = 0; dim < 2; ++dim) {
+		double e = dim == 0 ? xi0 : eta0;
+		double f = 1.0 + 2.0 * r_bar;
+		double qq = 1.0 / (2.0 * f) * ((e * (3.0 - 2.0 * r_bar) - 2.0 * e * e * e / f) * 1.0 / f - e);
+		double p = 1.0 / (3.0 * f * f) * (4.0 * r_bar * (1.0 - r_bar) + 3.0 * (1.0 - e * e));
+		double s = std::sqrt(qq * qq + p * p * p);
+		double gb = std::cbrt(-qq + s) + std::cbrt(-qq - s) + e / f;
+		double Q = 1.0 + 3.0 * gb * gb;
+		double a = (1.0 - r_bar) / Q, b = -3.0 * (1.0 - r_bar) * gb / Q, c = (3.0 * gb * gb + r_bar) / Q, dd = -b;
+		for (auto &pt : q) {
+			double x = dim == 0 ? pt.xi : pt.eta;
+			double nx = a * x * x * x + b * x * x + c * x + dd;
+			double jac = 3.0 * a * x * x + 2.0 * b * x + c;
+			if (dim == 0) pt.xi = nx; else pt.eta = nx;
+			pt.w *= jac;
+		}
+	}
+	return q;
+}
+
+// Near-singular integral with the Telles-transformed Gauss rule of degree 30
+// towards the image of the collocation point (inverse mapping tolerance 1e-6);
+// nihu/library/nearly_singular_collocational_telles.hpp:66-103.
+static cplx quad_near_telles(int family, int nx, int ny, double k, QElem const &e, V3 x0, V3 nxv, int degree)
+{
+	double xi0, eta0, zeta;
+	if (!quad_inverse_map(e, x0, 1e-6, 100, xi0, eta0, zeta))
+		throw std::runtime_error("Could not perform inverse mapping");
+	cplx I = 0.0;
+	for (auto const &q : telles_rule(degree, xi0, eta0, zeta))
+		I += q_kernel_jac(family, nx, ny, k, x0, nxv, e, q.xi, q.eta) * q.w;
+	return I;
+}
+
+// Near-singular method per kernel for collocation on elements other than
+// constant triangles (d / R_mean_edge < 1.5):
+// Helmholtz SLP polar (15, 15), DLP / DLPt / HSP Telles 30
+// (nihu/library/helmholtz_nearly_singular_integrals.hpp:266-310,360-404,
+// 461-505,560-604); Laplace SLP / DLP / DLPt polar (15, 15), HSP polar
+// (100, 100) (nihu/library/laplace_nearly_singular_integrals.hpp:443-486,
+// 539-582,634-677,730-773).
+static cplx quad_near(int family, int nx, int ny, double k, QElem const &e, V3 x0, V3 nxv)
+{
+	if (family == 1 && nx + ny > 0) return quad_near_telles(family, nx, ny, k, e, x0, nxv, 30);
+	if (family == 0 && nx == 1 && ny == 1) return quad_near_polar(family, nx, ny, k, e, x0, nxv, 100, 100);
+	return quad_near_polar(family, nx, ny, k, e, x0, nxv, 15, 15);
+}
+
+// Singular (same element): HSP by Guiggiani (5, 9) at the centre, the others
+// by the blind Duffy rule (nihu/library/helmholtz_singular_integrals.hpp:
+// 1190-1236, laplace_singular_integrals.hpp:1279-1318; DLP / DLPt are not
+// zero on quadrilaterals: normal_derivative_singular_integrals.hpp:13-58
+// covers tria_1 only).
+static cplx quad_singular(int family, int nx, int ny, double k, QElem const &e)
+{
+	if (nx == 1 && ny == 1) return quad_guiggiani(family, k, e, 0.0, 0.0, 5, 9);
+	return quad_singular_blind(family, nx, ny, k, e);
+}
+
+static bool near_needed_q(Policy const &p, double d2, QElem const &trial)
+{
+	if (p.near_mode == 2) return false;
+	double d = std::sqrt(d2);
+	double S = p.near_mode == 0 ? trial.R : trial.diam;
+	return d / S < p.near_limit;
+}
+
+// Pair integral with classification for a quadrilateral trial element; the
+// regular degree is 1 (field complexity) + the interval degree, or the fixed
+// degree of the policy (a Gauss square rule of that degree).
+static cplx quad_pair_integral(int family, int nx, int ny, double k, Policy const &p, TestPt const &t,
+	QElem const &trial, bool same_mesh)
+{
+	if (same_mesh && t.id == trial.id) return quad_singular(family, nx, ny, k, trial);
+	double d2 = dist2(t.x, trial.c);
+	if (near_needed_q(p, d2, trial)) return quad_near(family, nx, ny, k, trial, t.x, t.nu);
+	int degree = p.rule == 1 ? p.fixed_degree
+		: 1 + eval_interval(interval_table(1 + nx + ny), std::sqrt(d2) / trial.R);
+	return quad_regular(family, nx, ny, k, t.x, t.nu, trial, degree);
 }
 
 } // namespace orc
@@ -1116,6 +1618,150 @@ int orc_galerkin_block(int family, int nx, int ny, double k, const orc_policy *p
 	galerkin_block(family, nx, ny, k, p, te, se, tn3, sn3, same_mesh != 0, linear, blk);
 	for (int a = 0; a < 3; ++a)
 		for (int b = 0; b < 3; ++b) { blk18[2 * (3 * a + b)] = blk[a][b].real(); blk18[2 * (3 * a + b) + 1] = blk[a][b].imag(); }
+	ORC_CATCH
+}
+
+
+// Collocation with constant fields on meshes of triangles (npe = 3) or
+// quadrilaterals (npe = 4); test and trial meshes may differ in type.
+int orc_assemble_elems(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
+	const double *sxyz, int sn, const int *sel, int snt, int snpe, int same_mesh,
+	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	ORC_TRY
+	if (same_mesh && tnpe != snpe) throw std::invalid_argument("same mesh with different element types");
+	std::vector<Elem> ttri, stri;
+	std::vector<QElem> tq, sq;
+	std::vector<TestPt> tp;
+	read_elems(txyz, tn, tel, tnt, tnpe, &ttri, &tq, &tp);
+	if (!same_mesh) read_elems(sxyz, sn, sel, snt, snpe, &stri, &sq, nullptr);
+	std::vector<Elem> const &trit = same_mesh ? ttri : stri;
+	std::vector<QElem> const &quadt = same_mesh ? tq : sq;
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	int nr = rows ? n_rows : tnt;
+	cplx alpha(alpha_re, alpha_im);
+	if (n_threads <= 0) n_threads = orc_num_threads();
+	std::string err;
+	#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
+	for (int r = 0; r < nr; ++r) {
+		try {
+			int i = rows ? rows[r] : r;
+			for (int j = 0; j < snt; ++j) {
+				cplx v;
+				if (snpe == 4) v = quad_pair_integral(family, nx, ny, k, p, tp[i], quadt[j], same_mesh != 0);
+				else if (tnpe == 3) v = pair_integral(family, nx, ny, k, p, ttri[i], trit[j], same_mesh != 0);
+				else {
+					// quadrilateral test point against a triangle: never the same element
+					Elem const &trial = trit[j];
+					double d2 = dist2(tp[i].x, trial.c);
+					if (near_needed(p, d2, trial)) v = near_integral(family, nx, ny, k, trial, tp[i].x, tp[i].nu);
+					else v = regular_integral(family, nx, ny, k, tp[i].x, tp[i].nu, trial,
+						regular_degree(p, 0, nx, ny, d2, trial));
+				}
+				v *= alpha;
+				double *o = out + 2 * ((int64_t)j * ld + r);
+				if (accumulate) { o[0] += v.real(); o[1] += v.imag(); }
+				else { o[0] = v.real(); o[1] = v.imag(); }
+			}
+		} catch (std::exception const &ex) {
+			#pragma omp critical
+			err = ex.what();
+		}
+	}
+	if (!err.empty()) throw std::out_of_range(err);
+	ORC_CATCH
+}
+
+// Burton-Miller A and B (as orc_assemble_bm) on a mesh of npe-node elements
+int orc_assemble_bm_elems(const double *xyz, int nn, const int *el, int ne, int npe, double k, double alpha_re,
+	double alpha_im, const orc_policy *pol, const int *rows, int n_rows, double *A, double *B, int64_t ld, int n_threads)
+{
+	int nr = rows ? n_rows : ne;
+	for (int j = 0; j < ne; ++j)
+		for (int r = 0; r < nr; ++r) {
+			double *a = A + 2 * ((int64_t)j * ld + r), *b = B + 2 * ((int64_t)j * ld + r);
+			a[0] = a[1] = b[0] = b[1] = 0.0;
+		}
+	cplx al(alpha_re, alpha_im);
+	int s;
+	if ((s = orc_assemble_elems(xyz, nn, el, ne, npe, xyz, nn, el, ne, npe, 1, 1, 0, 0, k, 1.0, 0.0, pol, rows, n_rows,
+		B, ld, 1, n_threads))) return s;
+	if ((s = orc_assemble_elems(xyz, nn, el, ne, npe, xyz, nn, el, ne, npe, 1, 1, 1, 0, k, al.real(), al.imag(), pol,
+		rows, n_rows, B, ld, 1, n_threads))) return s;
+	orc_identity(ne, (al / 2.).real(), (al / 2.).imag(), rows, n_rows, B, ld);
+	if ((s = orc_assemble_elems(xyz, nn, el, ne, npe, xyz, nn, el, ne, npe, 1, 1, 1, 1, k, al.real(), al.imag(), pol,
+		rows, n_rows, A, ld, 1, n_threads))) return s;
+	orc_identity(ne, -.5, 0.0, rows, n_rows, A, ld);
+	if ((s = orc_assemble_elems(xyz, nn, el, ne, npe, xyz, nn, el, ne, npe, 1, 1, 0, 1, k, 1.0, 0.0, pol, rows, n_rows,
+		A, ld, 1, n_threads))) return s;
+	return 0;
+}
+
+// Element-level quadrilateral integrals for the tests.  coords12: the four
+// corners; x0, nx0: collocation point and unit normal (what 0-3, 7, 8);
+// xi0: a point of the reference square (what 4-6).
+//   0 regular, Gauss degree o1          4 Guiggiani HSP at xi0, orders (o1, o2)
+//   1 singular at the centre            5 blind Duffy rule of degree o1 at xi0
+//   2 polar near, orders (o1, o2)       6 inverse mapping of x0: (xi, eta, zeta, ok)
+//   3 Telles near, degree o1            7 near dispatch (NiHu's choice per kernel)
+//   8 planar integral over the element itself (planar elements only)
+int orc_quad_elem(int what, int family, int nx, int ny, double k, const double *coords12, const double *x0p,
+	const double *nx0p, const double *xi0p, int o1, int o2, double *out)
+{
+	ORC_TRY
+	V3 X[4];
+	for (int a = 0; a < 4; ++a) X[a] = {coords12[3 * a], coords12[3 * a + 1], coords12[3 * a + 2]};
+	QElem e = make_qelem(X, 0);
+	V3 x0 = x0p ? V3{x0p[0], x0p[1], x0p[2]} : e.c;
+	V3 n0 = nx0p ? V3{nx0p[0], nx0p[1], nx0p[2]} : e.nu;
+	cplx v = 0.0;
+	switch (what) {
+	case 0: v = quad_regular(family, nx, ny, k, x0, n0, e, o1); break;
+	case 1: v = quad_singular(family, nx, ny, k, e); break;
+	case 2: v = quad_near_polar(family, nx, ny, k, e, x0, n0, o1, o2); break;
+	case 3: v = quad_near_telles(family, nx, ny, k, e, x0, n0, o1); break;
+	case 4: v = quad_guiggiani(family, k, e, xi0p[0], xi0p[1], o1, o2); break;
+	case 5: {
+		V3 xs = q_x(e, xi0p[0], xi0p[1]);
+		V3 ns = normalized(q_normal(e, xi0p[0], xi0p[1]));
+		for (auto const &q : quad_duffy_face(o1, xi0p[0], xi0p[1]))
+			v += q_kernel_jac(family, nx, ny, k, xs, ns, e, q.xi, q.eta) * q.w;
+		break;
+	}
+	case 6: {
+		double xi, eta, zeta;
+		bool ok = quad_inverse_map(e, x0, o1 > 0 ? std::pow(10.0, -o1) : 1e-5, 100, xi, eta, zeta);
+		out[0] = xi; out[1] = eta; out[2] = zeta; out[3] = ok ? 1.0 : 0.0;
+		return 0;
+	}
+	case 7: v = quad_near(family, nx, ny, k, e, x0, n0); break;
+	case 8: v = quad_planar_near(family, nx, ny, k, e, x0, n0); break;
+	default: throw std::invalid_argument("unknown element integral");
+	}
+	out[0] = v.real(); out[1] = v.imag();
+	ORC_CATCH
+}
+
+// Guiggiani / Duffy / Telles rules for the tests: the Duffy rule of degree o1
+// around xi0 as (xi, eta, w) triples
+int orc_quad_duffy_rule(int degree, double xi0, double eta0, double *pts, int cap, int *n_out)
+{
+	ORC_TRY
+	auto r = quad_duffy_face(degree, xi0, eta0);
+	*n_out = (int)r.size();
+	if ((int)r.size() > cap) throw std::invalid_argument("buffer too small");
+	for (size_t i = 0; i < r.size(); ++i) { pts[3 * i] = r[i].xi; pts[3 * i + 1] = r[i].eta; pts[3 * i + 2] = r[i].w; }
+	ORC_CATCH
+}
+
+int orc_telles_rule(int degree, double xi0, double eta0, double d, double *pts, int cap, int *n_out)
+{
+	ORC_TRY
+	auto r = telles_rule(degree, xi0, eta0, d);
+	*n_out = (int)r.size();
+	if ((int)r.size() > cap) throw std::invalid_argument("buffer too small");
+	for (size_t i = 0; i < r.size(); ++i) { pts[3 * i] = r[i].xi; pts[3 * i + 1] = r[i].eta; pts[3 * i + 2] = r[i].w; }
 	ORC_CATCH
 }
 

@@ -64,6 +64,14 @@ def lib():
         l.orc_singular_rule.argtypes = [C.c_int, D, C.c_int, I]
         l.orc_galerkin_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, P, D, I, D, I, C.c_int, C.c_int,
                                          C.c_int, D]
+        l.orc_assemble_elems.argtypes = [D, C.c_int, I, C.c_int, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_assemble_bm_elems.argtypes = [D, C.c_int, I, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            P, I, C.c_int, D, D, C.c_int64, C.c_int]
+        l.orc_quad_elem.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, D, C.c_int, C.c_int, D]
+        l.orc_quad_duffy_rule.argtypes = [C.c_int, C.c_double, C.c_double, D, C.c_int, I]
+        l.orc_telles_rule.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, C.c_int, I]
         _lib = l
     return _lib
 
@@ -155,10 +163,18 @@ def assemble(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, rows=None,
         out = np.zeros((nr, len(s_tris)), dtype=np.complex128, order="F")
         accumulate = False
     alpha = complex(alpha)
-    _check(lib().orc_assemble(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
-                              _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
-                              family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol), _i(rows_a), nr,
-                              out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0], int(accumulate), threads))
+    if t_tris.shape[1] == 3 and s_tris.shape[1] == 3:
+        _check(lib().orc_assemble(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                                  _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
+                                  family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol), _i(rows_a), nr,
+                                  out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0], int(accumulate), threads))
+    else:
+        # quadrilaterals (4 nodes per element) on either side
+        _check(lib().orc_assemble_elems(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris), t_tris.shape[1],
+                                        _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), s_tris.shape[1],
+                                        int(same), family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol),
+                                        _i(rows_a), nr, out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0],
+                                        int(accumulate), threads))
     return out
 
 
@@ -171,9 +187,15 @@ def assemble_bm(nodes, tris, k, alpha=None, pol=None, rows=None, threads=0):
     nr = len(tris) if rows_a is None else len(rows_a)
     A = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
     B = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
-    _check(lib().orc_assemble_bm(_d(nodes), len(nodes), _i(tris), len(tris), k, alpha.real, alpha.imag,
-                                 C.byref(pol), _i(rows_a), nr, A.ctypes.data_as(C.POINTER(C.c_double)),
-                                 B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
+    if tris.shape[1] == 3:
+        _check(lib().orc_assemble_bm(_d(nodes), len(nodes), _i(tris), len(tris), k, alpha.real, alpha.imag,
+                                     C.byref(pol), _i(rows_a), nr, A.ctypes.data_as(C.POINTER(C.c_double)),
+                                     B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
+    else:
+        _check(lib().orc_assemble_bm_elems(_d(nodes), len(nodes), _i(tris), len(tris), tris.shape[1], k, alpha.real,
+                                           alpha.imag, C.byref(pol), _i(rows_a), nr,
+                                           A.ctypes.data_as(C.POINTER(C.c_double)),
+                                           B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
     return A, B
 
 
@@ -254,3 +276,39 @@ def overlap(a, b):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+QUAD_REGULAR, QUAD_SINGULAR, QUAD_POLAR, QUAD_TELLES, QUAD_GUIGGIANI, QUAD_DUFFY, QUAD_INVERSE, QUAD_NEAR, \
+    QUAD_PLANAR = range(9)
+
+
+def quad_elem(what, family, nx, ny, k, coords, x0=None, nx0=None, xi0=None, o1=0, o2=0):
+    """Element integral over a quadrilateral (coords: 4 x 3, one corner per
+    row); `what` is one of the QUAD_* constants (see orc_quad_elem).
+    QUAD_INVERSE returns (xi, eta, zeta, converged) of x0 (o1: -log10 tol)."""
+    cs = np.ascontiguousarray(coords, dtype=np.float64).reshape(12)
+    arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (x0, nx0, xi0)]
+    out = np.zeros(4)
+    _check(lib().orc_quad_elem(what, family, nx, ny, k, _d(cs), *[None if a is None else _d(a) for a in arrs],
+                               o1, o2, _d(out)))
+    if what == QUAD_INVERSE:
+        return out.copy()
+    return complex(out[0], out[1])
+
+
+def quad_duffy_rule(degree, xi0=0.0, eta0=0.0):
+    """Blind Duffy rule around (xi0, eta0) on [-1, 1]^2: rows (xi, eta, w)."""
+    cap = 4 * (degree // 2 + 1) ** 2
+    pts = np.zeros((cap, 3))
+    n = C.c_int(0)
+    _check(lib().orc_quad_duffy_rule(degree, xi0, eta0, _d(pts), cap, C.byref(n)))
+    return pts[:n.value].copy()
+
+
+def telles_rule(degree, xi0, eta0, d):
+    """Telles-transformed Gauss square rule: rows (xi, eta, w)."""
+    cap = (degree // 2 + 1) ** 2
+    pts = np.zeros((cap, 3))
+    n = C.c_int(0)
+    _check(lib().orc_telles_rule(degree, xi0, eta0, d, _d(pts), cap, C.byref(n)))
+    return pts[:n.value].copy()

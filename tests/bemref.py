@@ -179,3 +179,132 @@ def square_mesh(n):
             b, d = a + n + 1, a + 1
             tris += [[a, b, b + 1], [a, b + 1, d]]
     return nodes, np.array(tris, dtype=np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Quadrilaterals (bilinear, corners (-1,-1), (1,-1), (1,1), (-1,1))
+# ---------------------------------------------------------------------------
+QC = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])
+
+
+def quad_geom(coords, xi, eta):
+    """Points and Jacobian vectors dx/dxi x dx/deta of a bilinear quadrilateral
+    at arrays of reference coordinates."""
+    xi = np.asarray(xi, dtype=float)
+    eta = np.asarray(eta, dtype=float)
+    N = np.stack([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)], -1) / 4
+    Dxi = np.stack([-(1 - eta), (1 - eta), (1 + eta), -(1 + eta)], -1) / 4
+    Deta = np.stack([-(1 - xi), -(1 + xi), (1 + xi), (1 - xi)], -1) / 4
+    C = np.asarray(coords, dtype=float)
+    return N @ C, np.cross(Dxi @ C, Deta @ C)
+
+
+def np_kernel_pts(family, nx, ny, k, x, Y, nxv, NY):
+    """np_kernel with one trial normal per point (rows of NY)."""
+    rv = Y - x[None, :]
+    r = np.linalg.norm(rv, axis=1)
+    kk = 0.0 if family == 0 else k
+    e = np.exp(-1j * kk * r)
+    if nx == 0 and ny == 0:
+        return e / (FOUR_PI * r)
+    rdny = np.einsum("ij,ij->i", rv, NY) / r
+    rdnx = -(rv @ nxv) / r
+    if nx + ny == 1:
+        g1 = e * (-1.0 - 1j * kk * r) / (FOUR_PI * r * r)
+        return g1 * (rdny if ny == 1 else rdnx)
+    ikr = 1j * kk * r
+    g = e / (FOUR_PI * r ** 3)
+    return g * (3.0 + ikr * (3.0 + ikr)) * rdnx * rdny + g * (1.0 + ikr) * (NY @ nxv)
+
+
+def quad_brute(family, nx, ny, k, coords, x0, nx0, levels=4, n_gauss=8):
+    """Integral of K(x0, y) over a quadrilateral by Gauss rules on a uniform
+    2^levels x 2^levels subdivision of the reference square."""
+    g, w = np.polynomial.legendre.leggauss(n_gauss)
+    m = 2 ** levels
+    h = 2.0 / m
+    a = -1.0 + h * (np.arange(m) + 0.5)
+    cx = (a[:, None] + g[None, :] * h / 2).ravel()
+    cw = np.tile(w * h / 2, m)
+    XI, ETA = np.meshgrid(cx, cx, indexing="ij")
+    W = np.outer(cw, cw).ravel()
+    Y, NJ = quad_geom(coords, XI.ravel(), ETA.ravel())
+    J = np.linalg.norm(NJ, axis=1)
+    K = np_kernel_pts(family, nx, ny, k, np.asarray(x0, float), Y, np.asarray(nx0, float), NJ / J[:, None])
+    return np.sum(K * J * W)
+
+
+def polygon_exact(corners, p, kind):
+    """Closed forms over a planar polygon (corners: n x 2, counter-clockwise)
+    at an interior point p of its plane: kind 'slp' = int 1/(4 pi r) dA,
+    kind 'hsp' = finite part of int 1/(4 pi r^3) dA.  Per side, with h the
+    distance from p to the side's line and phi the angle from the foot of the
+    perpendicular: int rho dphi = h ln|sec + tan|, int dphi / rho = sin / h."""
+    C = np.asarray(corners, float)
+    p = np.asarray(p, float)
+    tot = 0.0
+    for i in range(len(C)):
+        a, b = C[i] - p, C[(i + 1) % len(C)] - p
+        t = (b - a) / np.linalg.norm(b - a)
+        foot = a - t * np.dot(a, t)
+        h = np.linalg.norm(foot)
+        s1, s2 = np.dot(a, t) / h, np.dot(b, t) / h     # tan(phi) at the side's ends
+        if kind == "slp":
+            tot += h * (np.arcsinh(s2) - np.arcsinh(s1))
+        else:
+            tot += (s2 / np.sqrt(1 + s2 * s2) - s1 / np.sqrt(1 + s1 * s1)) / h
+    return tot / FOUR_PI if kind == "slp" else -tot / FOUR_PI
+
+
+def hsp_finite_part(family, k, coords, xi0, eps=(4e-3, 2e-3, 1e-3), n_theta=48, n_rho=24):
+    """Hadamard finite part of the HSP kernel over a (warped) quadrilateral at
+    x0 = x(xi0), n_x = n(xi0): lim [int over the surface outside the ball
+    |y - x0| < eps of K dS - 1/(2 eps)], from polar integration in the
+    reference square (the ball's boundary found by Newton per direction) and
+    Richardson extrapolation over the eps values (error O(eps))."""
+    xi0 = np.asarray(xi0, float)
+    x0, n0 = quad_geom(coords, xi0[0], xi0[1])
+    nx0 = n0 / np.linalg.norm(n0)
+    gt, wt = np.polynomial.legendre.leggauss(n_theta)
+    gr, wr = np.polynomial.legendre.leggauss(n_rho)
+    vals = []
+    eps_list = tuple(eps)
+    for eps in eps_list:
+        tot = 0.0
+        for s in range(4):
+            c1, c2 = QC[s] - xi0, QC[(s + 1) % 4] - xi0
+            t1, t2 = np.arctan2(c1[1], c1[0]), np.arctan2(c2[1], c2[0])
+            if t2 < t1:
+                t2 += 2 * np.pi
+            th = (t1 + t2) / 2 + gt * (t2 - t1) / 2
+            wth = wt * (t2 - t1) / 2
+            d = np.stack([np.cos(th), np.sin(th)], -1)
+            # distance to the side along each direction
+            nrm = np.array([c2[1] - c1[1], -(c2[0] - c1[0])])
+            rho_hat = np.dot(c1, nrm) / (d @ nrm)
+            # ball boundary: |x(xi0 + rho d) - x0| = eps, Newton from the linear guess
+            rho_e = np.full(n_theta, eps / np.linalg.norm(quad_geom(coords, xi0[0] + 1e-7, xi0[1])[0] - x0) * 1e-7)
+            for it in range(30):
+                P = xi0[None, :] + rho_e[:, None] * d
+                Y, _ = quad_geom(coords, P[:, 0], P[:, 1])
+                f = np.linalg.norm(Y - x0, axis=1) - eps
+                h = 1e-9
+                P2 = xi0[None, :] + (rho_e + h)[:, None] * d
+                Y2, _ = quad_geom(coords, P2[:, 0], P2[:, 1])
+                df = (np.linalg.norm(Y2 - x0, axis=1) - eps - f) / h
+                rho_e = rho_e - f / df
+            # radial integral on geometric subintervals from rho_e to rho_hat
+            for i in range(n_theta):
+                edges = np.geomspace(rho_e[i], rho_hat[i], 12)
+                R = ((edges[:-1, None] + edges[1:, None]) / 2 + gr[None, :] * (edges[1:, None] - edges[:-1, None]) / 2).ravel()
+                WR = (wr[None, :] * (edges[1:, None] - edges[:-1, None]) / 2).ravel()
+                P = xi0[None, :] + R[:, None] * d[i][None, :]
+                Y, NJ = quad_geom(coords, P[:, 0], P[:, 1])
+                J = np.linalg.norm(NJ, axis=1)
+                K = np_kernel_pts(family, 1, 1, k, x0, Y, nx0, NJ / J[:, None])
+                tot += wth[i] * np.sum(K * J * R * WR)
+        vals.append(tot - 1.0 / (2.0 * eps))
+    # Richardson on eps, 2 eps ratios (errors O(eps) + O(eps^2))
+    e = np.array(eps_list)
+    A = np.stack([np.ones_like(e), e, e * e], -1)
+    return np.linalg.solve(A, np.array(vals, dtype=complex))[0]
