@@ -5,7 +5,8 @@
  *     result << test_space * op[trial_space]
  * for 3D Laplace / Helmholtz SLP, DLP, DLPt and HSP kernels on flat 3-node
  * triangles: collocation (dirac(constant_view) x constant_view) and Galerkin
- * (constant_view x constant_view, isoparametric_view x isoparametric_view).
+ * (constant_view x constant_view, isoparametric_view x isoparametric_view);
+ * and on 4-node bilinear quadrilaterals: collocation with constant fields.
  * Every entry point returns an
  * nb_status; on failure nb_last_error() returns a message for the calling
  * thread.  Plain pointers and sizes only.
@@ -125,6 +126,17 @@ int nb_device_info(char *name, int32_t name_len, int32_t *n_cu, int64_t *hbm_byt
  * element ids = row numbers).  Geometry (centres, normals, Jacobians, sizes)
  * is computed in FP64 and kept resident on the current device. */
 int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32_t n_tri, nb_mesh **out);
+/* Upload a mesh of 4-node bilinear quadrilaterals (quad_1_elem, possibly
+ * warped): quad[n_quad][4] with corners ordered around the element, the
+ * normal dx/dxi x dx/deta outwards.  Collocation with constant fields
+ * (nb_assemble with NB_COLLOCATION, nb_assemble_bm); the Galerkin formalism
+ * and the sweep reject these meshes (NB_ERR_UNSUPPORTED).
+ * Replaces create_mesh(nodes, elements, quad_1_tag()) and the quad_1 paths
+ * of the collocational double integral: nihu/core/double_integral.hpp:599-673,
+ * nihu/library/guiggiani_1992.hpp:138-330 (hypersingular diagonal),
+ * nihu/library/nearly_singular_collocational.hpp:71-203 and
+ * nearly_singular_collocational_telles.hpp:66-103 (near-singular pairs). */
+int nb_mesh_create_quads(const double *xyz, int32_t n_nodes, const int32_t *quad, int32_t n_quad, nb_mesh **out);
 int nb_mesh_destroy(nb_mesh *mesh);
 int nb_mesh_info(const nb_mesh *mesh, int32_t *n_nodes, int32_t *n_tri, int64_t *device_bytes);
 

@@ -21,6 +21,7 @@
 #include "nbem_kernels.hpp"
 #include "nbem_sweep.hpp"
 #include "nbem_galerkin.hpp"
+#include "nbem_quad.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -47,6 +48,10 @@ int nbg_singular_rule(int match, double *out, int cap);
 int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, int32_t *pf, int32_t *info);
 int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk, int32_t *order_out, int32_t *elem,
 	uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin);
+int nbg_geometry_quad(const double *xyz, int n_nodes, const int32_t *quad, int n, double *c, double *nu,
+	double *J, double *R, double *diam, double *X);
+int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out);
+int nbg_quad_duffy(int ng, double x0, double y0, double *out);
 }
 
 namespace {
@@ -195,6 +200,12 @@ int device_rules(const nbk::Rules **out)
 		nbg_gauss_legendre(21, r.gl21x, r.gl21w);
 		if (nbg_dunavant(7, r.d7xi, r.d7eta, r.d7w, 13) != 13) return fail(NB_ERR_RUNTIME, "Dunavant 7 table");
 		if (nbg_dunavant(14, r.d14xi, r.d14eta, r.d14w, 42) != 42) return fail(NB_ERR_RUNTIME, "Dunavant 14 table");
+		nbg_gauss_legendre(3, r.gl3x, r.gl3w);
+		nbg_gauss_legendre(5, r.gl5x, r.gl5w);
+		nbg_gauss_legendre(8, r.gl8x, r.gl8w);
+		nbg_gauss_legendre(16, r.gl16x, r.gl16w);
+		nbg_gauss_legendre(51, r.gl51x, r.gl51w);
+		if (nbg_quad_duffy(4, 0.0, 0.0, &r.duffy[0][0]) != 64) return fail(NB_ERR_RUNTIME, "Duffy rule");
 		DevPtr p;
 		int s = dev_alloc(p, sizeof(r));
 		if (s) return s;
@@ -242,6 +253,7 @@ struct nb_mesh {
 	uint64_t uid = 0;   // unique over the process: cache key that survives address reuse
 	int device = 0;
 	int n_nodes = 0, n = 0;
+	int npe = 3;                               // nodes per element: 3 triangles, 4 quadrilaterals
 	std::vector<double> c, nu, J, R, diam, X;   // SoA host copies (FP64)
 	DevPtr d_c, d_nuf, d_nu64, d_X, d_J;
 	int thr_mode = -1;
@@ -249,6 +261,7 @@ struct nb_mesh {
 	DevPtr d_thr, d_rec;
 	DevPtr d_qt[nbk::NSLOT];
 	int qn[nbk::NSLOT] = {0};
+	DevPtr d_qq[nbk::QN_MAX + 1];              // quadrilaterals: Gauss square tables by order
 	std::vector<std::unique_ptr<NearList> > near_cache;
 	std::vector<int32_t> tri;                  // element-major node ids (host)
 	DevPtr d_tri;                              // node ids [3][n]
@@ -316,6 +329,20 @@ int ensure_quad(nb_mesh *m, int degree)
 	int s = upload(m->d_qt[degree], tab);
 	if (s) return s;
 	m->qn[degree] = Q;
+	m->bytes += (int64_t)tab.size() * 4;
+	return NB_OK;
+}
+
+// Gauss square table with N x N points of a quadrilateral mesh
+int ensure_qq(nb_mesh *m, int N)
+{
+	if (N < 1 || N > nbk::QN_MAX) return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
+	if (m->d_qq[N]) return NB_OK;
+	std::vector<float> tab((size_t)m->n * N * N * 8);
+	if (nbg_quad4_table(N, m->n, m->c.data(), m->X.data(), tab.data()) != N * N)
+		return fail(NB_ERR_RUNTIME, "quadrilateral quadrature table");
+	int s = upload(m->d_qq[N], tab);
+	if (s) return s;
 	m->bytes += (int64_t)tab.size() * 4;
 	return NB_OK;
 }
@@ -630,9 +657,146 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 }
 
 // one assembly into device buffers (dtype, ld, accumulate taken from outs)
+// Collocation on a quadrilateral trial mesh: the regular pass (FP32) and the
+// near-singular and diagonal passes (FP64) on a side stream, disjoint entries.
+int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	nb_mesh *trial = jb.trial;
+	const nb_mesh *test = jb.test;
+	int s;
+	// The near-singular methods of quadrilaterals start from the image of the
+	// collocation point on the element (Newton from the centre), which is only
+	// reliable close to the element: NiHu applies them for d / R < 1.5.
+	if (jb.pol.near_mode == NB_NEAR_DIAMETER || (jb.pol.near_mode == NB_NEAR_MEAN_EDGE && jb.pol.near_limit > 1.5))
+		return fail(NB_ERR_UNSUPPORTED, "quadrilaterals: the near-singular rule must be mean edge with limit <= 1.5 "
+			"(NiHu's d / R < 1.5) or none");
+	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
+	int fixed_n = 0;
+	if (jb.pol.rule == NB_RULE_FIXED) {
+		fixed_n = jb.pol.fixed_degree / 2 + 1;
+		if ((s = ensure_qq(trial, fixed_n))) return s;
+	} else {
+		// degree 1 + {0, 2, 4, 6, 8, 9}: Gauss orders 1..6
+		for (int N = 1; N <= 6; ++N)
+			if ((s = ensure_qq(trial, N))) return s;
+	}
+	NearList *nl = nullptr;
+	if ((s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
+	nbk::QuadArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = jb.same;
+	a.tile_cols = jb.tile_cols > 0 ? jb.tile_cols : 32;
+	a.row_tiles = (test->n + 255) / 256;
+	a.tn = test->n; a.sn = trial->n;
+	a.tc = (const double *)test->d_c.get();
+	a.tnu = (const float *)test->d_nuf.get();
+	a.sc = (const double *)trial->d_c.get();
+	a.thr = (const double *)trial->d_thr.get();
+	for (int N = 1; N <= nbk::QN_MAX; ++N) a.qt[N] = (const nbk::QuadPt *)trial->d_qq[N].get();
+	a.fixed_n = fixed_n;
+	const double two_pi = 6.283185307179586476925286766559005768;
+	a.k = (float)jb.k;
+	a.k_rev = jb.k / two_pi;
+	a.k_revf = (float)a.k_rev;
+	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
+	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
+	const long long grid = (long long)a.row_tiles * ((trial->n + a.tile_cols - 1) / a.tile_cols);
+
+	hipStream_t side = st;
+	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+	const bool has_fp64 = nl->n > 0 || jb.same;
+	if (has_fp64) {
+		if ((s = side_stream(&side))) return s;
+		if (!side) side = st;
+		NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+		NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+		NB_HIP(hipEventRecord(ev_fork, st));
+		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+	}
+	if (grid > 0) {
+		using namespace nbk;
+		const int pe = prof_begin(0, st);
+#define NB_QR(F, K, M) hipLaunchKernelGGL((quad_regular_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, a)
+		if (jb.mode == OUT_BM) NB_QR(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_QR(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_QR(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_QR(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_QR(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_QR(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_QR(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_QR(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_QR(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_QR
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	const nbk::Rules *rules = nullptr;
+	if ((s = device_rules(&rules))) return s;
+	nbk::NearArgs na;
+	std::memset(&na, 0, sizeof(na));
+	na.tn = test->n; na.sn = trial->n;
+	na.tc = (const double *)test->d_c.get();
+	na.tnu = (const double *)test->d_nu64.get();
+	na.X = (const double *)trial->d_X.get();
+	na.sJ = (const double *)trial->d_J.get();
+	na.snu = (const double *)trial->d_nu64.get();
+	na.k = jb.k;
+	na.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+	na.out0 = out0; na.out1 = out1; na.ld = ld; na.dtype = dtype; na.accumulate = accumulate;
+	na.rules = rules;
+	for (int pass = 0; pass < 2; ++pass) {
+		const bool diag = pass == 1;
+		if (diag && !jb.same) continue;
+		na.n_pairs = diag ? test->n : nl->n;
+		na.pi = diag ? nullptr : (const int *)nl->pi.get();
+		na.pj = diag ? nullptr : (const int *)nl->pj.get();
+		if (na.n_pairs == 0) continue;
+		const int g = (na.n_pairs + 63) / 64;
+		const int pe = prof_begin(diag ? 2 : 1, side);
+		using namespace nbk;
+#define NB_Q64(F, K, M) do { \
+		if (diag) hipLaunchKernelGGL((quad_fp64_kernel<F, K, M, true>), dim3(g), dim3(64), 0, side, na); \
+		else hipLaunchKernelGGL((quad_fp64_kernel<F, K, M, false>), dim3(g), dim3(64), 0, side, na); } while (0)
+		if (jb.mode == OUT_BM) NB_Q64(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_Q64(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_Q64(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_Q64(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_Q64(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_Q64(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_Q64(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_Q64(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_Q64(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_Q64
+		NB_HIP(hipGetLastError());
+		prof_end(pe, side);
+	}
+	if (has_fp64) {
+		NB_HIP(hipEventRecord(ev_join, side));
+		NB_HIP(hipStreamWaitEvent(st, ev_join, 0));
+		NB_HIP(hipEventDestroy(ev_fork));
+		NB_HIP(hipEventDestroy(ev_join));
+	}
+	return NB_OK;
+}
+
 int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
 	if (jb.formalism != NB_COLLOCATION) return run_galerkin(jb, out0, ld, dtype, accumulate, st);
+	if (jb.trial->npe == 4) return run_quad(jb, out0, out1, ld, dtype, accumulate, st);
 	nb_mesh *trial = jb.trial;
 	const nb_mesh *test = jb.test;
 	int s;
@@ -862,6 +1026,7 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	int s;
 	if ((s = check_policy(policy))) return s;
 	if ((s = current_device_matches(mesh))) return s;
+	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "the frequency sweep is implemented for triangle meshes only");
 	const int n = mesh->n;
 	const int nblk = tune_block > 0 ? tune_block : n;
 	if (ld < nblk || k_stride < ld * (int64_t)nblk) return fail(NB_ERR_ARG, "ld / k_stride too small for the matrices");
@@ -1015,6 +1180,8 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 	const bool nodal = formalism == NB_GALERKIN_LINEAR;
 	const bool ident = kernel.family == NB_IDENTITY;
 	int s;
+	if (test->npe != 3 || trial->npe != 3)
+		return fail(NB_ERR_UNSUPPORTED, "the Galerkin formalism is implemented for triangles only");
 	if (!ident) {
 		if (kernel.family != NB_LAPLACE && kernel.family != NB_HELMHOLTZ) return fail(NB_ERR_ARG, "unknown kernel family");
 		if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
@@ -1094,6 +1261,36 @@ int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32
 		(s = upload(m->d_X, m->X)) || (s = upload(m->d_J, m->J)) || (s = upload(m->d_tri, tri_soa)))
 		return s;
 	m->bytes = (int64_t)n * (3 * 8 + 3 * 8 + 3 * 4 + 9 * 8 + 8 + 3 * 4);
+	*out = m.release();
+	return NB_OK;
+}
+
+int nb_mesh_create_quads(const double *xyz, int32_t n_nodes, const int32_t *quad, int32_t n_quad, nb_mesh **out)
+{
+	if (!out) return fail(NB_ERR_ARG, "null output handle");
+	*out = nullptr;
+	if (!xyz || !quad || n_nodes <= 0 || n_quad <= 0) return fail(NB_ERR_ARG, "empty mesh");
+	static std::atomic<uint64_t> next_uid(1ull << 62);
+	std::unique_ptr<nb_mesh> m(new nb_mesh());
+	m->uid = next_uid++;
+	const int n = n_quad;
+	m->n_nodes = n_nodes;
+	m->n = n;
+	m->npe = 4;
+	m->c.resize(3 * (size_t)n); m->nu.resize(3 * (size_t)n); m->J.resize(n); m->R.resize(n);
+	m->diam.resize(n); m->X.resize(12 * (size_t)n);
+	int g = nbg_geometry_quad(xyz, n_nodes, quad, n, m->c.data(), m->nu.data(), m->J.data(), m->R.data(),
+		m->diam.data(), m->X.data());
+	if (g == 1) return fail(NB_ERR_ARG, "node index out of range");
+	if (g == 2) return fail(NB_ERR_ARG, "degenerate element (zero normal)");
+	NB_HIP(hipGetDevice(&m->device));
+	m->tri.assign(quad, quad + 4 * (size_t)n);
+	std::vector<float> nuf(m->nu.begin(), m->nu.end());
+	int s;
+	if ((s = upload(m->d_c, m->c)) || (s = upload(m->d_nu64, m->nu)) || (s = upload(m->d_nuf, nuf)) ||
+		(s = upload(m->d_X, m->X)) || (s = upload(m->d_J, m->J)))
+		return s;
+	m->bytes = (int64_t)n * (3 * 8 + 3 * 8 + 3 * 4 + 12 * 8 + 8);
 	*out = m.release();
 	return NB_OK;
 }
@@ -1211,7 +1408,8 @@ int nb_classify(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 	a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = same_mesh ? 1 : 0;
 	a.tn = test->n; a.sn = trial->n; a.order = 1 + nx + ny;
 	a.fixed_degree = policy->rule == NB_RULE_FIXED ? policy->fixed_degree : -1;
-	a.field_complexity = 0;
+	// the jacobian order of quad_1 adds 1 to the regular degree
+	a.field_complexity = trial->npe == 4 ? 1 : 0;
 	a.tc = (const double *)test->d_c.get(); a.sc = (const double *)trial->d_c.get();
 	a.thr = (const double *)trial->d_thr.get();
 	a.cls = cls;
@@ -1233,6 +1431,7 @@ int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_la
 	if (n_k <= 0) return fail(NB_ERR_ARG, "n_k must be positive");
 	int s;
 	if ((s = check_policy(policy))) return s;
+	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "the frequency sweep is implemented for triangle meshes only");
 	if (policy->rule != NB_RULE_FIXED) {
 		// the NiHu-degree policy runs the per-wavenumber kernel: only the tile matters
 		*best = {32, 1, 0};

@@ -252,3 +252,148 @@ int nbg_quad_table(int degree, int n, const double *c, const double *J, const do
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Quadrilaterals (quad_1_elem): x(xi, eta) = sum_a N_a X_a on [-1, 1]^2 with
+// N = (1 -+ xi)(1 -+ eta) / 4 for the corners (-1,-1), (1,-1), (1,1), (-1,1)
+// (nihu/library/lib_shape.hpp:215-310, lib_domain.cpp:74-94); centre x(0, 0),
+// R = mean edge length over edges 01 12 23 30 (nihu/core/element.hpp:281-291),
+// diameter = largest corner distance.  Sums are accumulated from zero in
+// corner order, as the oracle does, so that centres and R round identically.
+
+namespace {
+const double QCX[4] = {-1.0, 1.0, 1.0, -1.0}, QCY[4] = {-1.0, -1.0, 1.0, 1.0};
+
+inline P3 quad_x(const P3 X[4], double xi, double eta)
+{
+	double N[4] = {(1.0 - xi) * (1.0 - eta) / 4.0, (1.0 + xi) * (1.0 - eta) / 4.0,
+		(1.0 + xi) * (1.0 + eta) / 4.0, (1.0 - xi) * (1.0 + eta) / 4.0};
+	P3 x = {0.0, 0.0, 0.0};
+	for (int a = 0; a < 4; ++a) x = {x.x + X[a].x * N[a], x.y + X[a].y * N[a], x.z + X[a].z * N[a]};
+	return x;
+}
+
+inline P3 quad_normal(const P3 X[4], double xi, double eta)
+{
+	double A[4] = {-(1.0 - eta) / 4.0, (1.0 - eta) / 4.0, (1.0 + eta) / 4.0, -(1.0 + eta) / 4.0};
+	double B[4] = {-(1.0 - xi) / 4.0, -(1.0 + xi) / 4.0, (1.0 + xi) / 4.0, (1.0 - xi) / 4.0};
+	P3 a = {0.0, 0.0, 0.0}, b = {0.0, 0.0, 0.0};
+	for (int c = 0; c < 4; ++c) {
+		a = {a.x + X[c].x * A[c], a.y + X[c].y * A[c], a.z + X[c].z * A[c]};
+		b = {b.x + X[c].x * B[c], b.y + X[c].y * B[c], b.z + X[c].z * B[c]};
+	}
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+inline void load_quad(const double *X, int n, int e, P3 V[4])
+{
+	for (int a = 0; a < 4; ++a) V[a] = {X[(3 * a) * n + e], X[(3 * a + 1) * n + e], X[(3 * a + 2) * n + e]};
+}
+} // namespace
+
+extern "C" {
+
+// Quadrilateral geometry: c[3][n], nu[3][n] (unit normal at the centre),
+// J[n] (|normal| at the centre), R[n], diam[n], X[12][n].  Returns 0, 1 for a
+// bad node index, 2 for a degenerate element (zero normal at a Gauss point of
+// order 2 or at the centre).
+int nbg_geometry_quad(const double *xyz, int n_nodes, const int32_t *quad, int n, double *c, double *nu,
+	double *J, double *R, double *diam, double *X)
+{
+	for (int e = 0; e < n; ++e) {
+		P3 V[4];
+		for (int a = 0; a < 4; ++a) {
+			int v = quad[4 * e + a];
+			if (v < 0 || v >= n_nodes) return 1;
+			V[a] = {xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
+		}
+		P3 cc = quad_x(V, 0.0, 0.0);
+		P3 nv = quad_normal(V, 0.0, 0.0);
+		double j = nrm(nv);
+		if (!(j > 0.0)) return 2;
+		const double g = 0.577350269189625764509148780502;
+		for (int q = 0; q < 4; ++q)
+			if (!(nrm(quad_normal(V, (q & 1) ? g : -g, (q & 2) ? g : -g)) > 0.0)) return 2;
+		double d = 0.0;
+		for (int a = 0; a < 4; ++a) d += nrm(sub(V[(a + 1) % 4], V[a]));
+		double dm = 0.0;
+		for (int a = 0; a < 4; ++a)
+			for (int b = a + 1; b < 4; ++b) dm = std::max(dm, nrm(sub(V[b], V[a])));
+		c[e] = cc.x; c[n + e] = cc.y; c[2 * n + e] = cc.z;
+		nu[e] = nv.x / j; nu[n + e] = nv.y / j; nu[2 * n + e] = nv.z / j;
+		J[e] = j;
+		R[e] = d / 4.0;
+		diam[e] = dm;
+		for (int a = 0; a < 4; ++a) {
+			X[(3 * a) * n + e] = V[a].x;
+			X[(3 * a + 1) * n + e] = V[a].y;
+			X[(3 * a + 2) * n + e] = V[a].z;
+		}
+	}
+	return 0;
+}
+
+// Gauss square rule with N x N points (xi outer, eta inner) for all
+// quadrilaterals: out[(e * N*N + q) * 8 + {0..7}] = (y_q - c_e) xyz,
+// w_q J(xi_q) / (4 pi), unit normal at y_q xyz, |y_q - c_e|^2 (floats).
+int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out)
+{
+	if (N < 1 || N > 64) return -1;
+	std::vector<double> gx(N), gw(N);
+	nbg_gauss_legendre(N, gx.data(), gw.data());
+	const double four_pi = 4.0 * 3.141592653589793238462643383279502884;
+	for (int e = 0; e < n; ++e) {
+		P3 V[4];
+		load_quad(X, n, e, V);
+		P3 cc = {c[e], c[n + e], c[2 * n + e]};
+		for (int i = 0; i < N; ++i)
+			for (int j = 0; j < N; ++j) {
+				P3 y = quad_x(V, gx[i], gx[j]);
+				P3 nv = quad_normal(V, gx[i], gx[j]);
+				double J = nrm(nv);
+				P3 o = sub(y, cc);
+				float *p = out + ((size_t)e * N * N + (size_t)i * N + j) * 8;
+				p[0] = (float)o.x; p[1] = (float)o.y; p[2] = (float)o.z;
+				p[3] = (float)(gw[i] * gw[j] * J / four_pi);
+				p[4] = (float)(nv.x / J); p[5] = (float)(nv.y / J); p[6] = (float)(nv.z / J);
+				p[7] = (float)(o.x * o.x + o.y * o.y + o.z * o.z);
+			}
+	}
+	return N * N;
+}
+
+// Duffy-type blind rule around (x0, y0) of [-1, 1]^2 from the Gauss square
+// rule with ng x ng points: four triangles (x0, corner s, corner s+1), each
+// the image of the square under the bilinear map of the degenerate
+// quadrilateral (x0, x0, corner s, corner s+1), weights times the Jacobian
+// (nihu/core/duffy_quadrature.hpp, nihu/core/quadrature.hpp:105-127).
+// out: 4 ng^2 rows (xi, eta, w).
+int nbg_quad_duffy(int ng, double x0, double y0, double *out)
+{
+	std::vector<double> gx(ng), gw(ng);
+	nbg_gauss_legendre(ng, gx.data(), gw.data());
+	int k = 0;
+	for (int s = 0; s < 4; ++s) {
+		const double cx[4] = {x0, x0, QCX[s], QCX[(s + 1) % 4]};
+		const double cy[4] = {y0, y0, QCY[s], QCY[(s + 1) % 4]};
+		for (int i = 0; i < ng; ++i)
+			for (int j = 0; j < ng; ++j) {
+				const double u = gx[i], v = gx[j];
+				const double N[4] = {(1.0 - u) * (1.0 - v) / 4.0, (1.0 + u) * (1.0 - v) / 4.0,
+					(1.0 + u) * (1.0 + v) / 4.0, (1.0 - u) * (1.0 + v) / 4.0};
+				const double Du[4] = {-(1.0 - v) / 4.0, (1.0 - v) / 4.0, (1.0 + v) / 4.0, -(1.0 + v) / 4.0};
+				const double Dv[4] = {-(1.0 - u) / 4.0, -(1.0 + u) / 4.0, (1.0 + u) / 4.0, (1.0 - u) / 4.0};
+				double px = 0, py = 0, a = 0, b = 0, cc = 0, d = 0;
+				for (int t = 0; t < 4; ++t) {
+					px += N[t] * cx[t]; py += N[t] * cy[t];
+					a += Du[t] * cx[t]; b += Du[t] * cy[t];
+					cc += Dv[t] * cx[t]; d += Dv[t] * cy[t];
+				}
+				out[3 * k] = px; out[3 * k + 1] = py; out[3 * k + 2] = gw[i] * gw[j] * (a * d - cc * b);
+				++k;
+			}
+	}
+	return k;
+}
+
+} // extern "C"

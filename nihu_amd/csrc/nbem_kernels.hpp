@@ -437,6 +437,10 @@ struct Rules {
 	double gl21x[21], gl21w[21];      // Gauss-Legendre order 40 (21 points)
 	double d7xi[13], d7eta[13], d7w[13];      // Dunavant degree 7
 	double d14xi[42], d14eta[42], d14w[42];   // Dunavant degree 14
+	// quadrilaterals: Gauss-Legendre orders 5, 9, 14 / 15, 30, 100 and the
+	// order-7 Duffy blind rule around the centre of [-1, 1]^2 (xi, eta, w)
+	double gl3x[3], gl3w[3], gl5x[5], gl5w[5], gl8x[8], gl8w[8], gl16x[16], gl16w[16], gl51x[51], gl51w[51];
+	double duffy[64][3];
 };
 
 struct NearArgs {
@@ -529,8 +533,8 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 // side n is P = dist_n / cos(th - th0_n), evaluated here as
 // dist_n / (cos th cos th0_n + sin th sin th0_n) with the sin and cos of th
 // that DLPt and HSP need anyway.
-template <int MASK>
-__device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, double (&st)[4])
+template <int MASK, int NC = 3>
+__device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, double (&st)[4])
 {
 	const D3 v1 = d3sub(V[1], V[0]), v2 = d3sub(V[2], V[0]);
 	const D3 T0 = d3scale(v1, 1.0 / d3norm(v1));
@@ -538,13 +542,13 @@ __device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, doubl
 	T2 = d3scale(T2, 1.0 / d3norm(T2));
 	D3 T1 = d3cross(T2, T0);
 	T1 = d3scale(T1, 1.0 / d3norm(T1));
-	D3 C[3];
-	for (int a = 0; a < 3; ++a) C[a] = {d3dot(T0, V[a]), d3dot(T1, V[a]), d3dot(T2, V[a])};
+	D3 C[NC];
+	for (int a = 0; a < NC; ++a) C[a] = {d3dot(T0, V[a]), d3dot(T1, V[a]), d3dot(T2, V[a])};
 	const D3 x0 = {d3dot(T0, x0g), d3dot(T1, x0g), d3dot(T2, x0g)};
 	const D3 nx = {d3dot(T0, nxg), d3dot(T1, nxg), d3dot(T2, nxg)};
-	double th_lim[3], c0[3], s0[3], dist[3];
-	for (int n = 0; n < 3; ++n) {
-		const int m = n == 2 ? 0 : n + 1;
+	double th_lim[NC], c0[NC], s0[NC], dist[NC];
+	for (int n = 0; n < NC; ++n) {
+		const int m = n == NC - 1 ? 0 : n + 1;
 		double lx = C[m].x - C[n].x, ly = C[m].y - C[n].y;
 		const double lninv = rsqrt64(lx * lx + ly * ly);
 		lx *= lninv; ly *= lninv;
@@ -571,8 +575,8 @@ __device__ void near_static(const D3 V[3], D3 x0g, D3 nxg, const Rules &R, doubl
 	const double z2 = z * z;
 	const double sgz = (0.0 < z) - (z < 0.0);
 	double rs = 0.0, rd = 0.0, rt = 0.0, rh = 0.0;
-	for (int n = 0; n < 3; ++n) {
-		double t1 = th_lim[n], t2 = th_lim[n == 2 ? 0 : n + 1];
+	for (int n = 0; n < NC; ++n) {
+		double t1 = th_lim[n], t2 = th_lim[n == NC - 1 ? 0 : n + 1];
 		if (fabs(t2 - t1) > PI_D) {
 			if (t2 > t1) t1 += 2.0 * PI_D;
 			else t2 += 2.0 * PI_D;

@@ -1,0 +1,588 @@
+// nbem_quad.hpp -- quadrilateral elements (4-node bilinear, possibly warped)
+// with constant fields in collocation: the regular FP32 pass and the FP64
+// near-singular / singular pass (gfx950).
+//
+// Geometry (nihu/library/lib_shape.hpp:215-310, lib_domain.cpp:74-94):
+//   x(xi, eta) = sum_a N_a X_a on [-1, 1]^2, N = (1 -+ xi)(1 -+ eta) / 4 with
+//   corners (-1,-1), (1,-1), (1,1), (-1,1); n = dx/dxi x dx/deta varies over
+//   the element; collocation point x(0, 0); R = mean edge length.
+// Data in HBM per quadrilateral mesh (SoA, n elements): corners X[12][n]
+// FP64, centres c[3][n] FP64, unit normals at the centre; quadrature tables
+// per Gauss order N (N x N points, N = degree / 2 + 1):
+//   QuadPt[n][N*N] = offset from the centre, w J / (4 pi), unit normal, |offset|^2
+// (FP32).
+//
+// Regular pass: as for triangles, one lane per test row, a wave-uniform trial
+// column; the regular degree is 1 (the jacobian order of quad_1) plus the
+// distance-table degree (nihu/core/complexity_estimator.hpp:48-62), a Gauss
+// square rule, and since the normal varies, r.n_y and n_x.n_y are per point.
+// FP64 pass, one lane per pair:
+//   diagonal: SLP / DLP / DLPt by the order-7 Duffy blind rule around the
+//     centre (nihu/core/singular_accelerator.hpp:363-456,
+//     nihu/core/duffy_quadrature.hpp); HSP by Guiggiani's method with orders
+//     (5, 9) (nihu/library/guiggiani_1992.hpp:138-330, Laurent coefficients
+//     of the Laplace HSP core nihu/library/laplace_kernel.hpp:266-295);
+//   near (d / R < limit): Helmholtz SLP and Laplace SLP / DLP / DLPt by polar
+//     integration around the image of x0 of K J - K_lin J_lin plus the planar
+//     integral over the linearised element (orders 15 x 15; Laplace HSP
+//     100 x 100) (nihu/library/nearly_singular_collocational.hpp:71-203);
+//     Helmholtz DLP / DLPt / HSP by the Telles-transformed order-30 Gauss rule
+//     (nihu/library/nearly_singular_collocational_telles.hpp:66-103,
+//     nihu/library/telles_1987.hpp:30-96); the image of x0 by Newton-Raphson
+//     (nihu/core/inverse_mapping.hpp:37-118).
+
+#pragma once
+
+#include "nbem_kernels.hpp"
+
+namespace nbk {
+
+constexpr int QN_MAX = 8;    // largest Gauss order per axis of the regular tables
+
+struct QuadPt { float ox, oy, oz, w, nx, ny, nz, oo; };
+
+struct QuadArgs {
+	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	int tn, sn;
+	const double *tc;            // test collocation points [3][tn]
+	const float *tnu;            // test unit normals [3][tn]
+	const double *sc;            // trial centres [3][sn]
+	const double *thr;           // trial thresholds [NTHR][sn]
+	const QuadPt *qt[QN_MAX + 1];   // by Gauss order per axis
+	int fixed_n;                 // > 0: fixed rule with fixed_n x fixed_n points
+	float k, k_revf;
+	double k_rev;
+	float2 alpha;
+	void *out0, *out1;
+	long long ld;
+	int dtype, accumulate;
+};
+
+struct QuadCtx {
+	float dcx, dcy, dcz;         // c_j - x (FP32)
+	float d2f, df, t0;           // |c_j - x|^2, |c_j - x|, reduced phase of k d (revolutions)
+	float nxx, nxy, nxz;         // test unit normal
+};
+
+// Kernel values times the weights at the points of one trial element (FP32):
+//   SLP  E,  DLP -a Q,  DLPt b Q,  HSP  nn Q - a b (3 Q - k^2 E) / r^2
+// with E = w e^{-ikr} / r, Q = E (1 + ikr) / r^2, a = r.n_y, b = r.n_x,
+// nn = n_x.n_y at the point (the kernels of nihu/library/helmholtz_kernel.hpp:
+// 169-228 and normal_derivative_kernel.hpp:114-257 with r = y - x).  The
+// phase is t0 + (k / 2pi) u / (r + d) revolutions, u = r^2 - d^2 =
+// 2 (c_j - x).off + |off|^2, as in the triangle pass.
+template <int FAM, int MASK>
+__device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const QuadCtx &p, float k_revf, float k,
+	float2 (&K)[4])
+{
+	const auto qf = uni((const float *)tab);
+	for (int q = 0; q < nq; ++q) {
+		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3];
+		const float nqx = qf[8 * q + 4], nqy = qf[8 * q + 5], nqz = qf[8 * q + 6], oo = qf[8 * q + 7];
+		const float u = __builtin_fmaf(2.0f * p.dcx, ox, __builtin_fmaf(2.0f * p.dcy, oy, __builtin_fmaf(2.0f * p.dcz, oz, oo)));
+		const float r2 = p.d2f + u;
+		const float rinv = __builtin_amdgcn_rsqf(r2);
+		const float g = w * rinv;
+		float ec, es, th = 0.0f;
+		if (FAM == HELM) {
+			const float r = r2 * rinv;
+			const float t = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(r + p.df), p.t0);
+			ec = g * __builtin_amdgcn_cosf(t);
+			es = -g * __builtin_amdgcn_sinf(t);
+			th = k * r;
+		} else {
+			ec = g;
+			es = 0.0f;
+		}
+		if (MASK & SLP) { K[0].x += ec; K[0].y += es; }
+		if (MASK & (DLP | DLPT | HSP)) {
+			const float rx = p.dcx + ox, ry = p.dcy + oy, rz = p.dcz + oz;
+			const float ri2 = rinv * rinv;
+			const float Gr = ec * ri2, Gi = es * ri2;
+			const float Qr = __builtin_fmaf(-th, Gi, Gr), Qi = __builtin_fmaf(th, Gr, Gi);
+			float a = 0.0f, b = 0.0f;
+			if (MASK & (DLP | HSP)) a = rx * nqx + ry * nqy + rz * nqz;
+			if (MASK & (DLPT | HSP)) b = rx * p.nxx + ry * p.nxy + rz * p.nxz;
+			if (MASK & DLP) { K[1].x = __builtin_fmaf(-a, Qr, K[1].x); K[1].y = __builtin_fmaf(-a, Qi, K[1].y); }
+			if (MASK & DLPT) { K[2].x = __builtin_fmaf(b, Qr, K[2].x); K[2].y = __builtin_fmaf(b, Qi, K[2].y); }
+			if (MASK & HSP) {
+				const float nn = p.nxx * nqx + p.nxy * nqy + p.nxz * nqz;
+				const float kk = FAM == HELM ? k * k : 0.0f;
+				const float ab = a * b * ri2;
+				const float hr = __builtin_fmaf(3.0f, Qr, -kk * ec), hi = __builtin_fmaf(3.0f, Qi, -kk * es);
+				K[3].x += __builtin_fmaf(nn, Qr, -ab * hr);
+				K[3].y += __builtin_fmaf(nn, Qi, -ab * hi);
+			}
+		}
+	}
+}
+
+// Lanes at their own Gauss order n (n <= 0: idle), one distinct order at a
+// time so that the table address stays wave-uniform.
+template <int FAM, int MASK>
+__device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, const QuadCtx &p, float2 (&K)[4])
+{
+	if (__ballot(n > 0) == 0) return;
+	for (;;) {
+		const unsigned long long m = __ballot(n > 0);
+		if (m == 0) break;
+		const int lane = __ffsll((long long)m) - 1;
+		const int nsel = __builtin_amdgcn_readlane(n, lane);
+		const unsigned long long msel = __ballot(n == nsel);
+		if ((msel >> __lane_id()) & 1ull) {
+			const int nq = nsel * nsel;
+			quad_points<FAM, MASK>(a.qt[nsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, K);
+			n = 0;
+		}
+	}
+}
+
+// Gauss order per axis of a Gauss square rule of a degree
+__device__ __forceinline__ int gauss_order(int degree) { return degree / 2 + 1; }
+
+template <int FAM, int KM, int MODE>
+__global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
+{
+	const int tile_r = blockIdx.x % a.row_tiles;
+	const int tile_c = blockIdx.x / a.row_tiles;
+	const int i = tile_r * 256 + threadIdx.x;
+	const bool row_ok = i < a.n_rows;
+	const int ii = row_ok ? i : a.n_rows - 1;
+	const double cix = a.tc[ii], ciy = a.tc[a.tn + ii], ciz = a.tc[2 * a.tn + ii];
+	QuadCtx p;
+	p.nxx = p.nxy = p.nxz = 0.0f;
+	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
+	const int j0 = tile_c * a.tile_cols;
+	const int j1 = min(j0 + a.tile_cols, a.n_cols);
+	const auto thr = uni(a.thr);
+	const auto sc = uni(a.sc);
+	const int sn = a.sn;
+	for (int j = j0; j < j1; ++j) {
+		double t[NTHR];
+#pragma unroll
+		for (int q = 0; q < NTHR; ++q) t[q] = thr[(size_t)q * sn + j];
+		const double cjx = sc[j], cjy = sc[sn + j], cjz = sc[2 * sn + j];
+		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
+		const bool act = row_ok && !(d2 < t[THR_NEAR]) && !(a.same_mesh && i == j);
+		if (__ballot(act) == 0) continue;
+		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
+		p.d2f = (float)d2;
+		if (FAM == HELM) {
+			double y = __builtin_amdgcn_rsq(d2);
+			y = y * (1.5 - 0.5 * d2 * y * y);
+			const double d = d2 * y;
+			double tt = d * a.k_rev;
+			tt -= rint(tt);
+			p.df = (float)d;
+			p.t0 = (float)tt;
+		} else {
+			p.df = 0.0f; p.t0 = 0.0f;
+		}
+		float2 K[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+		if (a.fixed_n > 0) {
+			quad_orders<FAM, KM>(a, j, act ? a.fixed_n : 0, p, K);
+		} else {
+			// Gauss order per kernel group: degree 1 + table degree
+			int nS = 0, nD = 0, nH = 0;
+			if (KM & SLP) nS = act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
+			if (KM & (DLP | DLPT)) nD = act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
+			if (KM & HSP) nH = act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
+			const int n0 = (KM & SLP) ? nS : (KM & (DLP | DLPT)) ? nD : nH;
+			const bool joint = act && (!(KM & SLP) || nS == n0) && (!(KM & (DLP | DLPT)) || nD == n0) &&
+				(!(KM & HSP) || nH == n0);
+			quad_orders<FAM, KM>(a, j, joint ? n0 : 0, p, K);
+			const bool sep = act && !joint;
+			if (__ballot(sep) != 0) {
+				if (KM & SLP) quad_orders<FAM, KM & SLP>(a, j, sep ? nS : 0, p, K);
+				if (KM & (DLP | DLPT)) quad_orders<FAM, KM & (DLP | DLPT)>(a, j, sep ? nD : 0, p, K);
+				if (KM & HSP) quad_orders<FAM, KM & HSP>(a, j, sep ? nH : 0, p, K);
+			}
+		}
+		if (act) {
+			const long long idx = (long long)j * a.ld + i;
+			if (MODE == OUT_SINGLE) {
+				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+				const float re = a.alpha.x * K[kk].x - a.alpha.y * K[kk].y;
+				const float im = a.alpha.x * K[kk].y + a.alpha.y * K[kk].x;
+				store_entry(a.out0, idx, a.dtype, a.accumulate, re, im);
+			} else {
+				const float ar = a.alpha.x * K[3].x - a.alpha.y * K[3].y + K[1].x;
+				const float ai = a.alpha.x * K[3].y + a.alpha.y * K[3].x + K[1].y;
+				const float br = K[0].x + a.alpha.x * K[2].x - a.alpha.y * K[2].y;
+				const float bi = K[0].y + a.alpha.x * K[2].y + a.alpha.y * K[2].x;
+				store_entry(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+				store_entry(a.out1, idx, a.dtype, a.accumulate, br, bi);
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------- FP64 pass
+
+struct Quad { D3 X[4]; };
+
+__device__ __forceinline__ D3 quad_x(const Quad &Q, double xi, double eta)
+{
+	const double N[4] = {(1.0 - xi) * (1.0 - eta) / 4.0, (1.0 + xi) * (1.0 - eta) / 4.0,
+		(1.0 + xi) * (1.0 + eta) / 4.0, (1.0 - xi) * (1.0 + eta) / 4.0};
+	D3 x = {0.0, 0.0, 0.0};
+	for (int c = 0; c < 4; ++c) x = {x.x + Q.X[c].x * N[c], x.y + Q.X[c].y * N[c], x.z + Q.X[c].z * N[c]};
+	return x;
+}
+
+__device__ __forceinline__ void quad_dx(const Quad &Q, double xi, double eta, D3 &dxi, D3 &deta)
+{
+	const double A[4] = {-(1.0 - eta) / 4.0, (1.0 - eta) / 4.0, (1.0 + eta) / 4.0, -(1.0 + eta) / 4.0};
+	const double B[4] = {-(1.0 - xi) / 4.0, -(1.0 + xi) / 4.0, (1.0 + xi) / 4.0, (1.0 - xi) / 4.0};
+	dxi = {0.0, 0.0, 0.0};
+	deta = {0.0, 0.0, 0.0};
+	for (int c = 0; c < 4; ++c) {
+		dxi = {dxi.x + Q.X[c].x * A[c], dxi.y + Q.X[c].y * A[c], dxi.z + Q.X[c].z * A[c]};
+		deta = {deta.x + Q.X[c].x * B[c], deta.y + Q.X[c].y * B[c], deta.z + Q.X[c].z * B[c]};
+	}
+}
+
+// d2x / dxi deta (the pure second derivatives of the bilinear map vanish)
+__device__ __forceinline__ D3 quad_dxe(const Quad &Q)
+{
+	return {(Q.X[0].x - Q.X[1].x + Q.X[2].x - Q.X[3].x) * 0.25, (Q.X[0].y - Q.X[1].y + Q.X[2].y - Q.X[3].y) * 0.25,
+		(Q.X[0].z - Q.X[1].z + Q.X[2].z - Q.X[3].z) * 0.25};
+}
+
+__device__ __forceinline__ D3 d3add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+
+// Kernels of MASK (FAM = Helmholtz or Laplace) times wj at one point, FP64,
+// added to s: SLP s[0], DLP s[1], DLPt s[2], HSP s[3] (1 / (4 pi) included).
+template <int FAM, int MASK>
+__device__ __forceinline__ void kernel_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4])
+{
+	const D3 rv = d3sub(y, x);
+	const double r2 = d3dot(rv, rv);
+	const double rinv = rsqrt64(r2);
+	double cs = 1.0, sn = 0.0, kr = 0.0;
+	if (FAM == HELM) {
+		kr = k * (r2 * rinv);
+		sincos(kr, &sn, &cs);
+	}
+	const double g0 = rinv * (wj / (4.0 * PI_D));
+	if (MASK & SLP) { s[0].x += cs * g0; s[0].y += -sn * g0; }
+	if (MASK & (DLP | DLPT | HSP)) {
+		const double rdny = d3dot(rv, ny) * rinv;
+		const double rdnx = -d3dot(rv, nx) * rinv;
+		const double g1 = g0 * rinv;
+		// e^{-ikr} (-1 - ikr)
+		const double er = -cs - kr * sn, ei = sn - kr * cs;
+		if (MASK & DLP) { s[1].x += er * g1 * rdny; s[1].y += ei * g1 * rdny; }
+		if (MASK & DLPT) { s[2].x += er * g1 * rdnx; s[2].y += ei * g1 * rdnx; }
+		if (MASK & HSP) {
+			const double g3 = g1 * rinv;
+			const double cc = rdnx * rdny, nn = d3dot(nx, ny);
+			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
+			s[3].x += (cs * ar + sn * ai) * g3;
+			s[3].y += (cs * ai - sn * ar) * g3;
+		}
+	}
+}
+
+// kernels at the point (xi, eta) of the element, times J(xi) w
+template <int FAM, int MASK>
+__device__ __forceinline__ void quad_kernel_at(const Quad &Q, double xi, double eta, D3 x0, D3 nx0, double k,
+	double w, double2 (&s)[4])
+{
+	D3 a, b;
+	quad_dx(Q, xi, eta, a, b);
+	const D3 n = d3cross(a, b);
+	const double J = d3norm(n);
+	kernel_point<FAM, MASK>(x0, quad_x(Q, xi, eta), nx0, d3scale(n, 1.0 / J), k, J * w, s);
+}
+
+// Singular SLP / DLP / DLPt: the order-7 Duffy rule around the centre
+template <int FAM, int MASK>
+__device__ void quad_blind(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4])
+{
+	for (int q = 0; q < 64; ++q) quad_kernel_at<FAM, MASK>(Q, R.duffy[q][0], R.duffy[q][1], x0, nx0, k, R.duffy[q][2], s);
+}
+
+// Guiggiani's finite-part integral of the HSP kernel at the centre, radial
+// order 5 (3 points), angular order 9 (5 points); x0 = x(0, 0), n_x the unit
+// normal there.
+template <int FAM>
+__device__ double2 quad_guiggiani(const Quad &Q, double k, const Rules &R)
+{
+	D3 dxi, deta;
+	quad_dx(Q, 0.0, 0.0, dxi, deta);
+	const D3 dxe = quad_dxe(Q);
+	const D3 nJ = d3cross(dxi, deta);
+	const D3 n0 = d3scale(nJ, 1.0 / d3norm(nJ));
+	const D3 x0 = quad_x(Q, 0.0, 0.0);
+	// T maps xi to a plane in which the first-order distance is unit length
+	const double la = d3norm(dxi), lb = d3norm(deta);
+	const double cosg = d3dot(dxi, deta) / la / lb;
+	const double u = lb / la;
+	const double T00 = la, T01 = la * cosg * u, T11 = la * sqrt(1.0 - cosg * cosg) * u;
+	const double detT = T00 * T11;
+	const double Ti00 = T11 / detT, Ti01 = -T01 / detT, Ti11 = T00 / detT;
+	const D3 J0 = d3scale(nJ, 1.0 / detT);
+	const double cx[4] = {-1.0, 1.0, 1.0, -1.0}, cy[4] = {-1.0, -1.0, 1.0, 1.0};
+	double th_lim[4], th0[4], dist[4];
+	for (int n = 0; n < 4; ++n) {
+		const int m = n == 3 ? 0 : n + 1;
+		const double c1x = T00 * cx[n] + T01 * cy[n], c1y = T11 * cy[n];
+		const double c2x = T00 * cx[m] + T01 * cy[m], c2y = T11 * cy[m];
+		double lx = 1.0, ly = 0.0;
+		const double ln = sqrt((c2x - c1x) * (c2x - c1x) + (c2y - c1y) * (c2y - c1y));
+		if (ln > 1e-3) { lx = (c2x - c1x) / ln; ly = (c2y - c1y) / ln; }
+		const double dl = c1x * lx + c1y * ly;
+		const double d0x = c1x - lx * dl, d0y = c1y - ly * dl;
+		th_lim[n] = atan2(c1y, c1x);
+		th0[n] = atan2(d0y, d0x);
+		dist[n] = sqrt(d0x * d0x + d0y * d0y);
+	}
+	double2 I = make_double2(0.0, 0.0);
+	const double f4 = 1.0 / (4.0 * PI_D);
+	for (int n = 0; n < 4; ++n) {
+		const double t1 = th_lim[n];
+		double t2 = th_lim[n == 3 ? 0 : n + 1];
+		if (fabs(t2 - t1) < 1e-3) continue;
+		if (fabs(t2 - t1) > PI_D) t2 += 2.0 * PI_D;
+		for (int it = 0; it < 5; ++it) {
+			const double th = ((1.0 - R.gl5x[it]) * t1 + (1.0 + R.gl5x[it]) * t2) / 2.0;
+			const double wt = R.gl5w[it] * (t2 - t1) / 2.0;
+			double st, ct;
+			sincos(th, &st, &ct);
+			const double vx = Ti00 * ct + Ti01 * st, vy = Ti11 * st;
+			const D3 r1 = d3add(d3scale(dxi, vx), d3scale(deta, vy));
+			const D3 r2 = d3scale(dxe, vx * vy);
+			const D3 J1 = d3scale(d3add(d3scale(d3cross(dxi, dxe), vx), d3scale(d3cross(dxe, deta), vy)), 1.0 / detT);
+			// Laurent coefficients (constant field)
+			const D3 g1 = d3scale(r1, d3dot(r2, J0) + d3dot(r1, J1));
+			const double a0 = -d3dot(J0, n0);
+			const double a1 = d3dot(d3sub(d3scale(g1, 3.0), J1), n0);
+			const double Sm2 = -3.0 * d3dot(r1, r2);
+			const double Fm1 = -(Sm2 * a0 + a1) * f4;
+			const double Fm2 = -a0 * f4;
+			const double rho_lim = dist[n] / cos(th - th0[n]);
+			I.x += (Fm1 * log(rho_lim) - Fm2 / rho_lim) * wt;
+			for (int ir = 0; ir < 3; ++ir) {
+				const double rho = (1.0 + R.gl3x[ir]) * rho_lim / 2.0;
+				const double wr = R.gl3w[ir] * rho_lim / 2.0;
+				const double ex = rho * ct, ey = rho * st;
+				const double xi = Ti00 * ex + Ti01 * ey, eta = Ti11 * ey;
+				double2 F[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+				D3 a, b;
+				quad_dx(Q, xi, eta, a, b);
+				const D3 nv = d3cross(a, b);
+				const double J = d3norm(nv);
+				kernel_point<FAM, HSP>(x0, quad_x(Q, xi, eta), n0, d3scale(nv, 1.0 / J), k, J / detT * rho, F);
+				const double sing = (Fm1 + Fm2 / rho) / rho;
+				I.x += (F[3].x - sing) * (wt * wr);
+				I.y += F[3].y * (wt * wr);
+			}
+		}
+	}
+	return I;
+}
+
+// Image (xi, eta) and normal offset zeta of x0: x(xi) + zeta n / sqrt|n| = x0
+// by Newton-Raphson from 0 (at most 100 steps, |residual| < tol).
+__device__ bool quad_inverse(const Quad &Q, D3 x0, double tol, double &xi, double &eta, double &zeta)
+{
+	double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+	const D3 dxe = quad_dxe(Q);
+	for (int it = 0; it < 100; ++it) {
+		D3 a, b;
+		quad_dx(Q, r0, r1, a, b);
+		D3 n = d3cross(a, b);
+		const double sj = sqrt(d3norm(n));
+		n = d3scale(n, 1.0 / sj);
+		const D3 f = d3sub(d3add(quad_x(Q, r0, r1), d3scale(n, r2)), x0);
+		if (d3norm(f) < tol) { xi = r0; eta = r1; zeta = r2; return true; }
+		const D3 c0 = d3add(a, d3scale(d3cross(a, dxe), r2 / sj));
+		const D3 c1 = d3add(b, d3scale(d3cross(dxe, b), r2 / sj));
+		// solve [c0 c1 n] d = f by Gaussian elimination with partial pivoting
+		double M[3][4] = {{c0.x, c1.x, n.x, f.x}, {c0.y, c1.y, n.y, f.y}, {c0.z, c1.z, n.z, f.z}};
+		for (int col = 0; col < 3; ++col) {
+			int piv = col;
+			for (int r = col + 1; r < 3; ++r)
+				if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
+			if (piv != col)
+				for (int c = 0; c < 4; ++c) { const double tmp = M[piv][c]; M[piv][c] = M[col][c]; M[col][c] = tmp; }
+			for (int r = col + 1; r < 3; ++r) {
+				const double fct = M[r][col] / M[col][col];
+				for (int c = col; c < 4; ++c) M[r][c] -= fct * M[col][c];
+			}
+		}
+		const double d2 = M[2][3] / M[2][2];
+		const double d1 = (M[1][3] - M[1][2] * d2) / M[1][1];
+		const double d0 = (M[0][3] - M[0][1] * d1 - M[0][2] * d2) / M[0][0];
+		r0 -= d0; r1 -= d1; r2 -= d2;
+	}
+	return false;
+}
+
+// Near-singular, polar: around the image xi0 of x0, K J - K_lin J_lin over
+// the reference square (radial and angular Gauss orders NR, NT given as rule
+// arrays), plus the planar integral over the linearised element.
+template <int FAM, int MASK>
+__device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, const double *grx, const double *grw,
+	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4])
+{
+	double xi0, eta0, zeta;
+	if (!quad_inverse(Q, x0, 1e-5, xi0, eta0, zeta)) return false;
+	// the tangent parallelogram at xi0: x_lin(xi) = xl0 + a (xi - xi0) + b (eta - eta0)
+	D3 a, b;
+	quad_dx(Q, xi0, eta0, a, b);
+	const D3 xl0 = quad_x(Q, xi0, eta0);
+	const D3 nl = d3cross(a, b);
+	const double Jl = d3norm(nl);
+	const D3 nul = d3scale(nl, 1.0 / Jl);
+	const double cx[4] = {-1.0, 1.0, 1.0, -1.0}, cy[4] = {-1.0, -1.0, 1.0, 1.0};
+	D3 L[4];
+	for (int c = 0; c < 4; ++c) L[c] = d3add(xl0, d3add(d3scale(a, cx[c] - xi0), d3scale(b, cy[c] - eta0)));
+	double th_lim[4], th0[4], dist[4];
+	for (int n = 0; n < 4; ++n) {
+		const int m = n == 3 ? 0 : n + 1;
+		double lx = 1.0, ly = 0.0;
+		const double ln = sqrt((cx[m] - cx[n]) * (cx[m] - cx[n]) + (cy[m] - cy[n]) * (cy[m] - cy[n]));
+		if (ln > 1e-3) { lx = (cx[m] - cx[n]) / ln; ly = (cy[m] - cy[n]) / ln; }
+		const double d1x = cx[n] - xi0, d1y = cy[n] - eta0;
+		const double dl = d1x * lx + d1y * ly;
+		const double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		th_lim[n] = atan2(d1y, d1x);
+		th0[n] = atan2(d0y, d0x);
+		dist[n] = sqrt(d0x * d0x + d0y * d0y);
+	}
+	for (int n = 0; n < 4; ++n) {
+		double t1 = th_lim[n], t2 = th_lim[n == 3 ? 0 : n + 1];
+		if (fabs(t2 - t1) > PI_D) {
+			if (t2 > t1) t1 += 2.0 * PI_D;
+			else t2 += 2.0 * PI_D;
+		}
+		if (fabs(t2 - t1) < 1e-3) continue;
+		for (int it = 0; it < nt; ++it) {
+			const double th = ((1.0 - gtx[it]) * t1 + (1.0 + gtx[it]) * t2) / 2.0;
+			const double wt = gtw[it] * (t2 - t1) / 2.0;
+			double st, ct;
+			sincos(th, &st, &ct);
+			const double rho_lim = dist[n] / cos(th - th0[n]);
+			for (int ir = 0; ir < nr; ++ir) {
+				const double rho = (1.0 + grx[ir]) * rho_lim / 2.0;
+				const double w = wt * (grw[ir] * rho_lim / 2.0) * rho;
+				const double dxi = rho * ct, deta = rho * st;
+				quad_kernel_at<FAM, MASK>(Q, dxi + xi0, deta + eta0, x0, nx0, k, w, s);
+				const D3 yl = d3add(xl0, d3add(d3scale(a, dxi), d3scale(b, deta)));
+				kernel_point<FAM, MASK>(x0, yl, nx0, nul, k, -Jl * w, s);
+			}
+		}
+	}
+	// planar integral over the linearised element: Laplace static part and,
+	// for Helmholtz, the 8 x 8-point Gauss rule of K_helm - K_lap
+	double st[4] = {0, 0, 0, 0};
+	near_static<MASK, 4>(L, x0, nx0, R, st);
+	for (int t = 0; t < 4; ++t) s[t].x += st[t];
+	if (FAM == HELM)
+		for (int i = 0; i < 8; ++i)
+			for (int j = 0; j < 8; ++j) {
+				const D3 yl = d3add(xl0, d3add(d3scale(a, R.gl8x[i] - xi0), d3scale(b, R.gl8x[j] - eta0)));
+				dynamic_point<MASK>(x0, yl, nx0, nul, k, R.gl8w[i] * R.gl8w[j] * Jl, s);
+			}
+	return true;
+}
+
+// Telles' cubic map of one axis of the order-30 rule towards e (distance
+// parameter rb): x -> ca x^3 + cb x^2 + cc x + cd, jacobian 3 ca x^2 + 2 cb x + cc
+__device__ __forceinline__ void telles_coeffs(double e, double rb, double &ca, double &cb, double &cc, double &cd)
+{
+	const double f = 1.0 + 2.0 * rb;
+	const double q = 1.0 / (2.0 * f) * ((e * (3.0 - 2.0 * rb) - 2.0 * e * e * e / f) * 1.0 / f - e);
+	const double p = 1.0 / (3.0 * f * f) * (4.0 * rb * (1.0 - rb) + 3.0 * (1.0 - e * e));
+	const double sq = sqrt(q * q + p * p * p);
+	const double gb = cbrt(-q + sq) + cbrt(-q - sq) + e / f;
+	const double Qd = 1.0 + 3.0 * gb * gb;
+	ca = (1.0 - rb) / Qd;
+	cb = -3.0 * (1.0 - rb) * gb / Qd;
+	cc = (3.0 * gb * gb + rb) / Qd;
+	cd = -cb;
+}
+
+template <int FAM, int MASK>
+__device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4])
+{
+	double xi0, eta0, zeta;
+	if (!quad_inverse(Q, x0, 1e-6, xi0, eta0, zeta)) return false;
+	double rb;
+	if (zeta < 0.05) rb = 0.0;
+	else if (zeta <= 1.3) rb = 0.85 + 0.24 * log(zeta);
+	else if (zeta <= 3.618) rb = 0.893 + 0.0832 * log(zeta);
+	else rb = 1.0;
+	double a0, b0, c0, e0, a1, b1, c1, e1;
+	telles_coeffs(xi0, rb, a0, b0, c0, e0);
+	telles_coeffs(eta0, rb, a1, b1, c1, e1);
+	for (int i = 0; i < 16; ++i) {
+		const double x = R.gl16x[i];
+		const double xi = a0 * x * x * x + b0 * x * x + c0 * x + e0;
+		const double jx = 3.0 * a0 * x * x + 2.0 * b0 * x + c0;
+		for (int j = 0; j < 16; ++j) {
+			const double y = R.gl16x[j];
+			const double eta = a1 * y * y * y + b1 * y * y + c1 * y + e1;
+			const double jy = 3.0 * a1 * y * y + 2.0 * b1 * y + c1;
+			quad_kernel_at<FAM, MASK>(Q, xi, eta, x0, nx0, k, R.gl16w[i] * R.gl16w[j] * jx * jy, s);
+		}
+	}
+	return true;
+}
+
+template <int FAM, int KM, int MODE, bool DIAG>
+__global__ void __launch_bounds__(64) quad_fp64_kernel(NearArgs a)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= a.n_pairs) return;
+	const int i = DIAG ? p : a.pi[p];
+	const int j = DIAG ? p : a.pj[p];
+	const Rules &R = *a.rules;
+	Quad Q;
+	for (int v = 0; v < 4; ++v)
+		Q.X[v] = {a.X[(3 * v) * a.sn + j], a.X[(3 * v + 1) * a.sn + j], a.X[(3 * v + 2) * a.sn + j]};
+	const D3 x0 = {a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i]};
+	const D3 nx = {a.tnu[i], a.tnu[a.tn + i], a.tnu[2 * a.tn + i]};
+	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+	bool ok = true;
+	if (DIAG) {
+		if (KM & (SLP | DLP | DLPT)) quad_blind<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, a.k, R, s);
+		if (KM & HSP) s[3] = quad_guiggiani<FAM>(Q, a.k, R);
+	} else if (FAM == HELM) {
+		if (KM & SLP) ok = quad_near_polar<FAM, SLP>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s);
+		if (ok && (KM & (DLP | DLPT | HSP))) ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP)>(Q, x0, nx, a.k, R, s);
+	} else {
+		if (KM & (SLP | DLP | DLPT))
+			ok = quad_near_polar<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, 0.0, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s);
+		if (ok && (KM & HSP))
+			ok = quad_near_polar<FAM, HSP>(Q, x0, nx, 0.0, 51, R.gl51x, R.gl51w, 51, R.gl51x, R.gl51w, R, s);
+	}
+	if (!ok) {
+		// the inverse mapping did not converge: NaN marks the entry
+		const double nan = __builtin_nan("");
+		for (int t = 0; t < 4; ++t) s[t] = make_double2(nan, nan);
+	}
+	const long long idx = (long long)j * a.ld + i;
+	if (MODE == OUT_SINGLE) {
+		const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+		const double re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
+		const double im = a.alpha.x * s[kk].y + a.alpha.y * s[kk].x;
+		store_entry64(a.out0, idx, a.dtype, a.accumulate, re, im);
+	} else {
+		double ar = a.alpha.x * s[3].x - a.alpha.y * s[3].y + s[1].x;
+		double ai = a.alpha.x * s[3].y + a.alpha.y * s[3].x + s[1].y;
+		double br = s[0].x + a.alpha.x * s[2].x - a.alpha.y * s[2].y;
+		double bi = s[0].y + a.alpha.x * s[2].y + a.alpha.y * s[2].x;
+		if (DIAG) {
+			ar += -0.5;
+			br += a.alpha.x / 2.0;
+			bi += a.alpha.y / 2.0;
+		}
+		store_entry64(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+		store_entry64(a.out1, idx, a.dtype, a.accumulate, br, bi);
+	}
+}
+
+} // namespace nbk

@@ -26,7 +26,8 @@ NB_COLLOCATION, NB_GALERKIN, NB_GALERKIN_LINEAR = 0, 1, 2
 FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_linear": NB_GALERKIN_LINEAR}
 
 # symbols declared in include/nbem.h
-EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_destroy", "nb_mesh_info",
+EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_create_quads",
+           "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
            "nb_synchronize", "nb_profile", "nb_profile_read"]
 
@@ -70,6 +71,7 @@ def lib():
         l.nb_last_error.restype = C.c_char_p
         l.nb_device_info.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         l.nb_mesh_create.argtypes = [P, C.c_int32, P, C.c_int32, C.POINTER(P)]
+        l.nb_mesh_create_quads.argtypes = [P, C.c_int32, P, C.c_int32, C.POINTER(P)]
         l.nb_mesh_destroy.argtypes = [P]
         l.nb_mesh_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         l.nb_assemble.argtypes = [P, P, C.c_int32, C.c_int32, NbKernel, NbScalar, C.POINTER(NbPolicy),
@@ -131,17 +133,22 @@ def _stream_ptr(stream):
 
 
 class Mesh:
-    """A triangle mesh resident on the current GPU (nb_mesh)."""
+    """A mesh resident on the current GPU (nb_mesh): elements with 3 node
+    indices (flat triangles) or 4 (bilinear quadrilaterals)."""
 
     def __init__(self, nodes, tris):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
         self.tris = np.ascontiguousarray(tris, dtype=np.int32)
-        if self.nodes.ndim != 2 or self.nodes.shape[1] != 3 or self.tris.ndim != 2 or self.tris.shape[1] != 3:
-            raise ValueError("nodes must be n x 3 and tris m x 3")
+        if self.nodes.ndim != 2 or self.nodes.shape[1] != 3 or self.tris.ndim != 2 or self.tris.shape[1] not in (3, 4):
+            raise ValueError("nodes must be n x 3 and elements m x 3 (triangles) or m x 4 (quadrilaterals)")
         h = C.c_void_p()
-        check(lib().nb_mesh_create(self.nodes.ctypes.data, len(self.nodes), self.tris.ctypes.data, len(self.tris),
-                                   C.byref(h)))
+        create = lib().nb_mesh_create if self.tris.shape[1] == 3 else lib().nb_mesh_create_quads
+        check(create(self.nodes.ctypes.data, len(self.nodes), self.tris.ctypes.data, len(self.tris), C.byref(h)))
         self._h = h
+
+    @property
+    def nodes_per_element(self):
+        return self.tris.shape[1]
 
     @property
     def handle(self):

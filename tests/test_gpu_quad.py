@@ -1,0 +1,175 @@
+"""Quadrilateral elements (4-node bilinear, warped after projection to the
+sphere) with constant fields in collocation, through the C ABI, against the
+FP64 oracle:
+
+* every kernel, Laplace and Helmholtz, NiHu's policy (Gauss degree 1 + the
+  distance table, polar / Telles near-singular, Duffy / Guiggiani diagonal)
+  and a fixed Gauss degree 5 with NiHu's near-singular rule: rows to 2e-6 of their largest entry and 1e-5
+  relative Frobenius; the FP64-pass entries (near-singular and diagonal,
+  where Guiggiani's hypersingular integrals live) to 1e-9;
+* the Burton-Miller pair and the solved surface field; the Mie series;
+* triangle test points on a quadrilateral trial mesh and the reverse;
+* the Galerkin formalism, the sweep and near-singular rules beyond NiHu's
+  d / R < 1.5 reject quadrilateral meshes.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROW_TOL = 2e-6
+NIHU = (dict(rule="nihu", near="mean_edge", limit=1.5), dict(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5))
+# fixed Gauss degree 5 (3 x 3 points) for regular pairs, NiHu's near-singular rule
+FIXED = (dict(rule="fixed", degree=5, near="mean_edge", limit=1.5),
+         dict(rule=1, fixed_degree=5, near_mode=0, near_limit=1.5))
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    from nihu_amd import nbem
+    return nbem
+
+
+@pytest.fixture(scope="module")
+def qmesh():
+    from nihu_amd.mesh import jittered_cubed_sphere
+    return jittered_cubed_sphere(5, seed=4)
+
+
+def rowscaled(got, ref):
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    scale[scale == 0] = 1.0
+    return (np.abs(got - ref) / scale).max()
+
+
+def relfro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _assemble(nb, test, trial, fam, nx, ny, k, gp, dtype="complex128", same=True):
+    import torch
+    mt = nb.Mesh(*test)
+    ms = mt if same else nb.Mesh(*trial)
+    out = nb.colmajor(len(test[1]), len(trial[1]), dtype)
+    nb.assemble(mt, ms, out, fam, nx, ny, k, pol=nb.policy(**gp), accumulate=False)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("fam,k", [(0, 0.0), (1, 3.0)])
+@pytest.mark.parametrize("nxy", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("pol", ["nihu", "fixed"])
+def test_quad_operators(nb, orc, qmesh, fam, k, nxy, pol):
+    gp, op = NIHU if pol == "nihu" else FIXED
+    nodes, quads = qmesh
+    print("quad operator", fam, nxy, pol, flush=True)
+    got = _assemble(nb, qmesh, qmesh, fam, nxy[0], nxy[1], k, gp)
+    ref = orc.assemble(nodes, quads, fam, nxy[0], nxy[1], k=k, pol=orc.policy(**op))
+    assert np.all(np.isfinite(got))
+    rs, fr = rowscaled(got, ref), relfro(got, ref)
+    # the FP64 pass: diagonal and near-singular pairs
+    mq = nb.Mesh(nodes, quads)
+    cls = nb.classify(mq, mq, nxy[0], nxy[1], nb.policy(**gp)).cpu().numpy()
+    fp64 = cls < 0
+    e64 = np.abs(got[fp64] - ref[fp64]).max() / np.abs(ref[fp64]).max()
+    print("  row-scaled %.2e  rel fro %.2e  fp64 entries %.2e (%d)" % (rs, fr, e64, fp64.sum()), flush=True)
+    assert rs <= ROW_TOL and fr <= 1e-5
+    assert e64 <= 1e-9
+
+
+def test_quad_bm_and_field(nb, orc, qmesh):
+    import torch
+    nodes, quads = qmesh
+    k = 4.0
+    m = nb.Mesh(nodes, quads)
+    n = len(quads)
+    A, B = nb.colmajor(n, n), nb.colmajor(n, n)
+    nb.assemble_bm(m, k, A, B, pol=nb.policy(**NIHU[0]))
+    torch.cuda.synchronize()
+    Ag, Bg = A.cpu().numpy().astype(np.complex128), B.cpu().numpy().astype(np.complex128)
+    Ar, Br = orc.assemble_bm(nodes, quads, k, pol=orc.policy(**NIHU[1]))
+    print("BM rel fro A %.2e B %.2e" % (relfro(Ag, Ar), relfro(Bg, Br)), flush=True)
+    assert relfro(Ag, Ar) <= 1e-5 and relfro(Bg, Br) <= 1e-5
+    # plane wave on the rigid sphere: solved field vs the oracle's and vs Mie
+    from nihu_amd.serve import mie_rigid_sphere_surface
+    from tests.bemref import quad_geom
+    c = np.array([quad_geom(nodes[q], 0.0, 0.0)[0] for q in quads])
+    nv = np.array([quad_geom(nodes[q], 0.0, 0.0)[1] for q in quads])
+    nu = nv / np.linalg.norm(nv, axis=1)[:, None]
+    d = np.array([0.0, 0.6, 0.8])
+    pinc = np.exp(-1j * k * (c @ d))
+    q = (1j * k) * (nu @ d) * pinc
+    pg = np.linalg.solve(Ag, Bg @ q)
+    pr = np.linalg.solve(Ar, Br @ q)
+    print("solved field vs oracle %.2e" % relfro(pg, pr), flush=True)
+    assert relfro(pg, pr) <= 1e-6
+    # discretisation check on a finer mesh (864 elements, ~11 per wavelength)
+    from nihu_amd.mesh import jittered_cubed_sphere
+    nodes, quads = jittered_cubed_sphere(12, seed=4)
+    m = nb.Mesh(nodes, quads)
+    n = len(quads)
+    A, B = nb.colmajor(n, n), nb.colmajor(n, n)
+    nb.assemble_bm(m, k, A, B, pol=nb.policy(**NIHU[0]))
+    torch.cuda.synchronize()
+    c = np.array([quad_geom(nodes[e], 0.0, 0.0)[0] for e in quads])
+    nv = np.array([quad_geom(nodes[e], 0.0, 0.0)[1] for e in quads])
+    nu = nv / np.linalg.norm(nv, axis=1)[:, None]
+    pinc = np.exp(-1j * k * (c @ d))
+    q = (1j * k) * (nu @ d) * pinc
+    p = np.linalg.solve(A.cpu().numpy().astype(np.complex128), B.cpu().numpy().astype(np.complex128) @ q)
+    mie = mie_rigid_sphere_surface(k, 1.0, (c @ d) / np.linalg.norm(c, axis=1))
+    em = np.linalg.norm(pinc + p - mie) / np.linalg.norm(mie)
+    print("total surface field vs Mie (864 quadrilaterals, ka = %g): %.2e" % (k, em), flush=True)
+    assert em < 0.03
+
+
+def test_quad_mixed_meshes(nb, orc, qmesh):
+    """Triangle collocation points against quadrilaterals and the reverse
+    (different meshes: no diagonal, near-singular pairs where close)."""
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(2, seed=5)
+    tri = (nodes * 1.05, tris)
+    for test, trial in ((tri, qmesh), (qmesh, tri)):
+        for fam, nx, ny, k in ((1, 0, 0, 2.0), (1, 1, 1, 2.0), (0, 0, 1, 0.0)):
+            got = _assemble(nb, test, trial, fam, nx, ny, k, NIHU[0], same=False)
+            ref = orc.assemble(test[0], test[1], fam, nx, ny, k=k, pol=orc.policy(**NIHU[1]), trial=trial)
+            rs = rowscaled(got, ref)
+            print("mixed", test[1].shape[1], "->", trial[1].shape[1], fam, nx, ny, "row-scaled %.2e" % rs, flush=True)
+            assert rs <= ROW_TOL
+
+
+def test_quad_large_rows(nb, orc):
+    """6144 warped quadrilaterals: sampled rows of SLP and HSP."""
+    import torch
+    from nihu_amd.mesh import jittered_cubed_sphere
+    nodes, quads = jittered_cubed_sphere(32, seed=1)
+    m = nb.Mesh(nodes, quads)
+    n = len(quads)
+    rows = np.sort(np.random.default_rng(2).choice(n, 48, replace=False)).astype(np.int32)
+    ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+    for nx, ny in ((0, 0), (1, 1)):
+        out = nb.colmajor(n, n)
+        nb.assemble(m, m, out, 1, nx, ny, 8.0, pol=nb.policy(**NIHU[0]), accumulate=False)
+        torch.cuda.synchronize()
+        got = out[ri, :].cpu().numpy().astype(np.complex128)
+        ref = orc.assemble(nodes, quads, 1, nx, ny, k=8.0, pol=orc.policy(**NIHU[1]), rows=rows)
+        rs = rowscaled(got, ref)
+        print("6144 quads", nx, ny, "row-scaled %.2e rel fro %.2e" % (rs, relfro(got, ref)), flush=True)
+        assert rs <= ROW_TOL and relfro(got, ref) <= 1e-5
+
+
+def test_quad_rejections(nb, qmesh):
+    import torch
+    m = nb.Mesh(*qmesh)
+    out = nb.colmajor(m.n, m.n)
+    with pytest.raises(Exception, match="Galerkin"):
+        nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(**NIHU[0]), formalism="galerkin")
+    A = torch.zeros((2, m.n, m.n), dtype=torch.complex64, device="cuda").transpose(1, 2)
+    with pytest.raises(Exception, match="sweep"):
+        nb.assemble_sweep(m, np.array([1.0, 2.0]), A, A.clone(), pol=nb.policy(**FIXED[0]))
+    # far "near-singular" pairs would need the inverse mapping far from the element
+    with pytest.raises(Exception, match="near-singular rule"):
+        nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(rule="fixed", degree=5, near="diameter", limit=2.0))
