@@ -71,49 +71,59 @@ inline void set_policy(nb_policy const &p) { default_policy() = p; }
 // ------------------------------------------------------------------ mesh
 
 struct tria_1_tag {};
+struct quad_1_tag {};
 enum { tria_1_id = 32303 };   // element type id of tria_1_elem (dimension 3, tria domain, 3 nodes)
+enum { quad_1_id = 32404 };   // quad_1_elem (dimension 3, quad domain, 4 nodes)
 
-// A triangle mesh resident on the current GPU.  Elements are numbered by
-// insertion order; shared ownership lets spaces and operators refer to it.
+// A mesh of flat triangles (3 nodes per element) or bilinear quadrilaterals
+// (4) resident on the current GPU.  Elements are numbered by insertion order;
+// shared ownership lets spaces and operators refer to it.
 class mesh {
 public:
-	mesh(std::vector<double> const &xyz, std::vector<int32_t> const &tri)
-		: m_xyz(xyz), m_tri(tri)
+	mesh(std::vector<double> const &xyz, std::vector<int32_t> const &elems, int nodes_per_element = 3)
+		: m_xyz(xyz), m_tri(elems), m_npe(nodes_per_element)
 	{
-		if (xyz.size() % 3 || tri.size() % 3) throw std::invalid_argument("node and element arrays must be n x 3");
+		if (m_npe != 3 && m_npe != 4) throw std::invalid_argument("elements must have 3 or 4 nodes");
+		if (xyz.size() % 3 || elems.size() % m_npe)
+			throw std::invalid_argument("node array must be n x 3 and element array m x nodes_per_element");
 		nb_mesh *h = nullptr;
-		check(nb_mesh_create(m_xyz.data(), (int32_t)(m_xyz.size() / 3), m_tri.data(), (int32_t)(m_tri.size() / 3), &h));
+		const int32_t nn = (int32_t)(m_xyz.size() / 3), ne = (int32_t)(m_tri.size() / m_npe);
+		check(m_npe == 3 ? nb_mesh_create(m_xyz.data(), nn, m_tri.data(), ne, &h)
+		                 : nb_mesh_create_quads(m_xyz.data(), nn, m_tri.data(), ne, &h));
 		m_h.reset(h, [](nb_mesh *p) { nb_mesh_destroy(p); });
 	}
 	nb_mesh *handle() const { return m_h.get(); }
-	int32_t num_elements() const { return (int32_t)(m_tri.size() / 3); }
+	int32_t num_elements() const { return (int32_t)(m_tri.size() / m_npe); }
 	int32_t num_nodes() const { return (int32_t)(m_xyz.size() / 3); }
+	int nodes_per_element() const { return m_npe; }
 	std::vector<double> const &nodes() const { return m_xyz; }
 	std::vector<int32_t> const &elements() const { return m_tri; }
 
 private:
 	std::vector<double> m_xyz;
 	std::vector<int32_t> m_tri;
+	int m_npe;
 	std::shared_ptr<nb_mesh> m_h;
 };
 
-// create_mesh(nodes, elements, tria_1_tag): nodes is an (n x 3) and elements
-// an (m x 4) row-major table [type_id, n0, n1, n2] with type_id = 32303, as
-// NiHu's mesh(nodes, elements) constructor reads them.
-inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> const &elements, tria_1_tag)
+namespace detail {
+// NiHu's element table rows [type_id, n0, ..., n_{npe-1}] of one element type
+inline mesh from_element_table(std::vector<double> const &nodes, std::vector<unsigned> const &elements,
+	unsigned type_id, int npe, char const *what)
 {
-	if (elements.size() % 4) throw std::invalid_argument("elements must be m x 4: type id and three node indices");
-	std::vector<int32_t> tri;
-	tri.reserve(elements.size() / 4 * 3);
-	for (size_t e = 0; e < elements.size(); e += 4) {
-		if (elements[e] != (unsigned)tria_1_id) throw std::invalid_argument("only tria_1 elements are supported");
-		for (int a = 1; a <= 3; ++a) tri.push_back((int32_t)elements[e + a]);
+	if (elements.size() % (npe + 1)) throw std::invalid_argument(std::string("elements must be m x ") +
+		std::to_string(npe + 1) + ": type id and the node indices");
+	std::vector<int32_t> el;
+	el.reserve(elements.size() / (npe + 1) * npe);
+	for (size_t e = 0; e < elements.size(); e += npe + 1) {
+		if (elements[e] != type_id) throw std::invalid_argument(std::string("only ") + what + " elements are supported");
+		for (int a = 1; a <= npe; ++a) el.push_back((int32_t)elements[e + a]);
 	}
-	return mesh(nodes, tri);
+	return mesh(nodes, el, npe);
 }
 
-// read_off_mesh(filename, tria_1_tag()): OFF reader with NiHu's error messages
-inline mesh read_off_mesh(std::string const &fname, tria_1_tag)
+// OFF reader with NiHu's error messages; every face must have npe vertices
+inline mesh read_off(std::string const &fname, int npe)
 {
 	std::ifstream is(fname);
 	std::string header;
@@ -123,20 +133,38 @@ inline mesh read_off_mesh(std::string const &fname, tria_1_tag)
 	std::vector<double> xyz(3 * (size_t)nn);
 	for (auto &v : xyz)
 		if (!(is >> v)) throw std::runtime_error("Error reading mesh nodes");
-	std::vector<int32_t> tri;
-	tri.reserve(3 * (size_t)ne);
+	std::vector<int32_t> el;
+	el.reserve((size_t)npe * ne);
 	for (unsigned e = 0; e < ne; ++e) {
 		unsigned nv;
 		if (!(is >> nv)) throw std::runtime_error("Error reading mesh elements");
-		if (nv != 3) throw std::runtime_error("Invalid number of vertices for current element tag vector");
-		for (unsigned a = 0; a < 3; ++a) {
+		if (nv != (unsigned)npe) throw std::runtime_error("Invalid number of vertices for current element tag vector");
+		for (unsigned a = 0; a < nv; ++a) {
 			unsigned v;
 			if (!(is >> v)) throw std::runtime_error("Error reading mesh elements");
-			tri.push_back((int32_t)v);
+			el.push_back((int32_t)v);
 		}
 	}
-	return mesh(xyz, tri);
+	return mesh(xyz, el, npe);
 }
+} // namespace detail
+
+// create_mesh(nodes, elements, tag): nodes is an (n x 3) and elements an
+// (m x 4) row-major table [32303, n0, n1, n2] (tria_1_tag) or (m x 5)
+// [32404, n0, n1, n2, n3] (quad_1_tag), as NiHu's mesh(nodes, elements)
+// constructor reads them.
+inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> const &elements, tria_1_tag)
+{
+	return detail::from_element_table(nodes, elements, tria_1_id, 3, "tria_1");
+}
+inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> const &elements, quad_1_tag)
+{
+	return detail::from_element_table(nodes, elements, quad_1_id, 4, "quad_1");
+}
+
+// read_off_mesh(filename, tag): OFF reader (nihu/interface/read_off_mesh.hpp:75-105)
+inline mesh read_off_mesh(std::string const &fname, tria_1_tag) { return detail::read_off(fname, 3); }
+inline mesh read_off_mesh(std::string const &fname, quad_1_tag) { return detail::read_off(fname, 4); }
 
 // ------------------------------------------------------------------ spaces
 

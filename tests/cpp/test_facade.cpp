@@ -4,8 +4,10 @@
 // Galerkin with constant fields), on the GPU through the C ABI, plus the same
 // answers with isoparametric (nodal) fields.
 //
-//   test_facade <mesh.off>
-// Prints the errors and exits non-zero if one is above its bound.
+//   test_facade <mesh.off> [quadrilateral mesh.off]
+// Prints the errors and exits non-zero if one is above its bound.  With a
+// quadrilateral mesh, also the collocation answers and the Burton-Miller
+// composition (hypersingular diagonal by Guiggiani's method) on it.
 
 #include "nihu_amd/nihu.hpp"
 
@@ -69,6 +71,23 @@ static std::vector<centre_normal> geometry(Space const &s)
 	std::vector<centre_normal> g(m.num_elements());
 	auto const &X = m.nodes();
 	auto const &T = m.elements();
+	if (m.nodes_per_element() == 4) {
+		// bilinear quadrilateral: x(0, 0) and dx/dxi x dx/deta there
+		for (int e = 0; e < m.num_elements(); ++e) {
+			const double *p[4];
+			for (int a = 0; a < 4; ++a) p[a] = &X[3 * T[4 * e + a]];
+			double u[3], v[3];
+			for (int d = 0; d < 3; ++d) {
+				g[e].c[d] = (p[0][d] + p[1][d] + p[2][d] + p[3][d]) / 4.0;
+				u[d] = (p[1][d] + p[2][d] - p[0][d] - p[3][d]) / 4.0;
+				v[d] = (p[2][d] + p[3][d] - p[0][d] - p[1][d]) / 4.0;
+			}
+			double nv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+			double l = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+			for (int d = 0; d < 3; ++d) g[e].n[d] = nv[d] / l;
+		}
+		return g;
+	}
 	for (int e = 0; e < m.num_elements(); ++e) {
 		const double *a = &X[3 * T[3 * e]], *b = &X[3 * T[3 * e + 1]], *c = &X[3 * T[3 * e + 2]];
 		double u[3], v[3];
@@ -250,6 +269,29 @@ int main(int argc, char *argv[])
 	double bm = burton_miller(mesh);
 	std::printf("Burton-Miller composition vs fused assembly: %g\n", bm);
 	bad |= !(bm < 1e-6);
+
+	if (argc > 2) {
+		auto qmesh = nihu_amd::read_off_mesh(argv[2], nihu_amd::quad_1_tag());
+		auto const &qspace = nihu_amd::constant_view(qmesh);
+		double qt = tester(nihu_amd::dirac(qspace), qspace);
+		std::printf("Quadrilateral collocation transparent error: %g\n", qt);
+		bad |= !(qt < 0.05);
+		double qp = pulsating(nihu_amd::dirac(qspace), qspace);
+		std::printf("Pulsating sphere relative error (quadrilateral collocation): %g\n", qp);
+		bad |= !(qp < 0.05);
+		double qbm = burton_miller(qmesh);
+		std::printf("Quadrilateral Burton-Miller composition vs fused assembly: %g\n", qbm);
+		bad |= !(qbm < 1e-6);
+		// the Galerkin formalism is not implemented for quadrilaterals
+		try {
+			nihu_amd::cMatrix X(qspace.get_num_dofs(), qspace.get_num_dofs());
+			X << qspace * nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(1.0))[qspace];
+			std::printf("no exception for quadrilateral Galerkin\n");
+			bad = 1;
+		} catch (std::logic_error const &ex) {
+			std::printf("quadrilateral Galerkin: logic_error as expected: %s\n", ex.what());
+		}
+	}
 
 	// error behaviour: a Dunavant degree above 14 throws std::out_of_range
 	try {

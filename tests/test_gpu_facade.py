@@ -2,7 +2,8 @@
 known-answer programs built against libnbem.so, on the reference's
 sphere_1_3.off mesh fixture (218 nodes, 432 triangles): transparent and
 pulsating-sphere answers in collocation, constant Galerkin and isoparametric
-Galerkin, and NiHu's exceptions."""
+Galerkin, and NiHu's exceptions; on a 384-element quadrilateral cubed sphere
+(OFF file): the collocation answers and the Burton-Miller composition."""
 
 import os
 import subprocess
@@ -14,16 +15,20 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_cpp_facade_known_answers():
+def test_cpp_facade_known_answers(tmp_path):
     exe = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
     assert os.path.exists(exe), "build the C++ tests first (__graft_entry__.build())"
     mesh = os.path.join(ROOT, "tests", "golden", "sphere_1_3.off")
-    r = subprocess.run([exe, mesh], capture_output=True, text=True, timeout=240)
+    from nihu_amd.mesh import jittered_cubed_sphere, write_off
+    qmesh = str(tmp_path / "cubed_sphere_8.off")
+    write_off(qmesh, *jittered_cubed_sphere(8, seed=6))
+    r = subprocess.run([exe, mesh, qmesh], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "out_of_range as expected" in r.stdout
     assert "runtime_error as expected: Invalid general singular quadrature" in r.stdout
     assert "logic_error as expected" in r.stdout
     for line in ("Galerkin transparent error", "Pulsating sphere relative error (Galerkin)",
-                 "Galerkin isoparametric transparent error"):
+                 "Galerkin isoparametric transparent error", "Quadrilateral collocation transparent error",
+                 "Quadrilateral Burton-Miller composition", "quadrilateral Galerkin: logic_error as expected"):
         assert line in r.stdout

@@ -314,3 +314,28 @@ def test_nihu_policy_vs_high_fixed(orc):
         b = orc.assemble(nodes, quads, 1, nx, ny, k=3.0, pol=orc.policy(rule=1, fixed_degree=13), rows=rows)
         cls = np.abs(a - b) / np.abs(b).max()
         assert np.median(cls) < 2e-5 and cls.max() < 1e-3, (nx, ny, np.median(cls), cls.max())
+
+
+def test_quad_mesh_io(tmp_path):
+    """Cubed sphere: closed, consistently oriented; OFF round trip with
+    4-vertex faces; NiHu element tables with the quad_1 id; mixed meshes
+    rejected."""
+    from nihu_amd.mesh import cubed_sphere, read_off, write_off, to_nihu_elements, from_nihu_elements
+    nodes, quads = cubed_sphere(3, jitter=0.01, seed=2)
+    assert quads.shape == (54, 4) and nodes.shape == (56, 3)
+    edges = {}
+    for q in quads:
+        for a in range(4):
+            edges[(q[a], q[(a + 1) % 4])] = edges.get((q[a], q[(a + 1) % 4]), 0) + 1
+    assert all(v == 1 and edges.get((b, a)) == 1 for (a, b), v in edges.items())
+    p = str(tmp_path / "q.off")
+    write_off(p, nodes, quads)
+    n2, q2 = read_off(p)
+    assert np.array_equal(n2, nodes) and np.array_equal(q2, quads)
+    el = to_nihu_elements(quads)
+    assert el.shape == (54, 5) and np.all(el[:, 0] == 32404)
+    assert np.array_equal(from_nihu_elements(el), quads)
+    with open(p, "w") as fh:
+        fh.write("OFF\n4 2 0\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n3 0 1 2\n4 0 1 2 3\n")
+    with pytest.raises(ValueError, match="mixing"):
+        read_off(p)
