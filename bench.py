@@ -133,10 +133,11 @@ def main():
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin"], default="bm",
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad"], default="bm",
                     help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
                          "serve: Poisson stream of assembly-plus-LU-solve requests; galerkin: SLP and DLP "
-                         "Galerkin matrices with linear triangles on the 20480-element ellipsoid")
+                         "Galerkin matrices with linear triangles on the 20480-element ellipsoid; quad: "
+                         "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere")
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
     args = ap.parse_args()
@@ -152,6 +153,8 @@ def main():
         return run_serve(args)
     if args.workload == "galerkin":
         return run_galerkin(args)
+    if args.workload == "quad":
+        return run_quad(args)
 
     import numpy as np
     import torch
@@ -421,6 +424,95 @@ def run_galerkin(args):
                    "regular_rule": "Dunavant degree 4 x degree 4 (6 x 6 points)",
                    "singular": "coincident/edge/vertex Duffy-type rules, 4-point Gauss per axis, FP64",
                    "pairs": {"regular": n_reg, "singular": n_sing}},
+        "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
+    }
+    print(json.dumps(out))
+
+
+def run_quad(args):
+    """Quadrilateral workload: Burton-Miller A and B (SLP, DLP, DLPt, HSP) on
+    the jittered unit cubed sphere with 58 x 58 bilinear quadrilaterals per
+    face (20184 elements), ka = 10, NiHu's policy: Gauss degree 1 + the
+    distance table for regular pairs (FP32), polar / Telles near-singular
+    pairs below 1.5 mean edge lengths and Duffy / Guiggiani diagonal (FP64)."""
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import jittered_cubed_sphere
+
+    torch.cuda.set_device(0)
+    nodes, quads = jittered_cubed_sphere(58, seed=0)
+    n = len(quads)
+    k = 10.0
+    alpha = complex(0.0, -1.0 / k)
+    pol = nbem.policy(**nbem.NIHU_POLICY)
+    name, ncu, hbm = nbem.device_info()
+    log("device %s, %d CUs; cubed sphere %d quadrilaterals; k = %g" % (name, ncu, n, k))
+    mesh = nbem.Mesh(nodes, quads)
+    A, B = nbem.colmajor(n, n), nbem.colmajor(n, n)
+    stream = torch.cuda.current_stream()
+    cls = nbem.classify(mesh, mesh, 1, 1, pol)
+    n_near = int((cls == -2).sum().item())
+    del cls
+    n_reg = n * n - n_near - n
+    if args.profile_copy_bytes > 0:
+        copy_calibration(args.profile_copy_bytes)
+    for w in range(args.warmup):
+        nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+        torch.cuda.synchronize()
+        log("warmup step %d done" % w)
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    ms_per_step = wall / args.steps * 1e3
+    value = 4 * n * n * args.steps / wall
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    log("timed %d steps: %.3f ms/step, %.4g entries/s; per-pass device ms %s" % (args.steps, ms_per_step, value,
+                                                                                pass_ms))
+    reg_ms = pass_ms["regular"]
+    write_bytes = 2 * 8 * n_reg
+    gbs = write_bytes / (reg_ms * 1e-3) / 1e9 if reg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": "quad_regular_kernel<HELM,ALL4,OUT_BM>", "kernel_ms": round(reg_ms, 4),
+                "algorithmic_bytes": write_bytes}
+    cpu = err = None
+    if args.cpu_rows != 0:
+        from oracle import oracle
+        nr = 512 if args.cpu_rows < 0 else args.cpu_rows
+        rows = np.sort(np.random.default_rng(123).choice(n, nr, replace=False)).astype(np.int32)
+        opol = oracle.policy(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5)
+        threads = oracle.num_threads()
+        tc = time.perf_counter()
+        Ar, Br = oracle.assemble_bm(nodes, quads, k, alpha=alpha, pol=opol, rows=rows, threads=threads)
+        dt = time.perf_counter() - tc
+        cpu_value = 4 * nr * n / dt
+        cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+               "sample": "%d of %d rows of A and B (4 operators), FP64 oracle, OpenMP over rows, %.1f s wall"
+                         % (nr, n, dt), "host": host_cpu()}
+        ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+        Ag, Bg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy()
+        err = {"rows": nr, "A_rel_fro": float(np.linalg.norm(Ag - Ar) / np.linalg.norm(Ar)),
+               "B_rel_fro": float(np.linalg.norm(Bg - Br) / np.linalg.norm(Br)),
+               "A_max_abs": float(np.abs(Ag - Ar).max()), "B_max_abs": float(np.abs(Bg - Br).max())}
+        log("sampled-row error vs oracle: %s; cpu %.4g entries/s on %d threads" % (err, cpu_value, threads))
+    out = {
+        "metric": "matrix entries/s, Helmholtz 3D coll. quadrilaterals, 20184 elems",
+        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64) regular pairs, fp64 near-singular and diagonal",
+        "data": "synthetic: seeded jittered cubed sphere",
+        "config": {"workload": "Helmholtz 3D collocation on bilinear quadrilaterals, Burton-Miller A and B",
+                   "n_elements": n, "k": k, "mesh_seed": 0, "policy": "NiHu (Gauss degree 1 + distance table, "
+                   "near-singular d/R < 1.5: polar SLP, Telles DLP/DLPt/HSP; diagonal Duffy, Guiggiani HSP)",
+                   "pairs": {"regular": n_reg, "near": n_near, "diagonal": n}},
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
     print(json.dumps(out))

@@ -308,7 +308,8 @@ int ensure_thresholds(nb_mesh *m, int mode, double limit)
 		std::memset(&r, 0, sizeof(r));
 		for (int t = 0; t < nbk::NTHR; ++t) r.thr[t] = thr[(size_t)t * n + e];
 		for (int d = 0; d < 3; ++d) { r.c[d] = m->c[(size_t)d * n + e]; r.nu[d] = (float)m->nu[(size_t)d * n + e]; }
-		r.w1 = (float)(0.5 * m->J[e] / four_pi);
+		// one-point rule: Dunavant centroid (weight 1/2) or the Gauss centre of [-1, 1]^2 (weight 4)
+		r.w1 = (float)((m->npe == 4 ? 4.0 : 0.5) * m->J[e] / four_pi);
 	}
 	if ((s = upload(m->d_rec, rec))) return s;
 	if (m->thr_mode < 0) m->bytes += (int64_t)n * (nbk::NTHR * 8 + sizeof(nbk::TrialRec));
@@ -690,8 +691,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	a.tn = test->n; a.sn = trial->n;
 	a.tc = (const double *)test->d_c.get();
 	a.tnu = (const float *)test->d_nuf.get();
-	a.sc = (const double *)trial->d_c.get();
-	a.thr = (const double *)trial->d_thr.get();
+	a.rec = (const nbk::TrialRec *)trial->d_rec.get();
 	for (int N = 1; N <= nbk::QN_MAX; ++N) a.qt[N] = (const nbk::QuadPt *)trial->d_qq[N].get();
 	a.fixed_n = fixed_n;
 	const double two_pi = 6.283185307179586476925286766559005768;
@@ -758,12 +758,14 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 		na.pi = diag ? nullptr : (const int *)nl->pi.get();
 		na.pj = diag ? nullptr : (const int *)nl->pj.get();
 		if (na.n_pairs == 0) continue;
-		const int g = (na.n_pairs + 63) / 64;
+		// diagonal: one lane per element; near: QLPP lanes per pair
+		const long long lanes = (long long)na.n_pairs * (diag ? 1 : nbk::QLPP);
+		const int g = (int)((lanes + 63) / 64);
 		const int pe = prof_begin(diag ? 2 : 1, side);
 		using namespace nbk;
 #define NB_Q64(F, K, M) do { \
-		if (diag) hipLaunchKernelGGL((quad_fp64_kernel<F, K, M, true>), dim3(g), dim3(64), 0, side, na); \
-		else hipLaunchKernelGGL((quad_fp64_kernel<F, K, M, false>), dim3(g), dim3(64), 0, side, na); } while (0)
+		if (diag) hipLaunchKernelGGL((quad_diag_kernel<F, K, M>), dim3(g), dim3(64), 0, side, na); \
+		else hipLaunchKernelGGL((quad_near_kernel<F, K, M>), dim3(g), dim3(64), 0, side, na); } while (0)
 		if (jb.mode == OUT_BM) NB_Q64(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {

@@ -533,8 +533,10 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 // side n is P = dist_n / cos(th - th0_n), evaluated here as
 // dist_n / (cos th cos th0_n + sin th sin th0_n) with the sin and cos of th
 // that DLPt and HSP need anyway.
+// lane / nl: this lane evaluates every nl-th point (lane 0 of 1: all of them),
+// for callers that split one pair's integral over several lanes.
 template <int MASK, int NC = 3>
-__device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, double (&st)[4])
+__device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, double (&st)[4], int lane = 0, int nl = 1)
 {
 	const D3 v1 = d3sub(V[1], V[0]), v2 = d3sub(V[2], V[0]);
 	const D3 T0 = d3scale(v1, 1.0 / d3norm(v1));
@@ -583,7 +585,7 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 		}
 		const double jac = (t2 - t1) / 2.0;
 		if (MASK & (SLP | DLP | DLPT)) {
-			for (int g = 0; g < 6; ++g) {
+			for (int g = (lane + 6 * nl - (6 * n) % nl) % nl; g < 6; g += nl) {
 				const double xi = R.gl6x[g], w = R.gl6w[g] * jac;
 				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
 				double sth, cth;
@@ -601,7 +603,7 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 			}
 		}
 		if (MASK & HSP) {
-			for (int g = 0; g < 21; ++g) {
+			for (int g = (lane + 21 * nl - (21 * n) % nl) % nl; g < 21; g += nl) {
 				const double xi = R.gl21x[g], w = R.gl21w[g] * jac;
 				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
 				double sth, cth;

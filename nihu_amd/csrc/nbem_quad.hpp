@@ -13,16 +13,19 @@
 // (FP32).
 //
 // Regular pass: as for triangles, one lane per test row, a wave-uniform trial
-// column; the regular degree is 1 (the jacobian order of quad_1) plus the
-// distance-table degree (nihu/core/complexity_estimator.hpp:48-62), a Gauss
-// square rule, and since the normal varies, r.n_y and n_x.n_y are per point.
-// FP64 pass, one lane per pair:
-//   diagonal: SLP / DLP / DLPt by the order-7 Duffy blind rule around the
+// column whose record (thresholds, centre, normal, one-point weight) comes
+// through the scalar cache; the regular degree is 1 (the jacobian order of
+// quad_1) plus the distance-table degree (nihu/core/complexity_estimator.hpp:
+// 48-62), a Gauss square rule; the normal varies over the element, so r.n_y
+// and n_x.n_y are per point.  The one-point rule (every far pair) is the
+// centre with weight 4 J(0, 0): no table load, FP64-reduced phase.
+// FP64 pass:
+//   diagonal (one lane per element): SLP / DLP / DLPt by the order-7 Duffy blind rule around the
 //     centre (nihu/core/singular_accelerator.hpp:363-456,
 //     nihu/core/duffy_quadrature.hpp); HSP by Guiggiani's method with orders
 //     (5, 9) (nihu/library/guiggiani_1992.hpp:138-330, Laurent coefficients
 //     of the Laplace HSP core nihu/library/laplace_kernel.hpp:266-295);
-//   near (d / R < limit): Helmholtz SLP and Laplace SLP / DLP / DLPt by polar
+//   near (d / R < limit; 4 lanes per pair): Helmholtz SLP and Laplace SLP / DLP / DLPt by polar
 //     integration around the image of x0 of K J - K_lin J_lin plus the planar
 //     integral over the linearised element (orders 15 x 15; Laplace HSP
 //     100 x 100) (nihu/library/nearly_singular_collocational.hpp:71-203);
@@ -46,8 +49,7 @@ struct QuadArgs {
 	int tn, sn;
 	const double *tc;            // test collocation points [3][tn]
 	const float *tnu;            // test unit normals [3][tn]
-	const double *sc;            // trial centres [3][sn]
-	const double *thr;           // trial thresholds [NTHR][sn]
+	const TrialRec *rec;         // trial records [sn]: thresholds, centre, unit normal, 4 J / (4 pi)
 	const QuadPt *qt[QN_MAX + 1];   // by Gauss order per axis
 	int fixed_n;                 // > 0: fixed rule with fixed_n x fixed_n points
 	float k, k_revf;
@@ -61,8 +63,52 @@ struct QuadArgs {
 struct QuadCtx {
 	float dcx, dcy, dcz;         // c_j - x (FP32)
 	float d2f, df, t0;           // |c_j - x|^2, |c_j - x|, reduced phase of k d (revolutions)
+	float rinv_d;                // 1 / |c_j - x| from FP64
 	float nxx, nxy, nxz;         // test unit normal
+	float nyx, nyy, nyz;         // trial unit normal at the centre
+	float w1;                    // weight of the one-point rule: 4 J(0, 0) / (4 pi)
 };
+
+// add one point's E = (ec, es), th = k r, 1/r, a = r.n_y, b = r.n_x, nn = n_x.n_y
+template <int FAM, int MASK>
+__device__ __forceinline__ void quad_add(float ec, float es, float th, float rinv, float a, float b, float nn, float k,
+	float2 (&K)[4])
+{
+	if (MASK & SLP) { K[0].x += ec; K[0].y += es; }
+	if (MASK & (DLP | DLPT | HSP)) {
+		const float ri2 = rinv * rinv;
+		const float Gr = ec * ri2, Gi = es * ri2;
+		const float Qr = __builtin_fmaf(-th, Gi, Gr), Qi = __builtin_fmaf(th, Gr, Gi);
+		if (MASK & DLP) { K[1].x = __builtin_fmaf(-a, Qr, K[1].x); K[1].y = __builtin_fmaf(-a, Qi, K[1].y); }
+		if (MASK & DLPT) { K[2].x = __builtin_fmaf(b, Qr, K[2].x); K[2].y = __builtin_fmaf(b, Qi, K[2].y); }
+		if (MASK & HSP) {
+			const float kk = FAM == HELM ? k * k : 0.0f;
+			const float ab = a * b * ri2;
+			const float hr = __builtin_fmaf(3.0f, Qr, -kk * ec), hi = __builtin_fmaf(3.0f, Qi, -kk * es);
+			K[3].x += __builtin_fmaf(nn, Qr, -ab * hr);
+			K[3].y += __builtin_fmaf(nn, Qi, -ab * hi);
+		}
+	}
+}
+
+// One-point Gauss rule (order 1): the point is the centre, so r = d with the
+// FP64-reduced phase, and the normal is the centre's.
+template <int FAM, int MASK>
+__device__ __forceinline__ void quad_centre(const QuadCtx &p, float k, float2 (&K)[4])
+{
+	const float g = p.w1 * p.rinv_d;
+	float ec = g, es = 0.0f, th = 0.0f;
+	if (FAM == HELM) {
+		ec = g * __builtin_amdgcn_cosf(p.t0);
+		es = -g * __builtin_amdgcn_sinf(p.t0);
+		th = k * p.df;
+	}
+	float a = 0.0f, b = 0.0f, nn = 0.0f;
+	if (MASK & (DLP | HSP)) a = p.dcx * p.nyx + p.dcy * p.nyy + p.dcz * p.nyz;
+	if (MASK & (DLPT | HSP)) b = p.dcx * p.nxx + p.dcy * p.nxy + p.dcz * p.nxz;
+	if (MASK & HSP) nn = p.nxx * p.nyx + p.nxy * p.nyy + p.nxz * p.nyz;
+	quad_add<FAM, MASK>(ec, es, th, p.rinv_d, a, b, nn, k, K);
+}
 
 // Kernel values times the weights at the points of one trial element (FP32):
 //   SLP  E,  DLP -a Q,  DLPt b Q,  HSP  nn Q - a b (3 Q - k^2 E) / r^2
@@ -94,26 +140,12 @@ __device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const Qua
 			ec = g;
 			es = 0.0f;
 		}
-		if (MASK & SLP) { K[0].x += ec; K[0].y += es; }
-		if (MASK & (DLP | DLPT | HSP)) {
-			const float rx = p.dcx + ox, ry = p.dcy + oy, rz = p.dcz + oz;
-			const float ri2 = rinv * rinv;
-			const float Gr = ec * ri2, Gi = es * ri2;
-			const float Qr = __builtin_fmaf(-th, Gi, Gr), Qi = __builtin_fmaf(th, Gr, Gi);
-			float a = 0.0f, b = 0.0f;
-			if (MASK & (DLP | HSP)) a = rx * nqx + ry * nqy + rz * nqz;
-			if (MASK & (DLPT | HSP)) b = rx * p.nxx + ry * p.nxy + rz * p.nxz;
-			if (MASK & DLP) { K[1].x = __builtin_fmaf(-a, Qr, K[1].x); K[1].y = __builtin_fmaf(-a, Qi, K[1].y); }
-			if (MASK & DLPT) { K[2].x = __builtin_fmaf(b, Qr, K[2].x); K[2].y = __builtin_fmaf(b, Qi, K[2].y); }
-			if (MASK & HSP) {
-				const float nn = p.nxx * nqx + p.nxy * nqy + p.nxz * nqz;
-				const float kk = FAM == HELM ? k * k : 0.0f;
-				const float ab = a * b * ri2;
-				const float hr = __builtin_fmaf(3.0f, Qr, -kk * ec), hi = __builtin_fmaf(3.0f, Qi, -kk * es);
-				K[3].x += __builtin_fmaf(nn, Qr, -ab * hr);
-				K[3].y += __builtin_fmaf(nn, Qi, -ab * hi);
-			}
-		}
+		const float rx = p.dcx + ox, ry = p.dcy + oy, rz = p.dcz + oz;
+		float a = 0.0f, b = 0.0f, nn = 0.0f;
+		if (MASK & (DLP | HSP)) a = rx * nqx + ry * nqy + rz * nqz;
+		if (MASK & (DLPT | HSP)) b = rx * p.nxx + ry * p.nxy + rz * p.nxz;
+		if (MASK & HSP) nn = p.nxx * nqx + p.nxy * nqy + p.nxz * nqz;
+		quad_add<FAM, MASK>(ec, es, th, rinv, a, b, nn, k, K);
 	}
 }
 
@@ -122,6 +154,10 @@ __device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const Qua
 template <int FAM, int MASK>
 __device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, const QuadCtx &p, float2 (&K)[4])
 {
+	if (__ballot(n == 1) != 0) {
+		if (n == 1) quad_centre<FAM, MASK>(p, a.k, K);
+		n = n == 1 ? 0 : n;
+	}
 	if (__ballot(n > 0) == 0) return;
 	for (;;) {
 		const unsigned long long m = __ballot(n > 0);
@@ -154,39 +190,46 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
-	const auto thr = uni(a.thr);
-	const auto sc = uni(a.sc);
-	const int sn = a.sn;
+	const auto rec = uni(a.rec);
 	for (int j = j0; j < j1; ++j) {
+		// the trial record in SGPRs
 		double t[NTHR];
 #pragma unroll
-		for (int q = 0; q < NTHR; ++q) t[q] = thr[(size_t)q * sn + j];
-		const double cjx = sc[j], cjy = sc[sn + j], cjz = sc[2 * sn + j];
+		for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
+		const double cjx = rec[j].c[0], cjy = rec[j].c[1], cjz = rec[j].c[2];
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
 		const bool act = row_ok && !(d2 < t[THR_NEAR]) && !(a.same_mesh && i == j);
 		if (__ballot(act) == 0) continue;
 		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
 		p.d2f = (float)d2;
+		p.nyx = rec[j].nu[0]; p.nyy = rec[j].nu[1]; p.nyz = rec[j].nu[2];
+		p.w1 = rec[j].w1;
+		// d and 1/d in FP64 from the hardware reciprocal square root and one Newton step
+		double y = __builtin_amdgcn_rsq(d2);
+		y = y * (1.5 - 0.5 * d2 * y * y);
+		const double d = d2 * y;
+		p.rinv_d = (float)y;
+		p.df = (float)d;
 		if (FAM == HELM) {
-			double y = __builtin_amdgcn_rsq(d2);
-			y = y * (1.5 - 0.5 * d2 * y * y);
-			const double d = d2 * y;
 			double tt = d * a.k_rev;
 			tt -= rint(tt);
-			p.df = (float)d;
 			p.t0 = (float)tt;
 		} else {
-			p.df = 0.0f; p.t0 = 0.0f;
+			p.t0 = 0.0f;
 		}
 		float2 K[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
 		if (a.fixed_n > 0) {
 			quad_orders<FAM, KM>(a, j, act ? a.fixed_n : 0, p, K);
 		} else {
-			// Gauss order per kernel group: degree 1 + table degree
-			int nS = 0, nD = 0, nH = 0;
-			if (KM & SLP) nS = act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
-			if (KM & (DLP | DLPT)) nD = act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
-			if (KM & HSP) nH = act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
+			// Gauss order per kernel group: degree 1 + table degree; beyond the
+			// last breakpoint (22.3 R) every table gives 0, order 1 (one point),
+			// and a wave of such pairs skips the selects
+			int nS = (KM & SLP) && act ? 1 : 0, nD = (KM & (DLP | DLPT)) && act ? 1 : 0, nH = (KM & HSP) && act ? 1 : 0;
+			if (__ballot(act && d2 < t[THR_22_3]) != 0) {
+				if (KM & SLP) nS = act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
+				if (KM & (DLP | DLPT)) nD = act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
+				if (KM & HSP) nH = act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
+			}
 			const int n0 = (KM & SLP) ? nS : (KM & (DLP | DLPT)) ? nD : nH;
 			const bool joint = act && (!(KM & SLP) || nS == n0) && (!(KM & (DLP | DLPT)) || nD == n0) &&
 				(!(KM & HSP) || nH == n0);
@@ -423,9 +466,11 @@ __device__ bool quad_inverse(const Quad &Q, D3 x0, double tol, double &xi, doubl
 // Near-singular, polar: around the image xi0 of x0, K J - K_lin J_lin over
 // the reference square (radial and angular Gauss orders NR, NT given as rule
 // arrays), plus the planar integral over the linearised element.
+// lane / nl: this lane takes every nl-th angular line, planar-part point and
+// dynamic point of the pair (the caller sums the lanes).
 template <int FAM, int MASK>
 __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, const double *grx, const double *grw,
-	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4])
+	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4], int lane, int nl)
 {
 	double xi0, eta0, zeta;
 	if (!quad_inverse(Q, x0, 1e-5, xi0, eta0, zeta)) return false;
@@ -433,9 +478,9 @@ __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, 
 	D3 a, b;
 	quad_dx(Q, xi0, eta0, a, b);
 	const D3 xl0 = quad_x(Q, xi0, eta0);
-	const D3 nl = d3cross(a, b);
-	const double Jl = d3norm(nl);
-	const D3 nul = d3scale(nl, 1.0 / Jl);
+	const D3 nlin = d3cross(a, b);
+	const double Jl = d3norm(nlin);
+	const D3 nul = d3scale(nlin, 1.0 / Jl);
 	const double cx[4] = {-1.0, 1.0, 1.0, -1.0}, cy[4] = {-1.0, -1.0, 1.0, 1.0};
 	D3 L[4];
 	for (int c = 0; c < 4; ++c) L[c] = d3add(xl0, d3add(d3scale(a, cx[c] - xi0), d3scale(b, cy[c] - eta0)));
@@ -459,7 +504,7 @@ __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, 
 			else t2 += 2.0 * PI_D;
 		}
 		if (fabs(t2 - t1) < 1e-3) continue;
-		for (int it = 0; it < nt; ++it) {
+		for (int it = (lane + nt * nl - (nt * n) % nl) % nl; it < nt; it += nl) {
 			const double th = ((1.0 - gtx[it]) * t1 + (1.0 + gtx[it]) * t2) / 2.0;
 			const double wt = gtw[it] * (t2 - t1) / 2.0;
 			double st, ct;
@@ -478,14 +523,14 @@ __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, 
 	// planar integral over the linearised element: Laplace static part and,
 	// for Helmholtz, the 8 x 8-point Gauss rule of K_helm - K_lap
 	double st[4] = {0, 0, 0, 0};
-	near_static<MASK, 4>(L, x0, nx0, R, st);
+	near_static<MASK, 4>(L, x0, nx0, R, st, lane, nl);
 	for (int t = 0; t < 4; ++t) s[t].x += st[t];
 	if (FAM == HELM)
-		for (int i = 0; i < 8; ++i)
-			for (int j = 0; j < 8; ++j) {
-				const D3 yl = d3add(xl0, d3add(d3scale(a, R.gl8x[i] - xi0), d3scale(b, R.gl8x[j] - eta0)));
-				dynamic_point<MASK>(x0, yl, nx0, nul, k, R.gl8w[i] * R.gl8w[j] * Jl, s);
-			}
+		for (int q = lane; q < 64; q += nl) {
+			const int i = q >> 3, j = q & 7;
+			const D3 yl = d3add(xl0, d3add(d3scale(a, R.gl8x[i] - xi0), d3scale(b, R.gl8x[j] - eta0)));
+			dynamic_point<MASK>(x0, yl, nx0, nul, k, R.gl8w[i] * R.gl8w[j] * Jl, s);
+		}
 	return true;
 }
 
@@ -506,7 +551,8 @@ __device__ __forceinline__ void telles_coeffs(double e, double rb, double &ca, d
 }
 
 template <int FAM, int MASK>
-__device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4])
+__device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4], int lane,
+	int nl)
 {
 	double xi0, eta0, zeta;
 	if (!quad_inverse(Q, x0, 1e-6, xi0, eta0, zeta)) return false;
@@ -518,7 +564,7 @@ __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const R
 	double a0, b0, c0, e0, a1, b1, c1, e1;
 	telles_coeffs(xi0, rb, a0, b0, c0, e0);
 	telles_coeffs(eta0, rb, a1, b1, c1, e1);
-	for (int i = 0; i < 16; ++i) {
+	for (int i = lane; i < 16; i += nl) {
 		const double x = R.gl16x[i];
 		const double xi = a0 * x * x * x + b0 * x * x + c0 * x + e0;
 		const double jx = 3.0 * a0 * x * x + 2.0 * b0 * x + c0;
@@ -532,38 +578,10 @@ __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const R
 	return true;
 }
 
-template <int FAM, int KM, int MODE, bool DIAG>
-__global__ void __launch_bounds__(64) quad_fp64_kernel(NearArgs a)
+// store one FP64 pair result (single operator or the Burton-Miller pair)
+template <int KM, int MODE, bool DIAG>
+__device__ __forceinline__ void quad_store64(const NearArgs &a, int i, int j, const double2 (&s)[4])
 {
-	const int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= a.n_pairs) return;
-	const int i = DIAG ? p : a.pi[p];
-	const int j = DIAG ? p : a.pj[p];
-	const Rules &R = *a.rules;
-	Quad Q;
-	for (int v = 0; v < 4; ++v)
-		Q.X[v] = {a.X[(3 * v) * a.sn + j], a.X[(3 * v + 1) * a.sn + j], a.X[(3 * v + 2) * a.sn + j]};
-	const D3 x0 = {a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i]};
-	const D3 nx = {a.tnu[i], a.tnu[a.tn + i], a.tnu[2 * a.tn + i]};
-	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-	bool ok = true;
-	if (DIAG) {
-		if (KM & (SLP | DLP | DLPT)) quad_blind<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, a.k, R, s);
-		if (KM & HSP) s[3] = quad_guiggiani<FAM>(Q, a.k, R);
-	} else if (FAM == HELM) {
-		if (KM & SLP) ok = quad_near_polar<FAM, SLP>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s);
-		if (ok && (KM & (DLP | DLPT | HSP))) ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP)>(Q, x0, nx, a.k, R, s);
-	} else {
-		if (KM & (SLP | DLP | DLPT))
-			ok = quad_near_polar<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, 0.0, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s);
-		if (ok && (KM & HSP))
-			ok = quad_near_polar<FAM, HSP>(Q, x0, nx, 0.0, 51, R.gl51x, R.gl51w, 51, R.gl51x, R.gl51w, R, s);
-	}
-	if (!ok) {
-		// the inverse mapping did not converge: NaN marks the entry
-		const double nan = __builtin_nan("");
-		for (int t = 0; t < 4; ++t) s[t] = make_double2(nan, nan);
-	}
 	const long long idx = (long long)j * a.ld + i;
 	if (MODE == OUT_SINGLE) {
 		const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
@@ -583,6 +601,74 @@ __global__ void __launch_bounds__(64) quad_fp64_kernel(NearArgs a)
 		store_entry64(a.out0, idx, a.dtype, a.accumulate, ar, ai);
 		store_entry64(a.out1, idx, a.dtype, a.accumulate, br, bi);
 	}
+}
+
+__device__ __forceinline__ void quad_load(const NearArgs &a, int i, int j, Quad &Q, D3 &x0, D3 &nx)
+{
+	for (int v = 0; v < 4; ++v)
+		Q.X[v] = {a.X[(3 * v) * a.sn + j], a.X[(3 * v + 1) * a.sn + j], a.X[(3 * v + 2) * a.sn + j]};
+	x0 = {a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i]};
+	nx = {a.tnu[i], a.tnu[a.tn + i], a.tnu[2 * a.tn + i]};
+}
+
+// Diagonal pairs: one lane per element (Duffy rule, Guiggiani).
+template <int FAM, int KM, int MODE>
+__global__ void __launch_bounds__(64) quad_diag_kernel(NearArgs a)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= a.n_pairs) return;
+	const Rules &R = *a.rules;
+	Quad Q;
+	D3 x0, nx;
+	quad_load(a, p, p, Q, x0, nx);
+	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+	if (KM & (SLP | DLP | DLPT)) quad_blind<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, a.k, R, s);
+	if (KM & HSP) s[3] = quad_guiggiani<FAM>(Q, a.k, R);
+	quad_store64<KM, MODE, true>(a, p, p, s);
+}
+
+// Near-singular pairs: QLPP lanes per pair share the angular lines, planar
+// and Telles points (each lane repeats the pair's inverse mapping), then a
+// butterfly sum over the lanes; the pair's first lane stores.
+constexpr int QLPP = 4;
+
+template <int FAM, int KM, int MODE>
+__global__ void __launch_bounds__(64) quad_near_kernel(NearArgs a)
+{
+	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+	const int p = gid / QLPP, lane = gid % QLPP;
+	// lanes past the last pair repeat it and do not store
+	const bool valid = p < a.n_pairs;
+	const int pp = valid ? p : a.n_pairs - 1;
+	const int i = a.pi[pp], j = a.pj[pp];
+	const Rules &R = *a.rules;
+	Quad Q;
+	D3 x0, nx;
+	quad_load(a, i, j, Q, x0, nx);
+	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+	bool ok = true;
+	if (FAM == HELM) {
+		if (KM & SLP) ok = quad_near_polar<FAM, SLP>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s, lane, QLPP);
+		if (ok && (KM & (DLP | DLPT | HSP)))
+			ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP)>(Q, x0, nx, a.k, R, s, lane, QLPP);
+	} else {
+		if (KM & (SLP | DLP | DLPT))
+			ok = quad_near_polar<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, 0.0, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s,
+				lane, QLPP);
+		if (ok && (KM & HSP))
+			ok = quad_near_polar<FAM, HSP>(Q, x0, nx, 0.0, 51, R.gl51x, R.gl51w, 51, R.gl51x, R.gl51w, R, s, lane, QLPP);
+	}
+	for (int off = QLPP / 2; off >= 1; off >>= 1)
+		for (int t = 0; t < 4; ++t) {
+			s[t].x += __shfl_xor(s[t].x, off, QLPP);
+			s[t].y += __shfl_xor(s[t].y, off, QLPP);
+		}
+	if (!ok) {
+		// the inverse mapping did not converge: NaN marks the entry
+		const double nan = __builtin_nan("");
+		for (int t = 0; t < 4; ++t) s[t] = make_double2(nan, nan);
+	}
+	if (valid && lane == 0) quad_store64<KM, MODE, false>(a, i, j, s);
 }
 
 } // namespace nbk
