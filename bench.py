@@ -480,7 +480,8 @@ def run_quad(args):
     write_bytes = 2 * 8 * n_reg
     gbs = write_bytes / (reg_ms * 1e-3) / 1e9 if reg_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                "frac": round(gbs / PEAK_HBM_GBS, 4),
+                "traffic": read_traffic(os.path.join(ROOT, "profiles", "quad_regular_cubed58_traffic.txt")),
                 "kernel": "quad_regular_kernel<HELM,ALL4,OUT_BM>", "kernel_ms": round(reg_ms, 4),
                 "algorithmic_bytes": write_bytes}
     cpu = err = None
