@@ -78,7 +78,11 @@ class Server:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = 0.8 * free
         self.budget = float(hbm_budget)
+        # the configuration policy: 7-point rule far, FP64 within 2 element
+        # diameters; quadrilaterals keep NiHu's near-singular rule (d/R < 1.5),
+        # the only one their near-singular methods accept
         self.policy = policy or nbem.policy(**nbem.CONFIG_POLICY)
+        self.quad_policy = policy or nbem.policy(rule="fixed", degree=5, near="mean_edge", limit=1.5)
         for key, m in self.meshes.items():
             if footprint_bytes(m.n) > self.budget:
                 raise ValueError("mesh %s does not fit the HBM budget" % key)
@@ -91,7 +95,8 @@ class Server:
         with torch.cuda.stream(stream):
             A = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
             B = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
-            nbem.assemble_bm(mesh, req.k, A, B, pol=self.policy, stream=stream)
+            pol = self.quad_policy if mesh.nodes_per_element == 4 else self.policy
+            nbem.assemble_bm(mesh, req.k, A, B, pol=pol, stream=stream)
             d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")
             phase = -req.k * (c @ d)
             pinc = torch.polar(torch.ones_like(phase), phase)
@@ -151,10 +156,15 @@ class Server:
         return results
 
 
-def _centres_normals(nodes, tris):
-    X = nodes[tris]
+def _centres_normals(nodes, elems):
+    """Collocation points and unit normals: triangle centroids and flat
+    normals, or for bilinear quadrilaterals x(0, 0) and dx/dxi x dx/deta there."""
+    X = nodes[elems]
     c = X.mean(axis=1)
-    nv = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+    if elems.shape[1] == 3:
+        nv = np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+    else:
+        nv = np.cross(X[:, 1] + X[:, 2] - X[:, 0] - X[:, 3], X[:, 2] + X[:, 3] - X[:, 0] - X[:, 1])
     return c, nv / np.linalg.norm(nv, axis=1)[:, None]
 
 

@@ -1765,4 +1765,21 @@ int orc_telles_rule(int degree, double xi0, double eta0, double d, double *pts, 
 	ORC_CATCH
 }
 
+// quadrilateral geometry as the oracle computes it: centre, unit normal at the
+// centre, mean edge, diameter (n x 3, n x 3, n, n)
+int orc_geometry_quad(const double *xyz, int n_nodes, const int *quad, int n, double *c, double *nu, double *R,
+	double *diam)
+{
+	ORC_TRY
+	std::vector<QElem> q;
+	read_elems(xyz, n_nodes, quad, n, 4, nullptr, &q, nullptr);
+	for (int e = 0; e < n; ++e) {
+		c[3 * e] = q[e].c.x; c[3 * e + 1] = q[e].c.y; c[3 * e + 2] = q[e].c.z;
+		nu[3 * e] = q[e].nu.x; nu[3 * e + 1] = q[e].nu.y; nu[3 * e + 2] = q[e].nu.z;
+		R[e] = q[e].R;
+		diam[e] = q[e].diam;
+	}
+	ORC_CATCH
+}
+
 } // extern "C"

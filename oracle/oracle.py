@@ -72,6 +72,7 @@ def lib():
         l.orc_quad_elem.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, D, C.c_int, C.c_int, D]
         l.orc_quad_duffy_rule.argtypes = [C.c_int, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_telles_rule.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, C.c_int, I]
+        l.orc_geometry_quad.argtypes = [D, C.c_int, I, C.c_int, D, D, D, D]
         _lib = l
     return _lib
 
@@ -312,3 +313,12 @@ def telles_rule(degree, xi0, eta0, d):
     n = C.c_int(0)
     _check(lib().orc_telles_rule(degree, xi0, eta0, d, _d(pts), cap, C.byref(n)))
     return pts[:n.value].copy()
+
+
+def geometry_quad(nodes, quads):
+    """Quadrilateral centres x(0, 0), unit normals there, mean edges, diameters."""
+    nodes, quads = _mesh(nodes, quads)
+    n = len(quads)
+    c, nu, R, diam = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    _check(lib().orc_geometry_quad(_d(nodes), len(nodes), _i(quads), n, _d(c), _d(nu), _d(R), _d(diam)))
+    return dict(center=c, normal=nu, R=R, diam=diam)

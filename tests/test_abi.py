@@ -81,6 +81,25 @@ def test_engine_geometry_bitwise_equals_oracle(orc, raw):
     assert np.allclose(nu.reshape(3, n).T, g["normal"], rtol=0, atol=1e-15)
 
 
+def test_engine_quad_geometry_bitwise_equals_oracle(orc, raw):
+    """Centres, mean edges and diameters of quadrilaterals round exactly as the
+    oracle's (the pair classification compares against thresholds built on them)."""
+    from nihu_amd.mesh import jittered_cubed_sphere
+    nodes, quads = jittered_cubed_sphere(7, seed=3)
+    n = len(quads)
+    c, nu, J, R, diam, X = (np.zeros(s) for s in (3 * n, 3 * n, n, n, n, 12 * n))
+    q32 = np.ascontiguousarray(quads, dtype=np.int32)
+    raw.nbg_geometry_quad.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int32), C.c_int] + \
+        [C.POINTER(C.c_double)] * 6
+    st = raw.nbg_geometry_quad(_p(nodes), len(nodes), q32.ctypes.data_as(C.POINTER(C.c_int32)), n,
+                               _p(c), _p(nu), _p(J), _p(R), _p(diam), _p(X))
+    assert st == 0
+    g = orc.geometry_quad(nodes, quads)
+    assert np.array_equal(c.reshape(3, n).T, g["center"])
+    assert np.array_equal(R, g["R"]) and np.array_equal(diam, g["diam"])
+    assert np.allclose(nu.reshape(3, n).T, g["normal"], rtol=0, atol=1e-15)
+
+
 def test_threshold_reproduces_sqrt_division_test(raw):
     """d2 < threshold(R, x)  <=>  sqrt(d2) / R < x, checked at and around the
     threshold (where a rounding slip would show)."""

@@ -70,3 +70,21 @@ def test_oversized_mesh_rejected(meshes):
     from nihu_amd import serve
     with pytest.raises(ValueError):
         serve.Server({"L4": meshes["L4"]}, slots=1, hbm_budget=1e6)
+
+
+def test_mie_rigid_sphere_quadrilaterals():
+    """The same Mie check on a quadrilateral cubed sphere (collocation at the
+    element centres, Guiggiani hypersingular diagonal)."""
+    from nihu_amd import serve
+    from nihu_amd.mesh import cubed_sphere
+    nodes, quads = cubed_sphere(12)
+    srv = serve.Server({"Q": (nodes, quads)}, slots=1)
+    k = 3.0
+    d = np.array([0.0, 0.0, 1.0])
+    res = srv.run([serve.Request(0, "Q", k, tuple(d))], realtime=False)[0]
+    c, _ = serve._centres_normals(nodes, quads)
+    ptot = np.exp(-1j * k * (c @ d)) + res.p
+    mie = serve.mie_rigid_sphere_surface(k, 1.0, (c @ d) / np.linalg.norm(c, axis=1))
+    err = np.linalg.norm(ptot - mie) / np.linalg.norm(mie)
+    print("Mie surface error, 864 quadrilaterals", err, flush=True)
+    assert err < 0.03
