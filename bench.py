@@ -140,6 +140,8 @@ def main():
                          "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere")
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
+    ap.add_argument("--refine", type=int, default=0,
+                    help="serve: steps of FP64 iterative refinement of each complex64 LU solve")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -617,7 +619,7 @@ def run_serve(args):
     torch.cuda.set_device(0)
     meshes = {"L3": jittered_icosphere(3, seed=0), "L4": jittered_icosphere(4, seed=0),
               "L5": jittered_icosphere(5, seed=0)}
-    srv = serve.Server(meshes, slots=4)
+    srv = serve.Server(meshes, slots=4, refine=args.refine)
     warm = [serve.Request(-1 - i, key, 5.0, (0.0, 0.0, 1.0)) for i, key in enumerate(meshes)]
     srv.run(warm, realtime=False)
     reqs = serve.poisson_stream(args.requests, rate=args.rate, seed=11)
@@ -636,7 +638,7 @@ def run_serve(args):
         "ms_per_step": wall / len(res) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 (complex64) far pairs and LU, fp64 near/diagonal", "data": "synthetic: seeded Poisson stream",
         "config": {"workload": "serving", "meshes_n_tri": {k: len(v[1]) for k, v in meshes.items()},
-                   "ka": [1.0, 40.0], "rate_per_s": args.rate, "slots": 4},
+                   "ka": [1.0, 40.0], "rate_per_s": args.rate, "slots": 4, "refine_steps": args.refine},
         "latency_s": {"p50": serve.percentile(lat, 50), "p99": serve.percentile(lat, 99),
                       "per_mesh_p50": {k: serve.percentile(v, 50) for k, v in by.items()},
                       "per_mesh_p99": {k: serve.percentile(v, 99) for k, v in by.items()}},

@@ -63,7 +63,11 @@ def footprint_bytes(n: int) -> int:
 
 
 class Server:
-    def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None):
+    def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None, refine: int = 0):
+        """refine: steps of iterative refinement of the complex64 LU solution
+        (residual b - A p accumulated in complex128 over column blocks,
+        correction solved with the same LU factor)."""
+        self.refine = int(refine)
         import torch
         self.torch = torch
         self.meshes = {}
@@ -104,6 +108,8 @@ class Server:
             rhs = (B @ q.to(torch.complex64)).unsqueeze(1)
             LU, piv, _ = torch.linalg.lu_factor_ex(A)
             p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
+            if self.refine > 0:
+                p = _refine(A, LU, piv, rhs.squeeze(1), p, self.refine)
             ev = torch.cuda.Event()
             ev.record(stream)
         # keep the buffers alive until the event has passed
@@ -154,6 +160,25 @@ class Server:
                 time.sleep(min(1e-3, max(0.0, pending[0].arrival - now())))
         torch.cuda.synchronize()
         return results
+
+
+def _refine(A, LU, piv, b, p, steps, block=2048):
+    """Iterative refinement of A p = b (complex64 A and LU factor): the
+    residual b - A p in complex128, A upcast one column block at a time so
+    that no complex128 copy of A is held, the correction solved in complex64
+    with the same factor; returns p in complex128."""
+    import torch
+    p = p.to(torch.complex128)
+    b = b.to(torch.complex128)
+    n = A.shape[1]
+    for _ in range(steps):
+        r = b.clone()
+        for j0 in range(0, n, block):
+            j1 = min(n, j0 + block)
+            r -= A[:, j0:j1].to(torch.complex128) @ p[j0:j1]
+        d = torch.linalg.lu_solve(LU, piv, r.to(torch.complex64).unsqueeze(1)).squeeze(1)
+        p = p + d.to(torch.complex128)
+    return p
 
 
 def _centres_normals(nodes, elems):

@@ -88,3 +88,34 @@ def test_mie_rigid_sphere_quadrilaterals():
     err = np.linalg.norm(ptot - mie) / np.linalg.norm(mie)
     print("Mie surface error, 864 quadrilaterals", err, flush=True)
     assert err < 0.03
+
+
+def test_iterative_refinement(meshes):
+    """FP64 iterative refinement of the complex64 LU solve: the refined field
+    approaches the complex128 solution of the same complex64 system; the
+    server with refinement agrees with the plain solve to the latter's
+    accuracy."""
+    import torch
+    from nihu_amd import serve, nbem
+    nodes, tris = meshes["L3"]
+    k, d = 7.0, np.array([0.0, 0.6, 0.8])
+    m = nbem.Mesh(nodes, tris)
+    n = len(tris)
+    A, B = nbem.colmajor(n, n), nbem.colmajor(n, n)
+    nbem.assemble_bm(m, k, A, B, pol=nbem.policy(**nbem.CONFIG_POLICY))
+    c, nu = serve._centres_normals(nodes, tris)
+    q = (1j * k) * (nu @ d) * np.exp(-1j * k * (c @ d))
+    b = B @ torch.from_numpy(q.astype(np.complex64)).cuda()
+    LU, piv, _ = torch.linalg.lu_factor_ex(A)
+    p0 = torch.linalg.lu_solve(LU, piv, b.unsqueeze(1)).squeeze(1)
+    p2 = serve._refine(A, LU, piv, b, p0, 2, block=256)
+    exact = torch.linalg.solve(A.to(torch.complex128), b.to(torch.complex128).unsqueeze(1)).squeeze(1)
+    torch.cuda.synchronize()
+    e0 = (torch.linalg.norm(p0.to(torch.complex128) - exact) / torch.linalg.norm(exact)).item()
+    e2 = (torch.linalg.norm(p2 - exact) / torch.linalg.norm(exact)).item()
+    print("complex64 LU %.2e, refined twice %.2e" % (e0, e2), flush=True)
+    assert e2 < 1e-3 * e0 or e2 < 1e-12
+    plain = serve.Server({"L3": (nodes, tris)}, slots=1).run([serve.Request(0, "L3", k, tuple(d))], realtime=False)[0].p
+    refd = serve.Server({"L3": (nodes, tris)}, slots=1, refine=2).run([serve.Request(0, "L3", k, tuple(d))],
+                                                                        realtime=False)[0].p
+    assert np.linalg.norm(refd - plain) / np.linalg.norm(refd) < 10 * e0 + 1e-6
