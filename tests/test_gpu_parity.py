@@ -197,3 +197,38 @@ def test_headline_workload_rows(nb, orc):
     assert ea < FRO_TOL and eb < FRO_TOL
     # every entry was written (no NaN, nothing left from a previous buffer)
     assert torch.isfinite(torch.view_as_real(A)).all() and torch.isfinite(torch.view_as_real(B)).all()
+
+
+def test_full_size_properties(nb):
+    """Size-independent properties on the whole 20480-triangle matrices (no
+    oracle needed): Gauss's identity for the Laplace double layer (every row of
+    D sums to -1/2 on a closed surface; the FP64 oracle's rows deviate by
+    3e-6), the hypersingular operator annihilating constants (oracle: 1e-7 of
+    the diagonal), and accumulation doubling every entry exactly."""
+    import torch
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(5, seed=0)
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    pol = nb.policy(**POLICIES["config"][0])
+    D = nb.colmajor(n, n, "float32")
+    nb.assemble(m, m, D, nb.NB_LAPLACE, 0, 1, 0.0, pol=pol, accumulate=False)
+    torch.cuda.synchronize()
+    rs = D.to(torch.float64).sum(dim=1)
+    dev = (rs + 0.5).abs().max().item()
+    del D
+    H = nb.colmajor(n, n, "float32")
+    nb.assemble(m, m, H, nb.NB_LAPLACE, 1, 1, 0.0, pol=pol, accumulate=False)
+    torch.cuda.synchronize()
+    hs = H.to(torch.float64).sum(dim=1).abs().max().item()
+    hd = torch.diagonal(H).abs().max().item()
+    del H
+    print("DLP row sums: max |sum + 1/2| %.2e; HSP row sums %.2e of the diagonal" % (dev, hs / hd), flush=True)
+    assert dev < 1e-4
+    assert hs < 1e-5 * hd
+    G = nb.colmajor(n, n)
+    nb.assemble(m, m, G, nb.NB_HELMHOLTZ, 0, 0, 10.0, pol=pol, accumulate=False)
+    G1 = G.clone()
+    nb.assemble(m, m, G, nb.NB_HELMHOLTZ, 0, 0, 10.0, pol=pol, accumulate=True)
+    torch.cuda.synchronize()
+    assert torch.equal(torch.view_as_real(G), 2 * torch.view_as_real(G1))
