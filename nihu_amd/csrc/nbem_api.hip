@@ -1129,7 +1129,19 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 		da.X = sa.X; da.J = (const double *)m->d_J.get(); da.rules = rules; da.st = sa.st;
 		da.k = (const double *)d_kd.get(); da.alpha = (const double2 *)d_ald.get();
 		da.A = (float2 *)A; da.B = (float2 *)B; da.ld = ld; da.kstride = k_stride;
-		const long long nt = (long long)np * n_k;
+		// uniformly spaced wavenumbers (to 1e-12 relative): the near pairs rotate
+		// e^{-ikr} from one wavenumber to the next instead of evaluating sincos
+		bool uniform = n_k >= 2 && nl->n > 0;
+		const double dk = n_k >= 2 ? (k[n_k - 1] - k[0]) / (n_k - 1) : 0.0;
+		for (int f = 0; f < n_k && uniform; ++f)
+			uniform = std::fabs(k[f] - (k[0] + f * dk)) <= 1e-12 * std::fabs(k[f]);
+		da.p0 = uniform ? nl->n : 0;
+		if (uniform) {
+			const long long nr = (long long)nl->n * ((n_k + nbk::KBLK - 1) / nbk::KBLK);
+			hipLaunchKernelGGL(nbk::fp64_dynamic_rec_kernel, dim3((nr + 63) / 64), dim3(64), 0, side, da, dk);
+			NB_HIP(hipGetLastError());
+		}
+		const long long nt = (long long)(np - da.p0) * n_k;
 		hipLaunchKernelGGL(nbk::fp64_dynamic_kernel, dim3((nt + 63) / 64), dim3(64), 0, side, da);
 		NB_HIP(hipGetLastError());
 		prof_end(pe1, side);

@@ -488,6 +488,34 @@ __device__ __forceinline__ double rsqrt64(double x)
 	return __fma_rn(0.5 * y, e, y);
 }
 
+// K_helm - K_lap of the kernels of MASK from e^{-ikr} = cs - i sn, kr and the
+// point's geometry (g0 = w J / (4 pi r), rinv = 1/r, rdny, rdnx, nn)
+template <int MASK>
+__device__ __forceinline__ void dynamic_terms(double cs, double sn, double kr, double g0, double rinv, double rdny,
+	double rdnx, double nn, double2 (&s)[4])
+{
+	if (MASK & SLP) {
+		s[0].x += (cs - 1.0) * g0;
+		s[0].y += -sn * g0;
+	}
+	if (MASK & (DLP | DLPT | HSP)) {
+		const double g1 = g0 * rinv;
+		// e(-1 - ikr) - (-1)
+		const double er = -cs - kr * sn + 1.0;
+		const double ei = -kr * cs + sn;
+		if (MASK & DLP) { s[1].x += er * g1 * rdny; s[1].y += ei * g1 * rdny; }
+		if (MASK & DLPT) { s[2].x += er * g1 * rdnx; s[2].y += ei * g1 * rdnx; }
+		if (MASK & HSP) {
+			const double g3 = g1 * rinv;
+			const double cc = rdnx * rdny;
+			// e [(3 - kr^2 + 3ikr) cc + (1 + ikr) nn] - [3 cc + nn]
+			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
+			s[3].x += ((cs * ar + sn * ai) - (3.0 * cc + nn)) * g3;
+			s[3].y += (cs * ai - sn * ar) * g3;
+		}
+	}
+}
+
 // K_helm - K_lap for the kernels of MASK at one point (FP64), r = y - x
 template <int MASK>
 __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4])
@@ -501,29 +529,13 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 	sincos(kr, &sn, &cs);
 	// e^{-ikr} = cs - i sn
 	const double g0 = rinv * (wj / (4.0 * PI_D));
-	if (MASK & SLP) {
-		s[0].x += (cs - 1.0) * g0;
-		s[0].y += -sn * g0;
-	}
+	double rdny = 0.0, rdnx = 0.0, nn = 0.0;
 	if (MASK & (DLP | DLPT | HSP)) {
-		const double rdny = d3dot(rv, ny) * rinv;
-		const double rdnx = -d3dot(rv, nx) * rinv;
-		const double g1 = g0 * rinv;
-		// e(-1 - ikr) - (-1)
-		const double er = -cs - kr * sn + 1.0;
-		const double ei = -kr * cs + sn;
-		if (MASK & DLP) { s[1].x += er * g1 * rdny; s[1].y += ei * g1 * rdny; }
-		if (MASK & DLPT) { s[2].x += er * g1 * rdnx; s[2].y += ei * g1 * rdnx; }
-		if (MASK & HSP) {
-			const double g3 = g1 * rinv;
-			const double cc = rdnx * rdny;
-			const double nn = d3dot(nx, ny);
-			// e [(3 - kr^2 + 3ikr) cc + (1 + ikr) nn] - [3 cc + nn]
-			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
-			s[3].x += ((cs * ar + sn * ai) - (3.0 * cc + nn)) * g3;
-			s[3].y += (cs * ai - sn * ar) * g3;
-		}
+		rdny = d3dot(rv, ny) * rinv;
+		rdnx = -d3dot(rv, nx) * rinv;
+		if (MASK & HSP) nn = d3dot(nx, ny);
 	}
+	dynamic_terms<MASK>(cs, sn, kr, g0, rinv, rdny, rdnx, nn, s);
 }
 
 // Laplace static part of the near-singular integral over a flat triangle in

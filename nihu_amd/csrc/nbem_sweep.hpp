@@ -176,6 +176,7 @@ __global__ void __launch_bounds__(64) fp64_static_kernel(StaticArgs a)
 
 struct DynArgs {
 	int n, n_near, n_k;
+	int p0;                     // first pair of the launch (pairs p0 .. n_near + n - 1)
 	const int *pi, *pj;
 	const double *c, *nu64, *X, *J;
 	const Rules *rules;
@@ -190,10 +191,10 @@ struct DynArgs {
 __global__ void __launch_bounds__(64) fp64_dynamic_kernel(DynArgs a)
 {
 	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	const long long np = (long long)a.n_near + a.n;
+	const long long np = (long long)a.n_near + a.n - a.p0;
 	if (t >= np * a.n_k) return;
 	const int f = (int)(t / np);
-	const int p = (int)(t % np);
+	const int p = a.p0 + (int)(t % np);
 	const bool diag = p >= a.n_near;
 	const int i = diag ? p - a.n_near : a.pi[p];
 	const int j = diag ? i : a.pj[p];
@@ -231,6 +232,74 @@ __global__ void __launch_bounds__(64) fp64_dynamic_kernel(DynArgs a)
 	const long long idx = (long long)f * a.kstride + (long long)j * a.ld + i;
 	a.A[idx] = make_float2((float)ar, (float)ai);
 	a.B[idx] = make_float2((float)br, (float)bi);
+}
+
+// Near pairs over uniformly spaced wavenumbers k_f = k_0 + f dk: one lane per
+// (near pair, block of KBLK consecutive wavenumbers); at each of the 42 points
+// e^{-i k r} comes from sincos once per block and is rotated by e^{-i dk r}
+// (one FP64 complex product) from one wavenumber to the next; the other
+// factors use the actual k_f.  Same terms as fp64_dynamic_kernel otherwise.
+constexpr int KBLK = 4;
+
+__global__ void __launch_bounds__(64) fp64_dynamic_rec_kernel(DynArgs a, double dk)
+{
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int nblk = (a.n_k + KBLK - 1) / KBLK;
+	if (t >= (long long)a.n_near * nblk) return;
+	const int fb = (int)(t / a.n_near);
+	const int p = (int)(t % a.n_near);
+	const int i = a.pi[p], j = a.pj[p];
+	const Rules &R = *a.rules;
+	D3 V[3];
+	for (int v = 0; v < 3; ++v)
+		V[v] = {a.X[(3 * v) * a.n + j], a.X[(3 * v + 1) * a.n + j], a.X[(3 * v + 2) * a.n + j]};
+	const D3 x0 = {a.c[i], a.c[a.n + i], a.c[2 * a.n + i]};
+	const D3 nx = {a.nu64[i], a.nu64[a.n + i], a.nu64[2 * a.n + i]};
+	const D3 ny = {a.nu64[j], a.nu64[a.n + j], a.nu64[2 * a.n + j]};
+	const double J = a.J[j];
+	const int f0 = fb * KBLK;
+	double kf[KBLK];
+	double2 s[KBLK][4];
+#pragma unroll
+	for (int m = 0; m < KBLK; ++m) {
+		kf[m] = a.k[min(f0 + m, a.n_k - 1)];
+		for (int q = 0; q < 4; ++q) s[m][q] = make_double2(a.st[(size_t)p * 4 + q], 0.0);
+	}
+	const double nn = d3dot(nx, ny);
+	for (int q = 0; q < 42; ++q) {
+		const D3 y = tri_x(V, R.d14xi[q], R.d14eta[q]);
+		const D3 rv = d3sub(y, x0);
+		const double r2 = d3dot(rv, rv);
+		const double rinv = rsqrt64(r2);
+		const double r = r2 * rinv;
+		const double g0 = rinv * ((R.d14w[q] * J) / (4.0 * PI_D));
+		const double rdny = d3dot(rv, ny) * rinv;
+		const double rdnx = -d3dot(rv, nx) * rinv;
+		double sn, cs, sd, cd;
+		sincos(kf[0] * r, &sn, &cs);
+		sincos(dk * r, &sd, &cd);
+#pragma unroll
+		for (int m = 0; m < KBLK; ++m) {
+			dynamic_terms<ALL4>(cs, sn, kf[m] * r, g0, rinv, rdny, rdnx, nn, s[m]);
+			// e^{-i (k + dk) r} = e^{-i k r} e^{-i dk r}
+			const double c2 = cs * cd - sn * sd;
+			sn = sn * cd + cs * sd;
+			cs = c2;
+		}
+	}
+#pragma unroll
+	for (int m = 0; m < KBLK; ++m) {
+		const int f = f0 + m;
+		if (f >= a.n_k) break;
+		const double2 al = a.alpha[f];
+		const double ar = al.x * s[m][3].x - al.y * s[m][3].y + s[m][1].x;
+		const double ai = al.x * s[m][3].y + al.y * s[m][3].x + s[m][1].y;
+		const double br = s[m][0].x + al.x * s[m][2].x - al.y * s[m][2].y;
+		const double bi = s[m][0].y + al.x * s[m][2].y + al.y * s[m][2].x;
+		const long long idx = (long long)f * a.kstride + (long long)j * a.ld + i;
+		a.A[idx] = make_float2((float)ar, (float)ai);
+		a.B[idx] = make_float2((float)br, (float)bi);
+	}
 }
 
 } // namespace nbk

@@ -56,6 +56,35 @@ def test_sweep_matches_single_and_oracle(nb, orc, l3, kb):
         assert ea < 1e-5 and eb < 1e-5
 
 
+def test_sweep_uniform_wavenumbers(nb, orc, l3):
+    """Uniformly spaced wavenumbers (9, not a multiple of the 4-wavenumber
+    block): the near-singular pairs rotate e^{-ikr} from one wavenumber to the
+    next; every matrix matches its single-wavenumber assembly, the FP64 near
+    entries to one complex64 rounding, and the oracle."""
+    import torch
+    nodes, tris = l3
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    ks = np.linspace(1.0, 17.0, 9)
+    A, B = stack(len(ks), n), stack(len(ks), n)
+    nb.assemble_sweep(m, ks, A, B, pol=nb.policy(**CONFIG))
+    torch.cuda.synchronize()
+    cls = nb.classify(m, m, 1, 1, nb.policy(**CONFIG)).cpu().numpy()
+    near = cls == -2
+    for f, k in enumerate(ks):
+        A1, B1 = nb.colmajor(n, n), nb.colmajor(n, n)
+        nb.assemble_bm(m, float(k), A1, B1, pol=nb.policy(**CONFIG))
+        torch.cuda.synchronize()
+        a, b, a1, b1 = A[f].cpu().numpy(), B[f].cpu().numpy(), A1.cpu().numpy(), B1.cpu().numpy()
+        assert relfro(a, a1) < 1e-6 and relfro(b, b1) < 1e-6, f
+        for x, y in ((a, a1), (b, b1)):
+            d = np.abs(x[near] - y[near]) / np.abs(y[near]).max()
+            assert d.max() < 2e-7, (f, d.max())
+    for f in (0, 8):
+        Ar, Br = orc.assemble_bm(nodes, tris, ks[f], pol=orc.policy(**CONFIG_O))
+        assert relfro(A[f].cpu().numpy(), Ar) < 1e-5 and relfro(B[f].cpu().numpy(), Br) < 1e-5
+
+
 def test_sweep_nihu_policy(nb, orc, l3):
     import torch
     nodes, tris = l3
