@@ -173,3 +173,61 @@ def test_quad_rejections(nb, qmesh):
     # far "near-singular" pairs would need the inverse mapping far from the element
     with pytest.raises(Exception, match="near-singular rule"):
         nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(rule="fixed", degree=5, near="diameter", limit=2.0))
+
+
+def _plate(n):
+    xs = np.linspace(0.0, 1.0, n + 1)
+    nodes = np.array([[x, y, 0.0] for y in xs for x in xs])
+    quads = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            quads.append([a, a + 1, a + n + 2, a + n + 1])
+    return nodes, np.array(quads, dtype=np.int32)
+
+
+def test_quad_single_element(nb, orc):
+    """One warped quadrilateral: only the diagonal pass (Duffy, Guiggiani)."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0.2], [1.1, 1, -0.1], [0, 0.9, 0.25]], float)
+    quads = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    for fam, k in ((0, 0.0), (1, 3.0)):
+        for nx, ny in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            got = _assemble(nb, (nodes, quads), (nodes, quads), fam, nx, ny, k, NIHU[0])
+            ref = orc.assemble(nodes, quads, fam, nx, ny, k=k, pol=orc.policy(**NIHU[1]))
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (fam, nx, ny)
+
+
+def test_quad_flat_plate(nb, orc):
+    """Coplanar open surface of squares: every near pair lies in the element
+    plane (normal offset 0 for the Telles map); DLP and DLPt vanish."""
+    nodes, quads = _plate(8)
+    for fam, k in ((0, 0.0), (1, 4.0)):
+        for nx, ny in ((0, 0), (0, 1), (1, 1)):
+            got = _assemble(nb, (nodes, quads), (nodes, quads), fam, nx, ny, k, NIHU[0])
+            ref = orc.assemble(nodes, quads, fam, nx, ny, k=k, pol=orc.policy(**NIHU[1]))
+            if nx + ny == 1:
+                assert np.all(got == 0) and np.all(ref == 0)
+                continue
+            rs = rowscaled(got, ref)
+            print("plate", fam, nx, ny, "row-scaled %.2e" % rs, flush=True)
+            assert rs <= ROW_TOL
+
+
+def test_quad_large_wavenumber(nb, orc, qmesh):
+    """k = 60 on the 150-element cubed sphere (the FP32 phase correction
+    k (r - d) reaches several radians per element)."""
+    nodes, quads = qmesh
+    for nxy in ((0, 0), (1, 1)):
+        got = _assemble(nb, qmesh, qmesh, 1, nxy[0], nxy[1], 60.0, NIHU[0])
+        ref = orc.assemble(nodes, quads, 1, nxy[0], nxy[1], k=60.0, pol=orc.policy(**NIHU[1]))
+        rs = rowscaled(got, ref)
+        print("k = 60", nxy, "row-scaled %.2e rel fro %.2e" % (rs, relfro(got, ref)), flush=True)
+        assert rs <= ROW_TOL and relfro(got, ref) <= 1e-5
+
+
+def test_quad_degenerate_rejected(nb):
+    """A quadrilateral whose normal vanishes (three collinear corners folded
+    onto the fourth) is refused at upload, as a zero-area triangle is."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [3, 0, 0]], float)
+    with pytest.raises(Exception, match="degenerate"):
+        nb.Mesh(nodes, np.array([[0, 1, 2, 3]], dtype=np.int32))
