@@ -1,7 +1,7 @@
 // nbem_oracle.cpp -- FP64 CPU restatement of NiHu's dense BEM assembly for
 // flat triangles (collocation with constant fields, Galerkin with constant
-// and linear isoparametric fields), 3D Laplace and Helmholtz SLP / DLP / DLPt
-// / HSP.
+// and linear isoparametric fields) and bilinear quadrilaterals (collocation
+// with constant fields), 3D Laplace and Helmholtz SLP / DLP / DLPt / HSP.
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the checker of the HIP engine in
 // nihu_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
@@ -22,6 +22,15 @@
 //     edge- and vertex-adjacent pairs under every labelling, the closed-form
 //     solid angle, the analytic self-potential of a square (NiHu's
 //     lenoir_salles_2012 test), Gauss's identity, the mass-matrix closed form.
+//   * for quadrilaterals (collocation, constant fields): closed forms of the
+//     planar SLP and hypersingular integrals over polygons (Duffy rule and
+//     Guiggiani's method at NiHu's guiggiani_test shapes and points), an
+//     independent Hadamard finite part on warped elements (integral outside a
+//     ball minus 1/(2 eps), Richardson), brute-force subdivided quadrature of
+//     the near-singular methods (polar with the linearised element, Telles),
+//     the planar part equal to the sum over two golden-pinned triangles,
+//     inverse mapping round trips, Gauss's identity and a transparent
+//     point-source answer on a cubed sphere.
 // The node-overlap scan keeps NiHu's start-corner rule, including the
 // labelling in which it misses an edge's orientation (documented in the
 // Galerkin tests).
