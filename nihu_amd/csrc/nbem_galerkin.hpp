@@ -416,7 +416,7 @@ __device__ __forceinline__ double2 kernel_value64(D3 x, D3 y, D3 nx, D3 ny, doub
 	double sn = 0.0, cs = 1.0, kr = 0.0;
 	if (FAM == HELM) {
 		kr = k * r;
-		sincos(kr, &sn, &cs);
+		sincos64(kr, &sn, &cs);
 	}
 	const double f = 1.0 / (4.0 * PI_D);
 	// e^{-ikr} = cs - i sn

@@ -113,6 +113,54 @@ __device__ __forceinline__ void store_entry(void *out, long long idx, int dtype,
 	}
 }
 
+// Both Burton-Miller entries of one pair (A to out0, B to out1) under one
+// dtype branch, so that neither value is computed inside a store branch (the
+// compiler would otherwise sink the B sums past the A store and keep the
+// per-point terms alive, costing registers and occupancy).  With accumulate
+// the A entry is written before the B entry is read, as two single stores would.
+__device__ __forceinline__ void store_pair(void *out0, void *out1, long long idx, int dtype, int accumulate, float ar,
+	float ai, float br, float bi)
+{
+	switch (dtype) {
+	case DT_C64: {
+		float2 *p = (float2 *)out0 + idx, *q = (float2 *)out1 + idx;
+		if (accumulate) {
+			const float2 u = *p;
+			*p = make_float2(u.x + ar, u.y + ai);
+			const float2 v = *q;
+			*q = make_float2(v.x + br, v.y + bi);
+		} else {
+			*p = make_float2(ar, ai);
+			*q = make_float2(br, bi);
+		}
+		break;
+	}
+	case DT_C128: {
+		double2 *p = (double2 *)out0 + idx, *q = (double2 *)out1 + idx;
+		if (accumulate) {
+			const double2 u = *p;
+			*p = make_double2(u.x + ar, u.y + ai);
+			const double2 v = *q;
+			*q = make_double2(v.x + br, v.y + bi);
+		} else {
+			*p = make_double2(ar, ai);
+			*q = make_double2(br, bi);
+		}
+		break;
+	}
+	case DT_F32: {
+		float *p = (float *)out0 + idx, *q = (float *)out1 + idx;
+		if (accumulate) { *p += ar; *q += br; } else { *p = ar; *q = br; }
+		break;
+	}
+	default: {
+		double *p = (double *)out0 + idx, *q = (double *)out1 + idx;
+		if (accumulate) { *p += ar; *q += br; } else { *p = ar; *q = br; }
+		break;
+	}
+	}
+}
+
 __device__ __forceinline__ void store_entry64(void *out, long long idx, int dtype, int accumulate, double re, double im)
 {
 	switch (dtype) {
@@ -180,7 +228,7 @@ struct PairCtx {
 //   s[2] = sum Q b                  DLPt  = s[2]
 //   s[3] = sum Q b/r^2              HSP   = nx.ny sum Q - a (3 s[3] - k^2 s[5])
 //   s[4] = sum Q   (HSP group, when it is evaluated apart from DLP)
-//   s[5] = sum E b/r^2
+//   s[5] = sum G b
 // with, per quadrature point y = c_j + off and r = y - x (|r| = r):
 //   E = w e^{-i th} / r,  G = E / r^2,  Q = G (1 + i th),  th = k r,
 //   a = r.n_y = (c_j - x).n_y (off lies in the trial plane), b = r.n_x,
@@ -188,33 +236,53 @@ struct PairCtx {
 // DLPt e(-1-ikr)/(4 pi r^2) (-r.nx)/r and
 // HSP e/(4 pi r^3)[(3+3ikr-(kr)^2) rdnx rdny + (1+ikr) nx.ny]
 // (nihu/library/helmholtz_kernel.hpp:169-228, normal_derivative_kernel.hpp:114-257);
-// since th^2 G = k^2 E, the (kr)^2 term is k^2 times a sum of E;
+// since th^2 G / r^2 = k^2 G, the (kr)^2 term is k^2 times a sum of G b;
 // the weights w carry J w_q / (4 pi).
 // r^2 = d^2 + u with u = 2 (c_j - x).off + |off|^2, and the phase is
 // t0 + (k/2pi) u / (r + d) revolutions: t0 is the pair's FP64-reduced phase of
 // k d and u / (r + d) = r - d keeps full FP32 relative accuracy; sin and cos
 // are the hardware instructions, which take revolutions.
-// Add one point's E = (ec, es), th = k r, 1/r and b = r.n_x to the sums.
+// Add one point to the sums from e^{-i th} = c - i sn, k, w, 1/r, r^2 and
+// b = r.n_x.  With a derivative kernel in MASK, G = w/r^3 e^{-i th} comes
+// first, E = G r^2 is folded into the SLP sum's FMA and th G = k (w/r^2) e^{-i th}
+// needs neither r nor th.
 template <int FAM, int MASK>
-__device__ __forceinline__ void add_point(float ec, float es, float th, float rinv, float b, float2 (&s)[6])
+__device__ __forceinline__ void add_point(float c, float sn, float k, float w, float rinv, float r2, float b,
+	float2 (&s)[6])
 {
-	if (MASK & SLP) { s[0].x += ec; s[0].y += es; }
-	if (MASK & (DLP | DLPT | HSP)) {
-		const float ri2 = rinv * rinv;
-		const float Gr = ec * ri2, Gi = es * ri2;
-		const float Qr = __builtin_fmaf(-th, Gi, Gr);
-		const float Qi = __builtin_fmaf(th, Gr, Gi);
-		if (MASK & DLP) { s[1].x += Qr; s[1].y += Qi; }
-		if ((MASK & HSP) && !(MASK & DLP)) { s[4].x += Qr; s[4].y += Qi; }
-		if (MASK & DLPT) { s[2].x = __builtin_fmaf(Qr, b, s[2].x); s[2].y = __builtin_fmaf(Qi, b, s[2].y); }
-		if (MASK & HSP) {
-			const float w2 = b * ri2;
-			s[3].x = __builtin_fmaf(Qr, w2, s[3].x);
+	// Laplace (c = 1, sn = 0): the imaginary sums stay zero
+	constexpr bool IM = FAM == HELM;
+	if (!(MASK & (DLP | DLPT | HSP))) {
+		const float g = w * rinv;
+		s[0].x = __builtin_fmaf(g, c, s[0].x);
+		if (IM) s[0].y = __builtin_fmaf(-g, sn, s[0].y);
+		return;
+	}
+	const float ri2 = rinv * rinv;
+	const float h = w * ri2;
+	const float g3 = h * rinv;
+	const float Gr = g3 * c, Gi = -g3 * sn;
+	// Q = G (1 + i th): Re = Gr - th Gi = Gr + k h sn, Im = Gi + th Gr = Gi + k h c
+	const float kh = k * h;
+	const float Qr = IM ? __builtin_fmaf(kh, sn, Gr) : Gr;
+	const float Qi = IM ? __builtin_fmaf(kh, c, Gi) : 0.0f;
+	if (MASK & SLP) {
+		s[0].x = __builtin_fmaf(Gr, r2, s[0].x);
+		if (IM) s[0].y = __builtin_fmaf(Gi, r2, s[0].y);
+	}
+	if (MASK & DLP) { s[1].x += Qr; if (IM) s[1].y += Qi; }
+	if ((MASK & HSP) && !(MASK & DLP)) { s[4].x += Qr; if (IM) s[4].y += Qi; }
+	if (MASK & DLPT) {
+		s[2].x = __builtin_fmaf(Qr, b, s[2].x);
+		if (IM) s[2].y = __builtin_fmaf(Qi, b, s[2].y);
+	}
+	if (MASK & HSP) {
+		const float w2 = b * ri2;
+		s[3].x = __builtin_fmaf(Qr, w2, s[3].x);
+		if (IM) {
 			s[3].y = __builtin_fmaf(Qi, w2, s[3].y);
-			if (FAM == HELM) {
-				s[5].x = __builtin_fmaf(ec, w2, s[5].x);
-				s[5].y = __builtin_fmaf(es, w2, s[5].y);
-			}
+			s[5].x = __builtin_fmaf(Gr, b, s[5].x);
+			s[5].y = __builtin_fmaf(Gi, b, s[5].y);
 		}
 	}
 }
@@ -233,22 +301,16 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 		const float u = __builtin_fmaf(p.dcx2, ox, __builtin_fmaf(p.dcy2, oy, __builtin_fmaf(p.dcz2, oz, oo)));
 		const float r2 = p.d2f + u;
 		const float rinv = __builtin_amdgcn_rsqf(r2);
-		const float g = w * rinv;
-		float ec, es, th = 0.0f;
+		float c = 1.0f, sn = 0.0f;
 		if (FAM == HELM) {
-			const float r = r2 * rinv;
-			const float t = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(r + p.df), p.t0);
-			ec = g * __builtin_amdgcn_cosf(t);
-			es = -g * __builtin_amdgcn_sinf(t);
-			th = k * r;
-		} else {
-			ec = g;
-			es = 0.0f;
+			const float t = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(__builtin_fmaf(r2, rinv, p.df)), p.t0);
+			c = __builtin_amdgcn_cosf(t);
+			sn = __builtin_amdgcn_sinf(t);
 		}
 		float b = 0.0f;
 		if (MASK & (DLPT | HSP))
 			b = __builtin_fmaf(oz, p.nxz, __builtin_fmaf(oy, p.nxy, __builtin_fmaf(ox, p.nxx, p.b0)));
-		add_point<FAM, MASK>(ec, es, th, rinv, b, s);
+		add_point<FAM, MASK>(c, sn, k, w, rinv, r2, b, s);
 	}
 }
 
@@ -258,14 +320,12 @@ template <int FAM, int MASK>
 __device__ __forceinline__ void accumulate_centroid(const PairCtx &p, float w1, float rinv_d, float k,
 	float2 (&s)[6])
 {
-	const float g = w1 * rinv_d;
-	float ec = g, es = 0.0f, th = 0.0f;
+	float c = 1.0f, sn = 0.0f;
 	if (FAM == HELM) {
-		ec = g * __builtin_amdgcn_cosf(p.t0);
-		es = -g * __builtin_amdgcn_sinf(p.t0);
-		th = k * p.df;
+		c = __builtin_amdgcn_cosf(p.t0);
+		sn = __builtin_amdgcn_sinf(p.t0);
 	}
-	add_point<FAM, MASK>(ec, es, th, rinv_d, p.b0, s);
+	add_point<FAM, MASK>(c, sn, k, w1, rinv_d, p.d2f, p.b0, s);
 }
 
 // Evaluate the kernels of MASK for the active lanes, each lane at its own
@@ -422,8 +482,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				const float ai = a.alpha.x * K[3].y + a.alpha.y * K[3].x + K[1].y;
 				const float br = K[0].x + a.alpha.x * K[2].x - a.alpha.y * K[2].y;
 				const float bi = K[0].y + a.alpha.x * K[2].y + a.alpha.y * K[2].x;
-				store_entry(a.out0, idx, a.dtype, a.accumulate, ar, ai);
-				store_entry(a.out1, idx, a.dtype, a.accumulate, br, bi);
+				store_pair(a.out0, a.out1, idx, a.dtype, a.accumulate, ar, ai, br, bi);
 			}
 		}
 	}
@@ -488,6 +547,55 @@ __device__ __forceinline__ double rsqrt64(double x)
 	return __fma_rn(0.5 * y, e, y);
 }
 
+// 1/x in FP64 (x finite and nonzero): hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rcp64(double x)
+{
+	double y = __builtin_amdgcn_rcp(x);
+	double e = __fma_rn(-x, y, 1.0);
+	y = __fma_rn(y, e, y);
+	e = __fma_rn(-x, y, 1.0);
+	return __fma_rn(y, e, y);
+}
+
+// sin and cos in FP64 for the arguments of the quadrature loops (angles and
+// k r): r = x - n pi/2 with pi/2 split in three parts and FMAs (Cody and
+// Waite's reduction, "Software Manual for the Elementary Functions", 1980;
+// the neglected tail of pi/2 is 2e-49, so the reduction stays exact to
+// rounding far beyond the |x| < 1e4 met here), then the Taylor series of sin
+// (to r^15) and cos (to r^16) on |r| <= pi/4, whose truncation is below
+// 5e-17; measured max abs error 1.5e-16 on |x| < 1e7.  NaN and infinities
+// give NaN.
+__device__ __forceinline__ void sincos64(double x, double *s, double *c)
+{
+	const double n = rint(x * 0.63661977236758138);         // 2 / pi
+	double r = __fma_rn(-n, 1.5707963267948966, x);
+	r = __fma_rn(-n, 6.123233995736766e-17, r);
+	r = __fma_rn(-n, -1.4973849048591698e-33, r);
+	const double r2 = r * r;
+	double ps = -1.0 / 1307674368000.0;                     // -1/15!
+	ps = __fma_rn(ps, r2, 1.0 / 6227020800.0);
+	ps = __fma_rn(ps, r2, -1.0 / 39916800.0);
+	ps = __fma_rn(ps, r2, 1.0 / 362880.0);
+	ps = __fma_rn(ps, r2, -1.0 / 5040.0);
+	ps = __fma_rn(ps, r2, 1.0 / 120.0);
+	ps = __fma_rn(ps, r2, -1.0 / 6.0);
+	const double sr = __fma_rn(ps * r2, r, r);
+	double pc = 1.0 / 20922789888000.0;                     // 1/16!
+	pc = __fma_rn(pc, r2, -1.0 / 87178291200.0);
+	pc = __fma_rn(pc, r2, 1.0 / 479001600.0);
+	pc = __fma_rn(pc, r2, -1.0 / 3628800.0);
+	pc = __fma_rn(pc, r2, 1.0 / 40320.0);
+	pc = __fma_rn(pc, r2, -1.0 / 720.0);
+	pc = __fma_rn(pc, r2, 1.0 / 24.0);
+	pc = __fma_rn(pc, r2, -0.5);
+	const double cr = __fma_rn(pc, r2, 1.0);
+	// x = n pi/2 + r: quadrant q = n mod 4 rotates (sin r, cos r)
+	const int q = (int)__fma_rn(-4.0, floor(n * 0.25), n);
+	const double ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+	*s = (q & 2) ? -ss : ss;
+	*c = ((q + 1) & 2) ? -cc : cc;
+}
+
 // K_helm - K_lap of the kernels of MASK from e^{-ikr} = cs - i sn, kr and the
 // point's geometry (g0 = w J / (4 pi r), rinv = 1/r, rdny, rdnx, nn)
 template <int MASK>
@@ -526,7 +634,7 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 	const double r = r2 * rinv;
 	const double kr = k * r;
 	double sn, cs;
-	sincos(kr, &sn, &cs);
+	sincos64(kr, &sn, &cs);
 	// e^{-ikr} = cs - i sn
 	const double g0 = rinv * (wj / (4.0 * PI_D));
 	double rdny = 0.0, rdnx = 0.0, nn = 0.0;
@@ -560,7 +668,7 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 	for (int a = 0; a < NC; ++a) C[a] = {d3dot(T0, V[a]), d3dot(T1, V[a]), d3dot(T2, V[a])};
 	const D3 x0 = {d3dot(T0, x0g), d3dot(T1, x0g), d3dot(T2, x0g)};
 	const D3 nx = {d3dot(T0, nxg), d3dot(T1, nxg), d3dot(T2, nxg)};
-	double th_lim[NC], c0[NC], s0[NC], dist[NC];
+	double th_lim[NC], c0[NC], s0[NC], dist[NC], dinv[NC];
 	for (int n = 0; n < NC; ++n) {
 		const int m = n == NC - 1 ? 0 : n + 1;
 		double lx = C[m].x - C[n].x, ly = C[m].y - C[n].y;
@@ -583,11 +691,13 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 			c0[n] = 1.0;
 			s0[n] = 0.0;
 		}
+		dinv[n] = 1.0 / dist[n];
 	}
 	const double z = x0.z - C[0].z;
 	const double az = fabs(z);
 	const double z2 = z * z;
 	const double sgz = (0.0 < z) - (z < 0.0);
+	const double zinv = 1.0 / z, nzz = nx.z * z;
 	double rs = 0.0, rd = 0.0, rt = 0.0, rh = 0.0;
 	for (int n = 0; n < NC; ++n) {
 		double t1 = th_lim[n], t2 = th_lim[n == NC - 1 ? 0 : n + 1];
@@ -601,8 +711,8 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 				const double xi = R.gl6x[g], w = R.gl6w[g] * jac;
 				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
 				double sth, cth;
-				sincos(th, &sth, &cth);
-				const double P = dist[n] / (cth * c0[n] + sth * s0[n]);
+				sincos64(th, &sth, &cth);
+				const double P = dist[n] * rcp64(cth * c0[n] + sth * s0[n]);
 				const double r2 = P * P + z2;
 				const double rinv = rsqrt64(r2);
 				const double r = r2 * rinv;
@@ -619,13 +729,21 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 				const double xi = R.gl21x[g], w = R.gl21w[g] * jac;
 				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
 				double sth, cth;
-				sincos(th, &sth, &cth);
-				const double Pinv = (cth * c0[n] + sth * s0[n]) / dist[n];
-				const double P = 1.0 / Pinv;
-				const double r2 = P * P + z2;
-				const double rdot = nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z);
-				// -P^2 / r^2 / z * (-r.n / r) with r.n = rdot, or -n_z / P in the plane
-				const double integrand = az > 1e-12 ? P * P * rdot * rsqrt64(r2) / (r2 * z) : -nx.z * Pinv;
+				sincos64(th, &sth, &cth);
+				const double Pinv = (cth * c0[n] + sth * s0[n]) * dinv[n];
+				// -P^2 / r^2 / z * (-r.n / r) with r^2 = P^2 + z^2 and
+				// r.n = P (n_x cos th + n_y sin th) - n_z z, written with q = 1/P as
+				// (n_x cos th + n_y sin th - n_z z q) / (z (1 + z^2 q^2)^{3/2});
+				// a side whose line holds the projected point (dist 0, P = 0) adds 0;
+				// in the plane: -n_z / P
+				double integrand;
+				if (az > 1e-12) {
+					const double rs3 = rsqrt64(__fma_rn(z2 * Pinv, Pinv, 1.0));
+					integrand = dist[n] > 0.0 ?
+						__fma_rn(-nzz, Pinv, nx.x * cth + nx.y * sth) * (rs3 * rs3 * rs3) * zinv : 0.0;
+				} else {
+					integrand = -nx.z * Pinv;
+				}
 				rh += integrand * w;
 			}
 		}
@@ -685,9 +803,9 @@ __device__ void diag_dynamic(const D3 V[3], D3 x0, double J, double k, const Rul
 		if (MASK & SLP) {
 			// -ik e^{-ik rq/2} sinc(k rq/2)
 			const double h = k * rq / 2.0;
-			const double sc = fabs(h) > 1e-3 ? sin(h) / h : 1.0 - h * h / 6.0 * (1.0 - h * h / 20.0);
 			double sh, ch;
-			sincos(h, &sh, &ch);
+			sincos64(h, &sh, &ch);
+			const double sc = fabs(h) > 1e-3 ? sh / h : 1.0 - h * h / 6.0 * (1.0 - h * h / 20.0);
 			dr += (-k * sh * sc) * Rr.d7w[q] * J;
 			di += (-k * ch * sc) * Rr.d7w[q] * J;
 		}
@@ -697,7 +815,7 @@ __device__ void diag_dynamic(const D3 V[3], D3 x0, double J, double k, const Rul
 				// (e^{-ikr}(1 + ikr) - 1 + (ikr)^2/2) / r^3
 				const double kr = k * rq;
 				double sn, cs;
-				sincos(kr, &sn, &cs);
+				sincos64(kr, &sn, &cs);
 				const double er = cs + kr * sn, ei = kr * cs - sn;
 				vr = (er - 1.0 - kr * kr / 2.0) / (rq * rq * rq);
 				vi = ei / (rq * rq * rq);

@@ -253,8 +253,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 				const float ai = a.alpha.x * K[3].y + a.alpha.y * K[3].x + K[1].y;
 				const float br = K[0].x + a.alpha.x * K[2].x - a.alpha.y * K[2].y;
 				const float bi = K[0].y + a.alpha.x * K[2].y + a.alpha.y * K[2].x;
-				store_entry(a.out0, idx, a.dtype, a.accumulate, ar, ai);
-				store_entry(a.out1, idx, a.dtype, a.accumulate, br, bi);
+				store_pair(a.out0, a.out1, idx, a.dtype, a.accumulate, ar, ai, br, bi);
 			}
 		}
 	}
@@ -305,7 +304,7 @@ __device__ __forceinline__ void kernel_point(D3 x, D3 y, D3 nx, D3 ny, double k,
 	double cs = 1.0, sn = 0.0, kr = 0.0;
 	if (FAM == HELM) {
 		kr = k * (r2 * rinv);
-		sincos(kr, &sn, &cs);
+		sincos64(kr, &sn, &cs);
 	}
 	const double g0 = rinv * (wj / (4.0 * PI_D));
 	if (MASK & SLP) { s[0].x += cs * g0; s[0].y += -sn * g0; }
@@ -392,7 +391,7 @@ __device__ double2 quad_guiggiani(const Quad &Q, double k, const Rules &R)
 			const double th = ((1.0 - R.gl5x[it]) * t1 + (1.0 + R.gl5x[it]) * t2) / 2.0;
 			const double wt = R.gl5w[it] * (t2 - t1) / 2.0;
 			double st, ct;
-			sincos(th, &st, &ct);
+			sincos64(th, &st, &ct);
 			const double vx = Ti00 * ct + Ti01 * st, vy = Ti11 * st;
 			const D3 r1 = d3add(d3scale(dxi, vx), d3scale(deta, vy));
 			const D3 r2 = d3scale(dxe, vx * vy);
@@ -508,7 +507,7 @@ __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, 
 			const double th = ((1.0 - gtx[it]) * t1 + (1.0 + gtx[it]) * t2) / 2.0;
 			const double wt = gtw[it] * (t2 - t1) / 2.0;
 			double st, ct;
-			sincos(th, &st, &ct);
+			sincos64(th, &st, &ct);
 			const double rho_lim = dist[n] / cos(th - th0[n]);
 			for (int ir = 0; ir < nr; ++ir) {
 				const double rho = (1.0 + grx[ir]) * rho_lim / 2.0;

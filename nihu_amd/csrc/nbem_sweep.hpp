@@ -73,8 +73,10 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 			// wavenumber-independent geometry of the points: the phase
 			// correction and, per kernel sum, the weight W that multiplies
 			// e^{-i th} (g = w/r, G-type sums carry 1/r^2, Q = G (1 + i k r)
-			// splits into a G sum and a k r G sum, b = r.n_x, w2 = b / r^2)
-			float gdel[Q], W[8][Q];
+			// splits into a G sum and a k r G sum, b = r.n_x, w2 = b / r^2;
+			// the E w2 sum of the hypersingular k^2 term has the weight
+			// g w2 = w b / r^3 of the G b sum and reuses it)
+			float gdel[Q], W[7][Q];
 			const auto qf = uni((const float *)(a.qt + (size_t)j * Q));
 #pragma unroll
 			for (int q = 0; q < Q; ++q) {
@@ -95,7 +97,6 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 				W[4][q] = W[3][q] * r;    // sum r G b
 				W[5][q] = gi2 * w2;       // sum G w2
 				W[6][q] = W[5][q] * r;    // sum r G w2
-				W[7][q] = g * w2;         // sum E w2
 			}
 			for (int fb = 0; fb < KB; ++fb) {
 				const int f = f0 + fb;
@@ -105,15 +106,15 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 				const float t0 = (float)td;
 				const float k = kf[f], kr = krevf[f];
 				// A[m] = sum_q W[m][q] (cos t_q, sin t_q); e^{-i th} = (cos, -sin)
-				float2 A[8];
+				float2 A[7];
 #pragma unroll
-				for (int m = 0; m < 8; ++m) A[m] = make_float2(0.f, 0.f);
+				for (int m = 0; m < 7; ++m) A[m] = make_float2(0.f, 0.f);
 #pragma unroll
 				for (int q = 0; q < Q; ++q) {
 					const float t = __builtin_fmaf(kr, gdel[q], t0);
 					const float c = __builtin_amdgcn_cosf(t), sn = __builtin_amdgcn_sinf(t);
 #pragma unroll
-					for (int m = 0; m < 8; ++m) {
+					for (int m = 0; m < 7; ++m) {
 						A[m].x = __builtin_fmaf(W[m][q], c, A[m].x);
 						A[m].y = __builtin_fmaf(W[m][q], sn, A[m].y);
 					}
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 				const float2 s2 = make_float2(A[3].x + k * A[4].y, -A[3].y + k * A[4].x);
 				const float2 sqw = make_float2(A[5].x + k * A[6].y, -A[5].y + k * A[6].x);
 				const float kk = k * k;
-				const float2 s3 = make_float2(3.0f * sqw.x - kk * A[7].x, 3.0f * sqw.y + kk * A[7].y);
+				const float2 s3 = make_float2(3.0f * sqw.x - kk * A[3].x, 3.0f * sqw.y + kk * A[3].y);
 				if (act) {
 					// SLP = s0, DLP = -a s1, DLPt = s2, HSP = nn s1 - a s3
 					const float2 al = make_float2(alpha[2 * f], alpha[2 * f + 1]);
@@ -276,8 +277,8 @@ __global__ void __launch_bounds__(64) fp64_dynamic_rec_kernel(DynArgs a, double 
 		const double rdny = d3dot(rv, ny) * rinv;
 		const double rdnx = -d3dot(rv, nx) * rinv;
 		double sn, cs, sd, cd;
-		sincos(kf[0] * r, &sn, &cs);
-		sincos(dk * r, &sd, &cd);
+		sincos64(kf[0] * r, &sn, &cs);
+		sincos64(dk * r, &sd, &cd);
 #pragma unroll
 		for (int m = 0; m < KBLK; ++m) {
 			dynamic_terms<ALL4>(cs, sn, kf[m] * r, g0, rinv, rdny, rdnx, nn, s[m]);
