@@ -18,7 +18,7 @@ with OpenMP over rows and is timed as the CPU baseline (beside it the serial
 pair loop on a row sample, and the host's CPU model); the solved surface field
 against the oracle's, and the total field against the Mie series.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 20]
 """
 
 from __future__ import annotations
@@ -123,7 +123,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps first; kernel durations settle over the first few launches of a process")
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--ka", type=float, default=10.0)
     ap.add_argument("--policy", choices=["config", "nihu"], default="config")
