@@ -206,6 +206,13 @@ int device_rules(const nbk::Rules **out)
 		nbg_gauss_legendre(16, r.gl16x, r.gl16w);
 		nbg_gauss_legendre(51, r.gl51x, r.gl51w);
 		if (nbg_quad_duffy(4, 0.0, 0.0, &r.duffy[0][0]) != 64) return fail(NB_ERR_RUNTIME, "Duffy rule");
+		for (int d = 0; d < nbk::NSLOT; ++d) {
+			double xi[64], eta[64], w[64];
+			const int q = d <= 14 ? nbg_dunavant(d, xi, eta, w, 64) : 0;
+			if (q > nbk::REF_MAXQ) return fail(NB_ERR_RUNTIME, "Dunavant reference table");
+			for (int p = 0; p < nbk::REF_MAXQ; ++p)
+				r.ref[d][p] = p < q ? make_float2((float)xi[p], (float)eta[p]) : make_float2(0.0f, 0.0f);
+		}
 		DevPtr p;
 		int s = dev_alloc(p, sizeof(r));
 		if (s) return s;
@@ -541,8 +548,11 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	if (jb.same && (s = ensure_sing_pairs(trial))) return s;
 	if (nodal && !accumulate && (s = zero())) return s;
 
+	const nbk::Rules *rules = nullptr;
+	if ((s = device_rules(&rules))) return s;
 	nbk::GalArgs a;
 	std::memset(&a, 0, sizeof(a));
+	a.ref = &rules->ref[0][0];     // device address (no host access)
 	if (nodal) {
 		a.n_tiles = test->g_tiles;
 		a.chunk = trial->g_chunk;

@@ -55,6 +55,7 @@ struct GalArgs {
 	const double *thr;           // trial distance thresholds [NTHR][sn]
 	const QPoint *tq[NSLOT], *sq[NSLOT];
 	int qn[NSLOT];
+	const float2 *ref;           // [NSLOT][REF_MAXQ] reference points (xi, eta) of each degree's rule
 	int fixed_degree;            // >= 0: fixed rule
 	float k, k_revf;             // wavenumber, k / (2 pi)
 	double k_rev;
@@ -126,12 +127,20 @@ __device__ __forceinline__ void atomic_add_entry64(void *out, long long idx, int
 // Kernel values: SLP E, DLP -Q r.n_y, DLPt Q r.n_x,
 // HSP n_x.n_y Q - (3Q - th^2 G) r.n_x r.n_y / r^2 with E = e^{-i th}/r,
 // G = E/r^2, Q = G (1 + i th), th = k r; the 1/(4 pi) rides on the trial weights.
+// G comes first and th G = k e^{-i th}/r^2 (no r or th needed); th^2 G = k^2 E.
+// DLP's -r.n_y does not depend on q and multiplies the test weight instead.
+// Nodal fields: the table rows carry w_q xi_q and w_q eta_q, so the sums are
+// T = sum K w (N_0 + N_1 + N_2 = 1), S_1 and S_2, and blk[a][0] receives the
+// T sums; the caller turns that column into N_0's, T - S_1 - S_2, after the
+// pair.  ref: the rule's reference points (xi_p, eta_p) for the test weights.
 template <int FAM, int KM, int NDOF, int NQ, class TP>
-__device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, int nq_rt, const GalPair &P, float k_revf,
-	float k, float2 (&blk)[NDOF][NDOF])
+__device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const float2 *ref, int nq_rt, const GalPair &P,
+	float k_revf, float k, float2 (&blk)[NDOF][NDOF])
 {
 	const int nq = NQ > 0 ? NQ : nq_rt;
 	const auto sf = uni((const float *)sq_in);
+	const auto rf = uni((const float *)ref);
+	constexpr bool IM = FAM == HELM;
 	constexpr int UNROLL = NQ > 0 ? NQ : 1;
 #pragma unroll UNROLL
 	for (int p = 0; p < nq; ++p) {
@@ -152,65 +161,70 @@ __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, int nq_rt
 			const float u = __builtin_fmaf(ex, ox, __builtin_fmaf(ey, oy, __builtin_fmaf(ez, oz, oo + vp)));
 			const float r2 = P.d2f + u;
 			const float rinv = __builtin_amdgcn_rsqf(r2);
-			float er, ei, th = 0.0f;
-			if (FAM == HELM) {
-				const float r = r2 * rinv;
-				const float tt = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(r + P.df), P.t0);
-				er = rinv * __builtin_amdgcn_cosf(tt);
-				ei = -rinv * __builtin_amdgcn_sinf(tt);
-				th = k * r;
-			} else {
-				er = rinv;
-				ei = 0.0f;
+			float c = 1.0f, sn = 0.0f;
+			if (IM) {
+				const float tt = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(__builtin_fmaf(r2, rinv, P.df)), P.t0);
+				c = __builtin_amdgcn_cosf(tt);
+				sn = __builtin_amdgcn_sinf(tt);
 			}
-			float kr, ki;
+			float kr, ki = 0.0f;
 			if (KM == SLP) {
-				kr = er; ki = ei;
+				kr = rinv * c;
+				if (IM) ki = -rinv * sn;
 			} else {
 				const float ri2 = rinv * rinv;
-				const float Gr = er * ri2, Gi = ei * ri2;
-				const float Qr = __builtin_fmaf(-th, Gi, Gr);
-				const float Qi = __builtin_fmaf(th, Gr, Gi);
+				const float g3 = rinv * ri2;
+				const float Gr = g3 * c, Gi = IM ? -g3 * sn : 0.0f;
+				const float kh = k * ri2;
+				// Q = G (1 + i th): Re = Gr + k sn / r^2, Im = Gi + k c / r^2
+				const float Qr = IM ? __builtin_fmaf(kh, sn, Gr) : Gr;
+				const float Qi = IM ? __builtin_fmaf(kh, c, Gi) : 0.0f;
 				if (KM == DLP) {
-					kr = -Qr * rny; ki = -Qi * rny;
+					kr = Qr; ki = Qi;
 				} else {
 					const float rnx = P.dcnx + __builtin_fmaf(ox, P.nxx, __builtin_fmaf(oy, P.nxy, oz * P.nxz));
 					if (KM == DLPT) {
 						kr = Qr * rnx; ki = Qi * rnx;
 					} else {
-						const float c = rnx * rny * ri2;
-						const float th2 = th * th;
-						const float hr = __builtin_fmaf(3.0f, Qr, -th2 * Gr);
-						const float hi = __builtin_fmaf(3.0f, Qi, -th2 * Gi);
-						kr = __builtin_fmaf(P.nn, Qr, -hr * c);
-						ki = __builtin_fmaf(P.nn, Qi, -hi * c);
+						const float cf = rnx * rny * ri2;
+						// 3 Q - th^2 G with th^2 G = k^2 E = k^2 e^{-i th} / r
+						const float kk = k * k * rinv;
+						const float hr = __builtin_fmaf(3.0f, Qr, -kk * c);
+						const float hi = __builtin_fmaf(3.0f, Qi, kk * sn);
+						kr = __builtin_fmaf(P.nn, Qr, -hr * cf);
+						ki = __builtin_fmaf(P.nn, Qi, -hi * cf);
 					}
 				}
 			}
 			if (NDOF == 1) {
 				S[0].x = __builtin_fmaf(kr, w, S[0].x);
-				S[0].y = __builtin_fmaf(ki, w, S[0].y);
+				if (IM) S[0].y = __builtin_fmaf(ki, w, S[0].y);
 			} else {
-				const float w1 = w * sf[8 * q + 5], w2 = w * sf[8 * q + 6];
-				const float w0 = w * (1.0f - sf[8 * q + 5] - sf[8 * q + 6]);
-				S[0].x = __builtin_fmaf(kr, w0, S[0].x); S[0].y = __builtin_fmaf(ki, w0, S[0].y);
-				S[1].x = __builtin_fmaf(kr, w1, S[1].x); S[1].y = __builtin_fmaf(ki, w1, S[1].y);
-				S[2].x = __builtin_fmaf(kr, w2, S[2].x); S[2].y = __builtin_fmaf(ki, w2, S[2].y);
+				const float w1 = sf[8 * q + 5], w2 = sf[8 * q + 6];
+				S[0].x = __builtin_fmaf(kr, w, S[0].x);
+				S[1].x = __builtin_fmaf(kr, w1, S[1].x);
+				S[2].x = __builtin_fmaf(kr, w2, S[2].x);
+				if (IM) {
+					S[0].y = __builtin_fmaf(ki, w, S[0].y);
+					S[1].y = __builtin_fmaf(ki, w1, S[1].y);
+					S[2].y = __builtin_fmaf(ki, w2, S[2].y);
+				}
 			}
 		}
+		const float wt = KM == DLP ? -rny * t.wj : t.wj;
 		if (NDOF == 1) {
-			blk[0][0].x = __builtin_fmaf(t.wj, S[0].x, blk[0][0].x);
-			blk[0][0].y = __builtin_fmaf(t.wj, S[0].y, blk[0][0].y);
+			blk[0][0].x = __builtin_fmaf(wt, S[0].x, blk[0][0].x);
+			if (IM) blk[0][0].y = __builtin_fmaf(wt, S[0].y, blk[0][0].y);
 		} else {
 			// the test element carries the same rule: N_a at the p-th reference point
-			const float xi = sf[8 * p + 5], eta = sf[8 * p + 6];
-			const float wa[3] = {t.wj * (1.0f - xi - eta), t.wj * xi, t.wj * eta};
+			const float xi = rf[2 * p], eta = rf[2 * p + 1];
+			const float wa[3] = {wt * (1.0f - xi - eta), wt * xi, wt * eta};
 #pragma unroll
 			for (int a = 0; a < 3; ++a)
 #pragma unroll
 				for (int b = 0; b < 3; ++b) {
 					blk[a][b].x = __builtin_fmaf(wa[a], S[b].x, blk[a][b].x);
-					blk[a][b].y = __builtin_fmaf(wa[a], S[b].y, blk[a][b].y);
+					if (IM) blk[a][b].y = __builtin_fmaf(wa[a], S[b].y, blk[a][b].y);
 				}
 		}
 	}
@@ -235,7 +249,7 @@ __device__ __forceinline__ void gal_eval_dynamic(const GalArgs &a, int e, int f,
 			const float *tq = (const float *)(a.tq[dsel] + (size_t)e * nq);
 			gal_points<FAM, KM, NDOF, 0>(
 				[&](int p) { return GTest{tq[8 * p], tq[8 * p + 1], tq[8 * p + 2], tq[8 * p + 4], tq[8 * p + 7]}; },
-				a.sq[dsel] + (size_t)f * nq, nq, P, a.k_revf, a.k, blk);
+				a.sq[dsel] + (size_t)f * nq, a.ref + dsel * REF_MAXQ, nq, P, a.k_revf, a.k, blk);
 			deg = -1;
 		}
 	}
@@ -368,7 +382,7 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 			if (FIXQ > 0) {
 				if (act)
 					gal_points<FAM, KM, NDOF, FIXQ>([&](int p) { return tpr[p]; }, a.sq[a.fixed_degree] + (size_t)f * FIXQ,
-						FIXQ, P, a.k_revf, a.k, blk);
+						a.ref + a.fixed_degree * REF_MAXQ, FIXQ, P, a.k_revf, a.k, blk);
 			} else {
 				gal_eval_dynamic<FAM, KM, NDOF>(a, ee, f, d2, act, P, blk);
 			}
@@ -381,6 +395,12 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 				store_entry(a.out, (long long)f * a.ld + e, a.dtype, a.accumulate, re, im);
 			}
 		} else if (act) {
+			// column 0 holds the sums with the trial weight w (T); N_0's is T - S_1 - S_2
+#pragma unroll
+			for (int x = 0; x < 3; ++x) {
+				blk[x][0].x -= blk[x][1].x + blk[x][2].x;
+				blk[x][0].y -= blk[x][1].y + blk[x][2].y;
+			}
 			const int cl[3] = {sl0, sl1, sl2};
 #pragma unroll
 			for (int x = 0; x < 3; ++x)

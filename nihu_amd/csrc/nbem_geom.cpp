@@ -225,7 +225,8 @@ double nbg_threshold(double scale, double limit)
 
 // Quadrature point table of one Dunavant degree for all elements:
 // out[(e * Q + q) * 8 + {0..7}] = (y_q - c_e) xyz, w_q * J_e / (4 pi), |y_q - c_e|^2,
-// xi_q, eta_q, w_q * J_e (floats rounded from FP64).
+// w_q * J_e / (4 pi) * xi_q, w_q * J_e / (4 pi) * eta_q, w_q * J_e (floats
+// rounded from FP64).
 int nbg_quad_table(int degree, int n, const double *c, const double *J, const double *X, float *out)
 {
 	double xi[64], eta[64], w[64];
@@ -243,8 +244,8 @@ int nbg_quad_table(int degree, int n, const double *c, const double *J, const do
 			p[0] = (float)o.x; p[1] = (float)o.y; p[2] = (float)o.z;
 			p[3] = (float)(w[q] * J[e] / four_pi);
 			p[4] = (float)(o.x * o.x + o.y * o.y + o.z * o.z);
-			p[5] = (float)xi[q];
-			p[6] = (float)eta[q];
+			p[5] = (float)(w[q] * J[e] / four_pi * xi[q]);
+			p[6] = (float)(w[q] * J[e] / four_pi * eta[q]);
 			p[7] = (float)(w[q] * J[e]);
 		}
 	}

@@ -32,9 +32,12 @@ enum { THR_NEAR = 0, THR_1_0, THR_1_1, THR_1_4, THR_1_5, THR_1_8, THR_1_9, THR_2
 	THR_15_8, THR_22_3, NTHR };
 
 constexpr int NSLOT = 16;   // quadrature tables indexed by Dunavant degree 0..14 (+1 spare)
+constexpr int REF_MAXQ = 48;   // points of the largest Dunavant rule used (degree 14: 42), rounded up
 
-// offset from the element centre, w J / (4 pi), |offset|^2, reference point, w J
-struct QPoint { float ox, oy, oz, w, oo, xi, eta, wj; };
+// offset from the element centre, w J / (4 pi), |offset|^2, the same weight
+// times the reference coordinates xi and eta (the linear shape functions N_1,
+// N_2 of the Galerkin pass), w J
+struct QPoint { float ox, oy, oz, w, oo, wxi, weta, wj; };
 
 // Per trial element, read as one 160-byte block through the scalar cache by
 // the regular pass: distance thresholds, centre, unit normal and the weight
@@ -500,6 +503,9 @@ struct Rules {
 	// order-7 Duffy blind rule around the centre of [-1, 1]^2 (xi, eta, w)
 	double gl3x[3], gl3w[3], gl5x[5], gl5w[5], gl8x[8], gl8w[8], gl16x[16], gl16w[16], gl51x[51], gl51w[51];
 	double duffy[64][3];
+	// reference points (xi, eta) of the Dunavant rule of each degree, FP32
+	// (the test-side shape functions of the regular Galerkin pass)
+	float2 ref[NSLOT][REF_MAXQ];
 };
 
 struct NearArgs {
