@@ -432,7 +432,8 @@ __device__ __forceinline__ double2 kernel_value64(D3 x, D3 y, D3 nx, D3 ny, doub
 {
 	const D3 rv = d3sub(y, x);
 	const double r2 = d3dot(rv, rv);
-	const double r = sqrt(r2);
+	const double rinv = rsqrt64(r2);
+	const double r = r2 * rinv;
 	double sn = 0.0, cs = 1.0, kr = 0.0;
 	if (FAM == HELM) {
 		kr = k * r;
@@ -440,18 +441,19 @@ __device__ __forceinline__ double2 kernel_value64(D3 x, D3 y, D3 nx, D3 ny, doub
 	}
 	const double f = 1.0 / (4.0 * PI_D);
 	// e^{-ikr} = cs - i sn
-	if (KM == SLP) return make_double2(cs / r * f, -sn / r * f);
-	const double rdny = d3dot(rv, ny) / r, rdnx = -d3dot(rv, nx) / r;
+	const double g1 = f * rinv;
+	if (KM == SLP) return make_double2(cs * g1, -sn * g1);
+	const double rdny = d3dot(rv, ny) * rinv, rdnx = -d3dot(rv, nx) * rinv;
+	const double g3 = g1 * rinv * rinv;
 	if (KM == DLP || KM == DLPT) {
 		// e (-1 - ikr) / (4 pi r^2) (r.n_y / r  or  -r.n_x / r)
-		const double g = f / r2 * (KM == DLP ? rdny : rdnx);
+		const double g = g3 * r * (KM == DLP ? rdny : rdnx);
 		return make_double2((-cs - kr * sn) * g, (sn - kr * cs) * g);
 	}
 	// e / (4 pi r^3) [(3 + 3ikr - (kr)^2) rdnx rdny + (1 + ikr) n_x.n_y]
 	const double cc = rdnx * rdny, nn = d3dot(nx, ny);
 	const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
-	const double g = f / (r2 * r);
-	return make_double2((cs * ar + sn * ai) * g, (cs * ai - sn * ar) * g);
+	return make_double2((cs * ar + sn * ai) * g3, (cs * ai - sn * ar) * g3);
 }
 
 struct GalSingArgs {

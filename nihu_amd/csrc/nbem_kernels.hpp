@@ -642,7 +642,7 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 	double sn, cs;
 	sincos64(kr, &sn, &cs);
 	// e^{-ikr} = cs - i sn
-	const double g0 = rinv * (wj / (4.0 * PI_D));
+	const double g0 = rinv * wj * (1.0 / (4.0 * PI_D));
 	double rdny = 0.0, rdnx = 0.0, nn = 0.0;
 	if (MASK & (DLP | DLPT | HSP)) {
 		rdny = d3dot(rv, ny) * rinv;

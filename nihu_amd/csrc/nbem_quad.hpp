@@ -306,7 +306,7 @@ __device__ __forceinline__ void kernel_point(D3 x, D3 y, D3 nx, D3 ny, double k,
 		kr = k * (r2 * rinv);
 		sincos64(kr, &sn, &cs);
 	}
-	const double g0 = rinv * (wj / (4.0 * PI_D));
+	const double g0 = rinv * wj * (1.0 / (4.0 * PI_D));
 	if (MASK & SLP) { s[0].x += cs * g0; s[0].y += -sn * g0; }
 	if (MASK & (DLP | DLPT | HSP)) {
 		const double rdny = d3dot(rv, ny) * rinv;
@@ -334,8 +334,9 @@ __device__ __forceinline__ void quad_kernel_at(const Quad &Q, double xi, double 
 	D3 a, b;
 	quad_dx(Q, xi, eta, a, b);
 	const D3 n = d3cross(a, b);
-	const double J = d3norm(n);
-	kernel_point<FAM, MASK>(x0, quad_x(Q, xi, eta), nx0, d3scale(n, 1.0 / J), k, J * w, s);
+	const double n2 = d3dot(n, n);
+	const double Jinv = rsqrt64(n2);
+	kernel_point<FAM, MASK>(x0, quad_x(Q, xi, eta), nx0, d3scale(n, Jinv), k, n2 * Jinv * w, s);
 }
 
 // Singular SLP / DLP / DLPt: the order-7 Duffy rule around the centre

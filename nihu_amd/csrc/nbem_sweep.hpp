@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(64) fp64_dynamic_rec_kernel(DynArgs a, double 
 		const double r2 = d3dot(rv, rv);
 		const double rinv = rsqrt64(r2);
 		const double r = r2 * rinv;
-		const double g0 = rinv * ((R.d14w[q] * J) / (4.0 * PI_D));
+		const double g0 = rinv * (R.d14w[q] * J) * (1.0 / (4.0 * PI_D));
 		const double rdny = d3dot(rv, ny) * rinv;
 		const double rdnx = -d3dot(rv, nx) * rinv;
 		double sn, cs, sd, cd;
