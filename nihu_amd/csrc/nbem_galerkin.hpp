@@ -154,7 +154,7 @@ __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const flo
 			rny = P.dcny - __builtin_fmaf(t.ax, P.nyx, __builtin_fmaf(t.ay, P.nyy, t.az * P.nyz));
 		float2 S[NDOF];
 #pragma unroll
-		for (int b = 0; b < NDOF; ++b) S[b] = make_float2(0.0f, 0.0f);
+		for (int b = 0; b < NDOF; ++b) S[b] = zero2();
 #pragma unroll UNROLL
 		for (int q = 0; q < nq; ++q) {
 			const float ox = sf[8 * q], oy = sf[8 * q + 1], oz = sf[8 * q + 2], w = sf[8 * q + 3], oo = sf[8 * q + 4];
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 #pragma unroll
 		for (int x = 0; x < NDOF; ++x)
 #pragma unroll
-			for (int y = 0; y < NDOF; ++y) blk[x][y] = make_float2(0.0f, 0.0f);
+			for (int y = 0; y < NDOF; ++y) blk[x][y] = zero2();
 		if (__ballot(act) != 0) {
 			const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);
 			P.ex0 = 2.0f * dcx; P.ey0 = 2.0f * dcy; P.ez0 = 2.0f * dcz;

@@ -74,6 +74,11 @@ struct RegArgs {
 
 // ---------------------------------------------------------------- helpers
 
+// Empty sum for FP32 accumulators: -0 is the additive identity (x + -0 = x for
+// every x, while 0 + -0 = +0), so the compiler may drop the first addition of
+// each sum (fma(a, b, -0) is a multiplication); the sums are otherwise the same.
+__device__ __forceinline__ float2 zero2() { return make_float2(-0.0f, -0.0f); }
+
 // View of read-only global memory in the constant address space: loads from a
 // wave-uniform address through it become scalar loads (SGPRs, scalar cache).
 template <class T>
@@ -425,8 +430,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			if (KM & HSP) nn = p.nxx * nyx + p.nxy * nyy + p.nxz * nyz;
 		}
 		if (KM & (DLPT | HSP)) p.b0 = dcx * p.nxx + dcy * p.nxy + dcz * p.nxz;
-		float2 s[6] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
-			make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+		float2 s[6] = {zero2(), zero2(), zero2(), zero2(), zero2(), zero2()};
 		// hsp_apart: the hypersingular sum of Q went to s[4] (its own degree)
 		bool hsp_apart = false;
 		if (FIXQ > 0) {

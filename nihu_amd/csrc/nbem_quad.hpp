@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 		} else {
 			p.t0 = 0.0f;
 		}
-		float2 K[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+		float2 K[4] = {zero2(), zero2(), zero2(), zero2()};
 		if (a.fixed_n > 0) {
 			quad_orders<FAM, KM>(a, j, act ? a.fixed_n : 0, p, K);
 		} else {

@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 				// A[m] = sum_q W[m][q] (cos t_q, sin t_q); e^{-i th} = (cos, -sin)
 				float2 A[7];
 #pragma unroll
-				for (int m = 0; m < 7; ++m) A[m] = make_float2(0.f, 0.f);
+				for (int m = 0; m < 7; ++m) A[m] = zero2();
 #pragma unroll
 				for (int q = 0; q < Q; ++q) {
 					const float t = __builtin_fmaf(kr, gdel[q], t0);
