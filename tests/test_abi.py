@@ -115,3 +115,22 @@ def test_threshold_reproduces_sqrt_division_test(raw):
 def test_bad_mesh_rejected_without_gpu(nb):
     with pytest.raises(ValueError):
         nb.Mesh(np.zeros((3, 3)), np.array([[0, 1, 5]]))
+
+
+def test_empty_and_degenerate_meshes_rejected_without_gpu(nb):
+    """Empty element lists, a zero-area triangle, a degenerate quadrilateral and
+    a node index past the end are refused by nb_mesh_create before any device
+    work (nihu's mesh would accept them and fail later, or divide by zero)."""
+    nodes = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [1.0, 1.0, 0.0]])
+    with pytest.raises(ValueError, match="empty mesh"):
+        nb.Mesh(nodes, np.zeros((0, 3), dtype=np.int32))
+    with pytest.raises(ValueError, match="empty mesh"):
+        nb.Mesh(nodes, np.zeros((0, 4), dtype=np.int32))
+    with pytest.raises(ValueError):
+        nb.Mesh(nodes, np.array([[0, 1, 1]]))            # zero area
+    with pytest.raises(ValueError):
+        nb.Mesh(nodes, np.array([[0, 1, 1, 0]]))         # collapsed quadrilateral
+    with pytest.raises(ValueError):
+        nb.Mesh(nodes, np.array([[0, 1, -1]]))           # negative node index
+    with pytest.raises(ValueError):
+        nb.Mesh(nodes, np.ones((2, 5), dtype=np.int32))   # neither triangles nor quadrilaterals
