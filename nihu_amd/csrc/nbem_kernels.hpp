@@ -442,30 +442,36 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			// (all far pairs: one point each) all kernels share one pass
 			const float w1 = rec[j].w1;
 			const int fc = a.field_complexity;
-			// beyond the last breakpoint (22.3 R, the largest of all tables)
-			// every degree is the table's 0; a wave of such pairs skips the selects
-			int dS = (KM & SLP) ? fc : -1, dD = (KM & (DLP | DLPT)) ? fc : -1, dH = (KM & HSP) ? fc : -1;
-			if (__ballot(act && d2 < t[THR_22_3]) != 0) {
-				if (KM & SLP) dS = fc + table_degree_rec<1>(d2, t);
-				if (KM & (DLP | DLPT)) dD = fc + table_degree_rec<2>(d2, t);
-				if (KM & HSP) dH = fc + table_degree_rec<3>(d2, t);
+			if (__ballot(act && d2 < t[THR_22_3]) == 0) {
+				// beyond the last breakpoint (22.3 R, the largest of all tables)
+				// every table degree is 0: the whole wave takes degree fc for all
+				// kernels in one pass (degrees 0 and 1: the centroid rule)
+				if (fc <= 1) {
+					if (act) accumulate_centroid<FAM, KM>(p, w1, rinv_d, a.k, s);
+				} else {
+					eval_degrees<FAM, KM>(a, j, act ? fc : -1, p, w1, rinv_d, s);
+				}
+			} else {
+				const int dS = (KM & SLP) ? fc + table_degree_rec<1>(d2, t) : -1;
+				const int dD = (KM & (DLP | DLPT)) ? fc + table_degree_rec<2>(d2, t) : -1;
+				const int dH = (KM & HSP) ? fc + table_degree_rec<3>(d2, t) : -1;
+				constexpr int NG = ((KM & SLP) ? 1 : 0) + ((KM & (DLP | DLPT)) ? 1 : 0) + ((KM & HSP) ? 1 : 0);
+				bool joint = false;
+				if (NG > 1) {
+					const int d0 = (KM & SLP) ? dS : (KM & (DLP | DLPT)) ? dD : dH;
+					joint = act && (!(KM & SLP) || dS == d0) && (!(KM & (DLP | DLPT)) || dD == d0) &&
+						(!(KM & HSP) || dH == d0);
+					eval_degrees<FAM, KM>(a, j, joint ? d0 : -1, p, w1, rinv_d, s);
+				}
+				const bool sep = act && !joint;
+				if (__ballot(sep) != 0) {
+					if (KM & SLP) eval_degrees<FAM, KM & SLP>(a, j, sep ? dS : -1, p, w1, rinv_d, s);
+					if (KM & (DLP | DLPT)) eval_degrees<FAM, KM & (DLP | DLPT)>(a, j, sep ? dD : -1, p, w1, rinv_d, s);
+					if (KM & HSP) eval_degrees<FAM, KM & HSP>(a, j, sep ? dH : -1, p, w1, rinv_d, s);
+				}
+				// the hypersingular sum of Q went to s[4] unless all kernels shared a pass
+				hsp_apart = !joint;
 			}
-			constexpr int NG = ((KM & SLP) ? 1 : 0) + ((KM & (DLP | DLPT)) ? 1 : 0) + ((KM & HSP) ? 1 : 0);
-			bool joint = false;
-			if (NG > 1) {
-				const int d0 = (KM & SLP) ? dS : (KM & (DLP | DLPT)) ? dD : dH;
-				joint = act && (!(KM & SLP) || dS == d0) && (!(KM & (DLP | DLPT)) || dD == d0) &&
-					(!(KM & HSP) || dH == d0);
-				eval_degrees<FAM, KM>(a, j, joint ? d0 : -1, p, w1, rinv_d, s);
-			}
-			const bool sep = act && !joint;
-			if (__ballot(sep) != 0) {
-				if (KM & SLP) eval_degrees<FAM, KM & SLP>(a, j, sep ? dS : -1, p, w1, rinv_d, s);
-				if (KM & (DLP | DLPT)) eval_degrees<FAM, KM & (DLP | DLPT)>(a, j, sep ? dD : -1, p, w1, rinv_d, s);
-				if (KM & HSP) eval_degrees<FAM, KM & HSP>(a, j, sep ? dH : -1, p, w1, rinv_d, s);
-			}
-			// the hypersingular sum of Q went to s[4] unless all kernels shared a pass
-			hsp_apart = !joint;
 		}
 		if (act) {
 			// kernel values from the accumulators
