@@ -581,29 +581,30 @@ __device__ __forceinline__ double rcp64(double x)
 // (to r^15) and cos (to r^16) on |r| <= pi/4, whose truncation is below
 // 5e-17; measured max abs error 1.5e-16 on |x| < 1e7.  NaN and infinities
 // give NaN.
+// Taylor coefficients of sin (-1/15! ... -1/3!) and cos (1/16! ... -1/2!), read
+// through the scalar cache: held in SGPRs, each Horner step is one v_fma_f64
+// (as VGPR-resident literals every step costs a copy plus a v_fmac_f64)
+__constant__ double SINCOS_TAYLOR[15] = {
+	-1.0 / 1307674368000.0, 1.0 / 6227020800.0, -1.0 / 39916800.0, 1.0 / 362880.0, -1.0 / 5040.0, 1.0 / 120.0,
+	-1.0 / 6.0,
+	1.0 / 20922789888000.0, -1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0,
+	-1.0 / 720.0, 1.0 / 24.0, -0.5};
+
 __device__ __forceinline__ void sincos64(double x, double *s, double *c)
 {
+	const auto C = uni((const double *)SINCOS_TAYLOR);
 	const double n = rint(x * 0.63661977236758138);         // 2 / pi
 	double r = __fma_rn(-n, 1.5707963267948966, x);
 	r = __fma_rn(-n, 6.123233995736766e-17, r);
 	r = __fma_rn(-n, -1.4973849048591698e-33, r);
 	const double r2 = r * r;
-	double ps = -1.0 / 1307674368000.0;                     // -1/15!
-	ps = __fma_rn(ps, r2, 1.0 / 6227020800.0);
-	ps = __fma_rn(ps, r2, -1.0 / 39916800.0);
-	ps = __fma_rn(ps, r2, 1.0 / 362880.0);
-	ps = __fma_rn(ps, r2, -1.0 / 5040.0);
-	ps = __fma_rn(ps, r2, 1.0 / 120.0);
-	ps = __fma_rn(ps, r2, -1.0 / 6.0);
+	double ps = C[0];
+#pragma unroll
+	for (int i = 1; i < 7; ++i) ps = __fma_rn(ps, r2, C[i]);
 	const double sr = __fma_rn(ps * r2, r, r);
-	double pc = 1.0 / 20922789888000.0;                     // 1/16!
-	pc = __fma_rn(pc, r2, -1.0 / 87178291200.0);
-	pc = __fma_rn(pc, r2, 1.0 / 479001600.0);
-	pc = __fma_rn(pc, r2, -1.0 / 3628800.0);
-	pc = __fma_rn(pc, r2, 1.0 / 40320.0);
-	pc = __fma_rn(pc, r2, -1.0 / 720.0);
-	pc = __fma_rn(pc, r2, 1.0 / 24.0);
-	pc = __fma_rn(pc, r2, -0.5);
+	double pc = C[7];
+#pragma unroll
+	for (int i = 8; i < 15; ++i) pc = __fma_rn(pc, r2, C[i]);
 	const double cr = __fma_rn(pc, r2, 1.0);
 	// x = n pi/2 + r: quadrant q = n mod 4 rotates (sin r, cos r)
 	const int q = (int)__fma_rn(-4.0, floor(n * 0.25), n);
