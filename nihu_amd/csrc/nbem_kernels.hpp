@@ -398,14 +398,19 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
 	const auto rec = uni(a.rec);
+	// the trial record fields of the one-point path, loaded one column ahead
+	struct Head { double cx, cy, cz, near, far; float nx, ny, nz, w1; };
+	auto head = [&](int jj) {
+		return Head{rec[jj].c[0], rec[jj].c[1], rec[jj].c[2], rec[jj].thr[THR_NEAR], rec[jj].thr[THR_22_3],
+			rec[jj].nu[0], rec[jj].nu[1], rec[jj].nu[2], rec[jj].w1};
+	};
+	Head nxt = head(j0);
 	for (int j = j0; j < j1; ++j) {
-		// the whole trial record in SGPRs (a few wide scalar loads)
-		double t[NTHR];
-#pragma unroll
-		for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
-		const double cjx = rec[j].c[0], cjy = rec[j].c[1], cjz = rec[j].c[2];
+		const Head h = nxt;
+		const double cjx = h.cx, cjy = h.cy, cjz = h.cz;
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
-		const bool act = row_ok && !(d2 < t[THR_NEAR]) && !(a.same_mesh && i == j);
+		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == j);
+		nxt = head(j + 1 < j1 ? j + 1 : j);
 		if (__ballot(act) == 0) continue;
 		const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);
 		p.dcx2 = 2.0f * dcx; p.dcy2 = 2.0f * dcy; p.dcz2 = 2.0f * dcz;
@@ -425,7 +430,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 		}
 		float av = 0.0f, nn = 0.0f;
 		if (KM & (DLP | HSP)) {
-			const float nyx = rec[j].nu[0], nyy = rec[j].nu[1], nyz = rec[j].nu[2];
+			const float nyx = h.nx, nyy = h.ny, nyz = h.nz;
 			av = dcx * nyx + dcy * nyy + dcz * nyz;
 			if (KM & HSP) nn = p.nxx * nyx + p.nxy * nyy + p.nxz * nyz;
 		}
@@ -436,13 +441,13 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 		if (FIXQ > 0) {
 			if (act) accumulate_points<FAM, KM, FIXQ>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s);
 		} else if (a.fixed_degree >= 0) {
-			eval_degrees<FAM, KM>(a, j, act ? a.fixed_degree : -1, p, rec[j].w1, rinv_d, s);
+			eval_degrees<FAM, KM>(a, j, act ? a.fixed_degree : -1, p, h.w1, rinv_d, s);
 		} else {
 			// per-kernel degrees from the distance tables; where they coincide
 			// (all far pairs: one point each) all kernels share one pass
-			const float w1 = rec[j].w1;
+			const float w1 = h.w1;
 			const int fc = a.field_complexity;
-			if (__ballot(act && d2 < t[THR_22_3]) == 0) {
+			if (__ballot(act && d2 < h.far) == 0) {
 				// beyond the last breakpoint (22.3 R, the largest of all tables)
 				// every table degree is 0: the whole wave takes degree fc for all
 				// kernels in one pass (degrees 0 and 1: the centroid rule)
@@ -452,6 +457,10 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 					eval_degrees<FAM, KM>(a, j, act ? fc : -1, p, w1, rinv_d, s);
 				}
 			} else {
+				// the distance-table thresholds of the record
+				double t[NTHR];
+#pragma unroll
+				for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
 				const int dS = (KM & SLP) ? fc + table_degree_rec<1>(d2, t) : -1;
 				const int dD = (KM & (DLP | DLPT)) ? fc + table_degree_rec<2>(d2, t) : -1;
 				const int dH = (KM & HSP) ? fc + table_degree_rec<3>(d2, t) : -1;

@@ -191,19 +191,24 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
 	const auto rec = uni(a.rec);
+	// the trial record fields of the one-point path, loaded one column ahead
+	struct Head { double cx, cy, cz, near, far; float nx, ny, nz, w1; };
+	auto head = [&](int jj) {
+		return Head{rec[jj].c[0], rec[jj].c[1], rec[jj].c[2], rec[jj].thr[THR_NEAR], rec[jj].thr[THR_22_3],
+			rec[jj].nu[0], rec[jj].nu[1], rec[jj].nu[2], rec[jj].w1};
+	};
+	Head nxt = head(j0);
 	for (int j = j0; j < j1; ++j) {
-		// the trial record in SGPRs
-		double t[NTHR];
-#pragma unroll
-		for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
-		const double cjx = rec[j].c[0], cjy = rec[j].c[1], cjz = rec[j].c[2];
+		const Head h = nxt;
+		const double cjx = h.cx, cjy = h.cy, cjz = h.cz;
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
-		const bool act = row_ok && !(d2 < t[THR_NEAR]) && !(a.same_mesh && i == j);
+		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == j);
+		nxt = head(j + 1 < j1 ? j + 1 : j);
 		if (__ballot(act) == 0) continue;
 		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
 		p.d2f = (float)d2;
-		p.nyx = rec[j].nu[0]; p.nyy = rec[j].nu[1]; p.nyz = rec[j].nu[2];
-		p.w1 = rec[j].w1;
+		p.nyx = h.nx; p.nyy = h.ny; p.nyz = h.nz;
+		p.w1 = h.w1;
 		// d and 1/d in FP64 from the hardware reciprocal square root and one Newton step
 		double y = __builtin_amdgcn_rsq(d2);
 		y = y * (1.5 - 0.5 * d2 * y * y);
@@ -225,7 +230,11 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 			// last breakpoint (22.3 R) every table gives 0, order 1 (one point),
 			// and a wave of such pairs skips the selects
 			int nS = (KM & SLP) && act ? 1 : 0, nD = (KM & (DLP | DLPT)) && act ? 1 : 0, nH = (KM & HSP) && act ? 1 : 0;
-			if (__ballot(act && d2 < t[THR_22_3]) != 0) {
+			if (__ballot(act && d2 < h.far) != 0) {
+				// the distance-table thresholds of the record
+				double t[NTHR];
+#pragma unroll
+				for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
 				if (KM & SLP) nS = act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
 				if (KM & (DLP | DLPT)) nD = act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
 				if (KM & HSP) nH = act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
