@@ -229,25 +229,28 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 			// Gauss order per kernel group: degree 1 + table degree; beyond the
 			// last breakpoint (22.3 R) every table gives 0, order 1 (one point),
 			// and a wave of such pairs skips the selects
-			int nS = (KM & SLP) && act ? 1 : 0, nD = (KM & (DLP | DLPT)) && act ? 1 : 0, nH = (KM & HSP) && act ? 1 : 0;
-			if (__ballot(act && d2 < h.far) != 0) {
+			if (__ballot(act && d2 < h.far) == 0) {
+				// no pair of the wave inside 22.3 R: the one-point rule for all
+				// kernels at once (inactive lanes store nothing)
+				quad_centre<FAM, KM>(p, a.k, K);
+			} else {
 				// the distance-table thresholds of the record
 				double t[NTHR];
 #pragma unroll
 				for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
-				if (KM & SLP) nS = act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
-				if (KM & (DLP | DLPT)) nD = act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
-				if (KM & HSP) nH = act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
-			}
-			const int n0 = (KM & SLP) ? nS : (KM & (DLP | DLPT)) ? nD : nH;
-			const bool joint = act && (!(KM & SLP) || nS == n0) && (!(KM & (DLP | DLPT)) || nD == n0) &&
-				(!(KM & HSP) || nH == n0);
-			quad_orders<FAM, KM>(a, j, joint ? n0 : 0, p, K);
-			const bool sep = act && !joint;
-			if (__ballot(sep) != 0) {
-				if (KM & SLP) quad_orders<FAM, KM & SLP>(a, j, sep ? nS : 0, p, K);
-				if (KM & (DLP | DLPT)) quad_orders<FAM, KM & (DLP | DLPT)>(a, j, sep ? nD : 0, p, K);
-				if (KM & HSP) quad_orders<FAM, KM & HSP>(a, j, sep ? nH : 0, p, K);
+				const int nS = (KM & SLP) && act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
+				const int nD = (KM & (DLP | DLPT)) && act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
+				const int nH = (KM & HSP) && act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
+				const int n0 = (KM & SLP) ? nS : (KM & (DLP | DLPT)) ? nD : nH;
+				const bool joint = act && (!(KM & SLP) || nS == n0) && (!(KM & (DLP | DLPT)) || nD == n0) &&
+					(!(KM & HSP) || nH == n0);
+				quad_orders<FAM, KM>(a, j, joint ? n0 : 0, p, K);
+				const bool sep = act && !joint;
+				if (__ballot(sep) != 0) {
+					if (KM & SLP) quad_orders<FAM, KM & SLP>(a, j, sep ? nS : 0, p, K);
+					if (KM & (DLP | DLPT)) quad_orders<FAM, KM & (DLP | DLPT)>(a, j, sep ? nD : 0, p, K);
+					if (KM & HSP) quad_orders<FAM, KM & HSP>(a, j, sep ? nH : 0, p, K);
+				}
 			}
 		}
 		if (act) {
