@@ -26,7 +26,6 @@ LU of a request runs on its own stream and buffers.
 from __future__ import annotations
 
 import dataclasses
-import threading
 import time
 from collections import deque
 from concurrent.futures import ThreadPoolExecutor
@@ -83,8 +82,6 @@ class Server:
             c, nu = _centres_normals(nodes, tris)
             self.geom[key] = (torch.from_numpy(c).cuda(), torch.from_numpy(nu).cuda())
         self.slots = [torch.cuda.Stream() for _ in range(slots)]
-        # the engine's per-mesh caches are filled on first use: one caller at a time
-        self._engine_lock = threading.Lock()
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = 0.8 * free
@@ -107,8 +104,8 @@ class Server:
             A = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
             B = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
             pol = self.quad_policy if mesh.nodes_per_element == 4 else self.policy
-            with self._engine_lock:
-                nbem.assemble_bm(mesh, req.k, A, B, pol=pol, stream=stream)
+            # (the engine serialises calls on one mesh itself)
+            nbem.assemble_bm(mesh, req.k, A, B, pol=pol, stream=stream)
             d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")
             phase = -req.k * (c @ d)
             pinc = torch.polar(torch.ones_like(phase), phase)
