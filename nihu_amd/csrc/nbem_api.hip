@@ -484,6 +484,18 @@ int dtype_bytes(int dt)
 	}
 }
 
+// zero_kernel over a rows x cols block: one grid row per column (up to 65535,
+// the kernel strides beyond), enough blocks along x for a column of 16-byte words
+void launch_zero(void *out, int64_t ld, int64_t rows, int64_t cols, int elem_bytes, hipStream_t st)
+{
+	if (rows <= 0 || cols <= 0) return;
+	const int64_t words16 = (rows * elem_bytes + 15) / 16;
+	const unsigned gx = (unsigned)std::min<int64_t>((words16 + 255) / 256, 64);
+	const unsigned gy = (unsigned)std::min<int64_t>(cols, 65535);
+	hipLaunchKernelGGL(nbk::zero_kernel, dim3(gx, gy), dim3(256), 0, st, out, (long long)ld, (long long)rows,
+		(long long)cols, elem_bytes);
+}
+
 int check_out(const nb_out *o, int64_t rows, int64_t cols, bool complex_needed)
 {
 	if (!o) return fail(NB_ERR_ARG, "null output descriptor");
@@ -518,10 +530,7 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	const int64_t rows = nodal ? test->n_nodes : test->n, cols = nodal ? trial->n_nodes : trial->n;
 	const int eb = dtype_bytes(dtype);
 	auto zero = [&]() -> int {
-		const long long words = rows * eb / 4 * cols;
-		if (words > 0)
-			hipLaunchKernelGGL(nbk::zero_kernel, dim3((words + 255) / 256), dim3(256), 0, st, out, (long long)ld,
-				(long long)rows, (long long)cols, eb);
+		launch_zero(out, ld, rows, cols, eb, st);
 		NB_HIP(hipGetLastError());
 		return NB_OK;
 	};
@@ -1364,12 +1373,7 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 			}
 			return NB_OK;
 		}
-		if (!out->accumulate) {
-			const int eb = dtype_bytes(out->dtype);
-			const long long words = out->rows * eb / 4 * out->cols;
-			hipLaunchKernelGGL(nbk::zero_kernel, dim3((words + 255) / 256), dim3(256), 0, st, out->ptr, (long long)out->ld,
-				(long long)out->rows, (long long)out->cols, eb);
-		}
+		if (!out->accumulate) launch_zero(out->ptr, out->ld, out->rows, out->cols, dtype_bytes(out->dtype), st);
 		hipLaunchKernelGGL(nbk::identity_kernel, dim3((test->n + 255) / 256), dim3(256), 0, st, out->ptr,
 			(long long)out->ld, test->n, out->dtype, alpha.re, alpha.im);
 		NB_HIP(hipGetLastError());

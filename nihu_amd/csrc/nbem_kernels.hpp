@@ -1027,14 +1027,23 @@ __global__ void identity_kernel(void *out, long long ld, int n, int dtype, doubl
 }
 
 // zero a column-major rows x cols block with leading dimension ld (a fill
-// kernel rather than a memset so that callers may capture it in a graph)
+// kernel rather than a memset so that callers may capture it in a graph):
+// columns along the grid's y, 16-byte stores where a column allows them
 __global__ void zero_kernel(void *out, long long ld, long long rows, long long cols, int elem_bytes)
 {
-	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	const long long words_per_col = rows * elem_bytes / 4;
-	if (t >= words_per_col * cols) return;
-	const long long c = t / words_per_col, w = t % words_per_col;
-	((uint32_t *)out)[c * (ld * elem_bytes / 4) + w] = 0u;
+	const long long col_bytes = rows * elem_bytes, ld_bytes = ld * elem_bytes;
+	const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x, dt = (long long)gridDim.x * blockDim.x;
+	for (long long c = blockIdx.y; c < cols; c += gridDim.y) {
+		char *base = (char *)out + c * ld_bytes;
+		if ((((size_t)base | (size_t)col_bytes) & 15) == 0) {
+			uint4 *q = (uint4 *)base;
+			for (long long w = t0; w < col_bytes / 16; w += dt) q[w] = make_uint4(0u, 0u, 0u, 0u);
+		} else {
+			// elements of 4, 8 or 16 bytes: every column starts on 4 bytes
+			uint32_t *q = (uint32_t *)base;
+			for (long long w = t0; w < col_bytes / 4; w += dt) q[w] = 0u;
+		}
+	}
 }
 
 } // namespace nbk
