@@ -907,14 +907,34 @@ __device__ __forceinline__ void diag_combine(const double (&st)[4], const double
 
 // dynamic part of the Helmholtz near-singular integral: sum over the 42
 // Dunavant-14 points of (K_helm - K_lap) J w
-// (nihu/library/helmholtz_nearly_singular_integrals.hpp:48-219)
+// (nihu/library/helmholtz_nearly_singular_integrals.hpp:48-219).
+// The element is flat: r = (V0 - x) + xi (V1 - V0) + eta (V2 - V0), so r.n_y
+// is the same at every point and r.n_x is linear in (xi, eta).
 template <int MASK>
 __device__ __forceinline__ void near_dynamic(const D3 V[3], D3 x0, D3 nx, D3 ny, double J, double k, const Rules &R,
 	double2 (&s)[4])
 {
+	const D3 a0 = d3sub(V[0], x0), e1 = d3sub(V[1], V[0]), e2 = d3sub(V[2], V[0]);
+	const double any = d3dot(a0, ny);
+	const double anx = d3dot(a0, nx), e1nx = d3dot(e1, nx), e2nx = d3dot(e2, nx);
+	const double nn = (MASK & HSP) ? d3dot(nx, ny) : 0.0;
+	const double J4 = J * (1.0 / (4.0 * PI_D));
 	for (int q = 0; q < 42; ++q) {
-		const D3 y = tri_x(V, R.d14xi[q], R.d14eta[q]);
-		dynamic_point<MASK>(x0, y, nx, ny, k, R.d14w[q] * J, s);
+		const double xi = R.d14xi[q], eta = R.d14eta[q];
+		const D3 rv = {__fma_rn(eta, e2.x, __fma_rn(xi, e1.x, a0.x)), __fma_rn(eta, e2.y, __fma_rn(xi, e1.y, a0.y)),
+			__fma_rn(eta, e2.z, __fma_rn(xi, e1.z, a0.z))};
+		const double r2 = d3dot(rv, rv);
+		const double rinv = rsqrt64(r2);
+		const double kr = k * (r2 * rinv);
+		double sn, cs;
+		sincos64(kr, &sn, &cs);
+		const double g0 = rinv * (R.d14w[q] * J4);
+		double rdny = 0.0, rdnx = 0.0;
+		if (MASK & (DLP | DLPT | HSP)) {
+			rdny = any * rinv;
+			rdnx = -__fma_rn(eta, e2nx, __fma_rn(xi, e1nx, anx)) * rinv;
+		}
+		dynamic_terms<MASK>(cs, sn, kr, g0, rinv, rdny, rdnx, nn, s);
 	}
 }
 
