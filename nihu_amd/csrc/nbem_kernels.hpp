@@ -752,52 +752,71 @@ __device__ void near_static(const D3 V[NC], D3 x0g, D3 nxg, const Rules &R, doub
 	const double sgz = (0.0 < z) - (z < 0.0);
 	const double zinv = 1.0 / z, nzz = nx.z * z;
 	double rs = 0.0, rd = 0.0, rt = 0.0, rh = 0.0;
+	// one Gauss node of the SLP / DLP / DLPt rule: cd = cos(th - th0_n),
+	// A = n_x cos th + n_y sin th (in-plane components of the test normal)
+	auto node6 = [&](int n, double cd, double A, double w) {
+		const double P = dist[n] * rcp64(cd);
+		const double r2 = P * P + z2;
+		const double rinv = rsqrt64(r2);
+		const double r = r2 * rinv;
+		if (MASK & SLP) rs += (r - az) * w;
+		if (MASK & DLP) rd += (1.0 - az * rinv) * w;
+		if (MASK & DLPT) {
+			const double rdot = P * A - nx.z * z;
+			rt += (-rdot * rinv + A * log64(P + r) - nx.z * sgz) * w;
+		}
+	};
+	// one node of the HSP rule: -P^2 / r^2 / z * (-r.n / r) with r^2 = P^2 + z^2
+	// and r.n = P A - n_z z, written with q = 1/P as
+	// (A - n_z z q) / (z (1 + z^2 q^2)^{3/2}); a side whose line holds the
+	// projected point (dist 0, P = 0) adds 0; in the plane: -n_z / P
+	auto node21 = [&](int n, double cd, double A, double w) {
+		const double Pinv = cd * dinv[n];
+		double integrand;
+		if (az > 1e-12) {
+			const double rs3 = rsqrt64(__fma_rn(z2 * Pinv, Pinv, 1.0));
+			integrand = dist[n] > 0.0 ? __fma_rn(-nzz, Pinv, A) * (rs3 * rs3 * rs3) * zinv : 0.0;
+		} else {
+			integrand = -nx.z * Pinv;
+		}
+		rh += integrand * w;
+	};
 	for (int n = 0; n < NC; ++n) {
 		double t1 = th_lim[n], t2 = th_lim[n == NC - 1 ? 0 : n + 1];
 		if (fabs(t2 - t1) > PI_D) {
 			if (t2 > t1) t1 += 2.0 * PI_D;
 			else t2 += 2.0 * PI_D;
 		}
-		const double jac = (t2 - t1) / 2.0;
+		const double jac = (t2 - t1) / 2.0, tm = (t1 + t2) / 2.0;
+		// the Gauss nodes th = tm + jac x_g come in mirror pairs tm +- a (the
+		// rules are symmetric, x_{N-1-g} = -x_g): with the sin and cos of tm
+		// and of a, cos(th - th0_n) = Cm cos a -+ Sm sin a and
+		// n_x cos th + n_y sin th = Am cos a +- Bm sin a
+		double sm, cm;
+		sincos64(tm, &sm, &cm);
+		const double Cm = cm * c0[n] + sm * s0[n], Sm = sm * c0[n] - cm * s0[n];
+		const double Am = nx.x * cm + nx.y * sm, Bm = nx.y * cm - nx.x * sm;
 		if (MASK & (SLP | DLP | DLPT)) {
-			for (int g = (lane + 6 * nl - (6 * n) % nl) % nl; g < 6; g += nl) {
-				const double xi = R.gl6x[g], w = R.gl6w[g] * jac;
-				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
-				double sth, cth;
-				sincos64(th, &sth, &cth);
-				const double P = dist[n] * rcp64(cth * c0[n] + sth * s0[n]);
-				const double r2 = P * P + z2;
-				const double rinv = rsqrt64(r2);
-				const double r = r2 * rinv;
-				if (MASK & SLP) rs += (r - az) * w;
-				if (MASK & DLP) rd += (1.0 - az * rinv) * w;
-				if (MASK & DLPT) {
-					const double rdot = nx.x * (P * cth) + nx.y * (P * sth) + nx.z * (-z);
-					rt += (-rdot * rinv + (nx.x * cth + nx.y * sth) * log64(P + r) - nx.z * sgz) * w;
-				}
+			for (int g = (lane + 3 * nl - (3 * n) % nl) % nl; g < 3; g += nl) {
+				double sa, ca;
+				sincos64(jac * R.gl6x[g], &sa, &ca);
+				const double w = R.gl6w[g] * jac;
+				node6(n, __fma_rn(-Sm, sa, Cm * ca), __fma_rn(Bm, sa, Am * ca), w);
+				node6(n, __fma_rn(Sm, sa, Cm * ca), __fma_rn(-Bm, sa, Am * ca), w);
 			}
 		}
 		if (MASK & HSP) {
-			for (int g = (lane + 21 * nl - (21 * n) % nl) % nl; g < 21; g += nl) {
-				const double xi = R.gl21x[g], w = R.gl21w[g] * jac;
-				const double th = t1 * (1.0 - xi) / 2.0 + t2 * (1.0 + xi) / 2.0;
-				double sth, cth;
-				sincos64(th, &sth, &cth);
-				const double Pinv = (cth * c0[n] + sth * s0[n]) * dinv[n];
-				// -P^2 / r^2 / z * (-r.n / r) with r^2 = P^2 + z^2 and
-				// r.n = P (n_x cos th + n_y sin th) - n_z z, written with q = 1/P as
-				// (n_x cos th + n_y sin th - n_z z q) / (z (1 + z^2 q^2)^{3/2});
-				// a side whose line holds the projected point (dist 0, P = 0) adds 0;
-				// in the plane: -n_z / P
-				double integrand;
-				if (az > 1e-12) {
-					const double rs3 = rsqrt64(__fma_rn(z2 * Pinv, Pinv, 1.0));
-					integrand = dist[n] > 0.0 ?
-						__fma_rn(-nzz, Pinv, nx.x * cth + nx.y * sth) * (rs3 * rs3 * rs3) * zinv : 0.0;
+			// 10 mirror pairs and the middle node x_10 = 0 (th = tm)
+			for (int g = (lane + 11 * nl - (11 * n) % nl) % nl; g < 11; g += nl) {
+				const double w = R.gl21w[g] * jac;
+				if (g < 10) {
+					double sa, ca;
+					sincos64(jac * R.gl21x[g], &sa, &ca);
+					node21(n, __fma_rn(-Sm, sa, Cm * ca), __fma_rn(Bm, sa, Am * ca), w);
+					node21(n, __fma_rn(Sm, sa, Cm * ca), __fma_rn(-Bm, sa, Am * ca), w);
 				} else {
-					integrand = -nx.z * Pinv;
+					node21(n, Cm, Am, w);
 				}
-				rh += integrand * w;
 			}
 		}
 	}
