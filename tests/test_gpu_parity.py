@@ -176,24 +176,28 @@ def test_error_paths(nb, l3):
         nb.assemble(m, m, nb.colmajor(n, n, "float32"), 1, 0, 0, 1.0)
 
 
-def test_headline_workload_rows(nb, orc):
-    """20480-triangle jittered icosphere, ka = 10, Burton-Miller A and B with the
-    7-point rule far and FP64 within 2 element diameters: 48 seeded rows of the
-    full GPU matrices against the oracle."""
+@pytest.mark.parametrize("pname", ["config", "nihu"])
+def test_headline_workload_rows(nb, orc, pname):
+    """20480-triangle jittered icosphere, ka = 10, Burton-Miller A and B: 48
+    seeded rows of the full GPU matrices against the oracle.  "config": the
+    7-point rule far and FP64 within 2 element diameters; "nihu": the
+    distance-driven degrees, whose 1-point rule beyond 22.3 mean edges (the
+    centroid fast path of whole far waves) only occurs on meshes this fine."""
     import torch
     from nihu_amd.mesh import jittered_icosphere
     nodes, tris = jittered_icosphere(5, seed=0)
     n = len(tris)
     m = nb.Mesh(nodes, tris)
     A, B = nb.colmajor(n, n), nb.colmajor(n, n)
-    nb.assemble_bm(m, 10.0, A, B, pol=nb.policy(**POLICIES["config"][0]))
+    nb.assemble_bm(m, 10.0, A, B, pol=nb.policy(**POLICIES[pname][0]))
     torch.cuda.synchronize()
     rows = np.sort(np.random.default_rng(1).choice(n, 48, replace=False)).astype(np.int32)
     ri = torch.from_numpy(rows.astype(np.int64)).cuda()
     Ag, Bg = to_np(A[ri, :]), to_np(B[ri, :])
-    Ar, Br = orc.assemble_bm(nodes, tris, 10.0, pol=orc.policy(**POLICIES["config"][1]), rows=rows)
+    Ar, Br = orc.assemble_bm(nodes, tris, 10.0, pol=orc.policy(**POLICIES[pname][1]), rows=rows)
     ea, eb = relfro(Ag, Ar), relfro(Bg, Br)
-    print("L5 rows: A", ea, "B", eb, "max abs", np.abs(Ag - Ar).max(), np.abs(Bg - Br).max(), flush=True)
+    print("L5 rows (%s): A" % pname, ea, "B", eb, "max abs", np.abs(Ag - Ar).max(), np.abs(Bg - Br).max(),
+          flush=True)
     assert ea < FRO_TOL and eb < FRO_TOL
     # every entry was written (no NaN, nothing left from a previous buffer)
     assert torch.isfinite(torch.view_as_real(A)).all() and torch.isfinite(torch.view_as_real(B)).all()

@@ -98,6 +98,30 @@ def test_sweep_nihu_policy(nb, orc, l3):
     assert relfro(A[1].cpu().numpy(), Ar) < 1e-5 and relfro(B[1].cpu().numpy(), Br) < 1e-5
 
 
+def test_sweep_nihu_policy_level4_rows(nb, orc):
+    """The config-4 mesh (5120 triangles), four wavenumbers in one launch under
+    the distance-driven policy, whose 1-point far rule (beyond 22.3 mean
+    edges) the level-3 mesh never reaches: 24 seeded rows of each matrix
+    against the oracle at the same wavenumber."""
+    import torch
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(4, seed=0)
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    ks = np.array([0.5, 7.0, 19.0, 32.0])
+    A, B = stack(len(ks), n), stack(len(ks), n)
+    nb.assemble_sweep(m, ks, A, B, pol=nb.policy("nihu"))
+    torch.cuda.synchronize()
+    rows = np.sort(np.random.default_rng(5).choice(n, 24, replace=False)).astype(np.int32)
+    ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+    for f, k in enumerate(ks):
+        Ar, Br = orc.assemble_bm(nodes, tris, float(k), rows=rows)
+        ea = relfro(A[f][ri, :].cpu().numpy(), Ar)
+        eb = relfro(B[f][ri, :].cpu().numpy(), Br)
+        print("k %.1f: A %.2e B %.2e" % (k, ea, eb), flush=True)
+        assert ea < 1e-5 and eb < 1e-5
+
+
 def test_autotune(nb, l3):
     nodes, tris = l3
     m = nb.Mesh(nodes, tris)
