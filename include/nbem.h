@@ -137,6 +137,21 @@ int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32
  * nihu/library/nearly_singular_collocational.hpp:71-203 and
  * nearly_singular_collocational_telles.hpp:66-103 (near-singular pairs). */
 int nb_mesh_create_quads(const double *xyz, int32_t n_nodes, const int32_t *quad, int32_t n_quad, nb_mesh **out);
+/* Upload a mesh from NiHu's element table: elem[n_elem][5] rows
+ * [type id, n0, n1, n2, n3], type id NB_TRIA_1_ID (n3 ignored) or
+ * NB_QUAD_1_ID; element id = row number, whatever the type (NiHu's
+ * mesh(nodes, elements) numbers elements in table order:
+ * nihu/core/mesh.hpp:137-193, push_element at :150; the tables that
+ * read_off_data builds, nihu/interface/read_off_mesh.hpp:75-105, and
+ * create_mesh(nodes, elements, tria_1_tag(), quad_1_tag())).  A table of one
+ * type gives the same mesh as nb_mesh_create / nb_mesh_create_quads.  With
+ * both types, collocation (nb_assemble with NB_COLLOCATION, nb_assemble_bm)
+ * assembles each (test type, trial type) block and places it at the element
+ * ids; the Galerkin formalism, the sweep and nb_classify reject such meshes
+ * (NB_ERR_UNSUPPORTED). */
+#define NB_TRIA_1_ID 32303
+#define NB_QUAD_1_ID 32404
+int nb_mesh_create_elements(const double *xyz, int32_t n_nodes, const int32_t *elem, int32_t n_elem, nb_mesh **out);
 int nb_mesh_destroy(nb_mesh *mesh);
 int nb_mesh_info(const nb_mesh *mesh, int32_t *n_nodes, int32_t *n_tri, int64_t *device_bytes);
 

@@ -278,6 +278,10 @@ struct nb_mesh {
 	// Galerkin singular pairs against itself (coincident pairs first)
 	int64_t sp_n = -1;
 	DevPtr sp_e, sp_f, sp_info;
+	// both element types (npe = 0): the triangle part, the quadrilateral part
+	// and, per part, the element id (table row) of each of its elements
+	std::unique_ptr<nb_mesh> part[2];
+	DevPtr d_ids[2];
 	std::mutex mu;
 	int64_t bytes = 0;
 };
@@ -679,17 +683,23 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 // one assembly into device buffers (dtype, ld, accumulate taken from outs)
 // Collocation on a quadrilateral trial mesh: the regular pass (FP32) and the
 // near-singular and diagonal passes (FP64) on a side stream, disjoint entries.
+// The near-singular methods of quadrilaterals start from the image of the
+// collocation point on the element (Newton from the centre), which is only
+// reliable close to the element: NiHu applies them for d / R < 1.5.
+int check_quad_policy(const nb_policy &p)
+{
+	if (p.near_mode == NB_NEAR_DIAMETER || (p.near_mode == NB_NEAR_MEAN_EDGE && p.near_limit > 1.5))
+		return fail(NB_ERR_UNSUPPORTED, "quadrilaterals: the near-singular rule must be mean edge with limit <= 1.5 "
+			"(NiHu's d / R < 1.5) or none");
+	return NB_OK;
+}
+
 int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
 	nb_mesh *trial = jb.trial;
 	const nb_mesh *test = jb.test;
 	int s;
-	// The near-singular methods of quadrilaterals start from the image of the
-	// collocation point on the element (Newton from the centre), which is only
-	// reliable close to the element: NiHu applies them for d / R < 1.5.
-	if (jb.pol.near_mode == NB_NEAR_DIAMETER || (jb.pol.near_mode == NB_NEAR_MEAN_EDGE && jb.pol.near_limit > 1.5))
-		return fail(NB_ERR_UNSUPPORTED, "quadrilaterals: the near-singular rule must be mean edge with limit <= 1.5 "
-			"(NiHu's d / R < 1.5) or none");
+	if ((s = check_quad_policy(jb.pol))) return s;
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	int fixed_n = 0;
 	if (jb.pol.rule == NB_RULE_FIXED) {
@@ -979,6 +989,74 @@ void host_add(void *dst, int64_t ld, const void *src, int64_t rows, int64_t cols
 	for (auto &t : th) t.join();
 }
 
+// the single-type parts of a mesh and the element ids of their elements
+// (null: the identity)
+int mesh_parts(const nb_mesh *m, nb_mesh *parts[2], const int *ids[2])
+{
+	if (m->npe != 0) {
+		parts[0] = const_cast<nb_mesh *>(m);
+		ids[0] = nullptr;
+		return 1;
+	}
+	for (int p = 0; p < 2; ++p) {
+		parts[p] = m->part[p].get();
+		ids[p] = (const int *)m->d_ids[p].get();
+	}
+	return 2;
+}
+
+// Collocation with a mesh that holds both element types (NiHu's mesh over
+// tria_1 and quad_1, element ids in table order).  Each (test part, trial
+// part) block runs the single-type passes into a scratch matrix, which is then
+// scattered into the result at the parts' element ids.  A block of one part
+// against itself on the same mesh keeps the diagonal pass; the other blocks
+// pair different elements, where NiHu's same-mesh loop finds no element match
+// and does what its different-mesh loop does (near check, else regular:
+// nihu/core/double_integral.hpp:599-673).  The blocks run one after another on
+// the stream and share the scratch matrix.
+int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	nb_mesh *tp[2], *sp[2];
+	const int *tid[2], *sid[2];
+	const int nt = mesh_parts(jb.test, tp, tid), ns = mesh_parts(jb.trial, sp, sid);
+	const int eb = dtype_bytes(dtype);
+	int s;
+	// refuse before any block has run
+	for (int b = 0; b < ns; ++b)
+		if (sp[b]->npe == 4 && (s = check_quad_policy(jb.pol))) return s;
+	size_t most = 0;
+	for (int a = 0; a < nt; ++a)
+		for (int b = 0; b < ns; ++b) most = std::max(most, (size_t)tp[a]->n * sp[b]->n * eb);
+	DevPtr t0, t1;
+	if ((s = dev_alloc(t0, most))) return s;
+	if (out1 && (s = dev_alloc(t1, most))) return s;
+	for (int a = 0; a < nt; ++a)
+		for (int b = 0; b < ns; ++b) {
+			Job sub = jb;
+			sub.test = tp[a];
+			sub.trial = sp[b];
+			sub.same = jb.same && tp[a] == sp[b];
+			// the parts are private to their mesh, whose lock the caller holds
+			if ((s = run_device(sub, t0.get(), out1 ? t1.get() : nullptr, tp[a]->n, dtype, 0, st))) return s;
+			const dim3 grid((unsigned)((tp[a]->n + 255) / 256), (unsigned)std::min(sp[b]->n, 65535));
+			hipLaunchKernelGGL(nbk::scatter_block_kernel, grid, dim3(256), 0, st, (const void *)t0.get(), tp[a]->n,
+				sp[b]->n, tid[a], sid[b], out0, (long long)ld, dtype, accumulate);
+			if (out1)
+				hipLaunchKernelGGL(nbk::scatter_block_kernel, grid, dim3(256), 0, st, (const void *)t1.get(),
+					tp[a]->n, sp[b]->n, tid[a], sid[b], out1, (long long)ld, dtype, accumulate);
+			NB_HIP(hipGetLastError());
+		}
+	// the scratch matrices are freed on return: wait for the last scatter
+	NB_HIP(hipStreamSynchronize(st));
+	return NB_OK;
+}
+
+int run_any(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	if (jb.test->npe == 0 || jb.trial->npe == 0) return run_mixed(jb, out0, out1, ld, dtype, accumulate, st);
+	return run_device(jb, out0, out1, ld, dtype, accumulate, st);
+}
+
 // run a job into up to two outputs (both device, or both host)
 int run_job(const Job &jb, const nb_out *o0, const nb_out *o1, hipStream_t st)
 {
@@ -987,14 +1065,14 @@ int run_job(const Job &jb, const nb_out *o0, const nb_out *o1, hipStream_t st)
 	const int64_t rows = o0->rows, cols = o0->cols;
 	if (o0->location == NB_DEVICE) {
 		if (o1 && o0->ld != o1->ld) return fail(NB_ERR_ARG, "A and B must share the leading dimension");
-		return run_device(jb, o0->ptr, o1 ? o1->ptr : nullptr, o0->ld, o0->dtype, o0->accumulate, st);
+		return run_any(jb, o0->ptr, o1 ? o1->ptr : nullptr, o0->ld, o0->dtype, o0->accumulate, st);
 	}
 	const size_t bytes = (size_t)rows * cols * dtype_bytes(o0->dtype);
 	DevPtr t0, t1;
 	int s;
 	if ((s = dev_alloc(t0, bytes))) return s;
 	if (o1 && (s = dev_alloc(t1, bytes))) return s;
-	if ((s = run_device(jb, t0.get(), o1 ? t1.get() : nullptr, rows, o0->dtype, 0, st))) return s;
+	if ((s = run_any(jb, t0.get(), o1 ? t1.get() : nullptr, rows, o0->dtype, 0, st))) return s;
 	std::vector<char> h(bytes);
 	const nb_out *outs[2] = {o0, o1};
 	void *tmps[2] = {t0.get(), o1 ? t1.get() : nullptr};
@@ -1328,6 +1406,43 @@ int nb_mesh_create_quads(const double *xyz, int32_t n_nodes, const int32_t *quad
 	return NB_OK;
 }
 
+int nb_mesh_create_elements(const double *xyz, int32_t n_nodes, const int32_t *elem, int32_t n_elem, nb_mesh **out)
+{
+	if (!out) return fail(NB_ERR_ARG, "null output handle");
+	*out = nullptr;
+	if (!xyz || !elem || n_nodes <= 0 || n_elem <= 0) return fail(NB_ERR_ARG, "empty mesh");
+	std::vector<int32_t> conn[2];
+	std::vector<int32_t> ids[2];
+	for (int32_t e = 0; e < n_elem; ++e) {
+		const int32_t *row = elem + 5 * (size_t)e;
+		const int p = row[0] == NB_TRIA_1_ID ? 0 : row[0] == NB_QUAD_1_ID ? 1 : -1;
+		if (p < 0) return fail(NB_ERR_ARG, "unknown element type id (tria_1 32303 and quad_1 32404 are supported)");
+		conn[p].insert(conn[p].end(), row + 1, row + (p == 0 ? 4 : 5));
+		ids[p].push_back(e);
+	}
+	// one element type: the plain mesh of that type (ids are the part's own)
+	if (ids[1].empty()) return nb_mesh_create(xyz, n_nodes, conn[0].data(), n_elem, out);
+	if (ids[0].empty()) return nb_mesh_create_quads(xyz, n_nodes, conn[1].data(), n_elem, out);
+	static std::atomic<uint64_t> next_uid(3ull << 61);
+	std::unique_ptr<nb_mesh> m(new nb_mesh());
+	m->uid = next_uid++;
+	m->n_nodes = n_nodes;
+	m->n = n_elem;
+	m->npe = 0;
+	NB_HIP(hipGetDevice(&m->device));
+	nb_mesh *h = nullptr;
+	int s;
+	if ((s = nb_mesh_create(xyz, n_nodes, conn[0].data(), (int32_t)ids[0].size(), &h))) return s;
+	m->part[0].reset(h);
+	if ((s = nb_mesh_create_quads(xyz, n_nodes, conn[1].data(), (int32_t)ids[1].size(), &h))) return s;
+	m->part[1].reset(h);
+	if ((s = upload(m->d_ids[0], ids[0])) || (s = upload(m->d_ids[1], ids[1]))) return s;
+	m->tri.assign(elem, elem + 5 * (size_t)n_elem);
+	m->bytes = m->part[0]->bytes + m->part[1]->bytes + 4 * (int64_t)n_elem;
+	*out = m.release();
+	return NB_OK;
+}
+
 int nb_mesh_destroy(nb_mesh *mesh)
 {
 	delete mesh;
@@ -1427,6 +1542,8 @@ int nb_classify(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 {
 	if (!test || !trial || !cls) return fail(NB_ERR_ARG, "null argument");
 	if (same_mesh && test != trial) return fail(NB_ERR_ARG, "same_mesh requires test == trial");
+	if (test->npe == 0 || trial->npe == 0)
+		return fail(NB_ERR_UNSUPPORTED, "pair classes are reported for single-type meshes only");
 	int s;
 	if ((s = check_policy(policy))) return s;
 	nb_mesh *tr = const_cast<nb_mesh *>(trial);

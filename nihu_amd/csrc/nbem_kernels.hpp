@@ -1085,4 +1085,43 @@ __global__ void zero_kernel(void *out, long long ld, long long rows, long long c
 	}
 }
 
+// one block of a mesh with both element types: the dense rows x cols block
+// src (column-major, ld = rows) goes to rows rmap[i] and columns cmap[j] of
+// the result (null map: the identity), stored or added; rows along x,
+// columns along the grid's y (the kernel strides beyond 65535)
+template <class T>
+__device__ __forceinline__ void scatter_one(const void *src, long long s, void *dst, long long d, int accumulate)
+{
+	const T v = ((const T *)src)[s];
+	T *p = (T *)dst + d;
+	*p = accumulate ? *p + v : v;
+}
+
+__global__ void __launch_bounds__(256) scatter_block_kernel(const void *src, int rows, int cols, const int *rmap,
+	const int *cmap, void *dst, long long ld, int dtype, int accumulate)
+{
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= rows) return;
+	const long long r = rmap ? rmap[i] : i;
+	for (int j = blockIdx.y; j < cols; j += gridDim.y) {
+		const long long s = (long long)j * rows + i, d = (long long)(cmap ? cmap[j] : j) * ld + r;
+		switch (dtype) {
+		case DT_C64: {
+			const float2 v = ((const float2 *)src)[s];
+			float2 *p = (float2 *)dst + d;
+			*p = accumulate ? make_float2(p->x + v.x, p->y + v.y) : v;
+			break;
+		}
+		case DT_C128: {
+			const double2 v = ((const double2 *)src)[s];
+			double2 *p = (double2 *)dst + d;
+			*p = accumulate ? make_double2(p->x + v.x, p->y + v.y) : v;
+			break;
+		}
+		case DT_F32: scatter_one<float>(src, s, dst, d, accumulate); break;
+		default: scatter_one<double>(src, s, dst, d, accumulate); break;
+		}
+	}
+}
+
 } // namespace nbk

@@ -75,9 +75,10 @@ struct quad_1_tag {};
 enum { tria_1_id = 32303 };   // element type id of tria_1_elem (dimension 3, tria domain, 3 nodes)
 enum { quad_1_id = 32404 };   // quad_1_elem (dimension 3, quad domain, 4 nodes)
 
-// A mesh of flat triangles (3 nodes per element) or bilinear quadrilaterals
-// (4) resident on the current GPU.  Elements are numbered by insertion order;
-// shared ownership lets spaces and operators refer to it.
+// A mesh of flat triangles (3 nodes per element), bilinear quadrilaterals
+// (4) or both (0: built from NiHu's element table) resident on the current
+// GPU.  Elements are numbered by insertion order; shared ownership lets
+// spaces and operators refer to it.
 class mesh {
 public:
 	mesh(std::vector<double> const &xyz, std::vector<int32_t> const &elems, int nodes_per_element = 3)
@@ -92,8 +93,21 @@ public:
 		                 : nb_mesh_create_quads(m_xyz.data(), nn, m_tri.data(), ne, &h));
 		m_h.reset(h, [](nb_mesh *p) { nb_mesh_destroy(p); });
 	}
+	// NiHu's element table with both element types: m rows [type id, n0, n1,
+	// n2, n3] (n3 unused by tria_1), element id = row
+	struct element_table {};
+	mesh(std::vector<double> const &xyz, std::vector<int32_t> const &table, element_table)
+		: m_xyz(xyz), m_tri(table), m_npe(0)
+	{
+		if (xyz.size() % 3 || table.size() % 5)
+			throw std::invalid_argument("node array must be n x 3 and the element table m x 5");
+		nb_mesh *h = nullptr;
+		check(nb_mesh_create_elements(m_xyz.data(), (int32_t)(m_xyz.size() / 3), m_tri.data(),
+			(int32_t)(m_tri.size() / 5), &h));
+		m_h.reset(h, [](nb_mesh *p) { nb_mesh_destroy(p); });
+	}
 	nb_mesh *handle() const { return m_h.get(); }
-	int32_t num_elements() const { return (int32_t)(m_tri.size() / m_npe); }
+	int32_t num_elements() const { return (int32_t)(m_tri.size() / (m_npe ? m_npe : 5)); }
 	int32_t num_nodes() const { return (int32_t)(m_xyz.size() / 3); }
 	int nodes_per_element() const { return m_npe; }
 	std::vector<double> const &nodes() const { return m_xyz; }
@@ -122,7 +136,27 @@ inline mesh from_element_table(std::vector<double> const &nodes, std::vector<uns
 	return mesh(nodes, el, npe);
 }
 
+// NiHu's element table of m x 5 rows [type id, n0, ..., n3] holding tria_1
+// and quad_1 rows (a table of one type gives the mesh of that type)
+inline mesh from_mixed_table(std::vector<double> const &nodes, std::vector<int32_t> const &table)
+{
+	if (table.size() % 5) throw std::invalid_argument("elements must be m x 5: type id and four node indices");
+	bool tri = false, quad = false;
+	for (size_t e = 0; e < table.size(); e += 5) {
+		if (table[e] == tria_1_id) tri = true;
+		else if (table[e] == quad_1_id) quad = true;
+		else throw std::invalid_argument("only tria_1 and quad_1 elements are supported");
+	}
+	if (tri && quad) return mesh(nodes, table, mesh::element_table());
+	const int npe = tri ? 3 : 4;
+	std::vector<int32_t> el;
+	for (size_t e = 0; e < table.size(); e += 5)
+		for (int a = 1; a <= npe; ++a) el.push_back(table[e + a]);
+	return mesh(nodes, el, npe);
+}
+
 // OFF reader with NiHu's error messages; every face must have npe vertices
+// (npe = 0: triangles and quadrilaterals alike)
 inline mesh read_off(std::string const &fname, int npe)
 {
 	std::ifstream is(fname);
@@ -134,18 +168,21 @@ inline mesh read_off(std::string const &fname, int npe)
 	for (auto &v : xyz)
 		if (!(is >> v)) throw std::runtime_error("Error reading mesh nodes");
 	std::vector<int32_t> el;
-	el.reserve((size_t)npe * ne);
+	el.reserve((size_t)(npe ? npe : 5) * ne);
 	for (unsigned e = 0; e < ne; ++e) {
 		unsigned nv;
 		if (!(is >> nv)) throw std::runtime_error("Error reading mesh elements");
-		if (nv != (unsigned)npe) throw std::runtime_error("Invalid number of vertices for current element tag vector");
+		if (npe ? nv != (unsigned)npe : nv != 3 && nv != 4)
+			throw std::runtime_error("Invalid number of vertices for current element tag vector");
+		if (!npe) el.push_back(nv == 3 ? (int32_t)tria_1_id : (int32_t)quad_1_id);
 		for (unsigned a = 0; a < nv; ++a) {
 			unsigned v;
 			if (!(is >> v)) throw std::runtime_error("Error reading mesh elements");
 			el.push_back((int32_t)v);
 		}
+		if (!npe && nv == 3) el.push_back(0);
 	}
-	return mesh(xyz, el, npe);
+	return npe ? mesh(xyz, el, npe) : from_mixed_table(xyz, el);
 }
 } // namespace detail
 
@@ -162,9 +199,26 @@ inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> 
 	return detail::from_element_table(nodes, elements, quad_1_id, 4, "quad_1");
 }
 
-// read_off_mesh(filename, tag): OFF reader (nihu/interface/read_off_mesh.hpp:75-105)
+// create_mesh(nodes, elements, tria_1_tag(), quad_1_tag()) (either tag
+// order): an (m x 5) table whose rows are tria_1 [32303, n0, n1, n2, any] or
+// quad_1 [32404, n0, n1, n2, n3], element id = row whatever the type
+inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> const &elements, tria_1_tag,
+	quad_1_tag)
+{
+	return detail::from_mixed_table(nodes, std::vector<int32_t>(elements.begin(), elements.end()));
+}
+inline mesh create_mesh(std::vector<double> const &nodes, std::vector<unsigned> const &elements, quad_1_tag,
+	tria_1_tag)
+{
+	return create_mesh(nodes, elements, tria_1_tag(), quad_1_tag());
+}
+
+// read_off_mesh(filename, tags...): OFF reader (nihu/interface/read_off_mesh.hpp:75-105);
+// with both tags, faces of 3 and 4 vertices in one file
 inline mesh read_off_mesh(std::string const &fname, tria_1_tag) { return detail::read_off(fname, 3); }
 inline mesh read_off_mesh(std::string const &fname, quad_1_tag) { return detail::read_off(fname, 4); }
+inline mesh read_off_mesh(std::string const &fname, tria_1_tag, quad_1_tag) { return detail::read_off(fname, 0); }
+inline mesh read_off_mesh(std::string const &fname, quad_1_tag, tria_1_tag) { return detail::read_off(fname, 0); }
 
 // ------------------------------------------------------------------ spaces
 

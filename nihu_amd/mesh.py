@@ -146,6 +146,28 @@ def jittered_cubed_sphere(n: int, seed: int = 0):
     return cubed_sphere(n, 1.0, 0.01, seed)
 
 
+def mixed_cubed_sphere(n: int, seed: int = 0, every: int = 3):
+    """A closed surface with both element types: the jittered cubed sphere of
+    n x n quadrilaterals per face with every `every`-th quadrilateral split
+    along its diagonal (0, 2) into two triangles of the same orientation.
+    Returns (nodes, table) with NiHu's element table rows [type id, n0, n1,
+    n2, n3] (n3 = 0 for triangles), the two types interleaved."""
+    nodes, quads = jittered_cubed_sphere(n, seed)
+    rows = []
+    for e, q in enumerate(quads):
+        if e % every == 0:
+            rows.append([NIHU_TRIA_1_ID, q[0], q[1], q[2], 0])
+            rows.append([NIHU_TRIA_1_ID, q[0], q[2], q[3], 0])
+        else:
+            rows.append([NIHU_QUAD_1_ID, q[0], q[1], q[2], q[3]])
+    return nodes, np.array(rows, dtype=np.int32)
+
+
+def table_faces(table):
+    """Node indices of each row of NiHu's element table (3 or 4 per row)."""
+    return [list(r[1:4]) if r[0] == NIHU_TRIA_1_ID else list(r[1:5]) for r in np.asarray(table)]
+
+
 def read_off(path):
     """Read an OFF file of triangles or of quadrilaterals
     (nihu/interface/read_off_mesh.hpp:75-105); returns (nodes, elements) with
@@ -153,7 +175,31 @@ def read_off(path):
 
     Raises ValueError for a bad header, truncated data, faces that are
     neither triangles nor quadrilaterals (NiHu raises std::runtime_error in the
-    same places) and for meshes that mix the two (not supported here)."""
+    same places) and for files that mix the two (read_off_elements reads
+    those)."""
+    nodes, elems = _read_off_faces(path)
+    sizes = {len(e) for e in elems}
+    if len(sizes) > 1:
+        raise ValueError("triangles and quadrilaterals in one file: use read_off_elements")
+    npe = sizes.pop() if sizes else 3
+    return nodes, np.array(elems, dtype=np.int32).reshape(len(elems), npe)
+
+
+def read_off_elements(path):
+    """Read an OFF file of triangles, quadrilaterals or both into NiHu's
+    element table, as read_off_data does with the tags tria_1 and quad_1
+    (nihu/interface/read_off_mesh.hpp:75-105): rows [type id, n0, n1, n2, n3],
+    32303 for a triangle (n3 = 0) and 32404 for a quadrilateral, in file
+    order.  Returns (nodes, table); nbem.Mesh.from_elements takes the table."""
+    nodes, elems = _read_off_faces(path)
+    table = np.zeros((len(elems), 5), dtype=np.int32)
+    for e, f in enumerate(elems):
+        table[e, 0] = NIHU_TRIA_1_ID if len(f) == 3 else NIHU_QUAD_1_ID
+        table[e, 1:1 + len(f)] = f
+    return nodes, table
+
+
+def _read_off_faces(path):
     with open(path, "r") as fh:
         tok = fh.read().split()
     if not tok or tok[0] != "OFF":
@@ -179,15 +225,11 @@ def read_off(path):
             raise ValueError("Error reading mesh elements")
         elems.append([int(t) for t in tok[pos + 1:pos + 1 + nv]])
         pos += 1 + nv
-    sizes = {len(e) for e in elems}
-    if len(sizes) > 1:
-        raise ValueError("meshes mixing triangles and quadrilaterals are not supported")
-    npe = sizes.pop() if sizes else 3
-    return nodes, np.array(elems, dtype=np.int32).reshape(n_elem, npe)
+    return nodes, elems
 
 
 def write_off(path, nodes, elems):
-    elems = np.asarray(elems)
+    """elems: rows of 3 or 4 node indices (a list may mix the two)."""
     with open(path, "w") as fh:
         fh.write("OFF\n%d %d 0\n" % (len(nodes), len(elems)))
         for p in nodes:
@@ -215,4 +257,4 @@ def from_nihu_elements(elements):
         return np.ascontiguousarray(elements[:, 1:4].astype(np.int32))
     if np.all(elements[:, 0] == NIHU_QUAD_1_ID) and elements.shape[1] >= 5:
         return np.ascontiguousarray(elements[:, 1:5].astype(np.int32))
-    raise ValueError("only tria_1 (32303) or quad_1 (32404) elements are supported, not mixed")
+    raise ValueError("a table with both tria_1 (32303) and quad_1 (32404) rows: use nbem.Mesh.from_elements")

@@ -23,11 +23,12 @@ NB_NEAR_MEAN_EDGE, NB_NEAR_DIAMETER, NB_NEAR_NONE = 0, 1, 2
 NB_C64, NB_C128, NB_F32, NB_F64 = 0, 1, 2, 3
 NB_HOST, NB_DEVICE = 0, 1
 NB_COLLOCATION, NB_GALERKIN, NB_GALERKIN_LINEAR = 0, 1, 2
+NB_TRIA_1_ID, NB_QUAD_1_ID = 32303, 32404   # element type ids of NiHu's tria_1 and quad_1
 FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_linear": NB_GALERKIN_LINEAR}
 
 # symbols declared in include/nbem.h
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_create_quads",
-           "nb_mesh_destroy", "nb_mesh_info",
+           "nb_mesh_create_elements", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
            "nb_synchronize", "nb_profile", "nb_profile_read"]
 
@@ -72,6 +73,7 @@ def lib():
         l.nb_device_info.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         l.nb_mesh_create.argtypes = [P, C.c_int32, P, C.c_int32, C.POINTER(P)]
         l.nb_mesh_create_quads.argtypes = [P, C.c_int32, P, C.c_int32, C.POINTER(P)]
+        l.nb_mesh_create_elements.argtypes = [P, C.c_int32, P, C.c_int32, C.POINTER(P)]
         l.nb_mesh_destroy.argtypes = [P]
         l.nb_mesh_info.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         l.nb_assemble.argtypes = [P, P, C.c_int32, C.c_int32, NbKernel, NbScalar, C.POINTER(NbPolicy),
@@ -134,21 +136,55 @@ def _stream_ptr(stream):
 
 class Mesh:
     """A mesh resident on the current GPU (nb_mesh): elements with 3 node
-    indices (flat triangles) or 4 (bilinear quadrilaterals)."""
+    indices (flat triangles) or 4 (bilinear quadrilaterals); from_elements
+    takes NiHu's element table, which may hold both."""
 
     def __init__(self, nodes, tris):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
         self.tris = np.ascontiguousarray(tris, dtype=np.int32)
         if self.nodes.ndim != 2 or self.nodes.shape[1] != 3 or self.tris.ndim != 2 or self.tris.shape[1] not in (3, 4):
             raise ValueError("nodes must be n x 3 and elements m x 3 (triangles) or m x 4 (quadrilaterals)")
+        self._npe = self.tris.shape[1]
         h = C.c_void_p()
-        create = lib().nb_mesh_create if self.tris.shape[1] == 3 else lib().nb_mesh_create_quads
+        create = lib().nb_mesh_create if self._npe == 3 else lib().nb_mesh_create_quads
         check(create(self.nodes.ctypes.data, len(self.nodes), self.tris.ctypes.data, len(self.tris), C.byref(h)))
         self._h = h
 
+    @classmethod
+    def from_elements(cls, nodes, elements):
+        """NiHu's element table: rows [type id, n0, n1, n2(, n3)] with type id
+        32303 (tria_1) or 32404 (quad_1), element id = row number whatever the
+        type (nihu/core/mesh.hpp:137-193).  A table of one type gives the mesh
+        of that type (tris: its node indices); with both types, tris is the
+        table itself (m x 5) and nodes_per_element is 0."""
+        el = np.asarray(elements)
+        if el.ndim != 2 or el.shape[1] not in (4, 5) or len(el) == 0:
+            raise ValueError("elements must be a non-empty m x 4 or m x 5 table [type id, node ids]")
+        table = np.zeros((len(el), 5), dtype=np.int32)
+        table[:, :el.shape[1]] = el
+        types = set(np.unique(table[:, 0]).tolist())
+        if not types <= {NB_TRIA_1_ID, NB_QUAD_1_ID}:
+            raise ValueError("unknown element type id (tria_1 32303 and quad_1 32404 are supported)")
+        if types == {NB_QUAD_1_ID} and el.shape[1] < 5:
+            raise ValueError("quad_1 rows need four node indices")
+        if len(types) == 1:
+            return cls(nodes, table[:, 1:4] if types == {NB_TRIA_1_ID} else table[:, 1:5])
+        self = cls.__new__(cls)
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        if self.nodes.ndim != 2 or self.nodes.shape[1] != 3:
+            raise ValueError("nodes must be n x 3")
+        self.tris = table
+        self._npe = 0
+        h = C.c_void_p()
+        check(lib().nb_mesh_create_elements(self.nodes.ctypes.data, len(self.nodes), table.ctypes.data, len(table),
+                                             C.byref(h)))
+        self._h = h
+        return self
+
     @property
     def nodes_per_element(self):
-        return self.tris.shape[1]
+        """3 (triangles), 4 (quadrilaterals) or 0 (both)"""
+        return self._npe
 
     @property
     def handle(self):

@@ -103,7 +103,7 @@ class Server:
         with torch.cuda.stream(stream):
             A = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
             B = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
-            pol = self.quad_policy if mesh.nodes_per_element == 4 else self.policy
+            pol = self.quad_policy if mesh.nodes_per_element != 3 else self.policy
             # (the engine serialises calls on one mesh itself)
             nbem.assemble_bm(mesh, req.k, A, B, pol=pol, stream=stream)
             d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")

@@ -4,10 +4,11 @@
 // Galerkin with constant fields), on the GPU through the C ABI, plus the same
 // answers with isoparametric (nodal) fields.
 //
-//   test_facade <mesh.off> [quadrilateral mesh.off]
+//   test_facade <mesh.off> [quadrilateral mesh.off [mixed mesh.off]]
 // Prints the errors and exits non-zero if one is above its bound.  With a
 // quadrilateral mesh, also the collocation answers and the Burton-Miller
-// composition (hypersingular diagonal by Guiggiani's method) on it.
+// composition (hypersingular diagonal by Guiggiani's method) on it; with a
+// mesh of both element types (read with both tags), the same again.
 
 #include "nihu_amd/nihu.hpp"
 
@@ -71,30 +72,26 @@ static std::vector<centre_normal> geometry(Space const &s)
 	std::vector<centre_normal> g(m.num_elements());
 	auto const &X = m.nodes();
 	auto const &T = m.elements();
-	if (m.nodes_per_element() == 4) {
-		// bilinear quadrilateral: x(0, 0) and dx/dxi x dx/deta there
-		for (int e = 0; e < m.num_elements(); ++e) {
-			const double *p[4];
-			for (int a = 0; a < 4; ++a) p[a] = &X[3 * T[4 * e + a]];
-			double u[3], v[3];
-			for (int d = 0; d < 3; ++d) {
+	for (int e = 0; e < m.num_elements(); ++e) {
+		// node ids of element e: its row of the element table when the mesh
+		// holds both types ([type id, n0, ..., n3])
+		const int npe = m.nodes_per_element() ? m.nodes_per_element()
+			: T[5 * e] == nihu_amd::tria_1_id ? 3 : 4;
+		const int32_t *ids = m.nodes_per_element() ? &T[(size_t)npe * e] : &T[5 * (size_t)e + 1];
+		const double *p[4];
+		for (int a = 0; a < npe; ++a) p[a] = &X[3 * ids[a]];
+		double u[3], v[3];
+		for (int d = 0; d < 3; ++d) {
+			if (npe == 4) {
+				// bilinear quadrilateral: x(0, 0) and dx/dxi x dx/deta there
 				g[e].c[d] = (p[0][d] + p[1][d] + p[2][d] + p[3][d]) / 4.0;
 				u[d] = (p[1][d] + p[2][d] - p[0][d] - p[3][d]) / 4.0;
 				v[d] = (p[2][d] + p[3][d] - p[0][d] - p[1][d]) / 4.0;
+			} else {
+				g[e].c[d] = (p[0][d] + p[1][d] + p[2][d]) / 3.0;
+				u[d] = p[1][d] - p[0][d];
+				v[d] = p[2][d] - p[0][d];
 			}
-			double nv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
-			double l = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-			for (int d = 0; d < 3; ++d) g[e].n[d] = nv[d] / l;
-		}
-		return g;
-	}
-	for (int e = 0; e < m.num_elements(); ++e) {
-		const double *a = &X[3 * T[3 * e]], *b = &X[3 * T[3 * e + 1]], *c = &X[3 * T[3 * e + 2]];
-		double u[3], v[3];
-		for (int d = 0; d < 3; ++d) {
-			g[e].c[d] = (a[d] + b[d] + c[d]) / 3.0;
-			u[d] = b[d] - a[d];
-			v[d] = c[d] - a[d];
 		}
 		double nv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
 		double l = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
@@ -290,6 +287,33 @@ int main(int argc, char *argv[])
 			bad = 1;
 		} catch (std::logic_error const &ex) {
 			std::printf("quadrilateral Galerkin: logic_error as expected: %s\n", ex.what());
+		}
+	}
+
+	if (argc > 3) {
+		// triangles and quadrilaterals in one mesh, read as NiHu's EAA
+		// benchmark drivers read theirs
+		auto mmesh = nihu_amd::read_off_mesh(argv[3], nihu_amd::quad_1_tag(), nihu_amd::tria_1_tag());
+		auto const &mspace = nihu_amd::constant_view(mmesh);
+		std::printf("mixed mesh: %d elements, nodes_per_element %d\n", mmesh.num_elements(),
+			mmesh.nodes_per_element());
+		bad |= mmesh.nodes_per_element() != 0;
+		double mt = tester(nihu_amd::dirac(mspace), mspace);
+		std::printf("Mixed-mesh collocation transparent error: %g\n", mt);
+		bad |= !(mt < 0.05);
+		double mp = pulsating(nihu_amd::dirac(mspace), mspace);
+		std::printf("Pulsating sphere relative error (mixed-mesh collocation): %g\n", mp);
+		bad |= !(mp < 0.05);
+		double mbm = burton_miller(mmesh);
+		std::printf("Mixed-mesh Burton-Miller composition vs fused assembly: %g\n", mbm);
+		bad |= !(mbm < 1e-6);
+		try {
+			nihu_amd::cMatrix X(mspace.get_num_dofs(), mspace.get_num_dofs());
+			X << mspace * nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(1.0))[mspace];
+			std::printf("no exception for mixed-mesh Galerkin\n");
+			bad = 1;
+		} catch (std::logic_error const &ex) {
+			std::printf("mixed-mesh Galerkin: logic_error as expected: %s\n", ex.what());
 		}
 	}
 

@@ -134,3 +134,36 @@ def test_empty_and_degenerate_meshes_rejected_without_gpu(nb):
         nb.Mesh(nodes, np.array([[0, 1, -1]]))           # negative node index
     with pytest.raises(ValueError):
         nb.Mesh(nodes, np.ones((2, 5), dtype=np.int32))   # neither triangles nor quadrilaterals
+    # NiHu element tables: unknown type ids, and a degenerate element in a table of one type
+    with pytest.raises(ValueError, match="type id"):
+        nb.Mesh.from_elements(nodes, np.array([[32303, 0, 1, 2, 0], [32505, 0, 1, 2, 3]]))
+    with pytest.raises(ValueError, match="empty"):
+        nb.Mesh.from_elements(nodes, np.zeros((0, 5), dtype=np.int32))
+    with pytest.raises(ValueError):
+        nb.Mesh.from_elements(nodes, np.array([[32303, 0, 1, 1]]))
+    raw = np.array([[32303, 0, 1, 2, 0], [32505, 0, 1, 2, 3]], dtype=np.int32)
+    h = __import__("ctypes").c_void_p()
+    assert nb.lib().nb_mesh_create_elements(nodes.ctypes.data, 4, raw.ctypes.data, 2,
+                                            __import__("ctypes").byref(h)) == nb.NB_ERR_ARG
+    assert "type id" in nb.lib().nb_last_error().decode()
+
+
+def test_mixed_table_helpers():
+    """The mixed cubed sphere is closed and consistently oriented (every
+    directed edge once, its reverse once) and round-trips through OFF."""
+    import tempfile
+    from nihu_amd.mesh import mixed_cubed_sphere, table_faces, write_off, read_off_elements
+    nodes, table = mixed_cubed_sphere(4, seed=1)
+    tri = table[:, 0] == 32303
+    assert tri.sum() == 2 * ((6 * 16 + 2) // 3) and (~tri).sum() == 6 * 16 - (6 * 16 + 2) // 3
+    edges = {}
+    for f in table_faces(table):
+        for a in range(len(f)):
+            e = (f[a], f[(a + 1) % len(f)])
+            edges[e] = edges.get(e, 0) + 1
+    assert all(v == 1 and edges.get((b, a)) == 1 for (a, b), v in edges.items())
+    with tempfile.TemporaryDirectory() as d:
+        p = d + "/m.off"
+        write_off(p, nodes, table_faces(table))
+        n2, t2 = read_off_elements(p)
+    assert np.array_equal(n2, nodes) and np.array_equal(t2, table)

@@ -3,7 +3,9 @@ known-answer programs built against libnbem.so, on the reference's
 sphere_1_3.off mesh fixture (218 nodes, 432 triangles): transparent and
 pulsating-sphere answers in collocation, constant Galerkin and isoparametric
 Galerkin, and NiHu's exceptions; on a 384-element quadrilateral cubed sphere
-(OFF file): the collocation answers and the Burton-Miller composition."""
+(OFF file): the collocation answers and the Burton-Miller composition; the
+same on a cubed sphere whose every third quadrilateral is split into two
+triangles (one OFF file with both face types)."""
 
 import os
 import subprocess
@@ -19,10 +21,13 @@ def test_cpp_facade_known_answers(tmp_path):
     exe = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
     assert os.path.exists(exe), "build the C++ tests first (__graft_entry__.build())"
     mesh = os.path.join(ROOT, "tests", "golden", "sphere_1_3.off")
-    from nihu_amd.mesh import jittered_cubed_sphere, write_off
+    from nihu_amd.mesh import jittered_cubed_sphere, write_off, mixed_cubed_sphere, table_faces
     qmesh = str(tmp_path / "cubed_sphere_8.off")
     write_off(qmesh, *jittered_cubed_sphere(8, seed=6))
-    r = subprocess.run([exe, mesh, qmesh], capture_output=True, text=True, timeout=240)
+    mmesh = str(tmp_path / "mixed_cubed_sphere_8.off")
+    mnodes, mtable = mixed_cubed_sphere(8, seed=6)
+    write_off(mmesh, mnodes, table_faces(mtable))
+    r = subprocess.run([exe, mesh, qmesh, mmesh], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "out_of_range as expected" in r.stdout
@@ -30,5 +35,7 @@ def test_cpp_facade_known_answers(tmp_path):
     assert "logic_error as expected" in r.stdout
     for line in ("Galerkin transparent error", "Pulsating sphere relative error (Galerkin)",
                  "Galerkin isoparametric transparent error", "Quadrilateral collocation transparent error",
-                 "Quadrilateral Burton-Miller composition", "quadrilateral Galerkin: logic_error as expected"):
+                 "Quadrilateral Burton-Miller composition", "quadrilateral Galerkin: logic_error as expected",
+                 "Mixed-mesh collocation transparent error", "Mixed-mesh Burton-Miller composition",
+                 "mixed-mesh Galerkin: logic_error as expected"):
         assert line in r.stdout

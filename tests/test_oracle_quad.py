@@ -318,9 +318,10 @@ def test_nihu_policy_vs_high_fixed(orc):
 
 def test_quad_mesh_io(tmp_path):
     """Cubed sphere: closed, consistently oriented; OFF round trip with
-    4-vertex faces; NiHu element tables with the quad_1 id; mixed meshes
-    rejected."""
-    from nihu_amd.mesh import cubed_sphere, read_off, write_off, to_nihu_elements, from_nihu_elements
+    4-vertex faces; NiHu element tables with the quad_1 id; a file with both
+    face types: read_off refuses it, read_off_elements gives NiHu's table."""
+    from nihu_amd.mesh import (cubed_sphere, read_off, read_off_elements, write_off, to_nihu_elements,
+                               from_nihu_elements)
     nodes, quads = cubed_sphere(3, jitter=0.01, seed=2)
     assert quads.shape == (54, 4) and nodes.shape == (56, 3)
     edges = {}
@@ -337,5 +338,10 @@ def test_quad_mesh_io(tmp_path):
     assert np.array_equal(from_nihu_elements(el), quads)
     with open(p, "w") as fh:
         fh.write("OFF\n4 2 0\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n3 0 1 2\n4 0 1 2 3\n")
-    with pytest.raises(ValueError, match="mixing"):
+    with pytest.raises(ValueError, match="read_off_elements"):
         read_off(p)
+    n3, table = read_off_elements(p)
+    assert n3.shape == (4, 3)
+    assert table.tolist() == [[32303, 0, 1, 2, 0], [32404, 0, 1, 2, 3]]
+    with pytest.raises(ValueError, match="from_elements"):
+        from_nihu_elements(table)
