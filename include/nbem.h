@@ -129,8 +129,9 @@ int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32
 /* Upload a mesh of 4-node bilinear quadrilaterals (quad_1_elem, possibly
  * warped): quad[n_quad][4] with corners ordered around the element, the
  * normal dx/dxi x dx/deta outwards.  Collocation with constant fields
- * (nb_assemble with NB_COLLOCATION, nb_assemble_bm); the Galerkin formalism
- * and the sweep reject these meshes (NB_ERR_UNSUPPORTED).
+ * (nb_assemble with NB_COLLOCATION, nb_assemble_bm; nb_assemble_sweep runs
+ * one Burton-Miller assembly per wavenumber); the Galerkin formalism and
+ * sweep autotuning reject these meshes (NB_ERR_UNSUPPORTED).
  * Replaces create_mesh(nodes, elements, quad_1_tag()) and the quad_1 paths
  * of the collocational double integral: nihu/core/double_integral.hpp:599-673,
  * nihu/library/guiggiani_1992.hpp:138-330 (hypersingular diagonal),
@@ -147,7 +148,8 @@ int nb_mesh_create_quads(const double *xyz, int32_t n_nodes, const int32_t *quad
  * type gives the same mesh as nb_mesh_create / nb_mesh_create_quads.  With
  * both types, collocation (nb_assemble with NB_COLLOCATION, nb_assemble_bm)
  * assembles each (test type, trial type) block and places it at the element
- * ids; the Galerkin formalism, the sweep and nb_classify reject such meshes
+ * ids (nb_assemble_sweep: one Burton-Miller assembly per wavenumber); the
+ * Galerkin formalism, sweep autotuning and nb_classify reject such meshes
  * (NB_ERR_UNSUPPORTED). */
 #define NB_TRIA_1_ID 32303
 #define NB_QUAD_1_ID 32404
@@ -171,7 +173,10 @@ int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_poli
 /* Burton-Miller matrices for n_k wavenumbers on one mesh in one persistent
  * launch (alpha = -i/k each).  A and B are device c64 arrays holding n_k
  * matrices of n_tri x n_tri with leading dimension ld, matrix f at
- * A + f * k_stride elements.  launch may be NULL (autotuned). */
+ * A + f * k_stride elements.  launch may be NULL (autotuned).  Triangle
+ * meshes with a fixed 1- or 7-point rule take the one-launch path; other
+ * policies and meshes with quadrilaterals take one nb_assemble_bm pass per
+ * wavenumber. */
 int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy,
                       void *A, void *B, int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream);
 

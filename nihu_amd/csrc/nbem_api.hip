@@ -1125,7 +1125,8 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	int s;
 	if ((s = check_policy(policy))) return s;
 	if ((s = current_device_matches(mesh))) return s;
-	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "the frequency sweep is implemented for triangle meshes only");
+	if (mesh->npe != 3 && tune_block > 0)
+		return fail(NB_ERR_UNSUPPORTED, "sweep autotuning is implemented for triangle meshes only");
 	const int n = mesh->n;
 	const int nblk = tune_block > 0 ? tune_block : n;
 	if (ld < nblk || k_stride < ld * (int64_t)nblk) return fail(NB_ERR_ARG, "ld / k_stride too small for the matrices");
@@ -1136,16 +1137,22 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	std::lock_guard<std::mutex> lk(m->mu);
 	const double two_pi = 6.283185307179586476925286766559005768;
 
-	const int Q = policy->rule == NB_RULE_FIXED ? (ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
+	const bool tri = mesh->npe == 3;
+	const int Q = tri && policy->rule == NB_RULE_FIXED ?
+		(ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
 	if (Q < 0) return NB_ERR_RANGE;
-	if (policy->rule != NB_RULE_FIXED || (Q != 7 && Q != 1)) {
-		// other policies: one Burton-Miller assembly per wavenumber
+	if (!tri || policy->rule != NB_RULE_FIXED || (Q != 7 && Q != 1)) {
+		// quadrilateral meshes, meshes with both element types and the other
+		// policies: one Burton-Miller assembly per wavenumber
+		if (!tri)
+			for (const nb_mesh *p : {m, m->part[1].get()})
+				if (p && p->npe == 4 && (s = check_quad_policy(*policy))) return s;
 		for (int f = 0; f < n_k; ++f) {
 			Job jb;
 			jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION; jb.same = 1; jb.family = NB_HELMHOLTZ;
 			jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
 			jb.k = k[f]; jb.alpha = {0.0, -1.0 / k[f]}; jb.pol = *policy; jb.tile_cols = 32;
-			if ((s = run_device(jb, (float2 *)A + f * k_stride, (float2 *)B + f * k_stride, ld, NB_C64, 0, st))) return s;
+			if ((s = run_any(jb, (float2 *)A + f * k_stride, (float2 *)B + f * k_stride, ld, NB_C64, 0, st))) return s;
 		}
 		return NB_OK;
 	}
@@ -1576,7 +1583,7 @@ int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_la
 	if (n_k <= 0) return fail(NB_ERR_ARG, "n_k must be positive");
 	int s;
 	if ((s = check_policy(policy))) return s;
-	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "the frequency sweep is implemented for triangle meshes only");
+	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "sweep autotuning is implemented for triangle meshes only");
 	if (policy->rule != NB_RULE_FIXED) {
 		// the NiHu-degree policy runs the per-wavenumber kernel: only the tile matters
 		*best = {32, 1, 0};

@@ -15,8 +15,9 @@ Bars as for single-type meshes: rows to 2e-6 of their largest entry and
 1e-5 relative Frobenius.  Also: the Burton-Miller pair against its single
 operators, host outputs with accumulation, a permutation-sensitive check that
 needs no oracle (Gauss's identity for the Laplace double layer and the
-transparent point-source test on the closed surface), and the refusals
-(Galerkin, the sweep, pair classes, near-singular rules beyond d / R = 1.5).
+transparent point-source test on the closed surface), the sweep (one
+Burton-Miller assembly per wavenumber) and the refusals (Galerkin, sweep
+autotuning, pair classes, near-singular rules beyond d / R = 1.5).
 """
 
 import numpy as np
@@ -197,6 +198,23 @@ def test_mixed_refusals(nb, mixed):
         nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(rule="fixed", degree=5, near="diameter", limit=2.0))
     with pytest.raises(NotImplementedError):
         nb.classify(m, m, 0, 0)
-    A = torch.zeros((2, n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
     with pytest.raises(NotImplementedError):
-        nb.assemble_sweep(m, np.array([1.0, 2.0]), A, A.clone())
+        nb.autotune(m, 8)
+
+
+def test_mixed_sweep(nb, mixed):
+    """The sweep on a mixed mesh: one Burton-Miller assembly per wavenumber,
+    equal to nb_assemble_bm at each."""
+    import torch
+    nodes, table = mixed
+    m = nb.Mesh.from_elements(nodes, table)
+    n = m.n
+    ks = np.array([1.5, 7.0])
+    A = torch.zeros((2, n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
+    B = torch.zeros_like(A)
+    nb.assemble_sweep(m, ks, A, B, pol=nb.policy(**NIHU[0]))
+    for f, k in enumerate(ks):
+        A1, B1 = nb.colmajor(n, n), nb.colmajor(n, n)
+        nb.assemble_bm(m, float(k), A1, B1, pol=nb.policy(**NIHU[0]))
+        torch.cuda.synchronize()
+        assert torch.equal(A[f], A1) and torch.equal(B[f], B1)

@@ -9,8 +9,9 @@ FP64 oracle:
   where Guiggiani's hypersingular integrals live) to 1e-9;
 * the Burton-Miller pair and the solved surface field; the Mie series;
 * triangle test points on a quadrilateral trial mesh and the reverse;
-* the Galerkin formalism, the sweep and near-singular rules beyond NiHu's
-  d / R < 1.5 reject quadrilateral meshes.
+* the sweep: one Burton-Miller assembly per wavenumber;
+* the Galerkin formalism, sweep autotuning and near-singular rules beyond
+  NiHu's d / R < 1.5 reject quadrilateral meshes.
 """
 
 import numpy as np
@@ -167,12 +168,31 @@ def test_quad_rejections(nb, qmesh):
     out = nb.colmajor(m.n, m.n)
     with pytest.raises(Exception, match="Galerkin"):
         nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(**NIHU[0]), formalism="galerkin")
-    A = torch.zeros((2, m.n, m.n), dtype=torch.complex64, device="cuda").transpose(1, 2)
-    with pytest.raises(Exception, match="sweep"):
-        nb.assemble_sweep(m, np.array([1.0, 2.0]), A, A.clone(), pol=nb.policy(**FIXED[0]))
     # far "near-singular" pairs would need the inverse mapping far from the element
     with pytest.raises(Exception, match="near-singular rule"):
         nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(rule="fixed", degree=5, near="diameter", limit=2.0))
+
+
+def test_quad_sweep_per_wavenumber(nb, qmesh):
+    """The sweep on a quadrilateral mesh: one Burton-Miller assembly per
+    wavenumber, equal to nb_assemble_bm at each; autotuning refuses it."""
+    import torch
+    m = nb.Mesh(*qmesh)
+    n = m.n
+    ks = np.array([1.0, 2.5, 6.0])
+    A = torch.zeros((3, n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
+    B = torch.zeros_like(A)
+    pol = nb.policy(**FIXED[0])
+    nb.assemble_sweep(m, ks, A, B, pol=pol)
+    for f, k in enumerate(ks):
+        A1, B1 = nb.colmajor(n, n), nb.colmajor(n, n)
+        nb.assemble_bm(m, float(k), A1, B1, pol=pol)
+        torch.cuda.synchronize()
+        assert torch.equal(A[f], A1) and torch.equal(B[f], B1)
+    with pytest.raises(NotImplementedError):
+        nb.autotune(m, 8, pol=pol)
+    with pytest.raises(NotImplementedError):
+        nb.assemble_sweep(m, ks, A, B, pol=nb.policy(rule="fixed", degree=5, near="diameter", limit=2.0))
 
 
 def _plate(n):
