@@ -134,11 +134,12 @@ def main():
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad"], default="bm",
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "mixed"], default="bm",
                     help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
                          "serve: Poisson stream of assembly-plus-LU-solve requests; galerkin: SLP and DLP "
                          "Galerkin matrices with linear triangles on the 20480-element ellipsoid; quad: "
-                         "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere")
+                         "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere; mixed: the "
+                         "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table)")
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
     ap.add_argument("--refine", type=int, default=0,
@@ -158,6 +159,8 @@ def main():
         return run_galerkin(args)
     if args.workload == "quad":
         return run_quad(args)
+    if args.workload == "mixed":
+        return run_mixed(args)
 
     import numpy as np
     import torch
@@ -518,6 +521,105 @@ def run_quad(args):
                    "near-singular d/R < 1.5: polar SLP, Telles DLP/DLPt/HSP; diagonal Duffy, Guiggiani HSP)",
                    "pairs": {"regular": n_reg, "near": n_near, "diagonal": n}},
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
+    }
+    print(json.dumps(out))
+
+
+def run_mixed(args):
+    """Both element types in one mesh: Burton-Miller A and B on the jittered
+    unit cubed sphere with 50 x 50 quadrilaterals per face and every third
+    one split into two triangles (10000 triangles and 10000 quadrilaterals,
+    interleaved in NiHu's element table), ka = 10, NiHu's policy.  The engine
+    assembles the four (test type, trial type) blocks and scatters them to the
+    element ids."""
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import mixed_cubed_sphere
+
+    torch.cuda.set_device(0)
+    nodes, table = mixed_cubed_sphere(50, seed=0)
+    n = len(table)
+    k = 10.0
+    alpha = complex(0.0, -1.0 / k)
+    pol = nbem.policy(**nbem.NIHU_POLICY)
+    name, ncu, hbm = nbem.device_info()
+    n_tri = int((table[:, 0] == nbem.NB_TRIA_1_ID).sum())
+    log("device %s, %d CUs; mixed cubed sphere %d elements (%d triangles); k = %g" % (name, ncu, n, n_tri, k))
+    mesh = nbem.Mesh.from_elements(nodes, table)
+    A, B = nbem.colmajor(n, n), nbem.colmajor(n, n)
+    stream = torch.cuda.current_stream()
+    for w in range(args.warmup):
+        nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+        torch.cuda.synchronize()
+        log("warmup step %d done" % w)
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nbem.assemble_bm(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    ms_per_step = wall / args.steps * 1e3
+    value = 4 * n * n * args.steps / wall
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    log("timed %d steps: %.3f ms/step, %.4g entries/s; per-pass device ms (per block) %s"
+        % (args.steps, ms_per_step, value, pass_ms))
+    cpu = err = None
+    if args.cpu_rows != 0:
+        from oracle import oracle
+        nr = 256 if args.cpu_rows < 0 else args.cpu_rows
+        rows = np.sort(np.random.default_rng(123).choice(n, nr, replace=False))
+        opol = oracle.policy(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5)
+        threads = oracle.num_threads()
+        parts = [np.flatnonzero(table[:, 0] == t) for t in (nbem.NB_TRIA_1_ID, nbem.NB_QUAD_1_ID)]
+        npe = (3, 4)
+        Ar = np.zeros((nr, n), dtype=np.complex128)
+        Br = np.zeros((nr, n), dtype=np.complex128)
+        tc = time.perf_counter()
+        # the oracle's single-type operators, placed by NiHu's rules for such a
+        # mesh: a type against itself on the same mesh, against the other type
+        # as different meshes; element id = table row
+        for a_, ia in enumerate(parts):
+            sel = np.flatnonzero(np.isin(rows, ia))
+            if sel.size == 0:
+                continue
+            loc = np.searchsorted(ia, rows[sel]).astype(np.int32)
+            ea = table[ia, 1:1 + npe[a_]]
+            for b_, ib in enumerate(parts):
+                eb = table[ib, 1:1 + npe[b_]]
+                trial = None if a_ == b_ else (nodes, eb)
+                op = {(nx, ny): oracle.assemble(nodes, ea, 1, nx, ny, k=k, pol=opol, rows=loc, trial=trial,
+                                                threads=threads)
+                      for nx, ny in ((0, 0), (0, 1), (1, 0), (1, 1))}
+                Ar[np.ix_(sel, ib)] = alpha * op[(1, 1)] + op[(0, 1)]
+                Br[np.ix_(sel, ib)] = op[(0, 0)] + alpha * op[(1, 0)]
+        Ar[np.arange(nr), rows] -= 0.5
+        Br[np.arange(nr), rows] += 0.5 * alpha
+        dt = time.perf_counter() - tc
+        cpu_value = 4 * nr * n / dt
+        cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+               "sample": "%d of %d rows of A and B (4 operators, per element-type block), FP64 oracle, OpenMP "
+                         "over rows, %.1f s wall" % (nr, n, dt), "host": host_cpu()}
+        ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+        Ag, Bg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy()
+        err = {"rows": nr, "A_rel_fro": float(np.linalg.norm(Ag - Ar) / np.linalg.norm(Ar)),
+               "B_rel_fro": float(np.linalg.norm(Bg - Br) / np.linalg.norm(Br)),
+               "A_max_abs": float(np.abs(Ag - Ar).max()), "B_max_abs": float(np.abs(Bg - Br).max())}
+        log("sampled-row error vs oracle: %s; cpu %.4g entries/s on %d threads" % (err, cpu_value, threads))
+    out = {
+        "metric": "matrix entries/s, Helmholtz 3D coll. triangles + quadrilaterals, 20000 elems",
+        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64) regular pairs, fp64 near-singular and diagonal",
+        "data": "synthetic: seeded jittered cubed sphere, every third quadrilateral split",
+        "config": {"workload": "Helmholtz 3D collocation on a mesh with both element types, Burton-Miller A and B",
+                   "n_elements": n, "n_triangles": n_tri, "k": k, "mesh_seed": 0,
+                   "policy": "NiHu (distance-table degrees, near-singular d/R < 1.5)"},
+        "roofline": None, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
     print(json.dumps(out))
 

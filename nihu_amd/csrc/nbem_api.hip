@@ -523,6 +523,9 @@ struct Job {
 	nb_scalar alpha;
 	nb_policy pol;
 	int tile_cols;
+	// a block of a mesh with both element types: result row / column of each
+	// test / trial element of the block (null: its own index)
+	const int *rmap = nullptr, *cmap = nullptr;
 };
 
 // Galerkin assembly into a device buffer
@@ -714,6 +717,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	if ((s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
 	nbk::QuadArgs a;
 	std::memset(&a, 0, sizeof(a));
+	a.rmap = jb.rmap; a.cmap = jb.cmap;
 	a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = jb.same;
 	a.tile_cols = jb.tile_cols > 0 ? jb.tile_cols : 32;
 	a.row_tiles = (test->n + 255) / 256;
@@ -745,7 +749,10 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	if (grid > 0) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
-#define NB_QR(F, K, M) hipLaunchKernelGGL((quad_regular_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, a)
+		const bool mapped = jb.rmap || jb.cmap;
+#define NB_QR(F, K, M) do { \
+		if (mapped) hipLaunchKernelGGL((quad_regular_kernel<F, K, M, true>), dim3(grid), dim3(256), 0, st, a); \
+		else hipLaunchKernelGGL((quad_regular_kernel<F, K, M, false>), dim3(grid), dim3(256), 0, st, a); } while (0)
 		if (jb.mode == OUT_BM) NB_QR(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
@@ -770,6 +777,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	if ((s = device_rules(&rules))) return s;
 	nbk::NearArgs na;
 	std::memset(&na, 0, sizeof(na));
+	na.rmap = jb.rmap; na.cmap = jb.cmap;
 	na.tn = test->n; na.sn = trial->n;
 	na.tc = (const double *)test->d_c.get();
 	na.tnu = (const double *)test->d_nu64.get();
@@ -842,6 +850,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	// 1. regular pairs
 	nbk::RegArgs a;
 	std::memset(&a, 0, sizeof(a));
+	a.rmap = jb.rmap; a.cmap = jb.cmap;
 	a.n_rows = test->n; a.n_cols = trial->n; a.same_mesh = jb.same;
 	a.tile_cols = jb.tile_cols > 0 ? jb.tile_cols : 32;
 	a.row_tiles = (test->n + 255) / 256;
@@ -880,24 +889,29 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 		const bool fix7 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 7;
+		const bool mapped = jb.rmap || jb.cmap;
+#define NB_RG(F, K, M, Q) do { \
+		if (mapped) hipLaunchKernelGGL((regular_kernel<F, K, M, Q, true>), dim3(grid), dim3(256), 0, st, a); \
+		else hipLaunchKernelGGL((regular_kernel<F, K, M, Q, false>), dim3(grid), dim3(256), 0, st, a); } while (0)
 		if (jb.mode == OUT_BM) {
-			if (fix7) hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM, 7>), dim3(grid), dim3(256), 0, st, a);
-			else hipLaunchKernelGGL((regular_kernel<HELM, ALL4, OUT_BM, 0>), dim3(grid), dim3(256), 0, st, a);
+			if (fix7) NB_RG(HELM, ALL4, OUT_BM, 7);
+			else NB_RG(HELM, ALL4, OUT_BM, 0);
 		} else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
-			case SLP: hipLaunchKernelGGL((regular_kernel<HELM, SLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLP: hipLaunchKernelGGL((regular_kernel<HELM, DLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLPT: hipLaunchKernelGGL((regular_kernel<HELM, DLPT, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
-			default: hipLaunchKernelGGL((regular_kernel<HELM, HSP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			case SLP: NB_RG(HELM, SLP, OUT_SINGLE, 0); break;
+			case DLP: NB_RG(HELM, DLP, OUT_SINGLE, 0); break;
+			case DLPT: NB_RG(HELM, DLPT, OUT_SINGLE, 0); break;
+			default: NB_RG(HELM, HSP, OUT_SINGLE, 0); break;
 			}
 		} else {
 			switch (jb.km) {
-			case SLP: hipLaunchKernelGGL((regular_kernel<LAP, SLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLP: hipLaunchKernelGGL((regular_kernel<LAP, DLP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
-			case DLPT: hipLaunchKernelGGL((regular_kernel<LAP, DLPT, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
-			default: hipLaunchKernelGGL((regular_kernel<LAP, HSP, OUT_SINGLE, 0>), dim3(grid), dim3(256), 0, st, a); break;
+			case SLP: NB_RG(LAP, SLP, OUT_SINGLE, 0); break;
+			case DLP: NB_RG(LAP, DLP, OUT_SINGLE, 0); break;
+			case DLPT: NB_RG(LAP, DLPT, OUT_SINGLE, 0); break;
+			default: NB_RG(LAP, HSP, OUT_SINGLE, 0); break;
 			}
 		}
+#undef NB_RG
 		NB_HIP(hipGetLastError());
 		prof_end(pe, st);
 	}
@@ -907,6 +921,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	if ((s = device_rules(&rules))) return s;
 	nbk::NearArgs na;
 	std::memset(&na, 0, sizeof(na));
+	na.rmap = jb.rmap; na.cmap = jb.cmap;
 	na.tn = test->n; na.sn = trial->n;
 	na.tc = (const double *)test->d_c.get();
 	na.tnu = (const double *)test->d_nu64.get();
@@ -1007,47 +1022,32 @@ int mesh_parts(const nb_mesh *m, nb_mesh *parts[2], const int *ids[2])
 
 // Collocation with a mesh that holds both element types (NiHu's mesh over
 // tria_1 and quad_1, element ids in table order).  Each (test part, trial
-// part) block runs the single-type passes into a scratch matrix, which is then
-// scattered into the result at the parts' element ids.  A block of one part
-// against itself on the same mesh keeps the diagonal pass; the other blocks
-// pair different elements, where NiHu's same-mesh loop finds no element match
-// and does what its different-mesh loop does (near check, else regular:
-// nihu/core/double_integral.hpp:599-673).  The blocks run one after another on
-// the stream and share the scratch matrix.
+// part) block runs the single-type passes, whose stores go through the parts'
+// element ids to the result's rows and columns.  A block of one part against
+// itself on the same mesh keeps the diagonal pass; the other blocks pair
+// different elements, where NiHu's same-mesh loop finds no element match and
+// does what its different-mesh loop does (near check, else regular:
+// nihu/core/double_integral.hpp:599-673).  The blocks write disjoint entries.
 int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
 	nb_mesh *tp[2], *sp[2];
 	const int *tid[2], *sid[2];
 	const int nt = mesh_parts(jb.test, tp, tid), ns = mesh_parts(jb.trial, sp, sid);
-	const int eb = dtype_bytes(dtype);
 	int s;
 	// refuse before any block has run
 	for (int b = 0; b < ns; ++b)
 		if (sp[b]->npe == 4 && (s = check_quad_policy(jb.pol))) return s;
-	size_t most = 0;
-	for (int a = 0; a < nt; ++a)
-		for (int b = 0; b < ns; ++b) most = std::max(most, (size_t)tp[a]->n * sp[b]->n * eb);
-	DevPtr t0, t1;
-	if ((s = dev_alloc(t0, most))) return s;
-	if (out1 && (s = dev_alloc(t1, most))) return s;
 	for (int a = 0; a < nt; ++a)
 		for (int b = 0; b < ns; ++b) {
 			Job sub = jb;
 			sub.test = tp[a];
 			sub.trial = sp[b];
 			sub.same = jb.same && tp[a] == sp[b];
+			sub.rmap = tid[a];
+			sub.cmap = sid[b];
 			// the parts are private to their mesh, whose lock the caller holds
-			if ((s = run_device(sub, t0.get(), out1 ? t1.get() : nullptr, tp[a]->n, dtype, 0, st))) return s;
-			const dim3 grid((unsigned)((tp[a]->n + 255) / 256), (unsigned)std::min(sp[b]->n, 65535));
-			hipLaunchKernelGGL(nbk::scatter_block_kernel, grid, dim3(256), 0, st, (const void *)t0.get(), tp[a]->n,
-				sp[b]->n, tid[a], sid[b], out0, (long long)ld, dtype, accumulate);
-			if (out1)
-				hipLaunchKernelGGL(nbk::scatter_block_kernel, grid, dim3(256), 0, st, (const void *)t1.get(),
-					tp[a]->n, sp[b]->n, tid[a], sid[b], out1, (long long)ld, dtype, accumulate);
-			NB_HIP(hipGetLastError());
+			if ((s = run_device(sub, out0, out1, ld, dtype, accumulate, st))) return s;
 		}
-	// the scratch matrices are freed on return: wait for the last scatter
-	NB_HIP(hipStreamSynchronize(st));
 	return NB_OK;
 }
 

@@ -53,6 +53,7 @@ static_assert(sizeof(TrialRec) == 160, "TrialRec layout");
 
 struct RegArgs {
 	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	const int *rmap, *cmap;      // result row / column of each test / trial element (null: its own index)
 	int tn, sn;                  // element counts of the test / trial mesh (SoA strides)
 	const double *tc;            // test centres [3][tn]
 	const float *tnu;            // test unit normals [3][tn]
@@ -382,8 +383,10 @@ __device__ __forceinline__ int table_degree_rec(double d2, const double (&t)[NTH
 }
 
 // FIXQ > 0: fixed rule with FIXQ points (the caller guarantees a.fixed_degree
-// has that many); 0: any policy.
-template <int FAM, int KM, int MODE, int FIXQ>
+// has that many); 0: any policy.  MAP: stores go through a.rmap / a.cmap (a
+// block of a mesh with both element types); without it the store address is
+// the plain j * ld + i of the single-type meshes.
+template <int FAM, int KM, int MODE, int FIXQ, bool MAP>
 __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 {
 	const int tile_r = blockIdx.x % a.row_tiles;
@@ -391,6 +394,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 	const int i = tile_r * 256 + threadIdx.x;
 	const bool row_ok = i < a.n_rows;
 	const int ii = row_ok ? i : a.n_rows - 1;
+	const int orow = MAP && a.rmap ? a.rmap[ii] : i;
 	const double cix = a.tc[ii], ciy = a.tc[a.tn + ii], ciz = a.tc[2 * a.tn + ii];
 	PairCtx p;
 	p.nxx = p.nxy = p.nxz = 0.0f;
@@ -492,7 +496,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			const float kk = FAM == HELM ? a.k * a.k : 0.0f;
 			const float hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x), hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
 			K[3] = make_float2(nn * sq.x - av * hr, nn * sq.y - av * hi);
-			const long long idx = (long long)j * a.ld + i;
+			const long long idx = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld + orow;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
 				const float re = a.alpha.x * K[kk].x - a.alpha.y * K[kk].y;
@@ -529,6 +533,7 @@ struct Rules {
 
 struct NearArgs {
 	int n_pairs;
+	const int *rmap, *cmap;      // result row / column of each test / trial element (null: its own index)
 	const int *pi, *pj;          // pair list (near pass); diagonal pass uses (p, p)
 	int tn, sn;
 	const double *tc;            // test centres [3][tn]
@@ -985,7 +990,7 @@ __global__ void __launch_bounds__(64) fp64_pass_kernel(NearArgs a)
 		for (int t = 0; t < 4; ++t) s[t].x = st[t];
 		if (FAM == HELM) near_dynamic<KM>(V, x0, nx, ny, J, a.k, R, s);
 	}
-	const long long idx = (long long)j * a.ld + i;
+	const long long idx = (long long)(a.cmap ? a.cmap[j] : j) * a.ld + (a.rmap ? a.rmap[i] : i);
 	if (MODE == OUT_SINGLE) {
 		const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
 		const double re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
@@ -1081,45 +1086,6 @@ __global__ void zero_kernel(void *out, long long ld, long long rows, long long c
 			// elements of 4, 8 or 16 bytes: every column starts on 4 bytes
 			uint32_t *q = (uint32_t *)base;
 			for (long long w = t0; w < col_bytes / 4; w += dt) q[w] = 0u;
-		}
-	}
-}
-
-// one block of a mesh with both element types: the dense rows x cols block
-// src (column-major, ld = rows) goes to rows rmap[i] and columns cmap[j] of
-// the result (null map: the identity), stored or added; rows along x,
-// columns along the grid's y (the kernel strides beyond 65535)
-template <class T>
-__device__ __forceinline__ void scatter_one(const void *src, long long s, void *dst, long long d, int accumulate)
-{
-	const T v = ((const T *)src)[s];
-	T *p = (T *)dst + d;
-	*p = accumulate ? *p + v : v;
-}
-
-__global__ void __launch_bounds__(256) scatter_block_kernel(const void *src, int rows, int cols, const int *rmap,
-	const int *cmap, void *dst, long long ld, int dtype, int accumulate)
-{
-	const int i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= rows) return;
-	const long long r = rmap ? rmap[i] : i;
-	for (int j = blockIdx.y; j < cols; j += gridDim.y) {
-		const long long s = (long long)j * rows + i, d = (long long)(cmap ? cmap[j] : j) * ld + r;
-		switch (dtype) {
-		case DT_C64: {
-			const float2 v = ((const float2 *)src)[s];
-			float2 *p = (float2 *)dst + d;
-			*p = accumulate ? make_float2(p->x + v.x, p->y + v.y) : v;
-			break;
-		}
-		case DT_C128: {
-			const double2 v = ((const double2 *)src)[s];
-			double2 *p = (double2 *)dst + d;
-			*p = accumulate ? make_double2(p->x + v.x, p->y + v.y) : v;
-			break;
-		}
-		case DT_F32: scatter_one<float>(src, s, dst, d, accumulate); break;
-		default: scatter_one<double>(src, s, dst, d, accumulate); break;
 		}
 	}
 }

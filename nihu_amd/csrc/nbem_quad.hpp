@@ -46,6 +46,7 @@ struct QuadPt { float ox, oy, oz, w, nx, ny, nz, oo; };
 
 struct QuadArgs {
 	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	const int *rmap, *cmap;      // result row / column of each test / trial element (null: its own index)
 	int tn, sn;
 	const double *tc;            // test collocation points [3][tn]
 	const float *tnu;            // test unit normals [3][tn]
@@ -176,7 +177,9 @@ __device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, con
 // Gauss order per axis of a Gauss square rule of a degree
 __device__ __forceinline__ int gauss_order(int degree) { return degree / 2 + 1; }
 
-template <int FAM, int KM, int MODE>
+// MAP: stores through a.rmap / a.cmap (a block of a mesh with both element
+// types); without it the plain j * ld + i
+template <int FAM, int KM, int MODE, bool MAP>
 __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 {
 	const int tile_r = blockIdx.x % a.row_tiles;
@@ -184,6 +187,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 	const int i = tile_r * 256 + threadIdx.x;
 	const bool row_ok = i < a.n_rows;
 	const int ii = row_ok ? i : a.n_rows - 1;
+	const int orow = MAP && a.rmap ? a.rmap[ii] : i;
 	const double cix = a.tc[ii], ciy = a.tc[a.tn + ii], ciz = a.tc[2 * a.tn + ii];
 	QuadCtx p;
 	p.nxx = p.nxy = p.nxz = 0.0f;
@@ -254,7 +258,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 			}
 		}
 		if (act) {
-			const long long idx = (long long)j * a.ld + i;
+			const long long idx = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld + orow;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
 				const float re = a.alpha.x * K[kk].x - a.alpha.y * K[kk].y;
@@ -594,7 +598,7 @@ __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const R
 template <int KM, int MODE, bool DIAG>
 __device__ __forceinline__ void quad_store64(const NearArgs &a, int i, int j, const double2 (&s)[4])
 {
-	const long long idx = (long long)j * a.ld + i;
+	const long long idx = (long long)(a.cmap ? a.cmap[j] : j) * a.ld + (a.rmap ? a.rmap[i] : i);
 	if (MODE == OUT_SINGLE) {
 		const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
 		const double re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
