@@ -282,6 +282,9 @@ struct nb_mesh {
 	// and, per part, the element id (table row) of each of its elements
 	std::unique_ptr<nb_mesh> part[2];
 	DevPtr d_ids[2];
+	// every element's collocation point and normal in table order (the test
+	// side of the regular pass over all rows)
+	std::unique_ptr<nb_mesh> view;
 	std::mutex mu;
 	int64_t bytes = 0;
 };
@@ -526,6 +529,8 @@ struct Job {
 	// a block of a mesh with both element types: result row / column of each
 	// test / trial element of the block (null: its own index)
 	const int *rmap = nullptr, *cmap = nullptr;
+	// passes to run: 1 regular pairs, 2 near-singular and diagonal pairs
+	int passes = 3;
 };
 
 // Galerkin assembly into a device buffer
@@ -714,7 +719,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 			if ((s = ensure_qq(trial, N))) return s;
 	}
 	NearList *nl = nullptr;
-	if ((s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
+	if ((jb.passes & 2) && (s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
 	nbk::QuadArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.rmap = jb.rmap; a.cmap = jb.cmap;
@@ -737,7 +742,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 
 	hipStream_t side = st;
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-	const bool has_fp64 = nl->n > 0 || jb.same;
+	const bool has_fp64 = (jb.passes & 2) && (nl->n > 0 || jb.same);
 	if (has_fp64) {
 		if ((s = side_stream(&side))) return s;
 		if (!side) side = st;
@@ -746,7 +751,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 		NB_HIP(hipEventRecord(ev_fork, st));
 		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
 	}
-	if (grid > 0) {
+	if (grid > 0 && (jb.passes & 1)) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 		const bool mapped = jb.rmap || jb.cmap;
@@ -773,6 +778,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 		NB_HIP(hipGetLastError());
 		prof_end(pe, st);
 	}
+	if (!(jb.passes & 2)) return NB_OK;
 	const nbk::Rules *rules = nullptr;
 	if ((s = device_rules(&rules))) return s;
 	nbk::NearArgs na;
@@ -845,7 +851,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	for (int d : degs)
 		if ((s = ensure_quad(trial, d))) return s;
 	NearList *nl = nullptr;
-	if ((s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
+	if ((jb.passes & 2) && (s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
 
 	// 1. regular pairs
 	nbk::RegArgs a;
@@ -876,7 +882,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	// fork the FP64 passes onto the side stream
 	hipStream_t side = st;
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-	const bool has_fp64 = nl->n > 0 || jb.same;
+	const bool has_fp64 = (jb.passes & 2) && (nl->n > 0 || jb.same);
 	if (has_fp64) {
 		if ((s = side_stream(&side))) return s;
 		if (!side) side = st;
@@ -885,7 +891,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		NB_HIP(hipEventRecord(ev_fork, st));
 		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
 	}
-	if (grid > 0) {
+	if (grid > 0 && (jb.passes & 1)) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 		const bool fix7 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 7;
@@ -916,6 +922,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		prof_end(pe, st);
 	}
 
+	if (!(jb.passes & 2)) return NB_OK;
 	// 2./3. near-singular and diagonal pairs in FP64
 	const nbk::Rules *rules = nullptr;
 	if ((s = device_rules(&rules))) return s;
@@ -1037,17 +1044,32 @@ int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 	// refuse before any block has run
 	for (int b = 0; b < ns; ++b)
 		if (sp[b]->npe == 4 && (s = check_quad_policy(jb.pol))) return s;
-	for (int a = 0; a < nt; ++a)
-		for (int b = 0; b < ns; ++b) {
+	// a test mesh with both types: the regular pairs of a trial part against
+	// all rows at once (the test view, table order: each wave stores 64
+	// consecutive rows; the diagonal is the row equal to the trial element's
+	// id), then the near-singular and diagonal pairs per test part
+	const bool all_rows = jb.test->npe == 0;
+	for (int b = 0; b < ns; ++b) {
+		if (all_rows) {
+			Job sub = jb;
+			sub.test = jb.test->view.get();
+			sub.trial = sp[b];
+			sub.cmap = sid[b];
+			sub.passes = 1;
+			if ((s = run_device(sub, out0, out1, ld, dtype, accumulate, st))) return s;
+		}
+		for (int a = 0; a < nt; ++a) {
 			Job sub = jb;
 			sub.test = tp[a];
 			sub.trial = sp[b];
 			sub.same = jb.same && tp[a] == sp[b];
 			sub.rmap = tid[a];
 			sub.cmap = sid[b];
+			sub.passes = all_rows ? 2 : 3;
 			// the parts are private to their mesh, whose lock the caller holds
 			if ((s = run_device(sub, out0, out1, ld, dtype, accumulate, st))) return s;
 		}
+	}
 	return NB_OK;
 }
 
@@ -1444,6 +1466,26 @@ int nb_mesh_create_elements(const double *xyz, int32_t n_nodes, const int32_t *e
 	if ((s = nb_mesh_create_quads(xyz, n_nodes, conn[1].data(), (int32_t)ids[1].size(), &h))) return s;
 	m->part[1].reset(h);
 	if ((s = upload(m->d_ids[0], ids[0])) || (s = upload(m->d_ids[1], ids[1]))) return s;
+	{
+		std::unique_ptr<nb_mesh> v(new nb_mesh());
+		v->uid = next_uid++;
+		v->device = m->device;
+		v->n_nodes = n_nodes;
+		v->n = n_elem;
+		v->c.resize(3 * (size_t)n_elem);
+		v->nu.resize(3 * (size_t)n_elem);
+		for (int p = 0; p < 2; ++p) {
+			const nb_mesh *q = m->part[p].get();
+			for (int e = 0; e < q->n; ++e)
+				for (int d = 0; d < 3; ++d) {
+					v->c[(size_t)d * n_elem + ids[p][e]] = q->c[(size_t)d * q->n + e];
+					v->nu[(size_t)d * n_elem + ids[p][e]] = q->nu[(size_t)d * q->n + e];
+				}
+		}
+		std::vector<float> nuf(v->nu.begin(), v->nu.end());
+		if ((s = upload(v->d_c, v->c)) || (s = upload(v->d_nu64, v->nu)) || (s = upload(v->d_nuf, nuf))) return s;
+		m->view = std::move(v);
+	}
 	m->tri.assign(elem, elem + 5 * (size_t)n_elem);
 	m->bytes = m->part[0]->bytes + m->part[1]->bytes + 4 * (int64_t)n_elem;
 	*out = m.release();

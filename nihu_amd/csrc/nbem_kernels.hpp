@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 		const Head h = nxt;
 		const double cjx = h.cx, cjy = h.cy, cjz = h.cz;
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
-		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == j);
+		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == (MAP && a.cmap ? a.cmap[j] : j));
 		nxt = head(j + 1 < j1 ? j + 1 : j);
 		if (__ballot(act) == 0) continue;
 		const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);

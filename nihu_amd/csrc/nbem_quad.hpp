@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 		const Head h = nxt;
 		const double cjx = h.cx, cjy = h.cy, cjz = h.cz;
 		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
-		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == j);
+		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == (MAP && a.cmap ? a.cmap[j] : j));
 		nxt = head(j + 1 < j1 ? j + 1 : j);
 		if (__ballot(act) == 0) continue;
 		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
