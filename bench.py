@@ -225,7 +225,7 @@ def main():
         if pts is not None:
             flops = pts * FLOPS_PER_POINT_BM
             ach = flops / (reg_ms * 1e-3) / 1e12
-            roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            roofline = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic(args.traffic_file),
                         "engine": "VALU fp32 (no MFMA; packed fp32 disabled at build, so half the peak is the "
                                   "issue limit)",
@@ -383,7 +383,7 @@ def run_galerkin(args):
     reg_ms_pair = 2 * pass_ms["regular"]       # SLP and DLP launches per step
     flops = n_reg * 36 * (FLOPS_PER_POINTPAIR_GAL["SLP"] + FLOPS_PER_POINTPAIR_GAL["DLP"])
     ach = flops / (reg_ms_pair * 1e-3) / 1e12 if reg_ms_pair > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
                 "engine": "VALU fp32 (no MFMA; packed fp32 disabled at build)",
                 "kernel": "galerkin_regular_kernel<HELM,{SLP,DLP},3,6>", "kernel_ms": round(reg_ms_pair, 4),

@@ -193,6 +193,12 @@ int nb_set_launch(const nb_launch *launch);
 
 int nb_synchronize(void *stream);
 
+/* Test support (no NiHu counterpart): evaluate the engine's own FP64 device
+ * routines of the near-singular and singular passes on n device doubles x
+ * into y[2 n]: fn 0 sin and cos (y[2i], y[2i+1]; Cody-Waite reduction and
+ * Taylor series), 1 natural logarithm, 2 1/sqrt, 3 1/x (y[2i]; y[2i+1] = 0). */
+int nb_eval_math(int32_t fn, const double *x, double *y, int64_t n, void *stream);
+
 /* Per-pass device timing with HIP events recorded on the launch stream
  * (pass 0 regular pairs, 1 near-singular pairs, 2 diagonal, 3 sweep).
  * nb_profile(1) starts recording, nb_profile_read sums the elapsed times

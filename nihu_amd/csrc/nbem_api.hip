@@ -269,7 +269,9 @@ struct nb_mesh {
 	DevPtr d_qt[nbk::NSLOT];
 	int qn[nbk::NSLOT] = {0};
 	DevPtr d_qq[nbk::QN_MAX + 1];              // quadrilaterals: Gauss square tables by order
-	std::vector<std::unique_ptr<NearList> > near_cache;
+	// near-pair lists per (test mesh, near policy), oldest first; callers hold
+	// a list by shared_ptr, so evicting it cannot free one in use
+	std::vector<std::shared_ptr<NearList> > near_cache;
 	std::vector<int32_t> tri;                  // element-major node ids (host)
 	DevPtr d_tri;                              // node ids [3][n]
 	// Galerkin, nodal dofs: element order, wave tiles, column-slot schedule
@@ -285,6 +287,8 @@ struct nb_mesh {
 	// every element's collocation point and normal in table order (the test
 	// side of the regular pass over all rows)
 	std::unique_ptr<nb_mesh> view;
+	// quadrilateral near pass: count of pairs without an inverse image
+	DevPtr d_status;
 	std::mutex mu;
 	int64_t bytes = 0;
 };
@@ -377,14 +381,14 @@ void policy_degrees(const nb_policy &p, int km, int field_complexity, std::vecto
 }
 
 int ensure_near_list(nb_mesh *trial, const nb_mesh *test, int same, const nb_policy &pol, hipStream_t st,
-	NearList **out)
+	std::shared_ptr<NearList> *out)
 {
 	for (auto &nl : trial->near_cache)
 		if (nl->test_uid == test->uid && nl->same == same && nl->mode == pol.near_mode && nl->limit == pol.near_limit) {
-			*out = nl.get();
+			*out = nl;
 			return NB_OK;
 		}
-	std::unique_ptr<NearList> nl(new NearList());
+	std::shared_ptr<NearList> nl(new NearList());
 	nl->test_uid = test->uid; nl->same = same; nl->mode = pol.near_mode; nl->limit = pol.near_limit; nl->n = 0;
 	if (pol.near_mode != NB_NEAR_NONE) {
 		nbk::ListArgs a;
@@ -425,9 +429,12 @@ int ensure_near_list(nb_mesh *trial, const nb_mesh *test, int same, const nb_pol
 		NB_HIP(hipStreamSynchronize(st));   // counts/offsets are freed on return
 		trial->bytes += (int64_t)nl->n * 8;
 	}
-	if (trial->near_cache.size() >= 8) trial->near_cache.erase(trial->near_cache.begin());
-	trial->near_cache.push_back(std::move(nl));
-	*out = trial->near_cache.back().get();
+	if (trial->near_cache.size() >= 8) {
+		trial->bytes -= (int64_t)trial->near_cache.front()->n * 8;
+		trial->near_cache.erase(trial->near_cache.begin());
+	}
+	trial->near_cache.push_back(nl);
+	*out = std::move(nl);
 	return NB_OK;
 }
 
@@ -702,6 +709,18 @@ int check_quad_policy(const nb_policy &p)
 	return NB_OK;
 }
 
+// Newton step limit of the quadrilateral inverse mapping: NiHu's 100
+// (nihu/core/inverse_mapping.hpp:37-118).  NBEM_INVERSE_MAPPING_STEPS
+// lowers it -- a test hook that makes the mapping fail on purpose, so that
+// the error path runs with every kernel finishing normally.
+int inverse_mapping_steps()
+{
+	const char *e = std::getenv("NBEM_INVERSE_MAPPING_STEPS");
+	if (!e || !e[0]) return 100;
+	const int v = std::atoi(e);
+	return v < 0 ? 0 : (v > 100 ? 100 : v);
+}
+
 int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
 	nb_mesh *trial = jb.trial;
@@ -718,7 +737,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 		for (int N = 1; N <= 6; ++N)
 			if ((s = ensure_qq(trial, N))) return s;
 	}
-	NearList *nl = nullptr;
+	std::shared_ptr<NearList> nl;
 	if ((jb.passes & 2) && (s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
 	nbk::QuadArgs a;
 	std::memset(&a, 0, sizeof(a));
@@ -794,6 +813,17 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	na.alpha = make_double2(jb.alpha.re, jb.alpha.im);
 	na.out0 = out0; na.out1 = out1; na.ld = ld; na.dtype = dtype; na.accumulate = accumulate;
 	na.rules = rules;
+	// the near pass counts pairs whose collocation point has no image on the
+	// element (NiHu throws "Could not perform inverse mapping"); the count lives
+	// with the trial mesh, whose lock this call holds until it has read it
+	const bool check_images = nl->n > 0;
+	na.inv_iter = inverse_mapping_steps();
+	if (check_images) {
+		if (!trial->d_status && (s = dev_alloc(trial->d_status, 256))) return s;
+		na.status = (int *)trial->d_status.get();
+		launch_zero(na.status, 1, 1, 1, 4, side);
+		NB_HIP(hipGetLastError());
+	}
 	for (int pass = 0; pass < 2; ++pass) {
 		const bool diag = pass == 1;
 		if (diag && !jb.same) continue;
@@ -835,6 +865,14 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 		NB_HIP(hipEventDestroy(ev_fork));
 		NB_HIP(hipEventDestroy(ev_join));
 	}
+	if (check_images) {
+		int failed = 0;
+		NB_HIP(hipMemcpyAsync(&failed, na.status, sizeof(int), hipMemcpyDeviceToHost, st));
+		NB_HIP(hipStreamSynchronize(st));
+		if (failed)
+			return fail(NB_ERR_RUNTIME, "Could not perform inverse mapping: " + std::to_string(failed) +
+				" near-singular quadrilateral pair(s) (their entries hold NaN)");
+	}
 	return NB_OK;
 }
 
@@ -850,7 +888,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	policy_degrees(jb.pol, jb.km, 0, degs);
 	for (int d : degs)
 		if ((s = ensure_quad(trial, d))) return s;
-	NearList *nl = nullptr;
+	std::shared_ptr<NearList> nl;
 	if ((jb.passes & 2) && (s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
 
 	// 1. regular pairs
@@ -1178,10 +1216,9 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 		}
 		return NB_OK;
 	}
-	if ((s = ensure_thresholds(m, policy->near_mode, policy->near_limit))) return s;
-	NearList *nl = nullptr;
-	if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
-
+	// the launch shape first: autotuning runs its own sweeps, which take the
+	// mesh lock, so it is released meanwhile -- before any per-mesh cache is
+	// fetched below, so that nothing fetched can change under this call
 	nb_launch lc = {32, 4, 4};
 	if (launch && launch->tile_cols > 0) lc = *launch;
 	else {
@@ -1202,6 +1239,10 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	}
 	if (lc.k_batch != 2 && lc.k_batch != 4 && lc.k_batch != 8) return fail(NB_ERR_ARG, "k_batch must be 2, 4 or 8");
 	if (lc.persistent <= 0 || lc.persistent > 32) return fail(NB_ERR_ARG, "persistent workgroups per CU must be 1..32");
+	if ((s = ensure_quad(m, policy->fixed_degree))) return s;
+	if ((s = ensure_thresholds(m, policy->near_mode, policy->near_limit))) return s;
+	std::shared_ptr<NearList> nl;
+	if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
 
 	// per-wavenumber constants
 	std::vector<double> krev(n_k), kd(n_k);
@@ -1712,6 +1753,19 @@ int nb_profile_read(double *ms, int64_t *launches, int32_t reset)
 		if (launches) launches[p] = g_prof_n[p];
 		if (reset) { g_prof_ms[p] = 0.0; g_prof_n[p] = 0; }
 	}
+	return NB_OK;
+}
+
+int nb_eval_math(int32_t fn, const double *x, double *y, int64_t n, void *stream)
+{
+	if (fn < 0 || fn > 3) return fail(NB_ERR_ARG, "unknown math routine");
+	if (n < 0 || (n > 0 && (!x || !y))) return fail(NB_ERR_ARG, "null argument");
+	if (n == 0) return NB_OK;
+	hipStream_t st = (hipStream_t)stream;
+	const long long g = (n + 255) / 256;
+	if (g > 0x7fffffffLL) return fail(NB_ERR_RANGE, "too many values");
+	hipLaunchKernelGGL(nbk::math_kernel, dim3((unsigned)g), dim3(256), 0, st, fn, x, y, (long long)n);
+	NB_HIP(hipGetLastError());
 	return NB_OK;
 }
 

@@ -547,6 +547,11 @@ struct NearArgs {
 	long long ld;
 	int dtype, accumulate;
 	const Rules *rules;
+	// quadrilateral near pass: count of pairs whose collocation point has no
+	// image on the element (the inverse mapping did not converge); null: none
+	// kept; and the Newton step limit of that mapping (NiHu: 100)
+	int *status;
+	int inv_iter;
 };
 
 struct D3 { double x, y, z; };
@@ -1062,6 +1067,25 @@ __global__ void __launch_bounds__(256) classify_kernel(ClassArgs a)
 	else if (a.order == 2) c = a.field_complexity + table_degree<2>(d2, a.thr, a.sn, j);
 	else c = a.field_complexity + table_degree<3>(d2, a.thr, a.sn, j);
 	a.cls[t] = (int8_t)c;
+}
+
+// The FP64 routines of the near-singular and singular passes, evaluated
+// elementwise (fn 0: sincos64 -> (sin, cos); 1: log64; 2: rsqrt64; 3: rcp64;
+// one value each in y[2 i], y[2 i + 1] = 0 for the last three).
+__global__ void math_kernel(int fn, const double *x, double *y, long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const double v = x[i];
+	double a = 0.0, b = 0.0;
+	switch (fn) {
+	case 0: sincos64(v, &a, &b); break;
+	case 1: a = log64(v); break;
+	case 2: a = rsqrt64(v); break;
+	default: a = rcp64(v); break;
+	}
+	y[2 * i] = a;
+	y[2 * i + 1] = b;
 }
 
 __global__ void identity_kernel(void *out, long long ld, int n, int dtype, double are, double aim)

@@ -443,12 +443,13 @@ __device__ double2 quad_guiggiani(const Quad &Q, double k, const Rules &R)
 }
 
 // Image (xi, eta) and normal offset zeta of x0: x(xi) + zeta n / sqrt|n| = x0
-// by Newton-Raphson from 0 (at most 100 steps, |residual| < tol).
-__device__ bool quad_inverse(const Quad &Q, D3 x0, double tol, double &xi, double &eta, double &zeta)
+// by Newton-Raphson from 0 (at most max_iter steps -- NiHu's 100 -- with
+// |residual| < tol).
+__device__ bool quad_inverse(const Quad &Q, D3 x0, double tol, int max_iter, double &xi, double &eta, double &zeta)
 {
 	double r0 = 0.0, r1 = 0.0, r2 = 0.0;
 	const D3 dxe = quad_dxe(Q);
-	for (int it = 0; it < 100; ++it) {
+	for (int it = 0; it < max_iter; ++it) {
 		D3 a, b;
 		quad_dx(Q, r0, r1, a, b);
 		D3 n = d3cross(a, b);
@@ -486,10 +487,10 @@ __device__ bool quad_inverse(const Quad &Q, D3 x0, double tol, double &xi, doubl
 // dynamic point of the pair (the caller sums the lanes).
 template <int FAM, int MASK>
 __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, const double *grx, const double *grw,
-	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4], int lane, int nl)
+	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4], int lane, int nl, int max_iter)
 {
 	double xi0, eta0, zeta;
-	if (!quad_inverse(Q, x0, 1e-5, xi0, eta0, zeta)) return false;
+	if (!quad_inverse(Q, x0, 1e-5, max_iter, xi0, eta0, zeta)) return false;
 	// the tangent parallelogram at xi0: x_lin(xi) = xl0 + a (xi - xi0) + b (eta - eta0)
 	D3 a, b;
 	quad_dx(Q, xi0, eta0, a, b);
@@ -568,10 +569,10 @@ __device__ __forceinline__ void telles_coeffs(double e, double rb, double &ca, d
 
 template <int FAM, int MASK>
 __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4], int lane,
-	int nl)
+	int nl, int max_iter)
 {
 	double xi0, eta0, zeta;
-	if (!quad_inverse(Q, x0, 1e-6, xi0, eta0, zeta)) return false;
+	if (!quad_inverse(Q, x0, 1e-6, max_iter, xi0, eta0, zeta)) return false;
 	double rb;
 	if (zeta < 0.05) rb = 0.0;
 	else if (zeta <= 1.3) rb = 0.85 + 0.24 * log(zeta);
@@ -664,15 +665,18 @@ __global__ void __launch_bounds__(64) quad_near_kernel(NearArgs a)
 	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
 	bool ok = true;
 	if (FAM == HELM) {
-		if (KM & SLP) ok = quad_near_polar<FAM, SLP>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s, lane, QLPP);
+		if (KM & SLP)
+			ok = quad_near_polar<FAM, SLP>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s, lane, QLPP,
+				a.inv_iter);
 		if (ok && (KM & (DLP | DLPT | HSP)))
-			ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP)>(Q, x0, nx, a.k, R, s, lane, QLPP);
+			ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP)>(Q, x0, nx, a.k, R, s, lane, QLPP, a.inv_iter);
 	} else {
 		if (KM & (SLP | DLP | DLPT))
 			ok = quad_near_polar<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, 0.0, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s,
-				lane, QLPP);
+				lane, QLPP, a.inv_iter);
 		if (ok && (KM & HSP))
-			ok = quad_near_polar<FAM, HSP>(Q, x0, nx, 0.0, 51, R.gl51x, R.gl51w, 51, R.gl51x, R.gl51w, R, s, lane, QLPP);
+			ok = quad_near_polar<FAM, HSP>(Q, x0, nx, 0.0, 51, R.gl51x, R.gl51w, 51, R.gl51x, R.gl51w, R, s, lane, QLPP,
+				a.inv_iter);
 	}
 	for (int off = QLPP / 2; off >= 1; off >>= 1)
 		for (int t = 0; t < 4; ++t) {
@@ -680,9 +684,11 @@ __global__ void __launch_bounds__(64) quad_near_kernel(NearArgs a)
 			s[t].y += __shfl_xor(s[t].y, off, QLPP);
 		}
 	if (!ok) {
-		// the inverse mapping did not converge: NaN marks the entry
+		// the inverse mapping did not converge (NiHu throws): NaN marks the
+		// entry and the pair is counted, so that the call reports an error
 		const double nan = __builtin_nan("");
 		for (int t = 0; t < 4; ++t) s[t] = make_double2(nan, nan);
+		if (valid && lane == 0 && a.status) atomicAdd(a.status, 1);
 	}
 	if (valid && lane == 0) quad_store64<KM, MODE, false>(a, i, j, s);
 }

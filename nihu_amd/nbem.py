@@ -30,7 +30,7 @@ FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_create_quads",
            "nb_mesh_create_elements", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
-           "nb_synchronize", "nb_profile", "nb_profile_read"]
+           "nb_synchronize", "nb_profile", "nb_profile_read", "nb_eval_math"]
 
 
 class NbKernel(C.Structure):
@@ -86,6 +86,7 @@ def lib():
         l.nb_autotune.argtypes = [P, C.c_int32, C.POINTER(NbPolicy), C.POINTER(NbLaunch)]
         l.nb_set_launch.argtypes = [C.POINTER(NbLaunch)]
         l.nb_synchronize.argtypes = [P]
+        l.nb_eval_math.argtypes = [C.c_int32, P, P, C.c_int64, P]
         l.nb_profile.argtypes = [C.c_int32]
         l.nb_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib = l
@@ -318,6 +319,23 @@ def autotune(mesh: Mesh, n_k=1, pol=None):
 
 def set_launch(tile_cols=0, k_batch=0, persistent=0):
     check(lib().nb_set_launch(C.byref(NbLaunch(tile_cols, k_batch, persistent))))
+
+
+MATH_ROUTINES = {"sincos": 0, "log": 1, "rsqrt": 2, "rcp": 3}
+
+
+def eval_math(fn, x):
+    """The engine's FP64 device routine `fn` ("sincos", "log", "rsqrt",
+    "rcp") at the values of the torch float64 GPU tensor x; returns (n, 2)
+    (sin, cos) or (n,) values."""
+    import torch
+    x = x.contiguous()
+    if x.dtype != torch.float64 or not x.is_cuda:
+        raise ValueError("x must be a float64 GPU tensor")
+    y = torch.empty((x.numel(), 2), dtype=torch.float64, device=x.device)
+    check(lib().nb_eval_math(MATH_ROUTINES[fn], C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), x.numel(),
+                             _stream_ptr(None)))
+    return y if fn == "sincos" else y[:, 0]
 
 
 def synchronize(stream=None):

@@ -11,16 +11,22 @@ HIP engine and solved by LU on the device.
 
 Scheduling: requests arrive over time (their `arrival` offsets); the loop
 admits the oldest waiting request as soon as a slot (a HIP stream with a host
-worker thread) is free and its HBM footprint fits the budget, hands its whole
-pipeline to that slot's thread, which launches it on the slot's stream, and
-goes on admitting while earlier requests are still on the device, so requests
-of very different sizes overlap.  The threads matter: the LU factorisation
-call returns to the host only when the factorisation is done (0.44 s at
-20480 unknowns), and launched from the loop itself it would hold back every
-other request meanwhile.  Completion is detected by events; latency =
-completion time - arrival time.  Results do not depend on the mix: every
-kernel computes each matrix entry in a fixed order without atomics, and the
-LU of a request runs on its own stream and buffers.
+worker thread) is free and its HBM footprint fits the budget, together with
+up to max_batch - 1 further arrived requests on the same mesh (continuous
+batching: their matrices come from one nb_assemble_sweep launch, the
+frequency-sweep pattern of benchmark/EAA/sphere/homogeneous/bem/bem.cpp:91-115),
+hands the batch's whole pipeline to that slot's thread, which launches it on
+the slot's stream, and goes on admitting while earlier batches are still on
+the device, so requests of very different sizes overlap.  The threads
+matter: the LU factorisation call returns to the host only when the
+factorisation is done (0.44 s at 20480 unknowns), and launched from the loop
+itself it would hold back every other request meanwhile.  Completion is
+detected per request by events (each request of a batch is done when its
+own solve is); latency = completion time - arrival time.  Results do not
+depend on the mix or the batch: every kernel computes each matrix entry in a
+fixed order without atomics, the sweep computes each wavenumber's matrices
+independently of the others in its launch, and each request's LU runs on
+its own buffers.
 """
 
 from __future__ import annotations
@@ -67,11 +73,14 @@ def footprint_bytes(n: int) -> int:
 
 
 class Server:
-    def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None, refine: int = 0):
+    def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None, refine: int = 0,
+                 max_batch: int = 8):
         """refine: steps of iterative refinement of the complex64 LU solution
         (residual b - A p accumulated in complex128 over column blocks,
-        correction solved with the same LU factor)."""
+        correction solved with the same LU factor).  max_batch: requests on
+        one mesh assembled in one sweep launch (1: one assembly per request)."""
         self.refine = int(refine)
+        self.max_batch = max(1, int(max_batch))
         import torch
         self.torch = torch
         self.meshes = {}
@@ -94,31 +103,53 @@ class Server:
         for key, m in self.meshes.items():
             if footprint_bytes(m.n) > self.budget:
                 raise ValueError("mesh %s does not fit the HBM budget" % key)
+        # launch shape of the sweep per mesh, tuned once here for batches of
+        # up to max_batch wavenumbers (so that no request pays for tuning)
+        self.launch = {}
+        for key, m in self.meshes.items():
+            pol = self._policy(m)
+            if m.nodes_per_element == 3 and pol.rule == nbem.NB_RULE_FIXED:
+                self.launch[key] = nbem.autotune(m, self.max_batch, pol)
+            else:
+                self.launch[key] = None
+        torch.cuda.synchronize()
 
-    def _launch(self, req: Request, stream):
+    def _policy(self, mesh):
+        return self.quad_policy if mesh.nodes_per_element != 3 else self.policy
+
+    def _launch(self, reqs, stream):
+        """One batch of requests on one mesh: the Burton-Miller matrices of
+        all their wavenumbers in one sweep launch, then per request the
+        right-hand side, LU factorisation and solve.  Returns per request
+        (event after its solve, its buffers)."""
         torch = self.torch
-        mesh = self.meshes[req.mesh]
+        key = reqs[0].mesh
+        mesh = self.meshes[key]
         n = mesh.n
-        c, nu = self.geom[req.mesh]
+        c, nu = self.geom[key]
+        ks = np.array([r.k for r in reqs], dtype=np.float64)
+        out = []
         with torch.cuda.stream(stream):
-            A = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
-            B = torch.empty((n, n), dtype=torch.complex64, device="cuda").t()
-            pol = self.quad_policy if mesh.nodes_per_element != 3 else self.policy
+            # column-major matrices, one per wavenumber
+            A = torch.empty((len(reqs), n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
+            B = torch.empty((len(reqs), n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
             # (the engine serialises calls on one mesh itself)
-            nbem.assemble_bm(mesh, req.k, A, B, pol=pol, stream=stream)
-            d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")
-            phase = -req.k * (c @ d)
-            pinc = torch.polar(torch.ones_like(phase), phase)
-            q = (1j * req.k) * (nu @ d) * pinc
-            rhs = (B @ q.to(torch.complex64)).unsqueeze(1)
-            LU, piv, _ = torch.linalg.lu_factor_ex(A)
-            p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
-            if self.refine > 0:
-                p = _refine(A, LU, piv, rhs.squeeze(1), p, self.refine)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        # keep the buffers alive until the event has passed
-        return ev, (A, B, LU, piv, rhs, p)
+            nbem.assemble_sweep(mesh, ks, A, B, pol=self._policy(mesh), launch=self.launch[key], stream=stream)
+            for g, req in enumerate(reqs):
+                d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")
+                phase = -req.k * (c @ d)
+                pinc = torch.polar(torch.ones_like(phase), phase)
+                q = (1j * req.k) * (nu @ d) * pinc
+                rhs = (B[g] @ q.to(torch.complex64)).unsqueeze(1)
+                LU, piv, _ = torch.linalg.lu_factor_ex(A[g])
+                p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
+                if self.refine > 0:
+                    p = _refine(A[g], LU, piv, rhs.squeeze(1), p, self.refine)
+                ev = torch.cuda.Event()
+                ev.record(stream)
+                # keep the buffers alive until the event has passed
+                out.append((ev, (A, B, LU, piv, rhs, p)))
+        return out
 
     def run(self, requests, realtime: bool = True):
         """Serve the requests; returns the results in completion order.
@@ -128,36 +159,49 @@ class Server:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         free_slots = list(range(len(self.slots)))
         in_use = 0.0
-        active = []   # (req, slot, future of (event, buffers), bytes, admitted)
+        active = []   # [batch, slot, future of [(event, buffers)], bytes, admitted, retired flags]
         results = []
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         with ThreadPoolExecutor(max_workers=len(self.slots), thread_name_prefix="nbem-slot") as pool:
             while pending or active:
-                # admit as many arrived requests as slots and memory allow (FIFO)
+                # admit as many arrived requests as slots and memory allow: the
+                # oldest one, with the next arrived requests on its mesh
                 while pending and free_slots:
-                    req = pending[0]
-                    if realtime and req.arrival > now():
+                    head = pending[0]
+                    if realtime and head.arrival > now():
                         break
-                    nb = footprint_bytes(self.meshes[req.mesh].n)
-                    if in_use + nb > self.budget:
+                    fp = footprint_bytes(self.meshes[head.mesh].n)
+                    if in_use + fp > self.budget:
                         break
-                    pending.popleft()
+                    batch = [head]
+                    for req in list(pending)[1:]:
+                        if len(batch) >= self.max_batch or (realtime and req.arrival > now()):
+                            break
+                        if req.mesh == head.mesh and in_use + fp * (len(batch) + 1) <= self.budget:
+                            batch.append(req)
+                    for req in batch:
+                        pending.remove(req)
                     slot = free_slots.pop(0)
                     t_adm = now()
-                    fut = pool.submit(self._launch, req, self.slots[slot])
-                    in_use += nb
-                    active.append((req, slot, fut, nb, t_adm))
-                # retire finished requests (a failed launch raises here)
+                    fut = pool.submit(self._launch, batch, self.slots[slot])
+                    in_use += fp * len(batch)
+                    active.append([batch, slot, fut, fp * len(batch), t_adm, [False] * len(batch)])
+                # retire finished requests (a failed launch raises here); a
+                # batch frees its slot and memory when all of it is done
                 still = []
                 for item in active:
-                    req, slot, fut, nb, t_adm = item
+                    batch, slot, fut, nbytes, t_adm, done = item
                     if fut.done():
-                        ev, bufs = fut.result()
-                        if ev.query():
-                            p = bufs[-1].cpu().numpy()
-                            results.append(Result(req.rid, req.mesh, req.k, len(p), req.arrival, t_adm, now(), p))
-                            in_use -= nb
+                        per_req = fut.result()
+                        for g, (ev, bufs) in enumerate(per_req):
+                            if not done[g] and ev.query():
+                                p = bufs[-1].cpu().numpy()
+                                req = batch[g]
+                                results.append(Result(req.rid, req.mesh, req.k, len(p), req.arrival, t_adm, now(), p))
+                                done[g] = True
+                        if all(done):
+                            in_use -= nbytes
                             free_slots.append(slot)
                             continue
                     still.append(item)

@@ -85,9 +85,11 @@ def test_mixed_operators(nb, orc, mixed, fam, nx, ny, k):
     m = nb.Mesh.from_elements(nodes, table)
     n = len(table)
     out = nb.colmajor(n, n, "complex128")
+    out.fill_(complex("nan+nanj"))   # an entry no block writes stays NaN
     nb.assemble(m, m, out, fam, nx, ny, k, pol=nb.policy(**NIHU[0]), accumulate=False)
     torch.cuda.synchronize()
     got = out.cpu().numpy()
+    assert np.isfinite(got).all()
     ref = oracle_mixed(orc, nodes, table, fam, nx, ny, k, NIHU[1])
     rs, fro = rowscaled(got, ref), relfro(got, ref)
     print("mixed %d elements fam %d (%d, %d): row-scaled %.2e rel fro %.2e" % (n, fam, nx, ny, rs, fro), flush=True)
@@ -218,3 +220,55 @@ def test_mixed_sweep(nb, mixed):
         nb.assemble_bm(m, float(k), A1, B1, pol=nb.policy(**NIHU[0]))
         torch.cuda.synchronize()
         assert torch.equal(A[f], A1) and torch.equal(B[f], B1)
+
+
+def test_mixed_several_row_tiles(nb, orc):
+    """1152 interleaved elements (576 triangles, 576 quadrilaterals: five
+    256-row tiles of the all-rows regular pass): seeded sampled rows of the
+    Burton-Miller A and B and of the Laplace double layer against the oracle's
+    single-type assemblies placed by NiHu's rules, with every entry of the
+    device matrices written (NaN fill)."""
+    import torch
+    from nihu_amd.mesh import mixed_cubed_sphere
+    nodes, table = mixed_cubed_sphere(12, seed=4)
+    n = len(table)
+    assert n == 1152
+    m = nb.Mesh.from_elements(nodes, table)
+    k = 6.0
+    al = complex(0.0, -1.0 / k)
+    pol = nb.policy(**NIHU[0])
+    A, B = nb.colmajor(n, n), nb.colmajor(n, n)
+    D = nb.colmajor(n, n, "float64")
+    for t in (A, B, D):
+        t.fill_(float("nan"))
+    nb.assemble_bm(m, k, A, B, pol=pol, accumulate=False)
+    nb.assemble(m, m, D, nb.NB_LAPLACE, 0, 1, 0.0, pol=pol, accumulate=False)
+    torch.cuda.synchronize()
+    for t in (A, B, D):
+        assert torch.isfinite(torch.view_as_real(t) if t.is_complex() else t).all()
+    rows = np.sort(np.random.default_rng(8).choice(n, 96, replace=False))
+    parts = [(np.flatnonzero(table[:, 0] == TRIA), 3), (np.flatnonzero(table[:, 0] == QUAD), 4)]
+    opol = orc.policy(**NIHU[1])
+    ops = {key: np.zeros((len(rows), n), dtype=np.complex128) for key in ((0, 0), (0, 1), (1, 0), (1, 1), "lap")}
+    for a_, (ia, na) in enumerate(parts):
+        sel = np.flatnonzero(np.isin(rows, ia))
+        if sel.size == 0:
+            continue
+        loc = np.searchsorted(ia, rows[sel]).astype(np.int32)
+        for b_, (ib, nb_) in enumerate(parts):
+            trial = None if a_ == b_ else (nodes, table[ib, 1:1 + nb_])
+            for nx, ny in ((0, 0), (0, 1), (1, 0), (1, 1)):
+                ops[(nx, ny)][np.ix_(sel, ib)] = orc.assemble(nodes, table[ia, 1:1 + na], 1, nx, ny, k=k, pol=opol,
+                                                              rows=loc, trial=trial)
+            ops["lap"][np.ix_(sel, ib)] = orc.assemble(nodes, table[ia, 1:1 + na], 0, 0, 1, pol=opol, rows=loc,
+                                                       trial=trial)
+    Ar = al * ops[(1, 1)] + ops[(0, 1)]
+    Br = ops[(0, 0)] + al * ops[(1, 0)]
+    Ar[np.arange(len(rows)), rows] -= 0.5
+    Br[np.arange(len(rows)), rows] += 0.5 * al
+    ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+    Ag, Bg, Dg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy(), D[ri, :].cpu().numpy()
+    errs = (relfro(Ag, Ar), relfro(Bg, Br), relfro(Dg, ops["lap"].real))
+    print("mixed 1152 elements, 96 rows: A %.2e B %.2e Laplace DLP %.2e" % errs, flush=True)
+    assert max(errs) < 1e-5
+    assert rowscaled(Ag, Ar) <= ROW_TOL and rowscaled(Bg, Br) <= ROW_TOL and rowscaled(Dg, ops["lap"].real) <= ROW_TOL

@@ -251,3 +251,32 @@ def test_quad_degenerate_rejected(nb):
     nodes = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [3, 0, 0]], float)
     with pytest.raises(Exception, match="degenerate"):
         nb.Mesh(nodes, np.array([[0, 1, 2, 3]], dtype=np.int32))
+
+
+def test_quad_inverse_mapping_failure_is_an_error(nb, qmesh):
+    """A near-singular quadrilateral pair whose collocation point has no image
+    (the inverse mapping does not converge; NiHu throws) makes the call fail
+    with NB_ERR_RUNTIME instead of returning NaN entries silently.  The
+    failure is forced from the host: NBEM_INVERSE_MAPPING_STEPS=1 leaves
+    Newton's iteration one step (only the element centre itself maps), so
+    every near-singular pair fails and every kernel still finishes normally.
+    Without the hook the same call succeeds with finite entries."""
+    import os
+    import torch
+    nodes, quads = qmesh
+    m = nb.Mesh(nodes, quads)
+    n = len(quads)
+    A, B = nb.colmajor(n, n), nb.colmajor(n, n)
+    pol = nb.policy(**NIHU[0])
+    os.environ["NBEM_INVERSE_MAPPING_STEPS"] = "1"
+    try:
+        with pytest.raises(nb.NbemError, match="inverse mapping"):
+            nb.assemble_bm(m, 3.0, A, B, pol=pol)
+        with pytest.raises(nb.NbemError, match="inverse mapping"):
+            nb.assemble(m, m, nb.colmajor(n, n), nb.NB_LAPLACE, 0, 0, 0.0, pol=pol, accumulate=False)
+    finally:
+        del os.environ["NBEM_INVERSE_MAPPING_STEPS"]
+    torch.cuda.synchronize()
+    nb.assemble_bm(m, 3.0, A, B, pol=pol)
+    torch.cuda.synchronize()
+    assert torch.isfinite(torch.view_as_real(A)).all() and torch.isfinite(torch.view_as_real(B)).all()

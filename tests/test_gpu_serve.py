@@ -119,3 +119,74 @@ def test_iterative_refinement(meshes):
     refd = serve.Server({"L3": (nodes, tris)}, slots=1, refine=2).run([serve.Request(0, "L3", k, tuple(d))],
                                                                         realtime=False)[0].p
     assert np.linalg.norm(refd - plain) / np.linalg.norm(refd) < 10 * e0 + 1e-6
+
+
+def _oracle_solve(orc, nodes, tris, req):
+    """The plane-wave system of one request with the oracle's FP64 matrices
+    (the configuration policy), solved in complex128 on the device."""
+    import torch
+    from nihu_amd import serve
+    Ar, Br = orc.assemble_bm(nodes, tris, req.k, pol=orc.policy(rule=1, fixed_degree=5, near_mode=1, near_limit=2.0))
+    c, nu = serve._centres_normals(nodes, tris)
+    d = np.asarray(req.direction)
+    q = (1j * req.k) * (nu @ d) * np.exp(-1j * req.k * (c @ d))
+    Ag = torch.from_numpy(Ar).cuda()
+    del Ar
+    rhs = torch.from_numpy(Br).cuda() @ torch.from_numpy(q).cuda()
+    del Br
+    p = torch.linalg.solve(Ag, rhs.unsqueeze(1)).squeeze(1).cpu().numpy()
+    del Ag, rhs
+    torch.cuda.empty_cache()
+    return p
+
+
+def test_config5_stream_with_20480_triangles(orc):
+    """The serving workload's meshes (1280, 5120 and 20480-triangle jittered
+    icospheres, seed 0), nine requests with ka in [1, 40] arriving together,
+    three per mesh, so that requests on one mesh are batched into one sweep
+    launch: the first 20480-triangle request gives the same surface field
+    alone and in the mix (1e-6), and one served solution per mesh matches
+    the complex128 solve of the oracle's FP64 matrices (relative 2-norm
+    1e-5; the complex64 LU alone is about 3e-7 from it)."""
+    from nihu_amd import serve
+    from nihu_amd.mesh import jittered_icosphere
+    meshes = {"L3": jittered_icosphere(3, seed=0), "L4": jittered_icosphere(4, seed=0),
+              "L5": jittered_icosphere(5, seed=0)}
+    srv = serve.Server(meshes, slots=4)
+    base = serve.poisson_stream(9, rate=1e9, seed=21, k_range=(1.0, 40.0))
+    reqs = [serve.Request(r.rid, ("L5", "L3", "L4")[i % 3], r.k, r.direction, r.arrival) for i, r in enumerate(base)]
+    served = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    assert len(served) == len(reqs) and all(np.isfinite(r.p).all() for r in served.values())
+    big = reqs[0]
+    alone = srv.run([serve.Request(big.rid, big.mesh, big.k, big.direction)], realtime=False)[0]
+    diff = np.abs(alone.p - served[big.rid].p).max() / np.abs(alone.p).max()
+    print("20480 triangles, ka %.2f: alone vs mix %.2e; latencies %s" % (
+        big.k, diff, {r.rid: round(r.latency, 3) for r in served.values()}), flush=True)
+    assert diff <= 1e-6
+    for key in ("L3", "L4", "L5"):
+        req = next(r for r in reqs if r.mesh == key)
+        p_ref = _oracle_solve(orc, *meshes[key], req)
+        err = np.linalg.norm(served[req.rid].p - p_ref) / np.linalg.norm(p_ref)
+        print("%s ka %.2f: served vs oracle complex128 solve %.2e" % (key, req.k, err), flush=True)
+        assert err < 1e-5
+
+
+def test_batched_equals_alone(meshes):
+    """Requests on one mesh batched into one sweep launch give the same
+    surface field as each request alone (1e-6), on the triangle meshes
+    (one-launch sweep) and on a quadrilateral mesh (one assembly per
+    wavenumber inside the sweep call)."""
+    from nihu_amd import serve
+    from nihu_amd.mesh import jittered_cubed_sphere
+    ms = dict(meshes)
+    ms["Q"] = jittered_cubed_sphere(6, seed=2)
+    srv = serve.Server(ms, slots=2, max_batch=8)
+    reqs = [serve.Request(i, key, k, (0.0, 0.6, 0.8)) for i, (key, k) in enumerate(
+        [("L3", 2.0), ("L3", 9.5), ("L3", 17.25), ("L3", 33.0), ("Q", 3.0), ("Q", 8.0), ("L4", 4.0), ("L4", 21.0)])]
+    batched = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    one = serve.Server(ms, slots=2, max_batch=1)
+    for rq in reqs:
+        alone = one.run([rq], realtime=False)[0]
+        diff = np.abs(alone.p - batched[rq.rid].p).max() / np.abs(alone.p).max()
+        print("request %d %s ka %.2f: batched vs alone %.2e" % (rq.rid, rq.mesh, rq.k, diff), flush=True)
+        assert diff <= 1e-6

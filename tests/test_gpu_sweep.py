@@ -139,3 +139,45 @@ def test_sweep_rejects_bad_arguments(nb, l3):
         nb.assemble_sweep(m, np.array([1.0, -2.0]), A, B, pol=nb.policy(**CONFIG))
     with pytest.raises(ValueError):
         nb.assemble_sweep(m, np.array([1.0, 2.0]), A, B, pol=nb.policy(**CONFIG), launch=nb.NbLaunch(32, 3, 4))
+
+
+def test_config4_sweep_64_wavenumbers(nb, orc):
+    """The sweep workload itself: Burton-Miller A and B for 64 wavenumbers ka
+    uniformly spaced in [0.5, 32] on the 5120-triangle jittered icosphere
+    (level 4, seed 0), 7-point rule far, FP64 within 2 element diameters, in
+    one persistent launch with the autotuned launch shape (launch=None).
+    Every wavenumber's matrices against its single-wavenumber assembly
+    (relative Frobenius 1e-6, computed on the device), and 16 seeded rows of
+    8 wavenumbers, the first and last among them, against the oracle's sweep
+    (1e-5)."""
+    import torch
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(4, seed=0)
+    n = len(tris)
+    m = nb.Mesh(nodes, tris)
+    ks = np.linspace(0.5, 32.0, 64)
+    pol = nb.policy(**CONFIG)
+    A, B = stack(len(ks), n), stack(len(ks), n)
+    A.fill_(complex("nan+nanj"))
+    B.fill_(complex("nan+nanj"))
+    nb.assemble_sweep(m, ks, A, B, pol=pol)
+    torch.cuda.synchronize()
+    assert torch.isfinite(torch.view_as_real(A)).all() and torch.isfinite(torch.view_as_real(B)).all()
+    A1, B1 = nb.colmajor(n, n), nb.colmajor(n, n)
+    worst = 0.0
+    for f, k in enumerate(ks):
+        nb.assemble_bm(m, float(k), A1, B1, pol=pol)
+        ea = (torch.linalg.norm(A[f] - A1) / torch.linalg.norm(A1)).item()
+        eb = (torch.linalg.norm(B[f] - B1) / torch.linalg.norm(B1)).item()
+        worst = max(worst, ea, eb)
+        assert ea < 1e-6 and eb < 1e-6, (f, k, ea, eb)
+    print("64 wavenumbers: max rel. Frobenius difference to single-wavenumber assembly %.2e" % worst, flush=True)
+    rows = np.sort(np.random.default_rng(44).choice(n, 16, replace=False)).astype(np.int32)
+    fsel = [0, 9, 18, 27, 36, 45, 54, 63]
+    Ar, Br = orc.sweep_bm(nodes, tris, ks[fsel], pol=orc.policy(**CONFIG_O), rows=rows)
+    ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+    for t, f in enumerate(fsel):
+        ea = relfro(A[f][ri, :].cpu().numpy(), Ar[t])
+        eb = relfro(B[f][ri, :].cpu().numpy(), Br[t])
+        print("ka %.2f: A %.2e B %.2e" % (ks[f], ea, eb), flush=True)
+        assert ea < 1e-5 and eb < 1e-5

@@ -345,3 +345,34 @@ def test_quad_mesh_io(tmp_path):
     assert table.tolist() == [[32303, 0, 1, 2, 0], [32404, 0, 1, 2, 3]]
     with pytest.raises(ValueError, match="from_elements"):
         from_nihu_elements(table)
+
+
+# A warped quadrilateral and a point 1.3 mean edges from its centre for which
+# Newton's iteration from the element centre (nihu/core/inverse_mapping.hpp:37-118,
+# 100 steps) does not reach the tolerance: found by a seeded search, kept as
+# exact doubles because the iteration is chaotic near such points.
+FAILING_IMAGE_QUAD = [[float.fromhex(v) for v in row] for row in (
+    ('-0x1.a8c55495fea39p-1', '-0x1.0d1631af42eeap+0', '0x1.3c778c86b4547p+1'),
+    ('0x1.fb7467c770627p-1', '-0x1.b4959ed1d6c18p-1', '-0x1.2ad78f6f50e12p-2'),
+    ('0x1.3788f851efb37p+0', '0x1.df50add3bd6a0p-1', '0x1.3c778c86b4547p+1'),
+    ('-0x1.98701423de83cp-1', '0x1.2ff977e877cfep+0', '-0x1.5ce5c3c4e14dfp+0'))]
+FAILING_IMAGE_POINT = [float.fromhex(v) for v in ('-0x1.ff2f8b9b9e538p+1', '0x1.deb29d841172ap+0',
+                                                   '0x1.3742316ce908ep+1')]
+
+
+def test_inverse_mapping_failure_raises(orc):
+    """NiHu throws "Could not perform inverse mapping" when the image of the
+    collocation point does not converge; the oracle's near-singular integrals
+    raise the same error (the engine reports it as NB_ERR_RUNTIME), while the
+    mapping itself reports convergence as a flag."""
+    X = np.array(FAILING_IMAGE_QUAD)
+    x0 = np.array(FAILING_IMAGE_POINT)
+    g = orc.geometry_quad(X, np.array([[0, 1, 2, 3]]))
+    assert np.linalg.norm(x0 - g["center"][0]) / g["R"][0] < 1.5      # a near-singular pair under NiHu's rule
+    assert orc.quad_elem(orc.QUAD_INVERSE, 0, 0, 0, 0.0, X, x0=x0)[3] == 0
+    for fam, nx, ny, k in ((1, 0, 0, 2.0), (1, 1, 1, 2.0), (0, 0, 0, 0.0)):
+        with pytest.raises(RuntimeError, match="inverse mapping"):
+            orc.quad_elem(orc.QUAD_NEAR, fam, nx, ny, k, X, x0=x0, nx0=(0.0, 0.0, 1.0))
+    # a point close to the element maps back
+    xi = orc.quad_elem(orc.QUAD_INVERSE, 0, 0, 0, 0.0, X, x0=g["center"][0] + 0.1 * g["normal"][0])
+    assert xi[3] == 1 and abs(xi[0]) < 1e-9 and abs(xi[1]) < 1e-9
