@@ -16,7 +16,14 @@ relative Frobenius and max errors of both matrices against the FP64 oracle
 (NiHu's algorithm restated), which assembles the same matrices on the host
 with OpenMP over rows and is timed as the CPU baseline (beside it the serial
 pair loop on a row sample, and the host's CPU model); the solved surface field
-against the oracle's, and the total field against the Mie series.
+against the oracle's, and the total field against the Mie series.  The same
+line carries two more sections after the timed headline steps: `sweep`, the
+64-wavenumber sweep on 5120 triangles in one launch against 64
+single-wavenumber assemblies (ms per sweep, p50/p99 sweep latency, errors
+against the oracle), and `serve`, a Poisson stream of assembly-plus-solve
+requests served with and without batching (p50/p99 latency, solves/s) and a
+rate sweep up to the first rate the server does not keep up with
+(--no-extras skips both).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 20]
 """
@@ -140,7 +147,9 @@ def main():
                          "Galerkin matrices with linear triangles on the 20480-element ellipsoid; quad: "
                          "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere; mixed: the "
                          "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table)")
-    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--requests", type=int, default=24, help="serve: requests per stream")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: skip the sweep and serving sections of the default line")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
     ap.add_argument("--refine", type=int, default=0,
                     help="serve: steps of FP64 iterative refinement of each complex64 LU solve")
@@ -286,8 +295,20 @@ def main():
         del Ar, Br
         log("error vs oracle: %s" % err)
 
+    # the second half of the metric: the served frequency sweep (config 4 in
+    # one launch against the per-frequency path; config 5 served stream, its
+    # p50/p99 and its saturation rate)
+    extras = {}
+    if not args.no_extras and args.level == 5:
+        del A, B
+        torch.cuda.empty_cache()
+        extras["sweep"] = sweep_summary(steps=5, warmup=2)
+        torch.cuda.empty_cache()
+        extras["serve"] = serve_summary(n_req=args.requests, rate=args.rate, refine=args.refine)
+        torch.cuda.empty_cache()
+
     out = {
-        "metric": "matrix entries/s, Helmholtz 3D coll. 20480 tris",
+        "metric": "matrix entries/s, Helmholtz 3D coll. 20480 tris; p50/p99 sweep latency",
         "value": value,
         "unit": "matrix entries/s",
         "n_gpus": 1,
@@ -310,6 +331,7 @@ def main():
         "accuracy": err,
         "pass_ms": pass_ms,
     }
+    out.update(extras)
     print(json.dumps(out))
 
 
@@ -624,18 +646,18 @@ def run_mixed(args):
     print(json.dumps(out))
 
 
-def run_sweep(args):
-    """Batched frequency sweep: Burton-Miller A and B for 64 wavenumbers ka
-    uniformly spaced in [0.5, 32] on the 5120-triangle jittered icosphere
-    (level 4), 7-point rule far, FP64 within 2 element diameters, one
-    persistent launch with an autotuned launch shape.  Compared with the same
-    matrices from 64 single-wavenumber assemblies."""
+def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
+    """Config 4: Burton-Miller A and B for 64 wavenumbers ka uniformly spaced
+    in [0.5, 32] on the 5120-triangle jittered icosphere (level 4, seed 0),
+    7-point rule far, FP64 within 2 element diameters, in one persistent
+    launch with the autotuned launch shape; compared with the same matrices
+    from 64 single-wavenumber assemblies, and 16 seeded rows of 8 wavenumbers
+    (the first and last among them) against the FP64 oracle's sweep."""
     import numpy as np
     import torch
     from nihu_amd import nbem
     from nihu_amd.mesh import jittered_icosphere
 
-    torch.cuda.set_device(0)
     level = 4
     nodes, tris = jittered_icosphere(level, seed=0)
     n = len(tris)
@@ -646,105 +668,169 @@ def run_sweep(args):
     B = torch.zeros_like(A)
     t = time.perf_counter()
     best = nbem.autotune(mesh, len(ks), pol)
-    log("autotune %.2f s: tile_cols %d, k_batch %d, persistent %d" % (time.perf_counter() - t, best.tile_cols,
-                                                                      best.k_batch, best.persistent))
-    for w in range(args.warmup):
+    tune_s = time.perf_counter() - t
+    log("%s: autotune %.2f s: tile_cols %d, k_batch %d, persistent %d" % (log_prefix, tune_s, best.tile_cols,
+                                                                       best.k_batch, best.persistent))
+    for _ in range(warmup):
         nbem.assemble_sweep(mesh, ks, A, B, pol=pol)
         torch.cuda.synchronize()
     nbem.profile(True)
     nbem.profile_read(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    lat = []
+    for _ in range(steps):
+        t = time.perf_counter()
         nbem.assemble_sweep(mesh, ks, A, B, pol=pol)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t)
     prof = nbem.profile_read(reset=True)
+    wall = float(np.sum(lat))
     entries = 4 * n * n * len(ks)
-    value = entries * args.steps / wall
-    log("sweep: %.3f ms per 64-wavenumber sweep, %.4g entries/s; passes %s" % (wall / args.steps * 1e3, value, prof))
-
+    value = entries * steps / wall
+    log("%s: %.3f ms per 64-wavenumber sweep, %.4g entries/s; passes %s" % (log_prefix, wall / steps * 1e3, value,
+                                                                          prof))
     # the same matrices, one Burton-Miller assembly per wavenumber
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f, k in enumerate(ks):
         nbem.assemble_bm(mesh, float(k), A[f], B[f], pol=pol)
     torch.cuda.synchronize()
     per_freq = entries / (time.perf_counter() - t0)
     nbem.profile(False)
-    log("per-frequency assemblies: %.4g entries/s" % per_freq)
+    log("%s: per-frequency assemblies %.4g entries/s" % (log_prefix, per_freq))
     nbem.assemble_sweep(mesh, ks, A, B, pol=pol)
     torch.cuda.synchronize()
 
     from oracle import oracle
-    rows = np.sort(np.random.default_rng(5).choice(n, 64, replace=False)).astype(np.int32)
+    rows = np.sort(np.random.default_rng(5).choice(n, 16, replace=False)).astype(np.int32)
+    fsel = [0, 9, 18, 27, 36, 45, 54, 63]
     opol = oracle.policy(rule=1, fixed_degree=5, near_mode=1, near_limit=2.0)
     threads = oracle.num_threads()
     tc = time.perf_counter()
-    Ar, Br = oracle.sweep_bm(nodes, tris, ks, pol=opol, rows=rows, threads=threads)
+    Ar, Br = oracle.sweep_bm(nodes, tris, ks[fsel], pol=opol, rows=rows, threads=threads)
     dt = time.perf_counter() - tc
-    cpu_value = 4 * len(rows) * n * len(ks) / dt
+    cpu_value = 4 * len(rows) * n * len(fsel) / dt
     ri = torch.from_numpy(rows.astype(np.int64)).cuda()
     errs = []
-    for f in range(len(ks)):
+    for t_, f in enumerate(fsel):
         a, b = A[f][ri, :].cpu().numpy(), B[f][ri, :].cpu().numpy()
-        errs.append(max(np.linalg.norm(a - Ar[f]) / np.linalg.norm(Ar[f]), np.linalg.norm(b - Br[f]) / np.linalg.norm(Br[f])))
-    log("max rel. Frobenius error over the 64 wavenumbers (sampled rows): %.3e" % max(errs))
+        errs.append(max(np.linalg.norm(a - Ar[t_]) / np.linalg.norm(Ar[t_]),
+                        np.linalg.norm(b - Br[t_]) / np.linalg.norm(Br[t_])))
+    log("%s: max rel. Frobenius error over %d wavenumbers (16 rows): %.3e" % (log_prefix, len(fsel), max(errs)))
+    del A, B
+    torch.cuda.empty_cache()
     reg_ms = prof["sweep"][0] / max(prof["sweep"][1], 1)
-    out = {
-        "metric": "matrix entries/s, Helmholtz 3D Burton-Miller sweep, 64 wavenumbers, 5120 tris",
-        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 (complex64) far pairs, fp64 near/diagonal", "data": "synthetic: seeded jittered icosphere",
-        "config": {"workload": "Burton-Miller sweep", "n_tri": n, "icosphere_level": level, "n_k": len(ks),
-                   "ka": [0.5, 32.0], "far_rule": "Dunavant degree 5 (7 points)",
-                   "near": "centroid distance < 2 element diameters, FP64",
-                   "launch": {"tile_cols": best.tile_cols, "k_batch": best.k_batch, "persistent": best.persistent}},
-        "per_frequency_value": per_freq,
+    return {
+        "workload": "Burton-Miller sweep, 64 wavenumbers ka in [0.5, 32], 5120-triangle jittered icosphere, "
+                    "7-point rule far, FP64 within 2 element diameters, one persistent launch",
+        "n_tri": n, "n_k": len(ks), "steps": steps,
+        "ms_per_sweep": wall / steps * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
+        "entries_per_s": value,
+        "per_frequency_entries_per_s": per_freq,
         "speedup_vs_per_frequency": value / per_freq,
         "regular_sweep_kernel_ms": reg_ms,
+        "launch": {"tile_cols": best.tile_cols, "k_batch": best.k_batch, "persistent": best.persistent,
+                   "autotune_s": tune_s},
+        "accuracy": {"max_rel_fro_over_k": float(max(errs)), "rows": len(rows), "wavenumbers": len(fsel)},
         "cpu_baseline": {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
-                         "sample": "%d of %d rows, 64 wavenumbers, NiHu's OpenMP-over-frequencies pattern (FP64 oracle)"
-                                   ", %.1f s wall" % (len(rows), n, dt), "host": host_cpu()},
-        "accuracy": {"max_rel_fro_over_k": float(max(errs)), "rows": len(rows)},
+                         "sample": "%d of %d rows, %d of the 64 wavenumbers, FP64 oracle (NiHu's algorithm), OpenMP "
+                                   "over frequencies, %.1f s wall" % (len(rows), n, len(fsel), dt)},
+    }
+
+
+def run_sweep(args):
+    import torch
+    torch.cuda.set_device(0)
+    sw = sweep_summary(steps=args.steps, warmup=max(1, min(args.warmup, 5)))
+    out = {
+        "metric": "matrix entries/s, Helmholtz 3D Burton-Miller sweep, 64 wavenumbers, 5120 tris",
+        "value": sw["entries_per_s"], "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sw["ms_per_sweep"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (complex64) far pairs, fp64 near/diagonal",
+        "data": "synthetic: seeded jittered icosphere",
+        "config": {"workload": sw["workload"], "n_tri": sw["n_tri"], "n_k": sw["n_k"], "launch": sw["launch"]},
+        "sweep": sw,
     }
     print(json.dumps(out))
 
 
-def run_serve(args):
-    """Poisson stream of Burton-Miller assembly + LU solve requests: mesh
-    uniform over the 1280/5120/20480-triangle jittered icospheres (resident on
-    the device), ka uniform in [1, 40], plane wave of seeded random direction;
-    admission by complex64 footprint, 4 slots (streams)."""
-    import numpy as np
-    import torch
+def _serve_stream(srv, n_req, rate, seed):
     from nihu_amd import serve
-    from nihu_amd.mesh import jittered_icosphere
-
-    torch.cuda.set_device(0)
-    meshes = {"L3": jittered_icosphere(3, seed=0), "L4": jittered_icosphere(4, seed=0),
-              "L5": jittered_icosphere(5, seed=0)}
-    srv = serve.Server(meshes, slots=4, refine=args.refine)
-    warm = [serve.Request(-1 - i, key, 5.0, (0.0, 0.0, 1.0)) for i, key in enumerate(meshes)]
-    srv.run(warm, realtime=False)
-    reqs = serve.poisson_stream(args.requests, rate=args.rate, seed=11)
+    reqs = serve.poisson_stream(n_req, rate=rate, seed=seed)
     t0 = time.perf_counter()
     res = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0
     lat = [r.latency for r in res]
-    by = {}
-    for r in res:
-        by.setdefault(r.mesh, []).append(r.latency)
-    log("served %d requests in %.2f s; p50 %.3f s p99 %.3f s; %s" % (len(res), wall, serve.percentile(lat, 50),
-        serve.percentile(lat, 99), {k: (len(v), round(serve.percentile(v, 50), 4)) for k, v in sorted(by.items())}))
+    last_arrival = max(r.arrival for r in reqs)
+    return {"rate_per_s": rate, "requests": len(res), "wall_s": wall,
+            "solves_per_s": len(res) / wall,
+            "p50_s": serve.percentile(lat, 50), "p99_s": serve.percentile(lat, 99),
+            # completions kept up with arrivals: the run ended within 1 s (or 20 %) of the last arrival
+            "kept_up": wall <= max(last_arrival * 1.2, last_arrival + 1.0),
+            "by_mesh_p50_s": {k: serve.percentile([r.latency for r in res if r.mesh == k], 50)
+                              for k in sorted({r.mesh for r in res})}}
+
+
+def serve_summary(n_req=24, rate=4.0, refine=0, saturation=True, log_prefix="serve"):
+    """Config 5: Poisson stream of Burton-Miller assembly + LU solve requests,
+    mesh uniform over the 1280 / 5120 / 20480-triangle jittered icospheres
+    (resident on the device), ka uniform in [1, 40], plane wave of seeded
+    random direction, admission by complex64 footprint, 4 slots.  The
+    batching server (requests on one mesh in one sweep launch) and the
+    per-request path (one assembly per request) serve the same stream at the
+    offered rate; then a rate sweep (doubling from the offered rate) stops at
+    the first rate the batching server does not keep up with."""
+    from nihu_amd import serve
+    from nihu_amd.mesh import jittered_icosphere
+
+    meshes = {"L3": jittered_icosphere(3, seed=0), "L4": jittered_icosphere(4, seed=0),
+              "L5": jittered_icosphere(5, seed=0)}
+    out = {"workload": "Poisson stream of Burton-Miller assembly + complex64 LU solve requests, meshes 1280 / 5120 / "
+                       "20480 triangles, ka U[1, 40], 4 slots, admission by HBM footprint",
+           "meshes_n_tri": {k: len(v[1]) for k, v in meshes.items()}, "refine_steps": refine}
+    for name, mb in (("batched", 8), ("per_request", 1)):
+        srv = serve.Server(meshes, slots=4, refine=refine, max_batch=mb)
+        warm = [serve.Request(-1 - i, key, 5.0 + i, (0.0, 0.0, 1.0)) for i, key in enumerate(meshes)]
+        srv.run(warm, realtime=False)
+        r = _serve_stream(srv, n_req, rate, seed=11)
+        log("%s (%s, max_batch %d): %d requests at %.1f/s in %.2f s; p50 %.3f s p99 %.3f s" % (
+            log_prefix, name, mb, r["requests"], rate, r["wall_s"], r["p50_s"], r["p99_s"]))
+        out[name] = r
+        if name == "batched" and saturation:
+            sweep = []
+            rt = rate
+            while rt <= 64 * rate:
+                rt *= 2
+                q = _serve_stream(srv, n_req, rt, seed=12)
+                log("%s: rate %.1f/s -> %.2f solves/s, p50 %.3f s p99 %.3f s, kept up %s" % (
+                    log_prefix, rt, q["solves_per_s"], q["p50_s"], q["p99_s"], q["kept_up"]))
+                sweep.append({k: q[k] for k in ("rate_per_s", "solves_per_s", "p50_s", "p99_s", "kept_up")})
+                if not q["kept_up"]:
+                    break
+            kept = [x["rate_per_s"] for x in sweep if x["kept_up"]]
+            out["rate_sweep"] = sweep
+            out["max_sustained_rate_per_s"] = max([rate] + kept) if out["batched"]["kept_up"] else None
+            out["saturation_solves_per_s"] = max(x["solves_per_s"] for x in sweep)
+        del srv
+    out["p50_s"], out["p99_s"] = out["batched"]["p50_s"], out["batched"]["p99_s"]
+    out["p50_speedup_vs_per_request"] = out["per_request"]["p50_s"] / out["batched"]["p50_s"]
+    return out
+
+
+def run_serve(args):
+    import torch
+    torch.cuda.set_device(0)
+    sv = serve_summary(n_req=args.requests, rate=args.rate, refine=args.refine)
     out = {
         "metric": "solves/s and p50/p99 latency, Poisson stream of Burton-Miller assembly + LU solves",
-        "value": len(res) / wall, "unit": "solves/s", "n_gpus": 1, "steps": len(res), "warmup": len(warm),
-        "ms_per_step": wall / len(res) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": sv["batched"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": sv["batched"]["requests"],
+        "warmup": 3, "ms_per_step": sv["batched"]["wall_s"] / max(sv["batched"]["requests"], 1) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 (complex64) far pairs and LU, fp64 near/diagonal", "data": "synthetic: seeded Poisson stream",
-        "config": {"workload": "serving", "meshes_n_tri": {k: len(v[1]) for k, v in meshes.items()},
-                   "ka": [1.0, 40.0], "rate_per_s": args.rate, "slots": 4, "refine_steps": args.refine},
-        "latency_s": {"p50": serve.percentile(lat, 50), "p99": serve.percentile(lat, 99),
-                      "per_mesh_p50": {k: serve.percentile(v, 50) for k, v in by.items()},
-                      "per_mesh_p99": {k: serve.percentile(v, 99) for k, v in by.items()}},
+        "config": {"workload": sv["workload"], "meshes_n_tri": sv["meshes_n_tri"], "rate_per_s": args.rate,
+                   "slots": 4, "refine_steps": args.refine},
+        "latency_s": {"p50": sv["p50_s"], "p99": sv["p99_s"]},
+        "serve": sv,
     }
     print(json.dumps(out))
 

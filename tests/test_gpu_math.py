@@ -42,9 +42,9 @@ def test_sincos64(nb):
         rng.uniform(-1e4, 1e4, 200000),
         np.exp(rng.uniform(np.log(1e-12), np.log(1e4), 100000)) * rng.choice([-1.0, 1.0], 100000),
         # around the quadrant boundaries n pi/4, where the reduction switches
-        (np.arange(-4000, 4001) * (np.pi / 4))[:, None] + np.array([-1e-9, 0.0, 1e-9])[None, :],
+        ((np.arange(-4000, 4001) * (np.pi / 4))[:, None] + np.array([-1e-9, 0.0, 1e-9])[None, :]).ravel(),
         [0.0, -0.0, 1e-300, np.pi / 2, np.pi, 1e4],
-    ]).ravel()
+    ])
     y = nb.eval_math("sincos", _dev(x)).cpu().numpy()
     es = np.abs(_ld(y[:, 0]) - np.sin(_ld(x))).max()
     ec = np.abs(_ld(y[:, 1]) - np.cos(_ld(x))).max()
