@@ -180,6 +180,27 @@ int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_poli
 int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy,
                       void *A, void *B, int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream);
 
+/* nb_assemble_sweep without any allocation -- no counterpart in NiHu: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as nb_assemble_sweep, restricted to its one-launch path (triangle
+ * meshes, fixed 1- or 7-point rule).  The per-call buffers (wavenumber
+ * constants and the FP64 static parts of the near-singular and diagonal
+ * pairs, 48 bytes per wavenumber plus 32 bytes per FP64 pair, each piece
+ * rounded up to 256 bytes) live in one 32 MiB region per device, held until
+ * the process ends; larger requests fail with NB_ERR_RANGE before anything
+ * runs.  The mesh's tables for the policy (thresholds, quadrature table,
+ * near-pair list) and the device's rule tables must be resident already: one
+ * nb_assemble_sweep call on the mesh with the policy builds them (else
+ * NB_ERR_UNSUPPORTED).  launch NULL takes the shape tuned for this mesh size
+ * if there is one, else the default; nothing is tuned.  The FP64 passes run
+ * on the caller's stream.  Calls on one device share its region: they take
+ * it one at a time and each returns only when its kernels are done, so no
+ * call sees another's contents half-written; nothing in the region needs to
+ * start at zero. */
+int nb_assemble_sweep_noalloc(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy, void *A,
+                              void *B, int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream);
+
 /* Pair classes as the engine sees them: cls[i * n_trial + j] = -1 diagonal,
  * -2 near-singular, otherwise the regular Dunavant degree of kernel (nx, ny).
  * cls is a device int8 array of n_test * n_trial. */

@@ -223,6 +223,18 @@ int device_rules(const nbk::Rules **out)
 	return NB_OK;
 }
 
+// the rule tables of the current device if they were built already (no allocation)
+int device_rules_ready(const nbk::Rules **out)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_rules_mu);
+	auto it = g_rules.find(dev);
+	if (it == g_rules.end()) return fail(NB_ERR_ARG, "the device's quadrature rule tables are not resident yet");
+	*out = (const nbk::Rules *)it->second.get();
+	return NB_OK;
+}
+
 // per-device canonical singular rules of the Galerkin pass (vertex, edge, face)
 struct SingRules {
 	DevPtr r[3];
@@ -380,14 +392,19 @@ void policy_degrees(const nb_policy &p, int km, int field_complexity, std::vecto
 	deg.erase(std::unique(deg.begin(), deg.end()), deg.end());
 }
 
+// the cached near-pair list of (test mesh, near policy), or null
+std::shared_ptr<NearList> find_near_list(const nb_mesh *trial, const nb_mesh *test, int same, const nb_policy &pol)
+{
+	for (auto &nl : trial->near_cache)
+		if (nl->test_uid == test->uid && nl->same == same && nl->mode == pol.near_mode && nl->limit == pol.near_limit)
+			return nl;
+	return nullptr;
+}
+
 int ensure_near_list(nb_mesh *trial, const nb_mesh *test, int same, const nb_policy &pol, hipStream_t st,
 	std::shared_ptr<NearList> *out)
 {
-	for (auto &nl : trial->near_cache)
-		if (nl->test_uid == test->uid && nl->same == same && nl->mode == pol.near_mode && nl->limit == pol.near_limit) {
-			*out = nl;
-			return NB_OK;
-		}
+	if ((*out = find_near_list(trial, test, same, pol))) return NB_OK;
 	std::shared_ptr<NearList> nl(new NearList());
 	nl->test_uid = test->uid; nl->same = same; nl->mode = pol.near_mode; nl->limit = pol.near_limit; nl->n = 0;
 	if (pol.near_mode != NB_NEAR_NONE) {
@@ -1177,8 +1194,44 @@ std::map<std::pair<int, int>, nb_launch> g_tuned;
 
 // Burton-Miller sweep.  tune_block > 0: only the regular pass on the leading
 // tune_block x tune_block block (autotuning); otherwise the full matrices.
+// Per-call device buffers of a sweep: allocated for the call, or carved from
+// a region given by the caller (256-byte aligned pieces; the region's size is
+// checked before anything runs).
+struct CallBufs {
+	char *base = nullptr;
+	size_t cap = 0, used = 0;
+	std::vector<DevPtr> owned;
+	static size_t piece(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+	int get(size_t bytes, void **out)
+	{
+		if (!base) {
+			owned.emplace_back();
+			int s = dev_alloc(owned.back(), bytes);
+			*out = owned.back().get();
+			return s;
+		}
+		if (used + piece(bytes) > cap) return fail(NB_ERR_RANGE, "scratch region exhausted");
+		*out = base + used;
+		used += piece(bytes);
+		return NB_OK;
+	}
+	template <class T>
+	int put(const std::vector<T> &v, hipStream_t st, void **out)
+	{
+		int s = get(v.size() * sizeof(T), out);
+		if (s) return s;
+		NB_HIP(hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+		return NB_OK;
+	}
+};
+
+// scratch: null -> per-call buffers are allocated; otherwise they are carved
+// from scratch[0, scratch_cap), every per-mesh table must already be
+// resident, no launch shape is tuned and the FP64 passes stay on the caller's
+// stream (nothing is allocated or created)
 int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy, void *A, void *B,
-	int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream, int tune_block, int regular_only)
+	int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream, int tune_block, int regular_only,
+	char *scratch = nullptr, size_t scratch_cap = 0)
 {
 	if (!mesh || !k || !A || !B) return fail(NB_ERR_ARG, "null argument");
 	if (n_k <= 0) return fail(NB_ERR_ARG, "n_k must be positive");
@@ -1198,6 +1251,11 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	const double two_pi = 6.283185307179586476925286766559005768;
 
 	const bool tri = mesh->npe == 3;
+	const bool noalloc = scratch != nullptr;
+	if (noalloc && !(tri && policy->rule == NB_RULE_FIXED && m->d_qt[policy->fixed_degree] &&
+		(m->qn[policy->fixed_degree] == 7 || m->qn[policy->fixed_degree] == 1)))
+		return fail(NB_ERR_UNSUPPORTED, "the allocation-free sweep takes triangle meshes with a fixed 1- or 7-point "
+			"rule whose table is resident (run nb_assemble_sweep once on this mesh and policy first)");
 	const int Q = tri && policy->rule == NB_RULE_FIXED ?
 		(ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
 	if (Q < 0) return NB_ERR_RANGE;
@@ -1228,7 +1286,7 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 			auto it = g_tuned.find({n, std::min<int>(n_k, 64)});
 			if (it != g_tuned.end()) { lc = it->second; found = true; }
 		}
-		if (!found && tune_block == 0) {
+		if (!found && tune_block == 0 && !noalloc) {
 			m->mu.unlock();
 			nb_launch best;
 			s = nb_autotune(mesh, n_k, policy, &best);
@@ -1239,10 +1297,34 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	}
 	if (lc.k_batch != 2 && lc.k_batch != 4 && lc.k_batch != 8) return fail(NB_ERR_ARG, "k_batch must be 2, 4 or 8");
 	if (lc.persistent <= 0 || lc.persistent > 32) return fail(NB_ERR_ARG, "persistent workgroups per CU must be 1..32");
-	if ((s = ensure_quad(m, policy->fixed_degree))) return s;
-	if ((s = ensure_thresholds(m, policy->near_mode, policy->near_limit))) return s;
+	if ((long long)((nblk + 255) / 256) * ((nblk + lc.tile_cols - 1) / lc.tile_cols) *
+		((n_k + lc.k_batch - 1) / lc.k_batch) > 0x7fffffffLL)
+		return fail(NB_ERR_RANGE, "too many work items");
 	std::shared_ptr<NearList> nl;
-	if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
+	const nbk::Rules *rules = nullptr;
+	if (noalloc) {
+		if (!m->d_thr || m->thr_mode != policy->near_mode || m->thr_limit != policy->near_limit ||
+			!(nl = find_near_list(m, m, 1, *policy)))
+			return fail(NB_ERR_UNSUPPORTED, "the allocation-free sweep needs the mesh's tables for this policy "
+				"resident (run nb_assemble_sweep once on this mesh and policy first)");
+		if (!regular_only && (s = device_rules_ready(&rules))) return s;
+		// the whole per-call region, checked before anything runs
+		const size_t np = (size_t)nl->n + n;
+		const size_t need = 2 * CallBufs::piece(8 * (size_t)n_k) + 2 * CallBufs::piece(4 * (size_t)n_k) +
+			CallBufs::piece(8 * (size_t)n_k) + CallBufs::piece(16 * (size_t)n_k) +
+			(regular_only ? 0 : CallBufs::piece(np * 4 * sizeof(double)));
+		if (need > scratch_cap)
+			return fail(NB_ERR_RANGE, "the allocation-free sweep needs " + std::to_string(need) + " bytes of scratch (" +
+				std::to_string(n_k) + " wavenumbers, " + std::to_string(np) + " FP64 pairs); the limit is " +
+				std::to_string(scratch_cap) + " bytes");
+	} else {
+		if ((s = ensure_quad(m, policy->fixed_degree))) return s;
+		if ((s = ensure_thresholds(m, policy->near_mode, policy->near_limit))) return s;
+		if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
+	}
+	CallBufs bufs;
+	bufs.base = scratch;
+	bufs.cap = scratch_cap;
 
 	// per-wavenumber constants
 	std::vector<double> krev(n_k), kd(n_k);
@@ -1257,9 +1339,9 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 		alf[f] = make_float2(0.0f, (float)(-1.0 / k[f]));
 		ald[f] = make_double2(0.0, -1.0 / k[f]);
 	}
-	DevPtr d_krev, d_kd, d_kf, d_krf, d_alf, d_ald;
-	if ((s = upload(d_krev, krev)) || (s = upload(d_kd, kd)) || (s = upload(d_kf, kf)) || (s = upload(d_krf, krf)) ||
-		(s = upload(d_alf, alf)) || (s = upload(d_ald, ald)))
+	void *d_krev, *d_kd, *d_kf, *d_krf, *d_alf, *d_ald;
+	if ((s = bufs.put(krev, st, &d_krev)) || (s = bufs.put(kd, st, &d_kd)) || (s = bufs.put(kf, st, &d_kf)) ||
+		(s = bufs.put(krf, st, &d_krf)) || (s = bufs.put(alf, st, &d_alf)) || (s = bufs.put(ald, st, &d_ald)))
 		return s;
 
 	int n_cu = 256;
@@ -1272,29 +1354,33 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	// FP64 passes on the side stream (static once per pair, dynamic per wavenumber)
 	hipStream_t side = st;
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-	DevPtr d_static;
-	const nbk::Rules *rules = nullptr;
+	const bool forked = !regular_only && !noalloc;
 	if (!regular_only) {
-		if ((s = device_rules(&rules))) return s;
-		if ((s = side_stream(&side))) return s;
-		if (!side) side = st;
+		if (!noalloc && (s = device_rules(&rules))) return s;
+		if (forked) {
+			if ((s = side_stream(&side))) return s;
+			if (!side) side = st;
+		}
 		const int np = nl->n + n;
-		if ((s = dev_alloc(d_static, (size_t)np * 4 * sizeof(double)))) return s;
-		NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-		NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-		NB_HIP(hipEventRecord(ev_fork, st));
-		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+		void *d_static = nullptr;
+		if ((s = bufs.get((size_t)np * 4 * sizeof(double), &d_static))) return s;
+		if (forked) {
+			NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+			NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+			NB_HIP(hipEventRecord(ev_fork, st));
+			NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+		}
 		nbk::StaticArgs sa;
 		sa.n = n; sa.n_near = nl->n; sa.pi = (const int *)nl->pi.get(); sa.pj = (const int *)nl->pj.get();
 		sa.c = (const double *)m->d_c.get(); sa.nu64 = (const double *)m->d_nu64.get();
-		sa.X = (const double *)m->d_X.get(); sa.rules = rules; sa.st = (double *)d_static.get();
+		sa.X = (const double *)m->d_X.get(); sa.rules = rules; sa.st = (double *)d_static;
 		const int pe1 = prof_begin(1, side);
 		hipLaunchKernelGGL(nbk::fp64_static_kernel, dim3((np + 63) / 64), dim3(64), 0, side, sa);
 		NB_HIP(hipGetLastError());
 		nbk::DynArgs da;
 		da.n = n; da.n_near = nl->n; da.n_k = n_k; da.pi = sa.pi; da.pj = sa.pj; da.c = sa.c; da.nu64 = sa.nu64;
 		da.X = sa.X; da.J = (const double *)m->d_J.get(); da.rules = rules; da.st = sa.st;
-		da.k = (const double *)d_kd.get(); da.alpha = (const double2 *)d_ald.get();
+		da.k = (const double *)d_kd; da.alpha = (const double2 *)d_ald;
 		da.A = (float2 *)A; da.B = (float2 *)B; da.ld = ld; da.kstride = k_stride;
 		// uniformly spaced wavenumbers (to 1e-12 relative): the near pairs rotate
 		// e^{-ikr} from one wavenumber to the next instead of evaluating sincos
@@ -1324,8 +1410,8 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	sw.n_items = (int)items;
 	sw.c = (const double *)m->d_c.get(); sw.nu = (const float *)m->d_nuf.get(); sw.thr = (const double *)m->d_thr.get();
 	sw.qt = (const nbk::QPoint *)m->d_qt[policy->fixed_degree].get();
-	sw.k_rev = (const double *)d_krev.get(); sw.kf = (const float *)d_kf.get(); sw.krevf = (const float *)d_krf.get();
-	sw.alpha = (const float2 *)d_alf.get();
+	sw.k_rev = (const double *)d_krev; sw.kf = (const float *)d_kf; sw.krevf = (const float *)d_krf;
+	sw.alpha = (const float2 *)d_alf;
 	sw.A = (float2 *)A; sw.B = (float2 *)B; sw.ld = ld; sw.kstride = k_stride;
 	const long long grid = std::min<long long>(items, (long long)lc.persistent * n_cu);
 	const int pe = prof_begin(3, st);
@@ -1339,15 +1425,48 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 #undef NB_SW
 	NB_HIP(hipGetLastError());
 	prof_end(pe, st);
-	if (!regular_only) {
+	if (forked) {
 		NB_HIP(hipEventRecord(ev_join, side));
 		NB_HIP(hipStreamWaitEvent(st, ev_join, 0));
 		NB_HIP(hipEventDestroy(ev_fork));
 		NB_HIP(hipEventDestroy(ev_join));
 	}
 	// the per-call buffers (wavenumber constants, static parts) are freed on
-	// return: wait for the kernels that read them
+	// return, or the scratch region is handed to the next call: wait for the
+	// kernels that read them
 	NB_HIP(hipStreamSynchronize(st));
+	return NB_OK;
+}
+
+// The allocation-free sweep's per-call region (wavenumber constants and the
+// FP64 static parts): one array per device, resident until the process ends.
+// Host code uses the address hipGetSymbolAddress gives for the current
+// device, never the array's name.  Nothing in it has to start at zero: every
+// piece is written before it is read within one call.
+constexpr size_t NB_NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NB_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_ptr;
+// calls on one device use its region one at a time (each call waits for its
+// kernels before it returns)
+std::map<int, std::unique_ptr<std::mutex> > g_scratch_use;
+
+int scratch_region(char **out, std::mutex **use)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	auto it = g_scratch_ptr.find(dev);
+	if (it == g_scratch_ptr.end()) {
+		void *p = nullptr;
+		NB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (!p) return fail(NB_ERR_DEVICE, "hipGetSymbolAddress gave no address for the scratch region");
+		it = g_scratch_ptr.emplace(dev, (char *)p).first;
+		g_scratch_use.emplace(dev, std::unique_ptr<std::mutex>(new std::mutex()));
+	}
+	*out = it->second;
+	*use = g_scratch_use[dev].get();
 	return NB_OK;
 }
 
@@ -1658,6 +1777,17 @@ int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const n
 	int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream)
 {
 	return sweep_impl(mesh, k, n_k, policy, A, B, ld, k_stride, launch, stream, 0, 0);
+}
+
+int nb_assemble_sweep_noalloc(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy, void *A,
+	void *B, int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream)
+{
+	char *region = nullptr;
+	std::mutex *use = nullptr;
+	int s;
+	if ((s = scratch_region(&region, &use))) return s;
+	std::lock_guard<std::mutex> lk(*use);
+	return sweep_impl(mesh, k, n_k, policy, A, B, ld, k_stride, launch, stream, 0, 0, region, NB_NOALLOC_SCRATCH_BYTES);
 }
 
 int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_launch *best)

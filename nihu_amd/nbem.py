@@ -30,7 +30,8 @@ FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_create_quads",
            "nb_mesh_create_elements", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
-           "nb_synchronize", "nb_profile", "nb_profile_read", "nb_eval_math"]
+           "nb_synchronize", "nb_profile", "nb_profile_read", "nb_eval_math",
+           "nb_assemble_sweep_noalloc"]
 
 
 class NbKernel(C.Structure):
@@ -82,6 +83,8 @@ def lib():
                                      C.POINTER(NbOut), P]
         l.nb_assemble_sweep.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
                                         C.POINTER(NbLaunch), P]
+        l.nb_assemble_sweep_noalloc.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
+                                                C.POINTER(NbLaunch), P]
         l.nb_classify.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(NbPolicy), P, P]
         l.nb_autotune.argtypes = [P, C.c_int32, C.POINTER(NbPolicy), C.POINTER(NbLaunch)]
         l.nb_set_launch.argtypes = [C.POINTER(NbLaunch)]
@@ -283,9 +286,11 @@ def assemble_bm(mesh: Mesh, k, A, B, alpha=None, pol=None, accumulate=False, str
     return A, B
 
 
-def assemble_sweep(mesh: Mesh, ks, A, B, pol=None, launch=None, stream=None):
+def assemble_sweep(mesh: Mesh, ks, A, B, pol=None, launch=None, stream=None, noalloc=False):
     """A, B: torch complex64 tensors of shape (n_k, n, n) holding column-major
-    matrices (e.g. a transposed view per frequency); alpha = -i/k each."""
+    matrices (e.g. a transposed view per frequency); alpha = -i/k each.
+    noalloc: nb_assemble_sweep_noalloc (nothing allocated; the mesh's tables
+    for the policy must be resident from an earlier sweep)."""
     pol = pol or policy(**NIHU_POLICY)
     ks = np.ascontiguousarray(ks, dtype=np.float64)
     n = mesh.n
@@ -295,8 +300,9 @@ def assemble_sweep(mesh: Mesh, ks, A, B, pol=None, launch=None, stream=None):
         raise ValueError("A and B must hold column-major matrices with equal strides")
     ld, kst = A.stride(2), A.stride(0)
     lp = C.byref(launch) if launch is not None else None
-    check(lib().nb_assemble_sweep(mesh.handle, ks.ctypes.data, len(ks), C.byref(pol), A.data_ptr(), B.data_ptr(),
-                                  ld, kst, lp, _stream_ptr(stream)))
+    fn = lib().nb_assemble_sweep_noalloc if noalloc else lib().nb_assemble_sweep
+    check(fn(mesh.handle, ks.ctypes.data, len(ks), C.byref(pol), A.data_ptr(), B.data_ptr(), ld, kst, lp,
+             _stream_ptr(stream)))
     return A, B
 
 
