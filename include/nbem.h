@@ -214,6 +214,18 @@ int nb_set_launch(const nb_launch *launch);
 
 int nb_synchronize(void *stream);
 
+/* y = A x for the solve step: A a column-major complex64 matrix (rows x
+ * cols, leading dimension ld, e.g. an assembled Burton-Miller matrix), x and
+ * y complex128 vectors, all products and sums in FP64 (the sum over columns
+ * in a fixed order: the result does not depend on scheduling).  work is a
+ * device buffer of at least nb_gemv_workspace(rows, cols) bytes.  Stream-
+ * ordered; no counterpart in NiHu, whose Burton-Miller application hands the
+ * complex128 matrix to Eigen's GMRES
+ * (nihu/applications/helmholtz/helmholtz_3d_coll.cpp:99-108). */
+int nb_gemv_workspace(int64_t rows, int64_t cols, int64_t *bytes);
+int nb_gemv_c64(const void *A, int64_t rows, int64_t cols, int64_t ld, const void *x, void *y, void *work,
+                int64_t work_bytes, void *stream);
+
 /* Test support (no NiHu counterpart): evaluate the engine's own FP64 device
  * routines of the near-singular and singular passes on n device doubles x
  * into y[2 n]: fn 0 sin and cos (y[2i], y[2i+1]; Cody-Waite reduction and

@@ -22,6 +22,7 @@
 #include "nbem_sweep.hpp"
 #include "nbem_galerkin.hpp"
 #include "nbem_quad.hpp"
+#include "nbem_solve.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -301,6 +302,11 @@ struct nb_mesh {
 	std::unique_ptr<nb_mesh> view;
 	// quadrilateral near pass: count of pairs without an inverse image
 	DevPtr d_status;
+	// the sweep's per-call buffers (wavenumber constants, FP64 static parts),
+	// kept between calls and grown when a call needs more: a hipFree per call
+	// would wait for every stream of the device
+	DevPtr sweep_ws;
+	size_t sweep_ws_bytes = 0;
 	std::mutex mu;
 	int64_t bytes = 0;
 };
@@ -1323,8 +1329,27 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 		if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
 	}
 	CallBufs bufs;
-	bufs.base = scratch;
-	bufs.cap = scratch_cap;
+	if (noalloc) {
+		bufs.base = scratch;
+		bufs.cap = scratch_cap;
+	} else {
+		// the mesh's own workspace (this call holds the mesh lock and waits for
+		// its kernels before it returns, so the next call may reuse it)
+		const size_t np = (size_t)nl->n + n;
+		const size_t need = 2 * CallBufs::piece(8 * (size_t)n_k) + 2 * CallBufs::piece(4 * (size_t)n_k) +
+			CallBufs::piece(8 * (size_t)n_k) + CallBufs::piece(16 * (size_t)n_k) +
+			(regular_only ? 0 : CallBufs::piece(np * 4 * sizeof(double)));
+		if (m->sweep_ws_bytes < need) {
+			m->bytes -= (int64_t)m->sweep_ws_bytes;
+			m->sweep_ws.reset();
+			m->sweep_ws_bytes = 0;
+			if ((s = dev_alloc(m->sweep_ws, need))) return s;
+			m->sweep_ws_bytes = need;
+			m->bytes += (int64_t)need;
+		}
+		bufs.base = (char *)m->sweep_ws.get();
+		bufs.cap = m->sweep_ws_bytes;
+	}
 
 	// per-wavenumber constants
 	std::vector<double> krev(n_k), kd(n_k);
@@ -1883,6 +1908,45 @@ int nb_profile_read(double *ms, int64_t *launches, int32_t reset)
 		if (launches) launches[p] = g_prof_n[p];
 		if (reset) { g_prof_ms[p] = 0.0; g_prof_n[p] = 0; }
 	}
+	return NB_OK;
+}
+
+int nb_gemv_workspace(int64_t rows, int64_t cols, int64_t *bytes)
+{
+	if (!bytes) return fail(NB_ERR_ARG, "null argument");
+	if (rows < 0 || cols < 0) return fail(NB_ERR_ARG, "negative size");
+	if (rows == 0 || cols == 0) { *bytes = 0; return NB_OK; }
+	int rt, ch;
+	long long cl;
+	nbk::gemv_shape(rows, cols, &rt, &ch, &cl);
+	*bytes = (int64_t)ch * rows * 16;
+	return NB_OK;
+}
+
+int nb_gemv_c64(const void *A, int64_t rows, int64_t cols, int64_t ld, const void *x, void *y, void *work,
+	int64_t work_bytes, void *stream)
+{
+	if (rows < 0 || cols < 0) return fail(NB_ERR_ARG, "negative size");
+	if (rows == 0) return NB_OK;
+	if (!A || !x || !y || (cols > 0 && !work)) return fail(NB_ERR_ARG, "null argument");
+	if (ld < rows) return fail(NB_ERR_ARG, "leading dimension smaller than the number of rows");
+	hipStream_t st = (hipStream_t)stream;
+	if (cols == 0) {
+		hipLaunchKernelGGL(nbk::zero_kernel, dim3(1, 1), dim3(256), 0, st, y, (long long)rows, (long long)rows, 1LL, 16);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	}
+	nbk::GemvArgs g;
+	nbk::gemv_shape(rows, cols, &g.row_tiles, &g.chunks, &g.clen);
+	if ((int64_t)g.chunks * rows * 16 > work_bytes) return fail(NB_ERR_ARG, "workspace smaller than nb_gemv_workspace");
+	const long long blocks = (long long)g.row_tiles * g.chunks;
+	if (blocks > 0x7fffffffLL) return fail(NB_ERR_RANGE, "matrix too large");
+	g.A = (const float2 *)A; g.rows = rows; g.cols = cols; g.ld = ld;
+	g.x = (const double2 *)x; g.y = (double2 *)y; g.work = (double2 *)work;
+	hipLaunchKernelGGL(nbk::gemv_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, st, g);
+	NB_HIP(hipGetLastError());
+	hipLaunchKernelGGL(nbk::gemv_reduce_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, g);
+	NB_HIP(hipGetLastError());
 	return NB_OK;
 }
 

@@ -258,3 +258,55 @@ def from_nihu_elements(elements):
     if np.all(elements[:, 0] == NIHU_QUAD_1_ID) and elements.shape[1] >= 5:
         return np.ascontiguousarray(elements[:, 1:5].astype(np.int32))
     raise ValueError("a table with both tria_1 (32303) and quad_1 (32404) rows: use nbem.Mesh.from_elements")
+
+
+def sphere_boundary(r0: float, n0: int):
+    """NiHu's blown-up cube sphere (nihu/matlab/meshing/create_sphere_boundary.m):
+    the boundary of the cube [-1, 1]^3 with 2 n0 quadrilaterals along each
+    edge, nodes moved radially onto the sphere of radius r0 (node * max|x_i| /
+    |x|, i.e. node / |node| on the cube's surface), normals outwards;
+    6 (2 n0)^2 elements.  The node set and the elements are those of the
+    MATLAB generator (numbering and corner start may differ; NiHu's
+    quad_1 integrals do not depend on them beyond rounding)."""
+    return cubed_sphere(2 * n0, radius=r0)
+
+
+def create_slab(corners, n1: int, n2: int):
+    """A bilinear patch of n1 x n2 quadrilaterals
+    (nihu/matlab/meshing/create_slab.m with four corner rows): the grid of
+    [-1, 1]^2, n1 segments along the first reference axis, n2 along the
+    second, mapped by the bilinear shape functions of corners (4 x 3, ordered
+    (-1,-1), (1,-1), (1,1), (-1,1)); nodes row by row, element (i, j) =
+    (n(i, j), n(i+1, j), n(i+1, j+1), n(i, j+1))."""
+    R = np.asarray(corners, dtype=np.float64).reshape(4, 3)
+    xi = np.linspace(-1.0, 1.0, n1 + 1)
+    eta = np.linspace(-1.0, 1.0, n2 + 1)
+    E, X = np.meshgrid(eta, xi, indexing="ij")        # node (i, j) at row j * (n1 + 1) + i
+    X, E = X.ravel(), E.ravel()
+    phi = np.stack([(1 - X) * (1 - E), (1 + X) * (1 - E), (1 + X) * (1 + E), (1 - X) * (1 + E)], axis=1) / 4.0
+    nodes = phi @ R
+    idx = lambda i, j: j * (n1 + 1) + i
+    quads = [[idx(i, j), idx(i + 1, j), idx(i + 1, j + 1), idx(i, j + 1)] for j in range(n2) for i in range(n1)]
+    return np.ascontiguousarray(nodes), np.ascontiguousarray(np.array(quads, dtype=np.int32))
+
+
+def select_elements(nodes, elems, node_predicate):
+    """Elements all of whose nodes satisfy node_predicate(nodes) (a boolean
+    per node): mesh_select(..., 'all') of nihu/matlab/meshing/mesh_select.m.
+    Returns (nodes, kept elements) with the node table unchanged."""
+    ok = np.asarray(node_predicate(np.asarray(nodes)), dtype=bool)
+    keep = ok[np.asarray(elems)].all(axis=1)
+    return nodes, np.ascontiguousarray(np.asarray(elems)[keep])
+
+
+def edge_size(nodes, elems):
+    """Longest edge of each element (nihu/matlab/meshing/mesh_edge_size.m)."""
+    X = np.asarray(nodes)[np.asarray(elems)]
+    m = X.shape[1]
+    return np.max(np.stack([np.linalg.norm(X[:, (a + 1) % m] - X[:, a], axis=1) for a in range(m)], axis=1), axis=1)
+
+
+def mesh_kmax(nodes, elems, ratio: float = 8.0):
+    """Largest wavenumber per element at `ratio` elements per wavelength:
+    2 pi / (edge size * ratio) (nihu/matlab/meshing/mesh_kmax.m)."""
+    return 2.0 * np.pi / (edge_size(nodes, elems) * ratio)

@@ -31,7 +31,7 @@ EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "n
            "nb_mesh_create_elements", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
            "nb_synchronize", "nb_profile", "nb_profile_read", "nb_eval_math",
-           "nb_assemble_sweep_noalloc"]
+           "nb_assemble_sweep_noalloc", "nb_gemv_workspace", "nb_gemv_c64"]
 
 
 class NbKernel(C.Structure):
@@ -90,6 +90,8 @@ def lib():
         l.nb_set_launch.argtypes = [C.POINTER(NbLaunch)]
         l.nb_synchronize.argtypes = [P]
         l.nb_eval_math.argtypes = [C.c_int32, P, P, C.c_int64, P]
+        l.nb_gemv_workspace.argtypes = [C.c_int64, C.c_int64, C.POINTER(C.c_int64)]
+        l.nb_gemv_c64.argtypes = [P, C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P]
         l.nb_profile.argtypes = [C.c_int32]
         l.nb_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib = l
@@ -325,6 +327,37 @@ def autotune(mesh: Mesh, n_k=1, pol=None):
 
 def set_launch(tile_cols=0, k_batch=0, persistent=0):
     check(lib().nb_set_launch(C.byref(NbLaunch(tile_cols, k_batch, persistent))))
+
+
+def gemv_workspace(A):
+    """The work buffer nb_gemv_c64 needs for the matrix A (a torch tensor)."""
+    import torch
+    b = C.c_int64()
+    check(lib().nb_gemv_workspace(A.shape[0], A.shape[1], C.byref(b)))
+    return torch.empty(max(1, b.value // 16), dtype=torch.complex128, device=A.device)
+
+
+def gemv(A, x, y=None, work=None, stream=None):
+    """y = A x in FP64 for a column-major complex64 GPU matrix A (stride(0)
+    == 1) and a complex128 GPU vector x (nb_gemv_c64); returns y."""
+    import torch
+    if A.dtype != torch.complex64 or not A.is_cuda or A.dim() != 2:
+        raise ValueError("A must be a complex64 GPU matrix")
+    if A.stride(0) != 1 and A.shape[0] > 1:
+        raise ValueError("A must be column-major")
+    rows, cols = A.shape
+    x = x.to(torch.complex128).contiguous()
+    if x.shape != (cols,):
+        raise ValueError("x must have one entry per column of A")
+    if y is None:
+        y = torch.empty(rows, dtype=torch.complex128, device=A.device)
+    if work is None:
+        work = gemv_workspace(A)
+    ld = A.stride(1) if cols > 1 else rows
+    check(lib().nb_gemv_c64(C.c_void_p(A.data_ptr()), rows, cols, ld, C.c_void_p(x.data_ptr()),
+                            C.c_void_p(y.data_ptr()), C.c_void_p(work.data_ptr()), work.numel() * 16,
+                            _stream_ptr(stream)))
+    return y
 
 
 MATH_ROUTINES = {"sincos": 0, "log": 1, "rsqrt": 2, "rcp": 3}

@@ -5,9 +5,12 @@ p_inc = exp(-i k d.x) (unit amplitude, direction d) on one of the resident
 meshes: the Burton-Miller collocation system
     A p = B q,   A = alpha*D - I/2 + H,   B = G + alpha*Ht + alpha/2 I,
     alpha = -i/k,  q = -dp_inc/dn = i k (d.n) exp(-i k d.x)
-(the composition of nihu/applications/helmholtz/helmholtz_3d_coll.cpp:35-120,
-with a dense LU solve where that application uses GMRES) is assembled by the
-HIP engine and solved by LU on the device.
+(the composition of nihu/applications/helmholtz/helmholtz_3d_coll.cpp:35-120)
+is assembled by the HIP engine and solved on the device, by default as that
+application does, with unpreconditioned GMRES to a relative residual of 1e-8
+(nihu_amd/solve.py: products with the complex64 matrices by the engine's
+FP64-accumulating kernel, complex128 Krylov basis), or by a complex64 LU
+factorisation (solver="lu", optionally with FP64 iterative refinement).
 
 Scheduling: requests arrive over time (their `arrival` offsets); the loop
 admits the oldest waiting request as soon as a slot (a HIP stream with a host
@@ -38,7 +41,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import nbem
+from . import nbem, solve
 
 
 @dataclasses.dataclass
@@ -66,19 +69,30 @@ class Result:
         return self.done - self.arrival
 
 
-def footprint_bytes(n: int) -> int:
-    """Device bytes a request holds while admitted: A, B and the LU factor
-    (complex64 n x n each) plus vectors and pivots."""
+def footprint_bytes(n: int, solver: str = "lu", restart: int = 3000) -> int:
+    """Device bytes a request holds while admitted: A and B (complex64 n x n
+    each) and, for LU, the factor (complex64 n x n) plus vectors and pivots;
+    for GMRES the complex128 Krylov basis of min(restart, n) + 1 vectors and
+    the product's work buffer instead of the factor."""
+    if solver == "gmres":
+        return 2 * 8 * n * n + 16 * n * (min(restart, n) + 1) + 16 * n * 80 + 64 * n
     return 3 * 8 * n * n + 64 * n
 
 
 class Server:
     def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None, refine: int = 0,
-                 max_batch: int = 8):
-        """refine: steps of iterative refinement of the complex64 LU solution
-        (residual b - A p accumulated in complex128 over column blocks,
-        correction solved with the same LU factor).  max_batch: requests on
-        one mesh assembled in one sweep launch (1: one assembly per request)."""
+                 max_batch: int = 8, solver: str = "gmres", tol: float = 1e-8, restart: int = 3000):
+        """solver: "gmres" (NiHu's application: unpreconditioned GMRES, tol
+        on the relative residual, restart) or "lu" (complex64 LU).  refine:
+        steps of iterative refinement of the complex64 LU solution (residual
+        b - A p accumulated in complex128 over column blocks, correction
+        solved with the same LU factor).  max_batch: requests on one mesh
+        assembled in one sweep launch (1: one assembly per request)."""
+        if solver not in ("gmres", "lu"):
+            raise ValueError("solver must be 'gmres' or 'lu'")
+        self.solver = solver
+        self.tol = float(tol)
+        self.restart = int(restart)
         self.refine = int(refine)
         self.max_batch = max(1, int(max_batch))
         import torch
@@ -101,7 +115,7 @@ class Server:
         self.policy = policy or nbem.policy(**nbem.CONFIG_POLICY)
         self.quad_policy = policy or nbem.policy(rule="fixed", degree=5, near="mean_edge", limit=1.5)
         for key, m in self.meshes.items():
-            if footprint_bytes(m.n) > self.budget:
+            if self._footprint(m.n) > self.budget:
                 raise ValueError("mesh %s does not fit the HBM budget" % key)
         # launch shape of the sweep per mesh, tuned once here for batches of
         # up to max_batch wavenumbers (so that no request pays for tuning)
@@ -116,6 +130,9 @@ class Server:
 
     def _policy(self, mesh):
         return self.quad_policy if mesh.nodes_per_element != 3 else self.policy
+
+    def _footprint(self, n):
+        return footprint_bytes(n, self.solver, self.restart)
 
     def _launch(self, reqs, stream):
         """One batch of requests on one mesh: the Burton-Miller matrices of
@@ -140,15 +157,25 @@ class Server:
                 phase = -req.k * (c @ d)
                 pinc = torch.polar(torch.ones_like(phase), phase)
                 q = (1j * req.k) * (nu @ d) * pinc
-                rhs = (B[g] @ q.to(torch.complex64)).unsqueeze(1)
-                LU, piv, _ = torch.linalg.lu_factor_ex(A[g])
-                p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
-                if self.refine > 0:
-                    p = _refine(A[g], LU, piv, rhs.squeeze(1), p, self.refine)
+                if self.solver == "gmres":
+                    rhs = nbem.gemv(B[g], q)
+                    res = solve.gmres(A[g], rhs, tol=self.tol, restart=self.restart)
+                    if not res.converged:
+                        raise RuntimeError("GMRES did not reach the relative residual %g (%g after %d iterations)"
+                                           % (self.tol, res.residual, res.iterations))
+                    p = res.x
+                    keep = (A, B, rhs, p)
+                else:
+                    rhs = (B[g] @ q.to(torch.complex64)).unsqueeze(1)
+                    LU, piv, _ = torch.linalg.lu_factor_ex(A[g])
+                    p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
+                    if self.refine > 0:
+                        p = _refine(A[g], LU, piv, rhs.squeeze(1), p, self.refine)
+                    keep = (A, B, LU, piv, rhs, p)
                 ev = torch.cuda.Event()
                 ev.record(stream)
                 # keep the buffers alive until the event has passed
-                out.append((ev, (A, B, LU, piv, rhs, p)))
+                out.append((ev, keep))
         return out
 
     def run(self, requests, realtime: bool = True):
@@ -171,7 +198,7 @@ class Server:
                     head = pending[0]
                     if realtime and head.arrival > now():
                         break
-                    fp = footprint_bytes(self.meshes[head.mesh].n)
+                    fp = self._footprint(self.meshes[head.mesh].n)
                     if in_use + fp > self.budget:
                         break
                     batch = [head]

@@ -36,7 +36,7 @@ def _p(a):
 
 def test_header_symbols_exported(nb):
     hdr = open(os.path.join(ROOT, "include", "nbem.h")).read()
-    declared = sorted(set(re.findall(r"\b(nb_[a-z_]+)\s*\(", hdr)))
+    declared = sorted(set(re.findall(r"\b(nb_[a-z0-9_]+)\s*\(", hdr)))
     assert set(declared) == set(nb.EXPORTS), declared
     l = nb.lib()
     for name in declared:

@@ -57,7 +57,7 @@ def test_same_alone_and_in_mix(meshes):
 def test_admission_by_footprint(meshes):
     from nihu_amd import serve
     n4 = len(meshes["L4"][1])
-    budget = 1.5 * serve.footprint_bytes(n4)
+    budget = 1.5 * serve.footprint_bytes(n4, "gmres")
     srv = serve.Server({"L4": meshes["L4"]}, slots=4, hbm_budget=budget)
     reqs = [serve.Request(i, "L4", 2.0 + i, (1.0, 0.0, 0.0)) for i in range(3)]
     res = sorted(srv.run(reqs, realtime=False), key=lambda r: r.admitted)
@@ -115,9 +115,10 @@ def test_iterative_refinement(meshes):
     e2 = (torch.linalg.norm(p2 - exact) / torch.linalg.norm(exact)).item()
     print("complex64 LU %.2e, refined twice %.2e" % (e0, e2), flush=True)
     assert e2 < 1e-3 * e0 or e2 < 1e-12
-    plain = serve.Server({"L3": (nodes, tris)}, slots=1).run([serve.Request(0, "L3", k, tuple(d))], realtime=False)[0].p
-    refd = serve.Server({"L3": (nodes, tris)}, slots=1, refine=2).run([serve.Request(0, "L3", k, tuple(d))],
-                                                                        realtime=False)[0].p
+    plain = serve.Server({"L3": (nodes, tris)}, slots=1, solver="lu").run([serve.Request(0, "L3", k, tuple(d))],
+                                                                          realtime=False)[0].p
+    refd = serve.Server({"L3": (nodes, tris)}, slots=1, refine=2, solver="lu").run(
+        [serve.Request(0, "L3", k, tuple(d))], realtime=False)[0].p
     assert np.linalg.norm(refd - plain) / np.linalg.norm(refd) < 10 * e0 + 1e-6
 
 
@@ -144,10 +145,10 @@ def test_config5_stream_with_20480_triangles(orc):
     """The serving workload's meshes (1280, 5120 and 20480-triangle jittered
     icospheres, seed 0), nine requests with ka in [1, 40] arriving together,
     three per mesh, so that requests on one mesh are batched into one sweep
-    launch: the first 20480-triangle request gives the same surface field
-    alone and in the mix (1e-6), and one served solution per mesh matches
-    the complex128 solve of the oracle's FP64 matrices (relative 2-norm
-    1e-5; the complex64 LU alone is about 3e-7 from it)."""
+    launch, each solved by GMRES to a relative residual of 1e-8: the first
+    20480-triangle request gives the same surface field alone and in the mix
+    (1e-6), and one served solution per mesh matches the complex128 solve of
+    the oracle's FP64 matrices (relative 2-norm 1e-5)."""
     from nihu_amd import serve
     from nihu_amd.mesh import jittered_icosphere
     meshes = {"L3": jittered_icosphere(3, seed=0), "L4": jittered_icosphere(4, seed=0),
@@ -190,3 +191,37 @@ def test_batched_equals_alone(meshes):
         diff = np.abs(alone.p - batched[rq.rid].p).max() / np.abs(alone.p).max()
         print("request %d %s ka %.2f: batched vs alone %.2e" % (rq.rid, rq.mesh, rq.k, diff), flush=True)
         assert diff <= 1e-6
+
+
+def test_gemv_and_gmres_against_lu(meshes):
+    """nb_gemv_c64 (complex64 matrix, FP64 products and sums) against the
+    complex128 product of the same matrix, bit-stable from call to call, and
+    the server's GMRES solution against the complex128 LU solve of the same
+    complex64 system."""
+    import torch
+    from nihu_amd import nbem, serve, solve
+    nodes, tris = meshes["L4"]
+    k, d = 12.0, np.array([0.6, 0.0, 0.8])
+    m = nbem.Mesh(nodes, tris)
+    n = len(tris)
+    A, B = nbem.colmajor(n, n), nbem.colmajor(n, n)
+    nbem.assemble_bm(m, k, A, B, pol=nbem.policy(**nbem.CONFIG_POLICY))
+    x = torch.randn(n, dtype=torch.complex128, device="cuda")
+    y = nbem.gemv(A, x)
+    ref = A.to(torch.complex128) @ x
+    e = (torch.linalg.norm(y - ref) / torch.linalg.norm(ref)).item()
+    assert e < 1e-14, e
+    assert torch.equal(nbem.gemv(A, x), y)
+    # a sub-block with leading dimension > rows, and an odd number of columns
+    As = A[5:1000, 3:1234]
+    ys = nbem.gemv(As, x[3:1234])
+    assert (torch.linalg.norm(ys - As.to(torch.complex128) @ x[3:1234]) / torch.linalg.norm(ys)).item() < 1e-14
+    c, nu = serve._centres_normals(nodes, tris)
+    q = torch.from_numpy((1j * k) * (nu @ d) * np.exp(-1j * k * (c @ d))).cuda()
+    b = nbem.gemv(B, q)
+    r = solve.gmres(A, b, tol=1e-10)
+    exact = torch.linalg.solve(A.to(torch.complex128), b.unsqueeze(1)).squeeze(1)
+    err = (torch.linalg.norm(r.x - exact) / torch.linalg.norm(exact)).item()
+    print("GMRES: %d iterations, residual %.2e, error vs complex128 LU %.2e" % (r.iterations, r.residual, err),
+          flush=True)
+    assert r.converged and err < 1e-8
