@@ -88,10 +88,36 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *uni(const 
 	return (const __attribute__((address_space(4))) T *)(size_t)p;
 }
 
+// The row's data (loaded once, before the column loop) are consumed here, so
+// that the wait for their loads happens once, before the loop.  Vector stores
+// count in the same counter as vector loads: a first use of these registers
+// inside the loop makes every iteration wait until the previous columns'
+// stores have drained.
+__device__ __forceinline__ void row_ready(double x, double y, double z, float a, float b, float c, int r)
+{
+	__asm__ volatile("" ::"v"(x), "v"(y), "v"(z), "v"(a), "v"(b), "v"(c), "v"(r));
+}
+
 __device__ __forceinline__ double d2_pair(double ax, double ay, double az, double bx, double by, double bz)
 {
 	double dx = ax - bx, dy = ay - by, dz = az - bz;
 	return __fma_rn(dz, dz, __fma_rn(dy, dy, dx * dx));
+}
+
+// Entry (row, col) of a column-major result as base + col (wave-uniform, in
+// elements of the dtype) + row (per lane): the stores then take a scalar base
+// address and a 32-bit lane offset
+__device__ __forceinline__ unsigned dtype_size(int dtype)
+{
+	return dtype == DT_C128 ? 16u : dtype == DT_F32 ? 4u : 8u;
+}
+
+// column base (wave-uniform) plus the lane's byte offset as a 32-bit value
+// (rows x 16 bytes < 4 GiB), one offset for every dtype
+template <class T>
+__device__ __forceinline__ T *entry_ptr(void *out, long long col, unsigned row_bytes)
+{
+	return (T *)((char *)out + col * (long long)sizeof(T) + row_bytes);
 }
 
 __device__ __forceinline__ void store_entry(void *out, long long idx, int dtype, int accumulate, float re, float im)
@@ -127,12 +153,13 @@ __device__ __forceinline__ void store_entry(void *out, long long idx, int dtype,
 // compiler would otherwise sink the B sums past the A store and keep the
 // per-point terms alive, costing registers and occupancy).  With accumulate
 // the A entry is written before the B entry is read, as two single stores would.
-__device__ __forceinline__ void store_pair(void *out0, void *out1, long long idx, int dtype, int accumulate, float ar,
-	float ai, float br, float bi)
+__device__ __forceinline__ void store_pair(void *out0, void *out1, long long col, int row, int dtype, int accumulate,
+	float ar, float ai, float br, float bi)
 {
+	const unsigned rb = (unsigned)row * dtype_size(dtype);
 	switch (dtype) {
 	case DT_C64: {
-		float2 *p = (float2 *)out0 + idx, *q = (float2 *)out1 + idx;
+		float2 *p = entry_ptr<float2>(out0, col, rb), *q = entry_ptr<float2>(out1, col, rb);
 		if (accumulate) {
 			const float2 u = *p;
 			*p = make_float2(u.x + ar, u.y + ai);
@@ -145,7 +172,7 @@ __device__ __forceinline__ void store_pair(void *out0, void *out1, long long idx
 		break;
 	}
 	case DT_C128: {
-		double2 *p = (double2 *)out0 + idx, *q = (double2 *)out1 + idx;
+		double2 *p = entry_ptr<double2>(out0, col, rb), *q = entry_ptr<double2>(out1, col, rb);
 		if (accumulate) {
 			const double2 u = *p;
 			*p = make_double2(u.x + ar, u.y + ai);
@@ -158,12 +185,12 @@ __device__ __forceinline__ void store_pair(void *out0, void *out1, long long idx
 		break;
 	}
 	case DT_F32: {
-		float *p = (float *)out0 + idx, *q = (float *)out1 + idx;
+		float *p = entry_ptr<float>(out0, col, rb), *q = entry_ptr<float>(out1, col, rb);
 		if (accumulate) { *p += ar; *q += br; } else { *p = ar; *q = br; }
 		break;
 	}
 	default: {
-		double *p = (double *)out0 + idx, *q = (double *)out1 + idx;
+		double *p = entry_ptr<double>(out0, col, rb), *q = entry_ptr<double>(out1, col, rb);
 		if (accumulate) { *p += ar; *q += br; } else { *p = ar; *q = br; }
 		break;
 	}
@@ -225,7 +252,7 @@ __device__ __forceinline__ int table_degree(double d2, const double *thr, int sn
 }
 
 struct PairCtx {
-	float dcx2, dcy2, dcz2;    // 2 (c_j - c_i) (FP32)
+	float dcx, dcy, dcz;       // c_j - c_i (FP32)
 	float d2f, df, t0;         // |c_j - c_i|^2, |c_j - c_i|, fract(k |c_j - c_i| / 2pi) in [-1/2, 1/2]
 	float nxx, nxy, nxz;       // test unit normal
 	float b0;                  // (c_j - c_i) . n_x
@@ -303,11 +330,12 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 {
 	const int nq = NQ > 0 ? NQ : nq_rt;
 	const auto qf = uni((const float *)qp_in);
+	const float dcx2 = 2.0f * p.dcx, dcy2 = 2.0f * p.dcy, dcz2 = 2.0f * p.dcz;
 	constexpr int UNROLL = NQ > 0 ? NQ : 1;
 #pragma unroll UNROLL
 	for (int q = 0; q < nq; ++q) {
 		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3], oo = qf[8 * q + 4];
-		const float u = __builtin_fmaf(p.dcx2, ox, __builtin_fmaf(p.dcy2, oy, __builtin_fmaf(p.dcz2, oz, oo)));
+		const float u = __builtin_fmaf(dcx2, ox, __builtin_fmaf(dcy2, oy, __builtin_fmaf(dcz2, oz, oo)));
 		const float r2 = p.d2f + u;
 		const float rinv = __builtin_amdgcn_rsqf(r2);
 		float c = 1.0f, sn = 0.0f;
@@ -370,6 +398,50 @@ __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, c
 	}
 }
 
+// The kernel groups of one pair may need different degrees (SLP, DLP / DLPt,
+// HSP each from its own distance table).  The distinct degrees present in the
+// wave are taken one at a time (wave-uniform, so the table address stays
+// uniform): the rule's points are evaluated once with every kernel of KM into
+// temporary sums, and each group whose degree for this lane is the current
+// one takes its sums by a select.  Every group of a lane is served exactly
+// once, so its sums are the rule's sums from -0 in point order, as a
+// separate evaluation per group would give them.  Degree < 0: group idle.
+// s[4] receives the hypersingular group's sum of Q (the double layer's is s[1]).
+template <int FAM, int KM>
+__device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int dD, int dH, const PairCtx &p,
+	float w1, float rinv_d, float2 (&s)[6])
+{
+	constexpr int NONE = 1 << 20;
+	int pS = (KM & SLP) && dS >= 0 ? dS : NONE;
+	int pD = (KM & (DLP | DLPT)) && dD >= 0 ? dD : NONE;
+	int pH = (KM & HSP) && dH >= 0 ? dH : NONE;
+	for (;;) {
+		const int mine = min(pS, min(pD, pH));
+		const unsigned long long m = __ballot(mine < NONE);
+		if (m == 0) break;
+		const int lane = __ffsll((long long)m) - 1;
+		const int dsel = __builtin_amdgcn_readlane(mine, lane);
+		const bool fS = pS == dsel, fD = pD == dsel, fH = pH == dsel;
+		float2 t[6] = {zero2(), zero2(), zero2(), zero2(), zero2(), zero2()};
+		if (fS || fD || fH) {
+			if (dsel <= 1) {
+				accumulate_centroid<FAM, KM>(p, w1, rinv_d, a.k, t);
+			} else {
+				const int nq = a.qn[dsel];
+				accumulate_points<FAM, KM, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, t);
+			}
+			// the Q sum serves both the double layer (s[1]) and the hypersingular (s[4]) groups
+			const float2 tq = (KM & DLP) ? t[1] : t[4];
+			if (fS) s[0] = t[0];
+			if (fD) { s[1] = tq; s[2] = t[2]; }
+			if (fH) { s[3] = t[3]; s[4] = tq; s[5] = t[5]; }
+		}
+		pS = fS ? NONE : pS;
+		pD = fD ? NONE : pD;
+		pH = fH ? NONE : pH;
+	}
+}
+
 // table_degree from the thresholds of a trial record, as selects
 template <int ORDER>
 __device__ __forceinline__ int table_degree_rec(double d2, const double (&t)[NTHR])
@@ -399,6 +471,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 	PairCtx p;
 	p.nxx = p.nxy = p.nxz = 0.0f;
 	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
+	row_ready(cix, ciy, ciz, p.nxx, p.nxy, p.nxz, orow);
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
 	const auto rec = uni(a.rec);
@@ -411,13 +484,15 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 	Head nxt = head(j0);
 	for (int j = j0; j < j1; ++j) {
 		const Head h = nxt;
-		const double cjx = h.cx, cjy = h.cy, cjz = h.cz;
-		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
+		// c_j - c_i in FP64 once: its squares equal those of c_i - c_j, so d2
+		// is d2_pair(c_i, c_j) bit for bit (the classification stays exact)
+		const double dxd = h.cx - cix, dyd = h.cy - ciy, dzd = h.cz - ciz;
+		const double d2 = __fma_rn(dzd, dzd, __fma_rn(dyd, dyd, dxd * dxd));
 		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == (MAP && a.cmap ? a.cmap[j] : j));
 		nxt = head(j + 1 < j1 ? j + 1 : j);
 		if (__ballot(act) == 0) continue;
-		const float dcx = (float)(cjx - cix), dcy = (float)(cjy - ciy), dcz = (float)(cjz - ciz);
-		p.dcx2 = 2.0f * dcx; p.dcy2 = 2.0f * dcy; p.dcz2 = 2.0f * dcz;
+		const float dcx = (float)dxd, dcy = (float)dyd, dcz = (float)dzd;
+		p.dcx = dcx; p.dcy = dcy; p.dcz = dcz;
 		p.d2f = (float)d2;
 		// d and 1/d in FP64 from the hardware reciprocal square root and one Newton step
 		double y = __builtin_amdgcn_rsq(d2);
@@ -465,25 +540,12 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				double t[NTHR];
 #pragma unroll
 				for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
-				const int dS = (KM & SLP) ? fc + table_degree_rec<1>(d2, t) : -1;
-				const int dD = (KM & (DLP | DLPT)) ? fc + table_degree_rec<2>(d2, t) : -1;
-				const int dH = (KM & HSP) ? fc + table_degree_rec<3>(d2, t) : -1;
-				constexpr int NG = ((KM & SLP) ? 1 : 0) + ((KM & (DLP | DLPT)) ? 1 : 0) + ((KM & HSP) ? 1 : 0);
-				bool joint = false;
-				if (NG > 1) {
-					const int d0 = (KM & SLP) ? dS : (KM & (DLP | DLPT)) ? dD : dH;
-					joint = act && (!(KM & SLP) || dS == d0) && (!(KM & (DLP | DLPT)) || dD == d0) &&
-						(!(KM & HSP) || dH == d0);
-					eval_degrees<FAM, KM>(a, j, joint ? d0 : -1, p, w1, rinv_d, s);
-				}
-				const bool sep = act && !joint;
-				if (__ballot(sep) != 0) {
-					if (KM & SLP) eval_degrees<FAM, KM & SLP>(a, j, sep ? dS : -1, p, w1, rinv_d, s);
-					if (KM & (DLP | DLPT)) eval_degrees<FAM, KM & (DLP | DLPT)>(a, j, sep ? dD : -1, p, w1, rinv_d, s);
-					if (KM & HSP) eval_degrees<FAM, KM & HSP>(a, j, sep ? dH : -1, p, w1, rinv_d, s);
-				}
-				// the hypersingular sum of Q went to s[4] unless all kernels shared a pass
-				hsp_apart = !joint;
+				const int dS = (KM & SLP) && act ? fc + table_degree_rec<1>(d2, t) : -1;
+				const int dD = (KM & (DLP | DLPT)) && act ? fc + table_degree_rec<2>(d2, t) : -1;
+				const int dH = (KM & HSP) && act ? fc + table_degree_rec<3>(d2, t) : -1;
+				eval_groups<FAM, KM>(a, j, dS, dD, dH, p, w1, rinv_d, s);
+				// the hypersingular sum of Q is in s[4]
+				hsp_apart = true;
 			}
 		}
 		if (act) {
@@ -496,19 +558,19 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			const float kk = FAM == HELM ? a.k * a.k : 0.0f;
 			const float hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x), hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
 			K[3] = make_float2(nn * sq.x - av * hr, nn * sq.y - av * hi);
-			const long long idx = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld + orow;
+			const long long col = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
 				const float re = a.alpha.x * K[kk].x - a.alpha.y * K[kk].y;
 				const float im = a.alpha.x * K[kk].y + a.alpha.y * K[kk].x;
-				store_entry(a.out0, idx, a.dtype, a.accumulate, re, im);
+				store_entry(a.out0, col + orow, a.dtype, a.accumulate, re, im);
 			} else {
 				// A = alpha D + H, B = G + alpha Ht
 				const float ar = a.alpha.x * K[3].x - a.alpha.y * K[3].y + K[1].x;
 				const float ai = a.alpha.x * K[3].y + a.alpha.y * K[3].x + K[1].y;
 				const float br = K[0].x + a.alpha.x * K[2].x - a.alpha.y * K[2].y;
 				const float bi = K[0].y + a.alpha.x * K[2].y + a.alpha.y * K[2].x;
-				store_pair(a.out0, a.out1, idx, a.dtype, a.accumulate, ar, ai, br, bi);
+				store_pair(a.out0, a.out1, col, orow, a.dtype, a.accumulate, ar, ai, br, bi);
 			}
 		}
 	}

@@ -123,10 +123,11 @@ __device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const Qua
 	float2 (&K)[4])
 {
 	const auto qf = uni((const float *)tab);
+	const float dcx2 = 2.0f * p.dcx, dcy2 = 2.0f * p.dcy, dcz2 = 2.0f * p.dcz;
 	for (int q = 0; q < nq; ++q) {
 		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3];
 		const float nqx = qf[8 * q + 4], nqy = qf[8 * q + 5], nqz = qf[8 * q + 6], oo = qf[8 * q + 7];
-		const float u = __builtin_fmaf(2.0f * p.dcx, ox, __builtin_fmaf(2.0f * p.dcy, oy, __builtin_fmaf(2.0f * p.dcz, oz, oo)));
+		const float u = __builtin_fmaf(dcx2, ox, __builtin_fmaf(dcy2, oy, __builtin_fmaf(dcz2, oz, oo)));
 		const float r2 = p.d2f + u;
 		const float rinv = __builtin_amdgcn_rsqf(r2);
 		const float g = w * rinv;
@@ -174,6 +175,41 @@ __device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, con
 	}
 }
 
+// Kernel groups at different Gauss orders (per lane: SLP nS, DLP / DLPt nD,
+// HSP nH; <= 0 idle): each distinct order present in the wave is evaluated
+// once with every kernel of KM into temporary values, and each group whose
+// order for this lane is the current one takes its kernel values by a select
+// (every group is served once, so its value is the rule's sum from -0 in
+// point order, as a separate evaluation per group gives it).
+template <int FAM, int KM>
+__device__ __forceinline__ void quad_groups(const QuadArgs &a, int j, int nS, int nD, int nH, const QuadCtx &p,
+	float2 (&K)[4])
+{
+	constexpr int NONE = 1 << 20;
+	int pS = (KM & SLP) && nS > 0 ? nS : NONE;
+	int pD = (KM & (DLP | DLPT)) && nD > 0 ? nD : NONE;
+	int pH = (KM & HSP) && nH > 0 ? nH : NONE;
+	for (;;) {
+		const int mine = min(pS, min(pD, pH));
+		const unsigned long long m = __ballot(mine < NONE);
+		if (m == 0) break;
+		const int lane = __ffsll((long long)m) - 1;
+		const int nsel = __builtin_amdgcn_readlane(mine, lane);
+		const bool fS = pS == nsel, fD = pD == nsel, fH = pH == nsel;
+		float2 t[4] = {zero2(), zero2(), zero2(), zero2()};
+		if (fS || fD || fH) {
+			if (nsel == 1) quad_centre<FAM, KM>(p, a.k, t);
+			else quad_points<FAM, KM>(a.qt[nsel] + (size_t)j * (nsel * nsel), nsel * nsel, p, a.k_revf, a.k, t);
+			if (fS) K[0] = t[0];
+			if (fD) { K[1] = t[1]; K[2] = t[2]; }
+			if (fH) K[3] = t[3];
+		}
+		pS = fS ? NONE : pS;
+		pD = fD ? NONE : pD;
+		pH = fH ? NONE : pH;
+	}
+}
+
 // Gauss order per axis of a Gauss square rule of a degree
 __device__ __forceinline__ int gauss_order(int degree) { return degree / 2 + 1; }
 
@@ -192,6 +228,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 	QuadCtx p;
 	p.nxx = p.nxy = p.nxz = 0.0f;
 	if (KM & (DLPT | HSP)) { p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii]; }
+	row_ready(cix, ciy, ciz, p.nxx, p.nxy, p.nxz, orow);
 	const int j0 = tile_c * a.tile_cols;
 	const int j1 = min(j0 + a.tile_cols, a.n_cols);
 	const auto rec = uni(a.rec);
@@ -204,12 +241,13 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 	Head nxt = head(j0);
 	for (int j = j0; j < j1; ++j) {
 		const Head h = nxt;
-		const double cjx = h.cx, cjy = h.cy, cjz = h.cz;
-		const double d2 = d2_pair(cix, ciy, ciz, cjx, cjy, cjz);
+		// c_j - c_i in FP64 once (d2 equals d2_pair(c_i, c_j) bit for bit)
+		const double dxd = h.cx - cix, dyd = h.cy - ciy, dzd = h.cz - ciz;
+		const double d2 = __fma_rn(dzd, dzd, __fma_rn(dyd, dyd, dxd * dxd));
 		const bool act = row_ok && !(d2 < h.near) && !(a.same_mesh && i == (MAP && a.cmap ? a.cmap[j] : j));
 		nxt = head(j + 1 < j1 ? j + 1 : j);
 		if (__ballot(act) == 0) continue;
-		p.dcx = (float)(cjx - cix); p.dcy = (float)(cjy - ciy); p.dcz = (float)(cjz - ciz);
+		p.dcx = (float)dxd; p.dcy = (float)dyd; p.dcz = (float)dzd;
 		p.d2f = (float)d2;
 		p.nyx = h.nx; p.nyy = h.ny; p.nyz = h.nz;
 		p.w1 = h.w1;
@@ -245,31 +283,22 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 				const int nS = (KM & SLP) && act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
 				const int nD = (KM & (DLP | DLPT)) && act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
 				const int nH = (KM & HSP) && act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
-				const int n0 = (KM & SLP) ? nS : (KM & (DLP | DLPT)) ? nD : nH;
-				const bool joint = act && (!(KM & SLP) || nS == n0) && (!(KM & (DLP | DLPT)) || nD == n0) &&
-					(!(KM & HSP) || nH == n0);
-				quad_orders<FAM, KM>(a, j, joint ? n0 : 0, p, K);
-				const bool sep = act && !joint;
-				if (__ballot(sep) != 0) {
-					if (KM & SLP) quad_orders<FAM, KM & SLP>(a, j, sep ? nS : 0, p, K);
-					if (KM & (DLP | DLPT)) quad_orders<FAM, KM & (DLP | DLPT)>(a, j, sep ? nD : 0, p, K);
-					if (KM & HSP) quad_orders<FAM, KM & HSP>(a, j, sep ? nH : 0, p, K);
-				}
+				quad_groups<FAM, KM>(a, j, nS, nD, nH, p, K);
 			}
 		}
 		if (act) {
-			const long long idx = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld + orow;
+			const long long col = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
 				const float re = a.alpha.x * K[kk].x - a.alpha.y * K[kk].y;
 				const float im = a.alpha.x * K[kk].y + a.alpha.y * K[kk].x;
-				store_entry(a.out0, idx, a.dtype, a.accumulate, re, im);
+				store_entry(a.out0, col + orow, a.dtype, a.accumulate, re, im);
 			} else {
 				const float ar = a.alpha.x * K[3].x - a.alpha.y * K[3].y + K[1].x;
 				const float ai = a.alpha.x * K[3].y + a.alpha.y * K[3].x + K[1].y;
 				const float br = K[0].x + a.alpha.x * K[2].x - a.alpha.y * K[2].y;
 				const float bi = K[0].y + a.alpha.x * K[2].y + a.alpha.y * K[2].x;
-				store_pair(a.out0, a.out1, idx, a.dtype, a.accumulate, ar, ai, br, bi);
+				store_pair(a.out0, a.out1, col, orow, a.dtype, a.accumulate, ar, ai, br, bi);
 			}
 		}
 	}

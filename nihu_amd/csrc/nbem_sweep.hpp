@@ -52,6 +52,7 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 		const int ii = row_ok ? i : a.nr - 1;
 		const double cix = a.c[ii], ciy = a.c[a.n + ii], ciz = a.c[2 * a.n + ii];
 		const float nxx = a.nu[ii], nxy = a.nu[a.n + ii], nxz = a.nu[2 * a.n + ii];
+		row_ready(cix, ciy, ciz, nxx, nxy, nxz, ii);
 		const int j0 = ct * a.tile_cols;
 		const int j1 = min(j0 + a.tile_cols, a.nc);
 		for (int j = j0; j < j1; ++j) {

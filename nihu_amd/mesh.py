@@ -146,13 +146,14 @@ def jittered_cubed_sphere(n: int, seed: int = 0):
     return cubed_sphere(n, 1.0, 0.01, seed)
 
 
-def mixed_cubed_sphere(n: int, seed: int = 0, every: int = 3):
-    """A closed surface with both element types: the jittered cubed sphere of
-    n x n quadrilaterals per face with every `every`-th quadrilateral split
-    along its diagonal (0, 2) into two triangles of the same orientation.
-    Returns (nodes, table) with NiHu's element table rows [type id, n0, n1,
-    n2, n3] (n3 = 0 for triangles), the two types interleaved."""
-    nodes, quads = jittered_cubed_sphere(n, seed)
+def mixed_cubed_sphere(n: int, seed: int = 0, every: int = 3, jitter: float = 0.01):
+    """A closed surface with both element types: the cubed sphere of n x n
+    quadrilaterals per face (radial jitter +-jitter from seed; 0 gives NiHu's
+    blown-up cube sphere) with every `every`-th quadrilateral split along its
+    diagonal (0, 2) into two triangles of the same orientation.  Returns
+    (nodes, table) with NiHu's element table rows [type id, n0, n1, n2, n3]
+    (n3 = 0 for triangles), the two types interleaved."""
+    nodes, quads = cubed_sphere(n, 1.0, jitter, seed)
     rows = []
     for e, q in enumerate(quads):
         if e % every == 0:
