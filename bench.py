@@ -96,6 +96,22 @@ def surface_field_error(nodes, tris, k, A, B, Ar, Br):
             "mie_rel_oracle": float(np.linalg.norm(pinc + pr - mie) / np.linalg.norm(mie))}
 
 
+ORACLE_BUILDS = {
+    "tuned": "g++ -O3 -march=x86-64-v4 -ffp-contract=fast -fopenmp (AVX-512, FMA contraction)",
+    "exact": "g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (the checker build)",
+}
+
+
+def oracle_tuned_build(oracle):
+    """Time the CPU baseline with the tuned oracle build where the host can
+    run it (x86-64-v4); returns the build's name."""
+    if oracle.tuned_supported():
+        oracle.use_build("tuned")
+        return "tuned"
+    oracle.use_build("exact")
+    return "exact"
+
+
 def host_cpu():
     """CPU model (from /proc/cpuinfo) and the number of cores this process may
     run on, stated beside every CPU number."""
@@ -259,7 +275,8 @@ def main():
         opol = oracle.policy(rule=pol.rule, fixed_degree=pol.fixed_degree, near_mode=pol.near_mode,
                              near_limit=pol.near_limit)
         threads = oracle.num_threads()
-        log("cpu baseline: oracle on %d rows, %d threads" % (nr, threads))
+        build = oracle_tuned_build(oracle)
+        log("cpu baseline: oracle (%s build) on %d rows, %d threads" % (build, nr, threads))
         tc = time.perf_counter()
         Ar, Br = oracle.assemble_bm(nodes, tris, k, alpha=alpha, pol=opol, rows=rows, threads=threads)
         dt = time.perf_counter() - tc
@@ -267,7 +284,8 @@ def main():
         cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
                "sample": ("the whole workload (A and B, all %d rows; 4 operators)" % n if full else
                           "%d of %d rows of A and B (4 operators, all %d columns)" % (nr, n, n)) +
-                         ", FP64 oracle restating NiHu's pair algorithm, OpenMP over rows, %.1f s wall" % dt}
+                         ", FP64 oracle restating NiHu's pair algorithm, OpenMP over rows, %.1f s wall" % dt,
+               "build": ORACLE_BUILDS[build]}
         log("cpu baseline: %.4g entries/s on %d threads (%.1fx faster on GPU)" % (cpu_value, threads,
                                                                                  value / cpu_value))
         # NiHu's own single-matrix path is the serial pair loop: the same
@@ -279,6 +297,7 @@ def main():
         cpu["serial"] = {"value": 4 * len(srows) * n / dts, "cores": 1,
                          "sample": "%d of %d rows of A and B, one thread, %.1f s wall" % (len(srows), n, dts)}
         cpu["host"] = host_cpu()
+        oracle.use_build("exact")
         log("cpu serial: %.4g entries/s; host %s" % (cpu["serial"]["value"], cpu["host"]))
         if full:
             Ag, Bg = A.cpu().numpy(), B.cpu().numpy()
@@ -709,9 +728,11 @@ def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
     fsel = [0, 9, 18, 27, 36, 45, 54, 63]
     opol = oracle.policy(rule=1, fixed_degree=5, near_mode=1, near_limit=2.0)
     threads = oracle.num_threads()
+    build = oracle_tuned_build(oracle)
     tc = time.perf_counter()
     Ar, Br = oracle.sweep_bm(nodes, tris, ks[fsel], pol=opol, rows=rows, threads=threads)
     dt = time.perf_counter() - tc
+    oracle.use_build("exact")
     cpu_value = 4 * len(rows) * n * len(fsel) / dt
     ri = torch.from_numpy(rows.astype(np.int64)).cuda()
     errs = []
@@ -738,7 +759,8 @@ def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
         "accuracy": {"max_rel_fro_over_k": float(max(errs)), "rows": len(rows), "wavenumbers": len(fsel)},
         "cpu_baseline": {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
                          "sample": "%d of %d rows, %d of the 64 wavenumbers, FP64 oracle (NiHu's algorithm), OpenMP "
-                                   "over frequencies, %.1f s wall" % (len(rows), n, len(fsel), dt)},
+                                   "over frequencies, %.1f s wall" % (len(rows), n, len(fsel), dt),
+                         "build": ORACLE_BUILDS[build]},
     }
 
 

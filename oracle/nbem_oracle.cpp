@@ -1739,7 +1739,8 @@ int orc_quad_elem(int what, int family, int nx, int ny, double k, const double *
 		break;
 	}
 	case 6: {
-		double xi, eta, zeta;
+		// no image: NaN coordinates beside the flag
+		double xi = std::nan(""), eta = std::nan(""), zeta = std::nan("");
 		bool ok = quad_inverse_map(e, x0, o1 > 0 ? std::pow(10.0, -o1) : 1e-5, 100, xi, eta, zeta);
 		out[0] = xi; out[1] = eta; out[2] = zeta; out[3] = ok ? 1.0 : 0.0;
 		return 0;

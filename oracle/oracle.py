@@ -15,7 +15,41 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+_LIB_V4 = os.path.join(_HERE, "liboracle_v4.so")
 _lib = None
+_loaded = {}
+# x86-64-v4: the AVX-512 subsets the tuned build may use
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+
+
+def tuned_supported():
+    """True if the tuned build exists and this host's CPU reports the
+    x86-64-v4 features it was compiled for (read from /proc/cpuinfo)."""
+    if not os.path.exists(_LIB_V4):
+        return False
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return _V4_FLAGS <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+def use_build(name="exact"):
+    """Switch the functions of this module between the checker build
+    ("exact": liboracle.so, no FMA contraction) and the tuned build
+    ("tuned": liboracle_v4.so, AVX-512 and FMA contraction; the timed CPU
+    baseline).  Returns the previous name."""
+    global _lib
+    prev = "tuned" if _lib is not None and _lib is _loaded.get(_LIB_V4) else "exact"
+    path = _LIB_V4 if name == "tuned" else _LIB
+    if name == "tuned" and not tuned_supported():
+        raise RuntimeError("the tuned oracle build is missing or this CPU lacks x86-64-v4")
+    if path not in _loaded:
+        _loaded[path] = _declare(C.CDLL(path))
+    _lib = _loaded[path]
+    return prev
 
 
 class Policy(C.Structure):
@@ -38,7 +72,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        l = C.CDLL(_LIB)
+        _lib = _loaded.setdefault(_LIB, _declare(C.CDLL(_LIB)))
+    return _lib
+
+
+def _declare(l):
+    if True:
         D = C.POINTER(C.c_double)
         I = C.POINTER(C.c_int)
         P = C.POINTER(Policy)
@@ -73,8 +112,7 @@ def lib():
         l.orc_quad_duffy_rule.argtypes = [C.c_int, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_telles_rule.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_geometry_quad.argtypes = [D, C.c_int, I, C.c_int, D, D, D, D]
-        _lib = l
-    return _lib
+    return l
 
 
 def _d(a):
