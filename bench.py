@@ -669,13 +669,15 @@ def run_mixed(args):
     print(json.dumps(out))
 
 
-def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
-    """Config 4: Burton-Miller A and B for 64 wavenumbers ka uniformly spaced
-    in [0.5, 32] on the 5120-triangle jittered icosphere (level 4, seed 0),
-    7-point rule far, FP64 within 2 element diameters, in one persistent
-    launch with the autotuned launch shape; compared with the same matrices
-    from 64 single-wavenumber assemblies, and 16 seeded rows of 8 wavenumbers
-    (the first and last among them) against the FP64 oracle's sweep."""
+def sweep_summary(steps=5, warmup=2, log_prefix="sweep", policy="config"):
+    """Burton-Miller A and B for 64 wavenumbers ka uniformly spaced in
+    [0.5, 32] on the 5120-triangle jittered icosphere (level 4, seed 0) in one
+    persistent launch with the autotuned launch shape -- policy "config":
+    7-point rule far, FP64 within 2 element diameters; "nihu": NiHu's
+    distance-driven degrees, FP64 within 1.5 mean edges -- compared with the
+    same matrices from 64 single-wavenumber assemblies, and 16 seeded rows of 8
+    wavenumbers (the first and last among them) against the FP64 oracle's
+    sweep."""
     import numpy as np
     import torch
     from nihu_amd import nbem
@@ -685,7 +687,7 @@ def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
     nodes, tris = jittered_icosphere(level, seed=0)
     n = len(tris)
     ks = np.linspace(0.5, 32.0, 64)
-    pol = nbem.policy(**nbem.CONFIG_POLICY)
+    pol = nbem.policy(**nbem.CONFIG_POLICY) if policy == "config" else nbem.policy("nihu")
     mesh = nbem.Mesh(nodes, tris)
     A = torch.zeros((len(ks), n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
     B = torch.zeros_like(A)
@@ -726,7 +728,7 @@ def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
     from oracle import oracle
     rows = np.sort(np.random.default_rng(5).choice(n, 16, replace=False)).astype(np.int32)
     fsel = [0, 9, 18, 27, 36, 45, 54, 63]
-    opol = oracle.policy(rule=1, fixed_degree=5, near_mode=1, near_limit=2.0)
+    opol = oracle.policy(rule=1, fixed_degree=5, near_mode=1, near_limit=2.0) if policy == "config" else oracle.policy()
     threads = oracle.num_threads()
     build = oracle_tuned_build(oracle)
     tc = time.perf_counter()
@@ -745,8 +747,10 @@ def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
     torch.cuda.empty_cache()
     reg_ms = prof["sweep"][0] / max(prof["sweep"][1], 1)
     return {
-        "workload": "Burton-Miller sweep, 64 wavenumbers ka in [0.5, 32], 5120-triangle jittered icosphere, "
-                    "7-point rule far, FP64 within 2 element diameters, one persistent launch",
+        "workload": "Burton-Miller sweep, 64 wavenumbers ka in [0.5, 32], 5120-triangle jittered icosphere, " +
+                    ("7-point rule far, FP64 within 2 element diameters" if policy == "config" else
+                     "NiHu's distance-driven degrees, FP64 within 1.5 mean edges") + ", one persistent launch",
+        "policy": policy,
         "n_tri": n, "n_k": len(ks), "steps": steps,
         "ms_per_sweep": wall / steps * 1e3,
         "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
@@ -767,14 +771,15 @@ def sweep_summary(steps=5, warmup=2, log_prefix="sweep"):
 def run_sweep(args):
     import torch
     torch.cuda.set_device(0)
-    sw = sweep_summary(steps=args.steps, warmup=max(1, min(args.warmup, 5)))
+    sw = sweep_summary(steps=args.steps, warmup=max(1, min(args.warmup, 5)), policy=args.policy)
     out = {
         "metric": "matrix entries/s, Helmholtz 3D Burton-Miller sweep, 64 wavenumbers, 5120 tris",
         "value": sw["entries_per_s"], "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": sw["ms_per_sweep"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32 (complex64) far pairs, fp64 near/diagonal",
         "data": "synthetic: seeded jittered icosphere",
-        "config": {"workload": sw["workload"], "n_tri": sw["n_tri"], "n_k": sw["n_k"], "launch": sw["launch"]},
+        "config": {"workload": sw["workload"], "n_tri": sw["n_tri"], "n_k": sw["n_k"], "policy": sw["policy"],
+                   "launch": sw["launch"]},
         "sweep": sw,
     }
     print(json.dumps(out))

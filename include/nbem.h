@@ -174,9 +174,11 @@ int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_poli
  * launch (alpha = -i/k each).  A and B are device c64 arrays holding n_k
  * matrices of n_tri x n_tri with leading dimension ld, matrix f at
  * A + f * k_stride elements.  launch may be NULL (autotuned).  Triangle
- * meshes with a fixed 1- or 7-point rule take the one-launch path; other
- * policies and meshes with quadrilaterals take one nb_assemble_bm pass per
- * wavenumber. */
+ * meshes take the one-launch path under NiHu's distance-driven degrees (each
+ * pair's distance, class and degrees found once for a batch of wavenumbers;
+ * every matrix equal to nb_assemble_bm's at its wavenumber, the regular
+ * entries bit for bit) or a fixed 1- or 7-point rule; other fixed rules and
+ * meshes with quadrilaterals take one nb_assemble_bm pass per wavenumber. */
 int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_policy *policy,
                       void *A, void *B, int64_t ld, int64_t k_stride, const nb_launch *launch, void *stream);
 
@@ -184,12 +186,12 @@ int nb_assemble_sweep(const nb_mesh *mesh, const double *k, int32_t n_k, const n
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
  * results as nb_assemble_sweep, restricted to its one-launch path (triangle
- * meshes, fixed 1- or 7-point rule).  The per-call buffers (wavenumber
+ * meshes; NiHu's degrees or a fixed 1- or 7-point rule).  The per-call buffers (wavenumber
  * constants and the FP64 static parts of the near-singular and diagonal
  * pairs, 48 bytes per wavenumber plus 32 bytes per FP64 pair, each piece
  * rounded up to 256 bytes) live in one 32 MiB region per device, held until
  * the process ends; larger requests fail with NB_ERR_RANGE before anything
- * runs.  The mesh's tables for the policy (thresholds, quadrature table,
+ * runs.  The mesh's tables for the policy (thresholds, quadrature tables,
  * near-pair list) and the device's rule tables must be resident already: one
  * nb_assemble_sweep call on the mesh with the policy builds them (else
  * NB_ERR_UNSUPPORTED).  launch NULL takes the shape tuned for this mesh size
@@ -207,8 +209,10 @@ int nb_assemble_sweep_noalloc(const nb_mesh *mesh, const double *k, int32_t n_k,
 int nb_classify(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, int32_t nx, int32_t ny,
                 const nb_policy *policy, int8_t *cls, void *stream);
 
-/* Choose the launch shape of nb_assemble_bm / nb_assemble_sweep for this mesh
- * by timing candidates on the device; the result is cached per mesh size. */
+/* Choose the launch shape of nb_assemble_sweep for this mesh by timing
+ * candidates on the device; the result is cached per (mesh size, number of
+ * wavenumbers up to 64, quadrature rule).  Fixed rules without a one-launch
+ * path return the default shape. */
 int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_launch *best);
 int nb_set_launch(const nb_launch *launch);
 

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <tuple>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1196,7 +1197,8 @@ int current_device_matches(const nb_mesh *m)
 }
 
 std::mutex g_tune_mu;
-std::map<std::pair<int, int>, nb_launch> g_tuned;
+// tuned launch shapes by (elements, wavenumbers up to 64, quadrature rule)
+std::map<std::tuple<int, int, int>, nb_launch> g_tuned;
 
 // Burton-Miller sweep.  tune_block > 0: only the regular pass on the leading
 // tune_block x tune_block block (autotuning); otherwise the full matrices.
@@ -1258,16 +1260,25 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 
 	const bool tri = mesh->npe == 3;
 	const bool noalloc = scratch != nullptr;
-	if (noalloc && !(tri && policy->rule == NB_RULE_FIXED && m->d_qt[policy->fixed_degree] &&
-		(m->qn[policy->fixed_degree] == 7 || m->qn[policy->fixed_degree] == 1)))
-		return fail(NB_ERR_UNSUPPORTED, "the allocation-free sweep takes triangle meshes with a fixed 1- or 7-point "
-			"rule whose table is resident (run nb_assemble_sweep once on this mesh and policy first)");
-	const int Q = tri && policy->rule == NB_RULE_FIXED ?
-		(ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
+	// NiHu's distance-driven degrees: every degree its tables may select
+	const bool nihu = policy->rule == NB_RULE_NIHU;
+	std::vector<int> degs;
+	policy_degrees(*policy, nbk::ALL4, 0, degs);
+	if (noalloc) {
+		bool ready = tri;
+		for (int d : degs) ready = ready && m->d_qt[d];
+		if (ready && !nihu) ready = m->qn[policy->fixed_degree] == 7 || m->qn[policy->fixed_degree] == 1;
+		if (!ready)
+			return fail(NB_ERR_UNSUPPORTED, "the allocation-free sweep takes triangle meshes with NiHu's degrees or a "
+				"fixed 1- or 7-point rule, whose tables are resident (run nb_assemble_sweep once on this mesh and "
+				"policy first)");
+	}
+	const int Q = tri && !nihu ? (ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
 	if (Q < 0) return NB_ERR_RANGE;
-	if (!tri || policy->rule != NB_RULE_FIXED || (Q != 7 && Q != 1)) {
+	if (!tri || (!nihu && Q != 7 && Q != 1)) {
 		// quadrilateral meshes, meshes with both element types and the other
-		// policies: one Burton-Miller assembly per wavenumber
+		// fixed rules: one Burton-Miller assembly per wavenumber
+		if (tune_block > 0) return fail(NB_ERR_UNSUPPORTED, "this rule has no one-launch sweep to tune");
 		if (!tri)
 			for (const nb_mesh *p : {m, m->part[1].get()})
 				if (p && p->npe == 4 && (s = check_quad_policy(*policy))) return s;
@@ -1289,7 +1300,7 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 		bool found = false;
 		{
 			std::lock_guard<std::mutex> lt(g_tune_mu);
-			auto it = g_tuned.find({n, std::min<int>(n_k, 64)});
+			auto it = g_tuned.find(std::make_tuple(n, std::min<int>(n_k, 64), (int)policy->rule));
 			if (it != g_tuned.end()) { lc = it->second; found = true; }
 		}
 		if (!found && tune_block == 0 && !noalloc) {
@@ -1324,7 +1335,8 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 				std::to_string(n_k) + " wavenumbers, " + std::to_string(np) + " FP64 pairs); the limit is " +
 				std::to_string(scratch_cap) + " bytes");
 	} else {
-		if ((s = ensure_quad(m, policy->fixed_degree))) return s;
+		for (int d : degs)
+			if ((s = ensure_quad(m, d))) return s;
 		if ((s = ensure_thresholds(m, policy->near_mode, policy->near_limit))) return s;
 		if ((s = ensure_near_list(m, m, 1, *policy, st, &nl))) return s;
 	}
@@ -1433,14 +1445,43 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	const long long items = (long long)sw.row_tiles * sw.col_tiles * kbt;
 	if (items > 0x7fffffffLL) return fail(NB_ERR_RANGE, "too many work items");
 	sw.n_items = (int)items;
+	const long long grid = std::min<long long>(items, (long long)lc.persistent * n_cu);
+	using namespace nbk;
+	if (nihu) {
+		// the regular pairs under NiHu's degrees, all wavenumbers in one launch
+		SweepNihuArgs sn;
+		std::memset(&sn, 0, sizeof(sn));
+		RegArgs &a = sn.r;
+		a.n_rows = nblk; a.n_cols = nblk; a.same_mesh = 1;
+		a.tile_cols = lc.tile_cols;
+		a.row_tiles = sw.row_tiles;
+		a.tn = n; a.sn = n;
+		a.tc = (const double *)m->d_c.get();
+		a.tnu = (const float *)m->d_nuf.get();
+		a.sc = a.tc;
+		a.snu = a.tnu;
+		a.thr = (const double *)m->d_thr.get();
+		a.rec = (const TrialRec *)m->d_rec.get();
+		for (int d = 0; d < NSLOT; ++d) { a.qt[d] = (const QPoint *)m->d_qt[d].get(); a.qn[d] = m->qn[d]; }
+		a.fixed_degree = -1;
+		a.field_complexity = 0;
+		sn.n_k = n_k; sn.n_items = sw.n_items; sn.col_tiles = sw.col_tiles;
+		sn.k_rev = (const double *)d_krev; sn.kf = (const float *)d_kf; sn.krevf = (const float *)d_krf;
+		sn.alpha = (const float2 *)d_alf;
+		sn.A = (float2 *)A; sn.B = (float2 *)B; sn.ld = ld; sn.kstride = k_stride;
+		const int pe = prof_begin(3, st);
+		if (lc.k_batch == 2) hipLaunchKernelGGL((sweep_nihu_kernel<2>), dim3(grid), dim3(256), 0, st, sn);
+		else if (lc.k_batch == 4) hipLaunchKernelGGL((sweep_nihu_kernel<4>), dim3(grid), dim3(256), 0, st, sn);
+		else hipLaunchKernelGGL((sweep_nihu_kernel<8>), dim3(grid), dim3(256), 0, st, sn);
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	} else {
 	sw.c = (const double *)m->d_c.get(); sw.nu = (const float *)m->d_nuf.get(); sw.thr = (const double *)m->d_thr.get();
 	sw.qt = (const nbk::QPoint *)m->d_qt[policy->fixed_degree].get();
 	sw.k_rev = (const double *)d_krev; sw.kf = (const float *)d_kf; sw.krevf = (const float *)d_krf;
 	sw.alpha = (const float2 *)d_alf;
 	sw.A = (float2 *)A; sw.B = (float2 *)B; sw.ld = ld; sw.kstride = k_stride;
-	const long long grid = std::min<long long>(items, (long long)lc.persistent * n_cu);
 	const int pe = prof_begin(3, st);
-	using namespace nbk;
 #define NB_SW(KB, QQ) hipLaunchKernelGGL((sweep_regular_kernel<KB, QQ>), dim3(grid), dim3(256), 0, st, sw)
 	if (Q == 7) {
 		if (lc.k_batch == 2) NB_SW(2, 7); else if (lc.k_batch == 4) NB_SW(4, 7); else NB_SW(8, 7);
@@ -1450,6 +1491,7 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 #undef NB_SW
 	NB_HIP(hipGetLastError());
 	prof_end(pe, st);
+	}
 	if (forked) {
 		NB_HIP(hipEventRecord(ev_join, side));
 		NB_HIP(hipStreamWaitEvent(st, ev_join, 0));
@@ -1822,10 +1864,16 @@ int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_la
 	int s;
 	if ((s = check_policy(policy))) return s;
 	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "sweep autotuning is implemented for triangle meshes only");
-	if (policy->rule != NB_RULE_FIXED) {
-		// the NiHu-degree policy runs the per-wavenumber kernel: only the tile matters
-		*best = {32, 1, 0};
-		return NB_OK;
+	if (policy->rule == NB_RULE_FIXED && policy->fixed_degree >= 0 && policy->fixed_degree < nbk::NSLOT) {
+		nb_mesh *m = const_cast<nb_mesh *>(mesh);
+		std::lock_guard<std::mutex> lk(m->mu);
+		if ((s = ensure_quad(m, policy->fixed_degree))) return s;
+		const int q = m->qn[policy->fixed_degree];
+		if (q != 1 && q != 7) {
+			// this rule runs one single-wavenumber assembly per wavenumber
+			*best = {32, 4, 4};
+			return NB_OK;
+		}
 	}
 	const int n = mesh->n;
 	const int nb = std::min(n, 4096);          // tune on a leading block of the matrix
@@ -1861,7 +1909,7 @@ int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_la
 	NB_HIP(hipEventDestroy(e1));
 	{
 		std::lock_guard<std::mutex> lk(g_tune_mu);
-		g_tuned[{n, std::min<int>(n_k, 64)}] = bl;
+		g_tuned[std::make_tuple(n, std::min<int>(n_k, 64), (int)policy->rule)] = bl;
 	}
 	*best = bl;
 	return NB_OK;

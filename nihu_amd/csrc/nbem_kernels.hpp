@@ -373,10 +373,10 @@ __device__ __forceinline__ void accumulate_centroid(const PairCtx &p, float w1, 
 // its tables for this) and take accumulate_centroid: no table load.
 template <int FAM, int MASK>
 __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, const PairCtx &p, float w1,
-	float rinv_d, float2 (&s)[6])
+	float rinv_d, float k, float k_revf, float2 (&s)[6])
 {
 	if (__ballot(deg >= 0 && deg <= 1) != 0) {
-		if (deg >= 0 && deg <= 1) accumulate_centroid<FAM, MASK>(p, w1, rinv_d, a.k, s);
+		if (deg >= 0 && deg <= 1) accumulate_centroid<FAM, MASK>(p, w1, rinv_d, k, s);
 		deg = deg <= 1 ? -1 : deg;
 	}
 	// wave-uniform guard: waves without a multi-point lane skip the loop and
@@ -392,7 +392,7 @@ __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, c
 		const unsigned long long msel = __ballot(deg == dsel);
 		if ((msel >> __lane_id()) & 1ull) {
 			const int nq = a.qn[dsel];
-			accumulate_points<FAM, MASK, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, s);
+			accumulate_points<FAM, MASK, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, k_revf, k, s);
 			deg = -1;
 		}
 	}
@@ -409,7 +409,7 @@ __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, c
 // s[4] receives the hypersingular group's sum of Q (the double layer's is s[1]).
 template <int FAM, int KM>
 __device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int dD, int dH, const PairCtx &p,
-	float w1, float rinv_d, float2 (&s)[6])
+	float w1, float rinv_d, float k, float k_revf, float2 (&s)[6])
 {
 	constexpr int NONE = 1 << 20;
 	int pS = (KM & SLP) && dS >= 0 ? dS : NONE;
@@ -425,10 +425,10 @@ __device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int
 		float2 t[6] = {zero2(), zero2(), zero2(), zero2(), zero2(), zero2()};
 		if (fS || fD || fH) {
 			if (dsel <= 1) {
-				accumulate_centroid<FAM, KM>(p, w1, rinv_d, a.k, t);
+				accumulate_centroid<FAM, KM>(p, w1, rinv_d, k, t);
 			} else {
 				const int nq = a.qn[dsel];
-				accumulate_points<FAM, KM, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, t);
+				accumulate_points<FAM, KM, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, k_revf, k, t);
 			}
 			// the Q sum serves both the double layer (s[1]) and the hypersingular (s[4]) groups
 			const float2 tq = (KM & DLP) ? t[1] : t[4];
@@ -440,6 +440,22 @@ __device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int
 		pD = fD ? NONE : pD;
 		pH = fH ? NONE : pH;
 	}
+}
+
+// Kernel values from the accumulators of add_point: SLP s[0], DLP -a s[1],
+// DLPt s[2], HSP nn sum Q - a (3 s[3] - k^2 s[5]) with sum Q in s[4] when the
+// hypersingular group had its own sums (hsp_apart), else s[1].
+template <int FAM, int KM>
+__device__ __forceinline__ void kernel_values(const float2 (&s)[6], float av, float nn, float k, bool hsp_apart,
+	float2 (&K)[4])
+{
+	K[0] = s[0];
+	K[1] = make_float2(-av * s[1].x, -av * s[1].y);
+	K[2] = s[2];
+	const float2 sq = (hsp_apart || !(KM & DLP)) ? s[4] : s[1];
+	const float kk = FAM == HELM ? k * k : 0.0f;
+	const float hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x), hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
+	K[3] = make_float2(nn * sq.x - av * hr, nn * sq.y - av * hi);
 }
 
 // table_degree from the thresholds of a trial record, as selects
@@ -520,7 +536,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 		if (FIXQ > 0) {
 			if (act) accumulate_points<FAM, KM, FIXQ>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s);
 		} else if (a.fixed_degree >= 0) {
-			eval_degrees<FAM, KM>(a, j, act ? a.fixed_degree : -1, p, h.w1, rinv_d, s);
+			eval_degrees<FAM, KM>(a, j, act ? a.fixed_degree : -1, p, h.w1, rinv_d, a.k, a.k_revf, s);
 		} else {
 			// per-kernel degrees from the distance tables; where they coincide
 			// (all far pairs: one point each) all kernels share one pass
@@ -533,7 +549,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				if (fc <= 1) {
 					if (act) accumulate_centroid<FAM, KM>(p, w1, rinv_d, a.k, s);
 				} else {
-					eval_degrees<FAM, KM>(a, j, act ? fc : -1, p, w1, rinv_d, s);
+					eval_degrees<FAM, KM>(a, j, act ? fc : -1, p, w1, rinv_d, a.k, a.k_revf, s);
 				}
 			} else {
 				// the distance-table thresholds of the record
@@ -543,21 +559,14 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				const int dS = (KM & SLP) && act ? fc + table_degree_rec<1>(d2, t) : -1;
 				const int dD = (KM & (DLP | DLPT)) && act ? fc + table_degree_rec<2>(d2, t) : -1;
 				const int dH = (KM & HSP) && act ? fc + table_degree_rec<3>(d2, t) : -1;
-				eval_groups<FAM, KM>(a, j, dS, dD, dH, p, w1, rinv_d, s);
+				eval_groups<FAM, KM>(a, j, dS, dD, dH, p, w1, rinv_d, a.k, a.k_revf, s);
 				// the hypersingular sum of Q is in s[4]
 				hsp_apart = true;
 			}
 		}
 		if (act) {
-			// kernel values from the accumulators
 			float2 K[4];
-			K[0] = s[0];
-			K[1] = make_float2(-av * s[1].x, -av * s[1].y);
-			K[2] = s[2];
-			const float2 sq = (hsp_apart || !(KM & DLP)) ? s[4] : s[1];
-			const float kk = FAM == HELM ? a.k * a.k : 0.0f;
-			const float hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x), hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
-			K[3] = make_float2(nn * sq.x - av * hr, nn * sq.y - av * hi);
+			kernel_values<FAM, KM>(s, av, nn, a.k, hsp_apart, K);
 			const long long col = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;

@@ -144,6 +144,110 @@ __global__ void __launch_bounds__(256) sweep_regular_kernel(SweepArgs a)
 	}
 }
 
+// NiHu's distance-driven policy over many wavenumbers in one persistent
+// launch: per pair the FP64 distance, the near-singular test and the
+// per-kernel degrees of the distance tables are found once; each wavenumber
+// of the work item's batch then reduces its own phase of k d in FP64 and
+// evaluates the pair as the single-wavenumber pass does (the one-point rule
+// for waves beyond the last breakpoint, else each distinct degree of the wave
+// once for all kernel groups), so every wavenumber's matrices equal those of
+// one Burton-Miller assembly at that wavenumber.
+struct SweepNihuArgs {
+	RegArgs r;                  // the mesh's tables, thresholds and records (its k, alpha, out unused)
+	int n_k, n_items, col_tiles;
+	const double *k_rev;        // [n_k] k / (2 pi), FP64
+	const float *kf;            // [n_k] k
+	const float *krevf;         // [n_k] k / (2 pi)
+	const float2 *alpha;        // [n_k]
+	float2 *A, *B;
+	long long ld, kstride;
+};
+
+template <int KB>
+__global__ void __launch_bounds__(256) sweep_nihu_kernel(SweepNihuArgs sa)
+{
+	const RegArgs &a = sa.r;
+	const auto rec = uni(a.rec);
+	const auto k_rev = uni(sa.k_rev);
+	const auto kf = uni(sa.kf);
+	const auto krevf = uni(sa.krevf);
+	const auto alpha = uni((const float *)sa.alpha);
+	for (int item = blockIdx.x; item < sa.n_items; item += gridDim.x) {
+		const int rt = item % a.row_tiles;
+		const int rest = item / a.row_tiles;
+		const int ct = rest % sa.col_tiles;
+		const int f0 = (rest / sa.col_tiles) * KB;
+		const int i = rt * 256 + threadIdx.x;
+		const bool row_ok = i < a.n_rows;
+		const int ii = row_ok ? i : a.n_rows - 1;
+		const double cix = a.tc[ii], ciy = a.tc[a.tn + ii], ciz = a.tc[2 * a.tn + ii];
+		PairCtx p;
+		p.nxx = a.tnu[ii]; p.nxy = a.tnu[a.tn + ii]; p.nxz = a.tnu[2 * a.tn + ii];
+		row_ready(cix, ciy, ciz, p.nxx, p.nxy, p.nxz, ii);
+		const int j0 = ct * a.tile_cols;
+		const int j1 = min(j0 + a.tile_cols, a.n_cols);
+		for (int j = j0; j < j1; ++j) {
+			const double dxd = rec[j].c[0] - cix, dyd = rec[j].c[1] - ciy, dzd = rec[j].c[2] - ciz;
+			const double d2 = __fma_rn(dzd, dzd, __fma_rn(dyd, dyd, dxd * dxd));
+			const bool act = row_ok && !(d2 < rec[j].thr[THR_NEAR]) && i != j;
+			if (__ballot(act) == 0) continue;
+			p.dcx = (float)dxd; p.dcy = (float)dyd; p.dcz = (float)dzd;
+			p.d2f = (float)d2;
+			double y = __builtin_amdgcn_rsq(d2);
+			y = y * (1.5 - 0.5 * d2 * y * y);
+			const double d = d2 * y;
+			const float rinv_d = (float)y;
+			p.df = (float)d;
+			const float nyx = rec[j].nu[0], nyy = rec[j].nu[1], nyz = rec[j].nu[2], w1 = rec[j].w1;
+			const float av = p.dcx * nyx + p.dcy * nyy + p.dcz * nyz;
+			const float nn = p.nxx * nyx + p.nxy * nyy + p.nxz * nyz;
+			p.b0 = p.dcx * p.nxx + p.dcy * p.nxy + p.dcz * p.nxz;
+			const bool allfar = __ballot(act && d2 < rec[j].thr[THR_22_3]) == 0;
+			// degrees: the field complexity plus the distance-table degree; in
+			// an all-far wave every kernel takes the field complexity alone
+			const int fc = a.field_complexity;
+			int dS = act ? fc : -1, dD = dS, dH = dS;
+			if (!allfar) {
+				double t[NTHR];
+#pragma unroll
+				for (int q = 0; q < NTHR; ++q) t[q] = rec[j].thr[q];
+				dS = act ? fc + table_degree_rec<1>(d2, t) : -1;
+				dD = act ? fc + table_degree_rec<2>(d2, t) : -1;
+				dH = act ? fc + table_degree_rec<3>(d2, t) : -1;
+			}
+			const long long col = (long long)j * sa.ld;
+			for (int fb = 0; fb < KB; ++fb) {
+				const int f = f0 + fb;
+				if (f >= sa.n_k) break;
+				double tt = d * k_rev[f];
+				tt -= rint(tt);
+				p.t0 = (float)tt;
+				const float k = kf[f];
+				float2 s[6] = {zero2(), zero2(), zero2(), zero2(), zero2(), zero2()};
+				if (allfar && fc <= 1) {
+					if (act) accumulate_centroid<HELM, ALL4>(p, w1, rinv_d, k, s);
+				} else if (allfar) {
+					eval_degrees<HELM, ALL4>(a, j, dS, p, w1, rinv_d, k, krevf[f], s);
+				} else {
+					eval_groups<HELM, ALL4>(a, j, dS, dD, dH, p, w1, rinv_d, k, krevf[f], s);
+				}
+				if (act) {
+					float2 K[4];
+					kernel_values<HELM, ALL4>(s, av, nn, k, !allfar, K);
+					const float2 al = make_float2(alpha[2 * f], alpha[2 * f + 1]);
+					const float ar = al.x * K[3].x - al.y * K[3].y + K[1].x;
+					const float ai = al.x * K[3].y + al.y * K[3].x + K[1].y;
+					const float br = K[0].x + al.x * K[2].x - al.y * K[2].y;
+					const float bi = K[0].y + al.x * K[2].y + al.y * K[2].x;
+					const long long at = (long long)f * sa.kstride + col;
+					*entry_ptr<float2>(sa.A, at, (unsigned)i * 8u) = make_float2(ar, ai);
+					*entry_ptr<float2>(sa.B, at, (unsigned)i * 8u) = make_float2(br, bi);
+				}
+			}
+		}
+	}
+}
+
 // Wavenumber-independent parts of the FP64 pairs: near pairs (p < n_near)
 // store the Laplace static integrals of SLP, DLP, DLPt, HSP; diagonal pairs
 // (n_near <= p < n_near + n) store diag_static's four sums.

@@ -85,24 +85,45 @@ def test_sweep_uniform_wavenumbers(nb, orc, l3):
         assert relfro(A[f].cpu().numpy(), Ar) < 1e-5 and relfro(B[f].cpu().numpy(), Br) < 1e-5
 
 
-def test_sweep_nihu_policy(nb, orc, l3):
+@pytest.mark.parametrize("kb", [2, 4, 8])
+def test_sweep_nihu_policy(nb, orc, l3, kb):
+    """NiHu's distance-driven degrees, five wavenumbers (not a multiple of the
+    batch) in one launch: the regular entries equal the single-wavenumber
+    assembly's bit for bit (same evaluation, the pair's degrees shared by the
+    batch), the FP64 near-singular and diagonal entries to one complex64
+    rounding; two wavenumbers against the oracle."""
     import torch
     nodes, tris = l3
     n = len(tris)
     m = nb.Mesh(nodes, tris)
-    ks = np.array([2.0, 9.0])
-    A, B = stack(2, n), stack(2, n)
-    nb.assemble_sweep(m, ks, A, B, pol=nb.policy("nihu"))
+    pol = nb.policy("nihu")
+    ks = np.array([2.0, 9.0, 0.6, 23.5, 14.25])
+    A, B = stack(len(ks), n), stack(len(ks), n)
+    A.fill_(complex("nan+nanj"))
+    B.fill_(complex("nan+nanj"))
+    nb.assemble_sweep(m, ks, A, B, pol=pol, launch=nb.NbLaunch(32, kb, 4))
     torch.cuda.synchronize()
-    Ar, Br = orc.assemble_bm(nodes, tris, 9.0)
-    assert relfro(A[1].cpu().numpy(), Ar) < 1e-5 and relfro(B[1].cpu().numpy(), Br) < 1e-5
+    cls = nb.classify(m, m, 1, 1, pol).cpu().numpy()
+    reg, fp64 = cls >= 0, cls < 0
+    assert reg.sum() > 0 and fp64.sum() > n
+    for f, k in enumerate(ks):
+        A1, B1 = nb.colmajor(n, n), nb.colmajor(n, n)
+        nb.assemble_bm(m, float(k), A1, B1, pol=pol)
+        torch.cuda.synchronize()
+        for x, y in ((A[f].cpu().numpy(), A1.cpu().numpy()), (B[f].cpu().numpy(), B1.cpu().numpy())):
+            assert np.array_equal(x[reg], y[reg]), (f, k)
+            d = np.abs(x[fp64] - y[fp64]) / np.abs(y[fp64]).max()
+            assert d.max() < 2e-7, (f, k, d.max())
+    for f in (1, 3):
+        Ar, Br = orc.assemble_bm(nodes, tris, float(ks[f]))
+        assert relfro(A[f].cpu().numpy(), Ar) < 1e-5 and relfro(B[f].cpu().numpy(), Br) < 1e-5
 
 
 def test_sweep_nihu_policy_level4_rows(nb, orc):
-    """The config-4 mesh (5120 triangles), four wavenumbers in one launch under
-    the distance-driven policy, whose 1-point far rule (beyond 22.3 mean
-    edges) the level-3 mesh never reaches: 24 seeded rows of each matrix
-    against the oracle at the same wavenumber."""
+    """The 5120-triangle jittered icosphere (level 4), four wavenumbers in one
+    launch under the distance-driven policy, whose 1-point far rule (beyond
+    22.3 mean edges) the level-3 mesh never reaches: 24 seeded rows of each
+    matrix against the oracle at the same wavenumber."""
     import torch
     from nihu_amd.mesh import jittered_icosphere
     nodes, tris = jittered_icosphere(4, seed=0)
@@ -122,11 +143,12 @@ def test_sweep_nihu_policy_level4_rows(nb, orc):
         assert ea < 1e-5 and eb < 1e-5
 
 
-def test_autotune(nb, l3):
+@pytest.mark.parametrize("rule", ["fixed", "nihu"])
+def test_autotune(nb, l3, rule):
     nodes, tris = l3
     m = nb.Mesh(nodes, tris)
-    best = nb.autotune(m, 8, nb.policy(**CONFIG))
-    print("tuned", best.tile_cols, best.k_batch, best.persistent, flush=True)
+    best = nb.autotune(m, 8, nb.policy(**CONFIG) if rule == "fixed" else nb.policy("nihu"))
+    print(rule, "tuned", best.tile_cols, best.k_batch, best.persistent, flush=True)
     assert best.tile_cols in (16, 32, 64) and best.k_batch in (2, 4, 8) and best.persistent in (2, 4, 8)
 
 

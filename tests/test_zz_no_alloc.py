@@ -1,7 +1,8 @@
 """nb_assemble_sweep_noalloc, the allocation-free variant of
 nb_assemble_sweep (per-call buffers in the library's static scratch region),
 through the C ABI: two calls in a row, the second on a smaller mesh with fewer
-wavenumbers, each equal bit for bit to nb_assemble_sweep on the same inputs;
+wavenumbers, each equal bit for bit to nb_assemble_sweep on the same inputs
+(a fixed 7-point rule and NiHu's distance-driven degrees);
 requests beyond the region and meshes without resident tables are refused
 before anything runs."""
 
@@ -28,10 +29,11 @@ def _stack(nk, n, fill=0.0):
     return t
 
 
-def test_noalloc_sweep_equals_sweep(nb):
+@pytest.mark.parametrize("rule", ["fixed", "nihu"])
+def test_noalloc_sweep_equals_sweep(nb, rule):
     import torch
     from nihu_amd.mesh import jittered_icosphere
-    pol = nb.policy(**CONFIG)
+    pol = nb.policy(**CONFIG) if rule == "fixed" else nb.policy("nihu")
     cases = []
     # first call: 1280 triangles, 8 uniformly spaced wavenumbers (the near
     # pairs' rotation kernel); second: 320 triangles, 3 scattered wavenumbers
@@ -45,7 +47,7 @@ def test_noalloc_sweep_equals_sweep(nb):
         cases.append((m, ks, Ar, Br))
     for m, ks, Ar, Br in cases:
         n = m.n
-        print("allocation-free sweep: %d triangles, %d wavenumbers" % (n, len(ks)), flush=True)
+        print("allocation-free sweep (%s): %d triangles, %d wavenumbers" % (rule, n, len(ks)), flush=True)
         A, B = _stack(len(ks), n, complex("nan+nanj")), _stack(len(ks), n, complex("nan+nanj"))
         nb.assemble_sweep(m, ks, A, B, pol=pol, noalloc=True)
         torch.cuda.synchronize()
@@ -64,7 +66,7 @@ def test_noalloc_sweep_refusals(nb):
     # no tables resident for this mesh yet
     with pytest.raises(NotImplementedError):
         nb.assemble_sweep(m, np.array([1.0, 2.0]), A, B, pol=pol, noalloc=True)
-    # the NiHu policy has no one-launch path
+    # nor those of NiHu's degrees
     with pytest.raises(NotImplementedError):
         nb.assemble_sweep(m, np.array([1.0, 2.0]), A, B, pol=nb.policy("nihu"), noalloc=True)
     # every pair near-singular (limit 100 diameters): 320^2 FP64 pairs of static
