@@ -154,7 +154,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=-1,
                     help="CPU baseline: -1 the whole matrices (default; also gives full-matrix and solved-field "
                          "errors), N > 0 a sample of N rows, 0 skip")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt"))
+    ap.add_argument("--traffic-file", default=None,
+                    help="rocprofv3 FETCH/WRITE summary of the regular kernel (default: the one of --policy)")
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
     ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "mixed"], default="bm",
@@ -170,6 +171,9 @@ def main():
     ap.add_argument("--refine", type=int, default=0,
                     help="serve: steps of FP64 iterative refinement of each complex64 LU solve")
     args = ap.parse_args()
+    if args.traffic_file is None:
+        args.traffic_file = os.path.join(ROOT, "profiles", "regular_bm_L5_traffic.txt" if args.policy == "config"
+                                         else "regular_bm_L5_nihu_traffic.txt")
 
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU path (row blocks per GPU) is design only. "
@@ -791,13 +795,18 @@ def _serve_stream(srv, n_req, rate, seed):
     t0 = time.perf_counter()
     res = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0
+    failed = [r for r in res if r.status != "ok"]
+    for r in failed:
+        log("serve: request %d (%s, k %.3f) %s: %s" % (r.rid, r.mesh, r.k, r.status, r.error))
+    res = [r for r in res if r.status == "ok"]
     lat = [r.latency for r in res]
-    return {"rate_per_s": rate, "requests": len(res), "wall_s": wall,
+    return {"rate_per_s": rate, "requests": len(res) + len(failed), "failed": len(failed), "wall_s": wall,
             "solves_per_s": len(res) / wall,
             "p50_s": serve.percentile(lat, 50), "p99_s": serve.percentile(lat, 99),
             # completions kept up with arrivals: over the whole stream (first
-            # arrival to last completion) at least 90 % of the offered rate
-            "kept_up": len(res) / wall >= 0.9 * rate,
+            # arrival to last completion) at least 90 % of the offered rate,
+            # every request solved
+            "kept_up": len(res) / wall >= 0.9 * rate and not failed,
             "by_mesh_p50_s": {k: serve.percentile([r.latency for r in res if r.mesh == k], 50)
                               for k in sorted({r.mesh for r in res})}}
 

@@ -90,11 +90,19 @@ typedef struct {
 	double im;
 } nb_scalar;
 
+/* arithmetic of the regular pairs (near-singular and singular pairs are FP64 in both) */
+enum nb_precision {
+	NB_PREC_MIXED = 0,     /* regular pairs in FP32 with the phase k r reduced in FP64 (default) */
+	NB_PREC_FP64 = 1       /* every pair in FP64: NiHu's plain method (collocation on triangles;
+	                          quadrilaterals and the Galerkin formalism: NB_ERR_UNSUPPORTED) */
+};
+
 typedef struct {
 	int32_t rule;          /* nb_rule */
 	int32_t fixed_degree;  /* Dunavant degree for NB_RULE_FIXED, 0..14 */
 	int32_t near_mode;     /* nb_near */
 	double near_limit;     /* 1.5 for NiHu */
+	int32_t precision;     /* nb_precision; 0 when the struct is zero-initialised */
 } nb_policy;
 
 /* Caller-owned result matrix, column-major: entry (i, j) at ptr[j * ld + i],

@@ -46,6 +46,7 @@ int nbg_geometry(const double *xyz, int n_nodes, const int32_t *tri, int n, doub
 	double *J, double *R, double *diam, double *X);
 double nbg_threshold(double scale, double limit);
 int nbg_quad_table(int degree, int n, const double *c, const double *J, const double *X, float *out);
+int nbg_quad_table64(int degree, int n, const double *c, const double *J, const double *X, double *out);
 int nbg_singular_rule(int match, double *out, int cap);
 int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, int32_t *pf, int32_t *info);
 int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk, int32_t *order_out, int32_t *elem,
@@ -282,6 +283,7 @@ struct nb_mesh {
 	DevPtr d_thr, d_rec;
 	DevPtr d_qt[nbk::NSLOT];
 	int qn[nbk::NSLOT] = {0};
+	DevPtr d_qt64[nbk::NSLOT];      // FP64 point tables of the FP64 precision policy
 	DevPtr d_qq[nbk::QN_MAX + 1];              // quadrilaterals: Gauss square tables by order
 	// near-pair lists per (test mesh, near policy), oldest first; callers hold
 	// a list by shared_ptr, so evicting it cannot free one in use
@@ -368,6 +370,20 @@ int ensure_quad(nb_mesh *m, int degree)
 	if (s) return s;
 	m->qn[degree] = Q;
 	m->bytes += (int64_t)tab.size() * 4;
+	return NB_OK;
+}
+
+// FP64 point table of one degree (offsets from the centre, w J / (4 pi))
+int ensure_quad64(nb_mesh *m, int degree)
+{
+	int s;
+	if ((s = ensure_quad(m, degree))) return s;
+	if (m->d_qt64[degree]) return NB_OK;
+	std::vector<double> tab((size_t)m->n * m->qn[degree] * 4);
+	if (nbg_quad_table64(degree, m->n, m->c.data(), m->J.data(), m->X.data(), tab.data()) != m->qn[degree])
+		return fail(NB_ERR_RUNTIME, "quadrature table");
+	if ((s = upload(m->d_qt64[degree], tab))) return s;
+	m->bytes += (int64_t)tab.size() * 8;
 	return NB_OK;
 }
 
@@ -567,6 +583,8 @@ struct Job {
 // Galerkin assembly into a device buffer
 int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
+	if (jb.pol.precision != NB_PREC_MIXED)
+		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
 	nb_mesh *trial = jb.trial;
 	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
 	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
@@ -750,6 +768,8 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	nb_mesh *trial = jb.trial;
 	const nb_mesh *test = jb.test;
 	int s;
+	if (jb.pol.precision != NB_PREC_MIXED)
+		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
 	if ((s = check_quad_policy(jb.pol))) return s;
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	int fixed_n = 0;
@@ -910,8 +930,9 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	std::vector<int> degs;
 	policy_degrees(jb.pol, jb.km, 0, degs);
+	const bool fp64 = jb.pol.precision == NB_PREC_FP64;
 	for (int d : degs)
-		if ((s = ensure_quad(trial, d))) return s;
+		if ((s = fp64 ? ensure_quad64(trial, d) : ensure_quad(trial, d))) return s;
 	std::shared_ptr<NearList> nl;
 	if ((jb.passes & 2) && (s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
 
@@ -953,7 +974,43 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		NB_HIP(hipEventRecord(ev_fork, st));
 		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
 	}
-	if (grid > 0 && (jb.passes & 1)) {
+	if (grid > 0 && (jb.passes & 1) && fp64) {
+		// every regular pair in FP64
+		using namespace nbk;
+		Reg64Args b;
+		std::memset(&b, 0, sizeof(b));
+		b.n_rows = a.n_rows; b.n_cols = a.n_cols; b.same_mesh = a.same_mesh; b.row_tiles = a.row_tiles;
+		b.tile_cols = a.tile_cols; b.rmap = a.rmap; b.cmap = a.cmap; b.tn = a.tn; b.sn = a.sn;
+		b.tc = a.tc; b.tnu = (const double *)test->d_nu64.get();
+		b.sc = a.sc; b.snu = (const double *)trial->d_nu64.get();
+		b.thr = a.thr;
+		for (int d = 0; d < NSLOT; ++d) { b.qt[d] = (const double *)trial->d_qt64[d].get(); b.qn[d] = trial->qn[d]; }
+		b.fixed_degree = a.fixed_degree;
+		b.k = jb.k;
+		b.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+		b.out0 = out0; b.out1 = out1; b.ld = ld; b.dtype = dtype; b.accumulate = accumulate;
+		const int pe = prof_begin(0, st);
+#define NB_R64(F, K, M) hipLaunchKernelGGL((regular64_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, b)
+		if (jb.mode == OUT_BM) NB_R64(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_R64(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_R64(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_R64(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_R64(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_R64(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_R64(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_R64(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_R64(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_R64
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	} else if (grid > 0 && (jb.passes & 1)) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 		const bool fix7 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 7;
@@ -1177,6 +1234,7 @@ int check_policy(const nb_policy *p)
 		return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
 	if (p->near_mode < NB_NEAR_MEAN_EDGE || p->near_mode > NB_NEAR_NONE) return fail(NB_ERR_ARG, "unknown near-singular mode");
 	if (p->near_mode != NB_NEAR_NONE && !(p->near_limit > 0.0)) return fail(NB_ERR_ARG, "near limit must be positive");
+	if (p->precision != NB_PREC_MIXED && p->precision != NB_PREC_FP64) return fail(NB_ERR_ARG, "unknown precision");
 	return NB_OK;
 }
 
@@ -1265,7 +1323,7 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	std::vector<int> degs;
 	policy_degrees(*policy, nbk::ALL4, 0, degs);
 	if (noalloc) {
-		bool ready = tri;
+		bool ready = tri && policy->precision == NB_PREC_MIXED;
 		for (int d : degs) ready = ready && m->d_qt[d];
 		if (ready && !nihu) ready = m->qn[policy->fixed_degree] == 7 || m->qn[policy->fixed_degree] == 1;
 		if (!ready)
@@ -1275,9 +1333,10 @@ int sweep_impl(const nb_mesh *mesh, const double *k, int32_t n_k, const nb_polic
 	}
 	const int Q = tri && !nihu ? (ensure_quad(m, policy->fixed_degree) ? -1 : m->qn[policy->fixed_degree]) : 0;
 	if (Q < 0) return NB_ERR_RANGE;
-	if (!tri || (!nihu && Q != 7 && Q != 1)) {
-		// quadrilateral meshes, meshes with both element types and the other
-		// fixed rules: one Burton-Miller assembly per wavenumber
+	if (!tri || (!nihu && Q != 7 && Q != 1) || policy->precision != NB_PREC_MIXED) {
+		// quadrilateral meshes, meshes with both element types, the other
+		// fixed rules and the FP64 precision policy: one Burton-Miller
+		// assembly per wavenumber
 		if (tune_block > 0) return fail(NB_ERR_UNSUPPORTED, "this rule has no one-launch sweep to tune");
 		if (!tri)
 			for (const nb_mesh *p : {m, m->part[1].get()})
@@ -1864,6 +1923,11 @@ int nb_autotune(const nb_mesh *mesh, int32_t n_k, const nb_policy *policy, nb_la
 	int s;
 	if ((s = check_policy(policy))) return s;
 	if (mesh->npe != 3) return fail(NB_ERR_UNSUPPORTED, "sweep autotuning is implemented for triangle meshes only");
+	if (policy->precision != NB_PREC_MIXED) {
+		// the FP64 precision policy runs one single-wavenumber assembly per wavenumber
+		*best = {32, 4, 4};
+		return NB_OK;
+	}
 	if (policy->rule == NB_RULE_FIXED && policy->fixed_degree >= 0 && policy->fixed_degree < nbk::NSLOT) {
 		nb_mesh *m = const_cast<nb_mesh *>(mesh);
 		std::lock_guard<std::mutex> lk(m->mu);

@@ -252,6 +252,29 @@ int nbg_quad_table(int degree, int n, const double *c, const double *J, const do
 	return Q;
 }
 
+// The same points in FP64 for the FP64 regular pass:
+// out[(e * Q + q) * 4 + {0..3}] = (y_q - c_e) xyz, w_q * J_e / (4 pi).
+int nbg_quad_table64(int degree, int n, const double *c, const double *J, const double *X, double *out)
+{
+	double xi[64], eta[64], w[64];
+	int Q = nbg_dunavant(degree, xi, eta, w, 64);
+	if (Q < 0) return -1;
+	const double four_pi = 4.0 * 3.141592653589793238462643383279502884;
+	for (int e = 0; e < n; ++e) {
+		P3 V[3];
+		for (int a = 0; a < 3; ++a)
+			V[a] = {X[(3 * a) * n + e], X[(3 * a + 1) * n + e], X[(3 * a + 2) * n + e]};
+		P3 cc = {c[e], c[n + e], c[2 * n + e]};
+		for (int q = 0; q < Q; ++q) {
+			P3 o = sub(shape_x(V, xi[q], eta[q]), cc);
+			double *p = out + ((size_t)e * Q + q) * 4;
+			p[0] = o.x; p[1] = o.y; p[2] = o.z;
+			p[3] = w[q] * J[e] / four_pi;
+		}
+	}
+	return Q;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------

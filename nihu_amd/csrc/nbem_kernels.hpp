@@ -1087,6 +1087,128 @@ __global__ void __launch_bounds__(64) fp64_pass_kernel(NearArgs a)
 	}
 }
 
+// ---------------------------------------------------------------- regular pairs in FP64
+
+// The regular pass of the FP64 precision policy (every pair in FP64, the
+// plain method of nihu/core/double_integral.hpp:543-589): the same pair
+// classes and per-kernel degrees as regular_kernel, each quadrature point
+// evaluated in FP64 from the point tables of nbg_quad_table64 (offset of the
+// point from its element's centre, w J / (4 pi)).
+struct Reg64Args {
+	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	const int *rmap, *cmap;      // result row / column of each test / trial element (null: its own index)
+	int tn, sn;
+	const double *tc, *tnu;      // test centres, unit normals [3][tn]
+	const double *sc, *snu;      // trial centres, unit normals [3][sn]
+	const double *thr;           // [NTHR][sn]
+	const double *qt[NSLOT];     // per degree: [sn][Q][4]
+	int qn[NSLOT];
+	int fixed_degree;            // >= 0: fixed rule
+	double k;
+	double2 alpha;
+	void *out0, *out1;
+	long long ld;
+	int dtype, accumulate;
+};
+
+// the kernels of MASK at one point, rv = y - x (FP64): s += K w J
+template <int FAM, int MASK>
+__device__ __forceinline__ void full_point64(D3 rv, D3 nx, D3 ny, double k, double w4, double2 (&s)[4])
+{
+	const double r2 = d3dot(rv, rv);
+	const double rinv = rsqrt64(r2);
+	double sn = 0.0, cs = 1.0, kr = 0.0;
+	if (FAM == HELM) {
+		kr = k * (r2 * rinv);
+		sincos64(kr, &sn, &cs);
+	}
+	// e^{-ikr} = cs - i sn; g0 = w J / (4 pi r)
+	const double g0 = w4 * rinv;
+	if (MASK & SLP) { s[0].x += cs * g0; s[0].y += -sn * g0; }
+	if (MASK & (DLP | DLPT | HSP)) {
+		const double g1 = g0 * rinv;
+		// e (-1 - ikr)
+		const double er = -cs - kr * sn, ei = -kr * cs + sn;
+		const double rdny = d3dot(rv, ny) * rinv, rdnx = -d3dot(rv, nx) * rinv;
+		if (MASK & DLP) { s[1].x += er * g1 * rdny; s[1].y += ei * g1 * rdny; }
+		if (MASK & DLPT) { s[2].x += er * g1 * rdnx; s[2].y += ei * g1 * rdnx; }
+		if (MASK & HSP) {
+			const double g3 = g1 * rinv, cc = rdnx * rdny, nn = d3dot(nx, ny);
+			// e [(3 - kr^2 + 3ikr) cc + (1 + ikr) nn]
+			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
+			s[3].x += (cs * ar + sn * ai) * g3;
+			s[3].y += (cs * ai - sn * ar) * g3;
+		}
+	}
+}
+
+// the kernels of MASK over the rule of each lane's degree (-1: none), one
+// distinct degree of the wave at a time (wave-uniform table address)
+template <int FAM, int MASK>
+__device__ __forceinline__ void eval64(const Reg64Args &a, int j, int deg, D3 dc, D3 nx, D3 ny, double2 (&s)[4])
+{
+	for (;;) {
+		const unsigned long long m = __ballot(deg >= 0);
+		if (m == 0) break;
+		const int dsel = __builtin_amdgcn_readlane(deg, __ffsll((long long)m) - 1);
+		const unsigned long long msel = __ballot(deg == dsel);
+		if ((msel >> __lane_id()) & 1ull) {
+			const int nq = a.qn[dsel];
+			const double *q = a.qt[dsel] + (size_t)j * nq * 4;
+			for (int t = 0; t < nq; ++t) {
+				const D3 rv = {dc.x + q[4 * t], dc.y + q[4 * t + 1], dc.z + q[4 * t + 2]};
+				full_point64<FAM, MASK>(rv, nx, ny, a.k, q[4 * t + 3], s);
+			}
+			deg = -1;
+		}
+	}
+}
+
+template <int FAM, int KM, int MODE>
+__global__ void __launch_bounds__(256) regular64_kernel(Reg64Args a)
+{
+	const int tile_r = blockIdx.x % a.row_tiles;
+	const int tile_c = blockIdx.x / a.row_tiles;
+	const int i = tile_r * 256 + threadIdx.x;
+	if (i >= a.n_rows) return;
+	const int orow = a.rmap ? a.rmap[i] : i;
+	const D3 x0 = {a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i]};
+	const D3 nx = {a.tnu[i], a.tnu[a.tn + i], a.tnu[2 * a.tn + i]};
+	const int j0 = tile_c * a.tile_cols;
+	const int j1 = min(j0 + a.tile_cols, a.n_cols);
+	for (int j = j0; j < j1; ++j) {
+		// classification exactly as regular_kernel: d2 from c_j - c_i in FP64
+		const D3 dc = {a.sc[j] - x0.x, a.sc[a.sn + j] - x0.y, a.sc[2 * a.sn + j] - x0.z};
+		const double d2 = __fma_rn(dc.z, dc.z, __fma_rn(dc.y, dc.y, dc.x * dc.x));
+		const int cj = a.cmap ? a.cmap[j] : j;
+		const bool act = !(d2 < a.thr[THR_NEAR * a.sn + j]) && !(a.same_mesh && i == cj);
+		const D3 ny = {a.snu[j], a.snu[a.sn + j], a.snu[2 * a.sn + j]};
+		const int fd = a.fixed_degree;
+		double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+		if (KM & SLP) eval64<FAM, SLP>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<1>(d2, a.thr, a.sn, j), dc, nx, ny, s);
+		if (KM & (DLP | DLPT))
+			eval64<FAM, KM & (DLP | DLPT)>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<2>(d2, a.thr, a.sn, j),
+				dc, nx, ny, s);
+		if (KM & HSP) eval64<FAM, HSP>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<3>(d2, a.thr, a.sn, j), dc, nx, ny, s);
+		if (!act) continue;
+		const long long idx = (long long)cj * a.ld + orow;
+		if (MODE == OUT_SINGLE) {
+			const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+			const double re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
+			const double im = a.alpha.x * s[kk].y + a.alpha.y * s[kk].x;
+			store_entry64(a.out0, idx, a.dtype, a.accumulate, re, im);
+		} else {
+			// A = alpha D + H, B = G + alpha Ht
+			const double ar = a.alpha.x * s[3].x - a.alpha.y * s[3].y + s[1].x;
+			const double ai = a.alpha.x * s[3].y + a.alpha.y * s[3].x + s[1].y;
+			const double br = s[0].x + a.alpha.x * s[2].x - a.alpha.y * s[2].y;
+			const double bi = s[0].y + a.alpha.x * s[2].y + a.alpha.y * s[2].x;
+			store_entry64(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+			store_entry64(a.out1, idx, a.dtype, a.accumulate, br, bi);
+		}
+	}
+}
+
 // ---------------------------------------------------------------- near list, classes, identity
 
 struct ListArgs {

@@ -20,6 +20,7 @@ NB_OK, NB_ERR_ARG, NB_ERR_UNSUPPORTED, NB_ERR_RANGE, NB_ERR_OOM, NB_ERR_DEVICE, 
 NB_LAPLACE, NB_HELMHOLTZ, NB_IDENTITY = 0, 1, 2
 NB_RULE_NIHU, NB_RULE_FIXED = 0, 1
 NB_NEAR_MEAN_EDGE, NB_NEAR_DIAMETER, NB_NEAR_NONE = 0, 1, 2
+NB_PREC_MIXED, NB_PREC_FP64 = 0, 1
 NB_C64, NB_C128, NB_F32, NB_F64 = 0, 1, 2, 3
 NB_HOST, NB_DEVICE = 0, 1
 NB_COLLOCATION, NB_GALERKIN, NB_GALERKIN_LINEAR = 0, 1, 2
@@ -43,7 +44,8 @@ class NbScalar(C.Structure):
 
 
 class NbPolicy(C.Structure):
-    _fields_ = [("rule", C.c_int32), ("fixed_degree", C.c_int32), ("near_mode", C.c_int32), ("near_limit", C.c_double)]
+    _fields_ = [("rule", C.c_int32), ("fixed_degree", C.c_int32), ("near_mode", C.c_int32), ("near_limit", C.c_double),
+                ("precision", C.c_int32)]
 
 
 class NbOut(C.Structure):
@@ -113,12 +115,16 @@ def check(status: int):
     raise NbemError(msg)
 
 
-def policy(rule: str | int = "nihu", degree: int = 5, near: str | int = "mean_edge", limit: float = 1.5) -> NbPolicy:
+def policy(rule: str | int = "nihu", degree: int = 5, near: str | int = "mean_edge", limit: float = 1.5,
+           precision: str | int = "mixed") -> NbPolicy:
     """'nihu' = NiHu's per-pair degree tables; 'fixed' = one Dunavant degree.
-    near: 'mean_edge' (NiHu, limit 1.5), 'diameter' (limit in element diameters) or 'none'."""
+    near: 'mean_edge' (NiHu, limit 1.5), 'diameter' (limit in element diameters) or 'none'.
+    precision: 'mixed' (regular pairs in FP32 with the phase reduced in FP64)
+    or 'fp64' (every pair in FP64; collocation on triangles)."""
     r = {"nihu": NB_RULE_NIHU, "fixed": NB_RULE_FIXED}.get(rule, rule)
     n = {"mean_edge": NB_NEAR_MEAN_EDGE, "diameter": NB_NEAR_DIAMETER, "none": NB_NEAR_NONE}.get(near, near)
-    return NbPolicy(int(r), int(degree), int(n), float(limit))
+    p = {"mixed": NB_PREC_MIXED, "fp64": NB_PREC_FP64}.get(precision, precision)
+    return NbPolicy(int(r), int(degree), int(n), float(limit), int(p))
 
 
 NIHU_POLICY = dict(rule="nihu", near="mean_edge", limit=1.5)

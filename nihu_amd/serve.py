@@ -30,6 +30,15 @@ depend on the mix or the batch: every kernel computes each matrix entry in a
 fixed order without atomics, the sweep computes each wavenumber's matrices
 independently of the others in its launch, and each request's LU runs on
 its own buffers.
+
+Failures stay with their request: an invalid request (unknown mesh, a
+wavenumber that is not finite and positive, a direction that is not a unit
+vector) is rejected before it is queued; one not admitted within its
+deadline leaves the queue as expired; a batch whose launch raises (the
+engine's error) ends its requests with that error and frees its slot and
+memory; a solver that does not converge, or a solution holding NaN or Inf
+(scanned on the device), ends that request alone with an error.  Every
+request ends with exactly one Result, and the loop goes on with the others.
 """
 
 from __future__ import annotations
@@ -51,6 +60,7 @@ class Request:
     k: float               # wavenumber
     direction: tuple       # unit propagation direction of the plane wave
     arrival: float = 0.0   # seconds after the start of the run
+    deadline: float | None = None   # seconds after arrival by which it must be admitted (None: no limit)
 
 
 @dataclasses.dataclass
@@ -62,11 +72,30 @@ class Result:
     arrival: float
     admitted: float
     done: float
-    p: np.ndarray          # scattered surface pressure at the element centres
+    p: np.ndarray | None   # scattered surface pressure at the element centres (None unless status is "ok")
+    status: str = "ok"     # "ok", "rejected" (invalid request), "expired" (not admitted by its deadline),
+                           # "error" (assembly failed, solver did not converge, or a non-finite solution)
+    error: str = ""
 
     @property
     def latency(self):
         return self.done - self.arrival
+
+
+def check_request(req: Request, meshes) -> str:
+    """Why a request cannot be served ("" if it can): unknown mesh, a
+    wavenumber that is not finite and positive, or a direction that is not a
+    finite unit 3-vector (to 1e-6)."""
+    if req.mesh not in meshes:
+        return "unknown mesh %r" % (req.mesh,)
+    if not (np.isfinite(req.k) and req.k > 0):
+        return "wavenumber must be finite and positive, got %r" % (req.k,)
+    d = np.asarray(req.direction, dtype=np.float64)
+    if d.shape != (3,) or not np.all(np.isfinite(d)) or abs(np.linalg.norm(d) - 1.0) > 1e-6:
+        return "direction must be a finite unit 3-vector"
+    if req.deadline is not None and not (req.deadline >= 0):
+        return "deadline must be non-negative"
+    return ""
 
 
 def footprint_bytes(n: int, solver: str = "lu", restart: int = 3000) -> int:
@@ -81,9 +110,11 @@ def footprint_bytes(n: int, solver: str = "lu", restart: int = 3000) -> int:
 
 class Server:
     def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None, refine: int = 0,
-                 max_batch: int = 8, solver: str = "gmres", tol: float = 1e-8, restart: int = 3000):
+                 max_batch: int = 8, solver: str = "gmres", tol: float = 1e-8, restart: int = 3000,
+                 maxiter: int | None = None):
         """solver: "gmres" (NiHu's application: unpreconditioned GMRES, tol
-        on the relative residual, restart) or "lu" (complex64 LU).  refine:
+        on the relative residual, restart, at most maxiter iterations, default
+        the number of unknowns) or "lu" (complex64 LU).  refine:
         steps of iterative refinement of the complex64 LU solution (residual
         b - A p accumulated in complex128 over column blocks, correction
         solved with the same LU factor).  max_batch: requests on one mesh
@@ -93,6 +124,7 @@ class Server:
         self.solver = solver
         self.tol = float(tol)
         self.restart = int(restart)
+        self.maxiter = None if maxiter is None else int(maxiter)
         self.refine = int(refine)
         self.max_batch = max(1, int(max_batch))
         import torch
@@ -122,11 +154,14 @@ class Server:
         self.launch = {}
         for key, m in self.meshes.items():
             pol = self._policy(m)
-            if m.nodes_per_element == 3 and pol.rule == nbem.NB_RULE_FIXED:
+            if m.nodes_per_element == 3:
                 self.launch[key] = nbem.autotune(m, self.max_batch, pol)
             else:
                 self.launch[key] = None
         torch.cuda.synchronize()
+
+    def _sync(self):
+        self.torch.cuda.synchronize()
 
     def _policy(self, mesh):
         return self.quad_policy if mesh.nodes_per_element != 3 else self.policy
@@ -137,8 +172,9 @@ class Server:
     def _launch(self, reqs, stream):
         """One batch of requests on one mesh: the Burton-Miller matrices of
         all their wavenumbers in one sweep launch, then per request the
-        right-hand side, LU factorisation and solve.  Returns per request
-        (event after its solve, its buffers)."""
+        right-hand side and solve.  Returns per request (event after its
+        solve, its buffers with the solution last, device flag "the solution
+        is finite", error message or "").  An exception fails the batch."""
         torch = self.torch
         key = reqs[0].mesh
         mesh = self.meshes[key]
@@ -157,12 +193,13 @@ class Server:
                 phase = -req.k * (c @ d)
                 pinc = torch.polar(torch.ones_like(phase), phase)
                 q = (1j * req.k) * (nu @ d) * pinc
+                err = ""
                 if self.solver == "gmres":
                     rhs = nbem.gemv(B[g], q)
-                    res = solve.gmres(A[g], rhs, tol=self.tol, restart=self.restart)
+                    res = solve.gmres(A[g], rhs, tol=self.tol, restart=self.restart, maxiter=self.maxiter)
                     if not res.converged:
-                        raise RuntimeError("GMRES did not reach the relative residual %g (%g after %d iterations)"
-                                           % (self.tol, res.residual, res.iterations))
+                        err = ("GMRES did not reach the relative residual %g (%g after %d iterations)"
+                               % (self.tol, res.residual, res.iterations))
                     p = res.x
                     keep = (A, B, rhs, p)
                 else:
@@ -172,26 +209,44 @@ class Server:
                     if self.refine > 0:
                         p = _refine(A[g], LU, piv, rhs.squeeze(1), p, self.refine)
                     keep = (A, B, LU, piv, rhs, p)
+                # NaN or Inf anywhere in the matrices or the right-hand side
+                # reaches the solution: one scan of the solution catches it
+                finite = torch.isfinite(torch.view_as_real(p)).all()
                 ev = torch.cuda.Event()
                 ev.record(stream)
                 # keep the buffers alive until the event has passed
-                out.append((ev, keep))
+                out.append((ev, keep, finite, err))
         return out
 
     def run(self, requests, realtime: bool = True):
         """Serve the requests; returns the results in completion order.
         realtime: wait for each arrival time (Poisson streams); otherwise all
-        requests are available at once (arrival offsets still set latency)."""
-        torch = self.torch
-        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        requests are available at once (arrival offsets still set latency).
+        A request that fails -- invalid, not admitted by its deadline, its
+        batch's launch raising, its solver not converging, a non-finite
+        solution -- ends with that status and message; the others go on."""
+        pending = deque()
+        results = []
+        for req in sorted(requests, key=lambda r: r.arrival):
+            why = check_request(req, self.meshes)
+            if why:
+                results.append(Result(req.rid, req.mesh, req.k, 0, req.arrival, float("nan"), req.arrival, None,
+                                      "rejected", why))
+            else:
+                pending.append(req)
         free_slots = list(range(len(self.slots)))
         in_use = 0.0
-        active = []   # [batch, slot, future of [(event, buffers)], bytes, admitted, retired flags]
-        results = []
+        active = []   # [batch, slot, future of per-request work, bytes, admitted, retired flags]
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         with ThreadPoolExecutor(max_workers=len(self.slots), thread_name_prefix="nbem-slot") as pool:
             while pending or active:
+                # requests whose deadline for admission has passed leave the queue
+                for req in [r for r in pending if r.deadline is not None and now() > r.arrival + r.deadline]:
+                    pending.remove(req)
+                    results.append(Result(req.rid, req.mesh, req.k, self.meshes[req.mesh].n, req.arrival,
+                                          float("nan"), now(), None, "expired",
+                                          "not admitted within %g s of its arrival" % req.deadline))
                 # admit as many arrived requests as slots and memory allow: the
                 # oldest one, with the next arrived requests on its mesh
                 while pending and free_slots:
@@ -214,19 +269,29 @@ class Server:
                     fut = pool.submit(self._launch, batch, self.slots[slot])
                     in_use += fp * len(batch)
                     active.append([batch, slot, fut, fp * len(batch), t_adm, [False] * len(batch)])
-                # retire finished requests (a failed launch raises here); a
-                # batch frees its slot and memory when all of it is done
+                # retire finished requests; a batch frees its slot and memory
+                # when all of it is done (or its launch has failed)
                 still = []
                 for item in active:
                     batch, slot, fut, nbytes, t_adm, done = item
                     if fut.done():
-                        per_req = fut.result()
-                        for g, (ev, bufs) in enumerate(per_req):
-                            if not done[g] and ev.query():
-                                p = bufs[-1].cpu().numpy()
-                                req = batch[g]
-                                results.append(Result(req.rid, req.mesh, req.k, len(p), req.arrival, t_adm, now(), p))
+                        exc = fut.exception()
+                        if exc is not None:
+                            for g, req in enumerate(batch):
+                                results.append(Result(req.rid, req.mesh, req.k, self.meshes[req.mesh].n, req.arrival,
+                                                      t_adm, now(), None, "error",
+                                                      "%s: %s" % (type(exc).__name__, exc)))
                                 done[g] = True
+                        else:
+                            for g, (ev, bufs, finite, err) in enumerate(fut.result()):
+                                if not done[g] and ev.query():
+                                    req = batch[g]
+                                    p = bufs[-1].cpu().numpy()
+                                    if not err and not bool(finite):
+                                        err = "non-finite values in the solution"
+                                    results.append(Result(req.rid, req.mesh, req.k, len(p), req.arrival, t_adm,
+                                                          now(), None if err else p, "error" if err else "ok", err))
+                                    done[g] = True
                         if all(done):
                             in_use -= nbytes
                             free_slots.append(slot)
@@ -237,7 +302,7 @@ class Server:
                 active = still
                 if not active and pending and realtime and pending[0].arrival > now():
                     time.sleep(min(1e-3, max(0.0, pending[0].arrival - now())))
-        torch.cuda.synchronize()
+        self._sync()
         return results
 
 

@@ -1,5 +1,6 @@
 """Serving loop: physics (rigid-sphere Mie series), same result alone and in a
-mix of concurrent requests, admission by HBM footprint."""
+mix of concurrent requests, admission by HBM footprint, and requests that fail
+(invalid, expired, solver not converged) ending alone with their status."""
 
 import numpy as np
 import pytest
@@ -225,3 +226,30 @@ def test_gemv_and_gmres_against_lu(meshes):
     print("GMRES: %d iterations, residual %.2e, error vs complex128 LU %.2e" % (r.iterations, r.residual, err),
           flush=True)
     assert r.converged and err < 1e-8
+
+
+def test_failures_on_the_device_path(meshes):
+    """The real launch path with failing requests in the stream: invalid
+    requests are rejected before any launch, a request that cannot be
+    admitted before its deadline expires, a GMRES that cannot converge in
+    its iteration limit (3 iterations) ends its request with an error; every
+    other request is solved and equals its result alone."""
+    from nihu_amd import serve
+    srv = serve.Server({"L2": meshes["L2"], "L3": meshes["L3"]}, slots=1, max_batch=1)
+    reqs = [serve.Request(0, "L3", 5.0, (1.0, 0.0, 0.0)), serve.Request(1, "L2", -3.0, (1.0, 0.0, 0.0)),
+            serve.Request(2, "L2", 4.0, (0.0, 2.0, 0.0)), serve.Request(3, "L3", 6.5, (0.0, 0.0, 1.0)),
+            serve.Request(4, "L2", 2.5, (0.0, 1.0, 0.0), deadline=0.0),
+            serve.Request(5, "L2", 3.5, (0.0, 1.0, 0.0))]
+    res = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    assert sorted(res) == list(range(6))
+    assert res[1].status == "rejected" and res[2].status == "rejected"
+    assert res[4].status == "expired" and res[4].p is None
+    for rid in (0, 3, 5):
+        assert res[rid].status == "ok", res[rid].error
+        rq = reqs[rid]
+        alone = srv.run([serve.Request(rq.rid, rq.mesh, rq.k, rq.direction)], realtime=False)[0]
+        assert np.abs(alone.p - res[rid].p).max() <= 1e-6 * np.abs(alone.p).max()
+    tight = serve.Server({"L3": meshes["L3"]}, slots=1, maxiter=3)
+    bad = tight.run([serve.Request(9, "L3", 30.0, (1.0, 0.0, 0.0))], realtime=False)[0]
+    print("3-iteration GMRES:", bad.status, bad.error, flush=True)
+    assert bad.status == "error" and "GMRES" in bad.error and bad.p is None
