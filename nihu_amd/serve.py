@@ -17,7 +17,10 @@ admits the oldest waiting request as soon as a slot (a HIP stream with a host
 worker thread) is free and its HBM footprint fits the budget, together with
 up to max_batch - 1 further arrived requests on the same mesh (continuous
 batching: their matrices come from one nb_assemble_sweep launch, the
-frequency-sweep pattern of benchmark/EAA/sphere/homogeneous/bem/bem.cpp:91-115),
+frequency-sweep pattern of benchmark/EAA/sphere/homogeneous/bem/bem.cpp:91-115,
+and their GMRES iterations run in lockstep, one host wait per step for the
+batch; a byte cap keeps the large meshes, whose assembly is a small part of
+their solve, at one request per launch),
 hands the batch's whole pipeline to that slot's thread, which launches it on
 the slot's stream, and goes on admitting while earlier batches are still on
 the device, so requests of very different sizes overlap.  The threads
@@ -111,14 +114,17 @@ def footprint_bytes(n: int, solver: str = "lu", restart: int = 3000) -> int:
 class Server:
     def __init__(self, meshes: dict, slots: int = 4, hbm_budget: float | None = None, policy=None, refine: int = 0,
                  max_batch: int = 8, solver: str = "gmres", tol: float = 1e-8, restart: int = 3000,
-                 maxiter: int | None = None):
+                 maxiter: int | None = None, max_batch_bytes: float = 8 * 2 ** 30):
         """solver: "gmres" (NiHu's application: unpreconditioned GMRES, tol
         on the relative residual, restart, at most maxiter iterations, default
         the number of unknowns) or "lu" (complex64 LU).  refine:
         steps of iterative refinement of the complex64 LU solution (residual
         b - A p accumulated in complex128 over column blocks, correction
         solved with the same LU factor).  max_batch: requests on one mesh
-        assembled in one sweep launch (1: one assembly per request)."""
+        assembled in one sweep launch and solved together (1: one assembly
+        per request); max_batch_bytes caps the batch's A and B matrices
+        (complex64): large meshes, whose assembly is a small part of their
+        solve, go one request per launch instead of holding one huge block."""
         if solver not in ("gmres", "lu"):
             raise ValueError("solver must be 'gmres' or 'lu'")
         self.solver = solver
@@ -127,6 +133,7 @@ class Server:
         self.maxiter = None if maxiter is None else int(maxiter)
         self.refine = int(refine)
         self.max_batch = max(1, int(max_batch))
+        self.max_batch_bytes = float(max_batch_bytes)
         import torch
         self.torch = torch
         self.meshes = {}
@@ -171,10 +178,11 @@ class Server:
 
     def _launch(self, reqs, stream):
         """One batch of requests on one mesh: the Burton-Miller matrices of
-        all their wavenumbers in one sweep launch, then per request the
-        right-hand side and solve.  Returns per request (event after its
-        solve, its buffers with the solution last, device flag "the solution
-        is finite", error message or "").  An exception fails the batch."""
+        all their wavenumbers in one sweep launch, the right-hand sides, then
+        the solves (GMRES: the batch's systems in lockstep; LU: one after the
+        other).  Returns per request (event after its solve, its buffers
+        with the solution last, device flag "the solution is finite", error
+        message or "").  An exception fails the batch."""
         torch = self.torch
         key = reqs[0].mesh
         mesh = self.meshes[key]
@@ -188,34 +196,44 @@ class Server:
             B = torch.empty((len(reqs), n, n), dtype=torch.complex64, device="cuda").transpose(1, 2)
             # (the engine serialises calls on one mesh itself)
             nbem.assemble_sweep(mesh, ks, A, B, pol=self._policy(mesh), launch=self.launch[key], stream=stream)
-            for g, req in enumerate(reqs):
+            qs = []
+            for req in reqs:
                 d = torch.tensor(req.direction, dtype=torch.float64, device="cuda")
                 phase = -req.k * (c @ d)
                 pinc = torch.polar(torch.ones_like(phase), phase)
-                q = (1j * req.k) * (nu @ d) * pinc
-                err = ""
-                if self.solver == "gmres":
-                    rhs = nbem.gemv(B[g], q)
-                    res = solve.gmres(A[g], rhs, tol=self.tol, restart=self.restart, maxiter=self.maxiter)
-                    if not res.converged:
-                        err = ("GMRES did not reach the relative residual %g (%g after %d iterations)"
-                               % (self.tol, res.residual, res.iterations))
-                    p = res.x
-                    keep = (A, B, rhs, p)
+                qs.append((1j * req.k) * (nu @ d) * pinc)
+            if self.solver == "gmres":
+                # right-hand sides B q, then the batch's systems in lockstep
+                # (one host wait per Arnoldi step for the whole batch)
+                rhs = torch.stack([nbem.gemv(B[g], q) for g, q in enumerate(qs)])
+                if len(reqs) > 1:
+                    sols = solve.gmres_batched([A[g] for g in range(len(reqs))], rhs, tol=self.tol,
+                                               restart=self.restart, maxiter=self.maxiter)
                 else:
-                    rhs = (B[g] @ q.to(torch.complex64)).unsqueeze(1)
-                    LU, piv, _ = torch.linalg.lu_factor_ex(A[g])
-                    p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
-                    if self.refine > 0:
-                        p = _refine(A[g], LU, piv, rhs.squeeze(1), p, self.refine)
-                    keep = (A, B, LU, piv, rhs, p)
-                # NaN or Inf anywhere in the matrices or the right-hand side
-                # reaches the solution: one scan of the solution catches it
+                    sols = [solve.gmres(A[0], rhs[0], tol=self.tol, restart=self.restart, maxiter=self.maxiter)]
+                for res in sols:
+                    err = "" if res.converged else (
+                        "GMRES did not reach the relative residual %g (%g after %d iterations)"
+                        % (self.tol, res.residual, res.iterations))
+                    p = res.x
+                    # NaN or Inf anywhere in the matrices or the right-hand
+                    # side reaches the solution: one scan of it catches it
+                    finite = torch.isfinite(torch.view_as_real(p)).all()
+                    done = torch.cuda.Event()
+                    done.record(stream)
+                    # keep the buffers alive until the event has passed
+                    out.append((done, (A, B, rhs, p), finite, err))
+                return out
+            for g, q in enumerate(qs):
+                rhs = (B[g] @ q.to(torch.complex64)).unsqueeze(1)
+                LU, piv, _ = torch.linalg.lu_factor_ex(A[g])
+                p = torch.linalg.lu_solve(LU, piv, rhs).squeeze(1)
+                if self.refine > 0:
+                    p = _refine(A[g], LU, piv, rhs.squeeze(1), p, self.refine)
                 finite = torch.isfinite(torch.view_as_real(p)).all()
                 ev = torch.cuda.Event()
                 ev.record(stream)
-                # keep the buffers alive until the event has passed
-                out.append((ev, keep, finite, err))
+                out.append((ev, (A, B, LU, piv, rhs, p), finite, ""))
         return out
 
     def run(self, requests, realtime: bool = True):
@@ -257,8 +275,10 @@ class Server:
                     if in_use + fp > self.budget:
                         break
                     batch = [head]
+                    n_head = self.meshes[head.mesh].n
+                    cap = max(1, min(self.max_batch, int(self.max_batch_bytes // (16.0 * n_head * n_head))))
                     for req in list(pending)[1:]:
-                        if len(batch) >= self.max_batch or (realtime and req.arrival > now()):
+                        if len(batch) >= cap or (realtime and req.arrival > now()):
                             break
                         if req.mesh == head.mesh and in_use + fp * (len(batch) + 1) <= self.budget:
                             batch.append(req)

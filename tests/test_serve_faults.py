@@ -33,6 +33,7 @@ class _HostServer(serve.Server):
         self.solver = "gmres"
         self.restart = 3000
         self.max_batch = max_batch
+        self.max_batch_bytes = 8 * 2 ** 30
         self.meshes = {"a": types.SimpleNamespace(n=10, nodes_per_element=3),
                        "b": types.SimpleNamespace(n=12, nodes_per_element=3)}
         self.slots = [None] * slots

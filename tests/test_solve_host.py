@@ -60,3 +60,32 @@ def test_gmres_burton_miller_matches_scipy(orc):
     err = np.linalg.norm(r.x.numpy() - xe) / np.linalg.norm(xe)
     print("iterations %d (scipy %d), error %.2e" % (r.iterations, its[0], err))
     assert r.converged and abs(r.iterations - its[0]) <= 1 and err < 1e-7
+
+
+def test_gmres_batched_matches_single():
+    """The lockstep batch: every system's solution and iteration count equal
+    gmres() alone on it (to rounding: the batch orthogonalises with batched
+    products); a zero right-hand side, a system that needs a restart (short
+    restart length) and one stopped by maxiter inside the same batch."""
+    from nihu_amd.solve import gmres_batched
+    rng = np.random.default_rng(3)
+    n = 120
+    mats, rhs = [], []
+    for q in range(5):
+        A = np.eye(n) * (3 + q) + (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(n)
+        mats.append(torch.from_numpy(A))
+        rhs.append(np.zeros(n, dtype=np.complex128) if q == 2 else rng.normal(size=n) + 1j * rng.normal(size=n))
+    bs = torch.from_numpy(np.stack(rhs))
+    mv = [lambda v, A=A: A @ v for A in mats]
+    for restart, maxiter in ((200, None), (6, None), (200, 4)):
+        out = gmres_batched(mats, bs, tol=1e-11, restart=restart, maxiter=maxiter, matvecs=mv)
+        for q in range(5):
+            one = gmres(mats[q], bs[q], tol=1e-11, restart=restart, maxiter=maxiter, matvec=mv[q])
+            assert out[q].converged == one.converged and out[q].iterations == one.iterations, (restart, q)
+            d = (out[q].x - one.x).abs().max().item()
+            assert d <= 1e-12 * max(1.0, one.x.abs().max().item()), (restart, q, d)
+            if maxiter is None:
+                A = mats[q].numpy()
+                x = out[q].x.numpy()
+                assert np.linalg.norm(A @ x - rhs[q]) <= 1e-10 * max(np.linalg.norm(rhs[q]), 1e-300) or q == 2
+        assert not out[2].x.abs().any() and out[2].converged
