@@ -22,8 +22,10 @@ line carries two more sections after the timed headline steps: `sweep`, the
 single-wavenumber assemblies (ms per sweep, p50/p99 sweep latency, errors
 against the oracle), and `serve`, a Poisson stream of assembly-plus-solve
 requests served with and without batching (p50/p99 latency, solves/s) and a
-rate sweep up to the first rate the server does not keep up with
-(--no-extras skips both).
+rate sweep up to the first rate the server does not keep up with; then short
+runs (5 timed steps, 32 rows or node groups against the oracle) of the Galerkin workload
+(config 3), the quadrilateral and the mixed-element workloads, each with its
+roofline: `galerkin`, `quad`, `mixed` (--no-extras skips all of them).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 20]
 """
@@ -166,7 +168,8 @@ def main():
                          "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table)")
     ap.add_argument("--requests", type=int, default=24, help="serve: requests per stream")
     ap.add_argument("--no-extras", action="store_true",
-                    help="headline only: skip the sweep and serving sections of the default line")
+                    help="headline only: skip the sweep, serving, Galerkin, quadrilateral and mixed-mesh "
+                         "sections of the default line")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: mean arrivals per second")
     ap.add_argument("--refine", type=int, default=0,
                     help="serve: steps of FP64 iterative refinement of each complex64 LU solve")
@@ -185,11 +188,14 @@ def main():
     if args.workload == "serve":
         return run_serve(args)
     if args.workload == "galerkin":
-        return run_galerkin(args)
+        print(json.dumps(run_galerkin(args)))
+        return
     if args.workload == "quad":
-        return run_quad(args)
+        print(json.dumps(run_quad(args)))
+        return
     if args.workload == "mixed":
-        return run_mixed(args)
+        print(json.dumps(run_mixed(args)))
+        return
 
     import numpy as np
     import torch
@@ -329,6 +335,15 @@ def main():
         torch.cuda.empty_cache()
         extras["serve"] = serve_summary(n_req=args.requests, rate=args.rate, refine=args.refine)
         torch.cuda.empty_cache()
+        # the other configurations at their full size, short: config 3
+        # (Galerkin: the test elements of 32 sampled nodes against the
+        # oracle), the quadrilateral and the mixed-element workloads (32 rows),
+        # each with its roofline
+        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("mixed", run_mixed, 32)):
+            rec = fn(argparse.Namespace(steps=5, warmup=5, cpu_rows=rows, profile_copy_bytes=0, level=5))
+            extras[key] = {k: rec.get(k) for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
+                                                   "cpu_baseline", "accuracy", "pass_ms")}
+            torch.cuda.empty_cache()
 
     out = {
         "metric": "matrix entries/s, Helmholtz 3D coll. 20480 tris; p50/p99 sweep latency",
@@ -477,7 +492,7 @@ def run_galerkin(args):
                    "pairs": {"regular": n_reg, "singular": n_sing}},
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
-    print(json.dumps(out))
+    return out
 
 
 def run_quad(args):
@@ -571,7 +586,7 @@ def run_quad(args):
                    "pairs": {"regular": n_reg, "near": n_near, "diagonal": n}},
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
-    print(json.dumps(out))
+    return out
 
 
 def run_mixed(args):
@@ -670,7 +685,7 @@ def run_mixed(args):
                    "policy": "NiHu (distance-table degrees, near-singular d/R < 1.5)"},
         "roofline": None, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
-    print(json.dumps(out))
+    return out
 
 
 def sweep_summary(steps=5, warmup=2, log_prefix="sweep", policy="config"):
