@@ -83,6 +83,13 @@ static inline double sinc(double x)
 		return std::sin(x) / x;
 	return 1. - x * x / 6. * (1. - x * x / 20.);
 }
+// the same for a complex argument (complex wavenumbers)
+static inline std::complex<double> sinc(std::complex<double> x)
+{
+	if (std::abs(x) > 1e-3)
+		return std::sin(x) / x;
+	return 1. - x * x / 6. * (1. - x * x / 20.);
+}
 
 // ---------------------------------------------------------------------------
 // Quadrature
@@ -327,8 +334,11 @@ static std::vector<Elem> make_mesh(const double *xyz, int n_nodes, const int *tr
 // nihu/library/laplace_kernel.hpp:133-171,
 // nihu/library/normal_derivative_kernel.hpp:114-257.  r = y - x.
 // family 0 = Laplace, 1 = Helmholtz.  Laplace results have zero imaginary part.
+// The wavenumber type K is double or complex (NiHu's WaveNumber template
+// parameter, helmholtz_kernel.hpp:103); the formulas are the same.
 // ---------------------------------------------------------------------------
-static cplx kernel_eval(int family, int nx, int ny, double k, V3 x, V3 y, V3 nxv, V3 nyv)
+template <class K>
+static cplx kernel_eval(int family, int nx, int ny, K k, V3 x, V3 y, V3 nxv, V3 nyv)
 {
 	V3 rvec = y - x;
 	double r = norm(rvec);
@@ -433,7 +443,8 @@ static int regular_degree(Policy const &p, int field_complexity, int nx, int ny,
 // Regular collocational integral: sum_q K(x, y_q) * (J * w_q)
 // nihu/core/double_integral.hpp:445-467 (eval_on_accelerator), :543-589
 // ---------------------------------------------------------------------------
-static cplx regular_integral(int family, int nx, int ny, double k, V3 x, V3 nxv, Elem const &trial, int degree)
+template <class K>
+static cplx regular_integral(int family, int nx, int ny, K k, V3 x, V3 nxv, Elem const &trial, int degree)
 {
 	if (degree > 14)
 		throw std::out_of_range("Too high quadrature degree selected for collocational integration");
@@ -469,7 +480,8 @@ static void plane_element_helper(Elem const &e, V3 x0, double r[3], double theta
 	}
 }
 
-static cplx singular_integral(int family, int nx, int ny, double k, Elem const &e)
+template <class K>
+static cplx singular_integral(int family, int nx, int ny, K k, Elem const &e)
 {
 	if (nx + ny == 1) return 0.0;
 	V3 x0 = e.c;
@@ -611,7 +623,8 @@ static double laplace_near_static(int nx, int ny, Elem const &e, V3 x0_in, V3 nx
 	return planar_near_static(nx, ny, e.X, 3, x0_in, nx_in, line_degree);
 }
 
-static cplx near_integral(int family, int nx, int ny, double k, Elem const &e, V3 x0, V3 nxv, int line_degree = 0)
+template <class K>
+static cplx near_integral(int family, int nx, int ny, K k, Elem const &e, V3 x0, V3 nxv, int line_degree = 0)
 {
 	double res_static = laplace_near_static(nx, ny, e, x0, nxv, line_degree);
 	if (family == 0) return res_static;
@@ -630,7 +643,8 @@ static cplx near_integral(int family, int nx, int ny, double k, Elem const &e, V
 // nihu/core/double_integral.hpp:599-673 (WITH / WITHOUT singularity check),
 // nihu/core/element_match.hpp:84-112 (collocation: face match = same id)
 // ---------------------------------------------------------------------------
-static cplx pair_integral(int family, int nx, int ny, double k, Policy const &p, Elem const &test, Elem const &trial,
+template <class K>
+static cplx pair_integral(int family, int nx, int ny, K k, Policy const &p, Elem const &test, Elem const &trial,
 	bool same_mesh)
 {
 	if (same_mesh && test.id == trial.id)
@@ -1364,6 +1378,17 @@ int orc_kernel(int family, int nx, int ny, double k, const double *x, const doub
 	ORC_CATCH
 }
 
+// complex wavenumber k_re + i k_im
+int orc_kernel_ck(int family, int nx, int ny, double k_re, double k_im, const double *x, const double *y,
+	const double *nxv, const double *nyv, double *out)
+{
+	ORC_TRY
+	cplx v = kernel_eval(family, nx, ny, cplx(k_re, k_im), {x[0], x[1], x[2]}, {y[0], y[1], y[2]},
+		{nxv[0], nxv[1], nxv[2]}, {nyv[0], nyv[1], nyv[2]});
+	out[0] = v.real(); out[1] = v.imag();
+	ORC_CATCH
+}
+
 // kind: 0 regular (Dunavant degree), 1 near-singular, 2 singular (x0 = centre).
 // For kind 1, degree > 0 replaces NiHu's line quadrature order of the static
 // part (used by tests to converge the formula to reference golden values).
@@ -1374,6 +1399,23 @@ int orc_elem_integral(int kind, int family, int nx, int ny, double k, const doub
 	Elem e = make_elem({coords9[0], coords9[1], coords9[2]}, {coords9[3], coords9[4], coords9[5]},
 		{coords9[6], coords9[7], coords9[8]}, 0);
 	V3 x = {x0[0], x0[1], x0[2]}, n = {nx0[0], nx0[1], nx0[2]};
+	cplx v;
+	if (kind == 0) v = regular_integral(family, nx, ny, k, x, n, e, degree);
+	else if (kind == 1) v = near_integral(family, nx, ny, k, e, x, n, degree);
+	else v = singular_integral(family, nx, ny, k, e);
+	out[0] = v.real(); out[1] = v.imag();
+	ORC_CATCH
+}
+
+// complex wavenumber k_re + i k_im
+int orc_elem_integral_ck(int kind, int family, int nx, int ny, double k_re, double k_im, const double *coords9,
+	const double *x0, const double *nx0, int degree, double *out)
+{
+	ORC_TRY
+	Elem e = make_elem({coords9[0], coords9[1], coords9[2]}, {coords9[3], coords9[4], coords9[5]},
+		{coords9[6], coords9[7], coords9[8]}, 0);
+	V3 x = {x0[0], x0[1], x0[2]}, n = {nx0[0], nx0[1], nx0[2]};
+	const cplx k(k_re, k_im);
 	cplx v;
 	if (kind == 0) v = regular_integral(family, nx, ny, k, x, n, e, degree);
 	else if (kind == 1) v = near_integral(family, nx, ny, k, e, x, n, degree);
@@ -1416,9 +1458,12 @@ int orc_classify(const double *txyz, int tn, const int *ttri, int tnt,
 // nihu/util/matrix_block.hpp:59-65); the scalar multiplies each pair result
 // (nihu/core/integral_operator.hpp:159-167).  OpenMP over rows is the oracle's
 // own extension; with n_threads = 1 it is NiHu's serial loop.
-int orc_assemble(const double *txyz, int tn, const int *ttri, int tnt,
+} // extern "C"
+
+template <class K>
+static int assemble_impl(const double *txyz, int tn, const int *ttri, int tnt,
 	const double *sxyz, int sn, const int *stri, int snt, int same_mesh,
-	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	int family, int nx, int ny, K k, double alpha_re, double alpha_im,
 	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
 {
 	ORC_TRY
@@ -1446,6 +1491,28 @@ int orc_assemble(const double *txyz, int tn, const int *ttri, int tnt,
 	}
 	if (!err.empty()) throw std::out_of_range(err);
 	ORC_CATCH
+}
+
+
+extern "C" {
+
+int orc_assemble(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh,
+	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	return assemble_impl(txyz, tn, ttri, tnt, sxyz, sn, stri, snt, same_mesh, family, nx, ny, k, alpha_re, alpha_im,
+		pol, rows, n_rows, out, ld, accumulate, n_threads);
+}
+
+// the same with a complex wavenumber k_re + i k_im
+int orc_assemble_ck(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh,
+	int family, int nx, int ny, double k_re, double k_im, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	return assemble_impl(txyz, tn, ttri, tnt, sxyz, sn, stri, snt, same_mesh, family, nx, ny, cplx(k_re, k_im),
+		alpha_re, alpha_im, pol, rows, n_rows, out, ld, accumulate, n_threads);
 }
 
 // identity operator: result(i,i) += alpha on the same mesh (local branch of
@@ -1483,6 +1550,32 @@ int orc_assemble_bm(const double *xyz, int nn, const int *tri, int nt, double k,
 	if ((s = orc_assemble(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, 1, 1, k, al.real(), al.imag(), pol, rows, n_rows, A, ld, 1, n_threads))) return s;
 	orc_identity(nt, -.5, 0.0, rows, n_rows, A, ld);
 	if ((s = orc_assemble(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, 0, 1, k, 1.0, 0.0, pol, rows, n_rows, A, ld, 1, n_threads))) return s;
+	return 0;
+}
+
+// the same with a complex wavenumber k_re + i k_im (alpha as given)
+int orc_assemble_bm_ck(const double *xyz, int nn, const int *tri, int nt, double k_re, double k_im, double alpha_re,
+	double alpha_im, const orc_policy *pol, const int *rows, int n_rows, double *A, double *B, int64_t ld,
+	int n_threads)
+{
+	int nr = rows ? n_rows : nt;
+	for (int j = 0; j < nt; ++j)
+		for (int r = 0; r < nr; ++r) {
+			double *a = A + 2 * ((int64_t)j * ld + r), *b = B + 2 * ((int64_t)j * ld + r);
+			a[0] = a[1] = b[0] = b[1] = 0.0;
+		}
+	cplx al(alpha_re, alpha_im);
+	int s;
+	auto op = [&](int nx, int ny, cplx sc, double *out) {
+		return orc_assemble_ck(xyz, nn, tri, nt, xyz, nn, tri, nt, 1, 1, nx, ny, k_re, k_im, sc.real(), sc.imag(), pol,
+			rows, n_rows, out, ld, 1, n_threads);
+	};
+	if ((s = op(0, 0, 1.0, B))) return s;
+	if ((s = op(1, 0, al, B))) return s;
+	orc_identity(nt, (al / 2.).real(), (al / 2.).imag(), rows, n_rows, B, ld);
+	if ((s = op(1, 1, al, A))) return s;
+	orc_identity(nt, -.5, 0.0, rows, n_rows, A, ld);
+	if ((s = op(0, 1, 1.0, A))) return s;
 	return 0;
 }
 

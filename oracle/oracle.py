@@ -87,6 +87,14 @@ def _declare(l):
         l.orc_geometry.argtypes = [D, C.c_int, I, C.c_int, D, D, D, D, D]
         l.orc_kernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, D, D]
         l.orc_elem_integral.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, C.c_int, D]
+        l.orc_kernel_ck.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D, D]
+        l.orc_elem_integral_ck.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D,
+                                           C.c_int, D]
+        l.orc_assemble_ck.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_assemble_bm_ck.argtypes = [D, C.c_int, I, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         P, I, C.c_int, D, D, C.c_int64, C.c_int]
         l.orc_classify.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int, C.c_int,
                                    P, I, C.c_int, C.POINTER(C.c_int8)]
         l.orc_assemble.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int,
@@ -156,10 +164,18 @@ def geometry(nodes, tris):
     return dict(center=c, normal=nu, jac=J, R=R, diam=diam)
 
 
+def _is_complex(k):
+    return isinstance(k, (complex, np.complexfloating))
+
+
 def kernel(family, nx, ny, k, x, y, nxv, nyv):
+    """k real or complex (a complex wavenumber)."""
     out = np.zeros(2)
     arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, nxv, nyv)]
-    _check(lib().orc_kernel(family, nx, ny, k, *[_d(a) for a in arr], _d(out)))
+    if _is_complex(k):
+        _check(lib().orc_kernel_ck(family, nx, ny, k.real, k.imag, *[_d(a) for a in arr], _d(out)))
+    else:
+        _check(lib().orc_kernel(family, nx, ny, k, *[_d(a) for a in arr], _d(out)))
     return complex(out[0], out[1])
 
 
@@ -170,7 +186,11 @@ def elem_integral(kind, family, nx, ny, k, coords, x0=(0, 0, 0), nx0=(0, 0, 1), 
     cs = np.ascontiguousarray(coords, dtype=np.float64).reshape(9)
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
     nx0 = np.ascontiguousarray(nx0, dtype=np.float64)
-    _check(lib().orc_elem_integral(kind, family, nx, ny, k, _d(cs), _d(x0), _d(nx0), degree, _d(out)))
+    if _is_complex(k):
+        _check(lib().orc_elem_integral_ck(kind, family, nx, ny, k.real, k.imag, _d(cs), _d(x0), _d(nx0), degree,
+                                          _d(out)))
+    else:
+        _check(lib().orc_elem_integral(kind, family, nx, ny, k, _d(cs), _d(x0), _d(nx0), degree, _d(out)))
     return complex(out[0], out[1])
 
 
@@ -202,7 +222,16 @@ def assemble(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, rows=None,
         out = np.zeros((nr, len(s_tris)), dtype=np.complex128, order="F")
         accumulate = False
     alpha = complex(alpha)
-    if t_tris.shape[1] == 3 and s_tris.shape[1] == 3:
+    if _is_complex(k):
+        # complex wavenumber: constant triangles only
+        if t_tris.shape[1] != 3 or s_tris.shape[1] != 3:
+            raise ValueError("complex wavenumbers: triangle meshes only")
+        _check(lib().orc_assemble_ck(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                                     _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
+                                     family, nx, ny, k.real, k.imag, alpha.real, alpha.imag, C.byref(pol), _i(rows_a),
+                                     nr, out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0], int(accumulate),
+                                     threads))
+    elif t_tris.shape[1] == 3 and s_tris.shape[1] == 3:
         _check(lib().orc_assemble(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
                                   _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
                                   family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol), _i(rows_a), nr,
@@ -226,7 +255,14 @@ def assemble_bm(nodes, tris, k, alpha=None, pol=None, rows=None, threads=0):
     nr = len(tris) if rows_a is None else len(rows_a)
     A = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
     B = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
-    if tris.shape[1] == 3:
+    if _is_complex(k):
+        if tris.shape[1] != 3:
+            raise ValueError("complex wavenumbers: triangle meshes only")
+        _check(lib().orc_assemble_bm_ck(_d(nodes), len(nodes), _i(tris), len(tris), k.real, k.imag, alpha.real,
+                                        alpha.imag, C.byref(pol), _i(rows_a), nr,
+                                        A.ctypes.data_as(C.POINTER(C.c_double)),
+                                        B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
+    elif tris.shape[1] == 3:
         _check(lib().orc_assemble_bm(_d(nodes), len(nodes), _i(tris), len(tris), k, alpha.real, alpha.imag,
                                      C.byref(pol), _i(rows_a), nr, A.ctypes.data_as(C.POINTER(C.c_double)),
                                      B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
