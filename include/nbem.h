@@ -76,12 +76,16 @@ enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1, NB_GALERKIN_LINEAR = 2 
 
 typedef struct nb_mesh nb_mesh;
 
-/* kernel: family, normal-derivative orders (nx, ny) in {0,1}^2, real wavenumber */
+/* kernel: family, normal-derivative orders (nx, ny) in {0,1}^2, wavenumber
+ * k + i k_im (NiHu's helmholtz_kernel<space, WaveNumber> with a real or
+ * complex WaveNumber, library/helmholtz_kernel.hpp:103; k > 0; a nonzero
+ * k_im is taken by collocation on triangle meshes, else NB_ERR_UNSUPPORTED) */
 typedef struct {
 	int32_t family;
 	int32_t nx;
 	int32_t ny;
 	double k;
+	double k_im;     /* 0 when the struct is zero-initialised */
 } nb_kernel;
 
 /* scalar multiplier of scaled_integral_operator (nihu/core/integral_operator.hpp:131-194) */
@@ -177,6 +181,11 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
  * with G, H, Ht, D the Helmholtz SLP, DLP, DLPt, HSP operators. */
 int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_policy *policy,
                    const nb_out *A, const nb_out *B, void *stream);
+/* The same for a complex wavenumber k.re + i k.im (k.re > 0): the
+ * composition of helmholtz_3d_coll.cpp:63-96 with a complex WaveNumber,
+ * triangle meshes only (else NB_ERR_UNSUPPORTED). */
+int nb_assemble_bm_ck(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const nb_policy *policy,
+                      const nb_out *A, const nb_out *B, void *stream);
 
 /* Burton-Miller matrices for n_k wavenumbers on one mesh in one persistent
  * launch (alpha = -i/k each).  A and B are device c64 arrays holding n_k

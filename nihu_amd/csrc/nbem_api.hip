@@ -570,6 +570,7 @@ struct Job {
 	int formalism;
 	int same, family, km, mode;
 	double k;
+	double ki = 0.0;      // imaginary part of a complex wavenumber
 	nb_scalar alpha;
 	nb_policy pol;
 	int tile_cols;
@@ -585,6 +586,7 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 {
 	if (jb.pol.precision != NB_PREC_MIXED)
 		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
+	if (jb.ki != 0.0) return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
 	nb_mesh *trial = jb.trial;
 	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
 	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
@@ -770,6 +772,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	int s;
 	if (jb.pol.precision != NB_PREC_MIXED)
 		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
+	if (jb.ki != 0.0) return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
 	if ((s = check_quad_policy(jb.pol))) return s;
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	int fixed_n = 0;
@@ -854,6 +857,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	na.sJ = (const double *)trial->d_J.get();
 	na.snu = (const double *)trial->d_nu64.get();
 	na.k = jb.k;
+	na.ki = jb.ki;
 	na.alpha = make_double2(jb.alpha.re, jb.alpha.im);
 	na.out0 = out0; na.out1 = out1; na.ld = ld; na.dtype = dtype; na.accumulate = accumulate;
 	na.rules = rules;
@@ -957,6 +961,8 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	a.k = (float)jb.k;
 	a.k_rev = jb.k / two_pi;
 	a.k_revf = (float)a.k_rev;
+	a.ki = (float)jb.ki;
+	const bool ck = jb.ki != 0.0 && jb.family == NB_HELMHOLTZ;
 	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
 	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
 	const long long col_tiles = (trial->n + a.tile_cols - 1) / a.tile_cols;
@@ -987,10 +993,13 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		for (int d = 0; d < NSLOT; ++d) { b.qt[d] = (const double *)trial->d_qt64[d].get(); b.qn[d] = trial->qn[d]; }
 		b.fixed_degree = a.fixed_degree;
 		b.k = jb.k;
+		b.ki = jb.ki;
 		b.alpha = make_double2(jb.alpha.re, jb.alpha.im);
 		b.out0 = out0; b.out1 = out1; b.ld = ld; b.dtype = dtype; b.accumulate = accumulate;
 		const int pe = prof_begin(0, st);
-#define NB_R64(F, K, M) hipLaunchKernelGGL((regular64_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, b)
+#define NB_R64(F, K, M) do { \
+		if (F == HELM && ck) hipLaunchKernelGGL((regular64_kernel<F, K, M, (F == HELM)>), dim3(grid), dim3(256), 0, st, b); \
+		else hipLaunchKernelGGL((regular64_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, b); } while (0)
 		if (jb.mode == OUT_BM) NB_R64(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
@@ -1016,7 +1025,8 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		const bool fix7 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 7;
 		const bool mapped = jb.rmap || jb.cmap;
 #define NB_RG(F, K, M, Q) do { \
-		if (mapped) hipLaunchKernelGGL((regular_kernel<F, K, M, Q, true>), dim3(grid), dim3(256), 0, st, a); \
+		if (F == HELM && ck) hipLaunchKernelGGL((regular_kernel<F, K, M, Q, false, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+		else if (mapped) hipLaunchKernelGGL((regular_kernel<F, K, M, Q, true>), dim3(grid), dim3(256), 0, st, a); \
 		else hipLaunchKernelGGL((regular_kernel<F, K, M, Q, false>), dim3(grid), dim3(256), 0, st, a); } while (0)
 		if (jb.mode == OUT_BM) {
 			if (fix7) NB_RG(HELM, ALL4, OUT_BM, 7);
@@ -1055,6 +1065,7 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 	na.sJ = (const double *)trial->d_J.get();
 	na.snu = (const double *)trial->d_nu64.get();
 	na.k = jb.k;
+	na.ki = jb.ki;
 	na.alpha = make_double2(jb.alpha.re, jb.alpha.im);
 	na.out0 = out0; na.out1 = out1; na.ld = ld; na.dtype = dtype; na.accumulate = accumulate;
 	na.rules = rules;
@@ -1069,7 +1080,10 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		const int pe = prof_begin(diag ? 2 : 1, side);
 		using namespace nbk;
 #define NB_FP64(F, K, M) do { \
-		if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true>), dim3(g), dim3(64), 0, side, na); \
+		if (F == HELM && ck) { \
+			if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true, (F == HELM)>), dim3(g), dim3(64), 0, side, na); \
+			else hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, false, (F == HELM)>), dim3(g), dim3(64), 0, side, na); \
+		} else if (diag) hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, true>), dim3(g), dim3(64), 0, side, na); \
 		else hipLaunchKernelGGL((fp64_pass_kernel<F, K, M, false>), dim3(g), dim3(64), 0, side, na); } while (0)
 		if (jb.mode == OUT_BM) NB_FP64(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
@@ -1161,6 +1175,9 @@ int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 	const int nt = mesh_parts(jb.test, tp, tid), ns = mesh_parts(jb.trial, sp, sid);
 	int s;
 	// refuse before any block has run
+	if (jb.ki != 0.0 || jb.pol.precision != NB_PREC_MIXED)
+		return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers and the FP64 precision policy are implemented for "
+			"collocation on meshes of triangles only");
 	for (int b = 0; b < ns; ++b)
 		if (sp[b]->npe == 4 && (s = check_quad_policy(jb.pol))) return s;
 	// a test mesh with both types: the regular pairs of a trial part against
@@ -1608,6 +1625,8 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 	int s;
 	if (test->npe != 3 || trial->npe != 3)
 		return fail(NB_ERR_UNSUPPORTED, "the Galerkin formalism is implemented for triangles only");
+	if (!ident && kernel.family == NB_HELMHOLTZ && kernel.k_im != 0.0)
+		return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
 	if (!ident) {
 		if (kernel.family != NB_LAPLACE && kernel.family != NB_HELMHOLTZ) return fail(NB_ERR_ARG, "unknown kernel family");
 		if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
@@ -1833,6 +1852,8 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 	if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
 		return fail(NB_ERR_UNSUPPORTED, "normal derivative orders must be 0 or 1");
 	if (kernel.family == NB_HELMHOLTZ && !(kernel.k > 0.0)) return fail(NB_ERR_ARG, "wavenumber must be positive");
+	if (kernel.family == NB_HELMHOLTZ && !std::isfinite(kernel.k_im))
+		return fail(NB_ERR_ARG, "imaginary part of the wavenumber must be finite");
 	if ((s = check_policy(policy))) return s;
 	bool cplx = kernel.family == NB_HELMHOLTZ || alpha.im != 0.0;
 	if ((s = check_out(out, test->n, trial->n, cplx))) return s;
@@ -1842,6 +1863,7 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 	jb.test = test; jb.trial = tr; jb.formalism = NB_COLLOCATION; jb.same = same_mesh ? 1 : 0; jb.family = kernel.family;
 	jb.km = kernel_mask(kernel.nx, kernel.ny); jb.mode = nbk::OUT_SINGLE;
 	jb.k = kernel.family == NB_HELMHOLTZ ? kernel.k : 0.0;
+	jb.ki = kernel.family == NB_HELMHOLTZ ? kernel.k_im : 0.0;
 	jb.alpha = alpha; jb.pol = *policy;
 	{
 		std::lock_guard<std::mutex> lc(g_cfg_mu);
@@ -1865,6 +1887,30 @@ int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_poli
 	jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4;
 	jb.mode = nbk::OUT_BM;
 	jb.k = k; jb.alpha = alpha; jb.pol = *policy;
+	{
+		std::lock_guard<std::mutex> lc(g_cfg_mu);
+		jb.tile_cols = g_cfg.tile_cols;
+	}
+	return run_job(jb, A, B, (hipStream_t)stream);
+}
+
+int nb_assemble_bm_ck(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const nb_policy *policy,
+	const nb_out *A, const nb_out *B, void *stream)
+{
+	if (!mesh) return fail(NB_ERR_ARG, "null mesh");
+	if (!(k.re > 0.0) || !std::isfinite(k.im)) return fail(NB_ERR_ARG, "wavenumber must have a positive real part");
+	if (mesh->npe != 3)
+		return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
+	int s;
+	if ((s = current_device_matches(mesh))) return s;
+	if ((s = check_policy(policy))) return s;
+	if ((s = check_out(A, mesh->n, mesh->n, true)) || (s = check_out(B, mesh->n, mesh->n, true))) return s;
+	nb_mesh *m = const_cast<nb_mesh *>(mesh);
+	std::lock_guard<std::mutex> lk(m->mu);
+	Job jb;
+	jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4;
+	jb.mode = nbk::OUT_BM;
+	jb.k = k.re; jb.ki = k.im; jb.alpha = alpha; jb.pol = *policy;
 	{
 		std::lock_guard<std::mutex> lc(g_cfg_mu);
 		jb.tile_cols = g_cfg.tile_cols;

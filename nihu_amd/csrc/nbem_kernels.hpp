@@ -67,6 +67,7 @@ struct RegArgs {
 	int field_complexity;        // added to the table degree (0 for constant fields)
 	float k, k_revf;             // wavenumber, k / (2 pi)
 	double k_rev;
+	float ki;                    // imaginary part of a complex wavenumber (kernels with CK)
 	float2 alpha;
 	void *out0, *out1;
 	long long ld;
@@ -256,6 +257,7 @@ struct PairCtx {
 	float d2f, df, t0;         // |c_j - c_i|^2, |c_j - c_i|, fract(k |c_j - c_i| / 2pi) in [-1/2, 1/2]
 	float nxx, nxy, nxz;       // test unit normal
 	float b0;                  // (c_j - c_i) . n_x
+	float ad;                  // complex wavenumber k + i ki: e^{ki d} (else unused)
 };
 
 // Per-pair accumulators of the regular pass (complex):
@@ -282,9 +284,12 @@ struct PairCtx {
 // b = r.n_x.  With a derivative kernel in MASK, G = w/r^3 e^{-i th} comes
 // first, E = G r^2 is folded into the SLP sum's FMA and th G = k (w/r^2) e^{-i th}
 // needs neither r nor th.
-template <int FAM, int MASK>
+// CK: complex wavenumber k + i ki.  The caller folds the attenuation e^{ki r}
+// into w, and Q = G (1 + i(k + i ki) r) gains -ki r G = -ki h e:
+// Re Q = Gr + k h sn - ki h c, Im Q = Gi + k h c + ki h sn.
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void add_point(float c, float sn, float k, float w, float rinv, float r2, float b,
-	float2 (&s)[6])
+	float2 (&s)[6], float ki = 0.0f)
 {
 	// Laplace (c = 1, sn = 0): the imaginary sums stay zero
 	constexpr bool IM = FAM == HELM;
@@ -300,8 +305,13 @@ __device__ __forceinline__ void add_point(float c, float sn, float k, float w, f
 	const float Gr = g3 * c, Gi = -g3 * sn;
 	// Q = G (1 + i th): Re = Gr - th Gi = Gr + k h sn, Im = Gi + th Gr = Gi + k h c
 	const float kh = k * h;
-	const float Qr = IM ? __builtin_fmaf(kh, sn, Gr) : Gr;
-	const float Qi = IM ? __builtin_fmaf(kh, c, Gi) : 0.0f;
+	float Qr = IM ? __builtin_fmaf(kh, sn, Gr) : Gr;
+	float Qi = IM ? __builtin_fmaf(kh, c, Gi) : 0.0f;
+	if (CK) {
+		const float kih = ki * h;
+		Qr = __builtin_fmaf(-kih, c, Qr);
+		Qi = __builtin_fmaf(kih, sn, Qi);
+	}
 	if (MASK & SLP) {
 		s[0].x = __builtin_fmaf(Gr, r2, s[0].x);
 		if (IM) s[0].y = __builtin_fmaf(Gi, r2, s[0].y);
@@ -324,9 +334,9 @@ __device__ __forceinline__ void add_point(float c, float sn, float k, float w, f
 }
 
 // NQ > 0: compile-time point count (fully unrolled).
-template <int FAM, int MASK, int NQ>
+template <int FAM, int MASK, int NQ, bool CK = false>
 __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt, const PairCtx &p,
-	float k_revf, float k, float2 (&s)[6])
+	float k_revf, float k, float2 (&s)[6], float ki = 0.0f)
 {
 	const int nq = NQ > 0 ? NQ : nq_rt;
 	const auto qf = uni((const float *)qp_in);
@@ -338,31 +348,34 @@ __device__ __forceinline__ void accumulate_points(const QPoint *qp_in, int nq_rt
 		const float u = __builtin_fmaf(dcx2, ox, __builtin_fmaf(dcy2, oy, __builtin_fmaf(dcz2, oz, oo)));
 		const float r2 = p.d2f + u;
 		const float rinv = __builtin_amdgcn_rsqf(r2);
-		float c = 1.0f, sn = 0.0f;
+		float c = 1.0f, sn = 0.0f, wa = w;
 		if (FAM == HELM) {
-			const float t = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(__builtin_fmaf(r2, rinv, p.df)), p.t0);
+			const float rd = u * __builtin_amdgcn_rcpf(__builtin_fmaf(r2, rinv, p.df));   // r - d
+			const float t = __builtin_fmaf(k_revf, rd, p.t0);
 			c = __builtin_amdgcn_cosf(t);
 			sn = __builtin_amdgcn_sinf(t);
+			// e^{ki r} = e^{ki d} 2^{ki log2(e) (r - d)}
+			if (CK) wa = w * (p.ad * __builtin_amdgcn_exp2f(ki * 1.4426950408889634f * rd));
 		}
 		float b = 0.0f;
 		if (MASK & (DLPT | HSP))
 			b = __builtin_fmaf(oz, p.nxz, __builtin_fmaf(oy, p.nxy, __builtin_fmaf(ox, p.nxx, p.b0)));
-		add_point<FAM, MASK>(c, sn, k, w, rinv, r2, b, s);
+		add_point<FAM, MASK, CK>(c, sn, k, wa, rinv, r2, b, s, ki);
 	}
 }
 
 // One-point (centroid) rule: y = c_j, so r = d, the phase is the pair's
 // FP64-reduced k d and b = (c_j - x).n_x; rinv_d = 1/d, w1 = 0.5 J / (4 pi).
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void accumulate_centroid(const PairCtx &p, float w1, float rinv_d, float k,
-	float2 (&s)[6])
+	float2 (&s)[6], float ki = 0.0f)
 {
 	float c = 1.0f, sn = 0.0f;
 	if (FAM == HELM) {
 		c = __builtin_amdgcn_cosf(p.t0);
 		sn = __builtin_amdgcn_sinf(p.t0);
 	}
-	add_point<FAM, MASK>(c, sn, k, w1, rinv_d, p.d2f, p.b0, s);
+	add_point<FAM, MASK, CK>(c, sn, k, CK ? w1 * p.ad : w1, rinv_d, p.d2f, p.b0, s, ki);
 }
 
 // Evaluate the kernels of MASK for the active lanes, each lane at its own
@@ -371,12 +384,12 @@ __device__ __forceinline__ void accumulate_centroid(const PairCtx &p, float w1, 
 // rules (degrees 0 and 1) take an unrolled path.
 // Degrees 0 and 1 are the one-point centroid rule (Dunavant; the host checks
 // its tables for this) and take accumulate_centroid: no table load.
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, const PairCtx &p, float w1,
-	float rinv_d, float k, float k_revf, float2 (&s)[6])
+	float rinv_d, float k, float k_revf, float2 (&s)[6], float ki = 0.0f)
 {
 	if (__ballot(deg >= 0 && deg <= 1) != 0) {
-		if (deg >= 0 && deg <= 1) accumulate_centroid<FAM, MASK>(p, w1, rinv_d, k, s);
+		if (deg >= 0 && deg <= 1) accumulate_centroid<FAM, MASK, CK>(p, w1, rinv_d, k, s, ki);
 		deg = deg <= 1 ? -1 : deg;
 	}
 	// wave-uniform guard: waves without a multi-point lane skip the loop and
@@ -392,7 +405,7 @@ __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, c
 		const unsigned long long msel = __ballot(deg == dsel);
 		if ((msel >> __lane_id()) & 1ull) {
 			const int nq = a.qn[dsel];
-			accumulate_points<FAM, MASK, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, k_revf, k, s);
+			accumulate_points<FAM, MASK, 0, CK>(a.qt[dsel] + (size_t)j * nq, nq, p, k_revf, k, s, ki);
 			deg = -1;
 		}
 	}
@@ -407,9 +420,9 @@ __device__ __forceinline__ void eval_degrees(const RegArgs &a, int j, int deg, c
 // once, so its sums are the rule's sums from -0 in point order, as a
 // separate evaluation per group would give them.  Degree < 0: group idle.
 // s[4] receives the hypersingular group's sum of Q (the double layer's is s[1]).
-template <int FAM, int KM>
+template <int FAM, int KM, bool CK = false>
 __device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int dD, int dH, const PairCtx &p,
-	float w1, float rinv_d, float k, float k_revf, float2 (&s)[6])
+	float w1, float rinv_d, float k, float k_revf, float2 (&s)[6], float ki = 0.0f)
 {
 	constexpr int NONE = 1 << 20;
 	int pS = (KM & SLP) && dS >= 0 ? dS : NONE;
@@ -425,10 +438,10 @@ __device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int
 		float2 t[6] = {zero2(), zero2(), zero2(), zero2(), zero2(), zero2()};
 		if (fS || fD || fH) {
 			if (dsel <= 1) {
-				accumulate_centroid<FAM, KM>(p, w1, rinv_d, k, t);
+				accumulate_centroid<FAM, KM, CK>(p, w1, rinv_d, k, t, ki);
 			} else {
 				const int nq = a.qn[dsel];
-				accumulate_points<FAM, KM, 0>(a.qt[dsel] + (size_t)j * nq, nq, p, k_revf, k, t);
+				accumulate_points<FAM, KM, 0, CK>(a.qt[dsel] + (size_t)j * nq, nq, p, k_revf, k, t, ki);
 			}
 			// the Q sum serves both the double layer (s[1]) and the hypersingular (s[4]) groups
 			const float2 tq = (KM & DLP) ? t[1] : t[4];
@@ -445,16 +458,25 @@ __device__ __forceinline__ void eval_groups(const RegArgs &a, int j, int dS, int
 // Kernel values from the accumulators of add_point: SLP s[0], DLP -a s[1],
 // DLPt s[2], HSP nn sum Q - a (3 s[3] - k^2 s[5]) with sum Q in s[4] when the
 // hypersingular group had its own sums (hsp_apart), else s[1].
-template <int FAM, int KM>
+// CK: complex wavenumber k + i ki, the (k r)^2 G term takes (k + i ki)^2.
+template <int FAM, int KM, bool CK = false>
 __device__ __forceinline__ void kernel_values(const float2 (&s)[6], float av, float nn, float k, bool hsp_apart,
-	float2 (&K)[4])
+	float2 (&K)[4], float ki = 0.0f)
 {
 	K[0] = s[0];
 	K[1] = make_float2(-av * s[1].x, -av * s[1].y);
 	K[2] = s[2];
 	const float2 sq = (hsp_apart || !(KM & DLP)) ? s[4] : s[1];
-	const float kk = FAM == HELM ? k * k : 0.0f;
-	const float hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x), hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
+	float hr, hi;
+	if (CK) {
+		const float kr2 = k * k - ki * ki, ki2 = 2.0f * k * ki;
+		hr = 3.0f * s[3].x - (kr2 * s[5].x - ki2 * s[5].y);
+		hi = 3.0f * s[3].y - (kr2 * s[5].y + ki2 * s[5].x);
+	} else {
+		const float kk = FAM == HELM ? k * k : 0.0f;
+		hr = __builtin_fmaf(3.0f, s[3].x, -kk * s[5].x);
+		hi = __builtin_fmaf(3.0f, s[3].y, -kk * s[5].y);
+	}
 	K[3] = make_float2(nn * sq.x - av * hr, nn * sq.y - av * hi);
 }
 
@@ -474,7 +496,10 @@ __device__ __forceinline__ int table_degree_rec(double d2, const double (&t)[NTH
 // has that many); 0: any policy.  MAP: stores go through a.rmap / a.cmap (a
 // block of a mesh with both element types); without it the store address is
 // the plain j * ld + i of the single-type meshes.
-template <int FAM, int KM, int MODE, int FIXQ, bool MAP>
+// CK: complex wavenumber k + i ki (Helmholtz): the attenuation e^{ki r} rides
+// on the weights (e^{ki d} per pair from the FP64 distance, times
+// 2^{ki log2(e) (r - d)} per point) and the kernels take i(k + i ki) r.
+template <int FAM, int KM, int MODE, int FIXQ, bool MAP, bool CK = false>
 __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 {
 	const int tile_r = blockIdx.x % a.row_tiles;
@@ -520,6 +545,7 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 			tt -= rint(tt);
 			p.df = (float)d;
 			p.t0 = (float)tt;
+			if (CK) p.ad = __builtin_amdgcn_exp2f((float)(d * (a.ki * 1.4426950408889634)));
 		} else {
 			p.df = 0.0f; p.t0 = 0.0f;
 		}
@@ -534,9 +560,11 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 		// hsp_apart: the hypersingular sum of Q went to s[4] (its own degree)
 		bool hsp_apart = false;
 		if (FIXQ > 0) {
-			if (act) accumulate_points<FAM, KM, FIXQ>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s);
+			if (act)
+				accumulate_points<FAM, KM, FIXQ, CK>(a.qt[a.fixed_degree] + (size_t)j * FIXQ, FIXQ, p, a.k_revf, a.k, s,
+					a.ki);
 		} else if (a.fixed_degree >= 0) {
-			eval_degrees<FAM, KM>(a, j, act ? a.fixed_degree : -1, p, h.w1, rinv_d, a.k, a.k_revf, s);
+			eval_degrees<FAM, KM, CK>(a, j, act ? a.fixed_degree : -1, p, h.w1, rinv_d, a.k, a.k_revf, s, a.ki);
 		} else {
 			// per-kernel degrees from the distance tables; where they coincide
 			// (all far pairs: one point each) all kernels share one pass
@@ -547,9 +575,9 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				// every table degree is 0: the whole wave takes degree fc for all
 				// kernels in one pass (degrees 0 and 1: the centroid rule)
 				if (fc <= 1) {
-					if (act) accumulate_centroid<FAM, KM>(p, w1, rinv_d, a.k, s);
+					if (act) accumulate_centroid<FAM, KM, CK>(p, w1, rinv_d, a.k, s, a.ki);
 				} else {
-					eval_degrees<FAM, KM>(a, j, act ? fc : -1, p, w1, rinv_d, a.k, a.k_revf, s);
+					eval_degrees<FAM, KM, CK>(a, j, act ? fc : -1, p, w1, rinv_d, a.k, a.k_revf, s, a.ki);
 				}
 			} else {
 				// the distance-table thresholds of the record
@@ -559,14 +587,14 @@ __global__ void __launch_bounds__(256) regular_kernel(RegArgs a)
 				const int dS = (KM & SLP) && act ? fc + table_degree_rec<1>(d2, t) : -1;
 				const int dD = (KM & (DLP | DLPT)) && act ? fc + table_degree_rec<2>(d2, t) : -1;
 				const int dH = (KM & HSP) && act ? fc + table_degree_rec<3>(d2, t) : -1;
-				eval_groups<FAM, KM>(a, j, dS, dD, dH, p, w1, rinv_d, a.k, a.k_revf, s);
+				eval_groups<FAM, KM, CK>(a, j, dS, dD, dH, p, w1, rinv_d, a.k, a.k_revf, s, a.ki);
 				// the hypersingular sum of Q is in s[4]
 				hsp_apart = true;
 			}
 		}
 		if (act) {
 			float2 K[4];
-			kernel_values<FAM, KM>(s, av, nn, a.k, hsp_apart, K);
+			kernel_values<FAM, KM, CK>(s, av, nn, a.k, hsp_apart, K, a.ki);
 			const long long col = (long long)(MAP && a.cmap ? a.cmap[j] : j) * a.ld;
 			if (MODE == OUT_SINGLE) {
 				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
@@ -613,6 +641,7 @@ struct NearArgs {
 	const double *sJ;            // trial Jacobians
 	const double *snu;           // trial unit normals FP64 [3][sn]
 	double k;
+	double ki;                   // imaginary part of a complex wavenumber (kernels with CK)
 	double2 alpha;
 	void *out0, *out1;
 	long long ld;
@@ -754,6 +783,49 @@ __device__ __forceinline__ void dynamic_terms(double cs, double sn, double kr, d
 			const double ar = (3.0 - kr * kr) * cc + nn, ai = kr * (3.0 * cc + nn);
 			s[3].x += ((cs * ar + sn * ai) - (3.0 * cc + nn)) * g3;
 			s[3].y += (cs * ai - sn * ar) * g3;
+		}
+	}
+}
+
+// ---- complex wavenumbers k + i ki (FP64): e = e^{-i(k + i ki) r} =
+// e^{ki r} (cos kr - i sin kr) and z = i (k + i ki) r = -ki r + i k r; the
+// kernels are polynomials in z times e:
+//   SLP e g0,  DLP e (-1 - z) g1 (r.n_y)/r,  DLPt the same with (-r.n_x)/r,
+//   HSP e [(3 + 3z + z^2) rdnx rdny + (1 + z) n_x.n_y] g3
+// (g0 = w J / (4 pi r), g1 = g0 / r, g3 = g1 / r^2; the real-k forms with z = i k r).
+__device__ __forceinline__ double2 cmul64(double2 a, double2 b)
+{
+	return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ void wave64(double k, double ki, double r, double2 *e, double2 *z)
+{
+	double sn, cs;
+	sincos64(k * r, &sn, &cs);
+	const double am = exp(ki * r);
+	*e = make_double2(am * cs, -am * sn);
+	*z = make_double2(-ki * r, k * r);
+}
+
+// K_helm - K_lap (minus = 1) or K_helm (minus = 0) of the kernels of MASK
+template <int MASK>
+__device__ __forceinline__ void wave_terms_c(double2 e, double2 z, double g0, double rinv, double rdny, double rdnx,
+	double nn, double minus, double2 (&s)[4])
+{
+	if (MASK & SLP) { s[0].x += (e.x - minus) * g0; s[0].y += e.y * g0; }
+	if (MASK & (DLP | DLPT | HSP)) {
+		const double g1 = g0 * rinv;
+		const double2 f = cmul64(e, make_double2(-1.0 - z.x, -z.y));
+		const double fr = f.x + minus, fi = f.y;
+		if (MASK & DLP) { s[1].x += fr * g1 * rdny; s[1].y += fi * g1 * rdny; }
+		if (MASK & DLPT) { s[2].x += fr * g1 * rdnx; s[2].y += fi * g1 * rdnx; }
+		if (MASK & HSP) {
+			const double g3 = g1 * rinv, cc = rdnx * rdny;
+			const double2 z2 = cmul64(z, z);
+			const double2 h = cmul64(e, make_double2((3.0 + 3.0 * z.x + z2.x) * cc + (1.0 + z.x) * nn,
+				(3.0 * z.y + z2.y) * cc + z.y * nn));
+			s[3].x += (h.x - minus * (3.0 * cc + nn)) * g3;
+			s[3].y += h.y * g3;
 		}
 	}
 }
@@ -947,12 +1019,63 @@ __device__ void diag_static(const D3 V[3], D3 x0, double (&st)[4])
 
 // Wavenumber-dependent (13-point) part of the Helmholtz diagonal integrals:
 // dyn[0] for SLP, dyn[1] for HSP, not yet divided by 4 pi.
-template <int MASK>
-__device__ void diag_dynamic(const D3 V[3], D3 x0, double J, double k, const Rules &Rr, double2 (&dyn)[2])
+template <int MASK, bool CK = false>
+__device__ void diag_dynamic(const D3 V[3], D3 x0, double J, double k, const Rules &Rr, double2 (&dyn)[2],
+	double ki = 0.0)
 {
 	double dr = 0.0, di = 0.0, ar = 0.0, ai = 0.0;
 	for (int q = 0; q < 13; ++q) {
 		const double rq = d3norm(d3sub(tri_x(V, Rr.d7xi[q], Rr.d7eta[q]), x0));
+		if (CK) {
+			// complex wavenumber kc = k + i ki, z = i kc rq, e = e^{-z}
+			const double wJ = Rr.d7w[q] * J;
+			double2 e, z;
+			wave64(k, ki, rq, &e, &z);
+			const double2 kc = make_double2(k, ki);
+			if (MASK & SLP) {
+				// -i kc e^{-i h} sinc(h), h = kc rq / 2: -(1 - e) / rq, or the series for |h| <= 1e-3
+				double2 v;
+				const double2 h = make_double2(k * rq / 2.0, ki * rq / 2.0);
+				if (sqrt(h.x * h.x + h.y * h.y) > 1e-3) {
+					v = make_double2(-(1.0 - e.x) / rq, e.y / rq);
+				} else {
+					// sinc(h) = 1 - h^2/6 (1 - h^2/20)
+					const double2 h2 = cmul64(h, h);
+					const double2 c6 = cmul64(make_double2(h2.x / 6.0, h2.y / 6.0),
+						make_double2(1.0 - h2.x / 20.0, -h2.y / 20.0));
+					const double2 sc = make_double2(1.0 - c6.x, -c6.y);
+					// e^{-i h} = e^{ki rq / 2} (cos(k rq / 2) - i sin(k rq / 2))
+					double2 eh, zh;
+					wave64(k, ki, rq / 2.0, &eh, &zh);
+					v = cmul64(cmul64(make_double2(ki, -k), eh), sc);
+				}
+				dr += v.x * wJ;
+				di += v.y * wJ;
+			}
+			if (MASK & HSP) {
+				double2 v;
+				if (fabs(rq) > 1e-3) {
+					// (e (1 + z) - 1 + z^2 / 2) / r^3
+					const double2 z2 = cmul64(z, z);
+					const double2 f = cmul64(e, make_double2(1.0 + z.x, z.y));
+					const double r3 = rq * rq * rq;
+					v = make_double2((f.x - 1.0 + z2.x / 2.0) / r3, (f.y + z2.y / 2.0) / r3);
+				} else {
+					// -i kc^3 (1/3 - z(1/8 - z(1/30 - z(1/144 - z(1/840 - z/5760)))))
+					double2 pp = make_double2(1.0 / 840.0 - z.x / 5760.0, -z.y / 5760.0);
+					const double c2[4] = {1.0 / 144.0, 1.0 / 30.0, 1.0 / 8.0, 1.0 / 3.0};
+					for (int t = 0; t < 4; ++t) {
+						const double2 zp = cmul64(z, pp);
+						pp = make_double2(c2[t] - zp.x, -zp.y);
+					}
+					const double2 k3 = cmul64(cmul64(kc, kc), kc);
+					v = cmul64(make_double2(k3.y, -k3.x), pp);
+				}
+				ar += v.x * wJ;
+				ai += v.y * wJ;
+			}
+			continue;
+		}
 		if (MASK & SLP) {
 			// -ik e^{-ik rq/2} sinc(k rq/2)
 			const double h = k * rq / 2.0;
@@ -995,14 +1118,20 @@ __device__ void diag_dynamic(const D3 V[3], D3 x0, double J, double k, const Rul
 }
 
 // combine the static and dynamic diagonal parts into kernel values
-template <int FAM, int MASK>
-__device__ __forceinline__ void diag_combine(const double (&st)[4], const double2 (&dyn)[2], double k, double2 (&s)[4])
+template <int FAM, int MASK, bool CK = false>
+__device__ __forceinline__ void diag_combine(const double (&st)[4], const double2 (&dyn)[2], double k, double2 (&s)[4],
+	double ki = 0.0)
 {
 	const double f = 1.0 / (4.0 * PI_D);
 	if (MASK & SLP) s[0] = FAM == HELM ? make_double2((st[0] + dyn[0].x) * f, dyn[0].y * f) : make_double2(st[0] * f, 0.0);
-	if (MASK & HSP)
-		s[3] = FAM == HELM ? make_double2((st[2] + k * k / 2.0 * st[1] + dyn[1].x) * f, dyn[1].y * f)
-		                   : make_double2(st[3] * f, 0.0);
+	if (MASK & HSP) {
+		if (CK)   // + kc^2 / 2 IG0 with kc = k + i ki
+			s[3] = make_double2((st[2] + (k * k - ki * ki) / 2.0 * st[1] + dyn[1].x) * f,
+				(k * ki * st[1] + dyn[1].y) * f);
+		else
+			s[3] = FAM == HELM ? make_double2((st[2] + k * k / 2.0 * st[1] + dyn[1].x) * f, dyn[1].y * f)
+			                   : make_double2(st[3] * f, 0.0);
+	}
 }
 
 // dynamic part of the Helmholtz near-singular integral: sum over the 42
@@ -1010,9 +1139,9 @@ __device__ __forceinline__ void diag_combine(const double (&st)[4], const double
 // (nihu/library/helmholtz_nearly_singular_integrals.hpp:48-219).
 // The element is flat: r = (V0 - x) + xi (V1 - V0) + eta (V2 - V0), so r.n_y
 // is the same at every point and r.n_x is linear in (xi, eta).
-template <int MASK>
+template <int MASK, bool CK = false>
 __device__ __forceinline__ void near_dynamic(const D3 V[3], D3 x0, D3 nx, D3 ny, double J, double k, const Rules &R,
-	double2 (&s)[4])
+	double2 (&s)[4], double ki = 0.0)
 {
 	const D3 a0 = d3sub(V[0], x0), e1 = d3sub(V[1], V[0]), e2 = d3sub(V[2], V[0]);
 	const double any = d3dot(a0, ny);
@@ -1025,20 +1154,26 @@ __device__ __forceinline__ void near_dynamic(const D3 V[3], D3 x0, D3 nx, D3 ny,
 			__fma_rn(eta, e2.z, __fma_rn(xi, e1.z, a0.z))};
 		const double r2 = d3dot(rv, rv);
 		const double rinv = rsqrt64(r2);
-		const double kr = k * (r2 * rinv);
-		double sn, cs;
-		sincos64(kr, &sn, &cs);
 		const double g0 = rinv * (R.d14w[q] * J4);
 		double rdny = 0.0, rdnx = 0.0;
 		if (MASK & (DLP | DLPT | HSP)) {
 			rdny = any * rinv;
 			rdnx = -__fma_rn(eta, e2nx, __fma_rn(xi, e1nx, anx)) * rinv;
 		}
-		dynamic_terms<MASK>(cs, sn, kr, g0, rinv, rdny, rdnx, nn, s);
+		if (CK) {
+			double2 e, z;
+			wave64(k, ki, r2 * rinv, &e, &z);
+			wave_terms_c<MASK>(e, z, g0, rinv, rdny, rdnx, nn, 1.0, s);
+		} else {
+			const double kr = k * (r2 * rinv);
+			double sn, cs;
+			sincos64(kr, &sn, &cs);
+			dynamic_terms<MASK>(cs, sn, kr, g0, rinv, rdny, rdnx, nn, s);
+		}
 	}
 }
 
-template <int FAM, int KM, int MODE, bool DIAG>
+template <int FAM, int KM, int MODE, bool DIAG, bool CK = false>
 __global__ void __launch_bounds__(64) fp64_pass_kernel(NearArgs a)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1058,13 +1193,13 @@ __global__ void __launch_bounds__(64) fp64_pass_kernel(NearArgs a)
 		double st[4];
 		double2 dyn[2] = {make_double2(0, 0), make_double2(0, 0)};
 		diag_static(V, x0, st);
-		if (FAM == HELM) diag_dynamic<KM & (SLP | HSP)>(V, x0, J, a.k, R, dyn);
-		diag_combine<FAM, KM & (SLP | HSP)>(st, dyn, a.k, s);
+		if (FAM == HELM) diag_dynamic<KM & (SLP | HSP), CK>(V, x0, J, a.k, R, dyn, a.ki);
+		diag_combine<FAM, KM & (SLP | HSP), CK>(st, dyn, a.k, s, a.ki);
 	} else {
 		double st[4] = {0, 0, 0, 0};
 		near_static<KM>(V, x0, nx, R, st);
 		for (int t = 0; t < 4; ++t) s[t].x = st[t];
-		if (FAM == HELM) near_dynamic<KM>(V, x0, nx, ny, J, a.k, R, s);
+		if (FAM == HELM) near_dynamic<KM, CK>(V, x0, nx, ny, J, a.k, R, s, a.ki);
 	}
 	const long long idx = (long long)(a.cmap ? a.cmap[j] : j) * a.ld + (a.rmap ? a.rmap[i] : i);
 	if (MODE == OUT_SINGLE) {
@@ -1105,6 +1240,7 @@ struct Reg64Args {
 	int qn[NSLOT];
 	int fixed_degree;            // >= 0: fixed rule
 	double k;
+	double ki;                   // imaginary part of a complex wavenumber (kernels with CK)
 	double2 alpha;
 	void *out0, *out1;
 	long long ld;
@@ -1112,11 +1248,19 @@ struct Reg64Args {
 };
 
 // the kernels of MASK at one point, rv = y - x (FP64): s += K w J
-template <int FAM, int MASK>
-__device__ __forceinline__ void full_point64(D3 rv, D3 nx, D3 ny, double k, double w4, double2 (&s)[4])
+template <int FAM, int MASK, bool CK = false>
+__device__ __forceinline__ void full_point64(D3 rv, D3 nx, D3 ny, double k, double w4, double2 (&s)[4],
+	double ki = 0.0)
 {
 	const double r2 = d3dot(rv, rv);
 	const double rinv = rsqrt64(r2);
+	if (CK) {
+		double2 e, z;
+		wave64(k, ki, r2 * rinv, &e, &z);
+		const double nn = (MASK & HSP) ? d3dot(nx, ny) : 0.0;
+		wave_terms_c<MASK>(e, z, w4 * rinv, rinv, d3dot(rv, ny) * rinv, -d3dot(rv, nx) * rinv, nn, 0.0, s);
+		return;
+	}
 	double sn = 0.0, cs = 1.0, kr = 0.0;
 	if (FAM == HELM) {
 		kr = k * (r2 * rinv);
@@ -1144,7 +1288,7 @@ __device__ __forceinline__ void full_point64(D3 rv, D3 nx, D3 ny, double k, doub
 
 // the kernels of MASK over the rule of each lane's degree (-1: none), one
 // distinct degree of the wave at a time (wave-uniform table address)
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void eval64(const Reg64Args &a, int j, int deg, D3 dc, D3 nx, D3 ny, double2 (&s)[4])
 {
 	for (;;) {
@@ -1157,14 +1301,14 @@ __device__ __forceinline__ void eval64(const Reg64Args &a, int j, int deg, D3 dc
 			const double *q = a.qt[dsel] + (size_t)j * nq * 4;
 			for (int t = 0; t < nq; ++t) {
 				const D3 rv = {dc.x + q[4 * t], dc.y + q[4 * t + 1], dc.z + q[4 * t + 2]};
-				full_point64<FAM, MASK>(rv, nx, ny, a.k, q[4 * t + 3], s);
+				full_point64<FAM, MASK, CK>(rv, nx, ny, a.k, q[4 * t + 3], s, a.ki);
 			}
 			deg = -1;
 		}
 	}
 }
 
-template <int FAM, int KM, int MODE>
+template <int FAM, int KM, int MODE, bool CK = false>
 __global__ void __launch_bounds__(256) regular64_kernel(Reg64Args a)
 {
 	const int tile_r = blockIdx.x % a.row_tiles;
@@ -1185,11 +1329,13 @@ __global__ void __launch_bounds__(256) regular64_kernel(Reg64Args a)
 		const D3 ny = {a.snu[j], a.snu[a.sn + j], a.snu[2 * a.sn + j]};
 		const int fd = a.fixed_degree;
 		double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-		if (KM & SLP) eval64<FAM, SLP>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<1>(d2, a.thr, a.sn, j), dc, nx, ny, s);
+		if (KM & SLP)
+			eval64<FAM, SLP, CK>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<1>(d2, a.thr, a.sn, j), dc, nx, ny, s);
 		if (KM & (DLP | DLPT))
-			eval64<FAM, KM & (DLP | DLPT)>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<2>(d2, a.thr, a.sn, j),
+			eval64<FAM, KM & (DLP | DLPT), CK>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<2>(d2, a.thr, a.sn, j),
 				dc, nx, ny, s);
-		if (KM & HSP) eval64<FAM, HSP>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<3>(d2, a.thr, a.sn, j), dc, nx, ny, s);
+		if (KM & HSP)
+			eval64<FAM, HSP, CK>(a, j, !act ? -1 : fd >= 0 ? fd : table_degree<3>(d2, a.thr, a.sn, j), dc, nx, ny, s);
 		if (!act) continue;
 		const long long idx = (long long)cj * a.ld + orow;
 		if (MODE == OUT_SINGLE) {

@@ -32,11 +32,12 @@ EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "n
            "nb_mesh_create_elements", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
            "nb_synchronize", "nb_profile", "nb_profile_read", "nb_eval_math",
-           "nb_assemble_sweep_noalloc", "nb_gemv_workspace", "nb_gemv_c64"]
+           "nb_assemble_sweep_noalloc", "nb_gemv_workspace", "nb_gemv_c64", "nb_assemble_bm_ck"]
 
 
 class NbKernel(C.Structure):
-    _fields_ = [("family", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("k", C.c_double)]
+    _fields_ = [("family", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("k", C.c_double),
+                ("k_im", C.c_double)]
 
 
 class NbScalar(C.Structure):
@@ -83,6 +84,8 @@ def lib():
                                   C.POINTER(NbOut), P]
         l.nb_assemble_bm.argtypes = [P, C.c_double, NbScalar, C.POINTER(NbPolicy), C.POINTER(NbOut),
                                      C.POINTER(NbOut), P]
+        l.nb_assemble_bm_ck.argtypes = [P, NbScalar, NbScalar, C.POINTER(NbPolicy), C.POINTER(NbOut),
+                                        C.POINTER(NbOut), P]
         l.nb_assemble_sweep.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
                                         C.POINTER(NbLaunch), P]
         l.nb_assemble_sweep_noalloc.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
@@ -274,23 +277,35 @@ def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.
              same_mesh=None, accumulate=True, stream=None, formalism="collocation"):
     """out (+)= alpha * operator; formalism "collocation" (dirac(constant) x
     constant), "galerkin" (constant x constant) or "galerkin_linear"
-    (isoparametric x isoparametric, rows / columns = mesh nodes)."""
+    (isoparametric x isoparametric, rows / columns = mesh nodes).  k may be
+    complex (collocation on triangle meshes)."""
     pol = pol or policy(**NIHU_POLICY)
     same = (test is trial) if same_mesh is None else bool(same_mesh)
     a = complex(alpha)
+    kc = complex(k)
     o = out_desc(out, accumulate)
     check(lib().nb_assemble(test.handle, trial.handle, int(same), FORMALISMS[formalism],
-                            NbKernel(family, nx, ny, float(k)), NbScalar(a.real, a.imag), C.byref(pol), C.byref(o),
-                            _stream_ptr(stream)))
+                            NbKernel(family, nx, ny, kc.real, kc.imag), NbScalar(a.real, a.imag), C.byref(pol),
+                            C.byref(o), _stream_ptr(stream)))
     return out
 
 
 def assemble_bm(mesh: Mesh, k, A, B, alpha=None, pol=None, accumulate=False, stream=None):
+    """Burton-Miller A and B; alpha defaults to -i/k.  A complex k goes to
+    nb_assemble_bm_ck (triangle meshes)."""
     pol = pol or policy(**NIHU_POLICY)
-    a = complex(0.0, -1.0 / k) if alpha is None else complex(alpha)
+    ck = isinstance(k, (complex, np.complexfloating))
+    if alpha is None:
+        a = complex(-1j / k) if ck else complex(0.0, -1.0 / k)
+    else:
+        a = complex(alpha)
     oa, ob = out_desc(A, accumulate), out_desc(B, accumulate)
-    check(lib().nb_assemble_bm(mesh.handle, float(k), NbScalar(a.real, a.imag), C.byref(pol), C.byref(oa),
-                               C.byref(ob), _stream_ptr(stream)))
+    if ck:
+        check(lib().nb_assemble_bm_ck(mesh.handle, NbScalar(k.real, k.imag), NbScalar(a.real, a.imag), C.byref(pol),
+                                      C.byref(oa), C.byref(ob), _stream_ptr(stream)))
+    else:
+        check(lib().nb_assemble_bm(mesh.handle, float(k), NbScalar(a.real, a.imag), C.byref(pol), C.byref(oa),
+                                   C.byref(ob), _stream_ptr(stream)))
     return A, B
 
 
