@@ -280,7 +280,12 @@ template <int Family, int Nx, int Ny, class WaveNumber = double>
 struct kernel_t {
 	explicit kernel_t(WaveNumber k = WaveNumber(0)) : m_k(k) {}
 	WaveNumber get_wave_number() const { return m_k; }
-	nb_kernel abi() const { return nb_kernel{Family, Nx, Ny, (double)m_k}; }
+	nb_kernel abi() const
+	{
+		// a real or complex wave number (NiHu's WaveNumber template parameter)
+		std::complex<double> const w(m_k);
+		return nb_kernel{Family, Nx, Ny, w.real(), w.imag()};
+	}
 	WaveNumber m_k;
 };
 

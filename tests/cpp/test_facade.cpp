@@ -146,16 +146,16 @@ static double tester(TestSpace const &test_space, TrialSpace const &trial_space)
 	return std::sqrt(e / nrm);
 }
 
-// pulsating sphere: p = -r/(1+ikr) q for a unit normal velocity
-template <class TestSpace, class TrialSpace>
-static double pulsating(TestSpace const &test_space, TrialSpace const &trial_space)
+// pulsating sphere: p = -r/(1+ikr) q for a unit normal velocity; the wave
+// number type W is double or std::complex<double> (NiHu's WaveNumber)
+template <class TestSpace, class TrialSpace, class W = double>
+static double pulsating(TestSpace const &test_space, TrialSpace const &trial_space, W k = W(1.0))
 {
 	const size_t N = trial_space.get_num_dofs();
-	double k = 1.0;
 	cMatrix L(N, N), M(N, N), I(N, N);
 	auto I_op = nihu_amd::identity_integral_operator();
-	auto L_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(k));
-	auto M_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLP_kernel<double>(k));
+	auto L_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<W>(k));
+	auto M_op = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLP_kernel<W>(k));
 	I << test_space * I_op[trial_space];
 	L << test_space * L_op[trial_space];
 	M << test_space * M_op[trial_space];
@@ -167,7 +167,7 @@ static double pulsating(TestSpace const &test_space, TrialSpace const &trial_spa
 			rhs[i] += L(i, j);
 		}
 	auto p = solve(S, rhs);
-	const cplx panal = -1.0 / (1.0 + cplx(0, k));
+	const cplx panal = -1.0 / (1.0 + cplx(0, 1) * cplx(k));
 	double e = 0;
 	for (auto const &v : p) e += std::norm(v - panal);
 	return std::sqrt(e / N) / std::abs(panal);
@@ -225,6 +225,10 @@ int main(int argc, char *argv[])
 	double puls = pulsating(nihu_amd::dirac(trial_space), trial_space);
 	std::printf("Pulsating sphere relative error (collocation): %g\n", puls);
 	bad |= !(puls < 0.05);
+
+	double puls_ck = pulsating(nihu_amd::dirac(trial_space), trial_space, cplx(1.0, -0.3));
+	std::printf("Pulsating sphere relative error (collocation, complex wave number): %g\n", puls_ck);
+	bad |= !(puls_ck < 0.05);
 
 	double gal_error = tester(trial_space, trial_space);
 	std::printf("Galerkin transparent error: %g\n", gal_error);

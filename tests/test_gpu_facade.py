@@ -34,6 +34,7 @@ def test_cpp_facade_known_answers(tmp_path):
     assert "runtime_error as expected: Invalid general singular quadrature" in r.stdout
     assert "logic_error as expected" in r.stdout
     for line in ("Galerkin transparent error", "Pulsating sphere relative error (Galerkin)",
+                 "Pulsating sphere relative error (collocation, complex wave number)",
                  "Galerkin isoparametric transparent error", "Quadrilateral collocation transparent error",
                  "Quadrilateral Burton-Miller composition", "quadrilateral Galerkin: logic_error as expected",
                  "Mixed-mesh collocation transparent error", "Mixed-mesh Burton-Miller composition",
