@@ -79,7 +79,8 @@ typedef struct nb_mesh nb_mesh;
 /* kernel: family, normal-derivative orders (nx, ny) in {0,1}^2, wavenumber
  * k + i k_im (NiHu's helmholtz_kernel<space, WaveNumber> with a real or
  * complex WaveNumber, library/helmholtz_kernel.hpp:103; k > 0; a nonzero
- * k_im is taken by collocation on triangle meshes, else NB_ERR_UNSUPPORTED) */
+ * k_im is taken on triangle meshes (collocation and Galerkin), else
+ * NB_ERR_UNSUPPORTED) */
 typedef struct {
 	int32_t family;
 	int32_t nx;

@@ -586,10 +586,10 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 {
 	if (jb.pol.precision != NB_PREC_MIXED)
 		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
-	if (jb.ki != 0.0) return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
 	nb_mesh *trial = jb.trial;
 	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
 	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
+	const bool ck = jb.ki != 0.0 && jb.family == NB_HELMHOLTZ;
 	const int64_t rows = nodal ? test->n_nodes : test->n, cols = nodal ? trial->n_nodes : trial->n;
 	const int eb = dtype_bytes(dtype);
 	auto zero = [&]() -> int {
@@ -656,6 +656,7 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	a.k = (float)jb.k;
 	a.k_rev = jb.k / two_pi;
 	a.k_revf = (float)a.k_rev;
+	a.ki = (float)jb.ki;
 	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
 	a.out = out; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
 	const long long grid = (long long)((a.n_tiles + 3) / 4) * ((trial->n + a.chunk - 1) / a.chunk);
@@ -664,7 +665,10 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 #define NB_GAL(F, K) do { \
-		if (nodal) { \
+		if (F == HELM && ck) { \
+			if (nodal) hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 3, 0, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+			else hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 1, 0, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+		} else if (nodal) { \
 			if (F == HELM && fix6) hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 3, (F == HELM ? 6 : 0)>), dim3(grid), dim3(256), 0, st, a); \
 			else hipLaunchKernelGGL((galerkin_regular_kernel<F, K, 3, 0>), dim3(grid), dim3(256), 0, st, a); \
 		} else { \
@@ -711,13 +715,17 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	g.tri = (const int *)trial->d_tri.get();
 	for (int m = 0; m < 3; ++m) { g.rule[m] = (const double *)sr->r[m].get(); g.rule_n[m] = sr->n[m]; }
 	g.k = jb.k;
+	g.ki = jb.ki;
 	g.alpha = make_double2(jb.alpha.re, jb.alpha.im);
 	g.out = out; g.ld = ld; g.dtype = dtype;
 	{
 		using namespace nbk;
 		const int pe = prof_begin(2, st);
 #define NB_GSING(F, K) do { \
-		if (nodal) hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 3>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+		if (F == HELM && ck) { \
+			if (nodal) hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 3, (F == HELM)>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+			else hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 1, (F == HELM)>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+		} else if (nodal) hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 3>), dim3(g.n_pairs), dim3(64), 0, st, g); \
 		else hipLaunchKernelGGL((galerkin_singular_kernel<F, K, 1>), dim3(g.n_pairs), dim3(64), 0, st, g); } while (0)
 		if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
@@ -1625,8 +1633,8 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 	int s;
 	if (test->npe != 3 || trial->npe != 3)
 		return fail(NB_ERR_UNSUPPORTED, "the Galerkin formalism is implemented for triangles only");
-	if (!ident && kernel.family == NB_HELMHOLTZ && kernel.k_im != 0.0)
-		return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
+	if (!ident && kernel.family == NB_HELMHOLTZ && !std::isfinite(kernel.k_im))
+		return fail(NB_ERR_ARG, "imaginary part of the wavenumber must be finite");
 	if (!ident) {
 		if (kernel.family != NB_LAPLACE && kernel.family != NB_HELMHOLTZ) return fail(NB_ERR_ARG, "unknown kernel family");
 		if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
@@ -1648,6 +1656,7 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 	jb.km = ident ? 0 : kernel_mask(kernel.nx, kernel.ny);
 	jb.mode = nbk::OUT_SINGLE;
 	jb.k = kernel.family == NB_HELMHOLTZ ? kernel.k : 0.0;
+	jb.ki = kernel.family == NB_HELMHOLTZ ? kernel.k_im : 0.0;
 	jb.alpha = alpha;
 	if (policy) jb.pol = *policy;
 	else jb.pol = nb_policy{NB_RULE_NIHU, 0, NB_NEAR_MEAN_EDGE, 1.5};

@@ -59,6 +59,7 @@ struct GalArgs {
 	int fixed_degree;            // >= 0: fixed rule
 	float k, k_revf;             // wavenumber, k / (2 pi)
 	double k_rev;
+	float ki;                    // imaginary part of a complex wavenumber (kernels with CK)
 	float2 alpha;
 	void *out;
 	long long ld;
@@ -74,6 +75,7 @@ struct GalPair {
 	float nyx, nyy, nyz;         // trial unit normal
 	float nxx, nxy, nxz;         // test unit normal
 	float dcny, dcnx, nn;        // (c_f - c_e).n_y, (c_f - c_e).n_x, n_x.n_y
+	float ad;                    // complex wavenumber k + i ki: e^{ki d}
 };
 
 __device__ __forceinline__ void atomic_add_entry(void *out, long long idx, int dtype, float re, float im)
@@ -133,9 +135,12 @@ __device__ __forceinline__ void atomic_add_entry64(void *out, long long idx, int
 // T = sum K w (N_0 + N_1 + N_2 = 1), S_1 and S_2, and blk[a][0] receives the
 // T sums; the caller turns that column into N_0's, T - S_1 - S_2, after the
 // pair.  ref: the rule's reference points (xi_p, eta_p) for the test weights.
-template <int FAM, int KM, int NDOF, int NQ, class TP>
+// CK: complex wavenumber k + i kim: e = e^{kim r} (c - i sn) (the attenuation
+// scales c and sn: every kernel is e times a polynomial in i(k + i kim) r),
+// Q gains -kim r G and the (k r)^2 G term takes (k + i kim)^2.
+template <int FAM, int KM, int NDOF, int NQ, class TP, bool CK = false>
 __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const float2 *ref, int nq_rt, const GalPair &P,
-	float k_revf, float k, float2 (&blk)[NDOF][NDOF])
+	float k_revf, float k, float2 (&blk)[NDOF][NDOF], float kim = 0.0f)
 {
 	const int nq = NQ > 0 ? NQ : nq_rt;
 	const auto sf = uni((const float *)sq_in);
@@ -163,9 +168,15 @@ __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const flo
 			const float rinv = __builtin_amdgcn_rsqf(r2);
 			float c = 1.0f, sn = 0.0f;
 			if (IM) {
-				const float tt = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(__builtin_fmaf(r2, rinv, P.df)), P.t0);
+				const float rd = u * __builtin_amdgcn_rcpf(__builtin_fmaf(r2, rinv, P.df));   // r - d
+				const float tt = __builtin_fmaf(k_revf, rd, P.t0);
 				c = __builtin_amdgcn_cosf(tt);
 				sn = __builtin_amdgcn_sinf(tt);
+				if (CK) {
+					const float am = P.ad * __builtin_amdgcn_exp2f(kim * 1.4426950408889634f * rd);
+					c *= am;
+					sn *= am;
+				}
 			}
 			float kr, ki = 0.0f;
 			if (KM == SLP) {
@@ -177,8 +188,13 @@ __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const flo
 				const float Gr = g3 * c, Gi = IM ? -g3 * sn : 0.0f;
 				const float kh = k * ri2;
 				// Q = G (1 + i th): Re = Gr + k sn / r^2, Im = Gi + k c / r^2
-				const float Qr = IM ? __builtin_fmaf(kh, sn, Gr) : Gr;
-				const float Qi = IM ? __builtin_fmaf(kh, c, Gi) : 0.0f;
+				float Qr = IM ? __builtin_fmaf(kh, sn, Gr) : Gr;
+				float Qi = IM ? __builtin_fmaf(kh, c, Gi) : 0.0f;
+				if (CK) {
+					const float kih = kim * ri2;
+					Qr = __builtin_fmaf(-kih, c, Qr);
+					Qi = __builtin_fmaf(kih, sn, Qi);
+				}
 				if (KM == DLP) {
 					kr = Qr; ki = Qi;
 				} else {
@@ -188,9 +204,16 @@ __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const flo
 					} else {
 						const float cf = rnx * rny * ri2;
 						// 3 Q - th^2 G with th^2 G = k^2 E = k^2 e^{-i th} / r
-						const float kk = k * k * rinv;
-						const float hr = __builtin_fmaf(3.0f, Qr, -kk * c);
-						const float hi = __builtin_fmaf(3.0f, Qi, kk * sn);
+						float hr, hi;
+						if (CK) {
+							const float kr2 = (k * k - kim * kim) * rinv, ki2 = 2.0f * k * kim * rinv;
+							hr = 3.0f * Qr - (kr2 * c + ki2 * sn);
+							hi = 3.0f * Qi - (ki2 * c - kr2 * sn);
+						} else {
+							const float kk = k * k * rinv;
+							hr = __builtin_fmaf(3.0f, Qr, -kk * c);
+							hi = __builtin_fmaf(3.0f, Qi, kk * sn);
+						}
 						kr = __builtin_fmaf(P.nn, Qr, -hr * cf);
 						ki = __builtin_fmaf(P.nn, Qi, -hi * cf);
 					}
@@ -231,7 +254,7 @@ __device__ __forceinline__ void gal_points(TP tp, const QPoint *sq_in, const flo
 }
 
 // policy-driven degree per lane; lanes are served one distinct degree at a time
-template <int FAM, int KM, int NDOF>
+template <int FAM, int KM, int NDOF, bool CK = false>
 __device__ __forceinline__ void gal_eval_dynamic(const GalArgs &a, int e, int f, double d2, bool act,
 	const GalPair &P, float2 (&blk)[NDOF][NDOF])
 {
@@ -247,9 +270,9 @@ __device__ __forceinline__ void gal_eval_dynamic(const GalArgs &a, int e, int f,
 		if ((msel >> __lane_id()) & 1ull) {
 			const int nq = a.qn[dsel];
 			const float *tq = (const float *)(a.tq[dsel] + (size_t)e * nq);
-			gal_points<FAM, KM, NDOF, 0>(
-				[&](int p) { return GTest{tq[8 * p], tq[8 * p + 1], tq[8 * p + 2], tq[8 * p + 4], tq[8 * p + 7]}; },
-				a.sq[dsel] + (size_t)f * nq, a.ref + dsel * REF_MAXQ, nq, P, a.k_revf, a.k, blk);
+			auto tp = [&](int p) { return GTest{tq[8 * p], tq[8 * p + 1], tq[8 * p + 2], tq[8 * p + 4], tq[8 * p + 7]}; };
+			gal_points<FAM, KM, NDOF, 0, decltype(tp), CK>(tp, a.sq[dsel] + (size_t)f * nq, a.ref + dsel * REF_MAXQ, nq,
+				P, a.k_revf, a.k, blk, a.ki);
 			deg = -1;
 		}
 	}
@@ -287,7 +310,7 @@ __device__ __forceinline__ void gal_flush(float2 *acc, int lane, int slot, int c
 
 // FIXQ > 0: fixed rule with FIXQ points per element (test points held in
 // registers); 0: any policy.  Grid: (test tiles / 4) x chunks, 256 lanes.
-template <int FAM, int KM, int NDOF, int FIXQ>
+template <int FAM, int KM, int NDOF, int FIXQ, bool CK = false>
 __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 {
 	__shared__ float2 acc_all[NDOF == 3 ? 4 * GAL_CS * 3 * GAL_TILE : 1];
@@ -368,6 +391,7 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 				t -= rint(t);
 				P.df = (float)d;
 				P.t0 = (float)t;
+				if (CK) P.ad = __builtin_amdgcn_exp2f((float)(d * (a.ki * 1.4426950408889634)));
 			} else {
 				P.df = 0.0f; P.t0 = 0.0f;
 			}
@@ -380,11 +404,13 @@ __global__ void __launch_bounds__(256) galerkin_regular_kernel(GalArgs a)
 			if (KM & (DLPT | HSP)) P.dcnx = dcx * P.nxx + dcy * P.nxy + dcz * P.nxz;
 			if (KM & HSP) P.nn = P.nxx * P.nyx + P.nxy * P.nyy + P.nxz * P.nyz;
 			if (FIXQ > 0) {
-				if (act)
-					gal_points<FAM, KM, NDOF, FIXQ>([&](int p) { return tpr[p]; }, a.sq[a.fixed_degree] + (size_t)f * FIXQ,
-						a.ref + a.fixed_degree * REF_MAXQ, FIXQ, P, a.k_revf, a.k, blk);
+				if (act) {
+					auto tp = [&](int p) { return tpr[p]; };
+					gal_points<FAM, KM, NDOF, FIXQ, decltype(tp), CK>(tp, a.sq[a.fixed_degree] + (size_t)f * FIXQ,
+						a.ref + a.fixed_degree * REF_MAXQ, FIXQ, P, a.k_revf, a.k, blk, a.ki);
+				}
 			} else {
-				gal_eval_dynamic<FAM, KM, NDOF>(a, ee, f, d2, act, P, blk);
+				gal_eval_dynamic<FAM, KM, NDOF, CK>(a, ee, f, d2, act, P, blk);
 			}
 		}
 		if (NDOF == 1) {
@@ -456,6 +482,21 @@ __device__ __forceinline__ double2 kernel_value64(D3 x, D3 y, D3 nx, D3 ny, doub
 	return make_double2((cs * ar + sn * ai) * g3, (cs * ai - sn * ar) * g3);
 }
 
+// the same for a complex wavenumber k + i ki
+template <int KM>
+__device__ __forceinline__ double2 kernel_value64_c(D3 x, D3 y, D3 nx, D3 ny, double k, double ki)
+{
+	const D3 rv = d3sub(y, x);
+	const double r2 = d3dot(rv, rv);
+	const double rinv = rsqrt64(r2);
+	double2 e, z;
+	wave64(k, ki, r2 * rinv, &e, &z);
+	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+	wave_terms_c<KM>(e, z, (1.0 / (4.0 * PI_D)) * rinv, rinv, d3dot(rv, ny) * rinv, -d3dot(rv, nx) * rinv,
+		(KM & HSP) ? d3dot(nx, ny) : 0.0, 0.0, s);
+	return s[KM == SLP ? 0 : KM == DLP ? 1 : KM == DLPT ? 2 : 3];
+}
+
 struct GalSingArgs {
 	int n_pairs;
 	const int *pe, *pf, *pinfo;  // test / trial element, match | ind1 << 4 | ind2 << 8
@@ -467,6 +508,7 @@ struct GalSingArgs {
 	const double *rule[3];       // canonical rules by match dimension: [points][5]
 	int rule_n[3];
 	double k;
+	double ki;                   // imaginary part of a complex wavenumber (kernels with CK)
 	double2 alpha;
 	void *out;
 	long long ld;
@@ -478,7 +520,7 @@ struct GalSingArgs {
 // for a common edge (it runs the other way) and (ind2 + k) % 3 for a common
 // vertex.  The vertex coordinates are permuted accordingly, so that the
 // shape function N_k at a rule point belongs to the permuted corner k.
-template <int FAM, int KM, int NDOF>
+template <int FAM, int KM, int NDOF, bool CK = false>
 __global__ void __launch_bounds__(64) galerkin_singular_kernel(GalSingArgs a)
 {
 	const int p = blockIdx.x, lane = threadIdx.x;
@@ -506,7 +548,7 @@ __global__ void __launch_bounds__(64) galerkin_singular_kernel(GalSingArgs a)
 	for (int q = lane; q < nr; q += 64) {
 		const double txi = R[5 * q], teta = R[5 * q + 1], sxi = R[5 * q + 2], seta = R[5 * q + 3], w = R[5 * q + 4];
 		const D3 x = tri_x(Vt, txi, teta), y = tri_x(Vs, sxi, seta);
-		const double2 K = kernel_value64<FAM, KM>(x, y, nx, ny, a.k);
+		const double2 K = CK ? kernel_value64_c<KM>(x, y, nx, ny, a.k, a.ki) : kernel_value64<FAM, KM>(x, y, nx, ny, a.k);
 		const double wt = Je * w, ws = Jf * 1.0;
 		if (NDOF == 1) {
 			acc[0] += wt * K.x * ws;

@@ -757,7 +757,8 @@ static inline void map_corners(double xi, double eta, const int c[3], double &ox
 // linear = 1: isoparametric field (3 nodal shape functions, field complexity 1);
 // linear = 0: constant field (one shape function 1, complexity 0,
 // nihu/core/complexity_estimator.hpp:39-75)
-static void galerkin_block(int family, int nx, int ny, double k, Policy const &p, Elem const &te, Elem const &se,
+template <class WN>
+static void galerkin_block(int family, int nx, int ny, WN k, Policy const &p, Elem const &te, Elem const &se,
 	const int *tn, const int *sn, bool same_mesh, int linear, cplx blk[3][3])
 {
 	for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) blk[a][b] = 0.0;
@@ -1608,9 +1609,12 @@ int orc_sweep_bm(const double *xyz, int nn, const int *tri, int nt, const double
 // nodes, nihu/core/field.hpp:234-258); linear = 0: constant_view (dofs =
 // elements).  family 2 = identity (numerical mass matrix,
 // nihu/core/single_integral.hpp:65-140).
-int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
+} // extern "C"
+
+template <class K>
+static int assemble_galerkin_impl(const double *txyz, int tn, const int *ttri, int tnt,
 	const double *sxyz, int sn, const int *stri, int snt, int same_mesh, int linear,
-	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	int family, int nx, int ny, K k, double alpha_re, double alpha_im,
 	const orc_policy *pol, const int *test_elems, int n_test_elems, double *out, int64_t ld, int accumulate,
 	int n_threads)
 {
@@ -1684,6 +1688,30 @@ int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
 	if (errcode == 2) throw std::out_of_range(err);
 	if (errcode == 1) throw std::runtime_error(err);
 	ORC_CATCH
+}
+
+
+extern "C" {
+
+int orc_assemble_galerkin(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh, int linear,
+	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *test_elems, int n_test_elems, double *out, int64_t ld, int accumulate,
+	int n_threads)
+{
+	return assemble_galerkin_impl(txyz, tn, ttri, tnt, sxyz, sn, stri, snt, same_mesh, linear, family, nx, ny, k,
+		alpha_re, alpha_im, pol, test_elems, n_test_elems, out, ld, accumulate, n_threads);
+}
+
+// the same with a complex wavenumber k_re + i k_im
+int orc_assemble_galerkin_ck(const double *txyz, int tn, const int *ttri, int tnt,
+	const double *sxyz, int sn, const int *stri, int snt, int same_mesh, int linear,
+	int family, int nx, int ny, double k_re, double k_im, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *test_elems, int n_test_elems, double *out, int64_t ld, int accumulate,
+	int n_threads)
+{
+	return assemble_galerkin_impl(txyz, tn, ttri, tnt, sxyz, sn, stri, snt, same_mesh, linear, family, nx, ny,
+		cplx(k_re, k_im), alpha_re, alpha_im, pol, test_elems, n_test_elems, out, ld, accumulate, n_threads);
 }
 
 // node overlap of two elements as NiHu's scan finds it: returns the count

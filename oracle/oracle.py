@@ -107,6 +107,9 @@ def _declare(l):
         l.orc_assemble_galerkin.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int,
                                             C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                             P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_assemble_galerkin_ck.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int,
+                                               C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                               C.c_double, P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
         l.orc_overlap.argtypes = [I, I, I, I]
         l.orc_singular_rule.argtypes = [C.c_int, D, C.c_int, I]
         l.orc_galerkin_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, P, D, I, D, I, C.c_int, C.c_int,
@@ -306,6 +309,14 @@ def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, t
         accumulate = False
     alpha = complex(alpha)
     te = None if test_elems is None else np.ascontiguousarray(test_elems, dtype=np.int32)
+    if _is_complex(k):
+        _check(lib().orc_assemble_galerkin_ck(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                                              _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
+                                              int(linear), family, nx, ny, k.real, k.imag, alpha.real, alpha.imag,
+                                              C.byref(pol), _i(te), 0 if te is None else len(te),
+                                              out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0],
+                                              int(accumulate), threads))
+        return out
     _check(lib().orc_assemble_galerkin(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
                                        _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same), int(linear),
                                        family, nx, ny, k, alpha.real, alpha.imag, C.byref(pol), _i(te),

@@ -4,8 +4,10 @@ the four Helmholtz operators and the Burton-Miller matrices, under NiHu's
 degrees and the fixed-rule configuration, in mixed precision (FP32 regular
 pairs with the attenuation e^{ki r} on the weights: relative Frobenius
 <= 1e-5) and under the FP64 precision policy (<= 1e-11); a zero imaginary
-part gives the real-wavenumber matrices bit for bit; quadrilateral meshes
-and the Galerkin formalism refuse complex wavenumbers."""
+part gives the real-wavenumber matrices bit for bit; the Galerkin
+formalism (constant and isoparametric linear fields) against the oracle's
+Galerkin matrices (<= 1e-5); quadrilateral meshes refuse complex
+wavenumbers."""
 
 import numpy as np
 import pytest
@@ -85,14 +87,30 @@ def test_zero_imaginary_part_is_the_real_path(nb):
         assert torch.equal(torch.view_as_real(B), torch.view_as_real(B2))
 
 
+@pytest.mark.parametrize("formalism", ["galerkin", "galerkin_linear"])
+@pytest.mark.parametrize("nx,ny", [(0, 0), (0, 1), (1, 0)])
+def test_galerkin(nb, orc, formalism, nx, ny):
+    import torch
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(2, seed=8)
+    m = nb.Mesh(nodes, tris)
+    k = complex(3.0, -0.7)
+    linear = formalism == "galerkin_linear"
+    n = m.n_nodes if linear else m.n
+    out = nb.colmajor(n, n, "complex128")
+    nb.assemble(m, m, out, nb.NB_HELMHOLTZ, nx, ny, k, alpha=0.8 - 0.1j, accumulate=False, formalism=formalism)
+    torch.cuda.synchronize()
+    ref = orc.assemble_galerkin(nodes, tris, 1, nx, ny, k, alpha=0.8 - 0.1j, linear=linear)
+    e = relfro(out.cpu().numpy(), ref)
+    print("Galerkin %s (%d, %d): rel fro %.2e" % (formalism, nx, ny, e), flush=True)
+    assert e <= 1e-5
+
+
 def test_refusals(nb):
     from nihu_amd.mesh import jittered_icosphere, sphere_boundary
     nodes, tris = jittered_icosphere(1, seed=3)
     m = nb.Mesh(nodes, tris)
     k = complex(2.0, -0.5)
-    with pytest.raises(NotImplementedError):
-        nb.assemble(m, m, nb.colmajor(m.n_nodes, m.n_nodes, "complex128"), nb.NB_HELMHOLTZ, 0, 0, k,
-                    formalism="galerkin_linear")
     qn, qe = sphere_boundary(1.0, 2)
     q = nb.Mesh(qn, qe)
     with pytest.raises(NotImplementedError):

@@ -160,3 +160,32 @@ def test_bm_is_sum_of_operators(orc):
     I = np.eye(len(tris))
     assert np.allclose(A, al * D - 0.5 * I + H, rtol=0, atol=1e-13 * np.abs(A).max())
     assert np.allclose(B, G + al * Ht + al / 2 * I, rtol=0, atol=1e-13 * np.abs(B).max())
+
+
+@pytest.mark.parametrize("linear", [False, True])
+def test_galerkin_pulsating_sphere(orc, linear):
+    """The pulsating-sphere answer in Galerkin form (constant and
+    isoparametric linear fields) at a complex wavenumber:
+    (M - M_I/2) p = L q with the mass matrix M_I, p = -1/(1 + ik) for q = 1."""
+    from nihu_amd.mesh import icosphere
+    nodes, tris = icosphere(2)
+    k = complex(1.0, -0.3)
+    L = orc.assemble_galerkin(nodes, tris, 1, 0, 0, k, linear=linear)
+    M = orc.assemble_galerkin(nodes, tris, 1, 0, 1, k, linear=linear)
+    I = orc.assemble_galerkin(nodes, tris, 2, 0, 0, 0.0, linear=linear)
+    p = np.linalg.solve(M - 0.5 * I, L @ np.ones(L.shape[1]))
+    panal = -1.0 / (1.0 + 1j * k)
+    err = np.abs(p - panal).max() / abs(panal)
+    print("Galerkin pulsating sphere (%s), complex k: %.2e" % ("linear" if linear else "constant", err))
+    assert err < 0.03
+
+
+def test_galerkin_real_limit(orc):
+    """k + 0i through the complex path equals the real-wavenumber Galerkin
+    matrices (the same quadrature, the real kernel)."""
+    from nihu_amd.mesh import jittered_icosphere
+    nodes, tris = jittered_icosphere(1, seed=2)
+    for nx, ny in ((0, 0), (0, 1), (1, 0)):
+        a = orc.assemble_galerkin(nodes, tris, 1, nx, ny, 2.5)
+        b = orc.assemble_galerkin(nodes, tris, 1, nx, ny, complex(2.5, 0.0))
+        assert np.abs(a - b).max() <= 1e-14 * np.abs(a).max()
