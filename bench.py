@@ -25,7 +25,8 @@ requests served with and without batching (p50/p99 latency, solves/s) and a
 rate sweep up to the first rate the server does not keep up with; then short
 runs (5 timed steps, 32 rows or node groups against the oracle) of the Galerkin workload
 (config 3), the quadrilateral and the mixed-element workloads, each with its
-roofline: `galerkin`, `quad`, `mixed` (--no-extras skips all of them).
+roofline: `galerkin`, `quad`, `mixed`, and config 1, the Laplace matrices of
+1280 triangles in FP64, `laplace_fp64` (--no-extras skips all of them).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 20]
 """
@@ -160,12 +161,14 @@ def main():
                     help="rocprofv3 FETCH/WRITE summary of the regular kernel (default: the one of --policy)")
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "mixed"], default="bm",
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "mixed", "laplace"],
+                    default="bm",
                     help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
                          "serve: Poisson stream of assembly-plus-LU-solve requests; galerkin: SLP and DLP "
                          "Galerkin matrices with linear triangles on the 20480-element ellipsoid; quad: "
                          "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere; mixed: the "
-                         "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table)")
+                         "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table); "
+                         "laplace: the Laplace SLP and DLP matrices of 1280 triangles in FP64 (plain method)")
     ap.add_argument("--requests", type=int, default=24, help="serve: requests per stream")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only: skip the sweep, serving, Galerkin, quadrilateral and mixed-mesh "
@@ -195,6 +198,9 @@ def main():
         return
     if args.workload == "mixed":
         print(json.dumps(run_mixed(args)))
+        return
+    if args.workload == "laplace":
+        print(json.dumps(run_laplace(args)))
         return
 
     import numpy as np
@@ -339,7 +345,8 @@ def main():
         # (Galerkin: the test elements of 32 sampled nodes against the
         # oracle), the quadrilateral and the mixed-element workloads (32 rows),
         # each with its roofline
-        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("mixed", run_mixed, 32)):
+        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("mixed", run_mixed, 32),
+                              ("laplace_fp64", run_laplace, -1)):
             rec = fn(argparse.Namespace(steps=5, warmup=5, cpu_rows=rows, profile_copy_bytes=0, level=5))
             extras[key] = {k: rec.get(k) for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
                                                    "cpu_baseline", "accuracy", "pass_ms")}
@@ -493,6 +500,77 @@ def run_galerkin(args):
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
     return out
+
+
+def run_laplace(args):
+    """Laplace 3D collocation, the plain method in FP64: SLP and DLP as real
+    float64 matrices on the 1280-triangle icosphere (level 3, +-1 % radial
+    jitter, seed 0), the 7-point rule for every regular pair, no
+    near-singular pass, analytic diagonal; every pair in FP64
+    (precision="fp64").  A step assembles both 1280^2 matrices.  The CPU
+    baseline is the oracle assembling the same two matrices whole."""
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import jittered_icosphere
+
+    torch.cuda.set_device(0)
+    nodes, tris = jittered_icosphere(3, seed=0)
+    n = len(tris)
+    pol = nbem.policy(rule="fixed", degree=5, near="none", precision="fp64")
+    mesh = nbem.Mesh(nodes, tris)
+    S, D = nbem.colmajor(n, n, "float64"), nbem.colmajor(n, n, "float64")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        nbem.assemble(mesh, mesh, S, nbem.NB_LAPLACE, 0, 0, 0.0, pol=pol, accumulate=False, stream=stream)
+        nbem.assemble(mesh, mesh, D, nbem.NB_LAPLACE, 0, 1, 0.0, pol=pol, accumulate=False, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    ms_per_step = wall / args.steps * 1e3
+    value = 2 * n * n * args.steps / wall
+    log("laplace FP64: timed %d steps: %.3f ms/step, %.4g entries/s; per-launch device ms %s" % (
+        args.steps, ms_per_step, value, pass_ms))
+    from oracle import oracle
+    threads = oracle.num_threads()
+    build = oracle_tuned_build(oracle)
+    opol = oracle.policy(rule=1, fixed_degree=5, near_mode=2)
+    tc = time.perf_counter()
+    Sr = oracle.assemble(nodes, tris, 0, 0, 0, pol=opol, threads=threads).real
+    Dr = oracle.assemble(nodes, tris, 0, 0, 1, pol=opol, threads=threads).real
+    dt = time.perf_counter() - tc
+    oracle.use_build("exact")
+    cpu_value = 2 * n * n / dt
+    Sg, Dg = S.cpu().numpy(), D.cpu().numpy()
+    err = {"SLP_rel_fro": float(np.linalg.norm(Sg - Sr) / np.linalg.norm(Sr)),
+           "DLP_rel_fro": float(np.linalg.norm(Dg - Dr) / np.linalg.norm(Dr)),
+           "SLP_max_abs": float(np.abs(Sg - Sr).max()), "DLP_max_abs": float(np.abs(Dg - Dr).max())}
+    log("laplace FP64: error vs oracle %s; cpu %.4g entries/s on %d threads" % (err, cpu_value, threads))
+    return {
+        "metric": "matrix entries/s, Laplace 3D coll. FP64, 1280 tris",
+        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64 (every pair)", "data": "synthetic: seeded jittered icosphere",
+        "config": {"workload": "Laplace 3D collocation, SLP and DLP real matrices, plain method in FP64",
+                   "n_tri": n, "icosphere_level": 3, "regular_rule": "Dunavant degree 5 (7 points), every pair",
+                   "near": "none", "diagonal": "analytic"},
+        "roofline": None, "accuracy": err, "pass_ms": pass_ms,
+        "cpu_baseline": {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+                         "sample": "the whole workload (both matrices), FP64 oracle, OpenMP over rows, %.2f s wall"
+                                   % dt, "build": ORACLE_BUILDS[build], "host": host_cpu()},
+    }
 
 
 def run_quad(args):
