@@ -79,8 +79,9 @@ typedef struct nb_mesh nb_mesh;
 /* kernel: family, normal-derivative orders (nx, ny) in {0,1}^2, wavenumber
  * k + i k_im (NiHu's helmholtz_kernel<space, WaveNumber> with a real or
  * complex WaveNumber, library/helmholtz_kernel.hpp:103; k > 0; a nonzero
- * k_im is taken on triangle meshes (collocation and Galerkin), else
- * NB_ERR_UNSUPPORTED) */
+ * k_im is taken in collocation on every mesh type and in the Galerkin
+ * formalism on triangles, not with the FP64 precision policy on meshes with
+ * quadrilaterals: NB_ERR_UNSUPPORTED) */
 typedef struct {
 	int32_t family;
 	int32_t nx;
@@ -183,8 +184,8 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_policy *policy,
                    const nb_out *A, const nb_out *B, void *stream);
 /* The same for a complex wavenumber k.re + i k.im (k.re > 0): the
- * composition of helmholtz_3d_coll.cpp:63-96 with a complex WaveNumber,
- * triangle meshes only (else NB_ERR_UNSUPPORTED). */
+ * composition of helmholtz_3d_coll.cpp:63-96 with a complex WaveNumber, on
+ * meshes of triangles, of quadrilaterals or of both. */
 int nb_assemble_bm_ck(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const nb_policy *policy,
                       const nb_out *A, const nb_out *B, void *stream);
 

@@ -780,7 +780,6 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	int s;
 	if (jb.pol.precision != NB_PREC_MIXED)
 		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
-	if (jb.ki != 0.0) return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
 	if ((s = check_quad_policy(jb.pol))) return s;
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	int fixed_n = 0;
@@ -810,9 +809,12 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	a.k = (float)jb.k;
 	a.k_rev = jb.k / two_pi;
 	a.k_revf = (float)a.k_rev;
+	a.ki = jb.ki;
 	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
 	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
 	const long long grid = (long long)a.row_tiles * ((trial->n + a.tile_cols - 1) / a.tile_cols);
+	// a complex wavenumber (Helmholtz only: the Laplace job carries ki = 0)
+	const bool ck = jb.ki != 0.0;
 
 	hipStream_t side = st;
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -832,7 +834,19 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 #define NB_QR(F, K, M) do { \
 		if (mapped) hipLaunchKernelGGL((quad_regular_kernel<F, K, M, true>), dim3(grid), dim3(256), 0, st, a); \
 		else hipLaunchKernelGGL((quad_regular_kernel<F, K, M, false>), dim3(grid), dim3(256), 0, st, a); } while (0)
-		if (jb.mode == OUT_BM) NB_QR(HELM, ALL4, OUT_BM);
+#define NB_QRC(K, M) do { \
+		if (mapped) hipLaunchKernelGGL((quad_regular_kernel<HELM, K, M, true, true>), dim3(grid), dim3(256), 0, st, a); \
+		else hipLaunchKernelGGL((quad_regular_kernel<HELM, K, M, false, true>), dim3(grid), dim3(256), 0, st, a); \
+		} while (0)
+		if (ck) {
+			if (jb.mode == OUT_BM) NB_QRC(ALL4, OUT_BM);
+			else switch (jb.km) {
+			case SLP: NB_QRC(SLP, OUT_SINGLE); break;
+			case DLP: NB_QRC(DLP, OUT_SINGLE); break;
+			case DLPT: NB_QRC(DLPT, OUT_SINGLE); break;
+			default: NB_QRC(HSP, OUT_SINGLE); break;
+			}
+		} else if (jb.mode == OUT_BM) NB_QR(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
 			case SLP: NB_QR(HELM, SLP, OUT_SINGLE); break;
@@ -849,6 +863,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 			}
 		}
 #undef NB_QR
+#undef NB_QRC
 		NB_HIP(hipGetLastError());
 		prof_end(pe, st);
 	}
@@ -895,7 +910,18 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 #define NB_Q64(F, K, M) do { \
 		if (diag) hipLaunchKernelGGL((quad_diag_kernel<F, K, M>), dim3(g), dim3(64), 0, side, na); \
 		else hipLaunchKernelGGL((quad_near_kernel<F, K, M>), dim3(g), dim3(64), 0, side, na); } while (0)
-		if (jb.mode == OUT_BM) NB_Q64(HELM, ALL4, OUT_BM);
+#define NB_Q64C(K, M) do { \
+		if (diag) hipLaunchKernelGGL((quad_diag_kernel<HELM, K, M, true>), dim3(g), dim3(64), 0, side, na); \
+		else hipLaunchKernelGGL((quad_near_kernel<HELM, K, M, true>), dim3(g), dim3(64), 0, side, na); } while (0)
+		if (ck) {
+			if (jb.mode == OUT_BM) NB_Q64C(ALL4, OUT_BM);
+			else switch (jb.km) {
+			case SLP: NB_Q64C(SLP, OUT_SINGLE); break;
+			case DLP: NB_Q64C(DLP, OUT_SINGLE); break;
+			case DLPT: NB_Q64C(DLPT, OUT_SINGLE); break;
+			default: NB_Q64C(HSP, OUT_SINGLE); break;
+			}
+		} else if (jb.mode == OUT_BM) NB_Q64(HELM, ALL4, OUT_BM);
 		else if (jb.family == NB_HELMHOLTZ) {
 			switch (jb.km) {
 			case SLP: NB_Q64(HELM, SLP, OUT_SINGLE); break;
@@ -912,6 +938,7 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 			}
 		}
 #undef NB_Q64
+#undef NB_Q64C
 		NB_HIP(hipGetLastError());
 		prof_end(pe, side);
 	}
@@ -1033,7 +1060,10 @@ int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int
 		const bool fix7 = jb.pol.rule == NB_RULE_FIXED && trial->qn[jb.pol.fixed_degree] == 7;
 		const bool mapped = jb.rmap || jb.cmap;
 #define NB_RG(F, K, M, Q) do { \
-		if (F == HELM && ck) hipLaunchKernelGGL((regular_kernel<F, K, M, Q, false, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+		if (F == HELM && ck && mapped) \
+			hipLaunchKernelGGL((regular_kernel<F, K, M, Q, true, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+		else if (F == HELM && ck) \
+			hipLaunchKernelGGL((regular_kernel<F, K, M, Q, false, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
 		else if (mapped) hipLaunchKernelGGL((regular_kernel<F, K, M, Q, true>), dim3(grid), dim3(256), 0, st, a); \
 		else hipLaunchKernelGGL((regular_kernel<F, K, M, Q, false>), dim3(grid), dim3(256), 0, st, a); } while (0)
 		if (jb.mode == OUT_BM) {
@@ -1183,9 +1213,9 @@ int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 	const int nt = mesh_parts(jb.test, tp, tid), ns = mesh_parts(jb.trial, sp, sid);
 	int s;
 	// refuse before any block has run
-	if (jb.ki != 0.0 || jb.pol.precision != NB_PREC_MIXED)
-		return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers and the FP64 precision policy are implemented for "
-			"collocation on meshes of triangles only");
+	if (jb.pol.precision != NB_PREC_MIXED)
+		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on meshes of "
+			"triangles only");
 	for (int b = 0; b < ns; ++b)
 		if (sp[b]->npe == 4 && (s = check_quad_policy(jb.pol))) return s;
 	// a test mesh with both types: the regular pairs of a trial part against
@@ -1908,8 +1938,6 @@ int nb_assemble_bm_ck(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const n
 {
 	if (!mesh) return fail(NB_ERR_ARG, "null mesh");
 	if (!(k.re > 0.0) || !std::isfinite(k.im)) return fail(NB_ERR_ARG, "wavenumber must have a positive real part");
-	if (mesh->npe != 3)
-		return fail(NB_ERR_UNSUPPORTED, "complex wavenumbers are implemented for collocation on triangles");
 	int s;
 	if ((s = current_device_matches(mesh))) return s;
 	if ((s = check_policy(policy))) return s;

@@ -830,18 +830,16 @@ __device__ __forceinline__ void wave_terms_c(double2 e, double2 z, double g0, do
 	}
 }
 
-// K_helm - K_lap for the kernels of MASK at one point (FP64), r = y - x
-template <int MASK>
-__device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4])
+// K_helm - K_lap for the kernels of MASK at one point (FP64), r = y - x;
+// CK: the complex wavenumber k + i ki
+template <int MASK, bool CK = false>
+__device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4],
+	double ki = 0.0)
 {
 	const D3 rv = d3sub(y, x);
 	const double r2 = d3dot(rv, rv);
 	const double rinv = rsqrt64(r2);
 	const double r = r2 * rinv;
-	const double kr = k * r;
-	double sn, cs;
-	sincos64(kr, &sn, &cs);
-	// e^{-ikr} = cs - i sn
 	const double g0 = rinv * wj * (1.0 / (4.0 * PI_D));
 	double rdny = 0.0, rdnx = 0.0, nn = 0.0;
 	if (MASK & (DLP | DLPT | HSP)) {
@@ -849,6 +847,16 @@ __device__ __forceinline__ void dynamic_point(D3 x, D3 y, D3 nx, D3 ny, double k
 		rdnx = -d3dot(rv, nx) * rinv;
 		if (MASK & HSP) nn = d3dot(nx, ny);
 	}
+	if (CK) {
+		double2 e, z;
+		wave64(k, ki, r, &e, &z);
+		wave_terms_c<MASK>(e, z, g0, rinv, rdny, rdnx, nn, 1.0, s);
+		return;
+	}
+	const double kr = k * r;
+	double sn, cs;
+	sincos64(kr, &sn, &cs);
+	// e^{-ikr} = cs - i sn
 	dynamic_terms<MASK>(cs, sn, kr, g0, rinv, rdny, rdnx, nn, s);
 }
 

@@ -55,6 +55,7 @@ struct QuadArgs {
 	int fixed_n;                 // > 0: fixed rule with fixed_n x fixed_n points
 	float k, k_revf;
 	double k_rev;
+	double ki;                   // imaginary part of a complex wavenumber (kernels with CK)
 	float2 alpha;
 	void *out0, *out1;
 	long long ld;
@@ -68,24 +69,33 @@ struct QuadCtx {
 	float nxx, nxy, nxz;         // test unit normal
 	float nyx, nyy, nyz;         // trial unit normal at the centre
 	float w1;                    // weight of the one-point rule: 4 J(0, 0) / (4 pi)
+	float ad;                    // e^{ki d} (complex wavenumber k + i ki; CK only)
 };
 
-// add one point's E = (ec, es), th = k r, 1/r, a = r.n_y, b = r.n_x, nn = n_x.n_y
-template <int FAM, int MASK>
+// add one point's E = (ec, es), th = k r, 1/r, a = r.n_y, b = r.n_x, nn = n_x.n_y;
+// CK (complex wavenumber k + i ki): E carries e^{ki r}, kih = ki r, and
+// i (k + i ki) r = -kih + i th, (k + i ki)^2 = k^2 - ki^2 + 2 i k ki
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void quad_add(float ec, float es, float th, float rinv, float a, float b, float nn, float k,
-	float2 (&K)[4])
+	float2 (&K)[4], float kih = 0.0f, float ki = 0.0f)
 {
 	if (MASK & SLP) { K[0].x += ec; K[0].y += es; }
 	if (MASK & (DLP | DLPT | HSP)) {
 		const float ri2 = rinv * rinv;
 		const float Gr = ec * ri2, Gi = es * ri2;
-		const float Qr = __builtin_fmaf(-th, Gi, Gr), Qi = __builtin_fmaf(th, Gr, Gi);
+		float Qr = __builtin_fmaf(-th, Gi, Gr), Qi = __builtin_fmaf(th, Gr, Gi);
+		if (CK) { Qr = __builtin_fmaf(-kih, Gr, Qr); Qi = __builtin_fmaf(-kih, Gi, Qi); }
 		if (MASK & DLP) { K[1].x = __builtin_fmaf(-a, Qr, K[1].x); K[1].y = __builtin_fmaf(-a, Qi, K[1].y); }
 		if (MASK & DLPT) { K[2].x = __builtin_fmaf(b, Qr, K[2].x); K[2].y = __builtin_fmaf(b, Qi, K[2].y); }
 		if (MASK & HSP) {
-			const float kk = FAM == HELM ? k * k : 0.0f;
+			const float kk = FAM == HELM ? (CK ? __builtin_fmaf(k, k, -ki * ki) : k * k) : 0.0f;
 			const float ab = a * b * ri2;
-			const float hr = __builtin_fmaf(3.0f, Qr, -kk * ec), hi = __builtin_fmaf(3.0f, Qi, -kk * es);
+			float hr = __builtin_fmaf(3.0f, Qr, -kk * ec), hi = __builtin_fmaf(3.0f, Qi, -kk * es);
+			if (CK) {
+				const float kq = 2.0f * k * ki;
+				hr = __builtin_fmaf(kq, es, hr);
+				hi = __builtin_fmaf(-kq, ec, hi);
+			}
 			K[3].x += __builtin_fmaf(nn, Qr, -ab * hr);
 			K[3].y += __builtin_fmaf(nn, Qi, -ab * hi);
 		}
@@ -94,10 +104,10 @@ __device__ __forceinline__ void quad_add(float ec, float es, float th, float rin
 
 // One-point Gauss rule (order 1): the point is the centre, so r = d with the
 // FP64-reduced phase, and the normal is the centre's.
-template <int FAM, int MASK>
-__device__ __forceinline__ void quad_centre(const QuadCtx &p, float k, float2 (&K)[4])
+template <int FAM, int MASK, bool CK = false>
+__device__ __forceinline__ void quad_centre(const QuadCtx &p, float k, float2 (&K)[4], float ki = 0.0f)
 {
-	const float g = p.w1 * p.rinv_d;
+	const float g = CK ? p.w1 * p.rinv_d * p.ad : p.w1 * p.rinv_d;
 	float ec = g, es = 0.0f, th = 0.0f;
 	if (FAM == HELM) {
 		ec = g * __builtin_amdgcn_cosf(p.t0);
@@ -108,7 +118,7 @@ __device__ __forceinline__ void quad_centre(const QuadCtx &p, float k, float2 (&
 	if (MASK & (DLP | HSP)) a = p.dcx * p.nyx + p.dcy * p.nyy + p.dcz * p.nyz;
 	if (MASK & (DLPT | HSP)) b = p.dcx * p.nxx + p.dcy * p.nxy + p.dcz * p.nxz;
 	if (MASK & HSP) nn = p.nxx * p.nyx + p.nxy * p.nyy + p.nxz * p.nyz;
-	quad_add<FAM, MASK>(ec, es, th, p.rinv_d, a, b, nn, k, K);
+	quad_add<FAM, MASK, CK>(ec, es, th, p.rinv_d, a, b, nn, k, K, CK ? ki * p.df : 0.0f, ki);
 }
 
 // Kernel values times the weights at the points of one trial element (FP32):
@@ -117,11 +127,13 @@ __device__ __forceinline__ void quad_centre(const QuadCtx &p, float k, float2 (&
 // nn = n_x.n_y at the point (the kernels of nihu/library/helmholtz_kernel.hpp:
 // 169-228 and normal_derivative_kernel.hpp:114-257 with r = y - x).  The
 // phase is t0 + (k / 2pi) u / (r + d) revolutions, u = r^2 - d^2 =
-// 2 (c_j - x).off + |off|^2, as in the triangle pass.
-template <int FAM, int MASK>
+// 2 (c_j - x).off + |off|^2, as in the triangle pass.  CK: the weight takes
+// e^{ki r} = e^{ki d} 2^{ki log2(e) (r - d)}.
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const QuadCtx &p, float k_revf, float k,
-	float2 (&K)[4])
+	float2 (&K)[4], float ki = 0.0f)
 {
+	const float kil2e = ki * 1.4426950408889634f;
 	const auto qf = uni((const float *)tab);
 	const float dcx2 = 2.0f * p.dcx, dcy2 = 2.0f * p.dcy, dcz2 = 2.0f * p.dcz;
 	for (int q = 0; q < nq; ++q) {
@@ -130,11 +142,16 @@ __device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const Qua
 		const float u = __builtin_fmaf(dcx2, ox, __builtin_fmaf(dcy2, oy, __builtin_fmaf(dcz2, oz, oo)));
 		const float r2 = p.d2f + u;
 		const float rinv = __builtin_amdgcn_rsqf(r2);
-		const float g = w * rinv;
-		float ec, es, th = 0.0f;
+		float g = w * rinv;
+		float ec, es, th = 0.0f, kih = 0.0f;
 		if (FAM == HELM) {
 			const float r = r2 * rinv;
-			const float t = __builtin_fmaf(k_revf, u * __builtin_amdgcn_rcpf(r + p.df), p.t0);
+			const float rd = u * __builtin_amdgcn_rcpf(r + p.df);
+			const float t = __builtin_fmaf(k_revf, rd, p.t0);
+			if (CK) {
+				g *= p.ad * exp2f(kil2e * rd);
+				kih = ki * r;
+			}
 			ec = g * __builtin_amdgcn_cosf(t);
 			es = -g * __builtin_amdgcn_sinf(t);
 			th = k * r;
@@ -147,17 +164,18 @@ __device__ __forceinline__ void quad_points(const QuadPt *tab, int nq, const Qua
 		if (MASK & (DLP | HSP)) a = rx * nqx + ry * nqy + rz * nqz;
 		if (MASK & (DLPT | HSP)) b = rx * p.nxx + ry * p.nxy + rz * p.nxz;
 		if (MASK & HSP) nn = p.nxx * nqx + p.nxy * nqy + p.nxz * nqz;
-		quad_add<FAM, MASK>(ec, es, th, rinv, a, b, nn, k, K);
+		quad_add<FAM, MASK, CK>(ec, es, th, rinv, a, b, nn, k, K, kih, ki);
 	}
 }
 
 // Lanes at their own Gauss order n (n <= 0: idle), one distinct order at a
 // time so that the table address stays wave-uniform.
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, const QuadCtx &p, float2 (&K)[4])
 {
+	const float ki = (float)a.ki;
 	if (__ballot(n == 1) != 0) {
-		if (n == 1) quad_centre<FAM, MASK>(p, a.k, K);
+		if (n == 1) quad_centre<FAM, MASK, CK>(p, a.k, K, ki);
 		n = n == 1 ? 0 : n;
 	}
 	if (__ballot(n > 0) == 0) return;
@@ -169,7 +187,7 @@ __device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, con
 		const unsigned long long msel = __ballot(n == nsel);
 		if ((msel >> __lane_id()) & 1ull) {
 			const int nq = nsel * nsel;
-			quad_points<FAM, MASK>(a.qt[nsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, K);
+			quad_points<FAM, MASK, CK>(a.qt[nsel] + (size_t)j * nq, nq, p, a.k_revf, a.k, K, ki);
 			n = 0;
 		}
 	}
@@ -181,11 +199,12 @@ __device__ __forceinline__ void quad_orders(const QuadArgs &a, int j, int n, con
 // order for this lane is the current one takes its kernel values by a select
 // (every group is served once, so its value is the rule's sum from -0 in
 // point order, as a separate evaluation per group gives it).
-template <int FAM, int KM>
+template <int FAM, int KM, bool CK = false>
 __device__ __forceinline__ void quad_groups(const QuadArgs &a, int j, int nS, int nD, int nH, const QuadCtx &p,
 	float2 (&K)[4])
 {
 	constexpr int NONE = 1 << 20;
+	const float ki = (float)a.ki;
 	int pS = (KM & SLP) && nS > 0 ? nS : NONE;
 	int pD = (KM & (DLP | DLPT)) && nD > 0 ? nD : NONE;
 	int pH = (KM & HSP) && nH > 0 ? nH : NONE;
@@ -198,8 +217,9 @@ __device__ __forceinline__ void quad_groups(const QuadArgs &a, int j, int nS, in
 		const bool fS = pS == nsel, fD = pD == nsel, fH = pH == nsel;
 		float2 t[4] = {zero2(), zero2(), zero2(), zero2()};
 		if (fS || fD || fH) {
-			if (nsel == 1) quad_centre<FAM, KM>(p, a.k, t);
-			else quad_points<FAM, KM>(a.qt[nsel] + (size_t)j * (nsel * nsel), nsel * nsel, p, a.k_revf, a.k, t);
+			if (nsel == 1) quad_centre<FAM, KM, CK>(p, a.k, t, ki);
+			else quad_points<FAM, KM, CK>(a.qt[nsel] + (size_t)j * (nsel * nsel), nsel * nsel, p, a.k_revf, a.k, t,
+				ki);
 			if (fS) K[0] = t[0];
 			if (fD) { K[1] = t[1]; K[2] = t[2]; }
 			if (fH) K[3] = t[3];
@@ -215,7 +235,7 @@ __device__ __forceinline__ int gauss_order(int degree) { return degree / 2 + 1; 
 
 // MAP: stores through a.rmap / a.cmap (a block of a mesh with both element
 // types); without it the plain j * ld + i
-template <int FAM, int KM, int MODE, bool MAP>
+template <int FAM, int KM, int MODE, bool MAP, bool CK = false>
 __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 {
 	const int tile_r = blockIdx.x % a.row_tiles;
@@ -264,9 +284,10 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 		} else {
 			p.t0 = 0.0f;
 		}
+		p.ad = CK ? exp2f((float)(d * (a.ki * 1.4426950408889634))) : 1.0f;
 		float2 K[4] = {zero2(), zero2(), zero2(), zero2()};
 		if (a.fixed_n > 0) {
-			quad_orders<FAM, KM>(a, j, act ? a.fixed_n : 0, p, K);
+			quad_orders<FAM, KM, CK>(a, j, act ? a.fixed_n : 0, p, K);
 		} else {
 			// Gauss order per kernel group: degree 1 + table degree; beyond the
 			// last breakpoint (22.3 R) every table gives 0, order 1 (one point),
@@ -274,7 +295,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 			if (__ballot(act && d2 < h.far) == 0) {
 				// no pair of the wave inside 22.3 R: the one-point rule for all
 				// kernels at once (inactive lanes store nothing)
-				quad_centre<FAM, KM>(p, a.k, K);
+				quad_centre<FAM, KM, CK>(p, a.k, K, (float)a.ki);
 			} else {
 				// the distance-table thresholds of the record
 				double t[NTHR];
@@ -283,7 +304,7 @@ __global__ void __launch_bounds__(256) quad_regular_kernel(QuadArgs a)
 				const int nS = (KM & SLP) && act ? gauss_order(1 + table_degree_rec<1>(d2, t)) : 0;
 				const int nD = (KM & (DLP | DLPT)) && act ? gauss_order(1 + table_degree_rec<2>(d2, t)) : 0;
 				const int nH = (KM & HSP) && act ? gauss_order(1 + table_degree_rec<3>(d2, t)) : 0;
-				quad_groups<FAM, KM>(a, j, nS, nD, nH, p, K);
+				quad_groups<FAM, KM, CK>(a, j, nS, nD, nH, p, K);
 			}
 		}
 		if (act) {
@@ -340,12 +361,26 @@ __device__ __forceinline__ D3 d3add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, 
 
 // Kernels of MASK (FAM = Helmholtz or Laplace) times wj at one point, FP64,
 // added to s: SLP s[0], DLP s[1], DLPt s[2], HSP s[3] (1 / (4 pi) included).
-template <int FAM, int MASK>
-__device__ __forceinline__ void kernel_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4])
+// CK: Helmholtz with the complex wavenumber k + i ki.
+template <int FAM, int MASK, bool CK = false>
+__device__ __forceinline__ void kernel_point(D3 x, D3 y, D3 nx, D3 ny, double k, double wj, double2 (&s)[4],
+	double ki = 0.0)
 {
 	const D3 rv = d3sub(y, x);
 	const double r2 = d3dot(rv, rv);
 	const double rinv = rsqrt64(r2);
+	if (FAM == HELM && CK) {
+		double2 e, z;
+		wave64(k, ki, r2 * rinv, &e, &z);
+		double rdny = 0.0, rdnx = 0.0, nn = 0.0;
+		if (MASK & (DLP | DLPT | HSP)) {
+			rdny = d3dot(rv, ny) * rinv;
+			rdnx = -d3dot(rv, nx) * rinv;
+			if (MASK & HSP) nn = d3dot(nx, ny);
+		}
+		wave_terms_c<MASK>(e, z, rinv * wj * (1.0 / (4.0 * PI_D)), rinv, rdny, rdnx, nn, 0.0, s);
+		return;
+	}
 	double cs = 1.0, sn = 0.0, kr = 0.0;
 	if (FAM == HELM) {
 		kr = k * (r2 * rinv);
@@ -372,30 +407,32 @@ __device__ __forceinline__ void kernel_point(D3 x, D3 y, D3 nx, D3 ny, double k,
 }
 
 // kernels at the point (xi, eta) of the element, times J(xi) w
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ __forceinline__ void quad_kernel_at(const Quad &Q, double xi, double eta, D3 x0, D3 nx0, double k,
-	double w, double2 (&s)[4])
+	double w, double2 (&s)[4], double ki = 0.0)
 {
 	D3 a, b;
 	quad_dx(Q, xi, eta, a, b);
 	const D3 n = d3cross(a, b);
 	const double n2 = d3dot(n, n);
 	const double Jinv = rsqrt64(n2);
-	kernel_point<FAM, MASK>(x0, quad_x(Q, xi, eta), nx0, d3scale(n, Jinv), k, n2 * Jinv * w, s);
+	kernel_point<FAM, MASK, CK>(x0, quad_x(Q, xi, eta), nx0, d3scale(n, Jinv), k, n2 * Jinv * w, s, ki);
 }
 
 // Singular SLP / DLP / DLPt: the order-7 Duffy rule around the centre
-template <int FAM, int MASK>
-__device__ void quad_blind(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4])
+template <int FAM, int MASK, bool CK = false>
+__device__ void quad_blind(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4], double ki = 0.0)
 {
-	for (int q = 0; q < 64; ++q) quad_kernel_at<FAM, MASK>(Q, R.duffy[q][0], R.duffy[q][1], x0, nx0, k, R.duffy[q][2], s);
+	for (int q = 0; q < 64; ++q)
+		quad_kernel_at<FAM, MASK, CK>(Q, R.duffy[q][0], R.duffy[q][1], x0, nx0, k, R.duffy[q][2], s, ki);
 }
 
 // Guiggiani's finite-part integral of the HSP kernel at the centre, radial
 // order 5 (3 points), angular order 9 (5 points); x0 = x(0, 0), n_x the unit
-// normal there.
-template <int FAM>
-__device__ double2 quad_guiggiani(const Quad &Q, double k, const Rules &R)
+// normal there.  A complex wavenumber (CK) leaves the Laurent terms as they
+// are: it enters the HSP kernel first at order 1 / r, bounded under rho.
+template <int FAM, bool CK = false>
+__device__ double2 quad_guiggiani(const Quad &Q, double k, const Rules &R, double ki = 0.0)
 {
 	D3 dxi, deta;
 	quad_dx(Q, 0.0, 0.0, dxi, deta);
@@ -461,7 +498,7 @@ __device__ double2 quad_guiggiani(const Quad &Q, double k, const Rules &R)
 				quad_dx(Q, xi, eta, a, b);
 				const D3 nv = d3cross(a, b);
 				const double J = d3norm(nv);
-				kernel_point<FAM, HSP>(x0, quad_x(Q, xi, eta), n0, d3scale(nv, 1.0 / J), k, J / detT * rho, F);
+				kernel_point<FAM, HSP, CK>(x0, quad_x(Q, xi, eta), n0, d3scale(nv, 1.0 / J), k, J / detT * rho, F, ki);
 				const double sing = (Fm1 + Fm2 / rho) / rho;
 				I.x += (F[3].x - sing) * (wt * wr);
 				I.y += F[3].y * (wt * wr);
@@ -514,9 +551,10 @@ __device__ bool quad_inverse(const Quad &Q, D3 x0, double tol, int max_iter, dou
 // arrays), plus the planar integral over the linearised element.
 // lane / nl: this lane takes every nl-th angular line, planar-part point and
 // dynamic point of the pair (the caller sums the lanes).
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, const double *grx, const double *grw,
-	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4], int lane, int nl, int max_iter)
+	int nt, const double *gtx, const double *gtw, const Rules &R, double2 (&s)[4], int lane, int nl, int max_iter,
+	double ki = 0.0)
 {
 	double xi0, eta0, zeta;
 	if (!quad_inverse(Q, x0, 1e-5, max_iter, xi0, eta0, zeta)) return false;
@@ -560,9 +598,9 @@ __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, 
 				const double rho = (1.0 + grx[ir]) * rho_lim / 2.0;
 				const double w = wt * (grw[ir] * rho_lim / 2.0) * rho;
 				const double dxi = rho * ct, deta = rho * st;
-				quad_kernel_at<FAM, MASK>(Q, dxi + xi0, deta + eta0, x0, nx0, k, w, s);
+				quad_kernel_at<FAM, MASK, CK>(Q, dxi + xi0, deta + eta0, x0, nx0, k, w, s, ki);
 				const D3 yl = d3add(xl0, d3add(d3scale(a, dxi), d3scale(b, deta)));
-				kernel_point<FAM, MASK>(x0, yl, nx0, nul, k, -Jl * w, s);
+				kernel_point<FAM, MASK, CK>(x0, yl, nx0, nul, k, -Jl * w, s, ki);
 			}
 		}
 	}
@@ -575,7 +613,7 @@ __device__ bool quad_near_polar(const Quad &Q, D3 x0, D3 nx0, double k, int nr, 
 		for (int q = lane; q < 64; q += nl) {
 			const int i = q >> 3, j = q & 7;
 			const D3 yl = d3add(xl0, d3add(d3scale(a, R.gl8x[i] - xi0), d3scale(b, R.gl8x[j] - eta0)));
-			dynamic_point<MASK>(x0, yl, nx0, nul, k, R.gl8w[i] * R.gl8w[j] * Jl, s);
+			dynamic_point<MASK, CK>(x0, yl, nx0, nul, k, R.gl8w[i] * R.gl8w[j] * Jl, s, ki);
 		}
 	return true;
 }
@@ -596,9 +634,9 @@ __device__ __forceinline__ void telles_coeffs(double e, double rb, double &ca, d
 	cd = -cb;
 }
 
-template <int FAM, int MASK>
+template <int FAM, int MASK, bool CK = false>
 __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const Rules &R, double2 (&s)[4], int lane,
-	int nl, int max_iter)
+	int nl, int max_iter, double ki = 0.0)
 {
 	double xi0, eta0, zeta;
 	if (!quad_inverse(Q, x0, 1e-6, max_iter, xi0, eta0, zeta)) return false;
@@ -618,7 +656,7 @@ __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const R
 			const double y = R.gl16x[j];
 			const double eta = a1 * y * y * y + b1 * y * y + c1 * y + e1;
 			const double jy = 3.0 * a1 * y * y + 2.0 * b1 * y + c1;
-			quad_kernel_at<FAM, MASK>(Q, xi, eta, x0, nx0, k, R.gl16w[i] * R.gl16w[j] * jx * jy, s);
+			quad_kernel_at<FAM, MASK, CK>(Q, xi, eta, x0, nx0, k, R.gl16w[i] * R.gl16w[j] * jx * jy, s, ki);
 		}
 	}
 	return true;
@@ -658,7 +696,7 @@ __device__ __forceinline__ void quad_load(const NearArgs &a, int i, int j, Quad 
 }
 
 // Diagonal pairs: one lane per element (Duffy rule, Guiggiani).
-template <int FAM, int KM, int MODE>
+template <int FAM, int KM, int MODE, bool CK = false>
 __global__ void __launch_bounds__(64) quad_diag_kernel(NearArgs a)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -668,8 +706,8 @@ __global__ void __launch_bounds__(64) quad_diag_kernel(NearArgs a)
 	D3 x0, nx;
 	quad_load(a, p, p, Q, x0, nx);
 	double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-	if (KM & (SLP | DLP | DLPT)) quad_blind<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, a.k, R, s);
-	if (KM & HSP) s[3] = quad_guiggiani<FAM>(Q, a.k, R);
+	if (KM & (SLP | DLP | DLPT)) quad_blind<FAM, KM & (SLP | DLP | DLPT), CK>(Q, x0, nx, a.k, R, s, a.ki);
+	if (KM & HSP) s[3] = quad_guiggiani<FAM, CK>(Q, a.k, R, a.ki);
 	quad_store64<KM, MODE, true>(a, p, p, s);
 }
 
@@ -678,7 +716,7 @@ __global__ void __launch_bounds__(64) quad_diag_kernel(NearArgs a)
 // butterfly sum over the lanes; the pair's first lane stores.
 constexpr int QLPP = 4;
 
-template <int FAM, int KM, int MODE>
+template <int FAM, int KM, int MODE, bool CK = false>
 __global__ void __launch_bounds__(64) quad_near_kernel(NearArgs a)
 {
 	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -695,10 +733,11 @@ __global__ void __launch_bounds__(64) quad_near_kernel(NearArgs a)
 	bool ok = true;
 	if (FAM == HELM) {
 		if (KM & SLP)
-			ok = quad_near_polar<FAM, SLP>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s, lane, QLPP,
-				a.inv_iter);
+			ok = quad_near_polar<FAM, SLP, CK>(Q, x0, nx, a.k, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s, lane, QLPP,
+				a.inv_iter, a.ki);
 		if (ok && (KM & (DLP | DLPT | HSP)))
-			ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP)>(Q, x0, nx, a.k, R, s, lane, QLPP, a.inv_iter);
+			ok = quad_near_telles<FAM, KM & (DLP | DLPT | HSP), CK>(Q, x0, nx, a.k, R, s, lane, QLPP, a.inv_iter,
+				a.ki);
 	} else {
 		if (KM & (SLP | DLP | DLPT))
 			ok = quad_near_polar<FAM, KM & (SLP | DLP | DLPT)>(Q, x0, nx, 0.0, 8, R.gl8x, R.gl8w, 8, R.gl8x, R.gl8w, R, s,

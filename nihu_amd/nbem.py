@@ -278,7 +278,7 @@ def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.
     """out (+)= alpha * operator; formalism "collocation" (dirac(constant) x
     constant), "galerkin" (constant x constant) or "galerkin_linear"
     (isoparametric x isoparametric, rows / columns = mesh nodes).  k may be
-    complex (collocation on triangle meshes)."""
+    complex (collocation on every mesh type, Galerkin on triangles)."""
     pol = pol or policy(**NIHU_POLICY)
     same = (test is trial) if same_mesh is None else bool(same_mesh)
     a = complex(alpha)
@@ -292,7 +292,7 @@ def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.
 
 def assemble_bm(mesh: Mesh, k, A, B, alpha=None, pol=None, accumulate=False, stream=None):
     """Burton-Miller A and B; alpha defaults to -i/k.  A complex k goes to
-    nb_assemble_bm_ck (triangle meshes)."""
+    nb_assemble_bm_ck."""
     pol = pol or policy(**NIHU_POLICY)
     ck = isinstance(k, (complex, np.complexfloating))
     if alpha is None:

@@ -933,7 +933,8 @@ static const std::vector<QP> &gauss_quad_store(int degree)
 }
 
 // kernel at a trial point of a quadrilateral times the Jacobian |n(xi)|
-static inline cplx q_kernel_jac(int family, int nx, int ny, double k, V3 x, V3 nxv, QElem const &e, double xi,
+template <class K>
+static inline cplx q_kernel_jac(int family, int nx, int ny, K k, V3 x, V3 nxv, QElem const &e, double xi,
 	double eta)
 {
 	V3 y = q_x(e, xi, eta);
@@ -945,7 +946,8 @@ static inline cplx q_kernel_jac(int family, int nx, int ny, double k, V3 x, V3 n
 // Regular pair: sum_q K(x, y_q) J(xi_q) w_q.  The degree is the field
 // complexity (jacobian order 1 of quad_1: nihu/core/complexity_estimator.hpp:
 // 48-62) plus the distance interval of the kernel (NiHu policy), or fixed.
-static cplx quad_regular(int family, int nx, int ny, double k, V3 x, V3 nxv, QElem const &e, int degree)
+template <class K>
+static cplx quad_regular(int family, int nx, int ny, K k, V3 x, V3 nxv, QElem const &e, int degree)
 {
 	if (degree > 14)
 		throw std::out_of_range("Too high quadrature degree selected for collocational integration");
@@ -990,7 +992,8 @@ static std::vector<QP> quad_duffy_face(int degree, double x0, double y0)
 // (singular_quadrature_order, nihu/library/helmholtz_kernel.hpp:69-71) around
 // the centre; nihu/core/singular_accelerator.hpp:363-456 and
 // nihu/core/double_integral.hpp:470-510 (eval_singular_on_accelerator).
-static cplx quad_singular_blind(int family, int nx, int ny, double k, QElem const &e)
+template <class K>
+static cplx quad_singular_blind(int family, int nx, int ny, K k, QElem const &e)
 {
 	static const std::vector<QP> rule = quad_duffy_face(7, 0.0, 0.0);
 	cplx res = 0.0;
@@ -1009,8 +1012,12 @@ static cplx quad_singular_blind(int family, int nx, int ny, double k, QElem cons
 // nihu/library/laplace_kernel.hpp:266-295 (also used for Helmholtz HSP, whose
 // singular core is Laplace's).  Radial rule of degree rad_order, angular rule
 // of degree tan_order (5 and 9 in NiHu's HSP shortcuts,
-// nihu/library/helmholtz_singular_integrals.hpp:1190-1236).
-static cplx quad_guiggiani(int family, double k, QElem const &e, double xi0x, double xi0y, int rad_order,
+// nihu/library/helmholtz_singular_integrals.hpp:1190-1236).  A complex
+// wavenumber leaves the Laurent terms as they are: it enters the expansion of
+// the HSP kernel first at order 1 / r (the k^2 term), which the polar Jacobian
+// rho makes bounded.
+template <class K>
+static cplx quad_guiggiani(int family, K k, QElem const &e, double xi0x, double xi0y, int rad_order,
 	int tan_order)
 {
 	V3 dxi, deta;
@@ -1150,7 +1157,8 @@ static QElem quad_linearized(QElem const &e, double xi0, double eta0)
 // the 8 x 8-point Gauss rule (degree 14) of K_helm - K_lap;
 // nihu/library/helmholtz_nearly_singular_integrals.hpp:40-219 with
 // elem_t = quad_1_elem, nihu/library/laplace_nearly_singular_integrals.hpp:62-389.
-static cplx quad_planar_near(int family, int nx, int ny, double k, QElem const &lin, V3 x0, V3 nxv)
+template <class K>
+static cplx quad_planar_near(int family, int nx, int ny, K k, QElem const &lin, V3 x0, V3 nxv)
 {
 	V3 C[4];
 	for (int i = 0; i < 4; ++i) C[i] = q_x(lin, QCX[i], QCY[i]);
@@ -1171,7 +1179,8 @@ static cplx quad_planar_near(int family, int nx, int ny, double k, QElem const &
 // the planar integral over the linearised element;
 // nihu/library/nearly_singular_collocational.hpp:71-203 (inverse mapping
 // tolerance 1e-5, 100 iterations).
-static cplx quad_near_polar(int family, int nx, int ny, double k, QElem const &e, V3 x0, V3 nxv, int rad_order,
+template <class K>
+static cplx quad_near_polar(int family, int nx, int ny, K k, QElem const &e, V3 x0, V3 nxv, int rad_order,
 	int tan_order)
 {
 	double xi0, eta0, zeta;
@@ -1253,7 +1262,8 @@ static std::vector<QP> telles_rule(int degree, double xi0, double eta0, double d
 // Near-singular integral with the Telles-transformed Gauss rule of degree 30
 // towards the image of the collocation point (inverse mapping tolerance 1e-6);
 // nihu/library/nearly_singular_collocational_telles.hpp:66-103.
-static cplx quad_near_telles(int family, int nx, int ny, double k, QElem const &e, V3 x0, V3 nxv, int degree)
+template <class K>
+static cplx quad_near_telles(int family, int nx, int ny, K k, QElem const &e, V3 x0, V3 nxv, int degree)
 {
 	double xi0, eta0, zeta;
 	if (!quad_inverse_map(e, x0, 1e-6, 100, xi0, eta0, zeta))
@@ -1271,7 +1281,8 @@ static cplx quad_near_telles(int family, int nx, int ny, double k, QElem const &
 // 461-505,560-604); Laplace SLP / DLP / DLPt polar (15, 15), HSP polar
 // (100, 100) (nihu/library/laplace_nearly_singular_integrals.hpp:443-486,
 // 539-582,634-677,730-773).
-static cplx quad_near(int family, int nx, int ny, double k, QElem const &e, V3 x0, V3 nxv)
+template <class K>
+static cplx quad_near(int family, int nx, int ny, K k, QElem const &e, V3 x0, V3 nxv)
 {
 	if (family == 1 && nx + ny > 0) return quad_near_telles(family, nx, ny, k, e, x0, nxv, 30);
 	if (family == 0 && nx == 1 && ny == 1) return quad_near_polar(family, nx, ny, k, e, x0, nxv, 100, 100);
@@ -1283,7 +1294,8 @@ static cplx quad_near(int family, int nx, int ny, double k, QElem const &e, V3 x
 // 1190-1236, laplace_singular_integrals.hpp:1279-1318; DLP / DLPt are not
 // zero on quadrilaterals: normal_derivative_singular_integrals.hpp:13-58
 // covers tria_1 only).
-static cplx quad_singular(int family, int nx, int ny, double k, QElem const &e)
+template <class K>
+static cplx quad_singular(int family, int nx, int ny, K k, QElem const &e)
 {
 	if (nx == 1 && ny == 1) return quad_guiggiani(family, k, e, 0.0, 0.0, 5, 9);
 	return quad_singular_blind(family, nx, ny, k, e);
@@ -1300,7 +1312,8 @@ static bool near_needed_q(Policy const &p, double d2, QElem const &trial)
 // Pair integral with classification for a quadrilateral trial element; the
 // regular degree is 1 (field complexity) + the interval degree, or the fixed
 // degree of the policy (a Gauss square rule of that degree).
-static cplx quad_pair_integral(int family, int nx, int ny, double k, Policy const &p, TestPt const &t,
+template <class K>
+static cplx quad_pair_integral(int family, int nx, int ny, K k, Policy const &p, TestPt const &t,
 	QElem const &trial, bool same_mesh)
 {
 	if (same_mesh && t.id == trial.id) return quad_singular(family, nx, ny, k, trial);
@@ -1752,11 +1765,12 @@ int orc_galerkin_block(int family, int nx, int ny, double k, const orc_policy *p
 }
 
 
-// Collocation with constant fields on meshes of triangles (npe = 3) or
-// quadrilaterals (npe = 4); test and trial meshes may differ in type.
-int orc_assemble_elems(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
+} // extern "C"
+
+template <class K>
+static int assemble_elems_impl(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
 	const double *sxyz, int sn, const int *sel, int snt, int snpe, int same_mesh,
-	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	int family, int nx, int ny, K k, double alpha_re, double alpha_im,
 	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
 {
 	ORC_TRY
@@ -1803,6 +1817,29 @@ int orc_assemble_elems(const double *txyz, int tn, const int *tel, int tnt, int 
 	ORC_CATCH
 }
 
+extern "C" {
+
+// Collocation with constant fields on meshes of triangles (npe = 3) or
+// quadrilaterals (npe = 4); test and trial meshes may differ in type.
+int orc_assemble_elems(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
+	const double *sxyz, int sn, const int *sel, int snt, int snpe, int same_mesh,
+	int family, int nx, int ny, double k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	return assemble_elems_impl(txyz, tn, tel, tnt, tnpe, sxyz, sn, sel, snt, snpe, same_mesh, family, nx, ny, k,
+		alpha_re, alpha_im, pol, rows, n_rows, out, ld, accumulate, n_threads);
+}
+
+// the same with a complex wavenumber k_re + i k_im
+int orc_assemble_elems_ck(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
+	const double *sxyz, int sn, const int *sel, int snt, int snpe, int same_mesh,
+	int family, int nx, int ny, double k_re, double k_im, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	return assemble_elems_impl(txyz, tn, tel, tnt, tnpe, sxyz, sn, sel, snt, snpe, same_mesh, family, nx, ny,
+		cplx(k_re, k_im), alpha_re, alpha_im, pol, rows, n_rows, out, ld, accumulate, n_threads);
+}
+
 // Burton-Miller A and B (as orc_assemble_bm) on a mesh of npe-node elements
 int orc_assemble_bm_elems(const double *xyz, int nn, const int *el, int ne, int npe, double k, double alpha_re,
 	double alpha_im, const orc_policy *pol, const int *rows, int n_rows, double *A, double *B, int64_t ld, int n_threads)
@@ -1828,6 +1865,32 @@ int orc_assemble_bm_elems(const double *xyz, int nn, const int *el, int ne, int 
 	return 0;
 }
 
+// the same with a complex wavenumber k_re + i k_im (alpha as given)
+int orc_assemble_bm_elems_ck(const double *xyz, int nn, const int *el, int ne, int npe, double k_re, double k_im,
+	double alpha_re, double alpha_im, const orc_policy *pol, const int *rows, int n_rows, double *A, double *B,
+	int64_t ld, int n_threads)
+{
+	int nr = rows ? n_rows : ne;
+	for (int j = 0; j < ne; ++j)
+		for (int r = 0; r < nr; ++r) {
+			double *a = A + 2 * ((int64_t)j * ld + r), *b = B + 2 * ((int64_t)j * ld + r);
+			a[0] = a[1] = b[0] = b[1] = 0.0;
+		}
+	cplx al(alpha_re, alpha_im);
+	int s;
+	auto op = [&](int nx, int ny, cplx sc, double *out) {
+		return orc_assemble_elems_ck(xyz, nn, el, ne, npe, xyz, nn, el, ne, npe, 1, 1, nx, ny, k_re, k_im, sc.real(),
+			sc.imag(), pol, rows, n_rows, out, ld, 1, n_threads);
+	};
+	if ((s = op(0, 0, 1.0, B))) return s;
+	if ((s = op(1, 0, al, B))) return s;
+	orc_identity(ne, (al / 2.).real(), (al / 2.).imag(), rows, n_rows, B, ld);
+	if ((s = op(1, 1, al, A))) return s;
+	orc_identity(ne, -.5, 0.0, rows, n_rows, A, ld);
+	if ((s = op(0, 1, 1.0, A))) return s;
+	return 0;
+}
+
 // Element-level quadrilateral integrals for the tests.  coords12: the four
 // corners; x0, nx0: collocation point and unit normal (what 0-3, 7, 8);
 // xi0: a point of the reference square (what 4-6).
@@ -1836,7 +1899,10 @@ int orc_assemble_bm_elems(const double *xyz, int nn, const int *el, int ne, int 
 //   2 polar near, orders (o1, o2)       6 inverse mapping of x0: (xi, eta, zeta, ok)
 //   3 Telles near, degree o1            7 near dispatch (NiHu's choice per kernel)
 //   8 planar integral over the element itself (planar elements only)
-int orc_quad_elem(int what, int family, int nx, int ny, double k, const double *coords12, const double *x0p,
+} // extern "C"
+
+template <class K>
+static int quad_elem_impl(int what, int family, int nx, int ny, K k, const double *coords12, const double *x0p,
 	const double *nx0p, const double *xi0p, int o1, int o2, double *out)
 {
 	ORC_TRY
@@ -1872,6 +1938,21 @@ int orc_quad_elem(int what, int family, int nx, int ny, double k, const double *
 	}
 	out[0] = v.real(); out[1] = v.imag();
 	ORC_CATCH
+}
+
+extern "C" {
+
+int orc_quad_elem(int what, int family, int nx, int ny, double k, const double *coords12, const double *x0p,
+	const double *nx0p, const double *xi0p, int o1, int o2, double *out)
+{
+	return quad_elem_impl(what, family, nx, ny, k, coords12, x0p, nx0p, xi0p, o1, o2, out);
+}
+
+// the same with a complex wavenumber k_re + i k_im
+int orc_quad_elem_ck(int what, int family, int nx, int ny, double k_re, double k_im, const double *coords12,
+	const double *x0p, const double *nx0p, const double *xi0p, int o1, int o2, double *out)
+{
+	return quad_elem_impl(what, family, nx, ny, cplx(k_re, k_im), coords12, x0p, nx0p, xi0p, o1, o2, out);
 }
 
 // Guiggiani / Duffy / Telles rules for the tests: the Duffy rule of degree o1

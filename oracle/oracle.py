@@ -120,6 +120,13 @@ def _declare(l):
         l.orc_assemble_bm_elems.argtypes = [D, C.c_int, I, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                             P, I, C.c_int, D, D, C.c_int64, C.c_int]
         l.orc_quad_elem.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D, D, D, C.c_int, C.c_int, D]
+        l.orc_assemble_elems_ck.argtypes = [D, C.c_int, I, C.c_int, C.c_int, D, C.c_int, I, C.c_int, C.c_int,
+                                            C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_assemble_bm_elems_ck.argtypes = [D, C.c_int, I, C.c_int, C.c_int, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, P, I, C.c_int, D, D, C.c_int64, C.c_int]
+        l.orc_quad_elem_ck.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D,
+                                       C.c_int, C.c_int, D]
         l.orc_quad_duffy_rule.argtypes = [C.c_int, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_telles_rule.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_geometry_quad.argtypes = [D, C.c_int, I, C.c_int, D, D, D, D]
@@ -225,10 +232,14 @@ def assemble(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, rows=None,
         out = np.zeros((nr, len(s_tris)), dtype=np.complex128, order="F")
         accumulate = False
     alpha = complex(alpha)
-    if _is_complex(k):
-        # complex wavenumber: constant triangles only
-        if t_tris.shape[1] != 3 or s_tris.shape[1] != 3:
-            raise ValueError("complex wavenumbers: triangle meshes only")
+    if _is_complex(k) and (t_tris.shape[1] != 3 or s_tris.shape[1] != 3):
+        # complex wavenumber, quadrilaterals on either side
+        _check(lib().orc_assemble_elems_ck(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris), t_tris.shape[1],
+                                           _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), s_tris.shape[1],
+                                           int(same), family, nx, ny, k.real, k.imag, alpha.real, alpha.imag,
+                                           C.byref(pol), _i(rows_a), nr, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                           out.shape[0], int(accumulate), threads))
+    elif _is_complex(k):
         _check(lib().orc_assemble_ck(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
                                      _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
                                      family, nx, ny, k.real, k.imag, alpha.real, alpha.imag, C.byref(pol), _i(rows_a),
@@ -253,14 +264,19 @@ def assemble_bm(nodes, tris, k, alpha=None, pol=None, rows=None, threads=0):
     """Burton-Miller (A, B) in NiHu's composition order; alpha defaults to -i/k."""
     pol = pol or policy()
     nodes, tris = _mesh(nodes, tris)
-    alpha = complex(0.0, -1.0 / k) if alpha is None else complex(alpha)
+    if alpha is None:
+        alpha = -1j / complex(k) if _is_complex(k) else complex(0.0, -1.0 / k)
+    alpha = complex(alpha)
     rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
     nr = len(tris) if rows_a is None else len(rows_a)
     A = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
     B = np.zeros((nr, len(tris)), dtype=np.complex128, order="F")
-    if _is_complex(k):
-        if tris.shape[1] != 3:
-            raise ValueError("complex wavenumbers: triangle meshes only")
+    if _is_complex(k) and tris.shape[1] != 3:
+        _check(lib().orc_assemble_bm_elems_ck(_d(nodes), len(nodes), _i(tris), len(tris), tris.shape[1], k.real,
+                                              k.imag, alpha.real, alpha.imag, C.byref(pol), _i(rows_a), nr,
+                                              A.ctypes.data_as(C.POINTER(C.c_double)),
+                                              B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
+    elif _is_complex(k):
         _check(lib().orc_assemble_bm_ck(_d(nodes), len(nodes), _i(tris), len(tris), k.real, k.imag, alpha.real,
                                         alpha.imag, C.byref(pol), _i(rows_a), nr,
                                         A.ctypes.data_as(C.POINTER(C.c_double)),
@@ -375,8 +391,11 @@ def quad_elem(what, family, nx, ny, k, coords, x0=None, nx0=None, xi0=None, o1=0
     cs = np.ascontiguousarray(coords, dtype=np.float64).reshape(12)
     arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (x0, nx0, xi0)]
     out = np.zeros(4)
-    _check(lib().orc_quad_elem(what, family, nx, ny, k, _d(cs), *[None if a is None else _d(a) for a in arrs],
-                               o1, o2, _d(out)))
+    ptrs = [None if a is None else _d(a) for a in arrs]
+    if _is_complex(k):
+        _check(lib().orc_quad_elem_ck(what, family, nx, ny, k.real, k.imag, _d(cs), *ptrs, o1, o2, _d(out)))
+    else:
+        _check(lib().orc_quad_elem(what, family, nx, ny, k, _d(cs), *ptrs, o1, o2, _d(out)))
     if what == QUAD_INVERSE:
         return out.copy()
     return complex(out[0], out[1])

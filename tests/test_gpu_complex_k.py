@@ -6,8 +6,11 @@ pairs with the attenuation e^{ki r} on the weights: relative Frobenius
 <= 1e-5) and under the FP64 precision policy (<= 1e-11); a zero imaginary
 part gives the real-wavenumber matrices bit for bit; the Galerkin
 formalism (constant and isoparametric linear fields) against the oracle's
-Galerkin matrices (<= 1e-5); quadrilateral meshes refuse complex
-wavenumbers."""
+Galerkin matrices (<= 1e-5); on quadrilateral meshes and on meshes with
+both element types (collocation, NiHu's degrees and a fixed Gauss degree 5):
+rows to 2e-6 of their largest entry, 1e-5 relative Frobenius and the FP64-pass
+entries (near-singular by polar / Telles, diagonal by Duffy / Guiggiani) to
+1e-9; the FP64 precision policy refuses meshes with quadrilaterals."""
 
 import numpy as np
 import pytest
@@ -114,6 +117,102 @@ def test_refusals(nb):
     qn, qe = sphere_boundary(1.0, 2)
     q = nb.Mesh(qn, qe)
     with pytest.raises(NotImplementedError):
-        nb.assemble_bm(q, k, nb.colmajor(q.n, q.n), nb.colmajor(q.n, q.n), pol=nb.policy(rule="fixed", degree=5))
+        nb.assemble_bm(q, k, nb.colmajor(q.n, q.n, "complex128"), nb.colmajor(q.n, q.n, "complex128"),
+                       pol=nb.policy(rule="fixed", degree=5, precision="fp64"))
     with pytest.raises(ValueError):
         nb.assemble_bm(m, complex(-1.0, 0.5), nb.colmajor(m.n, m.n), nb.colmajor(m.n, m.n))
+
+
+# ---------------------------------------------------------------- quadrilaterals
+QROW_TOL = 2e-6
+QNIHU = (dict(rule="nihu", near="mean_edge", limit=1.5), dict(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5))
+QFIXED = (dict(rule="fixed", degree=5, near="mean_edge", limit=1.5),
+          dict(rule=1, fixed_degree=5, near_mode=0, near_limit=1.5))
+
+
+def rowscaled(got, ref):
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    scale[scale == 0] = 1.0
+    return (np.abs(got - ref) / scale).max()
+
+
+@pytest.fixture(scope="module")
+def qmesh():
+    from nihu_amd.mesh import jittered_cubed_sphere
+    return jittered_cubed_sphere(5, seed=4)
+
+
+@pytest.mark.parametrize("nx,ny", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("which", ["nihu", "fixed"])
+def test_quad_operators(nb, orc, qmesh, nx, ny, which):
+    import torch
+    gp, op = QNIHU if which == "nihu" else QFIXED
+    nodes, quads = qmesh
+    n = len(quads)
+    m = nb.Mesh(nodes, quads)
+    k = complex(3.0, -0.5)
+    alpha = 0.7 - 0.2j
+    print("quadrilateral operator (%d, %d) %s, k = %s" % (nx, ny, which, k), flush=True)
+    out = nb.colmajor(n, n, "complex128")
+    out.fill_(complex("nan+nanj"))
+    nb.assemble(m, m, out, nb.NB_HELMHOLTZ, nx, ny, k, alpha=alpha, pol=nb.policy(**gp), accumulate=False)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.isfinite(got).all()
+    ref = orc.assemble(nodes, quads, 1, nx, ny, k=k, alpha=alpha, pol=orc.policy(**op))
+    cls = nb.classify(m, m, nx, ny, nb.policy(**gp)).cpu().numpy()
+    fp64 = cls < 0
+    e64 = np.abs(got[fp64] - ref[fp64]).max() / np.abs(ref[fp64]).max()
+    rs, fr = rowscaled(got, ref), relfro(got, ref)
+    print("  row-scaled %.2e  rel fro %.2e  fp64 entries %.2e (%d)" % (rs, fr, e64, fp64.sum()), flush=True)
+    assert rs <= QROW_TOL and fr <= 1e-5
+    assert e64 <= 1e-9
+    # the attenuation is there: the real-k matrix differs by far more
+    real = orc.assemble(nodes, quads, 1, nx, ny, k=k.real, alpha=alpha, pol=orc.policy(**op))
+    assert relfro(real, ref) > 100 * fr
+
+
+def test_quad_burton_miller(nb, orc, qmesh):
+    import torch
+    nodes, quads = qmesh
+    n = len(quads)
+    m = nb.Mesh(nodes, quads)
+    k = complex(4.0, -0.8)
+    A, B = nb.colmajor(n, n), nb.colmajor(n, n)
+    nb.assemble_bm(m, k, A, B, pol=nb.policy(**QNIHU[0]))
+    torch.cuda.synchronize()
+    Ar, Br = orc.assemble_bm(nodes, quads, k, pol=orc.policy(**QNIHU[1]))
+    ea, eb = relfro(A.cpu().numpy(), Ar), relfro(B.cpu().numpy(), Br)
+    print("quadrilateral Burton-Miller k = %s: A %.2e B %.2e" % (k, ea, eb), flush=True)
+    assert ea <= 1e-5 and eb <= 1e-5
+    # k + 0i takes the real-wavenumber kernels: bit for bit
+    A1, B1, A2, B2 = (nb.colmajor(n, n) for _ in range(4))
+    nb.assemble_bm(m, 4.0, A1, B1, pol=nb.policy(**QNIHU[0]))
+    nb.assemble_bm(m, complex(4.0, 0.0), A2, B2, alpha=complex(0.0, -0.25), pol=nb.policy(**QNIHU[0]))
+    torch.cuda.synchronize()
+    assert torch.equal(torch.view_as_real(A1), torch.view_as_real(A2))
+    assert torch.equal(torch.view_as_real(B1), torch.view_as_real(B2))
+
+
+@pytest.mark.parametrize("nx,ny", [(0, 0), (1, 1)])
+def test_mixed_mesh_operators(nb, orc, nx, ny):
+    """Both element types in one mesh (every third quadrilateral of the cubed
+    sphere split into two triangles): each (test type, trial type) block
+    against the oracle's single-type assembly at a complex wavenumber."""
+    import torch
+    from nihu_amd.mesh import mixed_cubed_sphere
+    from tests.test_gpu_mixed import oracle_mixed
+    nodes, table = mixed_cubed_sphere(5, seed=3)
+    m = nb.Mesh.from_elements(nodes, table)
+    n = len(table)
+    k = complex(3.0, -0.6)
+    out = nb.colmajor(n, n, "complex128")
+    out.fill_(complex("nan+nanj"))
+    nb.assemble(m, m, out, nb.NB_HELMHOLTZ, nx, ny, k, pol=nb.policy(**QNIHU[0]), accumulate=False)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.isfinite(got).all()
+    ref = oracle_mixed(orc, nodes, table, 1, nx, ny, k, QNIHU[1])
+    rs, fr = rowscaled(got, ref), relfro(got, ref)
+    print("mixed mesh (%d, %d), k = %s: row-scaled %.2e rel fro %.2e" % (nx, ny, k, rs, fr), flush=True)
+    assert rs <= QROW_TOL and fr <= 1e-5
