@@ -45,6 +45,7 @@
 // The file is compiled with -ffp-contract=off so that no other fusion happens.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -1324,6 +1325,286 @@ static cplx quad_pair_integral(int family, int nx, int ny, K k, Policy const &p,
 	return quad_regular(family, nx, ny, k, t.x, t.nu, trial, degree);
 }
 
+
+// ---------------------------------------------------------------------------
+// quad_1_gauss_field (nihu/library/quad_1_gauss_field.hpp,
+// quad_1_gauss_shape_set.hpp): a scalar field over a quadrilateral with four
+// dofs at the 2 x 2 Gauss points, nodes in the order of
+// nihu/library/lib_shape.cpp:128-134 -- (-,-), (+,-), (-,+), (+,+) at
+// +-1/sqrt(3) -- and the bilinear Lagrange shape functions on those nodes,
+//   N_b(xi, eta) = (1 + s_b sqrt3 xi)(1 + t_b sqrt3 eta) / 4.
+// Collocation with it (dirac of the field) puts the collocation points at the
+// nodes; each element pair is a 4 x 4 block (test node a, trial dof b) or,
+// for a constant test field (field points), 1 x 4.
+//   classification: element centres, as for constant fields
+//     (helmholtz_nearly_singular_integrals.hpp:281-293, element_match.hpp:
+//     84-112: only the same element is singular in collocation);
+//   regular: degree 2 (polynomial order 1 + jacobian order 1 of the field,
+//     nihu/core/complexity_estimator.hpp:47-75) + the distance-table degree,
+//     one Gauss square rule for the whole block, N_b(xi_q) in the weights
+//     (double_integral.hpp:445-467);
+//   near-singular: per test node, polar with F = K J N(xi) - K_lin J_lin N(xi0)
+//     plus the planar integral times N(xi0) (nearly_singular_collocational.hpp:
+//     86-197), or Telles with N(xi) in the weights;
+//   singular: per test node xi_a, HSP by Guiggiani (5, 9) at xi_a with the
+//     shape-function terms of the Laurent coefficients
+//     (helmholtz_singular_integrals.hpp:1189-1236, guiggiani_1992.hpp:180-330,
+//     laplace_kernel.hpp:266-295), the others by the blind Duffy rule of
+//     degree 7 around xi_a (singular_accelerator.hpp:400-456,
+//     double_integral.hpp:483-503).
+// ---------------------------------------------------------------------------
+typedef std::array<cplx, 4> C4;
+static const double GSX[4] = {-1.0, 1.0, -1.0, 1.0}, GSY[4] = {-1.0, -1.0, 1.0, 1.0};
+
+static inline void gauss_node(int a, double &xi, double &eta)
+{
+	xi = GSX[a] * (std::sqrt(3.0) / 3.0);
+	eta = GSY[a] * (std::sqrt(3.0) / 3.0);
+}
+
+static inline void gauss_shape(double xi, double eta, double N[4])
+{
+	const double s3 = std::sqrt(3.0);
+	for (int b = 0; b < 4; ++b) N[b] = (1.0 + GSX[b] * s3 * xi) * (1.0 + GSY[b] * s3 * eta) / 4.0;
+}
+
+static inline void gauss_dshape(double xi, double eta, double dNx[4], double dNy[4])
+{
+	const double s3 = std::sqrt(3.0);
+	for (int b = 0; b < 4; ++b) {
+		dNx[b] = GSX[b] * s3 * (1.0 + GSY[b] * s3 * eta) / 4.0;
+		dNy[b] = (1.0 + GSX[b] * s3 * xi) * GSY[b] * s3 / 4.0;
+	}
+}
+
+static inline void c4_zero(C4 &v) { for (auto &z : v) z = 0.0; }
+
+template <class K>
+static void gauss_regular(int family, int nx, int ny, K k, V3 x, V3 nxv, QElem const &e, int degree, C4 &out)
+{
+	if (degree > 14)
+		throw std::out_of_range("Too high quadrature degree selected for collocational integration");
+	c4_zero(out);
+	for (auto const &q : gauss_quad_store(degree)) {
+		cplx v = q_kernel_jac(family, nx, ny, k, x, nxv, e, q.xi, q.eta) * q.w;
+		double N[4];
+		gauss_shape(q.xi, q.eta, N);
+		for (int b = 0; b < 4; ++b) out[b] += v * N[b];
+	}
+}
+
+template <class K>
+static void gauss_near_polar(int family, int nx, int ny, K k, QElem const &e, V3 x0, V3 nxv, int rad_order,
+	int tan_order, C4 &out)
+{
+	double xi0, eta0, zeta;
+	if (!quad_inverse_map(e, x0, 1e-5, 100, xi0, eta0, zeta))
+		throw std::runtime_error("Could not perform inverse mapping");
+	QElem lin = quad_linearized(e, xi0, eta0);
+	double N0[4];
+	gauss_shape(xi0, eta0, N0);
+	double theta_lim[4], theta0[4], ref_distance[4];
+	for (int n = 0; n < 4; ++n) {
+		int m = (n + 1) % 4;
+		double c1x = QCX[n], c1y = QCY[n], c2x = QCX[m], c2y = QCY[m];
+		double lx = 1.0, ly = 0.0;
+		double ln = std::sqrt((c2x - c1x) * (c2x - c1x) + (c2y - c1y) * (c2y - c1y));
+		if (ln > 1e-3) { lx = (c2x - c1x) / ln; ly = (c2y - c1y) / ln; }
+		double d1x = c1x - xi0, d1y = c1y - eta0;
+		double dl = d1x * lx + d1y * ly;
+		double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		theta_lim[n] = std::atan2(d1y, d1x);
+		theta0[n] = std::atan2(d0y, d0x);
+		ref_distance[n] = std::sqrt(d0x * d0x + d0y * d0y);
+	}
+	auto const &gt = gauss_line(tan_order);
+	auto const &gr = gauss_line(rad_order);
+	c4_zero(out);
+	for (int n = 0; n < 4; ++n) {
+		double t1 = theta_lim[n], t2 = theta_lim[(n + 1) % 4];
+		if (std::abs(t2 - t1) > PI) {
+			if (t2 > t1) t1 += TWO_PI;
+			else t2 += TWO_PI;
+		}
+		if (std::abs(t2 - t1) < 1e-3) continue;
+		for (auto const &qt : gt) {
+			double theta = ((1.0 - qt.first) * t1 + (1.0 + qt.first) * t2) / 2.0;
+			double w_theta = qt.second * (t2 - t1) / 2.0;
+			double rho_lim = ref_distance[n] / std::cos(theta - theta0[n]);
+			for (auto const &qr : gr) {
+				double rho = (1.0 + qr.first) * rho_lim / 2.0;
+				double w_rho = qr.second * rho_lim / 2.0;
+				double xi = rho * std::cos(theta) + xi0, eta = rho * std::sin(theta) + eta0;
+				cplx F = q_kernel_jac(family, nx, ny, k, x0, nxv, e, xi, eta);
+				cplx Fl = q_kernel_jac(family, nx, ny, k, x0, nxv, lin, xi, eta);
+				double N[4];
+				gauss_shape(xi, eta, N);
+				for (int b = 0; b < 4; ++b) out[b] += (F * N[b] - Fl * N0[b]) * (w_theta * w_rho * rho);
+			}
+		}
+	}
+	cplx planar = quad_planar_near(family, nx, ny, k, lin, x0, nxv);
+	for (int b = 0; b < 4; ++b) out[b] += planar * N0[b];
+}
+
+template <class K>
+static void gauss_near_telles(int family, int nx, int ny, K k, QElem const &e, V3 x0, V3 nxv, int degree, C4 &out)
+{
+	double xi0, eta0, zeta;
+	if (!quad_inverse_map(e, x0, 1e-6, 100, xi0, eta0, zeta))
+		throw std::runtime_error("Could not perform inverse mapping");
+	c4_zero(out);
+	for (auto const &q : telles_rule(degree, xi0, eta0, zeta)) {
+		cplx v = q_kernel_jac(family, nx, ny, k, x0, nxv, e, q.xi, q.eta) * q.w;
+		double N[4];
+		gauss_shape(q.xi, q.eta, N);
+		for (int b = 0; b < 4; ++b) out[b] += v * N[b];
+	}
+}
+
+// NiHu's near-singular method per kernel, as quad_near
+template <class K>
+static void gauss_near(int family, int nx, int ny, K k, QElem const &e, V3 x0, V3 nxv, C4 &out)
+{
+	if (family == 1 && nx + ny > 0) return gauss_near_telles(family, nx, ny, k, e, x0, nxv, 30, out);
+	if (family == 0 && nx == 1 && ny == 1) return gauss_near_polar(family, nx, ny, k, e, x0, nxv, 100, 100, out);
+	gauss_near_polar(family, nx, ny, k, e, x0, nxv, 15, 15, out);
+}
+
+// the blind Duffy rule of a degree around xi0 with the shape functions
+template <class K>
+static void gauss_duffy(int family, int nx, int ny, K k, QElem const &e, double xi0, double eta0, int degree, C4 &out)
+{
+	V3 x = q_x(e, xi0, eta0);
+	V3 n = normalized(q_normal(e, xi0, eta0));
+	c4_zero(out);
+	for (auto const &q : quad_duffy_face(degree, xi0, eta0)) {
+		cplx v = q_kernel_jac(family, nx, ny, k, x, n, e, q.xi, q.eta) * q.w;
+		double N[4];
+		gauss_shape(q.xi, q.eta, N);
+		for (int b = 0; b < 4; ++b) out[b] += v * N[b];
+	}
+}
+
+// Guiggiani's finite part at xi0 with the shape functions: per dof b the
+// Laurent coefficients a0 N_b(xi0) and a1 N_b(xi0) + a0 (grad N_b(xi0) . v),
+// v the direction in the reference square (laplace_kernel.hpp:272-292,
+// guiggiani_1992.hpp:215-252); otherwise as quad_guiggiani.
+template <class K>
+static void gauss_guiggiani(int family, K k, QElem const &e, double xi0x, double xi0y, int rad_order, int tan_order,
+	C4 &out)
+{
+	V3 dxi, deta;
+	q_dx(e, xi0x, xi0y, dxi, deta);
+	V3 dxe = q_dxe(e);
+	V3 nJ = cross(dxi, deta);
+	V3 n0 = normalized(nJ);
+	V3 x0 = q_x(e, xi0x, xi0y);
+	double N0[4], dNx[4], dNy[4];
+	gauss_shape(xi0x, xi0y, N0);
+	gauss_dshape(xi0x, xi0y, dNx, dNy);
+
+	double la = norm(dxi), lb = norm(deta);
+	double cosg = dot(dxi, deta) / la / lb;
+	double u = lb / la;
+	double T00 = la, T01 = la * cosg * u, T10 = 0.0, T11 = la * std::sqrt(1.0 - cosg * cosg) * u;
+	double detT = T00 * T11 - T10 * T01;
+	double Ti00 = T11 / detT, Ti01 = -T01 / detT, Ti10 = -T10 / detT, Ti11 = T00 / detT;
+	double e0x = T00 * xi0x + T01 * xi0y, e0y = T10 * xi0x + T11 * xi0y;
+	V3 J0 = nJ / detT;
+
+	double theta_lim[4], theta0[4], ref_distance[4];
+	for (int n = 0; n < 4; ++n) {
+		double c1x = T00 * QCX[n] + T01 * QCY[n], c1y = T10 * QCX[n] + T11 * QCY[n];
+		int m = (n + 1) % 4;
+		double c2x = T00 * QCX[m] + T01 * QCY[m], c2y = T10 * QCX[m] + T11 * QCY[m];
+		double lx = 1.0, ly = 0.0;
+		double ln = std::sqrt((c2x - c1x) * (c2x - c1x) + (c2y - c1y) * (c2y - c1y));
+		if (ln > 1e-3) { lx = (c2x - c1x) / ln; ly = (c2y - c1y) / ln; }
+		double d1x = c1x - e0x, d1y = c1y - e0y;
+		double dl = d1x * lx + d1y * ly;
+		double d0x = d1x - lx * dl, d0y = d1y - ly * dl;
+		theta_lim[n] = std::atan2(d1y, d1x);
+		theta0[n] = std::atan2(d0y, d0x);
+		ref_distance[n] = std::sqrt(d0x * d0x + d0y * d0y);
+	}
+
+	auto const &gt = gauss_line(tan_order);
+	auto const &gr = gauss_line(rad_order);
+	c4_zero(out);
+	for (int n = 0; n < 4; ++n) {
+		double t1 = theta_lim[n], t2 = theta_lim[(n + 1) % 4];
+		if (std::abs(t2 - t1) < 1e-3) continue;
+		if (std::abs(t2 - t1) > PI) t2 += TWO_PI;
+		for (auto const &qt : gt) {
+			double theta = ((1.0 - qt.first) * t1 + (1.0 + qt.first) * t2) / 2.0;
+			double w_theta = qt.second * (t2 - t1) / 2.0;
+			double ct = std::cos(theta), st = std::sin(theta);
+			double vx = Ti00 * ct + Ti01 * st, vy = Ti10 * ct + Ti11 * st;
+			V3 r1 = dxi * vx + deta * vy;
+			V3 r2 = dxe * (vx * vy);
+			V3 J1 = (cross(dxi, dxe) * vx + cross(dxe, deta) * vy) / detT;
+			V3 g1 = r1 * (dot(r2, J0) + dot(r1, J1));
+			V3 b0 = J0 * -1.0;
+			V3 b1 = g1 * 3.0 - J1;
+			double a0 = dot(b0, n0);
+			double a1 = dot(b1, n0);
+			double Sm2 = -3.0 * dot(r1, r2);
+			double rho_lim = ref_distance[n] / std::cos(theta - theta0[n]);
+			double Fm1[4], Fm2[4];
+			for (int b = 0; b < 4; ++b) {
+				double A0 = a0 * N0[b];
+				double A1 = a1 * N0[b] + a0 * (dNx[b] * vx + dNy[b] * vy);
+				Fm1[b] = -(Sm2 * A0 + A1) / (4.0 * PI);
+				Fm2[b] = -A0 / (4.0 * PI);
+				out[b] += (Fm1[b] * std::log(rho_lim) - Fm2[b] / rho_lim) * w_theta;
+			}
+			for (auto const &qr : gr) {
+				double rho = (1.0 + qr.first) * rho_lim / 2.0;
+				double w_rho = qr.second * rho_lim / 2.0;
+				double ex = rho * ct, ey = rho * st;
+				double xi = Ti00 * ex + Ti01 * ey + xi0x, eta = Ti10 * ex + Ti11 * ey + xi0y;
+				V3 y = q_x(e, xi, eta);
+				V3 nv = q_normal(e, xi, eta);
+				double J = norm(nv);
+				cplx F = kernel_eval(family, 1, 1, k, x0, y, n0, nv / J) * (J / detT * rho);
+				double N[4];
+				gauss_shape(xi, eta, N);
+				for (int b = 0; b < 4; ++b) {
+					double sing = (Fm1[b] + Fm2[b] / rho) / rho;
+					out[b] += (F * N[b] - sing) * (w_theta * w_rho);
+				}
+			}
+		}
+	}
+}
+
+// the same element, collocation at node a
+template <class K>
+static void gauss_singular(int family, int nx, int ny, K k, QElem const &e, int a, C4 &out)
+{
+	double xi0, eta0;
+	gauss_node(a, xi0, eta0);
+	if (nx == 1 && ny == 1) return gauss_guiggiani(family, k, e, xi0, eta0, 5, 9, out);
+	gauss_duffy(family, nx, ny, k, e, xi0, eta0, 7, out);
+}
+
+// One block row: collocation point x (unit normal nxv) of the test element
+// with centre tc and id tid against a quadrilateral trial element with the
+// gauss field; node a >= 0: x is the test element's node a (same-mesh
+// singular check), else a constant test field.
+template <class K>
+static void gauss_pair_row(int family, int nx, int ny, K k, Policy const &p, V3 x, V3 nxv, V3 tc, int tid, int a,
+	QElem const &trial, bool same_mesh, C4 &out)
+{
+	if (same_mesh && a >= 0 && tid == trial.id) return gauss_singular(family, nx, ny, k, trial, a, out);
+	double d2 = dist2(tc, trial.c);
+	if (near_needed_q(p, d2, trial)) return gauss_near(family, nx, ny, k, trial, x, nxv, out);
+	int degree = p.rule == 1 ? p.fixed_degree
+		: 2 + eval_interval(interval_table(1 + nx + ny), std::sqrt(d2) / trial.R);
+	gauss_regular(family, nx, ny, k, x, nxv, trial, degree, out);
+}
+
 } // namespace orc
 
 // ===========================================================================
@@ -1992,6 +2273,163 @@ int orc_geometry_quad(const double *xyz, int n_nodes, const int *quad, int n, do
 		diam[e] = q[e].diam;
 	}
 	ORC_CATCH
+}
+
+} // extern "C"
+
+// ===========================================================================
+// quad_1_gauss_field collocation
+// ===========================================================================
+template <class K>
+static int assemble_gauss_impl(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
+	const double *sxyz, int sn, const int *sel, int snt, int same_mesh, int test_gauss,
+	int family, int nx, int ny, K k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	ORC_TRY
+	if (test_gauss && tnpe != 4) throw std::invalid_argument("a gauss test field needs quadrilaterals");
+	if (same_mesh && !test_gauss) throw std::invalid_argument("same mesh needs the gauss test field");
+	std::vector<Elem> ttri;
+	std::vector<QElem> tq, sq;
+	std::vector<TestPt> tp;
+	read_elems(txyz, tn, tel, tnt, tnpe, &ttri, &tq, &tp);
+	if (!same_mesh) read_elems(sxyz, sn, sel, snt, 4, nullptr, &sq, nullptr);
+	std::vector<QElem> const &trial = same_mesh ? tq : sq;
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	const int n_test_rows = test_gauss ? 4 * tnt : tnt;
+	int nr = rows ? n_rows : n_test_rows;
+	cplx alpha(alpha_re, alpha_im);
+	if (n_threads <= 0) n_threads = orc_num_threads();
+	std::string err;
+	#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
+	for (int r = 0; r < nr; ++r) {
+		try {
+			int row = rows ? rows[r] : r;
+			if (row < 0 || row >= n_test_rows) throw std::out_of_range("row out of range");
+			V3 x, nv;
+			int a = -1, ie = row;
+			if (test_gauss) {
+				ie = row / 4;
+				a = row % 4;
+				double xi, eta;
+				gauss_node(a, xi, eta);
+				x = q_x(tq[ie], xi, eta);
+				nv = normalized(q_normal(tq[ie], xi, eta));
+			} else {
+				x = tp[ie].x;
+				nv = tp[ie].nu;
+			}
+			for (int j = 0; j < snt; ++j) {
+				C4 v;
+				gauss_pair_row(family, nx, ny, k, p, x, nv, tp[ie].x, tp[ie].id, a, trial[j], same_mesh != 0, v);
+				for (int b = 0; b < 4; ++b) {
+					cplx z = alpha * v[b];
+					double *o = out + 2 * ((int64_t)(4 * j + b) * ld + r);
+					if (accumulate) { o[0] += z.real(); o[1] += z.imag(); }
+					else { o[0] = z.real(); o[1] = z.imag(); }
+				}
+			}
+		} catch (std::exception const &ex) {
+			#pragma omp critical
+			err = ex.what();
+		}
+	}
+	if (!err.empty()) throw std::out_of_range(err);
+	ORC_CATCH
+}
+
+template <class K>
+static int gauss_elem_impl(int what, int family, int nx, int ny, K k, const double *coords12, const double *x0p,
+	const double *nx0p, const double *xi0p, int o1, int o2, double *out8)
+{
+	ORC_TRY
+	V3 X[4];
+	for (int a = 0; a < 4; ++a) X[a] = {coords12[3 * a], coords12[3 * a + 1], coords12[3 * a + 2]};
+	QElem e = make_qelem(X, 0);
+	V3 x0 = x0p ? V3{x0p[0], x0p[1], x0p[2]} : e.c;
+	V3 n0 = nx0p ? V3{nx0p[0], nx0p[1], nx0p[2]} : e.nu;
+	C4 v;
+	switch (what) {
+	case 0: gauss_regular(family, nx, ny, k, x0, n0, e, o1, v); break;
+	case 1: gauss_singular(family, nx, ny, k, e, o1, v); break;
+	case 2: gauss_near_polar(family, nx, ny, k, e, x0, n0, o1, o2, v); break;
+	case 3: gauss_near_telles(family, nx, ny, k, e, x0, n0, o1, v); break;
+	case 4: gauss_guiggiani(family, k, e, xi0p[0], xi0p[1], o1, o2, v); break;
+	case 5: gauss_duffy(family, nx, ny, k, e, xi0p[0], xi0p[1], o1, v); break;
+	case 7: gauss_near(family, nx, ny, k, e, x0, n0, v); break;
+	default: throw std::invalid_argument("unknown element integral");
+	}
+	for (int b = 0; b < 4; ++b) { out8[2 * b] = v[b].real(); out8[2 * b + 1] = v[b].imag(); }
+	ORC_CATCH
+}
+
+extern "C" {
+
+// Collocation with the quadrilateral gauss field of the trial mesh (4 dofs
+// per element, column 4 j + b) and, test_gauss = 1, the dirac of the test
+// mesh's gauss field (rows 4 i + a: node a of test element i) or, 0, the
+// dirac of its constant field (rows = test elements, any element type: field
+// points).  k_im: imaginary part of the wavenumber.
+int orc_assemble_gauss(const double *txyz, int tn, const int *tel, int tnt, int tnpe,
+	const double *sxyz, int sn, const int *sel, int snt, int same_mesh, int test_gauss,
+	int family, int nx, int ny, double k_re, double k_im, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *rows, int n_rows, double *out, int64_t ld, int accumulate, int n_threads)
+{
+	if (k_im == 0.0)
+		return assemble_gauss_impl(txyz, tn, tel, tnt, tnpe, sxyz, sn, sel, snt, same_mesh, test_gauss, family, nx,
+			ny, k_re, alpha_re, alpha_im, pol, rows, n_rows, out, ld, accumulate, n_threads);
+	return assemble_gauss_impl(txyz, tn, tel, tnt, tnpe, sxyz, sn, sel, snt, same_mesh, test_gauss, family, nx, ny,
+		cplx(k_re, k_im), alpha_re, alpha_im, pol, rows, n_rows, out, ld, accumulate, n_threads);
+}
+
+// Burton-Miller A and B on a quadrilateral mesh with the gauss field (as
+// orc_assemble_bm; the identity is the unit matrix over the 4 n dofs: the
+// shape functions are Lagrange polynomials on the collocation nodes)
+int orc_assemble_bm_gauss(const double *xyz, int nn, const int *quad, int ne, double k_re, double k_im,
+	double alpha_re, double alpha_im, const orc_policy *pol, const int *rows, int n_rows, double *A, double *B,
+	int64_t ld, int n_threads)
+{
+	const int nd = 4 * ne;
+	int nr = rows ? n_rows : nd;
+	for (int j = 0; j < nd; ++j)
+		for (int r = 0; r < nr; ++r) {
+			double *a = A + 2 * ((int64_t)j * ld + r), *b = B + 2 * ((int64_t)j * ld + r);
+			a[0] = a[1] = b[0] = b[1] = 0.0;
+		}
+	cplx al(alpha_re, alpha_im);
+	int s;
+	auto op = [&](int nx, int ny, cplx sc, double *out) {
+		return orc_assemble_gauss(xyz, nn, quad, ne, 4, xyz, nn, quad, ne, 1, 1, 1, nx, ny, k_re, k_im, sc.real(),
+			sc.imag(), pol, rows, n_rows, out, ld, 1, n_threads);
+	};
+	if ((s = op(0, 0, 1.0, B))) return s;
+	if ((s = op(1, 0, al, B))) return s;
+	orc_identity(nd, (al / 2.).real(), (al / 2.).imag(), rows, n_rows, B, ld);
+	if ((s = op(1, 1, al, A))) return s;
+	orc_identity(nd, -.5, 0.0, rows, n_rows, A, ld);
+	if ((s = op(0, 1, 1.0, A))) return s;
+	return 0;
+}
+
+// Element-level gauss-field integrals for the tests (4 complex results, one
+// per trial dof): 0 regular (Gauss degree o1) at (x0, nx0), 1 singular at
+// node o1, 2 polar near (o1, o2), 3 Telles near (degree o1), 4 Guiggiani at
+// xi0 (o1, o2), 5 Duffy rule of degree o1 around xi0, 7 near dispatch.
+int orc_gauss_elem(int what, int family, int nx, int ny, double k_re, double k_im, const double *coords12,
+	const double *x0p, const double *nx0p, const double *xi0p, int o1, int o2, double *out8)
+{
+	if (k_im == 0.0) return gauss_elem_impl(what, family, nx, ny, k_re, coords12, x0p, nx0p, xi0p, o1, o2, out8);
+	return gauss_elem_impl(what, family, nx, ny, cplx(k_re, k_im), coords12, x0p, nx0p, xi0p, o1, o2, out8);
+}
+
+// shape functions and nodes of the gauss field (N[4], dN/dxi[4], dN/deta[4]
+// at (xi, eta); nodes[8])
+int orc_gauss_shape(double xi, double eta, double *N12, double *nodes8)
+{
+	gauss_shape(xi, eta, N12);
+	gauss_dshape(xi, eta, N12 + 4, N12 + 8);
+	for (int a = 0; a < 4; ++a) gauss_node(a, nodes8[2 * a], nodes8[2 * a + 1]);
+	return 0;
 }
 
 } // extern "C"

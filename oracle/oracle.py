@@ -127,6 +127,14 @@ def _declare(l):
                                                C.c_double, C.c_double, P, I, C.c_int, D, D, C.c_int64, C.c_int]
         l.orc_quad_elem_ck.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D,
                                        C.c_int, C.c_int, D]
+        l.orc_assemble_gauss.argtypes = [D, C.c_int, I, C.c_int, C.c_int, D, C.c_int, I, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_assemble_bm_gauss.argtypes = [D, C.c_int, I, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, P, I, C.c_int, D, D, C.c_int64, C.c_int]
+        l.orc_gauss_elem.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D,
+                                     C.c_int, C.c_int, D]
+        l.orc_gauss_shape.argtypes = [C.c_double, C.c_double, D, D]
         l.orc_quad_duffy_rule.argtypes = [C.c_int, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_telles_rule.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, C.c_int, I]
         l.orc_geometry_quad.argtypes = [D, C.c_int, I, C.c_int, D, D, D, D]
@@ -426,3 +434,75 @@ def geometry_quad(nodes, quads):
     c, nu, R, diam = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
     _check(lib().orc_geometry_quad(_d(nodes), len(nodes), _i(quads), n, _d(c), _d(nu), _d(R), _d(diam)))
     return dict(center=c, normal=nu, R=R, diam=diam)
+
+
+# ---------------------------------------------------------------- gauss field
+GAUSS_REGULAR, GAUSS_SINGULAR, GAUSS_POLAR, GAUSS_TELLES, GAUSS_GUIGGIANI, GAUSS_DUFFY = range(6)
+GAUSS_NEAR = 7
+
+
+def gauss_shape(xi, eta):
+    """(N[4], dN/dxi[4], dN/deta[4]) of the quadrilateral gauss field at
+    (xi, eta), and its nodes (4 x 2)."""
+    out, nodes = np.zeros(12), np.zeros(8)
+    lib().orc_gauss_shape(float(xi), float(eta), _d(out), _d(nodes))
+    return out[:4].copy(), out[4:8].copy(), out[8:].copy(), nodes.reshape(4, 2)
+
+
+def gauss_elem(what, family, nx, ny, k, coords, x0=None, nx0=None, xi0=None, o1=0, o2=0):
+    """Element integral over a quadrilateral with the gauss field: 4 complex
+    values, one per trial dof (what: a GAUSS_* constant, see orc_gauss_elem)."""
+    cs = np.ascontiguousarray(coords, dtype=np.float64).reshape(12)
+    arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (x0, nx0, xi0)]
+    out = np.zeros(8)
+    kc = complex(k)
+    _check(lib().orc_gauss_elem(what, family, nx, ny, kc.real, kc.imag, _d(cs),
+                                *[None if a is None else _d(a) for a in arrs], o1, o2, _d(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def assemble_gauss(nodes, quads, family, nx, ny, k=0.0, alpha=1.0, pol=None, rows=None, trial=None,
+                   test_gauss=True, out=None, accumulate=False, threads=0):
+    """alpha * operator with the quadrilateral gauss field (4 dofs per
+    element, column 4 j + b) on the trial mesh (trial=None: the test mesh);
+    rows: the nodes 4 i + a of the test mesh's gauss field (test_gauss) or
+    its elements (field points, any element type), complex128 Fortran order."""
+    pol = pol or policy()
+    t_nodes, t_el = _mesh(nodes, quads)
+    same = trial is None
+    s_nodes, s_el = (t_nodes, t_el) if same else _mesh(*trial)
+    if s_el.shape[1] != 4:
+        raise ValueError("the gauss field needs a quadrilateral trial mesh")
+    rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    n_test_rows = 4 * len(t_el) if test_gauss else len(t_el)
+    nr = n_test_rows if rows_a is None else len(rows_a)
+    if out is None:
+        out = np.zeros((nr, 4 * len(s_el)), dtype=np.complex128, order="F")
+        accumulate = False
+    alpha = complex(alpha)
+    kc = complex(k)
+    _check(lib().orc_assemble_gauss(_d(t_nodes), len(t_nodes), _i(t_el), len(t_el), t_el.shape[1],
+                                    _d(s_nodes), len(s_nodes), _i(s_el), len(s_el), int(same), int(test_gauss),
+                                    family, nx, ny, kc.real, kc.imag, alpha.real, alpha.imag, C.byref(pol),
+                                    _i(rows_a), nr, out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0],
+                                    int(accumulate), threads))
+    return out
+
+
+def assemble_bm_gauss(nodes, quads, k, alpha=None, pol=None, rows=None, threads=0):
+    """Burton-Miller (A, B) with the quadrilateral gauss field; alpha
+    defaults to -i/k."""
+    pol = pol or policy()
+    nodes, quads = _mesh(nodes, quads)
+    kc = complex(k)
+    alpha = -1j / kc if alpha is None else complex(alpha)
+    rows_a = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nd = 4 * len(quads)
+    nr = nd if rows_a is None else len(rows_a)
+    A = np.zeros((nr, nd), dtype=np.complex128, order="F")
+    B = np.zeros((nr, nd), dtype=np.complex128, order="F")
+    _check(lib().orc_assemble_bm_gauss(_d(nodes), len(nodes), _i(quads), len(quads), kc.real, kc.imag,
+                                       alpha.real, alpha.imag, C.byref(pol), _i(rows_a), nr,
+                                       A.ctypes.data_as(C.POINTER(C.c_double)),
+                                       B.ctypes.data_as(C.POINTER(C.c_double)), nr, threads))
+    return A, B

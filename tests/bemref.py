@@ -217,9 +217,10 @@ def np_kernel_pts(family, nx, ny, k, x, Y, nxv, NY):
     return g * (3.0 + ikr * (3.0 + ikr)) * rdnx * rdny + g * (1.0 + ikr) * (NY @ nxv)
 
 
-def quad_brute(family, nx, ny, k, coords, x0, nx0, levels=4, n_gauss=8):
-    """Integral of K(x0, y) over a quadrilateral by Gauss rules on a uniform
-    2^levels x 2^levels subdivision of the reference square."""
+def quad_brute(family, nx, ny, k, coords, x0, nx0, levels=4, n_gauss=8, weight=None):
+    """Integral of K(x0, y) (times weight(xi, eta) if given) over a
+    quadrilateral by Gauss rules on a uniform 2^levels x 2^levels subdivision
+    of the reference square."""
     g, w = np.polynomial.legendre.leggauss(n_gauss)
     m = 2 ** levels
     h = 2.0 / m
@@ -231,6 +232,8 @@ def quad_brute(family, nx, ny, k, coords, x0, nx0, levels=4, n_gauss=8):
     Y, NJ = quad_geom(coords, XI.ravel(), ETA.ravel())
     J = np.linalg.norm(NJ, axis=1)
     K = np_kernel_pts(family, nx, ny, k, np.asarray(x0, float), Y, np.asarray(nx0, float), NJ / J[:, None])
+    if weight is not None:
+        K = K * weight(XI.ravel(), ETA.ravel())
     return np.sum(K * J * W)
 
 
@@ -256,7 +259,7 @@ def polygon_exact(corners, p, kind):
     return tot / FOUR_PI if kind == "slp" else -tot / FOUR_PI
 
 
-def hsp_finite_part(family, k, coords, xi0, eps=(4e-3, 2e-3, 1e-3), n_theta=48, n_rho=24):
+def hsp_finite_part(family, k, coords, xi0, eps=(4e-3, 2e-3, 1e-3), n_theta=48, n_rho=24, weight=None):
     """Hadamard finite part of the HSP kernel over a (warped) quadrilateral at
     x0 = x(xi0), n_x = n(xi0): lim [int over the surface outside the ball
     |y - x0| < eps of K dS - 1/(2 eps)], from polar integration in the
@@ -302,9 +305,25 @@ def hsp_finite_part(family, k, coords, xi0, eps=(4e-3, 2e-3, 1e-3), n_theta=48, 
                 Y, NJ = quad_geom(coords, P[:, 0], P[:, 1])
                 J = np.linalg.norm(NJ, axis=1)
                 K = np_kernel_pts(family, 1, 1, k, x0, Y, nx0, NJ / J[:, None])
+                if weight is not None:
+                    K = K * weight(P[:, 0], P[:, 1])
                 tot += wth[i] * np.sum(K * J * R * WR)
-        vals.append(tot - 1.0 / (2.0 * eps))
+        w0 = 1.0 if weight is None else float(weight(np.array([xi0[0]]), np.array([xi0[1]]))[0])
+        vals.append(tot - w0 / (2.0 * eps))
     # Richardson on eps, 2 eps ratios (errors O(eps) + O(eps^2))
     e = np.array(eps_list)
     A = np.stack([np.ones_like(e), e, e * e], -1)
     return np.linalg.solve(A, np.array(vals, dtype=complex))[0]
+
+
+def gauss_field_shape(b):
+    """The shape function of dof b of the quadrilateral gauss field, written
+    from its definition (the bilinear Lagrange polynomial equal to 1 at node
+    b and 0 at the other three nodes, nodes (-,-), (+,-), (-,+), (+,+) at
+    +-1/sqrt(3)), as a function of (xi, eta)."""
+    g = 1.0 / np.sqrt(3.0)
+    nodes = np.array([[-g, -g], [g, -g], [-g, g], [g, g]])
+    xb, yb = nodes[b]
+    xo = [x for x in (-g, g) if x != xb][0]
+    yo = [y for y in (-g, g) if y != yb][0]
+    return lambda xi, eta: (np.asarray(xi) - xo) / (xb - xo) * (np.asarray(eta) - yo) / (yb - yo)
