@@ -72,7 +72,23 @@ enum nb_location { NB_HOST = 0, NB_DEVICE = 1 };
  * nihu/core/singular_galerkin_quadrature.hpp:197-455; the hypersingular kernel
  * has none for coincident and edge-adjacent pairs and fails with
  * NB_ERR_RUNTIME "Invalid general singular quadrature", as NiHu throws. */
-enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1, NB_GALERKIN_LINEAR = 2 };
+/* NB_COLLOCATION_GAUSS: collocation with the quadrilateral gauss field
+ * (NiHu's quad_1_gauss_field, nihu/library/quad_1_gauss_field.hpp, as the
+ * GAUSS builds of nihu/applications/helmholtz/helmholtz_3d_coll.cpp:221-240
+ * -- GAUSS is defined there, :20 -- and benchmark/EAA/sphere/homogeneous/
+ * bem/bem.cpp:203-222 create it): four
+ * dofs per quadrilateral at the 2 x 2 Gauss points, dof 4 e + b at node b in
+ * the order (-,-), (+,-), (-,+), (+,+) of nihu/library/lib_shape.cpp:128-134
+ * (the field matrix rows [666, nodes, 4 e .. 4 e + 3] of those programs);
+ * test = dirac of the test mesh's gauss field (rows 4 i + a), trial = the
+ * trial mesh's gauss field (columns 4 j + b); both meshes of quadrilaterals.
+ * NB_COLLOCATION_GAUSS_TRIAL: the dirac of the test mesh's constant field
+ * (rows = its elements, any one element type: field points,
+ * dirac(constant_view(field)) of bem.cpp:233) against the trial mesh's
+ * gauss field, on different meshes.  NB_IDENTITY with NB_COLLOCATION_GAUSS on
+ * the same mesh: the unit matrix over the dofs. */
+enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1, NB_GALERKIN_LINEAR = 2, NB_COLLOCATION_GAUSS = 3,
+                    NB_COLLOCATION_GAUSS_TRIAL = 4 };
 
 typedef struct nb_mesh nb_mesh;
 
@@ -188,6 +204,12 @@ int nb_assemble_bm(const nb_mesh *mesh, double k, nb_scalar alpha, const nb_poli
  * meshes of triangles, of quadrilaterals or of both. */
 int nb_assemble_bm_ck(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const nb_policy *policy,
                       const nb_out *A, const nb_out *B, void *stream);
+/* Burton-Miller A and B (as nb_assemble_bm, real or complex k) with the
+ * quadrilateral gauss field (NB_COLLOCATION_GAUSS) on a quadrilateral mesh:
+ * 4 n x 4 n, the identity the unit matrix over the dofs; the Burton-Miller
+ * composition of helmholtz_3d_coll.cpp:63-96 in its GAUSS build. */
+int nb_assemble_bm_gauss(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const nb_policy *policy,
+                         const nb_out *A, const nb_out *B, void *stream);
 
 /* Burton-Miller matrices for n_k wavenumbers on one mesh in one persistent
  * launch (alpha = -i/k each).  A and B are device c64 arrays holding n_k

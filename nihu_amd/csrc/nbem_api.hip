@@ -22,6 +22,7 @@
 #include "nbem_sweep.hpp"
 #include "nbem_galerkin.hpp"
 #include "nbem_quad.hpp"
+#include "nbem_gauss.hpp"
 #include "nbem_solve.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -55,6 +56,8 @@ int nbg_geometry_quad(const double *xyz, int n_nodes, const int32_t *quad, int n
 	double *J, double *R, double *diam, double *X);
 int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out);
 int nbg_quad_duffy(int ng, double x0, double y0, double *out);
+int nbg_gauss_nodes(int n, const double *X, double *x, double *nu);
+int nbg_gauss_shapes(int N, double *out);
 }
 
 namespace {
@@ -209,6 +212,11 @@ int device_rules(const nbk::Rules **out)
 		nbg_gauss_legendre(16, r.gl16x, r.gl16w);
 		nbg_gauss_legendre(51, r.gl51x, r.gl51w);
 		if (nbg_quad_duffy(4, 0.0, 0.0, &r.duffy[0][0]) != 64) return fail(NB_ERR_RUNTIME, "Duffy rule");
+		for (int a = 0; a < 4; ++a) {
+			const double g = std::sqrt(3.0) / 3.0;
+			if (nbg_quad_duffy(4, (a & 1) ? g : -g, (a & 2) ? g : -g, &r.gduffy[a][0][0]) != 64)
+				return fail(NB_ERR_RUNTIME, "Duffy rule");
+		}
 		for (int d = 0; d < nbk::NSLOT; ++d) {
 			double xi[64], eta[64], w[64];
 			const int q = d <= 14 ? nbg_dunavant(d, xi, eta, w, 64) : 0;
@@ -305,6 +313,9 @@ struct nb_mesh {
 	std::unique_ptr<nb_mesh> view;
 	// quadrilateral near pass: count of pairs without an inverse image
 	DevPtr d_status;
+	// the gauss field's nodes as test points (quadrilaterals): x(xi_a) [3][4n]
+	// FP64, the unit normals there FP32 and FP64
+	DevPtr d_gx, d_gnuf, d_gnu64;
 	// the sweep's per-call buffers (wavenumber constants, FP64 static parts),
 	// kept between calls and grown when a call needs more: a hipFree per call
 	// would wait for every stream of the device
@@ -398,6 +409,50 @@ int ensure_qq(nb_mesh *m, int N)
 	int s = upload(m->d_qq[N], tab);
 	if (s) return s;
 	m->bytes += (int64_t)tab.size() * 4;
+	return NB_OK;
+}
+
+// the gauss field's nodes of a quadrilateral mesh (test points)
+int ensure_gauss_nodes(nb_mesh *m)
+{
+	if (m->d_gx) return NB_OK;
+	const size_t r = (size_t)4 * m->n;
+	std::vector<double> x(3 * r), nu(3 * r);
+	if (nbg_gauss_nodes(m->n, m->X.data(), x.data(), nu.data()))
+		return fail(NB_ERR_ARG, "degenerate quadrilateral (zero normal at a Gauss point)");
+	std::vector<float> nuf(nu.begin(), nu.end());
+	int s;
+	if ((s = upload(m->d_gx, x)) || (s = upload(m->d_gnu64, nu)) || (s = upload(m->d_gnuf, nuf))) return s;
+	m->bytes += (int64_t)r * (24 + 24 + 12);
+	return NB_OK;
+}
+
+// per-device shape-function tables of the gauss field by Gauss order
+std::mutex g_gshape_mu;
+std::map<int, DevPtr> g_gshape;
+
+int device_gauss_shapes(const nbk::GaussShapes **out)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_gshape_mu);
+	auto it = g_gshape.find(dev);
+	if (it == g_gshape.end()) {
+		std::unique_ptr<nbk::GaussShapes> g(new nbk::GaussShapes());
+		std::memset(g.get(), 0, sizeof(nbk::GaussShapes));
+		std::vector<double> v(4 * nbk::QN_MAX * nbk::QN_MAX);
+		for (int N = 1; N <= nbk::QN_MAX; ++N) {
+			if (nbg_gauss_shapes(N, v.data()) != N * N) return fail(NB_ERR_RUNTIME, "gauss field shape table");
+			for (int q = 0; q < N * N; ++q)
+				for (int b = 0; b < 4; ++b) g->n[N][q][b] = (float)v[4 * q + b];
+		}
+		DevPtr p;
+		int s = dev_alloc(p, sizeof(nbk::GaussShapes));
+		if (s) return s;
+		NB_HIP(hipMemcpy(p.get(), g.get(), sizeof(nbk::GaussShapes), hipMemcpyHostToDevice));
+		it = g_gshape.emplace(dev, std::move(p)).first;
+	}
+	*out = (const nbk::GaussShapes *)it->second.get();
 	return NB_OK;
 }
 
@@ -959,6 +1014,163 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	return NB_OK;
 }
 
+
+// Collocation with the quadrilateral gauss field on the trial mesh (4 dofs
+// per element, column 4 j + b): the regular pass (FP32), then the
+// near-singular and diagonal passes (FP64), disjoint entries.  Test side:
+// NB_COLLOCATION_GAUSS the dirac of the test mesh's gauss field (rows 4 i + a),
+// NB_COLLOCATION_GAUSS_TRIAL the dirac of its constant field (rows = elements).
+int run_gauss(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	nb_mesh *trial = jb.trial;
+	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
+	const bool tg = jb.formalism == NB_COLLOCATION_GAUSS;
+	int s;
+	if (trial->npe != 4) return fail(NB_ERR_UNSUPPORTED, "the gauss field lives on quadrilateral meshes");
+	if (tg && test->npe != 4) return fail(NB_ERR_UNSUPPORTED, "the gauss test field needs a quadrilateral test mesh");
+	if (!tg && test->npe == 0)
+		return fail(NB_ERR_UNSUPPORTED, "field points of the gauss field: a test mesh of one element type");
+	if (!tg && jb.same)
+		return fail(NB_ERR_UNSUPPORTED, "a constant test field against the gauss field is taken on another mesh only");
+	if (jb.pol.precision != NB_PREC_MIXED)
+		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
+	if ((s = check_quad_policy(jb.pol))) return s;
+	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
+	int fixed_n = 0;
+	if (jb.pol.rule == NB_RULE_FIXED) {
+		fixed_n = jb.pol.fixed_degree / 2 + 1;
+		if ((s = ensure_qq(trial, fixed_n))) return s;
+	} else {
+		// degree 2 + {0, 2, 4, 6, 8, 9}: Gauss orders 2..6
+		for (int N = 2; N <= 6; ++N)
+			if ((s = ensure_qq(trial, N))) return s;
+	}
+	if (tg && (s = ensure_gauss_nodes(test))) return s;
+	const nbk::GaussShapes *gs = nullptr;
+	if ((s = device_gauss_shapes(&gs))) return s;
+	std::shared_ptr<NearList> nl;
+	if ((s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
+	const int nodes = tg ? 4 : 1;
+	nbk::GaussArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.n_rows = nodes * test->n; a.n_cols = trial->n; a.same_mesh = jb.same; a.nodes = nodes;
+	a.tile_cols = jb.tile_cols > 0 ? jb.tile_cols : 32;
+	a.row_tiles = (a.n_rows + 255) / 256;
+	a.tn = a.n_rows; a.te = test->n;
+	a.tx = (const double *)(tg ? test->d_gx.get() : test->d_c.get());
+	a.tnu = (const float *)(tg ? test->d_gnuf.get() : test->d_nuf.get());
+	a.tc = (const double *)test->d_c.get();
+	a.rec = (const nbk::TrialRec *)trial->d_rec.get();
+	for (int N = 1; N <= nbk::QN_MAX; ++N) a.qt[N] = (const nbk::QuadPt *)trial->d_qq[N].get();
+	a.gs = gs;
+	a.fixed_n = fixed_n;
+	const double two_pi = 6.283185307179586476925286766559005768;
+	a.k = (float)jb.k;
+	a.k_rev = jb.k / two_pi;
+	a.k_revf = (float)a.k_rev;
+	a.ki = jb.ki;
+	a.alpha = make_float2((float)jb.alpha.re, (float)jb.alpha.im);
+	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
+	const long long grid = (long long)a.row_tiles * ((trial->n + a.tile_cols - 1) / a.tile_cols);
+	const bool ck = jb.ki != 0.0;
+	if (grid > 0) {
+		using namespace nbk;
+		const int pe = prof_begin(0, st);
+#define NB_GR(F, K, M) do { \
+		if (F == HELM && ck) hipLaunchKernelGGL((gauss_regular_kernel<F, K, M, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+		else hipLaunchKernelGGL((gauss_regular_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, a); } while (0)
+		if (jb.mode == OUT_BM) NB_GR(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_GR(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_GR(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_GR(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_GR(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_GR(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_GR(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_GR(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_GR(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_GR
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	const nbk::Rules *rules = nullptr;
+	if ((s = device_rules(&rules))) return s;
+	nbk::GaussNearArgs na;
+	std::memset(&na, 0, sizeof(na));
+	na.nodes = nodes;
+	na.tn = a.n_rows;
+	na.tx = (const double *)(tg ? test->d_gx.get() : test->d_c.get());
+	na.tnu = (const double *)(tg ? test->d_gnu64.get() : test->d_nu64.get());
+	na.sn = trial->n;
+	na.X = (const double *)trial->d_X.get();
+	na.k = jb.k;
+	na.ki = jb.ki;
+	na.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+	na.out0 = out0; na.out1 = out1; na.ld = ld; na.dtype = dtype; na.accumulate = accumulate;
+	na.rules = rules;
+	na.inv_iter = inverse_mapping_steps();
+	const bool check_images = nl->n > 0;
+	if (check_images) {
+		if (!trial->d_status && (s = dev_alloc(trial->d_status, 256))) return s;
+		na.status = (int *)trial->d_status.get();
+		launch_zero(na.status, 1, 1, 1, 4, st);
+		NB_HIP(hipGetLastError());
+	}
+	for (int pass = 0; pass < 2; ++pass) {
+		const bool diag = pass == 1;
+		if (diag && !jb.same) continue;
+		na.n_pairs = diag ? test->n : nl->n;
+		na.pi = diag ? nullptr : (const int *)nl->pi.get();
+		na.pj = diag ? nullptr : (const int *)nl->pj.get();
+		if (na.n_pairs == 0) continue;
+		// diagonal: one lane per (element, node); near: QLPP lanes per (pair, node)
+		const long long lanes = (long long)na.n_pairs * nodes * (diag ? 1 : nbk::QLPP);
+		const int g = (int)((lanes + 63) / 64);
+		const int pe = prof_begin(diag ? 2 : 1, st);
+		using namespace nbk;
+#define NB_G64(F, K, M) do { \
+		if (F == HELM && ck) { \
+			if (diag) hipLaunchKernelGGL((gauss_diag_kernel<F, K, M, (F == HELM)>), dim3(g), dim3(64), 0, st, na); \
+			else hipLaunchKernelGGL((gauss_near_kernel<F, K, M, (F == HELM)>), dim3(g), dim3(64), 0, st, na); \
+		} else if (diag) hipLaunchKernelGGL((gauss_diag_kernel<F, K, M>), dim3(g), dim3(64), 0, st, na); \
+		else hipLaunchKernelGGL((gauss_near_kernel<F, K, M>), dim3(g), dim3(64), 0, st, na); } while (0)
+		if (jb.mode == OUT_BM) NB_G64(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_G64(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_G64(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_G64(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_G64(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_G64(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_G64(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_G64(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_G64(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_G64
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	if (check_images) {
+		int failed = 0;
+		NB_HIP(hipMemcpyAsync(&failed, na.status, sizeof(int), hipMemcpyDeviceToHost, st));
+		NB_HIP(hipStreamSynchronize(st));
+		if (failed)
+			return fail(NB_ERR_RUNTIME, "Could not perform inverse mapping: " + std::to_string(failed) +
+				" near-singular quadrilateral pair(s) (their entries hold NaN)");
+	}
+	return NB_OK;
+}
+
 int run_device(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
 	if (jb.formalism != NB_COLLOCATION) return run_galerkin(jb, out0, ld, dtype, accumulate, st);
@@ -1249,6 +1461,8 @@ int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 
 int run_any(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
+	if (jb.formalism == NB_COLLOCATION_GAUSS || jb.formalism == NB_COLLOCATION_GAUSS_TRIAL)
+		return run_gauss(jb, out0, out1, ld, dtype, accumulate, st);
 	if (jb.test->npe == 0 || jb.trial->npe == 0) return run_mixed(jb, out0, out1, ld, dtype, accumulate, st);
 	return run_device(jb, out0, out1, ld, dtype, accumulate, st);
 }
@@ -1694,6 +1908,77 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 	return run_job(jb, out, nullptr, st);
 }
 
+
+// collocation with the quadrilateral gauss field (NB_COLLOCATION_GAUSS,
+// NB_COLLOCATION_GAUSS_TRIAL): rows = (4 or 1) x test elements, columns =
+// 4 x trial elements
+int assemble_gauss(const nb_mesh *test, const nb_mesh *trial, int same_mesh, int formalism, nb_kernel kernel,
+	nb_scalar alpha, const nb_policy *policy, const nb_out *out, hipStream_t st)
+{
+	const bool tg = formalism == NB_COLLOCATION_GAUSS;
+	const int64_t rows = (tg ? 4 : 1) * (int64_t)test->n, cols = 4 * (int64_t)trial->n;
+	int s;
+	if (trial->npe != 4) return fail(NB_ERR_UNSUPPORTED, "the gauss field lives on quadrilateral meshes");
+	if (tg && test->npe != 4) return fail(NB_ERR_UNSUPPORTED, "the gauss test field needs a quadrilateral test mesh");
+	if (kernel.family == NB_IDENTITY) {
+		if ((s = check_out(out, rows, cols, alpha.im != 0.0))) return s;
+		// local operator: the unit matrix over the dofs on the same mesh (the
+		// shape functions are Lagrange polynomials on the collocation nodes),
+		// nothing on different meshes (nihu/core/assembly.hpp:141-142)
+		if (!same_mesh) return NB_OK;
+		if (!tg) return fail(NB_ERR_UNSUPPORTED, "a constant test field against the gauss field is taken on another mesh only");
+		if (out->location == NB_HOST) {
+			const int eb = dtype_bytes(out->dtype);
+			for (int64_t j = 0; j < out->cols; ++j) {
+				char *col = (char *)out->ptr + j * out->ld * eb;
+				if (!out->accumulate) std::memset(col, 0, out->rows * eb);
+				char *e = col + j * eb;
+				switch (out->dtype) {
+				case NB_C64: ((float *)e)[0] += (float)alpha.re; ((float *)e)[1] += (float)alpha.im; break;
+				case NB_C128: ((double *)e)[0] += alpha.re; ((double *)e)[1] += alpha.im; break;
+				case NB_F32: *(float *)e += (float)alpha.re; break;
+				default: *(double *)e += alpha.re; break;
+				}
+			}
+			return NB_OK;
+		}
+		if (!out->accumulate) launch_zero(out->ptr, out->ld, out->rows, out->cols, dtype_bytes(out->dtype), st);
+		hipLaunchKernelGGL(nbk::identity_kernel, dim3((int)((rows + 255) / 256)), dim3(256), 0, st, out->ptr,
+			(long long)out->ld, (int)rows, out->dtype, alpha.re, alpha.im);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	}
+	if (kernel.family != NB_LAPLACE && kernel.family != NB_HELMHOLTZ) return fail(NB_ERR_ARG, "unknown kernel family");
+	if ((kernel.nx != 0 && kernel.nx != 1) || (kernel.ny != 0 && kernel.ny != 1))
+		return fail(NB_ERR_UNSUPPORTED, "normal derivative orders must be 0 or 1");
+	if (kernel.family == NB_HELMHOLTZ && !(kernel.k > 0.0)) return fail(NB_ERR_ARG, "wavenumber must be positive");
+	if (kernel.family == NB_HELMHOLTZ && !std::isfinite(kernel.k_im))
+		return fail(NB_ERR_ARG, "imaginary part of the wavenumber must be finite");
+	if ((s = check_policy(policy))) return s;
+	if ((s = check_out(out, rows, cols, kernel.family == NB_HELMHOLTZ || alpha.im != 0.0))) return s;
+	if (rows * 16 > ((int64_t)1 << 32)) return fail(NB_ERR_RANGE, "too many rows");
+	nb_mesh *tr = const_cast<nb_mesh *>(trial);
+	nb_mesh *te = const_cast<nb_mesh *>(test);
+	// the test mesh's node tables are built on first use too
+	std::unique_lock<std::mutex> l1(tr->mu, std::defer_lock), l2(te->mu, std::defer_lock);
+	if (te == tr) l1.lock();
+	else std::lock(l1, l2);
+	Job jb;
+	jb.test = test; jb.trial = tr; jb.formalism = formalism; jb.same = same_mesh ? 1 : 0;
+	jb.family = kernel.family;
+	jb.km = kernel_mask(kernel.nx, kernel.ny);
+	jb.mode = nbk::OUT_SINGLE;
+	jb.k = kernel.family == NB_HELMHOLTZ ? kernel.k : 0.0;
+	jb.ki = kernel.family == NB_HELMHOLTZ ? kernel.k_im : 0.0;
+	jb.alpha = alpha;
+	jb.pol = *policy;
+	{
+		std::lock_guard<std::mutex> lc(g_cfg_mu);
+		jb.tile_cols = g_cfg.tile_cols;
+	}
+	return run_job(jb, out, nullptr, st);
+}
+
 } // namespace
 
 extern "C" {
@@ -1856,11 +2141,14 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 {
 	if (!test || !trial) return fail(NB_ERR_ARG, "null mesh");
 	if (same_mesh && test != trial) return fail(NB_ERR_ARG, "same_mesh requires test == trial");
-	if (formalism != NB_COLLOCATION && formalism != NB_GALERKIN && formalism != NB_GALERKIN_LINEAR)
+	if (formalism != NB_COLLOCATION && formalism != NB_GALERKIN && formalism != NB_GALERKIN_LINEAR &&
+		formalism != NB_COLLOCATION_GAUSS && formalism != NB_COLLOCATION_GAUSS_TRIAL)
 		return fail(NB_ERR_ARG, "unknown formalism");
 	int s;
 	if ((s = current_device_matches(test)) || (s = current_device_matches(trial))) return s;
 	hipStream_t st = (hipStream_t)stream;
+	if (formalism == NB_COLLOCATION_GAUSS || formalism == NB_COLLOCATION_GAUSS_TRIAL)
+		return assemble_gauss(test, trial, same_mesh, formalism, kernel, alpha, policy, out, st);
 	if (formalism != NB_COLLOCATION) return assemble_galerkin(test, trial, same_mesh, formalism, kernel, alpha, policy, out, st);
 	if (kernel.family == NB_IDENTITY) {
 		if ((s = check_out(out, test->n, trial->n, alpha.im != 0.0))) return s;
@@ -1947,6 +2235,31 @@ int nb_assemble_bm_ck(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const n
 	Job jb;
 	jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION; jb.same = 1; jb.family = NB_HELMHOLTZ; jb.km = nbk::ALL4;
 	jb.mode = nbk::OUT_BM;
+	jb.k = k.re; jb.ki = k.im; jb.alpha = alpha; jb.pol = *policy;
+	{
+		std::lock_guard<std::mutex> lc(g_cfg_mu);
+		jb.tile_cols = g_cfg.tile_cols;
+	}
+	return run_job(jb, A, B, (hipStream_t)stream);
+}
+
+int nb_assemble_bm_gauss(const nb_mesh *mesh, nb_scalar k, nb_scalar alpha, const nb_policy *policy,
+	const nb_out *A, const nb_out *B, void *stream)
+{
+	if (!mesh) return fail(NB_ERR_ARG, "null mesh");
+	if (!(k.re > 0.0) || !std::isfinite(k.im)) return fail(NB_ERR_ARG, "wavenumber must have a positive real part");
+	if (mesh->npe != 4) return fail(NB_ERR_UNSUPPORTED, "the gauss field lives on quadrilateral meshes");
+	int s;
+	if ((s = current_device_matches(mesh))) return s;
+	if ((s = check_policy(policy))) return s;
+	const int64_t nd = 4 * (int64_t)mesh->n;
+	if ((s = check_out(A, nd, nd, true)) || (s = check_out(B, nd, nd, true))) return s;
+	if (nd * 16 > ((int64_t)1 << 32)) return fail(NB_ERR_RANGE, "too many rows");
+	nb_mesh *m = const_cast<nb_mesh *>(mesh);
+	std::lock_guard<std::mutex> lk(m->mu);
+	Job jb;
+	jb.test = m; jb.trial = m; jb.formalism = NB_COLLOCATION_GAUSS; jb.same = 1; jb.family = NB_HELMHOLTZ;
+	jb.km = nbk::ALL4; jb.mode = nbk::OUT_BM;
 	jb.k = k.re; jb.ki = k.im; jb.alpha = alpha; jb.pol = *policy;
 	{
 		std::lock_guard<std::mutex> lc(g_cfg_mu);

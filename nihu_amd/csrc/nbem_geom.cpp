@@ -386,6 +386,49 @@ int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out)
 	return N * N;
 }
 
+// Nodes of the quadrilateral gauss field (nihu/library/lib_shape.cpp:128-134:
+// (-,-), (+,-), (-,+), (+,+) at +-sqrt(3)/3): node a of element e is row
+// 4 e + a of x[3][4n] (the collocation point x(xi_a)) and nu[3][4n] (the unit
+// normal there).
+int nbg_gauss_nodes(int n, const double *X, double *x, double *nu)
+{
+	const int m = 4 * n;
+	const double g = std::sqrt(3.0) / 3.0;
+	for (int e = 0; e < n; ++e) {
+		P3 V[4];
+		load_quad(X, n, e, V);
+		for (int a = 0; a < 4; ++a) {
+			const double xi = (a & 1) ? g : -g, eta = (a & 2) ? g : -g;
+			const P3 y = quad_x(V, xi, eta);
+			const P3 nv = quad_normal(V, xi, eta);
+			const double j = nrm(nv);
+			if (!(j > 0.0)) return 2;
+			const int r = 4 * e + a;
+			x[r] = y.x; x[m + r] = y.y; x[2 * m + r] = y.z;
+			nu[r] = nv.x / j; nu[m + r] = nv.y / j; nu[2 * m + r] = nv.z / j;
+		}
+	}
+	return 0;
+}
+
+// Shape functions of the gauss field, N_b = (1 + s_b sqrt3 xi)(1 + t_b sqrt3
+// eta) / 4 (quad_1_gauss_shape_set.hpp), at the N x N Gauss square rule's
+// points in the order of nbg_quad4_table: out[q * 4 + b].
+int nbg_gauss_shapes(int N, double *out)
+{
+	if (N < 1 || N > 64) return -1;
+	std::vector<double> gx(N), gw(N);
+	nbg_gauss_legendre(N, gx.data(), gw.data());
+	const double s3 = std::sqrt(3.0);
+	for (int i = 0; i < N; ++i)
+		for (int j = 0; j < N; ++j)
+			for (int b = 0; b < 4; ++b) {
+				const double sx = (b & 1) ? 1.0 : -1.0, sy = (b & 2) ? 1.0 : -1.0;
+				out[(i * N + j) * 4 + b] = (1.0 + sx * s3 * gx[i]) * (1.0 + sy * s3 * gx[j]) / 4.0;
+			}
+	return N * N;
+}
+
 // Duffy-type blind rule around (x0, y0) of [-1, 1]^2 from the Gauss square
 // rule with ng x ng points: four triangles (x0, corner s, corner s+1), each
 // the image of the square under the bilinear map of the degenerate

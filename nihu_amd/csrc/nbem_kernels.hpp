@@ -625,6 +625,9 @@ struct Rules {
 	// order-7 Duffy blind rule around the centre of [-1, 1]^2 (xi, eta, w)
 	double gl3x[3], gl3w[3], gl5x[5], gl5w[5], gl8x[8], gl8w[8], gl16x[16], gl16w[16], gl51x[51], gl51w[51];
 	double duffy[64][3];
+	// the same rule around each node of the quadrilateral gauss field
+	// ((-,-), (+,-), (-,+), (+,+) at +-sqrt(3)/3)
+	double gduffy[4][64][3];
 	// reference points (xi, eta) of the Dunavant rule of each degree, FP32
 	// (the test-side shape functions of the regular Galerkin pass)
 	float2 ref[NSLOT][REF_MAXQ];

@@ -24,15 +24,21 @@ NB_PREC_MIXED, NB_PREC_FP64 = 0, 1
 NB_C64, NB_C128, NB_F32, NB_F64 = 0, 1, 2, 3
 NB_HOST, NB_DEVICE = 0, 1
 NB_COLLOCATION, NB_GALERKIN, NB_GALERKIN_LINEAR = 0, 1, 2
+# collocation with the quadrilateral gauss field (4 dofs per element at the
+# 2 x 2 Gauss points, NiHu's quad_1_gauss_field): gauss test field, or the
+# constant test field of another mesh (field points)
+NB_COLLOCATION_GAUSS, NB_COLLOCATION_GAUSS_TRIAL = 3, 4
 NB_TRIA_1_ID, NB_QUAD_1_ID = 32303, 32404   # element type ids of NiHu's tria_1 and quad_1
-FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_linear": NB_GALERKIN_LINEAR}
+FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_linear": NB_GALERKIN_LINEAR,
+              "collocation_gauss": NB_COLLOCATION_GAUSS, "collocation_gauss_trial": NB_COLLOCATION_GAUSS_TRIAL}
 
 # symbols declared in include/nbem.h
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_create_quads",
            "nb_mesh_create_elements", "nb_mesh_destroy", "nb_mesh_info",
            "nb_assemble", "nb_assemble_bm", "nb_assemble_sweep", "nb_classify", "nb_autotune", "nb_set_launch",
            "nb_synchronize", "nb_profile", "nb_profile_read", "nb_eval_math",
-           "nb_assemble_sweep_noalloc", "nb_gemv_workspace", "nb_gemv_c64", "nb_assemble_bm_ck"]
+           "nb_assemble_sweep_noalloc", "nb_gemv_workspace", "nb_gemv_c64", "nb_assemble_bm_ck",
+           "nb_assemble_bm_gauss"]
 
 
 class NbKernel(C.Structure):
@@ -86,6 +92,8 @@ def lib():
                                      C.POINTER(NbOut), P]
         l.nb_assemble_bm_ck.argtypes = [P, NbScalar, NbScalar, C.POINTER(NbPolicy), C.POINTER(NbOut),
                                         C.POINTER(NbOut), P]
+        l.nb_assemble_bm_gauss.argtypes = [P, NbScalar, NbScalar, C.POINTER(NbPolicy), C.POINTER(NbOut),
+                                           C.POINTER(NbOut), P]
         l.nb_assemble_sweep.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
                                         C.POINTER(NbLaunch), P]
         l.nb_assemble_sweep_noalloc.argtypes = [P, P, C.c_int32, C.POINTER(NbPolicy), P, P, C.c_int64, C.c_int64,
@@ -215,8 +223,12 @@ class Mesh:
 
     def dofs(self, formalism="collocation"):
         """rows / columns of this mesh's side of a matrix: nodes for
-        isoparametric Galerkin, elements otherwise"""
-        return self.n_nodes if FORMALISMS[formalism] == NB_GALERKIN_LINEAR else self.n
+        isoparametric Galerkin, 4 per element for the gauss field, elements
+        otherwise"""
+        f = FORMALISMS[formalism]
+        if f == NB_GALERKIN_LINEAR:
+            return self.n_nodes
+        return 4 * self.n if f == NB_COLLOCATION_GAUSS else self.n
 
     def device_bytes(self):
         b = C.c_int64()
@@ -306,6 +318,18 @@ def assemble_bm(mesh: Mesh, k, A, B, alpha=None, pol=None, accumulate=False, str
     else:
         check(lib().nb_assemble_bm(mesh.handle, float(k), NbScalar(a.real, a.imag), C.byref(pol), C.byref(oa),
                                    C.byref(ob), _stream_ptr(stream)))
+    return A, B
+
+
+def assemble_bm_gauss(mesh: Mesh, k, A, B, alpha=None, pol=None, accumulate=False, stream=None):
+    """Burton-Miller A and B (4 n x 4 n) with the quadrilateral gauss field;
+    alpha defaults to -i/k; k real or complex."""
+    pol = pol or policy(**NIHU_POLICY)
+    kc = complex(k)
+    a = complex(-1j / kc) if alpha is None else complex(alpha)
+    oa, ob = out_desc(A, accumulate), out_desc(B, accumulate)
+    check(lib().nb_assemble_bm_gauss(mesh.handle, NbScalar(kc.real, kc.imag), NbScalar(a.real, a.imag),
+                                     C.byref(pol), C.byref(oa), C.byref(ob), _stream_ptr(stream)))
     return A, B
 
 
