@@ -24,9 +24,10 @@ against the oracle), and `serve`, a Poisson stream of assembly-plus-solve
 requests served with and without batching (p50/p99 latency, solves/s) and a
 rate sweep up to the first rate the server does not keep up with; then short
 runs (5 timed steps, 32 rows or node groups against the oracle) of the Galerkin workload
-(config 3), the quadrilateral and the mixed-element workloads, each with its
-roofline: `galerkin`, `quad`, `mixed`, and config 1, the Laplace matrices of
-1280 triangles in FP64, `laplace_fp64` (--no-extras skips all of them).
+(config 3), the quadrilateral, the gauss-field and the mixed-element workloads,
+each with its roofline: `galerkin`, `quad`, `gauss`, `mixed`, and config 1, the
+Laplace matrices of 1280 triangles in FP64, `laplace_fp64` (--no-extras skips
+all of them).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 20]
 """
@@ -161,12 +162,13 @@ def main():
                     help="rocprofv3 FETCH/WRITE summary of the regular kernel (default: the one of --policy)")
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "mixed", "laplace"],
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "gauss", "mixed", "laplace"],
                     default="bm",
                     help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
                          "serve: Poisson stream of assembly-plus-LU-solve requests; galerkin: SLP and DLP "
                          "Galerkin matrices with linear triangles on the 20480-element ellipsoid; quad: "
-                         "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere; mixed: the "
+                         "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere; gauss: the "
+                         "same with the gauss field (4 dofs per element) on 5400 quadrilaterals; mixed: the "
                          "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table); "
                          "laplace: the Laplace SLP and DLP matrices of 1280 triangles in FP64 (plain method)")
     ap.add_argument("--requests", type=int, default=24, help="serve: requests per stream")
@@ -195,6 +197,9 @@ def main():
         return
     if args.workload == "quad":
         print(json.dumps(run_quad(args)))
+        return
+    if args.workload == "gauss":
+        print(json.dumps(run_gauss(args)))
         return
     if args.workload == "mixed":
         print(json.dumps(run_mixed(args)))
@@ -345,8 +350,8 @@ def main():
         # (Galerkin: the test elements of 32 sampled nodes against the
         # oracle), the quadrilateral and the mixed-element workloads (32 rows),
         # each with its roofline
-        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("mixed", run_mixed, 32),
-                              ("laplace_fp64", run_laplace, -1)):
+        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("gauss", run_gauss, 32),
+                              ("mixed", run_mixed, 32), ("laplace_fp64", run_laplace, -1)):
             rec = fn(argparse.Namespace(steps=5, warmup=5, cpu_rows=rows, profile_copy_bytes=0, level=5))
             extras[key] = {k: rec.get(k) for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
                                                    "cpu_baseline", "accuracy", "pass_ms")}
@@ -665,6 +670,101 @@ def run_quad(args):
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
     return out
+
+
+def run_gauss(args):
+    """Gauss-field workload: Burton-Miller A and B with NiHu's quad_1_gauss_field
+    (four dofs per quadrilateral at the 2 x 2 Gauss points, collocated there;
+    the default build of nihu/applications/helmholtz/helmholtz_3d_coll.cpp)
+    on NiHu's blown-up cube sphere with 30 x 30 quadrilaterals per face
+    (create_sphere_boundary.m, n0 = 15: 5400 elements, 21600 dofs), ka = 10,
+    NiHu's policy: Gauss degree 2 + the distance table per element pair
+    (FP32), polar / Telles near-singular blocks below 1.5 mean edge lengths
+    and Duffy / Guiggiani at the four nodes of each diagonal block (FP64)."""
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import sphere_boundary
+
+    torch.cuda.set_device(0)
+    nodes, quads = sphere_boundary(1.0, 15)
+    ne = len(quads)
+    nd = 4 * ne
+    k = 10.0
+    alpha = complex(0.0, -1.0 / k)
+    pol = nbem.policy(**nbem.NIHU_POLICY)
+    name, ncu, hbm = nbem.device_info()
+    log("device %s, %d CUs; cubed sphere %d quadrilaterals, %d gauss dofs; k = %g" % (name, ncu, ne, nd, k))
+    mesh = nbem.Mesh(nodes, quads)
+    A, B = nbem.colmajor(nd, nd), nbem.colmajor(nd, nd)
+    stream = torch.cuda.current_stream()
+    cls = nbem.classify(mesh, mesh, 1, 1, pol)
+    n_near = int((cls == -2).sum().item())
+    del cls
+    n_reg_blocks = ne * ne - n_near - ne
+    if args.profile_copy_bytes > 0:
+        copy_calibration(args.profile_copy_bytes)
+    for w in range(args.warmup):
+        nbem.assemble_bm_gauss(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+        torch.cuda.synchronize()
+        log("warmup step %d done" % w)
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nbem.assemble_bm_gauss(mesh, k, A, B, alpha=alpha, pol=pol, stream=stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    ms_per_step = wall / args.steps * 1e3
+    value = 4 * nd * nd * args.steps / wall
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    log("timed %d steps: %.3f ms/step, %.4g entries/s; per-pass device ms %s" % (args.steps, ms_per_step, value,
+                                                                                pass_ms))
+    reg_ms = pass_ms["regular"]
+    # A and B, complex64, 16 entries per regular element-pair block
+    write_bytes = 2 * 8 * 16 * n_reg_blocks
+    gbs = write_bytes / (reg_ms * 1e-3) / 1e9 if reg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(gbs / PEAK_HBM_GBS, 4),
+                "traffic": read_traffic(os.path.join(ROOT, "profiles", "gauss_regular_cubed30_traffic.txt")),
+                "kernel": "gauss_regular_kernel<HELM,ALL4,OUT_BM>", "kernel_ms": round(reg_ms, 4),
+                "algorithmic_bytes": write_bytes}
+    cpu = err = None
+    if args.cpu_rows != 0:
+        from oracle import oracle
+        nr = 512 if args.cpu_rows < 0 else args.cpu_rows
+        rows = np.sort(np.random.default_rng(123).choice(nd, nr, replace=False)).astype(np.int32)
+        opol = oracle.policy(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5)
+        threads = oracle.num_threads()
+        tc = time.perf_counter()
+        Ar, Br = oracle.assemble_bm_gauss(nodes, quads, k, alpha=alpha, pol=opol, rows=rows, threads=threads)
+        dt = time.perf_counter() - tc
+        cpu_value = 4 * nr * nd / dt
+        cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+               "sample": "%d of %d rows of A and B (4 operators), FP64 oracle, OpenMP over rows, %.1f s wall"
+                         % (nr, nd, dt), "host": host_cpu()}
+        ri = torch.from_numpy(rows.astype(np.int64)).cuda()
+        Ag, Bg = A[ri, :].cpu().numpy(), B[ri, :].cpu().numpy()
+        err = {"rows": nr, "A_rel_fro": float(np.linalg.norm(Ag - Ar) / np.linalg.norm(Ar)),
+               "B_rel_fro": float(np.linalg.norm(Bg - Br) / np.linalg.norm(Br)),
+               "A_max_abs": float(np.abs(Ag - Ar).max()), "B_max_abs": float(np.abs(Bg - Br).max())}
+        log("sampled-row error vs oracle: %s; cpu %.4g entries/s on %d threads" % (err, cpu_value, threads))
+    return {
+        "metric": "matrix entries/s, Helmholtz 3D coll. quadrilateral gauss field, 21600 dofs",
+        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64) regular pairs, fp64 near-singular and diagonal",
+        "data": "synthetic: NiHu's blown-up cube sphere (create_sphere_boundary.m, r0 = 1, n0 = 15)",
+        "config": {"workload": "Helmholtz 3D collocation with the quadrilateral gauss field, Burton-Miller A and B",
+                   "n_elements": ne, "n_dofs": nd, "k": k,
+                   "policy": "NiHu (Gauss degree 2 + distance table per element pair, near-singular d/R < 1.5: "
+                             "polar SLP, Telles DLP/DLPt/HSP; diagonal Duffy, Guiggiani HSP at the nodes)",
+                   "element_pairs": {"regular": n_reg_blocks, "near": n_near, "diagonal": ne}},
+        "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
+    }
 
 
 def run_mixed(args):

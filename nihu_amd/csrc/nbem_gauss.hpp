@@ -58,10 +58,41 @@ struct GaussArgs {
 	int dtype, accumulate;
 };
 
-// The kernels of MASK at the points of one trial element, each point's values
-// distributed over the four dofs with the shape functions: T[b][kernel].
-// Point geometry and phase as in quad_points.
+// One point's kernel values of MASK (quad_add into zero: E, -a Q, b Q,
+// nn Q - a b (3 Q - k^2 E) / r^2 with E = (ec, es), Q = E (1 + i th) / r^2,
+// and CK's attenuation and kappa^2).
 template <int FAM, int MASK, bool CK>
+__device__ __forceinline__ void gauss_vals(float ec, float es, float th, float rinv, float a, float b, float nn,
+	float k, float kih, float ki, float2 (&V)[4])
+{
+	if (MASK & SLP) V[0] = make_float2(ec, es);
+	if (MASK & (DLP | DLPT | HSP)) {
+		const float ri2 = rinv * rinv;
+		const float Gr = ec * ri2, Gi = es * ri2;
+		float Qr = __builtin_fmaf(-th, Gi, Gr), Qi = __builtin_fmaf(th, Gr, Gi);
+		if (CK) { Qr = __builtin_fmaf(-kih, Gr, Qr); Qi = __builtin_fmaf(-kih, Gi, Qi); }
+		if (MASK & DLP) V[1] = make_float2(-a * Qr, -a * Qi);
+		if (MASK & DLPT) V[2] = make_float2(b * Qr, b * Qi);
+		if (MASK & HSP) {
+			const float kk = FAM == HELM ? (CK ? __builtin_fmaf(k, k, -ki * ki) : k * k) : 0.0f;
+			const float ab = a * b * ri2;
+			float hr = __builtin_fmaf(3.0f, Qr, -kk * ec), hi = __builtin_fmaf(3.0f, Qi, -kk * es);
+			if (CK) {
+				const float kq = 2.0f * k * ki;
+				hr = __builtin_fmaf(kq, es, hr);
+				hi = __builtin_fmaf(-kq, ec, hi);
+			}
+			V[3] = make_float2(__builtin_fmaf(nn, Qr, -ab * hr), __builtin_fmaf(nn, Qi, -ab * hi));
+		}
+	}
+}
+
+// The kernels of MASK at the points of one trial element, each point's values
+// distributed over the four dofs with the shape functions: T[b][kernel].  Point geometry and phase as in quad_points.
+// O2: the order-2 rule, whose points are the field's nodes, so N_b(xi_q) is
+// 1 at the node of dof b and 0 at the others (to rounding, 6e-17): point q
+// goes to dof (0, 2, 1, 3)[q] (points xi outer, eta inner; dofs xi first).
+template <int FAM, int MASK, bool CK, bool O2 = false>
 __device__ __forceinline__ void gauss_points(const QuadPt *tab, const float *nsh, int nq, const QuadCtx &p,
 	float k_revf, float k, float ki, float2 (&T)[GDOF][4])
 {
@@ -69,7 +100,9 @@ __device__ __forceinline__ void gauss_points(const QuadPt *tab, const float *nsh
 	const auto nf = uni(nsh);
 	const float kil2e = ki * 1.4426950408889634f;
 	const float dcx2 = 2.0f * p.dcx, dcy2 = 2.0f * p.dcy, dcz2 = 2.0f * p.dcz;
-	for (int q = 0; q < nq; ++q) {
+	const int nqq = O2 ? 4 : nq;
+#pragma unroll (O2 ? 4 : 1)
+	for (int q = 0; q < nqq; ++q) {
 		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3];
 		const float nqx = qf[8 * q + 4], nqy = qf[8 * q + 5], nqz = qf[8 * q + 6], oo = qf[8 * q + 7];
 		const float u = __builtin_fmaf(dcx2, ox, __builtin_fmaf(dcy2, oy, __builtin_fmaf(dcz2, oz, oo)));
@@ -98,22 +131,36 @@ __device__ __forceinline__ void gauss_points(const QuadPt *tab, const float *nsh
 		if (MASK & (DLPT | HSP)) b = rx * p.nxx + ry * p.nxy + rz * p.nxz;
 		if (MASK & HSP) nn = p.nxx * nqx + p.nxy * nqy + p.nxz * nqz;
 		float2 V[4] = {zero2(), zero2(), zero2(), zero2()};
-		quad_add<FAM, MASK, CK>(ec, es, th, rinv, a, b, nn, k, V, kih, ki);
-#pragma unroll
-		for (int d = 0; d < GDOF; ++d) {
-			const float N = nf[GDOF * q + d];
+		gauss_vals<FAM, MASK, CK>(ec, es, th, rinv, a, b, nn, k, kih, ki, V);
+		if (O2) {
+			const int d = q == 1 ? 2 : q == 2 ? 1 : q;
 #pragma unroll
 			for (int t = 0; t < 4; ++t)
 				if (MASK & (1 << t)) {
-					T[d][t].x = __builtin_fmaf(N, V[t].x, T[d][t].x);
-					if (FAM == HELM) T[d][t].y = __builtin_fmaf(N, V[t].y, T[d][t].y);
+					T[d][t].x += V[t].x;
+					if (FAM == HELM) T[d][t].y += V[t].y;
 				}
+		} else {
+#pragma unroll
+			for (int d = 0; d < GDOF; ++d) {
+				const float N = nf[GDOF * q + d];
+#pragma unroll
+				for (int t = 0; t < 4; ++t)
+					if (MASK & (1 << t)) {
+						T[d][t].x = __builtin_fmaf(N, V[t].x, T[d][t].x);
+						if (FAM == HELM) T[d][t].y = __builtin_fmaf(N, V[t].y, T[d][t].y);
+					}
+			}
 		}
 	}
 }
 
 // Kernel groups at their own Gauss orders (SLP nS, DLP / DLPt nD, HSP nH;
 // <= 0 idle), each distinct order of the wave evaluated once, as quad_groups.
+// An order above 2 that no lane of the wave takes for SLP (or for SLP and the
+// double layers) is evaluated without those kernels: a lane's degrees grow
+// from SLP to DLP to HSP at every distance, so the higher orders mostly serve
+// the hypersingular group alone.
 template <int FAM, int KM, bool CK>
 __device__ __forceinline__ void gauss_groups(const GaussArgs &a, int j, int nS, int nD, int nH, const QuadCtx &p,
 	float2 (&K)[GDOF][4])
@@ -130,12 +177,22 @@ __device__ __forceinline__ void gauss_groups(const GaussArgs &a, int j, int nS, 
 		const int lane = __ffsll((long long)m) - 1;
 		const int nsel = __builtin_amdgcn_readlane(mine, lane);
 		const bool fS = pS == nsel, fD = pD == nsel, fH = pH == nsel;
+		const bool needS = __ballot(fS) != 0, needD = __ballot(fD) != 0;
 		if (fS || fD || fH) {
 			float2 t[GDOF][4];
 #pragma unroll
 			for (int d = 0; d < GDOF; ++d) t[d][0] = t[d][1] = t[d][2] = t[d][3] = zero2();
 			const int nq = nsel * nsel;
-			gauss_points<FAM, KM, CK>(a.qt[nsel] + (size_t)j * nq, &a.gs->n[nsel][0][0], nq, p, a.k_revf, a.k, ki, t);
+			const QuadPt *tab = a.qt[nsel] + (size_t)j * nq;
+			const float *nsh = &a.gs->n[nsel][0][0];
+			if (nsel == 2)
+				gauss_points<FAM, KM, CK, true>(tab, nullptr, 4, p, a.k_revf, a.k, ki, t);
+			else if (!needS && !needD)
+				gauss_points<FAM, KM & HSP, CK>(tab, nsh, nq, p, a.k_revf, a.k, ki, t);
+			else if (!needS)
+				gauss_points<FAM, KM & (DLP | DLPT | HSP), CK>(tab, nsh, nq, p, a.k_revf, a.k, ki, t);
+			else
+				gauss_points<FAM, KM, CK>(tab, nsh, nq, p, a.k_revf, a.k, ki, t);
 #pragma unroll
 			for (int d = 0; d < GDOF; ++d) {
 				if (fS) K[d][0] = t[d][0];
@@ -198,12 +255,16 @@ __global__ void __launch_bounds__(256) gauss_regular_kernel(GaussArgs a)
 		for (int dd = 0; dd < GDOF; ++dd) K[dd][0] = K[dd][1] = K[dd][2] = K[dd][3] = zero2();
 		if (a.fixed_n > 0) {
 			const int nq = a.fixed_n * a.fixed_n;
-			if (act)
-				gauss_points<FAM, KM, CK>(a.qt[a.fixed_n] + (size_t)j * nq, &a.gs->n[a.fixed_n][0][0], nq, p, a.k_revf,
-					a.k, ki, K);
+			if (act) {
+				if (a.fixed_n == 2)
+					gauss_points<FAM, KM, CK, true>(a.qt[2] + (size_t)j * 4, nullptr, 4, p, a.k_revf, a.k, ki, K);
+				else
+					gauss_points<FAM, KM, CK>(a.qt[a.fixed_n] + (size_t)j * nq, &a.gs->n[a.fixed_n][0][0], nq, p,
+						a.k_revf, a.k, ki, K);
+			}
 		} else if (__ballot(act && d2 < rec[j].thr[THR_22_3]) == 0) {
 			// beyond the last breakpoint every table degree is 0: order 2 for all kernels
-			if (act) gauss_points<FAM, KM, CK>(a.qt[2] + (size_t)j * 4, &a.gs->n[2][0][0], 4, p, a.k_revf, a.k, ki, K);
+			if (act) gauss_points<FAM, KM, CK, true>(a.qt[2] + (size_t)j * 4, nullptr, 4, p, a.k_revf, a.k, ki, K);
 		} else {
 			double t[NTHR];
 #pragma unroll
