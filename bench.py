@@ -350,7 +350,7 @@ def main():
         # (Galerkin: the test elements of 32 sampled nodes against the
         # oracle), the quadrilateral and the mixed-element workloads (32 rows),
         # each with its roofline
-        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("gauss", run_gauss, 32),
+        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("gauss", run_gauss, 1024),
                               ("mixed", run_mixed, 32), ("laplace_fp64", run_laplace, -1)):
             rec = fn(argparse.Namespace(steps=5, warmup=5, cpu_rows=rows, profile_copy_bytes=0, level=5))
             extras[key] = {k: rec.get(k) for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
