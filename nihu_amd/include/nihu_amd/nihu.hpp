@@ -222,7 +222,8 @@ inline mesh read_off_mesh(std::string const &fname, quad_1_tag, tria_1_tag) { re
 
 // ------------------------------------------------------------------ spaces
 
-enum space_kind { constant_kind = 0, isoparametric_kind = 1, dirac_constant_kind = 2 };
+enum space_kind { constant_kind = 0, isoparametric_kind = 1, dirac_constant_kind = 2, gauss_kind = 3,
+	dirac_gauss_kind = 4 };
 
 // constant_view: one dof per element (the element id)
 class constant_space {
@@ -261,9 +262,66 @@ private:
 	constant_space m_space;
 };
 
+// The quadrilateral gauss field (quad_1_gauss_field, nihu/library/
+// quad_1_gauss_field.hpp): four dofs per quadrilateral at the 2 x 2 Gauss
+// points, dof 4 e + c at node c in the order (-,-), (+,-), (-,+), (+,+)
+// (nihu/library/lib_shape.cpp:128-134).
+struct quad_1_gauss_field_tag {};
+enum { quad_1_gauss_field_id = 666 };   // field_traits::id<quad_1_gauss_field>
+
+class gauss_space {
+public:
+	static const int kind = gauss_kind;
+	explicit gauss_space(mesh const &m) : m_mesh(m)
+	{
+		if (m.nodes_per_element() != 4) throw std::invalid_argument("the gauss field needs a quadrilateral mesh");
+	}
+	mesh const &get_mesh() const { return m_mesh; }
+	int32_t get_num_dofs() const { return 4 * m_mesh.num_elements(); }
+
+private:
+	mesh m_mesh;
+};
+
+// dirac(gauss field): collocation at the field's nodes
+class dirac_gauss_space {
+public:
+	static const int kind = dirac_gauss_kind;
+	explicit dirac_gauss_space(gauss_space const &s) : m_space(s) {}
+	mesh const &get_mesh() const { return m_space.get_mesh(); }
+	int32_t get_num_dofs() const { return m_space.get_num_dofs(); }
+
+private:
+	gauss_space m_space;
+};
+
 inline constant_space constant_view(mesh const &m) { return constant_space(m); }
 inline isoparametric_space isoparametric_view(mesh const &m) { return isoparametric_space(m); }
 inline dirac_space dirac(constant_space const &s) { return dirac_space(s); }
+inline dirac_gauss_space dirac(gauss_space const &s) { return dirac_gauss_space(s); }
+
+// create_function_space(nodes, fields, quad_1_gauss_field_tag()) as
+// nihu/core/function_space.hpp:613-617 takes it, with the field matrix rows
+// that nihu/applications/helmholtz/helmholtz_3d_coll.cpp:228-237 and
+// benchmark/EAA/sphere/homogeneous/bem/bem.cpp:209-219 build:
+// [666, n0, n1, n2, n3, d0, d1, d2, d3] with dofs d_c = 4 e + c (the
+// numbering this engine assembles; another one throws).
+inline gauss_space create_function_space(std::vector<double> const &nodes, std::vector<unsigned> const &fields,
+	quad_1_gauss_field_tag)
+{
+	if (fields.size() % 9) throw std::invalid_argument("field rows must be [666, 4 node ids, 4 dof ids]");
+	const size_t m = fields.size() / 9;
+	std::vector<int32_t> el(4 * m);
+	for (size_t e = 0; e < m; ++e) {
+		unsigned const *r = &fields[9 * e];
+		if (r[0] != (unsigned)quad_1_gauss_field_id) throw std::invalid_argument("field type id must be 666");
+		for (int c = 0; c < 4; ++c) {
+			el[4 * e + c] = (int32_t)r[1 + c];
+			if (r[5 + c] != 4 * e + c) throw std::invalid_argument("gauss field dofs must be numbered 4 e + c");
+		}
+	}
+	return gauss_space(mesh(nodes, el, 4));
+}
 
 // a space as the assembly sees it: mesh and kind
 struct space_ref {
@@ -346,8 +404,8 @@ private:
 template <class Space>
 integral_transform integral_operator::operator[](Space const &trial) const
 {
-	static_assert(std::is_same<Space, constant_space>::value || std::is_same<Space, isoparametric_space>::value,
-		"trial spaces: constant_view or isoparametric_view");
+	static_assert(std::is_same<Space, constant_space>::value || std::is_same<Space, isoparametric_space>::value ||
+		std::is_same<Space, gauss_space>::value, "trial spaces: constant_view, isoparametric_view or the gauss field");
 	return integral_transform(*this, make_space_ref(trial));
 }
 
@@ -357,6 +415,8 @@ inline int formalism_of(int test_kind, int trial_kind)
 	if (test_kind == dirac_constant_kind && trial_kind == constant_kind) return NB_COLLOCATION;
 	if (test_kind == constant_kind && trial_kind == constant_kind) return NB_GALERKIN;
 	if (test_kind == isoparametric_kind && trial_kind == isoparametric_kind) return NB_GALERKIN_LINEAR;
+	if (test_kind == dirac_gauss_kind && trial_kind == gauss_kind) return NB_COLLOCATION_GAUSS;
+	if (test_kind == dirac_constant_kind && trial_kind == gauss_kind) return NB_COLLOCATION_GAUSS_TRIAL;
 	return -1;
 }
 
@@ -370,8 +430,8 @@ public:
 	weighted_residual(space_ref const &test, integral_transform const &tr)
 	{
 		if (formalism_of(test.kind, tr.trial().kind) < 0)
-			throw std::logic_error("test and trial spaces: dirac(constant) x constant, constant x constant or "
-				"isoparametric x isoparametric");
+			throw std::logic_error("test and trial spaces: dirac(constant) x constant, constant x constant, "
+				"isoparametric x isoparametric, dirac(gauss) x gauss or dirac(constant) x gauss");
 		m_terms.push_back(term{tr, test});
 	}
 	std::vector<term> const &terms() const { return m_terms; }
@@ -390,8 +450,8 @@ template <class Space>
 weighted_residual operator*(Space const &test, integral_transform const &tr)
 {
 	static_assert(std::is_same<Space, dirac_space>::value || std::is_same<Space, constant_space>::value ||
-		std::is_same<Space, isoparametric_space>::value, "test spaces: dirac(constant_view), constant_view, "
-		"isoparametric_view");
+		std::is_same<Space, isoparametric_space>::value || std::is_same<Space, dirac_gauss_space>::value,
+		"test spaces: dirac(constant_view), constant_view, isoparametric_view, dirac(gauss field)");
 	return weighted_residual(make_space_ref(test), tr);
 }
 

@@ -208,6 +208,40 @@ static double burton_miller(nihu_amd::mesh const &m)
 	return std::sqrt(e / nrm);
 }
 
+// the same composition over the quadrilateral gauss field against
+// nb_assemble_bm_gauss
+static double burton_miller_gauss(nihu_amd::gauss_space const &trial_space)
+{
+	auto test_space = nihu_amd::dirac(trial_space);
+	const size_t N = trial_space.get_num_dofs();
+	double k = 4.0;
+	std::complex<double> alpha(0., -1. / k);
+	auto Gop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(k));
+	auto Hop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLP_kernel<double>(k));
+	auto Htop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_DLPt_kernel<double>(k));
+	auto Dop = nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_HSP_kernel<double>(k));
+	auto Iop = nihu_amd::identity_integral_operator();
+	cMatrix Gs(N, N), Hs(N, N);
+	Gs << test_space * Gop[trial_space];
+	Gs << test_space * (alpha * Htop)[trial_space];
+	Gs << test_space * (alpha / 2. * Iop)[trial_space];
+	Hs << test_space * (alpha * Dop)[trial_space];
+	Hs << test_space * (-.5 * Iop)[trial_space];
+	Hs << test_space * Hop[trial_space];
+	cMatrix A(N, N), B(N, N);
+	nb_out oa = {A.data(), (int64_t)N, (int64_t)N, (int64_t)N, NB_C128, NB_HOST, 0};
+	nb_out ob = {B.data(), (int64_t)N, (int64_t)N, (int64_t)N, NB_C128, NB_HOST, 0};
+	nihu_amd::check(nb_assemble_bm_gauss(trial_space.get_mesh().handle(), nb_scalar{k, 0.0},
+		nb_scalar{alpha.real(), alpha.imag()}, &nihu_amd::default_policy(), &oa, &ob, nullptr));
+	double e = 0, nrm = 0;
+	for (size_t i = 0; i < N; ++i)
+		for (size_t j = 0; j < N; ++j) {
+			e += std::norm(A(i, j) - Hs(i, j)) + std::norm(B(i, j) - Gs(i, j));
+			nrm += std::norm(Hs(i, j)) + std::norm(Gs(i, j));
+		}
+	return std::sqrt(e / nrm);
+}
+
 int main(int argc, char *argv[])
 {
 	if (argc < 2) {
@@ -283,6 +317,29 @@ int main(int argc, char *argv[])
 		double qbm = burton_miller(qmesh);
 		std::printf("Quadrilateral Burton-Miller composition vs fused assembly: %g\n", qbm);
 		bad |= !(qbm < 1e-6);
+		// the gauss field as helmholtz_3d_coll.cpp:228-240 builds it
+		std::vector<unsigned> fields;
+		for (int e = 0; e < qmesh.num_elements(); ++e) {
+			fields.push_back(nihu_amd::quad_1_gauss_field_id);
+			for (int c = 0; c < 4; ++c) fields.push_back((unsigned)qmesh.elements()[4 * e + c]);
+			for (int c = 0; c < 4; ++c) fields.push_back((unsigned)(4 * e + c));
+		}
+		auto gspace = nihu_amd::create_function_space(qmesh.nodes(), fields, nihu_amd::quad_1_gauss_field_tag());
+		double gp = pulsating(nihu_amd::dirac(gspace), gspace);
+		std::printf("Pulsating sphere relative error (quadrilateral gauss field): %g\n", gp);
+		bad |= !(gp < 0.05);
+		double gbm = burton_miller_gauss(gspace);
+		std::printf("Gauss-field Burton-Miller composition vs fused assembly: %g\n", gbm);
+		bad |= !(gbm < 1e-6);
+		// another dof numbering is refused
+		std::swap(fields[5], fields[6]);
+		try {
+			nihu_amd::create_function_space(qmesh.nodes(), fields, nihu_amd::quad_1_gauss_field_tag());
+			std::printf("no exception for a permuted gauss dof numbering\n");
+			bad = 1;
+		} catch (std::invalid_argument const &ex) {
+			std::printf("gauss dof numbering: invalid_argument as expected: %s\n", ex.what());
+		}
 		// the Galerkin formalism is not implemented for quadrilaterals
 		try {
 			nihu_amd::cMatrix X(qspace.get_num_dofs(), qspace.get_num_dofs());

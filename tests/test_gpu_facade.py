@@ -3,7 +3,8 @@ known-answer programs built against libnbem.so, on the reference's
 sphere_1_3.off mesh fixture (218 nodes, 432 triangles): transparent and
 pulsating-sphere answers in collocation, constant Galerkin and isoparametric
 Galerkin, and NiHu's exceptions; on a 384-element quadrilateral cubed sphere
-(OFF file): the collocation answers and the Burton-Miller composition; the
+(OFF file): the collocation answers and the Burton-Miller composition, also
+with the gauss field (create_function_space with NiHu's field rows); the
 same on a cubed sphere whose every third quadrilateral is split into two
 triangles (one OFF file with both face types)."""
 
@@ -37,6 +38,8 @@ def test_cpp_facade_known_answers(tmp_path):
                  "Pulsating sphere relative error (collocation, complex wave number)",
                  "Galerkin isoparametric transparent error", "Quadrilateral collocation transparent error",
                  "Quadrilateral Burton-Miller composition", "quadrilateral Galerkin: logic_error as expected",
+                 "Pulsating sphere relative error (quadrilateral gauss field)",
+                 "Gauss-field Burton-Miller composition", "gauss dof numbering: invalid_argument as expected",
                  "Mixed-mesh collocation transparent error", "Mixed-mesh Burton-Miller composition",
                  "mixed-mesh Galerkin: logic_error as expected"):
         assert line in r.stdout
