@@ -588,8 +588,10 @@ __device__ __forceinline__ void gauss_store64(const GaussNearArgs &a, int row, i
 
 __device__ __forceinline__ void gauss_load(const GaussNearArgs &a, int row, int j, Quad &Q, D3 &x0, D3 &nx)
 {
+	D3 X[4];
 	for (int v = 0; v < 4; ++v)
-		Q.X[v] = {a.X[(3 * v) * a.sn + j], a.X[(3 * v + 1) * a.sn + j], a.X[(3 * v + 2) * a.sn + j]};
+		X[v] = {a.X[(3 * v) * a.sn + j], a.X[(3 * v + 1) * a.sn + j], a.X[(3 * v + 2) * a.sn + j]};
+	Q = quad_from_corners(X);
 	x0 = {a.tx[row], a.tx[a.tn + row], a.tx[2 * a.tn + row]};
 	nx = {a.tnu[row], a.tnu[a.tn + row], a.tnu[2 * a.tn + row]};
 }
