@@ -727,7 +727,7 @@ __global__ void __launch_bounds__(64) quad_diag_kernel(NearArgs a)
 constexpr int QLPP = 4;
 
 template <int FAM, int KM, int MODE, bool CK = false>
-__global__ void __launch_bounds__(64) quad_near_kernel(NearArgs a)
+__global__ void __launch_bounds__(64, 3) quad_near_kernel(NearArgs a)
 {
 	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
 	const int p = gid / QLPP, lane = gid % QLPP;
