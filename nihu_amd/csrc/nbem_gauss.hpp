@@ -101,7 +101,7 @@ __device__ __forceinline__ void gauss_points(const QuadPt *tab, const float *nsh
 	const float kil2e = ki * 1.4426950408889634f;
 	const float dcx2 = 2.0f * p.dcx, dcy2 = 2.0f * p.dcy, dcz2 = 2.0f * p.dcz;
 	const int nqq = O2 ? 4 : nq;
-#pragma unroll (O2 ? 4 : 1)
+#pragma unroll (O2 ? 4 : 2)
 	for (int q = 0; q < nqq; ++q) {
 		const float ox = qf[8 * q], oy = qf[8 * q + 1], oz = qf[8 * q + 2], w = qf[8 * q + 3];
 		const float nqx = qf[8 * q + 4], nqy = qf[8 * q + 5], nqz = qf[8 * q + 6], oo = qf[8 * q + 7];

@@ -11,7 +11,8 @@ the FP64 oracle on the same inputs:
 * the Burton-Miller pair and the identity over the dofs; field points (a
   constant test field on another mesh) against the gauss trial field;
 * sampled rows of a mesh with many row tiles; the pulsating-sphere answer
-  from the device matrices; the refusals.
+  from the device matrices; host outputs with accumulation; a failed inverse
+  mapping (forced from the host) as an error; the refusals.
 """
 
 import numpy as np
@@ -169,6 +170,43 @@ def test_gauss_pulsating_sphere(nb):
     err = np.abs(p - panal).max() / abs(panal)
     print("gauss-field pulsating sphere on the device: %.2e" % err, flush=True)
     assert err < 0.015
+
+
+def test_gauss_host_output_accumulates(nb, orc, qmesh):
+    """Host outputs (numpy, Fortran order) with accumulation: two calls give
+    twice the oracle's matrix."""
+    nodes, quads = qmesh
+    m = nb.Mesh(nodes, quads)
+    nd = 4 * m.n
+    out = np.zeros((nd, nd), dtype=np.complex128, order="F")
+    for _ in range(2):
+        nb.assemble(m, m, out, nb.NB_HELMHOLTZ, 0, 1, 2.0, pol=nb.policy(**NIHU[0]), accumulate=True,
+                    formalism="collocation_gauss")
+    ref = orc.assemble_gauss(nodes, quads, 1, 0, 1, k=2.0, pol=orc.policy(**NIHU[1]))
+    assert relfro(out, 2 * ref) <= 1e-5
+
+
+def test_gauss_inverse_mapping_failure_is_an_error(nb, qmesh):
+    """As for constant quadrilaterals: a near-singular block whose collocation
+    point has no image makes the call fail (NiHu throws), forced from the host
+    with NBEM_INVERSE_MAPPING_STEPS=1 so that every kernel still finishes."""
+    import os
+    import torch
+    nodes, quads = qmesh
+    m = nb.Mesh(nodes, quads)
+    nd = 4 * m.n
+    A, B = nb.colmajor(nd, nd), nb.colmajor(nd, nd)
+    pol = nb.policy(**NIHU[0])
+    os.environ["NBEM_INVERSE_MAPPING_STEPS"] = "1"
+    try:
+        with pytest.raises(nb.NbemError, match="inverse mapping"):
+            nb.assemble_bm_gauss(m, 3.0, A, B, pol=pol)
+    finally:
+        del os.environ["NBEM_INVERSE_MAPPING_STEPS"]
+    torch.cuda.synchronize()
+    nb.assemble_bm_gauss(m, 3.0, A, B, pol=pol)
+    torch.cuda.synchronize()
+    assert torch.isfinite(torch.view_as_real(A)).all() and torch.isfinite(torch.view_as_real(B)).all()
 
 
 def test_gauss_refusals(nb, qmesh):
