@@ -96,8 +96,7 @@ typedef struct nb_mesh nb_mesh;
  * k + i k_im (NiHu's helmholtz_kernel<space, WaveNumber> with a real or
  * complex WaveNumber, library/helmholtz_kernel.hpp:103; k > 0; a nonzero
  * k_im is taken in collocation on every mesh type and in the Galerkin
- * formalism on triangles, not with the FP64 precision policy on meshes with
- * quadrilaterals: NB_ERR_UNSUPPORTED) */
+ * formalism on triangles) */
 typedef struct {
 	int32_t family;
 	int32_t nx;
@@ -115,8 +114,9 @@ typedef struct {
 /* arithmetic of the regular pairs (near-singular and singular pairs are FP64 in both) */
 enum nb_precision {
 	NB_PREC_MIXED = 0,     /* regular pairs in FP32 with the phase k r reduced in FP64 (default) */
-	NB_PREC_FP64 = 1       /* every pair in FP64: NiHu's plain method (collocation on triangles;
-	                          quadrilaterals and the Galerkin formalism: NB_ERR_UNSUPPORTED) */
+	NB_PREC_FP64 = 1       /* every pair in FP64: NiHu's plain method (collocation with constant
+	                          fields on triangles, quadrilaterals and meshes of both; the
+	                          Galerkin formalism and the gauss field: NB_ERR_UNSUPPORTED) */
 };
 
 typedef struct {

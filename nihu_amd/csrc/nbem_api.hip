@@ -57,6 +57,7 @@ int nbg_geometry_quad(const double *xyz, int n_nodes, const int32_t *quad, int n
 int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out);
 int nbg_quad_duffy(int ng, double x0, double y0, double *out);
 int nbg_gauss_nodes(int n, const double *X, double *x, double *nu);
+int nbg_quad4_table64(int N, int n, const double *X, double *out);
 int nbg_gauss_shapes(int N, double *out);
 }
 
@@ -293,6 +294,7 @@ struct nb_mesh {
 	int qn[nbk::NSLOT] = {0};
 	DevPtr d_qt64[nbk::NSLOT];      // FP64 point tables of the FP64 precision policy
 	DevPtr d_qq[nbk::QN_MAX + 1];              // quadrilaterals: Gauss square tables by order
+	DevPtr d_qq64[nbk::QN_MAX + 1];            // the same in FP64 (y, w J, unit normal): FP64 precision policy
 	// near-pair lists per (test mesh, near policy), oldest first; callers hold
 	// a list by shared_ptr, so evicting it cannot free one in use
 	std::vector<std::shared_ptr<NearList> > near_cache;
@@ -409,6 +411,20 @@ int ensure_qq(nb_mesh *m, int N)
 	int s = upload(m->d_qq[N], tab);
 	if (s) return s;
 	m->bytes += (int64_t)tab.size() * 4;
+	return NB_OK;
+}
+
+// FP64 Gauss square table of order N (the FP64 precision policy)
+int ensure_qq64(nb_mesh *m, int N)
+{
+	if (N < 1 || N > nbk::QN_MAX) return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
+	if (m->d_qq64[N]) return NB_OK;
+	std::vector<double> tab((size_t)m->n * N * N * 8);
+	if (nbg_quad4_table64(N, m->n, m->X.data(), tab.data()) != N * N)
+		return fail(NB_ERR_RUNTIME, "quadrilateral quadrature table");
+	int s = upload(m->d_qq64[N], tab);
+	if (s) return s;
+	m->bytes += (int64_t)tab.size() * 8;
 	return NB_OK;
 }
 
@@ -833,18 +849,18 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 	nb_mesh *trial = jb.trial;
 	const nb_mesh *test = jb.test;
 	int s;
-	if (jb.pol.precision != NB_PREC_MIXED)
-		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
+	// the FP64 precision policy: the regular pairs in FP64 too
+	const bool fp64 = jb.pol.precision == NB_PREC_FP64;
 	if ((s = check_quad_policy(jb.pol))) return s;
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	int fixed_n = 0;
 	if (jb.pol.rule == NB_RULE_FIXED) {
 		fixed_n = jb.pol.fixed_degree / 2 + 1;
-		if ((s = ensure_qq(trial, fixed_n))) return s;
+		if ((s = fp64 ? ensure_qq64(trial, fixed_n) : ensure_qq(trial, fixed_n))) return s;
 	} else {
 		// degree 1 + {0, 2, 4, 6, 8, 9}: Gauss orders 1..6
 		for (int N = 1; N <= 6; ++N)
-			if ((s = ensure_qq(trial, N))) return s;
+			if ((s = fp64 ? ensure_qq64(trial, N) : ensure_qq(trial, N))) return s;
 	}
 	std::shared_ptr<NearList> nl;
 	if ((jb.passes & 2) && (s = ensure_near_list(trial, test, jb.same, jb.pol, st, &nl))) return s;
@@ -882,7 +898,47 @@ int run_quad(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int a
 		NB_HIP(hipEventRecord(ev_fork, st));
 		NB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
 	}
-	if (grid > 0 && (jb.passes & 1)) {
+	if (grid > 0 && (jb.passes & 1) && fp64) {
+		using namespace nbk;
+		const int pe = prof_begin(0, st);
+		Quad64Args b;
+		std::memset(&b, 0, sizeof(b));
+		b.n_rows = a.n_rows; b.n_cols = a.n_cols; b.same_mesh = a.same_mesh;
+		b.row_tiles = a.row_tiles; b.tile_cols = a.tile_cols;
+		b.rmap = jb.rmap; b.cmap = jb.cmap;
+		b.tn = test->n; b.sn = trial->n;
+		b.tc = (const double *)test->d_c.get();
+		b.tnu = (const double *)test->d_nu64.get();
+		b.sc = (const double *)trial->d_c.get();
+		b.thr = (const double *)trial->d_thr.get();
+		for (int N = 1; N <= QN_MAX; ++N) b.qt[N] = (const double *)trial->d_qq64[N].get();
+		b.fixed_n = fixed_n;
+		b.k = jb.k; b.ki = jb.ki;
+		b.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+		b.out0 = out0; b.out1 = out1; b.ld = ld; b.dtype = dtype; b.accumulate = accumulate;
+#define NB_Q64R(F, K, M) do { \
+		if (F == HELM && ck) hipLaunchKernelGGL((quad_regular64_kernel<F, K, M, (F == HELM)>), dim3(grid), dim3(256), 0, st, b); \
+		else hipLaunchKernelGGL((quad_regular64_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, b); } while (0)
+		if (jb.mode == OUT_BM) NB_Q64R(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_Q64R(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_Q64R(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_Q64R(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_Q64R(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_Q64R(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_Q64R(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_Q64R(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_Q64R(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_Q64R
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	} else if (grid > 0 && (jb.passes & 1)) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 		const bool mapped = jb.rmap || jb.cmap;
@@ -1425,9 +1481,6 @@ int run_mixed(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 	const int nt = mesh_parts(jb.test, tp, tid), ns = mesh_parts(jb.trial, sp, sid);
 	int s;
 	// refuse before any block has run
-	if (jb.pol.precision != NB_PREC_MIXED)
-		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on meshes of "
-			"triangles only");
 	for (int b = 0; b < ns; ++b)
 		if (sp[b]->npe == 4 && (s = check_quad_policy(jb.pol))) return s;
 	// a test mesh with both types: the regular pairs of a trial part against

@@ -386,6 +386,32 @@ int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out)
 	return N * N;
 }
 
+// FP64 Gauss square rule with N x N points (xi outer, eta inner) for all
+// quadrilaterals: out[(e * N*N + q) * 8 + {0..7}] = y_q xyz, w_q J(xi_q),
+// unit normal at y_q xyz, 0 (the FP64 precision policy's regular pass).
+int nbg_quad4_table64(int N, int n, const double *X, double *out)
+{
+	if (N < 1 || N > 64) return -1;
+	std::vector<double> gx(N), gw(N);
+	nbg_gauss_legendre(N, gx.data(), gw.data());
+	for (int e = 0; e < n; ++e) {
+		P3 V[4];
+		load_quad(X, n, e, V);
+		for (int i = 0; i < N; ++i)
+			for (int j = 0; j < N; ++j) {
+				const P3 y = quad_x(V, gx[i], gx[j]);
+				const P3 nv = quad_normal(V, gx[i], gx[j]);
+				const double J = nrm(nv);
+				double *p = out + ((size_t)e * N * N + (size_t)i * N + j) * 8;
+				p[0] = y.x; p[1] = y.y; p[2] = y.z;
+				p[3] = gw[i] * gw[j] * J;
+				p[4] = nv.x / J; p[5] = nv.y / J; p[6] = nv.z / J;
+				p[7] = 0.0;
+			}
+	}
+	return N * N;
+}
+
 // Nodes of the quadrilateral gauss field (nihu/library/lib_shape.cpp:128-134:
 // (-,-), (+,-), (-,+), (+,+) at +-sqrt(3)/3): node a of element e is row
 // 4 e + a of x[3][4n] (the collocation point x(xi_a)) and nu[3][4n] (the unit

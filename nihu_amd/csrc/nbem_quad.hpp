@@ -670,6 +670,86 @@ __device__ bool quad_near_telles(const Quad &Q, D3 x0, D3 nx0, double k, const R
 	return true;
 }
 
+// ---------------------------------------------------------------- FP64 regular pass
+
+// The FP64 precision policy: every regular pair in FP64 (NiHu's double
+// kernels), one group of kernels at a time at its own Gauss order, the
+// points from a per-element FP64 table (y, w J, unit normal).
+struct Quad64Args {
+	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	const int *rmap, *cmap;      // result row / column of each test / trial element (null: its own index)
+	int tn, sn;
+	const double *tc, *tnu;      // test collocation points, unit normals [3][tn]
+	const double *sc;            // trial centres [3][sn]
+	const double *thr;           // [NTHR][sn]
+	const double *qt[QN_MAX + 1];   // per Gauss order N: [sn][N * N][8]
+	int fixed_n;                 // > 0: fixed rule with fixed_n x fixed_n points
+	double k;
+	double ki;                   // imaginary part of a complex wavenumber (kernels with CK)
+	double2 alpha;
+	void *out0, *out1;
+	long long ld;
+	int dtype, accumulate;
+};
+
+template <int FAM, int MASK, bool CK>
+__device__ __forceinline__ void quad_eval64(const Quad64Args &a, int j, int n, D3 x0, D3 nx, double2 (&s)[4])
+{
+	if (n <= 0) return;
+	const int nq = n * n;
+	const double *t = a.qt[n] + (size_t)j * nq * 8;
+	for (int q = 0; q < nq; ++q) {
+		const D3 y = {t[8 * q], t[8 * q + 1], t[8 * q + 2]};
+		const D3 ny = {t[8 * q + 4], t[8 * q + 5], t[8 * q + 6]};
+		kernel_point<FAM, MASK, CK>(x0, y, nx, ny, a.k, t[8 * q + 3], s, a.ki);
+	}
+}
+
+template <int FAM, int KM, int MODE, bool CK = false>
+__global__ void __launch_bounds__(256) quad_regular64_kernel(Quad64Args a)
+{
+	const int tile_r = blockIdx.x % a.row_tiles;
+	const int tile_c = blockIdx.x / a.row_tiles;
+	const int i = tile_r * 256 + threadIdx.x;
+	if (i >= a.n_rows) return;
+	const int orow = a.rmap ? a.rmap[i] : i;
+	const D3 x0 = {a.tc[i], a.tc[a.tn + i], a.tc[2 * a.tn + i]};
+	const D3 nx = {a.tnu[i], a.tnu[a.tn + i], a.tnu[2 * a.tn + i]};
+	const int j0 = tile_c * a.tile_cols;
+	const int j1 = min(j0 + a.tile_cols, a.n_cols);
+	for (int j = j0; j < j1; ++j) {
+		// classification exactly as quad_regular_kernel: d2 from c_j - c_i in FP64
+		const double d2 = d2_pair(a.sc[j], a.sc[a.sn + j], a.sc[2 * a.sn + j], x0.x, x0.y, x0.z);
+		const int cj = a.cmap ? a.cmap[j] : j;
+		if (d2 < a.thr[THR_NEAR * a.sn + j] || (a.same_mesh && i == cj)) continue;
+		const int fn = a.fixed_n;
+		double2 s[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+		if (KM & SLP)
+			quad_eval64<FAM, SLP, CK>(a, j, fn > 0 ? fn : gauss_order(1 + table_degree<1>(d2, a.thr, a.sn, j)), x0, nx,
+				s);
+		if (KM & (DLP | DLPT))
+			quad_eval64<FAM, KM & (DLP | DLPT), CK>(a, j,
+				fn > 0 ? fn : gauss_order(1 + table_degree<2>(d2, a.thr, a.sn, j)), x0, nx, s);
+		if (KM & HSP)
+			quad_eval64<FAM, HSP, CK>(a, j, fn > 0 ? fn : gauss_order(1 + table_degree<3>(d2, a.thr, a.sn, j)), x0, nx,
+				s);
+		const long long idx = (long long)cj * a.ld + orow;
+		if (MODE == OUT_SINGLE) {
+			const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+			const double re = a.alpha.x * s[kk].x - a.alpha.y * s[kk].y;
+			const double im = a.alpha.x * s[kk].y + a.alpha.y * s[kk].x;
+			store_entry64(a.out0, idx, a.dtype, a.accumulate, re, im);
+		} else {
+			const double ar = a.alpha.x * s[3].x - a.alpha.y * s[3].y + s[1].x;
+			const double ai = a.alpha.x * s[3].y + a.alpha.y * s[3].x + s[1].y;
+			const double br = s[0].x + a.alpha.x * s[2].x - a.alpha.y * s[2].y;
+			const double bi = s[0].y + a.alpha.x * s[2].y + a.alpha.y * s[2].x;
+			store_entry64(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+			store_entry64(a.out1, idx, a.dtype, a.accumulate, br, bi);
+		}
+	}
+}
+
 // store one FP64 pair result (single operator or the Burton-Miller pair)
 template <int KM, int MODE, bool DIAG>
 __device__ __forceinline__ void quad_store64(const NearArgs &a, int i, int j, const double2 (&s)[4])

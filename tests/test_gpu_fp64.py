@@ -9,8 +9,11 @@ the FP64 oracle on the same inputs, through the C ABI:
   configuration policy (k = 10) and under NiHu's distance-driven degrees
   (k = 5), complex128 outputs: <= 1e-11;
 * a frequency sweep with the FP64 policy equals its single-wavenumber
-  assemblies bit for bit; quadrilateral meshes and the Galerkin formalism
-  refuse the policy.
+  assemblies bit for bit; the Galerkin formalism and the quadrilateral gauss
+  field refuse the policy;
+* quadrilateral meshes (Gauss square rules in FP64 from per-element tables)
+  and meshes with both element types: the four kernels, real and complex k,
+  NiHu's degrees and a fixed degree, and the Burton-Miller pair: <= 1e-11.
 
 The bound is set by the rounding of the quadrature points: the engine forms
 r = (c_j - c_i) + (y_q - c_j) from tables, the oracle y_q - x directly."""
@@ -116,6 +119,64 @@ def test_sweep_and_refusals_fp64(nb):
                     formalism="galerkin_linear")
     qn, qe = sphere_boundary(1.0, 3)
     q = nb.Mesh(qn, qe)
-    Aq, Bq = nb.colmajor(q.n, q.n), nb.colmajor(q.n, q.n)
+    Aq, Bq = nb.colmajor(4 * q.n, 4 * q.n), nb.colmajor(4 * q.n, 4 * q.n)
     with pytest.raises(NotImplementedError):
-        nb.assemble_bm(q, 2.0, Aq, Bq, pol=nb.policy(rule="fixed", degree=5, precision="fp64"))
+        nb.assemble_bm_gauss(q, 2.0, Aq, Bq, pol=nb.policy(rule="fixed", degree=5, precision="fp64"))
+
+
+QNIHU = (dict(rule="nihu", near="mean_edge", limit=1.5, precision="fp64"),
+         dict(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5))
+QFIXED = (dict(rule="fixed", degree=5, near="mean_edge", limit=1.5, precision="fp64"),
+          dict(rule=1, fixed_degree=5, near_mode=0, near_limit=1.5))
+
+
+@pytest.mark.parametrize("fam,k", [(0, 0.0), (1, 3.0), (1, complex(3.0, -0.5))])
+@pytest.mark.parametrize("nx,ny", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("which", ["nihu", "fixed"])
+def test_quadrilateral_kernels_fp64(nb, orc, fam, k, nx, ny, which):
+    import torch
+    from nihu_amd.mesh import jittered_cubed_sphere
+    gp, op = QNIHU if which == "nihu" else QFIXED
+    nodes, quads = jittered_cubed_sphere(5, seed=4)
+    n = len(quads)
+    m = nb.Mesh(nodes, quads)
+    out = nb.colmajor(n, n, "complex128")
+    out.fill_(complex("nan+nanj"))
+    nb.assemble(m, m, out, fam, nx, ny, k, alpha=0.7 + 0.2j, pol=nb.policy(**gp), accumulate=False)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.isfinite(got).all()
+    ref = orc.assemble(nodes, quads, fam, nx, ny, k=k, alpha=0.7 + 0.2j, pol=orc.policy(**op))
+    e = relfro(got, ref)
+    print("quadrilaterals FP64 fam %d (%d, %d) k = %s %s: rel fro %.2e" % (fam, nx, ny, k, which, e), flush=True)
+    assert e <= 1e-11
+
+
+def test_quadrilateral_and_mixed_burton_miller_fp64(nb, orc):
+    import torch
+    from nihu_amd.mesh import jittered_cubed_sphere, mixed_cubed_sphere
+    from tests.test_gpu_mixed import oracle_mixed
+    nodes, quads = jittered_cubed_sphere(5, seed=4)
+    n = len(quads)
+    m = nb.Mesh(nodes, quads)
+    A, B = nb.colmajor(n, n, "complex128"), nb.colmajor(n, n, "complex128")
+    nb.assemble_bm(m, 4.0, A, B, pol=nb.policy(**QNIHU[0]))
+    torch.cuda.synchronize()
+    Ar, Br = orc.assemble_bm(nodes, quads, 4.0, pol=orc.policy(**QNIHU[1]))
+    ea, eb = relfro(A.cpu().numpy(), Ar), relfro(B.cpu().numpy(), Br)
+    print("quadrilateral Burton-Miller FP64: A %.2e B %.2e" % (ea, eb), flush=True)
+    assert ea <= 1e-11 and eb <= 1e-11
+    # both element types in one mesh
+    mn, table = mixed_cubed_sphere(5, seed=3)
+    mm = nb.Mesh.from_elements(mn, table)
+    nm = len(table)
+    for nx, ny in ((0, 0), (1, 1)):
+        out = nb.colmajor(nm, nm, "complex128")
+        out.fill_(complex("nan+nanj"))
+        nb.assemble(mm, mm, out, nb.NB_HELMHOLTZ, nx, ny, 3.0, pol=nb.policy(**QNIHU[0]), accumulate=False)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        ref = oracle_mixed(orc, mn, table, 1, nx, ny, 3.0, QNIHU[1])
+        e = relfro(got, ref)
+        print("mixed mesh FP64 (%d, %d): rel fro %.2e" % (nx, ny, e), flush=True)
+        assert np.isfinite(got).all() and e <= 1e-11
