@@ -114,9 +114,9 @@ typedef struct {
 /* arithmetic of the regular pairs (near-singular and singular pairs are FP64 in both) */
 enum nb_precision {
 	NB_PREC_MIXED = 0,     /* regular pairs in FP32 with the phase k r reduced in FP64 (default) */
-	NB_PREC_FP64 = 1       /* every pair in FP64: NiHu's plain method (collocation with constant
-	                          fields on triangles, quadrilaterals and meshes of both; the
-	                          Galerkin formalism and the gauss field: NB_ERR_UNSUPPORTED) */
+	NB_PREC_FP64 = 1       /* every pair in FP64: NiHu's plain method (collocation on triangles,
+	                          quadrilaterals, meshes of both and with the quadrilateral gauss
+	                          field; the Galerkin formalism: NB_ERR_UNSUPPORTED) */
 };
 
 typedef struct {

@@ -460,7 +460,10 @@ int device_gauss_shapes(const nbk::GaussShapes **out)
 		for (int N = 1; N <= nbk::QN_MAX; ++N) {
 			if (nbg_gauss_shapes(N, v.data()) != N * N) return fail(NB_ERR_RUNTIME, "gauss field shape table");
 			for (int q = 0; q < N * N; ++q)
-				for (int b = 0; b < 4; ++b) g->n[N][q][b] = (float)v[4 * q + b];
+				for (int b = 0; b < 4; ++b) {
+					g->n[N][q][b] = (float)v[4 * q + b];
+					g->n64[N][q][b] = v[4 * q + b];
+				}
 		}
 		DevPtr p;
 		int s = dev_alloc(p, sizeof(nbk::GaussShapes));
@@ -1088,18 +1091,18 @@ int run_gauss(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 		return fail(NB_ERR_UNSUPPORTED, "field points of the gauss field: a test mesh of one element type");
 	if (!tg && jb.same)
 		return fail(NB_ERR_UNSUPPORTED, "a constant test field against the gauss field is taken on another mesh only");
-	if (jb.pol.precision != NB_PREC_MIXED)
-		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
+	// the FP64 precision policy: the regular blocks in FP64 too
+	const bool fp64 = jb.pol.precision == NB_PREC_FP64;
 	if ((s = check_quad_policy(jb.pol))) return s;
 	if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
 	int fixed_n = 0;
 	if (jb.pol.rule == NB_RULE_FIXED) {
 		fixed_n = jb.pol.fixed_degree / 2 + 1;
-		if ((s = ensure_qq(trial, fixed_n))) return s;
+		if ((s = fp64 ? ensure_qq64(trial, fixed_n) : ensure_qq(trial, fixed_n))) return s;
 	} else {
 		// degree 2 + {0, 2, 4, 6, 8, 9}: Gauss orders 2..6
 		for (int N = 2; N <= 6; ++N)
-			if ((s = ensure_qq(trial, N))) return s;
+			if ((s = fp64 ? ensure_qq64(trial, N) : ensure_qq(trial, N))) return s;
 	}
 	if (tg && (s = ensure_gauss_nodes(test))) return s;
 	const nbk::GaussShapes *gs = nullptr;
@@ -1129,7 +1132,49 @@ int run_gauss(const Job &jb, void *out0, void *out1, int64_t ld, int dtype, int 
 	a.out0 = out0; a.out1 = out1; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
 	const long long grid = (long long)a.row_tiles * ((trial->n + a.tile_cols - 1) / a.tile_cols);
 	const bool ck = jb.ki != 0.0;
-	if (grid > 0) {
+	if (grid > 0 && fp64) {
+		using namespace nbk;
+		const int pe = prof_begin(0, st);
+		Gauss64Args b;
+		std::memset(&b, 0, sizeof(b));
+		b.n_rows = a.n_rows; b.n_cols = a.n_cols; b.same_mesh = a.same_mesh;
+		b.row_tiles = a.row_tiles; b.tile_cols = a.tile_cols; b.nodes = nodes;
+		b.tn = a.tn; b.te = a.te;
+		b.tx = a.tx;
+		b.tnu = (const double *)(tg ? test->d_gnu64.get() : test->d_nu64.get());
+		b.tc = a.tc;
+		b.sc = (const double *)trial->d_c.get();
+		b.sn = trial->n;
+		b.thr = (const double *)trial->d_thr.get();
+		for (int N = 1; N <= QN_MAX; ++N) b.qt[N] = (const double *)trial->d_qq64[N].get();
+		b.gs = gs;
+		b.fixed_n = fixed_n;
+		b.k = jb.k; b.ki = jb.ki;
+		b.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+		b.out0 = out0; b.out1 = out1; b.ld = ld; b.dtype = dtype; b.accumulate = accumulate;
+#define NB_GR64(F, K, M) do { \
+		if (F == HELM && ck) hipLaunchKernelGGL((gauss_regular64_kernel<F, K, M, (F == HELM)>), dim3(grid), dim3(256), 0, st, b); \
+		else hipLaunchKernelGGL((gauss_regular64_kernel<F, K, M>), dim3(grid), dim3(256), 0, st, b); } while (0)
+		if (jb.mode == OUT_BM) NB_GR64(HELM, ALL4, OUT_BM);
+		else if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_GR64(HELM, SLP, OUT_SINGLE); break;
+			case DLP: NB_GR64(HELM, DLP, OUT_SINGLE); break;
+			case DLPT: NB_GR64(HELM, DLPT, OUT_SINGLE); break;
+			default: NB_GR64(HELM, HSP, OUT_SINGLE); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_GR64(LAP, SLP, OUT_SINGLE); break;
+			case DLP: NB_GR64(LAP, DLP, OUT_SINGLE); break;
+			case DLPT: NB_GR64(LAP, DLPT, OUT_SINGLE); break;
+			default: NB_GR64(LAP, HSP, OUT_SINGLE); break;
+			}
+		}
+#undef NB_GR64
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	} else if (grid > 0) {
 		using namespace nbk;
 		const int pe = prof_begin(0, st);
 #define NB_GR(F, K, M) do { \

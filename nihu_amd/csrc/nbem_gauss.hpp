@@ -36,7 +36,10 @@ constexpr int GDOF = 4;
 
 // N_b at the points of the Gauss square rule of each order (rows as in the
 // QuadPt tables: xi outer, eta inner)
-struct GaussShapes { float n[QN_MAX + 1][QN_MAX * QN_MAX][GDOF]; };
+struct GaussShapes {
+	float n[QN_MAX + 1][QN_MAX * QN_MAX][GDOF];
+	double n64[QN_MAX + 1][QN_MAX * QN_MAX][GDOF];   // the same in FP64 (FP64 precision policy)
+};
 
 struct GaussArgs {
 	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
@@ -551,6 +554,96 @@ __device__ void gauss_guiggiani(const Quad &Q, double xi0x, double xi0y, double 
 					s[d][3].x += (F[3].x * N[d] - sing) * (wt * wr);
 					s[d][3].y += F[3].y * N[d] * (wt * wr);
 				}
+			}
+		}
+	}
+}
+
+// The FP64 precision policy for the gauss field: every regular block in
+// FP64, each kernel group at its Gauss order from the trial element's FP64
+// table (y, w J, unit normal), each point's values spread over the four dofs
+// with the FP64 shape-function table.
+struct Gauss64Args {
+	int n_rows, n_cols, same_mesh, row_tiles, tile_cols;
+	int nodes;                   // test nodes per test element (4 or 1)
+	int tn, te;
+	const double *tx, *tnu;      // collocation points, unit normals [3][tn]
+	const double *tc;            // test element centres [3][te]
+	const double *sc;            // trial centres [3][sn]
+	int sn;
+	const double *thr;           // [NTHR][sn]
+	const double *qt[QN_MAX + 1];   // per Gauss order N: [sn][N * N][8]
+	const GaussShapes *gs;
+	int fixed_n;
+	double k, ki;
+	double2 alpha;
+	void *out0, *out1;
+	long long ld;
+	int dtype, accumulate;
+};
+
+template <int FAM, int MASK, bool CK>
+__device__ __forceinline__ void gauss_eval64(const Gauss64Args &a, int j, int n, D3 x0, D3 nx,
+	double2 (&s)[GDOF][4])
+{
+	if (n <= 0) return;
+	const int nq = n * n;
+	const double *t = a.qt[n] + (size_t)j * nq * 8;
+	for (int q = 0; q < nq; ++q) {
+		const D3 y = {t[8 * q], t[8 * q + 1], t[8 * q + 2]};
+		const D3 ny = {t[8 * q + 4], t[8 * q + 5], t[8 * q + 6]};
+		double2 f[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+		kernel_point<FAM, MASK, CK>(x0, y, nx, ny, a.k, t[8 * q + 3], f, a.ki);
+		double N[GDOF];
+		for (int d = 0; d < GDOF; ++d) N[d] = a.gs->n64[n][q][d];
+		gspread<MASK>(N, f, s);
+	}
+}
+
+template <int FAM, int KM, int MODE, bool CK = false>
+__global__ void __launch_bounds__(256) gauss_regular64_kernel(Gauss64Args a)
+{
+	const int tile_r = blockIdx.x % a.row_tiles;
+	const int tile_c = blockIdx.x / a.row_tiles;
+	const int r = tile_r * 256 + threadIdx.x;
+	if (r >= a.n_rows) return;
+	const int ie = a.nodes == GDOF ? r >> 2 : r;
+	const D3 x0 = {a.tx[r], a.tx[a.tn + r], a.tx[2 * a.tn + r]};
+	const D3 nx = {a.tnu[r], a.tnu[a.tn + r], a.tnu[2 * a.tn + r]};
+	const double cix = a.tc[ie], ciy = a.tc[a.te + ie], ciz = a.tc[2 * a.te + ie];
+	const int j0 = tile_c * a.tile_cols;
+	const int j1 = min(j0 + a.tile_cols, a.n_cols);
+	for (int j = j0; j < j1; ++j) {
+		// classification by element centres, as gauss_regular_kernel
+		const double d2 = d2_pair(a.sc[j], a.sc[a.sn + j], a.sc[2 * a.sn + j], cix, ciy, ciz);
+		if (d2 < a.thr[THR_NEAR * a.sn + j] || (a.same_mesh && ie == j)) continue;
+		const int fn = a.fixed_n;
+		double2 s[GDOF][4];
+		for (int d = 0; d < GDOF; ++d)
+			for (int t = 0; t < 4; ++t) s[d][t] = make_double2(0, 0);
+		if (KM & SLP)
+			gauss_eval64<FAM, SLP, CK>(a, j, fn > 0 ? fn : gauss_order(2 + table_degree<1>(d2, a.thr, a.sn, j)), x0, nx,
+				s);
+		if (KM & (DLP | DLPT))
+			gauss_eval64<FAM, KM & (DLP | DLPT), CK>(a, j,
+				fn > 0 ? fn : gauss_order(2 + table_degree<2>(d2, a.thr, a.sn, j)), x0, nx, s);
+		if (KM & HSP)
+			gauss_eval64<FAM, HSP, CK>(a, j, fn > 0 ? fn : gauss_order(2 + table_degree<3>(d2, a.thr, a.sn, j)), x0, nx,
+				s);
+		for (int d = 0; d < GDOF; ++d) {
+			const long long idx = (long long)(GDOF * j + d) * a.ld + r;
+			if (MODE == OUT_SINGLE) {
+				const int kk = (KM & SLP) ? 0 : (KM & DLP) ? 1 : (KM & DLPT) ? 2 : 3;
+				const double re = a.alpha.x * s[d][kk].x - a.alpha.y * s[d][kk].y;
+				const double im = a.alpha.x * s[d][kk].y + a.alpha.y * s[d][kk].x;
+				store_entry64(a.out0, idx, a.dtype, a.accumulate, re, im);
+			} else {
+				const double ar = a.alpha.x * s[d][3].x - a.alpha.y * s[d][3].y + s[d][1].x;
+				const double ai = a.alpha.x * s[d][3].y + a.alpha.y * s[d][3].x + s[d][1].y;
+				const double br = s[d][0].x + a.alpha.x * s[d][2].x - a.alpha.y * s[d][2].y;
+				const double bi = s[d][0].y + a.alpha.x * s[d][2].y + a.alpha.y * s[d][2].x;
+				store_entry64(a.out0, idx, a.dtype, a.accumulate, ar, ai);
+				store_entry64(a.out1, idx, a.dtype, a.accumulate, br, bi);
 			}
 		}
 	}

@@ -131,8 +131,8 @@ def policy(rule: str | int = "nihu", degree: int = 5, near: str | int = "mean_ed
     """'nihu' = NiHu's per-pair degree tables; 'fixed' = one Dunavant degree.
     near: 'mean_edge' (NiHu, limit 1.5), 'diameter' (limit in element diameters) or 'none'.
     precision: 'mixed' (regular pairs in FP32 with the phase reduced in FP64)
-    or 'fp64' (every pair in FP64; collocation with constant fields on
-    triangles, quadrilaterals and meshes of both)."""
+    or 'fp64' (every pair in FP64; collocation on every mesh type and
+    field; not the Galerkin formalism)."""
     r = {"nihu": NB_RULE_NIHU, "fixed": NB_RULE_FIXED}.get(rule, rule)
     n = {"mean_edge": NB_NEAR_MEAN_EDGE, "diameter": NB_NEAR_DIAMETER, "none": NB_NEAR_NONE}.get(near, near)
     p = {"mixed": NB_PREC_MIXED, "fp64": NB_PREC_FP64}.get(precision, precision)

@@ -10,7 +10,7 @@ Galerkin matrices (<= 1e-5); on quadrilateral meshes and on meshes with
 both element types (collocation, NiHu's degrees and a fixed Gauss degree 5):
 rows to 2e-6 of their largest entry, 1e-5 relative Frobenius and the FP64-pass
 entries (near-singular by polar / Telles, diagonal by Duffy / Guiggiani) to
-1e-9; the quadrilateral gauss field refuses the FP64 precision policy."""
+1e-9; the Galerkin formalism on quadrilaterals is refused."""
 
 import numpy as np
 import pytest
@@ -117,9 +117,7 @@ def test_refusals(nb):
     qn, qe = sphere_boundary(1.0, 2)
     q = nb.Mesh(qn, qe)
     with pytest.raises(NotImplementedError):
-        nb.assemble_bm_gauss(q, k, nb.colmajor(4 * q.n, 4 * q.n, "complex128"),
-                             nb.colmajor(4 * q.n, 4 * q.n, "complex128"),
-                             pol=nb.policy(rule="fixed", degree=5, precision="fp64"))
+        nb.assemble(q, q, nb.colmajor(q.n, q.n), nb.NB_HELMHOLTZ, 0, 0, k, formalism="galerkin")
     with pytest.raises(ValueError):
         nb.assemble_bm(m, complex(-1.0, 0.5), nb.colmajor(m.n, m.n), nb.colmajor(m.n, m.n))
 

@@ -9,8 +9,7 @@ the FP64 oracle on the same inputs, through the C ABI:
   configuration policy (k = 10) and under NiHu's distance-driven degrees
   (k = 5), complex128 outputs: <= 1e-11;
 * a frequency sweep with the FP64 policy equals its single-wavenumber
-  assemblies bit for bit; the Galerkin formalism and the quadrilateral gauss
-  field refuse the policy;
+  assemblies bit for bit; the Galerkin formalism refuses the policy;
 * quadrilateral meshes (Gauss square rules in FP64 from per-element tables)
   and meshes with both element types: the four kernels, real and complex k,
   NiHu's degrees and a fixed degree, and the Burton-Miller pair: <= 1e-11.
@@ -99,7 +98,7 @@ def test_burton_miller_fp64(nb, orc, which):
 
 def test_sweep_and_refusals_fp64(nb):
     import torch
-    from nihu_amd.mesh import jittered_icosphere, sphere_boundary
+    from nihu_amd.mesh import jittered_icosphere
     nodes, tris = jittered_icosphere(2, seed=1)
     n = len(tris)
     m = nb.Mesh(nodes, tris)
@@ -117,11 +116,6 @@ def test_sweep_and_refusals_fp64(nb):
     with pytest.raises(NotImplementedError):
         nb.assemble(m, m, nb.colmajor(m.n_nodes, m.n_nodes, "complex128"), nb.NB_HELMHOLTZ, 0, 0, 2.0, pol=pol,
                     formalism="galerkin_linear")
-    qn, qe = sphere_boundary(1.0, 3)
-    q = nb.Mesh(qn, qe)
-    Aq, Bq = nb.colmajor(4 * q.n, 4 * q.n), nb.colmajor(4 * q.n, 4 * q.n)
-    with pytest.raises(NotImplementedError):
-        nb.assemble_bm_gauss(q, 2.0, Aq, Bq, pol=nb.policy(rule="fixed", degree=5, precision="fp64"))
 
 
 QNIHU = (dict(rule="nihu", near="mean_edge", limit=1.5, precision="fp64"),

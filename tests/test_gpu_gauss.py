@@ -11,7 +11,8 @@ the FP64 oracle on the same inputs:
 * the Burton-Miller pair and the identity over the dofs; field points (a
   constant test field on another mesh) against the gauss trial field;
 * sampled rows of a mesh with many row tiles; the pulsating-sphere answer
-  from the device matrices; host outputs with accumulation; a failed inverse
+  from the device matrices; the FP64 precision policy (<= 1e-11); host outputs
+  with accumulation; a failed inverse
   mapping (forced from the host) as an error; the refusals.
 """
 
@@ -172,6 +173,27 @@ def test_gauss_pulsating_sphere(nb):
     assert err < 0.015
 
 
+@pytest.mark.parametrize("fam,k", [(0, 0.0), (1, 3.0), (1, complex(2.5, -0.5))])
+@pytest.mark.parametrize("nxy", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gauss_operators_fp64(nb, orc, qmesh, fam, k, nxy):
+    """The FP64 precision policy: every block in FP64, against the oracle to 1e-11."""
+    import torch
+    nodes, quads = qmesh
+    nx, ny = nxy
+    m = nb.Mesh(nodes, quads)
+    nd = 4 * m.n
+    out = nb.colmajor(nd, nd, "complex128")
+    out.fill_(complex("nan+nanj"))
+    nb.assemble(m, m, out, fam, nx, ny, k, alpha=0.6 - 0.3j, pol=nb.policy(**NIHU[0], precision="fp64"),
+                accumulate=False, formalism="collocation_gauss")
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ref = orc.assemble_gauss(nodes, quads, fam, nx, ny, k=k, alpha=0.6 - 0.3j, pol=orc.policy(**NIHU[1]))
+    e = relfro(got, ref)
+    print("gauss field FP64 fam %d (%d, %d) k = %s: rel fro %.2e" % (fam, nx, ny, k, e), flush=True)
+    assert np.isfinite(got).all() and e <= 1e-11
+
+
 def test_gauss_host_output_accumulates(nb, orc, qmesh):
     """Host outputs (numpy, Fortran order) with accumulation: two calls give
     twice the oracle's matrix."""
@@ -220,9 +242,6 @@ def test_gauss_refusals(nb, qmesh):
         nb.assemble(t, t, nb.colmajor(4 * t.n, 4 * t.n), 1, 0, 0, 1.0, formalism="collocation_gauss")
     with pytest.raises(NotImplementedError):
         nb.assemble(m, m, nb.colmajor(m.n, nd), 1, 0, 0, 1.0, formalism="collocation_gauss_trial")
-    with pytest.raises(NotImplementedError):
-        nb.assemble(m, m, nb.colmajor(nd, nd, "complex128"), 1, 0, 0, 1.0, formalism="collocation_gauss",
-                    pol=nb.policy(precision="fp64"))
     with pytest.raises(ValueError):
         nb.assemble(m, m, nb.colmajor(m.n, m.n), 1, 0, 0, 1.0, formalism="collocation_gauss")
     with pytest.raises(NotImplementedError):
