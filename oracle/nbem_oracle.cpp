@@ -1325,6 +1325,217 @@ static cplx quad_pair_integral(int family, int nx, int ny, K k, Policy const &p,
 	return quad_regular(family, nx, ny, k, t.x, t.nu, trial, degree);
 }
 
+// ---------------------------------------------------------------------------
+// Galerkin formalism on quadrilaterals (quad_1_elem): constant_view (one dof
+// per element; field complexity = polynomial order 0 + jacobian order 1) or
+// isoparametric_view (the four corner nodes with the bilinear shape
+// functions; complexity 1 + 1), nihu/core/complexity_estimator.hpp:39-75,
+// nihu/library/lib_shape.hpp:215-230.
+//   regular pairs: the Gauss rule of one degree on both elements (diadic),
+//     degree = complexity + the kernel's distance interval of d / R_trial, or
+//     fixed (nihu/core/double_integral.hpp:283-320, :181-206);
+//   node-sharing pairs on one mesh (nihu/core/element_match.hpp:84-112,
+//     element.hpp:375-413): the quad x quad rules of
+//     nihu/core/singular_galerkin_quadrature.hpp:457-625 with the subdomain
+//     corners of nihu/library/lib_singular_galerkin_quadrature.cpp:6-31, of
+//     singular_quadrature_order 7 (helmholtz_kernel.hpp:69-71), rotated to
+//     the shared corners as nihu/core/singular_accelerator.hpp:283-325 does
+//     (the rotations and reflections of the square, |J| = 1);
+//   the hypersingular kernel has no such rule on face and edge matches
+//     (singular_accelerator.hpp:483-503), as for triangles; the double layer
+//     of a coincident quadrilateral is integrated (NiHu's zero shortcut,
+//     normal_derivative_singular_integrals.hpp:13-58, is for tria_1 only).
+// ---------------------------------------------------------------------------
+
+// element::get_overlapping for two quadrilaterals
+static Overlap get_overlapping_quads(const int a[4], const int b[4])
+{
+	int num = 0, s1 = 0, s2 = 0;
+	for (int i = 0; i < 4; ++i)
+		for (int j = 0; j < 4; ++j)
+			if (a[i] == b[j]) {
+				if (num == 0) { s1 = i; s2 = j; }
+				else {
+					if (i < s1 || (i == 3 && s1 == 0 && num == 1)) s1 = i;
+					if (j < s2 || (j == 3 && s2 == 0 && num == 1)) s2 = j;
+				}
+				++num;
+			}
+	return {num, s1, s2};
+}
+
+// bilinear shape functions of the reference square, corner order
+// (-1,-1), (1,-1), (1,1), (-1,1)
+static inline void quad_shapes(double xi, double eta, double N[4])
+{
+	N[0] = (1.0 - xi) * (1.0 - eta) / 4.0;
+	N[1] = (1.0 + xi) * (1.0 - eta) / 4.0;
+	N[2] = (1.0 + xi) * (1.0 + eta) / 4.0;
+	N[3] = (1.0 - xi) * (1.0 + eta) / 4.0;
+}
+
+// Subdomains of the plane of the difference variable mu = eta - xi, as four
+// corners each (a repeated first corner makes a triangle fanned from the
+// origin), for the face, edge and corner matches; the table of
+// lib_singular_galerkin_quadrature.cpp:6-31.
+static const double QG_FACE[4][4][2] = {
+	{{0, 0}, {0, 0}, {2, -2}, {2, 0}}, {{0, 0}, {0, 0}, {2, 0}, {2, 2}},
+	{{0, 0}, {0, 0}, {2, 2}, {0, 2}}, {{0, 0}, {0, 0}, {0, 2}, {-2, 2}}};
+static const double QG_EDGE[6][4][2] = {
+	{{0, 0}, {0, 0}, {-2, 0}, {-2, -2}}, {{0, 0}, {0, 0}, {-2, -2}, {0, -2}},
+	{{0, 0}, {0, 0}, {0, -2}, {2, -2}}, {{0, 0}, {0, 0}, {2, -2}, {2, 0}},
+	{{-2, -2}, {-2, -4}, {0, -4}, {0, -2}}, {{0, -2}, {0, -4}, {2, -4}, {2, -2}}};
+static const double QG_CORNER[5][4][2] = {
+	{{0, 0}, {0, 0}, {-2, 0}, {-2, -2}}, {{0, 0}, {0, 0}, {-2, -2}, {0, -2}},
+	{{-4, 0}, {-4, -2}, {-2, -2}, {-2, 0}}, {{-4, -2}, {-4, -4}, {-2, -4}, {-2, -2}},
+	{{-2, -2}, {-2, -4}, {0, -4}, {0, -2}}};
+
+// the bilinear map of the reference square onto the plane quadrilateral of
+// corners C: image point and signed Jacobian (quadrature::transform)
+static inline void plane_bilinear(const double C[4][2], double xi, double eta, double &u, double &v, double &J)
+{
+	double N[4];
+	quad_shapes(xi, eta, N);
+	const double dxi[4] = {-(1.0 - eta) / 4.0, (1.0 - eta) / 4.0, (1.0 + eta) / 4.0, -(1.0 + eta) / 4.0};
+	const double deta[4] = {-(1.0 - xi) / 4.0, -(1.0 + xi) / 4.0, (1.0 + xi) / 4.0, (1.0 - xi) / 4.0};
+	double a = 0, b = 0, c = 0, d = 0;
+	u = v = 0.0;
+	for (int k = 0; k < 4; ++k) {
+		u += N[k] * C[k][0]; v += N[k] * C[k][1];
+		a += dxi[k] * C[k][0]; b += dxi[k] * C[k][1];
+		c += deta[k] * C[k][0]; d += deta[k] * C[k][1];
+	}
+	J = a * d - b * c;
+}
+
+// the trial point of the canonical configuration from mu + xi: the identity
+// (face), a reflection across the edge eta_2 = -1 (edge) or through the
+// corner (-1, -1) (corner)
+static inline void qg_trial(int match, double m0, double m1, double &e0, double &e1)
+{
+	if (match == 2) { e0 = m0; e1 = m1; }
+	else if (match == 1) { e0 = m0; e1 = -2.0 - m1; }
+	else { e0 = -2.0 - m0; e1 = -2.0 - m1; }
+}
+
+// canonical quad x quad singular rule of a match dimension and Gauss degree:
+// for each subdomain, an outer rule over mu, and for each outer point an inner
+// rule over the rectangle of test points xi with mu + xi inside the trial
+// square (clamped to the test square); the face rule is symmetrised by
+// swapping test and trial.  Points (txi, teta, sxi, seta, w).
+static std::vector<SPt> make_quad_singular_rule(int match, int degree)
+{
+	const std::vector<QP> base = gauss_quad(degree);
+	const double (*dom)[4][2] = match == 2 ? QG_FACE : match == 1 ? QG_EDGE : QG_CORNER;
+	const int ndom = match == 2 ? 4 : match == 1 ? 6 : 5;
+	double lo[2], hi[2];
+	qg_trial(match, -1.0, -1.0, lo[0], lo[1]);
+	qg_trial(match, 1.0, 1.0, hi[0], hi[1]);
+	for (int j = 0; j < 2; ++j)
+		if (lo[j] > hi[j]) std::swap(lo[j], hi[j]);
+	std::vector<SPt> out;
+	for (int d = 0; d < ndom; ++d)
+		for (auto const &po : base) {
+			double m0, m1, Jo;
+			plane_bilinear(dom[d], po.xi, po.eta, m0, m1, Jo);
+			const double wo = po.w * Jo;
+			double C[4][2] = {{lo[0] - m0, lo[1] - m1}, {hi[0] - m0, lo[1] - m1}, {hi[0] - m0, hi[1] - m1},
+				{lo[0] - m0, hi[1] - m1}};
+			for (auto &c : C)
+				for (double &v : c) v = std::max(std::min(v, 1.0), -1.0);
+			for (auto const &pi : base) {
+				double x0, x1, Ji, e0, e1;
+				plane_bilinear(C, pi.xi, pi.eta, x0, x1, Ji);
+				qg_trial(match, m0 + x0, m1 + x1, e0, e1);
+				const double w = wo * (pi.w * Ji);
+				out.push_back({x0, x1, e0, e1, w});
+				if (match == 2) out.push_back({e0, e1, x0, x1, w});
+			}
+		}
+	return out;
+}
+
+static const std::vector<SPt> &quad_singular_rule(int match)
+{
+	static const std::vector<SPt> r0 = make_quad_singular_rule(0, 7), r1 = make_quad_singular_rule(1, 7),
+		r2 = make_quad_singular_rule(2, 7);
+	return match == 0 ? r0 : match == 1 ? r1 : r2;
+}
+
+// point of the reference square mapped by the corner permutation c[]
+// (quadrature::transform with isoparam_shape_set and these corners)
+static inline void map_quad_corners(double xi, double eta, const int c[4], double &oxi, double &oeta)
+{
+	double N[4];
+	quad_shapes(xi, eta, N);
+	oxi = oeta = 0.0;
+	for (int k = 0; k < 4; ++k) { oxi += N[k] * QCX[c[k]]; oeta += N[k] * QCY[c[k]]; }
+}
+
+// 4x4 Galerkin block of a quadrilateral pair (test shape a, trial shape b;
+// [0][0] for constant fields); throws like NiHu
+template <class WN>
+static void galerkin_quad_block(int family, int nx, int ny, WN k, Policy const &p, QElem const &te,
+	QElem const &se, const int *tn, const int *sn, bool same_mesh, int linear, cplx blk[4][4])
+{
+	for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) blk[a][b] = 0.0;
+	const int nd = linear ? 4 : 1;
+	// test point (N_a J w) and trial point (N_b J) of one point pair
+	auto add = [&](double txi, double teta, double sxi, double seta, double wt, double ws) {
+		const V3 x = q_x(te, txi, teta), y = q_x(se, sxi, seta);
+		const V3 n1 = q_normal(te, txi, teta), n2 = q_normal(se, sxi, seta);
+		const double J1 = norm(n1), J2 = norm(n2);
+		const cplx K = kernel_eval(family, nx, ny, k, x, y, n1 / J1, n2 / J2);
+		double Nt[4], Ns[4];
+		quad_shapes(txi, teta, Nt);
+		quad_shapes(sxi, seta, Ns);
+		if (!linear) { Nt[0] = 1.0; Ns[0] = 1.0; }
+		for (int a = 0; a < nd; ++a)
+			for (int b = 0; b < nd; ++b) blk[a][b] += (Nt[a] * (J1 * wt)) * K * (Ns[b] * (J2 * ws));
+	};
+	int match = -1;
+	Overlap ov = {0, 0, 0};
+	if (same_mesh) {
+		if (te.id == se.id) match = 2;
+		else {
+			ov = get_overlapping_quads(tn, sn);
+			if (ov.num > 1) match = 1;
+			else if (ov.num == 1) match = 0;
+		}
+	}
+	if (match >= 0) {
+		if (nx == 1 && ny == 1 && match >= 1) throw std::runtime_error("Invalid general singular quadrature");
+		int tc[4] = {0, 1, 2, 3}, scn[4] = {0, 1, 2, 3};
+		if (match < 2)
+			for (int q = 0; q < 4; ++q) {
+				tc[q] = (ov.ind1 + q) % 4;
+				scn[q] = match == 1 ? (ov.ind2 + 1 + 4 - q) % 4 : (ov.ind2 + q) % 4;
+			}
+		for (auto const &P : quad_singular_rule(match)) {
+			double txi, teta, sxi, seta;
+			map_quad_corners(P.txi, P.teta, tc, txi, teta);
+			map_quad_corners(P.sxi, P.seta, scn, sxi, seta);
+			add(txi, teta, sxi, seta, P.w, 1.0);
+		}
+		return;
+	}
+	const double d2 = dist2(te.c, se.c);
+	const int degree = p.rule == 1 ? p.fixed_degree
+		: (linear ? 2 : 1) + eval_interval(interval_table(1 + nx + ny), std::sqrt(d2) / se.R);
+	if (degree > 14) throw std::out_of_range("Too high quadrature degree selected for collocational integration");
+	auto const &Q = gauss_quad_store(degree);
+	for (auto const &qp : Q)
+		for (auto const &qs : Q) add(qp.xi, qp.eta, qs.xi, qs.eta, qp.w, qs.w);
+}
+
+// the quadrilaterals of a mesh
+static std::vector<QElem> make_qmesh(const double *xyz, int n_nodes, const int *quad, int n)
+{
+	std::vector<QElem> out;
+	read_elems(xyz, n_nodes, quad, n, 4, nullptr, &out, nullptr);
+	return out;
+}
+
 
 // ---------------------------------------------------------------------------
 // quad_1_gauss_field (nihu/library/quad_1_gauss_field.hpp,
@@ -2430,6 +2641,153 @@ int orc_gauss_shape(double xi, double eta, double *N12, double *nodes8)
 	gauss_dshape(xi, eta, N12 + 4, N12 + 8);
 	for (int a = 0; a < 4; ++a) gauss_node(a, nodes8[2 * a], nodes8[2 * a + 1]);
 	return 0;
+}
+
+} // extern "C"
+
+
+// Galerkin assembly on quadrilateral meshes, result(dof_a, dof_b) (+)= alpha *
+// block (complex interleaved, column-major, ld >= test dofs).  linear = 1:
+// dofs = mesh nodes (isoparametric_view), 0: elements (constant_view).
+// family 2 = identity: the single integral of N_a N_b J with the Gauss degree
+// of the shape orders plus the jacobian order 1 (nihu/core/single_integral.hpp:
+// 65-140).
+template <class K>
+static int assemble_galerkin_quad_impl(const double *txyz, int tn, const int *tq, int tnt,
+	const double *sxyz, int sn, const int *sq, int snt, int same_mesh, int linear,
+	int family, int nx, int ny, K k, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *test_elems, int n_test_elems, double *out, int64_t ld, int accumulate,
+	int n_threads)
+{
+	ORC_TRY
+	auto test = make_qmesh(txyz, tn, tq, tnt);
+	auto trial = same_mesh ? test : make_qmesh(sxyz, sn, sq, snt);
+	if (same_mesh) { sn = tn; sq = tq; snt = tnt; }
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	const cplx alpha(alpha_re, alpha_im);
+	const int nd = linear ? 4 : 1;
+	auto tdof = [&](int e, int a) { return linear ? tq[4 * e + a] : e; };
+	auto sdof = [&](int e, int b) { return linear ? sq[4 * e + b] : e; };
+	const int nrows = linear ? tn : tnt, ncols = linear ? sn : snt;
+	if (!accumulate)
+		for (int j = 0; j < ncols; ++j)
+			for (int i = 0; i < nrows; ++i) { out[2 * ((int64_t)j * ld + i)] = 0; out[2 * ((int64_t)j * ld + i) + 1] = 0; }
+	if (family == 2) {
+		if (!same_mesh) return 0;
+		auto const &qm = gauss_quad_store(linear ? 3 : 1);
+		for (int e = 0; e < tnt; ++e)
+			for (int a = 0; a < nd; ++a)
+				for (int b = 0; b < nd; ++b) {
+					double m = 0.0;
+					for (auto const &q : qm) {
+						double N[4];
+						quad_shapes(q.xi, q.eta, N);
+						if (!linear) N[0] = 1.0;
+						m += N[a] * N[b] * norm(q_normal(test[e], q.xi, q.eta)) * q.w;
+					}
+					const cplx v = alpha * m;
+					double *o = out + 2 * ((int64_t)tdof(e, b) * ld + tdof(e, a));
+					o[0] += v.real(); o[1] += v.imag();
+				}
+		return 0;
+	}
+	if (n_threads <= 0) n_threads = orc_num_threads();
+	std::string err;
+	int errcode = 0;
+	const int n_loop = test_elems ? n_test_elems : tnt;
+	#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
+	for (int ii = 0; ii < n_loop; ++ii) {
+		const int i = test_elems ? test_elems[ii] : ii;
+		try {
+			for (int j = 0; j < snt; ++j) {
+				cplx blk[4][4];
+				galerkin_quad_block(family, nx, ny, k, p, test[i], trial[j], tq + 4 * i, sq + 4 * j, same_mesh != 0,
+					linear, blk);
+				for (int a = 0; a < nd; ++a)
+					for (int b = 0; b < nd; ++b) {
+						const cplx v = alpha * blk[a][b];
+						double *o = out + 2 * ((int64_t)sdof(j, b) * ld + tdof(i, a));
+						#pragma omp atomic
+						o[0] += v.real();
+						#pragma omp atomic
+						o[1] += v.imag();
+					}
+			}
+		} catch (std::out_of_range const &ex) {
+			#pragma omp critical
+			{ err = ex.what(); errcode = 2; }
+		} catch (std::exception const &ex) {
+			#pragma omp critical
+			{ err = ex.what(); errcode = 1; }
+		}
+	}
+	if (errcode == 2) throw std::out_of_range(err);
+	if (errcode == 1) throw std::runtime_error(err);
+	ORC_CATCH
+}
+
+extern "C" {
+
+int orc_assemble_galerkin_quad(const double *txyz, int tn, const int *tq, int tnt,
+	const double *sxyz, int sn, const int *sq, int snt, int same_mesh, int linear,
+	int family, int nx, int ny, double k_re, double k_im, double alpha_re, double alpha_im,
+	const orc_policy *pol, const int *test_elems, int n_test_elems, double *out, int64_t ld, int accumulate,
+	int n_threads)
+{
+	if (k_im == 0.0)
+		return assemble_galerkin_quad_impl(txyz, tn, tq, tnt, sxyz, sn, sq, snt, same_mesh, linear, family, nx, ny,
+			k_re, alpha_re, alpha_im, pol, test_elems, n_test_elems, out, ld, accumulate, n_threads);
+	return assemble_galerkin_quad_impl(txyz, tn, tq, tnt, sxyz, sn, sq, snt, same_mesh, linear, family, nx, ny,
+		cplx(k_re, k_im), alpha_re, alpha_im, pol, test_elems, n_test_elems, out, ld, accumulate, n_threads);
+}
+
+// node overlap of two quadrilaterals as NiHu's scan finds it
+int orc_overlap_quads(const int *a, const int *b, int *ind1, int *ind2)
+{
+	Overlap o = get_overlapping_quads(a, b);
+	*ind1 = o.ind1;
+	*ind2 = o.ind2;
+	return o.num;
+}
+
+// canonical quad x quad singular rule of a match dimension and Gauss degree
+int orc_quad_singular_rule(int match, int degree, double *pts, int cap, int *n_out)
+{
+	ORC_TRY
+	if (match < 0 || match > 2) throw std::invalid_argument("match must be 0, 1 or 2");
+	const std::vector<SPt> r = make_quad_singular_rule(match, degree);
+	if ((int)r.size() > cap) throw std::invalid_argument("capacity");
+	for (size_t q = 0; q < r.size(); ++q) {
+		pts[5 * q] = r[q].txi; pts[5 * q + 1] = r[q].teta; pts[5 * q + 2] = r[q].sxi;
+		pts[5 * q + 3] = r[q].seta; pts[5 * q + 4] = r[q].w;
+	}
+	*n_out = (int)r.size();
+	ORC_CATCH
+}
+
+// one Galerkin block of a quadrilateral pair given by corner coordinates
+// (12 each) and node ids; same_elem: the pair is one element
+int orc_galerkin_quad_block(int family, int nx, int ny, double k_re, double k_im, const orc_policy *pol,
+	const double *tc12, const int *tn4, const double *sc12, const int *sn4, int same_mesh, int same_elem,
+	int linear, double *blk32)
+{
+	ORC_TRY
+	V3 X[4], Y[4];
+	for (int a = 0; a < 4; ++a) {
+		X[a] = {tc12[3 * a], tc12[3 * a + 1], tc12[3 * a + 2]};
+		Y[a] = {sc12[3 * a], sc12[3 * a + 1], sc12[3 * a + 2]};
+	}
+	QElem te = make_qelem(X, 0), se = make_qelem(Y, same_elem ? 0 : 1);
+	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
+	cplx blk[4][4];
+	if (k_im == 0.0) galerkin_quad_block(family, nx, ny, k_re, p, te, se, tn4, sn4, same_mesh != 0, linear, blk);
+	else galerkin_quad_block(family, nx, ny, cplx(k_re, k_im), p, te, se, tn4, sn4, same_mesh != 0, linear, blk);
+	for (int a = 0; a < 4; ++a)
+		for (int b = 0; b < 4; ++b) {
+			blk32[2 * (4 * a + b)] = blk[a][b].real();
+			blk32[2 * (4 * a + b) + 1] = blk[a][b].imag();
+		}
+	ORC_CATCH
 }
 
 } // extern "C"

@@ -111,6 +111,13 @@ def _declare(l):
                                                C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                                C.c_double, P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
         l.orc_overlap.argtypes = [I, I, I, I]
+        l.orc_assemble_galerkin_quad.argtypes = [D, C.c_int, I, C.c_int, D, C.c_int, I, C.c_int, C.c_int, C.c_int,
+                                                 C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                                 C.c_double, P, I, C.c_int, D, C.c_int64, C.c_int, C.c_int]
+        l.orc_overlap_quads.argtypes = [I, I, I, I]
+        l.orc_quad_singular_rule.argtypes = [C.c_int, C.c_int, D, C.c_int, I]
+        l.orc_galerkin_quad_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, D, I, D, I,
+                                              C.c_int, C.c_int, C.c_int, D]
         l.orc_singular_rule.argtypes = [C.c_int, D, C.c_int, I]
         l.orc_galerkin_block.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, P, D, I, D, I, C.c_int, C.c_int,
                                          C.c_int, D]
@@ -333,6 +340,17 @@ def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, t
         accumulate = False
     alpha = complex(alpha)
     te = None if test_elems is None else np.ascontiguousarray(test_elems, dtype=np.int32)
+    if t_tris.shape[1] == 4 or s_tris.shape[1] == 4:
+        if t_tris.shape[1] != 4 or s_tris.shape[1] != 4:
+            raise ValueError("Galerkin pairs of a triangle and a quadrilateral are not built")
+        kc = complex(k)
+        _check(lib().orc_assemble_galerkin_quad(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
+                                                _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
+                                                int(linear), family, nx, ny, kc.real, kc.imag, alpha.real,
+                                                alpha.imag, C.byref(pol), _i(te), 0 if te is None else len(te),
+                                                out.ctypes.data_as(C.POINTER(C.c_double)), out.shape[0],
+                                                int(accumulate), threads))
+        return out
     if _is_complex(k):
         _check(lib().orc_assemble_galerkin_ck(_d(t_nodes), len(t_nodes), _i(t_tris), len(t_tris),
                                               _d(s_nodes), len(s_nodes), _i(s_tris), len(s_tris), int(same),
@@ -348,6 +366,43 @@ def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, t
                                        out.shape[0],
                                        int(accumulate), threads))
     return out
+
+
+def galerkin_quad_block(family, nx, ny, k, test_coords, test_nodes, trial_coords, trial_nodes,
+                        same_mesh=True, same_elem=False, pol=None, linear=True):
+    """4x4 Galerkin block (test shape a, trial shape b) of one quadrilateral
+    pair, corners (4, 3) each (linear=False: the constant-field value in [0, 0])."""
+    pol = pol or policy()
+    kc = complex(k)
+    tc = np.ascontiguousarray(test_coords, dtype=np.float64).reshape(12)
+    sc = np.ascontiguousarray(trial_coords, dtype=np.float64).reshape(12)
+    tn = np.ascontiguousarray(test_nodes, dtype=np.int32)
+    sn = np.ascontiguousarray(trial_nodes, dtype=np.int32)
+    out = np.zeros(32)
+    _check(lib().orc_galerkin_quad_block(family, nx, ny, kc.real, kc.imag, C.byref(pol), _d(tc), _i(tn), _d(sc),
+                                         _i(sn), int(same_mesh), int(same_elem), int(linear), _d(out)))
+    return (out[0::2] + 1j * out[1::2]).reshape(4, 4)
+
+
+def quad_singular_rule(match, degree=7):
+    """Canonical quad x quad singular Galerkin rule of a match dimension (2
+    face, 1 edge, 0 corner) built on the Gauss rule of a degree: columns
+    (test xi, test eta, trial xi, trial eta, weight)."""
+    n1 = degree // 2 + 1
+    cap = 6 * n1 ** 4 * 2
+    pts = np.zeros((cap, 5))
+    n = C.c_int(0)
+    _check(lib().orc_quad_singular_rule(match, degree, _d(pts), cap, C.byref(n)))
+    return pts[:n.value].copy()
+
+
+def overlap_quads(a, b):
+    """(count, ind1, ind2) of NiHu's node-overlap scan of two quadrilaterals"""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    i1, i2 = C.c_int(0), C.c_int(0)
+    n = lib().orc_overlap_quads(_i(a), _i(b), C.byref(i1), C.byref(i2))
+    return n, i1.value, i2.value
 
 
 def singular_rule(match):

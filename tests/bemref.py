@@ -327,3 +327,60 @@ def gauss_field_shape(b):
     xo = [x for x in (-g, g) if x != xb][0]
     yo = [y for y in (-g, g) if y != yb][0]
     return lambda xi, eta: (np.asarray(xi) - xo) / (xb - xo) * (np.asarray(eta) - yo) / (yb - yo)
+
+
+# ---------------------------------------------------------------------------
+# Galerkin pairs of coplanar squares: the inner integral of 1/r over a
+# rectangle at a point of its plane in closed form, the outer integral by
+# Gauss rules graded toward the ends of each axis.
+# ---------------------------------------------------------------------------
+def _corner_antiderivative(p, q):
+    """F(p, q) with d^2 F / dp dq = 1 / sqrt(p^2 + q^2):
+    p ln(q + r) + q ln(p + r), each term 0 where its factor is 0."""
+    p = np.asarray(p, float)
+    q = np.asarray(q, float)
+    r = np.hypot(p, q)
+
+    def log_sum(a, b):
+        # ln(a + r) without cancellation: a + r = b^2 / (r - a) for a < 0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(a >= 0.0, np.log(a + r), np.log(b * b) - np.log(r - a))
+
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t1 = np.where(p == 0.0, 0.0, p * log_sum(q, p))
+        t2 = np.where(q == 0.0, 0.0, q * log_sum(p, q))
+    return t1 + t2
+
+
+def rect_potential(x, y, u0, u1, v0, v1):
+    """int_{u0}^{u1} int_{v0}^{v1} du dv / |(u, v) - (x, y)| at in-plane points."""
+    F = _corner_antiderivative
+    return (F(u1 - x, v1 - y) - F(u0 - x, v1 - y) - F(u1 - x, v0 - y) + F(u0 - x, v0 - y))
+
+
+def graded_rule(a, b, ratio=0.2, levels=24, n=12):
+    """Composite Gauss rule on [a, b] with subintervals shrinking
+    geometrically toward both ends (for integrands like t ln t there)."""
+    g, w = np.polynomial.legendre.leggauss(n)
+    m = 0.5 * (a + b)
+    cuts = [0.0] + [ratio ** j for j in range(levels, 0, -1)] + [1.0]
+    X, W = [], []
+    for lo, hi in zip(cuts[:-1], cuts[1:]):
+        for s in (-1.0, 1.0):
+            # [lo, hi] of the half length, measured from the end
+            t0, t1 = (a + lo * (m - a), a + hi * (m - a)) if s < 0 else (b - hi * (b - m), b - lo * (b - m))
+            X.append(0.5 * (t0 + t1) + 0.5 * (t1 - t0) * g)
+            W.append(0.5 * (t1 - t0) * w)
+    return np.concatenate(X), np.concatenate(W)
+
+
+def coplanar_squares_slp(test, trial):
+    """int_test int_trial 1 / |x - y| for axis-aligned squares
+    ((x0, x1), (y0, y1)) of one plane."""
+    (a0, a1), (b0, b1) = test
+    X, WX = graded_rule(a0, a1)
+    Y, WY = graded_rule(b0, b1)
+    XX, YY = np.meshgrid(X, Y, indexing="ij")
+    (u0, u1), (v0, v1) = trial
+    P = rect_potential(XX, YY, u0, u1, v0, v1)
+    return float(np.sum(P * np.outer(WX, WY)))
