@@ -23,6 +23,7 @@
 #include "nbem_galerkin.hpp"
 #include "nbem_quad.hpp"
 #include "nbem_gauss.hpp"
+#include "nbem_galquad.hpp"
 #include "nbem_solve.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -59,6 +60,11 @@ int nbg_quad_duffy(int ng, double x0, double y0, double *out);
 int nbg_gauss_nodes(int n, const double *X, double *x, double *nu);
 int nbg_quad4_table64(int N, int n, const double *X, double *out);
 int nbg_gauss_shapes(int N, double *out);
+int nbg_quad_gal_table(int N, int n, const double *c, const double *X, float *out);
+int nbg_quad_singular_rule(int match, double *out, int cap);
+int64_t nbg_singular_pairs_n(int n_nodes, int n, const int32_t *el, int npe, int32_t *pe, int32_t *pf, int32_t *info);
+int nbg_galerkin_layout_n(int n, const int32_t *tri, int npe, int cs, const double *c, int chunk,
+	int32_t *order_out, int32_t *elem, uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin);
 }
 
 namespace {
@@ -278,6 +284,33 @@ int device_sing_rules(const SingRules **out)
 	return NB_OK;
 }
 
+// the quad x quad rules of the Galerkin singular pass, per device
+std::mutex g_qsrules_mu;
+std::map<int, std::unique_ptr<SingRules> > g_qsrules;
+
+int device_quad_sing_rules(const SingRules **out)
+{
+	int dev = 0;
+	NB_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_qsrules_mu);
+	auto it = g_qsrules.find(dev);
+	if (it == g_qsrules.end()) {
+		std::unique_ptr<SingRules> sr(new SingRules());
+		std::vector<double> buf(5 * 2048);
+		for (int m = 0; m < 3; ++m) {
+			const int n = nbg_quad_singular_rule(m, buf.data(), 2048);
+			if (n <= 0) return fail(NB_ERR_RUNTIME, "singular Galerkin rule of quadrilaterals");
+			std::vector<double> v(buf.begin(), buf.begin() + 5 * n);
+			int s = upload(sr->r[m], v);
+			if (s) return s;
+			sr->n[m] = n;
+		}
+		it = g_qsrules.emplace(dev, std::move(sr)).first;
+	}
+	*out = it->second.get();
+	return NB_OK;
+}
+
 } // namespace
 
 struct nb_mesh {
@@ -299,7 +332,7 @@ struct nb_mesh {
 	// a list by shared_ptr, so evicting it cannot free one in use
 	std::vector<std::shared_ptr<NearList> > near_cache;
 	std::vector<int32_t> tri;                  // element-major node ids (host)
-	DevPtr d_tri;                              // node ids [3][n]
+	DevPtr d_tri;                              // node ids [npe][n] (quadrilaterals: uploaded on first use)
 	// Galerkin, nodal dofs: element order, wave tiles, column-slot schedule
 	int g_tiles = 0, g_chunk = 0;
 	DevPtr g_order, g_elem, g_contrib, g_rows, g_col, g_fin;
@@ -318,6 +351,8 @@ struct nb_mesh {
 	// the gauss field's nodes as test points (quadrilaterals): x(xi_a) [3][4n]
 	// FP64, the unit normals there FP32 and FP64
 	DevPtr d_gx, d_gnuf, d_gnu64;
+	// Galerkin on quadrilaterals: point rows of the N x N Gauss rules
+	DevPtr d_gq[nbk::GALQ_NMAX + 1];
 	// the sweep's per-call buffers (wavenumber constants, FP64 static parts),
 	// kept between calls and grown when a call needs more: a hipFree per call
 	// would wait for every stream of the device
@@ -563,19 +598,23 @@ int gal_chunk(int n_tiles, int sn)
 	return ch;
 }
 
+// the nodal-dof layout of a mesh of triangles (GAL_CS slots per lane) or
+// quadrilaterals (GALQ_CS slots)
 int ensure_gal_layout(nb_mesh *m)
 {
 	if (m->g_tiles > 0) return NB_OK;
-	const int n = m->n;
-	const int tiles = nbg_galerkin_layout(n, m->tri.data(), m->c.data(), 512, nullptr, nullptr, nullptr, nullptr,
-		nullptr, nullptr);
+	const int n = m->n, npe = m->npe;
+	const int cs = npe == 4 ? nbk::GALQ_CS : nbk::GAL_CS;
+	const int tiles = nbg_galerkin_layout_n(n, m->tri.data(), npe, cs, m->c.data(), 512, nullptr, nullptr, nullptr,
+		nullptr, nullptr, nullptr);
+	if (tiles <= 0) return fail(NB_ERR_RUNTIME, "Galerkin layout");
 	const int chunk = gal_chunk(tiles, n);
 	const size_t nl = (size_t)tiles * nbk::GAL_TILE;
-	std::vector<int32_t> order(n), elem(nl), rows(nl), col(4 * (size_t)n),
-		fin((size_t)((n + chunk - 1) / chunk) * nbk::GAL_CS);
+	std::vector<int32_t> order(n), elem(nl), rows(nl), col((size_t)(npe + 1) * n),
+		fin((size_t)((n + chunk - 1) / chunk) * cs);
 	std::vector<uint32_t> contrib(2 * nl);
-	if (nbg_galerkin_layout(n, m->tri.data(), m->c.data(), chunk, order.data(), elem.data(), contrib.data(),
-		rows.data(), col.data(), fin.data()) != tiles)
+	if (nbg_galerkin_layout_n(n, m->tri.data(), npe, cs, m->c.data(), chunk, order.data(), elem.data(),
+		contrib.data(), rows.data(), col.data(), fin.data()) != tiles)
 		return fail(NB_ERR_RUNTIME, "Galerkin layout");
 	int s;
 	if ((s = upload(m->g_order, order)) || (s = upload(m->g_elem, elem)) || (s = upload(m->g_contrib, contrib)) ||
@@ -590,10 +629,10 @@ int ensure_gal_layout(nb_mesh *m)
 int ensure_sing_pairs(nb_mesh *m)
 {
 	if (m->sp_n >= 0) return NB_OK;
-	const int64_t cnt = nbg_singular_pairs(m->n_nodes, m->n, m->tri.data(), nullptr, nullptr, nullptr);
+	const int64_t cnt = nbg_singular_pairs_n(m->n_nodes, m->n, m->tri.data(), m->npe, nullptr, nullptr, nullptr);
 	if (cnt > 0x7fffffff) return fail(NB_ERR_RANGE, "too many singular element pairs");
 	std::vector<int32_t> pe(cnt), pf(cnt), info(cnt);
-	nbg_singular_pairs(m->n_nodes, m->n, m->tri.data(), pe.data(), pf.data(), info.data());
+	nbg_singular_pairs_n(m->n_nodes, m->n, m->tri.data(), m->npe, pe.data(), pf.data(), info.data());
 	int s;
 	if ((s = upload(m->sp_e, pe)) || (s = upload(m->sp_f, pf)) || (s = upload(m->sp_info, info))) return s;
 	m->sp_n = cnt;
@@ -655,11 +694,206 @@ struct Job {
 	int passes = 3;
 };
 
+// element nodes [npe][n] on the device (quadrilateral meshes upload them
+// on first use)
+int ensure_elem_nodes(nb_mesh *m)
+{
+	if (m->d_tri) return NB_OK;
+	const int n = m->n, npe = m->npe;
+	std::vector<int32_t> soa((size_t)npe * n);
+	for (int e = 0; e < n; ++e)
+		for (int a = 0; a < npe; ++a) soa[(size_t)a * n + e] = m->tri[(size_t)npe * e + a];
+	int s = upload(m->d_tri, soa);
+	if (s) return s;
+	m->bytes += (int64_t)soa.size() * 4;
+	return NB_OK;
+}
+
+// Galerkin point rows of the N x N Gauss rule of a quadrilateral mesh
+int ensure_gq(nb_mesh *m, int N)
+{
+	if (N < 1 || N > nbk::GALQ_NMAX) return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
+	if (m->d_gq[N]) return NB_OK;
+	std::vector<float> tab((size_t)m->n * N * N * nbk::GALQ_ROW);
+	if (nbg_quad_gal_table(N, m->n, m->c.data(), m->X.data(), tab.data()) != N * N)
+		return fail(NB_ERR_RUNTIME, "quadrilateral Galerkin table");
+	int s = upload(m->d_gq[N], tab);
+	if (s) return s;
+	m->bytes += (int64_t)tab.size() * 4;
+	return NB_OK;
+}
+
+// Galerkin assembly on quadrilateral meshes into a device buffer: the FP32
+// regular pass over all pairs (node-sharing pairs skipped), then on one mesh
+// the FP64 pass over the coincident, edge- and corner-adjacent pairs
+int run_galerkin_quad(const Job &jb, void *out, int64_t ld, int dtype, int accumulate, hipStream_t st)
+{
+	nb_mesh *trial = jb.trial;
+	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
+	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
+	const bool ck = jb.ki != 0.0 && jb.family == NB_HELMHOLTZ;
+	const int64_t rows = nodal ? test->n_nodes : test->n, cols = nodal ? trial->n_nodes : trial->n;
+	const int eb = dtype_bytes(dtype);
+	const double four_pi = 4.0 * 3.141592653589793238462643383279502884;
+	auto zero = [&]() -> int {
+		launch_zero(out, ld, rows, cols, eb, st);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	};
+	int s;
+	if ((s = ensure_elem_nodes(test)) || (s = ensure_elem_nodes(trial))) return s;
+	if (jb.family == NB_IDENTITY) {
+		if (!accumulate && (s = zero())) return s;
+		// local operator: nothing on different meshes (nihu/core/assembly.hpp:141-142)
+		if (jb.same)
+			hipLaunchKernelGGL(nbk::galq_identity_kernel, dim3((test->n + 255) / 256), dim3(256), 0, st, out,
+				(long long)ld, test->n, (const double *)test->d_X.get(), (const int *)test->d_tri.get(),
+				nodal ? 1 : 0, dtype, jb.alpha.re, jb.alpha.im);
+		NB_HIP(hipGetLastError());
+		return NB_OK;
+	}
+	// 1/r^3 has no valid singular rule on coincident and edge-adjacent pairs
+	// (nihu/core/singular_accelerator.hpp:483-503, double_integral.hpp:255-273)
+	if (jb.same && jb.km == nbk::HSP) return fail(NB_ERR_RUNTIME, "Invalid general singular quadrature");
+	const int complexity = nodal ? 2 : 1;
+	int fixed_n = 0;
+	std::vector<int> ns;
+	if (jb.pol.rule == NB_RULE_FIXED) {
+		if (jb.pol.fixed_degree > 14)
+			return fail(NB_ERR_RANGE, "Too high quadrature degree selected for collocational integration");
+		fixed_n = jb.pol.fixed_degree / 2 + 1;
+		ns.push_back(fixed_n);
+	} else {
+		if ((s = ensure_thresholds(trial, jb.pol.near_mode, jb.pol.near_limit))) return s;
+		std::vector<int> degs;
+		policy_degrees(jb.pol, jb.km, complexity, degs);
+		for (int d : degs) ns.push_back(d / 2 + 1);
+	}
+	for (int N : ns)
+		if ((s = ensure_gq(trial, N)) || (s = ensure_gq(test, N))) return s;
+	if (nodal && ((s = ensure_gal_layout(test)) || (s = ensure_gal_layout(trial)))) return s;
+	if (jb.same && (s = ensure_sing_pairs(trial))) return s;
+	if (nodal && !accumulate && (s = zero())) return s;
+
+	nbk::GalQArgs a;
+	std::memset(&a, 0, sizeof(a));
+	if (nodal) {
+		a.n_tiles = test->g_tiles;
+		a.chunk = trial->g_chunk;
+		a.t_elem = (const int *)test->g_elem.get();
+		a.t_contrib = (const uint2 *)test->g_contrib.get();
+		a.t_rows = (const int *)test->g_rows.get();
+		a.s_order = (const int *)trial->g_order.get();
+		a.s_col = (const int *)trial->g_col.get();
+		a.s_fin = (const int *)trial->g_fin.get();
+	} else {
+		a.n_tiles = (test->n + nbk::GAL_TILE - 1) / nbk::GAL_TILE;
+		a.chunk = gal_chunk(a.n_tiles, trial->n);
+	}
+	a.tn = test->n; a.sn = trial->n; a.same_mesh = jb.same;
+	a.t_quad = (const int *)test->d_tri.get();
+	a.s_quad = (const int *)trial->d_tri.get();
+	a.tc = (const double *)test->d_c.get();
+	a.sc = (const double *)trial->d_c.get();
+	a.thr = (const double *)trial->d_thr.get();
+	for (int N = 1; N <= nbk::GALQ_NMAX; ++N) {
+		a.tq[N] = (const float *)test->d_gq[N].get();
+		a.sq[N] = (const float *)trial->d_gq[N].get();
+	}
+	a.fixed_n = fixed_n;
+	a.complexity = complexity;
+	const double two_pi = 6.283185307179586476925286766559005768;
+	a.k = (float)jb.k;
+	a.k_rev = jb.k / two_pi;
+	a.k_revf = (float)a.k_rev;
+	a.ki = (float)jb.ki;
+	a.alpha = make_float2((float)(jb.alpha.re / four_pi), (float)(jb.alpha.im / four_pi));
+	a.out = out; a.ld = ld; a.dtype = dtype; a.accumulate = accumulate;
+	const long long grid = (long long)((a.n_tiles + 3) / 4) * ((trial->n + a.chunk - 1) / a.chunk);
+	{
+		using namespace nbk;
+		const int pe = prof_begin(0, st);
+#define NB_GQ(F, K) do { \
+		if (F == HELM && ck) { \
+			if (nodal) hipLaunchKernelGGL((galq_regular_kernel<F, K, 4, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+			else hipLaunchKernelGGL((galq_regular_kernel<F, K, 1, (F == HELM)>), dim3(grid), dim3(256), 0, st, a); \
+		} else if (nodal) hipLaunchKernelGGL((galq_regular_kernel<F, K, 4>), dim3(grid), dim3(256), 0, st, a); \
+		else hipLaunchKernelGGL((galq_regular_kernel<F, K, 1>), dim3(grid), dim3(256), 0, st, a); } while (0)
+		if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_GQ(HELM, SLP); break;
+			case DLP: NB_GQ(HELM, DLP); break;
+			case DLPT: NB_GQ(HELM, DLPT); break;
+			default: NB_GQ(HELM, HSP); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_GQ(LAP, SLP); break;
+			case DLP: NB_GQ(LAP, DLP); break;
+			case DLPT: NB_GQ(LAP, DLPT); break;
+			default: NB_GQ(LAP, HSP); break;
+			}
+		}
+#undef NB_GQ
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	if (!jb.same) return NB_OK;
+
+	// node-sharing pairs in FP64 (the double layer of a coincident
+	// quadrilateral is integrated: it vanishes only on flat elements)
+	const SingRules *sr = nullptr;
+	if ((s = device_quad_sing_rules(&sr))) return s;
+	nbk::GalQSingArgs g;
+	std::memset(&g, 0, sizeof(g));
+	g.n_pairs = (int)trial->sp_n;
+	if (g.n_pairs <= 0) return NB_OK;
+	g.pe = (const int *)trial->sp_e.get();
+	g.pf = (const int *)trial->sp_f.get();
+	g.pinfo = (const int *)trial->sp_info.get();
+	g.n = trial->n;
+	g.X = (const double *)trial->d_X.get();
+	g.quad = (const int *)trial->d_tri.get();
+	for (int m = 0; m < 3; ++m) { g.rule[m] = (const double *)sr->r[m].get(); g.rule_n[m] = sr->n[m]; }
+	g.k = jb.k;
+	g.ki = jb.ki;
+	g.alpha = make_double2(jb.alpha.re, jb.alpha.im);
+	g.out = out; g.ld = ld; g.dtype = dtype;
+	{
+		using namespace nbk;
+		const int pe = prof_begin(2, st);
+#define NB_GQS(F, K) do { \
+		if (F == HELM && ck) { \
+			if (nodal) hipLaunchKernelGGL((galq_singular_kernel<F, K, 4, (F == HELM)>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+			else hipLaunchKernelGGL((galq_singular_kernel<F, K, 1, (F == HELM)>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+		} else if (nodal) hipLaunchKernelGGL((galq_singular_kernel<F, K, 4>), dim3(g.n_pairs), dim3(64), 0, st, g); \
+		else hipLaunchKernelGGL((galq_singular_kernel<F, K, 1>), dim3(g.n_pairs), dim3(64), 0, st, g); } while (0)
+		if (jb.family == NB_HELMHOLTZ) {
+			switch (jb.km) {
+			case SLP: NB_GQS(HELM, SLP); break;
+			case DLP: NB_GQS(HELM, DLP); break;
+			default: NB_GQS(HELM, DLPT); break;
+			}
+		} else {
+			switch (jb.km) {
+			case SLP: NB_GQS(LAP, SLP); break;
+			case DLP: NB_GQS(LAP, DLP); break;
+			default: NB_GQS(LAP, DLPT); break;
+			}
+		}
+#undef NB_GQS
+		NB_HIP(hipGetLastError());
+		prof_end(pe, st);
+	}
+	return NB_OK;
+}
+
 // Galerkin assembly into a device buffer
 int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate, hipStream_t st)
 {
 	if (jb.pol.precision != NB_PREC_MIXED)
-		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation on triangles");
+		return fail(NB_ERR_UNSUPPORTED, "the FP64 precision policy is implemented for collocation");
+	if (jb.trial->npe == 4) return run_galerkin_quad(jb, out, ld, dtype, accumulate, st);
 	nb_mesh *trial = jb.trial;
 	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
 	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
@@ -1973,8 +2207,9 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 	const bool nodal = formalism == NB_GALERKIN_LINEAR;
 	const bool ident = kernel.family == NB_IDENTITY;
 	int s;
-	if (test->npe != 3 || trial->npe != 3)
-		return fail(NB_ERR_UNSUPPORTED, "the Galerkin formalism is implemented for triangles only");
+	if (!((test->npe == 3 && trial->npe == 3) || (test->npe == 4 && trial->npe == 4)))
+		return fail(NB_ERR_UNSUPPORTED, "the Galerkin formalism takes meshes of one element type, triangles or "
+			"quadrilaterals, on both sides");
 	if (!ident && kernel.family == NB_HELMHOLTZ && !std::isfinite(kernel.k_im))
 		return fail(NB_ERR_ARG, "imaginary part of the wavenumber must be finite");
 	if (!ident) {

@@ -386,6 +386,43 @@ int nbg_quad4_table(int N, int n, const double *c, const double *X, float *out)
 	return N * N;
 }
 
+// Gauss square rule with N x N points (xi outer, eta inner) for the Galerkin
+// passes on quadrilaterals: out[(e * N*N + q) * 16 + {0..15}] =
+// o = y_q - c_e (xyz), |o|^2, unit normal n at y_q (xyz), o.n,
+// w_q J(xi_q) N_b(xi_q) for the four corners b, w_q J(xi_q), 0, 0, 0 (floats).
+int nbg_quad_gal_table(int N, int n, const double *c, const double *X, float *out)
+{
+	if (N < 1 || N > 64) return -1;
+	std::vector<double> gx(N), gw(N);
+	nbg_gauss_legendre(N, gx.data(), gw.data());
+	for (int e = 0; e < n; ++e) {
+		P3 V[4];
+		load_quad(X, n, e, V);
+		const P3 cc = {c[e], c[n + e], c[2 * n + e]};
+		for (int i = 0; i < N; ++i)
+			for (int j = 0; j < N; ++j) {
+				const double xi = gx[i], eta = gx[j];
+				const P3 y = quad_x(V, xi, eta);
+				const P3 nv = quad_normal(V, xi, eta);
+				const double J = nrm(nv);
+				const P3 o = sub(y, cc);
+				const P3 u = {nv.x / J, nv.y / J, nv.z / J};
+				const double w = gw[i] * gw[j] * J;
+				const double N4[4] = {0.25 * (1 - xi) * (1 - eta), 0.25 * (1 + xi) * (1 - eta),
+					0.25 * (1 + xi) * (1 + eta), 0.25 * (1 - xi) * (1 + eta)};
+				float *p = out + ((size_t)e * N * N + (size_t)i * N + j) * 16;
+				p[0] = (float)o.x; p[1] = (float)o.y; p[2] = (float)o.z;
+				p[3] = (float)(o.x * o.x + o.y * o.y + o.z * o.z);
+				p[4] = (float)u.x; p[5] = (float)u.y; p[6] = (float)u.z;
+				p[7] = (float)(o.x * u.x + o.y * u.y + o.z * u.z);
+				for (int b = 0; b < 4; ++b) p[8 + b] = (float)(w * N4[b]);
+				p[12] = (float)w;
+				p[13] = p[14] = p[15] = 0.0f;
+			}
+	}
+	return N * N;
+}
+
 // FP64 Gauss square rule with N x N points (xi outer, eta inner) for all
 // quadrilaterals: out[(e * N*N + q) * 8 + {0..7}] = y_q xyz, w_q J(xi_q),
 // unit normal at y_q xyz, 0 (the FP64 precision policy's regular pass).

@@ -291,7 +291,8 @@ def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.
     """out (+)= alpha * operator; formalism "collocation" (dirac(constant) x
     constant), "galerkin" (constant x constant) or "galerkin_linear"
     (isoparametric x isoparametric, rows / columns = mesh nodes).  k may be
-    complex (collocation on every mesh type, Galerkin on triangles)."""
+    complex (collocation on every mesh type, Galerkin on meshes of triangles
+    or of quadrilaterals)."""
     pol = pol or policy(**NIHU_POLICY)
     same = (test is trial) if same_mesh is None else bool(same_mesh)
     a = complex(alpha)

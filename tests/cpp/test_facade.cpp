@@ -340,15 +340,14 @@ int main(int argc, char *argv[])
 		} catch (std::invalid_argument const &ex) {
 			std::printf("gauss dof numbering: invalid_argument as expected: %s\n", ex.what());
 		}
-		// the Galerkin formalism is not implemented for quadrilaterals
-		try {
-			nihu_amd::cMatrix X(qspace.get_num_dofs(), qspace.get_num_dofs());
-			X << qspace * nihu_amd::create_integral_operator(nihu_amd::helmholtz_3d_SLP_kernel<double>(1.0))[qspace];
-			std::printf("no exception for quadrilateral Galerkin\n");
-			bad = 1;
-		} catch (std::logic_error const &ex) {
-			std::printf("quadrilateral Galerkin: logic_error as expected: %s\n", ex.what());
-		}
+		// the Galerkin formalism on quadrilaterals, constant and isoparametric
+		// (bilinear) fields
+		double gqc = pulsating(qspace, qspace);
+		auto const &qiso = nihu_amd::isoparametric_view(qmesh);
+		double gql = pulsating(qiso, qiso);
+		std::printf("Pulsating sphere relative error (quadrilateral Galerkin): constant %g, isoparametric %g\n",
+			gqc, gql);
+		bad |= !(gqc < 0.05 && gql < 0.05);
 	}
 
 	if (argc > 3) {

@@ -3,7 +3,8 @@ canonical singular rules and the node-overlap scan agree with the oracle, the
 singular pair list is every node-sharing pair, and the wave tiles and the
 column-slot schedule of the nodal regular kernel are consistent (every
 element in exactly one lane, every corner's local row and slot holds its node,
-every evicted node is the slot's previous occupant)."""
+every evicted node is the slot's previous occupant) -- for triangles and for
+quadrilaterals."""
 
 import ctypes as C
 
@@ -26,6 +27,11 @@ def raw():
     l.nbg_singular_pairs.argtypes = [C.c_int, C.c_int, I32, I32, I32, I32]
     l.nbg_singular_pairs.restype = C.c_int64
     l.nbg_galerkin_layout.argtypes = [C.c_int, I32, D, C.c_int, I32, I32, I32, I32, I32, I32]
+    l.nbg_quad_singular_rule.argtypes = [C.c_int, D, C.c_int]
+    l.nbg_overlap_n.argtypes = [I32, I32, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    l.nbg_singular_pairs_n.argtypes = [C.c_int, C.c_int, I32, C.c_int, I32, I32, I32]
+    l.nbg_singular_pairs_n.restype = C.c_int64
+    l.nbg_galerkin_layout_n.argtypes = [C.c_int, I32, C.c_int, C.c_int, D, C.c_int, I32, I32, I32, I32, I32, I32]
     return l
 
 
@@ -149,3 +155,116 @@ def test_column_slot_schedule_replays(raw, chunk):
     # every node of the mesh is flushed at least once, and on average few times
     assert set(flushed) == set(range(len(nodes)))
     assert sum(flushed.values()) < 2.5 * len(nodes) * max(1, 512 // chunk)
+
+
+# ------------------------------------------------------------ quadrilaterals
+QCS = 6
+
+
+@pytest.mark.parametrize("match", [0, 1, 2])
+def test_quad_singular_rule_matches_oracle(orc, raw, match):
+    buf = np.zeros((2048, 5))
+    n = raw.nbg_quad_singular_rule(match, buf.ctypes.data_as(D), 2048)
+    ref = orc.quad_singular_rule(match)
+    assert n == len(ref)
+    assert np.abs(buf[:n] - ref).max() <= 4e-16
+    assert raw.nbg_quad_singular_rule(match, buf.ctypes.data_as(D), n - 1) == -1
+
+
+def test_quad_overlap_scan_matches_oracle(orc, raw):
+    rng = np.random.default_rng(6)
+    i1, i2 = C.c_int(), C.c_int()
+    for _ in range(3000):
+        a = rng.choice(7, 4, replace=False).astype(np.int32)
+        b = rng.choice(7, 4, replace=False).astype(np.int32)
+        n = raw.nbg_overlap_n(_i(a), _i(b), 4, C.byref(i1), C.byref(i2))
+        assert (n, i1.value, i2.value) == orc.overlap_quads(a, b), (a, b)
+
+
+def _qmesh(n=4):
+    from nihu_amd.mesh import jittered_cubed_sphere
+    nodes, quads = jittered_cubed_sphere(n, seed=1)
+    return nodes, np.ascontiguousarray(quads, dtype=np.int32)
+
+
+def test_quad_singular_pairs_are_node_sharing_pairs(orc, raw):
+    nodes, quads = _qmesh(4)
+    n = len(quads)
+    cnt = raw.nbg_singular_pairs_n(len(nodes), n, _i(quads), 4, None, None, None)
+    pe, pf, info = (np.zeros(cnt, np.int32) for _ in range(3))
+    assert raw.nbg_singular_pairs_n(len(nodes), n, _i(quads), 4, _i(pe), _i(pf), _i(info)) == cnt
+    assert np.all(pe[:n] == np.arange(n)) and np.all(info[:n] == 2)
+    ref = {(e, f) for e in range(n) for f in range(n) if e != f and set(quads[e]) & set(quads[f])}
+    got = set(zip(pe[n:].tolist(), pf[n:].tolist()))
+    assert got == ref and len(got) == cnt - n
+    for e, f, inf in zip(pe[n:], pf[n:], info[n:]):
+        num, i1, i2 = orc.overlap_quads(quads[e], quads[f])
+        assert inf == (1 if num > 1 else 0) | (i1 << 4) | (i2 << 8)
+
+
+def _qlayout(raw, nodes, quads, chunk):
+    n = len(quads)
+    c = np.ascontiguousarray(nodes[quads].mean(1).T)
+    tiles = raw.nbg_galerkin_layout_n(n, _i(quads), 4, QCS, c.ctypes.data_as(D), chunk, None, None, None, None,
+                                      None, None)
+    order = np.zeros(n, np.int32)
+    elem, rows = (np.zeros(tiles * TILE, np.int32) for _ in range(2))
+    contrib = np.zeros(tiles * TILE * 2, np.uint32)
+    col = np.zeros(5 * n, np.int32)
+    fin = np.zeros(((n + chunk - 1) // chunk) * QCS, np.int32)
+    assert raw.nbg_galerkin_layout_n(n, _i(quads), 4, QCS, c.ctypes.data_as(D), chunk, _i(order), _i(elem),
+                                     contrib.ctypes.data_as(I32), _i(rows), _i(col), _i(fin)) == tiles
+    return tiles, order, elem, contrib.reshape(-1, 2), rows, col.reshape(n, 5), fin.reshape(-1, QCS)
+
+
+def test_quad_layout_rejects_bad_arguments(raw):
+    nodes, quads = _qmesh(2)
+    c = np.ascontiguousarray(nodes[quads].mean(1).T)
+    for npe, cs in ((4, 3), (4, 9), (5, 6), (2, 6)):
+        assert raw.nbg_galerkin_layout_n(len(quads), _i(quads), npe, cs, c.ctypes.data_as(D), 64, None, None,
+                                         None, None, None, None) == -1
+
+
+def test_quad_wave_tiles_and_slot_schedule(raw):
+    nodes, quads = _qmesh(6)
+    n = len(quads)
+    for chunk in (64, 256):
+        tiles, order, elem, contrib, rows, col, fin = _qlayout(raw, nodes, quads, chunk)
+        assert sorted(order.tolist()) == list(range(n))
+        used = elem[elem >= 0]
+        assert np.all(used == order)
+        for t in range(tiles):
+            r = rows[t * TILE:(t + 1) * TILE]
+            nr = int((r >= 0).sum())
+            assert len(set(r[:nr].tolist())) == nr and np.all(r[nr:] == -1)
+            seen = set()
+            for l in range(TILE):
+                bytes8 = [(int(contrib[t * TILE + l][q >> 2]) >> (8 * (q & 3))) & 255 for q in range(8)]
+                cs = [b for b in bytes8 if b != 255]
+                for b in cs:
+                    lane, a = b >> 2, b & 3
+                    e = elem[t * TILE + lane]
+                    assert e >= 0 and quads[e][a] == r[l]
+                    seen.add((lane, a))
+            assert seen == {(l, a) for l in range(TILE) if elem[t * TILE + l] >= 0 for a in range(4)}
+        flushed = {}
+        for c0 in range(0, n, chunk):
+            slot = [-1] * QCS
+            for i in range(c0, min(n, c0 + chunk)):
+                e = order[i]
+                sl = [(col[i, 4] >> (5 * b)) & 31 for b in range(4)]
+                assert len(set(sl)) == 4 and max(sl) < QCS
+                for b in range(4):
+                    v = col[i, b]
+                    if v >= 0:
+                        assert slot[sl[b]] == v
+                        flushed[v] = flushed.get(v, 0) + 1
+                        slot[sl[b]] = -1
+                    if slot[sl[b]] == -1:
+                        slot[sl[b]] = quads[e][b]
+                    assert slot[sl[b]] == quads[e][b]
+            assert slot == fin[c0 // chunk].tolist()
+            for v in slot:
+                if v >= 0:
+                    flushed[v] = flushed.get(v, 0) + 1
+        assert set(flushed) == set(range(len(nodes)))

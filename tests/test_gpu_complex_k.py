@@ -10,7 +10,8 @@ Galerkin matrices (<= 1e-5); on quadrilateral meshes and on meshes with
 both element types (collocation, NiHu's degrees and a fixed Gauss degree 5):
 rows to 2e-6 of their largest entry, 1e-5 relative Frobenius and the FP64-pass
 entries (near-singular by polar / Telles, diagonal by Duffy / Guiggiani) to
-1e-9; the Galerkin formalism on quadrilaterals is refused."""
+1e-9 (the Galerkin formalism on quadrilaterals with complex wavenumbers is in
+test_gpu_galerkin_quad.py)."""
 
 import numpy as np
 import pytest
@@ -110,14 +111,9 @@ def test_galerkin(nb, orc, formalism, nx, ny):
 
 
 def test_refusals(nb):
-    from nihu_amd.mesh import jittered_icosphere, sphere_boundary
+    from nihu_amd.mesh import jittered_icosphere
     nodes, tris = jittered_icosphere(1, seed=3)
     m = nb.Mesh(nodes, tris)
-    k = complex(2.0, -0.5)
-    qn, qe = sphere_boundary(1.0, 2)
-    q = nb.Mesh(qn, qe)
-    with pytest.raises(NotImplementedError):
-        nb.assemble(q, q, nb.colmajor(q.n, q.n), nb.NB_HELMHOLTZ, 0, 0, k, formalism="galerkin")
     with pytest.raises(ValueError):
         nb.assemble_bm(m, complex(-1.0, 0.5), nb.colmajor(m.n, m.n), nb.colmajor(m.n, m.n))
 

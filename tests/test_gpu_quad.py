@@ -10,8 +10,8 @@ FP64 oracle:
 * the Burton-Miller pair and the solved surface field; the Mie series;
 * triangle test points on a quadrilateral trial mesh and the reverse;
 * the sweep: one Burton-Miller assembly per wavenumber;
-* the Galerkin formalism, sweep autotuning and near-singular rules beyond
-  NiHu's d / R < 1.5 reject quadrilateral meshes.
+* near-singular rules beyond NiHu's d / R < 1.5 reject quadrilateral meshes
+  (the Galerkin formalism on quadrilaterals is in test_gpu_galerkin_quad.py).
 """
 
 import numpy as np
@@ -166,8 +166,6 @@ def test_quad_rejections(nb, qmesh):
     import torch
     m = nb.Mesh(*qmesh)
     out = nb.colmajor(m.n, m.n)
-    with pytest.raises(Exception, match="Galerkin"):
-        nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(**NIHU[0]), formalism="galerkin")
     # far "near-singular" pairs would need the inverse mapping far from the element
     with pytest.raises(Exception, match="near-singular rule"):
         nb.assemble(m, m, out, 1, 0, 0, 1.0, pol=nb.policy(rule="fixed", degree=5, near="diameter", limit=2.0))
