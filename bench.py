@@ -24,8 +24,9 @@ against the oracle), and `serve`, a Poisson stream of assembly-plus-solve
 requests served with and without batching (p50/p99 latency, solves/s) and a
 rate sweep up to the first rate the server does not keep up with; then short
 runs (5 timed steps, 32 rows or node groups against the oracle) of the Galerkin workload
-(config 3), the quadrilateral, the gauss-field and the mixed-element workloads,
-each with its roofline: `galerkin`, `quad`, `gauss`, `mixed`, and config 1, the
+(config 3), the Galerkin workload on quadrilaterals, the quadrilateral, the
+gauss-field and the mixed-element workloads, each with its roofline:
+`galerkin`, `galerkin_quad`, `quad`, `gauss`, `mixed`, and config 1, the
 Laplace matrices of 1280 triangles in FP64, `laplace_fp64` (--no-extras skips
 all of them).
 
@@ -162,11 +163,13 @@ def main():
                     help="rocprofv3 FETCH/WRITE summary of the regular kernel (default: the one of --policy)")
     ap.add_argument("--profile-copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM-counter calibration)")
-    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "quad", "gauss", "mixed", "laplace"],
+    ap.add_argument("--workload", choices=["bm", "sweep", "serve", "galerkin", "galerkin_quad", "quad", "gauss", "mixed",
+                                           "laplace"],
                     default="bm",
                     help="bm: the headline single-wavenumber workload; sweep: 64 wavenumbers on 5120 triangles; "
                          "serve: Poisson stream of assembly-plus-LU-solve requests; galerkin: SLP and DLP "
-                         "Galerkin matrices with linear triangles on the 20480-element ellipsoid; quad: "
+                         "Galerkin matrices with linear triangles on the 20480-element ellipsoid; galerkin_quad: "
+                         "the same with bilinear quadrilaterals on a 9600-element cube sphere; quad: "
                          "Burton-Miller collocation on a 20184-element quadrilateral cubed sphere; gauss: the "
                          "same with the gauss field (4 dofs per element) on 5400 quadrilaterals; mixed: the "
                          "same on a 20000-element surface of triangles and quadrilaterals (NiHu element table); "
@@ -194,6 +197,9 @@ def main():
         return run_serve(args)
     if args.workload == "galerkin":
         print(json.dumps(run_galerkin(args)))
+        return
+    if args.workload == "galerkin_quad":
+        print(json.dumps(run_galerkin_quad(args)))
         return
     if args.workload == "quad":
         print(json.dumps(run_quad(args)))
@@ -350,7 +356,8 @@ def main():
         # (Galerkin: the test elements of 32 sampled nodes against the
         # oracle), the quadrilateral and the mixed-element workloads (32 rows),
         # each with its roofline
-        for key, fn, rows in (("galerkin", run_galerkin, -1), ("quad", run_quad, 32), ("gauss", run_gauss, 1024),
+        for key, fn, rows in (("galerkin", run_galerkin, -1), ("galerkin_quad", run_galerkin_quad, -1),
+                              ("quad", run_quad, 32), ("gauss", run_gauss, 1024),
                               ("mixed", run_mixed, 32), ("laplace_fp64", run_laplace, -1)):
             rec = fn(argparse.Namespace(steps=5, warmup=5, cpu_rows=rows, profile_copy_bytes=0, level=5))
             extras[key] = {k: rec.get(k) for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
@@ -505,6 +512,156 @@ def run_galerkin(args):
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
     }
     return out
+
+
+# FP32 operations per (test point, trial point) of the quadrilateral Galerkin
+# regular pass with bilinear nodal fields, as the kernel writes them: r^2
+# (3 FMA, 2 adds), rsq, the phase k (r - d) = k u / (r + d) (FMA, rcp, mul,
+# FMA), sin and cos, the kernel (SLP 2 muls; DLP the 1/r^3 and k/r^2 terms,
+# r.n_y with the point's own normal, 3 FMA, and 2 muls) and the four
+# shape-weighted complex sums (8 FMA); the per-test-point block update is not
+# counted
+FLOPS_PER_POINTPAIR_GALQ = {"SLP": 35, "DLP": 50}
+
+
+def galq_point_pairs(nodes, quads, complexity=2):
+    """Point pairs per kernel of the regular quadrilateral Galerkin pairs under
+    NiHu's degrees: N^4 per element pair, N = (complexity + table degree) / 2
+    + 1 from d / R_trial (1/r table for SLP, 1/r^2 for DLP); node-sharing
+    pairs are the FP64 pass's."""
+    import numpy as np
+    X = nodes[quads]
+    c = X.mean(1)
+    R = np.mean([np.linalg.norm(X[:, (a + 1) % 4] - X[:, a], axis=1) for a in range(4)], axis=0)
+    tabs = {"SLP": [(1.1, 6), (1.8, 4), (9.1, 2), (np.inf, 0)],
+            "DLP": [(1.1, 8), (1.5, 6), (2.8, 4), (15.8, 2), (np.inf, 0)]}
+    out = {"SLP": 0, "DLP": 0}
+    n = len(quads)
+    for i0 in range(0, n, 512):
+        d = np.linalg.norm(c[i0:i0 + 512, None, :] - c[None, :, :], axis=2) / R[None, :]
+        for key, tab in tabs.items():
+            deg = np.full(d.shape, complexity + tab[-1][1])
+            for lim, dg in reversed(tab[:-1]):
+                deg[d < lim] = complexity + dg
+            N = deg // 2 + 1
+            out[key] += int((N.astype(np.int64) ** 4).sum())
+    # node-sharing pairs (subtract at the smallest distances: the finest rule)
+    node_elems = [[] for _ in range(len(nodes))]
+    for e, q in enumerate(quads):
+        for v in q:
+            node_elems[v].append(e)
+    for e, q in enumerate(quads):
+        near = set()
+        for v in q:
+            near.update(node_elems[v])
+        for f in near:
+            dd = np.linalg.norm(c[e] - c[f]) / R[f]
+            for key, tab in tabs.items():
+                dg = next(g for lim, g in tab if dd < lim)
+                out[key] -= ((complexity + dg) // 2 + 1) ** 4
+    return out
+
+
+def run_galerkin_quad(args):
+    """Quadrilateral Galerkin workload: Helmholtz single- and double-layer
+    matrices with isoparametric (bilinear, nodal) fields on NiHu's blown-up
+    cube sphere of 9600 quadrilaterals (40 x 40 per face, 9602 nodes), k = 5,
+    NiHu's degree policy (Gauss degree 2 + the distance interval on both
+    elements) in FP32, coincident / edge / corner pairs with the quad x quad
+    rules in FP64, complex64 results.  A step assembles both 9602^2
+    matrices."""
+    import numpy as np
+    import torch
+    from nihu_amd import nbem
+    from nihu_amd.mesh import cubed_sphere
+
+    torch.cuda.set_device(0)
+    nodes, quads = cubed_sphere(40)
+    ne, nn = len(quads), len(nodes)
+    k = 5.0
+    pol = nbem.policy(rule="nihu")
+    name, ncu, hbm = nbem.device_info()
+    log("device %s, %d CUs; cube sphere %d quadrilaterals, %d nodes; k = %g" % (name, ncu, ne, nn, k))
+    mesh = nbem.Mesh(nodes, quads)
+    L, M = nbem.colmajor(nn, nn), nbem.colmajor(nn, nn)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        nbem.assemble(mesh, mesh, L, 1, 0, 0, k, pol=pol, accumulate=False, stream=stream,
+                      formalism="galerkin_linear")
+        nbem.assemble(mesh, mesh, M, 1, 0, 1, k, pol=pol, accumulate=False, stream=stream,
+                      formalism="galerkin_linear")
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log("warmup step %d done" % w)
+    nbem.profile(True)
+    nbem.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = nbem.profile_read(reset=True)
+    nbem.profile(False)
+    ms_per_step = wall / args.steps * 1e3
+    value = 2 * nn * nn * args.steps / wall
+    pass_ms = {p: (v[0] / v[1] if v[1] else 0.0) for p, v in prof.items()}
+    log("timed %d steps: %.3f ms/step, %.4g entries/s; per-launch device ms %s" % (args.steps, ms_per_step, value,
+                                                                                 pass_ms))
+    pp = galq_point_pairs(nodes, quads)
+    reg_ms_pair = 2 * pass_ms["regular"]       # SLP and DLP launches per step
+    flops = pp["SLP"] * FLOPS_PER_POINTPAIR_GALQ["SLP"] + pp["DLP"] * FLOPS_PER_POINTPAIR_GALQ["DLP"]
+    ach = flops / (reg_ms_pair * 1e-3) / 1e12 if reg_ms_pair > 0 else 0.0
+    roofline = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "engine": "VALU fp32 (no MFMA; packed fp32 disabled at build)",
+                "kernel": "galq_regular_kernel<HELM,{SLP,DLP},4>", "kernel_ms": round(reg_ms_pair, 4),
+                "algorithmic_flops": flops, "point_pairs": pp}
+
+    cpu = err = None
+    if args.cpu_rows != 0:
+        from oracle import oracle
+        rng = np.random.default_rng(123)
+        n_sample = 32 if args.cpu_rows < 0 else max(1, args.cpu_rows // 16)
+        sample = np.sort(rng.choice(nn, n_sample, replace=False))
+        elems = np.unique(np.nonzero(np.isin(quads, sample))[0]).astype(np.int32)
+        opol = oracle.policy()
+        threads = oracle.num_threads()
+        tc = time.perf_counter()
+        Lr = oracle.assemble_galerkin(nodes, quads, 1, 0, 0, k=k, pol=opol, linear=True, test_elems=elems,
+                                      threads=threads)[sample]
+        Mr = oracle.assemble_galerkin(nodes, quads, 1, 0, 1, k=k, pol=opol, linear=True, test_elems=elems,
+                                      threads=threads)[sample]
+        dt = time.perf_counter() - tc
+        cpu_value = 2 * nn * nn / (dt * ne / len(elems))
+        cpu = {"value": cpu_value, "unit": "matrix entries/s", "cores": threads, "kind": "port",
+               "sample": "test elements around %d sampled nodes (%d of %d elements) against all %d trial "
+                         "elements, SLP and DLP, FP64 oracle, %.1f s wall; rate scaled by %d/%d to the whole "
+                         "matrices" % (len(sample), len(elems), ne, ne, dt, len(elems), ne),
+               "host": host_cpu()}
+        si = torch.from_numpy(sample.astype(np.int64)).cuda()
+        Lg = L[si, :].cpu().numpy().astype(np.complex128)
+        Mg = M[si, :].cpu().numpy().astype(np.complex128)
+        err = {"rows": int(len(sample)),
+               "L_rel_fro": float(np.linalg.norm(Lg - Lr) / np.linalg.norm(Lr)),
+               "M_rel_fro": float(np.linalg.norm(Mg - Mr) / np.linalg.norm(Mr))}
+        log("sampled-row error vs oracle: %s; cpu %.4g entries/s on %d threads" % (err, cpu_value, threads))
+
+    return {
+        "metric": "matrix entries/s, Helmholtz 3D Galerkin bilinear quads, 9600 elems",
+        "value": value, "unit": "matrix entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64) regular pairs, fp64 singular pairs",
+        "data": "synthetic: NiHu's blown-up cube sphere",
+        "config": {"workload": "Helmholtz 3D Galerkin, isoparametric bilinear quadrilaterals, SLP and DLP matrices",
+                   "n_elements": ne, "n_nodes": nn, "k": k,
+                   "regular_rule": "NiHu degrees: Gauss degree 2 + distance interval, N x N points on both elements",
+                   "singular": "coincident/edge/corner quad x quad rules, 4-point Gauss per axis, FP64"},
+        "roofline": roofline, "cpu_baseline": cpu, "accuracy": err, "pass_ms": pass_ms,
+    }
 
 
 def run_laplace(args):

@@ -6,7 +6,9 @@
  * for 3D Laplace / Helmholtz SLP, DLP, DLPt and HSP kernels on flat 3-node
  * triangles: collocation (dirac(constant_view) x constant_view) and Galerkin
  * (constant_view x constant_view, isoparametric_view x isoparametric_view);
- * and on 4-node bilinear quadrilaterals: collocation with constant fields.
+ * and on 4-node bilinear quadrilaterals: collocation with constant fields
+ * and with the gauss field, and Galerkin with constant and isoparametric
+ * (bilinear) fields.
  * Every entry point returns an
  * nb_status; on failure nb_last_error() returns a message for the calling
  * thread.  Plain pointers and sizes only.
@@ -69,9 +71,14 @@ enum nb_location { NB_HOST = 0, NB_DEVICE = 1 };
  *   NB_GALERKIN_LINEAR  isoparametric_view x isoparametric_view: rows / columns
  *                       are mesh nodes (nihu/core/field.hpp:234-258)
  * Galerkin pairs that share a node on the same mesh use the Duffy-type rules of
- * nihu/core/singular_galerkin_quadrature.hpp:197-455; the hypersingular kernel
+ * nihu/core/singular_galerkin_quadrature.hpp:197-455 (triangles) and :457-625
+ * (quadrilaterals, the subdomains of nihu/library/
+ * lib_singular_galerkin_quadrature.cpp:6-31), rotated to the shared corners
+ * as nihu/core/singular_accelerator.hpp:283-325 does; the hypersingular kernel
  * has none for coincident and edge-adjacent pairs and fails with
- * NB_ERR_RUNTIME "Invalid general singular quadrature", as NiHu throws. */
+ * NB_ERR_RUNTIME "Invalid general singular quadrature", as NiHu throws.  Both
+ * meshes of a Galerkin assembly hold one element type, triangles or
+ * quadrilaterals (otherwise NB_ERR_UNSUPPORTED). */
 /* NB_COLLOCATION_GAUSS: collocation with the quadrilateral gauss field
  * (NiHu's quad_1_gauss_field, nihu/library/quad_1_gauss_field.hpp, as the
  * GAUSS builds of nihu/applications/helmholtz/helmholtz_3d_coll.cpp:221-240
@@ -95,8 +102,7 @@ typedef struct nb_mesh nb_mesh;
 /* kernel: family, normal-derivative orders (nx, ny) in {0,1}^2, wavenumber
  * k + i k_im (NiHu's helmholtz_kernel<space, WaveNumber> with a real or
  * complex WaveNumber, library/helmholtz_kernel.hpp:103; k > 0; a nonzero
- * k_im is taken in collocation on every mesh type and in the Galerkin
- * formalism on triangles) */
+ * k_im is taken in every formalism on every mesh type) */
 typedef struct {
 	int32_t family;
 	int32_t nx;
@@ -160,8 +166,10 @@ int nb_mesh_create(const double *xyz, int32_t n_nodes, const int32_t *tri, int32
  * warped): quad[n_quad][4] with corners ordered around the element, the
  * normal dx/dxi x dx/deta outwards.  Collocation with constant fields
  * (nb_assemble with NB_COLLOCATION, nb_assemble_bm; nb_assemble_sweep runs
- * one Burton-Miller assembly per wavenumber); the Galerkin formalism and
- * sweep autotuning reject these meshes (NB_ERR_UNSUPPORTED).
+ * one Burton-Miller assembly per wavenumber), the gauss field
+ * (NB_COLLOCATION_GAUSS) and Galerkin (NB_GALERKIN, NB_GALERKIN_LINEAR with
+ * the four corner nodes as dofs); sweep autotuning rejects these meshes
+ * (NB_ERR_UNSUPPORTED).
  * Replaces create_mesh(nodes, elements, quad_1_tag()) and the quad_1 paths
  * of the collocational double integral: nihu/core/double_integral.hpp:599-673,
  * nihu/library/guiggiani_1992.hpp:138-330 (hypersingular diagonal),

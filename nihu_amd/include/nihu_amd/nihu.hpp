@@ -17,10 +17,10 @@
 // type with data(), rows() and cols() (Eigen::Matrix qualifies) holding
 // std::complex<double>, std::complex<float>, double or float, on the host, or
 // a nihu_amd::device_matrix on the GPU.  Implemented formalisms: collocation
-// (dirac(constant_view) test, constant_view trial) and Galerkin
-// (constant_view x constant_view, isoparametric_view x isoparametric_view,
-// dofs = elements or mesh nodes); other test / trial combinations throw
-// std::logic_error.  Errors map to NiHu's exception types:
+// (dirac(constant_view) test, constant_view trial; the quadrilateral gauss
+// field) and Galerkin (constant_view x constant_view, isoparametric_view x
+// isoparametric_view, dofs = elements or mesh nodes, on meshes of triangles or
+// of quadrilaterals); other test / trial combinations throw std::logic_error.  Errors map to NiHu's exception types:
 // std::out_of_range (quadrature degree), std::invalid_argument,
 // std::logic_error, std::bad_alloc and std::runtime_error.
 

@@ -1,7 +1,9 @@
 // nbem_oracle.cpp -- FP64 CPU restatement of NiHu's dense BEM assembly for
 // flat triangles (collocation with constant fields, Galerkin with constant
 // and linear isoparametric fields) and bilinear quadrilaterals (collocation
-// with constant fields), 3D Laplace and Helmholtz SLP / DLP / DLPt / HSP.
+// with constant fields and with the gauss field, Galerkin with constant and
+// bilinear isoparametric fields), 3D Laplace and Helmholtz SLP / DLP / DLPt /
+// HSP, real and complex wavenumbers.
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the checker of the HIP engine in
 // nihu_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
@@ -30,7 +32,13 @@
 //     the near-singular methods (polar with the linearised element, Telles),
 //     the planar part equal to the sum over two golden-pinned triangles,
 //     inverse mapping round trips, Gauss's identity and a transparent
-//     point-source answer on a cubed sphere.
+//     point-source answer on a cubed sphere;
+//   * for Galerkin on quadrilaterals: the closed-form double integral of 1/r
+//     over the unit square, a semi-analytic value for squares sharing an
+//     edge or a corner (closed-form inner integral, graded outer rule), every
+//     cyclic relabelling of element pairs, brute-force tensor rules for
+//     regular warped pairs, the mass matrix of a square, Gauss's identity and
+//     the pulsating sphere.
 // The node-overlap scan keeps NiHu's start-corner rule, including the
 // labelling in which it misses an edge's orientation (documented in the
 // Galerkin tests).
