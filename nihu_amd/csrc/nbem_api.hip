@@ -609,10 +609,10 @@ int ensure_gal_layout(nb_mesh *m)
 		nullptr, nullptr, nullptr);
 	if (tiles <= 0) return fail(NB_ERR_RUNTIME, "Galerkin layout");
 	const int chunk = gal_chunk(tiles, n);
-	const size_t nl = (size_t)tiles * nbk::GAL_TILE;
-	std::vector<int32_t> order(n), elem(nl), rows(nl), col((size_t)(npe + 1) * n),
+	const size_t nl = (size_t)tiles * nbk::GAL_TILE, rpl = npe == 4 ? 2 : 1;
+	std::vector<int32_t> order(n), elem(nl), rows(rpl * nl), col((size_t)(npe + 1) * n),
 		fin((size_t)((n + chunk - 1) / chunk) * cs);
-	std::vector<uint32_t> contrib(2 * nl);
+	std::vector<uint32_t> contrib(2 * rpl * nl);
 	if (nbg_galerkin_layout_n(n, m->tri.data(), npe, cs, m->c.data(), chunk, order.data(), elem.data(),
 		contrib.data(), rows.data(), col.data(), fin.data()) != tiles)
 		return fail(NB_ERR_RUNTIME, "Galerkin layout");
@@ -622,7 +622,7 @@ int ensure_gal_layout(nb_mesh *m)
 		return s;
 	m->g_tiles = tiles;
 	m->g_chunk = chunk;
-	m->bytes += (int64_t)(order.size() + 4 * nl + col.size() + fin.size()) * 4;
+	m->bytes += (int64_t)(order.size() + elem.size() + rows.size() + contrib.size() + col.size() + fin.size()) * 4;
 	return NB_OK;
 }
 

@@ -41,8 +41,8 @@ struct GalQArgs {
 	int tn, sn;                  // element counts of test / trial mesh (SoA strides)
 	int same_mesh;
 	const int *t_elem;           // nodal: [n_tiles][64] test element of each lane (-1: idle)
-	const uint2 *t_contrib;      // nodal: [n_tiles][64] contributors lane * 4 + corner of each local row
-	const int *t_rows;           // nodal: node of each local row (-1: unused)
+	const uint2 *t_contrib;      // nodal: [n_tiles][128] contributors lane * 4 + corner of local row r (lane r % 64)
+	const int *t_rows;           // nodal: [n_tiles][128] node of each local row (-1: unused)
 	const int *t_quad;           // test element nodes [4][tn]
 	const int *s_order;          // nodal: trial element at each sequence position
 	const int *s_col;            // nodal: [sn][5] previous slot occupant of corners 0..3, packed slots
@@ -184,28 +184,35 @@ __device__ __forceinline__ float2 *galq_acc(float2 *acc, int slot, int corner, i
 	return acc + (slot * 4 + corner) * GAL_TILE + lane;
 }
 
-// add one column slot to the result and clear it: lane l sums local row l
-// over its contributors, then every lane clears its own four entries
-__device__ __forceinline__ void galq_flush(float2 *acc, int lane, int slot, int col, int rnode, uint2 con,
-	const GalQArgs &a)
+// add one column slot to the result and clear it: lane l sums its local rows
+// l and l + 64 over their contributors, then every lane clears its own four
+// entries
+__device__ __forceinline__ void galq_flush(float2 *acc, int lane, int slot, int col, const int (&rnode)[2],
+	const uint2 (&con)[2], const GalQArgs &a)
 {
-	float2 v = make_float2(0.0f, 0.0f);
+	float2 v[2];
 #pragma unroll
-	for (int q = 0; q < 8; ++q) {
-		const unsigned c = ((q < 4 ? con.x : con.y) >> (8 * (q & 3))) & 255u;
-		if (c != 255u) {
-			const float2 t = *galq_acc(acc, slot, c & 3u, c >> 2);
-			v.x += t.x;
-			v.y += t.y;
+	for (int r = 0; r < 2; ++r) {
+		v[r] = make_float2(0.0f, 0.0f);
+#pragma unroll
+		for (int q = 0; q < 8; ++q) {
+			const unsigned c = ((q < 4 ? con[r].x : con[r].y) >> (8 * (q & 3))) & 255u;
+			if (c != 255u) {
+				const float2 t = *galq_acc(acc, slot, c & 3u, c >> 2);
+				v[r].x += t.x;
+				v[r].y += t.y;
+			}
 		}
 	}
 #pragma unroll
 	for (int c = 0; c < 4; ++c) *galq_acc(acc, slot, c, lane) = make_float2(0.0f, 0.0f);
-	if (rnode >= 0 && (v.x != 0.0f || v.y != 0.0f)) {
-		const float re = a.alpha.x * v.x - a.alpha.y * v.y;
-		const float im = a.alpha.x * v.y + a.alpha.y * v.x;
-		atomic_add_entry(a.out, (long long)col * a.ld + rnode, a.dtype, re, im);
-	}
+#pragma unroll
+	for (int r = 0; r < 2; ++r)
+		if (rnode[r] >= 0 && (v[r].x != 0.0f || v[r].y != 0.0f)) {
+			const float re = a.alpha.x * v[r].x - a.alpha.y * v[r].y;
+			const float im = a.alpha.x * v[r].y + a.alpha.y * v[r].x;
+			atomic_add_entry(a.out, (long long)col * a.ld + rnode[r], a.dtype, re, im);
+		}
 }
 
 // Grid: (test tiles / 4) x chunks, 256 lanes (4 waves, one test tile each).
@@ -231,11 +238,14 @@ __global__ void __launch_bounds__(256) galq_regular_kernel(GalQArgs a)
 	if (a.same_mesh)
 #pragma unroll
 		for (int c = 0; c < 4; ++c) tv[c] = a.t_quad[c * a.tn + ee];
-	int rnode = -1;
-	uint2 con = make_uint2(0xffffffffu, 0xffffffffu);
+	int rnode[2] = {-1, -1};
+	uint2 con[2] = {make_uint2(0xffffffffu, 0xffffffffu), make_uint2(0xffffffffu, 0xffffffffu)};
 	if (ND == 4) {
-		con = a.t_contrib[tile * GAL_TILE + lane];
-		rnode = a.t_rows[tile * GAL_TILE + lane];
+#pragma unroll
+		for (int r = 0; r < 2; ++r) {
+			con[r] = a.t_contrib[(tile * 2 + r) * GAL_TILE + lane];
+			rnode[r] = a.t_rows[(tile * 2 + r) * GAL_TILE + lane];
+		}
 #pragma unroll
 		for (int s = 0; s < GALQ_CS; ++s)
 #pragma unroll

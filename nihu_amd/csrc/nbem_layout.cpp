@@ -43,10 +43,10 @@ struct Tiles {
 };
 
 // greedy wave tiles over an element order: up to TILE elements whose npe
-// corners use at most RMAX distinct nodes, no node in more than KMAX of them.
-// Row r of a tile lists its contributors (lane * 4 + corner) in 8 bytes,
-// 0xff = none.
-Tiles make_tiles(const std::vector<int> &order, const int32_t *tri, int npe)
+// corners use at most rpl * RMAX distinct nodes (rpl local rows per lane),
+// no node in more than KMAX of them.  Local row r of a tile (lane r % 64 owns
+// it) lists its contributors (lane * 4 + corner) in 8 bytes, 0xff = none.
+Tiles make_tiles(const std::vector<int> &order, const int32_t *tri, int npe, int rpl)
 {
 	Tiles t;
 	const int n = (int)order.size();
@@ -57,7 +57,10 @@ Tiles make_tiles(const std::vector<int> &order, const int32_t *tri, int npe)
 		rows.clear();
 		con.clear();
 		int cnt = 0;
-		while (pos + cnt < n && cnt < TILE) {
+		// a contributor byte lane * 4 + corner must not read 255 (none): with
+		// four corners lane 63 stays idle
+		const int max_elem = npe == 4 ? TILE - 1 : TILE;
+		while (pos + cnt < n && cnt < max_elem) {
 			const int e = order[pos + cnt];
 			int l3[4];
 			int extra = 0;
@@ -69,7 +72,7 @@ Tiles make_tiles(const std::vector<int> &order, const int32_t *tri, int npe)
 				l3[a] = (int)(it - rows.begin());
 				if ((int)con[l3[a]].size() >= KMAX) full = true;
 			}
-			if (full || (int)rows.size() + extra > RMAX) break;
+			if (full || (int)rows.size() + extra > rpl * RMAX) break;
 			for (int a = 0; a < npe; ++a) {
 				if (l3[a] < 0) {
 					l3[a] = (int)rows.size();
@@ -80,8 +83,8 @@ Tiles make_tiles(const std::vector<int> &order, const int32_t *tri, int npe)
 			}
 			++cnt;
 		}
-		for (int l = 0; l < TILE; ++l) {
-			t.elem.push_back(l < cnt ? order[pos + l] : -1);
+		for (int l = 0; l < TILE; ++l) t.elem.push_back(l < cnt ? order[pos + l] : -1);
+		for (int l = 0; l < rpl * TILE; ++l) {
 			t.rows.push_back(l < (int)rows.size() ? rows[l] : -1);
 			uint32_t w[2] = {0xffffffffu, 0xffffffffu};
 			if (l < (int)rows.size())
@@ -368,10 +371,16 @@ int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, 
 // with cs column-node slots per lane.  Picks the element order (the mesh's
 // own, or Morton order of the centres when that packs the wave tiles
 // better), builds the wave tiles and the column-slot schedule of the trial
-// sequence in chunks of `chunk` elements.  Call with null arrays to get
-// n_tiles; then with arrays of n_tiles * 64 (elem, rows), n_tiles * 128
-// (contrib), n (order), (npe + 1) n (col) and ceil(n / chunk) * cs (fin).
-// Returns -1 for npe outside 3..4 or cs outside npe..8.
+// sequence in chunks of `chunk` elements.  A tile holds up to 64 elements
+// (63 quadrilaterals: the contributor byte of lane 63, corner 3 would read as
+// the empty marker 255);
+// triangles use up to 64 distinct nodes (one local row per lane),
+// quadrilaterals up to 128 (two local rows per lane: 64 quadrilaterals touch
+// about 81 nodes).  Call with null arrays to get n_tiles; then with arrays of
+// n_tiles * 64 (elem), n_tiles * 64 * rpl (rows), n_tiles * 128 * rpl
+// (contrib), n (order), (npe + 1) n (col) and ceil(n / chunk) * cs (fin),
+// rpl = 1 for triangles and 2 for quadrilaterals.  Returns -1 for npe
+// outside 3..4 or cs outside npe..8.
 int nbg_galerkin_layout_n(int n, const int32_t *tri, int npe, int cs, const double *c, int chunk,
 	int32_t *order_out, int32_t *elem, uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin)
 {
@@ -396,7 +405,8 @@ int nbg_galerkin_layout_n(int n, const int32_t *tri, int npe, int cs, const doub
 	std::stable_sort(key.begin(), key.end(),
 		[](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) { return a.first < b.first; });
 	for (int e = 0; e < n; ++e) morton[e] = key[e].second;
-	Tiles t0 = make_tiles(mesh_order, tri, npe), t1 = make_tiles(morton, tri, npe);
+	const int rpl = npe == 4 ? 2 : 1;
+	Tiles t0 = make_tiles(mesh_order, tri, npe, rpl), t1 = make_tiles(morton, tri, npe, rpl);
 	const int64_t m0 = schedule(mesh_order, tri, npe, cs, chunk, nullptr, nullptr);
 	const int64_t m1 = schedule(morton, tri, npe, cs, chunk, nullptr, nullptr);
 	const bool use_morton = t1.n_tiles < t0.n_tiles || (t1.n_tiles == t0.n_tiles && m1 < m0);

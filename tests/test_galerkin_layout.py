@@ -208,8 +208,9 @@ def _qlayout(raw, nodes, quads, chunk):
     tiles = raw.nbg_galerkin_layout_n(n, _i(quads), 4, QCS, c.ctypes.data_as(D), chunk, None, None, None, None,
                                       None, None)
     order = np.zeros(n, np.int32)
-    elem, rows = (np.zeros(tiles * TILE, np.int32) for _ in range(2))
-    contrib = np.zeros(tiles * TILE * 2, np.uint32)
+    elem = np.zeros(tiles * TILE, np.int32)
+    rows = np.zeros(tiles * TILE * 2, np.int32)          # two local rows per lane
+    contrib = np.zeros(tiles * TILE * 2 * 2, np.uint32)
     col = np.zeros(5 * n, np.int32)
     fin = np.zeros(((n + chunk - 1) // chunk) * QCS, np.int32)
     assert raw.nbg_galerkin_layout_n(n, _i(quads), 4, QCS, c.ctypes.data_as(D), chunk, _i(order), _i(elem),
@@ -233,13 +234,16 @@ def test_quad_wave_tiles_and_slot_schedule(raw):
         assert sorted(order.tolist()) == list(range(n))
         used = elem[elem >= 0]
         assert np.all(used == order)
+        # 64 quadrilaterals touch about 81 nodes: the two rows per lane let
+        # the tiles fill their lanes
+        assert tiles <= (n + TILE - 1) // TILE + 1
         for t in range(tiles):
-            r = rows[t * TILE:(t + 1) * TILE]
+            r = rows[2 * t * TILE:2 * (t + 1) * TILE]
             nr = int((r >= 0).sum())
             assert len(set(r[:nr].tolist())) == nr and np.all(r[nr:] == -1)
             seen = set()
-            for l in range(TILE):
-                bytes8 = [(int(contrib[t * TILE + l][q >> 2]) >> (8 * (q & 3))) & 255 for q in range(8)]
+            for l in range(2 * TILE):
+                bytes8 = [(int(contrib[2 * t * TILE + l][q >> 2]) >> (8 * (q & 3))) & 255 for q in range(8)]
                 cs = [b for b in bytes8 if b != 255]
                 for b in cs:
                     lane, a = b >> 2, b & 3
