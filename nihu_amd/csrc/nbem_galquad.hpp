@@ -165,9 +165,11 @@ __device__ __forceinline__ void galq_points(const float *tp, const float *sq_in,
 				}
 			}
 		}
+		// w J N_a of the test point (ND = 4), w J (ND = 1)
+		const float w4[4] = {t2.x, t2.y, t2.z, t2.w};
 		float wa[ND];
-		if (ND == 1) wa[0] = tw;
-		else { wa[0] = t2.x; wa[1 % ND] = t2.y; wa[2 % ND] = t2.z; wa[3 % ND] = t2.w; }
+#pragma unroll
+		for (int x = 0; x < ND; ++x) wa[x] = ND == 1 ? tw : w4[x];
 #pragma unroll
 		for (int x = 0; x < ND; ++x)
 #pragma unroll

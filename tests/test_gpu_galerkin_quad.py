@@ -143,6 +143,22 @@ def test_accumulate_host_and_complex64(nb, orc, q4, form):
 
 
 @pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
+def test_real_outputs(nb, orc, q4, form):
+    """Laplace double layer into float64 and float32 device outputs (real
+    results): float64 within the bar of the complex outputs, float32 within
+    FP32 rounding of it."""
+    nodes, quads = q4
+    gp, op = POLICIES["nihu"]
+    m = nb.Mesh(nodes, quads)
+    ref = orc.assemble_galerkin(nodes, quads, 0, 0, 1, k=0.0, alpha=-0.7, pol=orc.policy(**op),
+                                linear=FORMS[form]).real
+    f64 = _assemble(nb, m, m, form, 0, 0, 1, 0.0, -0.7, gp, dtype="float64")
+    f32 = _assemble(nb, m, m, form, 0, 0, 1, 0.0, -0.7, gp, dtype="float32")
+    assert rowscaled(f64, ref) <= ROW_TOL and relfro(f64, ref) <= FRO_TOL
+    assert rowscaled(f32.astype(np.float64), f64) <= 1e-6
+
+
+@pytest.mark.parametrize("form", ["galerkin", "galerkin_linear"])
 def test_many_tiles_sampled_rows(nb, orc, form):
     """864 quadrilaterals (866 nodes): many wave tiles and trial chunks, the
     column-slot evictions of the nodal kernel; whole rows of 12 sampled dofs
