@@ -762,15 +762,23 @@ static inline void map_corners(double xi, double eta, const int c[3], double &ox
 	oeta = N0 * cy[c[0]] + N1 * cy[c[1]] + N2 * cy[c[2]];
 }
 
-// 3x3 Galerkin block of one element pair (not yet scaled); throws like NiHu
-// linear = 1: isoparametric field (3 nodal shape functions, field complexity 1);
-// linear = 0: constant field (one shape function 1, complexity 0,
-// nihu/core/complexity_estimator.hpp:39-75)
+// the fields of a Galerkin pair: linear = 0 constant x constant, 1
+// isoparametric x isoparametric, 2 constant test x isoparametric trial, 3
+// isoparametric test x constant trial
+static inline bool linear_test(int linear) { return linear == 1 || linear == 3; }
+static inline bool linear_trial(int linear) { return linear == 1 || linear == 2; }
+
+// 3x3 Galerkin block of one element pair (not yet scaled); throws like NiHu.
+// An isoparametric field has 3 nodal shape functions (field complexity 1), a
+// constant field one shape function 1 (complexity 0); the pair's degree takes
+// the larger complexity (nihu/core/complexity_estimator.hpp:39-75).  The
+// block is [test shape][trial shape], [0][*] / [*][0] for a constant side.
 template <class WN>
 static void galerkin_block(int family, int nx, int ny, WN k, Policy const &p, Elem const &te, Elem const &se,
 	const int *tn, const int *sn, bool same_mesh, int linear, cplx blk[3][3])
 {
 	for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) blk[a][b] = 0.0;
+	const bool lt = linear_test(linear), ls = linear_trial(linear);
 	int match = -1;
 	Overlap ov = {0, 0, 0};
 	if (same_mesh) {
@@ -799,28 +807,29 @@ static void galerkin_block(int family, int nx, int ny, WN k, Policy const &p, El
 			V3 x = elem_x(te, txi, teta), y = elem_x(se, sxi, seta);
 			cplx K = kernel_eval(family, nx, ny, k, x, y, te.nu, se.nu);
 			double Nt[3] = {1.0 - txi - teta, txi, teta}, Ns[3] = {1.0 - sxi - seta, sxi, seta};
-			if (!linear) { Nt[0] = 1.0; Ns[0] = 1.0; }
-			for (int a = 0; a < (linear ? 3 : 1); ++a)
-				for (int b = 0; b < (linear ? 3 : 1); ++b)
+			if (!lt) Nt[0] = 1.0;
+			if (!ls) Ns[0] = 1.0;
+			for (int a = 0; a < (lt ? 3 : 1); ++a)
+				for (int b = 0; b < (ls ? 3 : 1); ++b)
 					blk[a][b] += (Nt[a] * (te.J * P.w)) * K * (Ns[b] * (se.J * 1.0));
 		}
 		return;
 	}
 	double d2 = dist2(te.c, se.c);
-	int degree = regular_degree(p, linear ? 1 : 0, nx, ny, d2, se);
+	int degree = regular_degree(p, (lt || ls) ? 1 : 0, nx, ny, d2, se);
 	if (degree > 14) throw std::out_of_range("Too high quadrature degree selected for collocational integration");
 	auto const &Q = dunavant_store(degree);
 	for (auto const &qp : Q) {
 		V3 x = elem_x(te, qp.xi, qp.eta);
 		double Nt[3] = {1.0 - qp.xi - qp.eta, qp.xi, qp.eta};
-		if (!linear) Nt[0] = 1.0;
+		if (!lt) Nt[0] = 1.0;
 		for (auto const &qs : Q) {
 			V3 y = elem_x(se, qs.xi, qs.eta);
 			double Ns[3] = {1.0 - qs.xi - qs.eta, qs.xi, qs.eta};
-			if (!linear) Ns[0] = 1.0;
+			if (!ls) Ns[0] = 1.0;
 			cplx K = kernel_eval(family, nx, ny, k, x, y, te.nu, se.nu);
-			for (int a = 0; a < (linear ? 3 : 1); ++a)
-				for (int b = 0; b < (linear ? 3 : 1); ++b)
+			for (int a = 0; a < (lt ? 3 : 1); ++a)
+				for (int b = 0; b < (ls ? 3 : 1); ++b)
 					blk[a][b] += (Nt[a] * (te.J * qp.w)) * K * (Ns[b] * (se.J * qs.w));
 		}
 	}
@@ -1487,7 +1496,7 @@ static void galerkin_quad_block(int family, int nx, int ny, WN k, Policy const &
 	QElem const &se, const int *tn, const int *sn, bool same_mesh, int linear, cplx blk[4][4])
 {
 	for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) blk[a][b] = 0.0;
-	const int nd = linear ? 4 : 1;
+	const bool lt = linear_test(linear), ls = linear_trial(linear);
 	// test point (N_a J w) and trial point (N_b J) of one point pair
 	auto add = [&](double txi, double teta, double sxi, double seta, double wt, double ws) {
 		const V3 x = q_x(te, txi, teta), y = q_x(se, sxi, seta);
@@ -1497,9 +1506,10 @@ static void galerkin_quad_block(int family, int nx, int ny, WN k, Policy const &
 		double Nt[4], Ns[4];
 		quad_shapes(txi, teta, Nt);
 		quad_shapes(sxi, seta, Ns);
-		if (!linear) { Nt[0] = 1.0; Ns[0] = 1.0; }
-		for (int a = 0; a < nd; ++a)
-			for (int b = 0; b < nd; ++b) blk[a][b] += (Nt[a] * (J1 * wt)) * K * (Ns[b] * (J2 * ws));
+		if (!lt) Nt[0] = 1.0;
+		if (!ls) Ns[0] = 1.0;
+		for (int a = 0; a < (lt ? 4 : 1); ++a)
+			for (int b = 0; b < (ls ? 4 : 1); ++b) blk[a][b] += (Nt[a] * (J1 * wt)) * K * (Ns[b] * (J2 * ws));
 	};
 	int match = -1;
 	Overlap ov = {0, 0, 0};
@@ -1529,7 +1539,7 @@ static void galerkin_quad_block(int family, int nx, int ny, WN k, Policy const &
 	}
 	const double d2 = dist2(te.c, se.c);
 	const int degree = p.rule == 1 ? p.fixed_degree
-		: (linear ? 2 : 1) + eval_interval(interval_table(1 + nx + ny), std::sqrt(d2) / se.R);
+		: ((lt || ls) ? 2 : 1) + eval_interval(interval_table(1 + nx + ny), std::sqrt(d2) / se.R);
 	if (degree > 14) throw std::out_of_range("Too high quadrature degree selected for collocational integration");
 	auto const &Q = gauss_quad_store(degree);
 	for (auto const &qp : Q)
@@ -2136,12 +2146,14 @@ static int assemble_galerkin_impl(const double *txyz, int tn, const int *ttri, i
 	auto trial = same_mesh ? test : make_mesh(sxyz, sn, stri, snt);
 	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
 	cplx alpha(alpha_re, alpha_im);
-	const int nd = linear ? 3 : 1;
+	const bool lt = linear_test(linear), ls = linear_trial(linear);
+	const int ndt = lt ? 3 : 1, nds = ls ? 3 : 1;
 	// dof of (element, local shape function): node id, or the element itself
-	auto tdof = [&](int e, int a) { return linear ? ttri[3 * e + a] : e; };
-	auto sdof = [&](int e, int b) { return linear ? stri[3 * e + b] : e; };
-	int nrows = linear ? tn : tnt;
-	int ncols = linear ? (same_mesh ? tn : sn) : (same_mesh ? tnt : snt);
+	if (same_mesh) stri = ttri;
+	auto tdof = [&](int e, int a) { return lt ? ttri[3 * e + a] : e; };
+	auto sdof = [&](int e, int b) { return ls ? stri[3 * e + b] : e; };
+	int nrows = lt ? tn : tnt;
+	int ncols = ls ? (same_mesh ? tn : sn) : (same_mesh ? tnt : snt);
 	if (!accumulate)
 		for (int j = 0; j < ncols; ++j)
 			for (int i = 0; i < nrows; ++i) { out[2 * ((int64_t)j * ld + i)] = 0; out[2 * ((int64_t)j * ld + i) + 1] = 0; }
@@ -2149,18 +2161,19 @@ static int assemble_galerkin_impl(const double *txyz, int tn, const int *ttri, i
 		// identity: single integral of N_a N_b J with the Dunavant degree
 		// test order + trial order + jacobian order
 		if (!same_mesh) return 0;
-		auto const &qm = dunavant_store(linear ? 2 : 0);
+		auto const &qm = dunavant_store((lt ? 1 : 0) + (ls ? 1 : 0));
 		for (int e = 0; e < tnt; ++e) {
-			for (int a = 0; a < nd; ++a)
-				for (int b = 0; b < nd; ++b) {
+			for (int a = 0; a < ndt; ++a)
+				for (int b = 0; b < nds; ++b) {
 					double m = 0.0;
 					for (auto const &q : qm) {
-						double N[3] = {1.0 - q.xi - q.eta, q.xi, q.eta};
-						if (!linear) N[0] = 1.0;
-						m += N[a] * N[b] * test[e].J * q.w;
+						double Nt[3] = {1.0 - q.xi - q.eta, q.xi, q.eta}, Ns[3] = {Nt[0], Nt[1], Nt[2]};
+						if (!lt) Nt[0] = 1.0;
+						if (!ls) Ns[0] = 1.0;
+						m += Nt[a] * Ns[b] * test[e].J * q.w;
 					}
 					cplx v = alpha * m;
-					double *o = out + 2 * ((int64_t)tdof(e, b) * ld + tdof(e, a));
+					double *o = out + 2 * ((int64_t)sdof(e, b) * ld + tdof(e, a));
 					o[0] += v.real(); o[1] += v.imag();
 				}
 		}
@@ -2180,8 +2193,8 @@ static int assemble_galerkin_impl(const double *txyz, int tn, const int *ttri, i
 				cplx blk[3][3];
 				galerkin_block(family, nx, ny, k, p, test[i], trial[j], ttri + 3 * i, stri + 3 * j, same_mesh != 0,
 					linear, blk);
-				for (int a = 0; a < nd; ++a)
-					for (int b = 0; b < nd; ++b) {
+				for (int a = 0; a < ndt; ++a)
+					for (int b = 0; b < nds; ++b) {
 						cplx v = alpha * blk[a][b];
 						double *o = out + 2 * ((int64_t)sdof(j, b) * ld + tdof(i, a));
 						#pragma omp atomic
@@ -2673,28 +2686,31 @@ static int assemble_galerkin_quad_impl(const double *txyz, int tn, const int *tq
 	if (same_mesh) { sn = tn; sq = tq; snt = tnt; }
 	Policy p = {pol->rule, pol->fixed_degree, pol->near_mode, pol->near_limit};
 	const cplx alpha(alpha_re, alpha_im);
-	const int nd = linear ? 4 : 1;
-	auto tdof = [&](int e, int a) { return linear ? tq[4 * e + a] : e; };
-	auto sdof = [&](int e, int b) { return linear ? sq[4 * e + b] : e; };
-	const int nrows = linear ? tn : tnt, ncols = linear ? sn : snt;
+	const bool lt = linear_test(linear), ls = linear_trial(linear);
+	const int ndt = lt ? 4 : 1, nds = ls ? 4 : 1;
+	auto tdof = [&](int e, int a) { return lt ? tq[4 * e + a] : e; };
+	auto sdof = [&](int e, int b) { return ls ? sq[4 * e + b] : e; };
+	const int nrows = lt ? tn : tnt, ncols = ls ? sn : snt;
 	if (!accumulate)
 		for (int j = 0; j < ncols; ++j)
 			for (int i = 0; i < nrows; ++i) { out[2 * ((int64_t)j * ld + i)] = 0; out[2 * ((int64_t)j * ld + i) + 1] = 0; }
 	if (family == 2) {
 		if (!same_mesh) return 0;
-		auto const &qm = gauss_quad_store(linear ? 3 : 1);
+		auto const &qm = gauss_quad_store((lt ? 1 : 0) + (ls ? 1 : 0) + 1);
 		for (int e = 0; e < tnt; ++e)
-			for (int a = 0; a < nd; ++a)
-				for (int b = 0; b < nd; ++b) {
+			for (int a = 0; a < ndt; ++a)
+				for (int b = 0; b < nds; ++b) {
 					double m = 0.0;
 					for (auto const &q : qm) {
-						double N[4];
-						quad_shapes(q.xi, q.eta, N);
-						if (!linear) N[0] = 1.0;
-						m += N[a] * N[b] * norm(q_normal(test[e], q.xi, q.eta)) * q.w;
+						double Nt[4], Ns[4];
+						quad_shapes(q.xi, q.eta, Nt);
+						quad_shapes(q.xi, q.eta, Ns);
+						if (!lt) Nt[0] = 1.0;
+						if (!ls) Ns[0] = 1.0;
+						m += Nt[a] * Ns[b] * norm(q_normal(test[e], q.xi, q.eta)) * q.w;
 					}
 					const cplx v = alpha * m;
-					double *o = out + 2 * ((int64_t)tdof(e, b) * ld + tdof(e, a));
+					double *o = out + 2 * ((int64_t)sdof(e, b) * ld + tdof(e, a));
 					o[0] += v.real(); o[1] += v.imag();
 				}
 		return 0;
@@ -2711,8 +2727,8 @@ static int assemble_galerkin_quad_impl(const double *txyz, int tn, const int *tq
 				cplx blk[4][4];
 				galerkin_quad_block(family, nx, ny, k, p, test[i], trial[j], tq + 4 * i, sq + 4 * j, same_mesh != 0,
 					linear, blk);
-				for (int a = 0; a < nd; ++a)
-					for (int b = 0; b < nd; ++b) {
+				for (int a = 0; a < ndt; ++a)
+					for (int b = 0; b < nds; ++b) {
 						const cplx v = alpha * blk[a][b];
 						double *o = out + 2 * ((int64_t)sdof(j, b) * ld + tdof(i, a));
 						#pragma omp atomic

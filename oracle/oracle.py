@@ -334,8 +334,13 @@ def assemble_galerkin(nodes, tris, family, nx, ny, k=0.0, alpha=1.0, pol=None, t
     t_nodes, t_tris = _mesh(nodes, tris)
     same = trial is None
     s_nodes, s_tris = (t_nodes, t_tris) if same else _mesh(*trial)
+    # linear: True / 1 isoparametric x isoparametric, False / 0 constant x
+    # constant, 2 constant test x isoparametric trial, 3 isoparametric test x
+    # constant trial
+    linear = int(linear)
+    lt, ls = linear in (1, 3), linear in (1, 2)
     if out is None:
-        shape = (len(t_nodes), len(s_nodes)) if linear else (len(t_tris), len(s_tris))
+        shape = (len(t_nodes) if lt else len(t_tris), len(s_nodes) if ls else len(s_tris))
         out = np.zeros(shape, dtype=np.complex128, order="F")
         accumulate = False
     alpha = complex(alpha)

@@ -270,3 +270,45 @@ def test_other_mesh_has_no_singular_check(orc):
     far = (nodes + np.array([0.0, 0.0, 2.0]), tris)
     M = orc.assemble_galerkin(nodes, tris, 1, 1, 1, k=1.0, trial=far)
     assert M.shape == (4, 4) and np.all(np.isfinite(M))
+
+
+# --------------------------------------------------- mixed field types
+
+@pytest.mark.parametrize("kind", ["tri", "quad"])
+def test_mixed_fields(orc, kind):
+    """Constant test x isoparametric trial (linear = 2) and the reverse (3):
+    a constant-test row is the nodal rows of its element summed over the
+    element's nodes (partition of unity, same degree: the larger field
+    complexity); the single layer's two mixed matrices are transposes (a
+    symmetric kernel) to the singular rules' error (the edge rules are not
+    symmetric, and NiHu's double-wrap labelling misses the edge); the mass matrix of a
+    constant test against a nodal trial is the integral of N_b."""
+    from nihu_amd.mesh import icosphere, cubed_sphere
+    nodes, el = icosphere(1) if kind == "tri" else cubed_sphere(2)
+    npe = el.shape[1]
+    for fam, nx, ny, k in ((1, 0, 0, 2.0), (1, 0, 1, 2.0), (0, 1, 0, 0.0)):
+        CL = orc.assemble_galerkin(nodes, el, fam, nx, ny, k=k, linear=2)
+        LC = orc.assemble_galerkin(nodes, el, fam, nx, ny, k=k, linear=3)
+        assert CL.shape == (len(el), len(nodes)) and LC.shape == (len(nodes), len(el))
+        for e in (0, 7, len(el) - 1):
+            LL = orc.assemble_galerkin(nodes, el, fam, nx, ny, k=k, linear=1, test_elems=[e])
+            row = LL[el[e]].sum(0)
+            assert np.abs(CL[e] - row).max() <= 1e-13 * np.abs(row).max(), (kind, fam, nx, ny, e)
+        if nx == ny == 0:
+            assert np.abs(LC - CL.T).max() <= 1e-3 * np.abs(CL).max()
+    M = orc.assemble_galerkin(nodes, el, 2, 0, 0, linear=2).real
+    for e, q in enumerate(el):
+        X = nodes[q]
+        if npe == 3:
+            J = np.linalg.norm(np.cross(X[1] - X[0], X[2] - X[0]))
+            ref = J / 6.0
+            assert np.allclose(M[e, q], ref, rtol=1e-14, atol=0)
+        else:
+            g, w = np.polynomial.legendre.leggauss(2)
+            XI, ETA = np.meshgrid(g, g, indexing="ij")
+            xi, eta = XI.ravel(), ETA.ravel()
+            N = np.stack([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)], -1) / 4
+            J = np.linalg.norm(bemref.quad_geom(X, xi, eta)[1], axis=1)
+            ref = (N * (J * np.outer(w, w).ravel())[:, None]).sum(0)
+            assert np.allclose(M[e, q], ref, rtol=1e-13, atol=0)
+        assert np.count_nonzero(M[e]) == npe
