@@ -94,8 +94,14 @@ enum nb_location { NB_HOST = 0, NB_DEVICE = 1 };
  * dirac(constant_view(field)) of bem.cpp:233) against the trial mesh's
  * gauss field, on different meshes.  NB_IDENTITY with NB_COLLOCATION_GAUSS on
  * the same mesh: the unit matrix over the dofs. */
+/* NB_GALERKIN_CONST_LINEAR: constant_view test x isoparametric_view trial
+ * (rows = test elements, columns = trial nodes); NB_GALERKIN_LINEAR_CONST:
+ * the reverse (rows = test nodes, columns = trial elements).  NiHu's general
+ * formalism with two field types: the pair's degree takes the larger field
+ * complexity (nihu/core/complexity_estimator.hpp:39-75), the singular rules
+ * are those of the element types (nihu/core/match_types.hpp:54-83). */
 enum nb_formalism { NB_COLLOCATION = 0, NB_GALERKIN = 1, NB_GALERKIN_LINEAR = 2, NB_COLLOCATION_GAUSS = 3,
-                    NB_COLLOCATION_GAUSS_TRIAL = 4 };
+                    NB_COLLOCATION_GAUSS_TRIAL = 4, NB_GALERKIN_CONST_LINEAR = 5, NB_GALERKIN_LINEAR_CONST = 6 };
 
 typedef struct nb_mesh nb_mesh;
 

@@ -334,8 +334,13 @@ struct nb_mesh {
 	std::vector<int32_t> tri;                  // element-major node ids (host)
 	DevPtr d_tri;                              // node ids [npe][n] (quadrilaterals: uploaded on first use)
 	// Galerkin, nodal dofs: element order, wave tiles, column-slot schedule
-	int g_tiles = 0, g_chunk = 0;
-	DevPtr g_order, g_elem, g_contrib, g_rows, g_col, g_fin;
+	// (glay[0]: node dofs; glay[1]: element dofs, the dof map of a constant
+	// field in a pair with a nodal one)
+	struct GalLayout {
+		int tiles = 0, chunk = 0;
+		DevPtr order, elem, contrib, rows, col, fin;
+	} glay[2];
+	DevPtr d_edof;                             // element dof map [npe][n]: e at every corner
 	// Galerkin singular pairs against itself (coincident pairs first)
 	int64_t sp_n = -1;
 	DevPtr sp_e, sp_f, sp_info;
@@ -600,12 +605,39 @@ int gal_chunk(int n_tiles, int sn)
 
 // the nodal-dof layout of a mesh of triangles (GAL_CS slots per lane) or
 // quadrilaterals (GALQ_CS slots)
-int ensure_gal_layout(nb_mesh *m)
+// dof map of a mesh's corners: kind 0 the nodes, kind 1 the element itself
+// (a constant field: the nodal kernels' sums over the corners then give the
+// constant field's integrals, as the shape functions sum to one)
+std::vector<int32_t> dof_map(const nb_mesh *m, int kind)
 {
-	if (m->g_tiles > 0) return NB_OK;
+	if (kind == 0) return m->tri;
+	std::vector<int32_t> d((size_t)m->npe * m->n);
+	for (int e = 0; e < m->n; ++e)
+		for (int a = 0; a < m->npe; ++a) d[(size_t)m->npe * e + a] = e;
+	return d;
+}
+
+// the element dof map on the device, [npe][n]
+int ensure_edof(nb_mesh *m)
+{
+	if (m->d_edof) return NB_OK;
+	std::vector<int32_t> soa((size_t)m->npe * m->n);
+	for (int e = 0; e < m->n; ++e)
+		for (int a = 0; a < m->npe; ++a) soa[(size_t)a * m->n + e] = e;
+	int s = upload(m->d_edof, soa);
+	if (s) return s;
+	m->bytes += (int64_t)soa.size() * 4;
+	return NB_OK;
+}
+
+int ensure_gal_layout(nb_mesh *m, int kind = 0)
+{
+	nb_mesh::GalLayout &L = m->glay[kind];
+	if (L.tiles > 0) return NB_OK;
 	const int n = m->n, npe = m->npe;
+	const std::vector<int32_t> dm = dof_map(m, kind);
 	const int cs = npe == 4 ? nbk::GALQ_CS : nbk::GAL_CS;
-	const int tiles = nbg_galerkin_layout_n(n, m->tri.data(), npe, cs, m->c.data(), 512, nullptr, nullptr, nullptr,
+	const int tiles = nbg_galerkin_layout_n(n, dm.data(), npe, cs, m->c.data(), 512, nullptr, nullptr, nullptr,
 		nullptr, nullptr, nullptr);
 	if (tiles <= 0) return fail(NB_ERR_RUNTIME, "Galerkin layout");
 	const int chunk = gal_chunk(tiles, n);
@@ -613,15 +645,15 @@ int ensure_gal_layout(nb_mesh *m)
 	std::vector<int32_t> order(n), elem(nl), rows(rpl * nl), col((size_t)(npe + 1) * n),
 		fin((size_t)((n + chunk - 1) / chunk) * cs);
 	std::vector<uint32_t> contrib(2 * rpl * nl);
-	if (nbg_galerkin_layout_n(n, m->tri.data(), npe, cs, m->c.data(), chunk, order.data(), elem.data(),
+	if (nbg_galerkin_layout_n(n, dm.data(), npe, cs, m->c.data(), chunk, order.data(), elem.data(),
 		contrib.data(), rows.data(), col.data(), fin.data()) != tiles)
 		return fail(NB_ERR_RUNTIME, "Galerkin layout");
 	int s;
-	if ((s = upload(m->g_order, order)) || (s = upload(m->g_elem, elem)) || (s = upload(m->g_contrib, contrib)) ||
-		(s = upload(m->g_rows, rows)) || (s = upload(m->g_col, col)) || (s = upload(m->g_fin, fin)))
+	if ((s = upload(L.order, order)) || (s = upload(L.elem, elem)) || (s = upload(L.contrib, contrib)) ||
+		(s = upload(L.rows, rows)) || (s = upload(L.col, col)) || (s = upload(L.fin, fin)))
 		return s;
-	m->g_tiles = tiles;
-	m->g_chunk = chunk;
+	L.tiles = tiles;
+	L.chunk = chunk;
 	m->bytes += (int64_t)(order.size() + elem.size() + rows.size() + contrib.size() + col.size() + fin.size()) * 4;
 	return NB_OK;
 }
@@ -730,9 +762,13 @@ int run_galerkin_quad(const Job &jb, void *out, int64_t ld, int dtype, int accum
 {
 	nb_mesh *trial = jb.trial;
 	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
-	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
+	// the nodal kernels serve every pair with an isoparametric side; a
+	// constant side runs through its element dof map
+	const bool lt = jb.formalism == NB_GALERKIN_LINEAR || jb.formalism == NB_GALERKIN_LINEAR_CONST;
+	const bool ls = jb.formalism == NB_GALERKIN_LINEAR || jb.formalism == NB_GALERKIN_CONST_LINEAR;
+	const bool nodal = lt || ls;
 	const bool ck = jb.ki != 0.0 && jb.family == NB_HELMHOLTZ;
-	const int64_t rows = nodal ? test->n_nodes : test->n, cols = nodal ? trial->n_nodes : trial->n;
+	const int64_t rows = lt ? test->n_nodes : test->n, cols = ls ? trial->n_nodes : trial->n;
 	const int eb = dtype_bytes(dtype);
 	const double four_pi = 4.0 * 3.141592653589793238462643383279502884;
 	auto zero = [&]() -> int {
@@ -742,12 +778,15 @@ int run_galerkin_quad(const Job &jb, void *out, int64_t ld, int dtype, int accum
 	};
 	int s;
 	if ((s = ensure_elem_nodes(test)) || (s = ensure_elem_nodes(trial))) return s;
+	if ((!lt && nodal && (s = ensure_edof(test))) || (!ls && nodal && (s = ensure_edof(trial)))) return s;
+	const int *t_map = (const int *)(lt ? test->d_tri.get() : test->d_edof.get());
+	const int *s_map = (const int *)(ls ? trial->d_tri.get() : trial->d_edof.get());
 	if (jb.family == NB_IDENTITY) {
 		if (!accumulate && (s = zero())) return s;
 		// local operator: nothing on different meshes (nihu/core/assembly.hpp:141-142)
 		if (jb.same)
 			hipLaunchKernelGGL(nbk::galq_identity_kernel, dim3((test->n + 255) / 256), dim3(256), 0, st, out,
-				(long long)ld, test->n, (const double *)test->d_X.get(), (const int *)test->d_tri.get(),
+				(long long)ld, test->n, (const double *)test->d_X.get(), t_map, s_map,
 				nodal ? 1 : 0, dtype, jb.alpha.re, jb.alpha.im);
 		NB_HIP(hipGetLastError());
 		return NB_OK;
@@ -771,21 +810,22 @@ int run_galerkin_quad(const Job &jb, void *out, int64_t ld, int dtype, int accum
 	}
 	for (int N : ns)
 		if ((s = ensure_gq(trial, N)) || (s = ensure_gq(test, N))) return s;
-	if (nodal && ((s = ensure_gal_layout(test)) || (s = ensure_gal_layout(trial)))) return s;
+	const int kt = lt ? 0 : 1, ks = ls ? 0 : 1;   // layout kinds: node or element dofs
+	if (nodal && ((s = ensure_gal_layout(test, kt)) || (s = ensure_gal_layout(trial, ks)))) return s;
 	if (jb.same && (s = ensure_sing_pairs(trial))) return s;
 	if (nodal && !accumulate && (s = zero())) return s;
 
 	nbk::GalQArgs a;
 	std::memset(&a, 0, sizeof(a));
 	if (nodal) {
-		a.n_tiles = test->g_tiles;
-		a.chunk = trial->g_chunk;
-		a.t_elem = (const int *)test->g_elem.get();
-		a.t_contrib = (const uint2 *)test->g_contrib.get();
-		a.t_rows = (const int *)test->g_rows.get();
-		a.s_order = (const int *)trial->g_order.get();
-		a.s_col = (const int *)trial->g_col.get();
-		a.s_fin = (const int *)trial->g_fin.get();
+		a.n_tiles = test->glay[kt].tiles;
+		a.chunk = trial->glay[ks].chunk;
+		a.t_elem = (const int *)test->glay[kt].elem.get();
+		a.t_contrib = (const uint2 *)test->glay[kt].contrib.get();
+		a.t_rows = (const int *)test->glay[kt].rows.get();
+		a.s_order = (const int *)trial->glay[ks].order.get();
+		a.s_col = (const int *)trial->glay[ks].col.get();
+		a.s_fin = (const int *)trial->glay[ks].fin.get();
 	} else {
 		a.n_tiles = (test->n + nbk::GAL_TILE - 1) / nbk::GAL_TILE;
 		a.chunk = gal_chunk(a.n_tiles, trial->n);
@@ -853,7 +893,8 @@ int run_galerkin_quad(const Job &jb, void *out, int64_t ld, int dtype, int accum
 	g.pinfo = (const int *)trial->sp_info.get();
 	g.n = trial->n;
 	g.X = (const double *)trial->d_X.get();
-	g.quad = (const int *)trial->d_tri.get();
+	g.t_dof = t_map;
+	g.s_dof = s_map;
 	for (int m = 0; m < 3; ++m) { g.rule[m] = (const double *)sr->r[m].get(); g.rule_n[m] = sr->n[m]; }
 	g.k = jb.k;
 	g.ki = jb.ki;
@@ -896,9 +937,13 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	if (jb.trial->npe == 4) return run_galerkin_quad(jb, out, ld, dtype, accumulate, st);
 	nb_mesh *trial = jb.trial;
 	nb_mesh *test = const_cast<nb_mesh *>(jb.test);
-	const bool nodal = jb.formalism == NB_GALERKIN_LINEAR;
+	// the nodal kernels serve every pair with an isoparametric side; a
+	// constant side runs through its element dof map
+	const bool lt = jb.formalism == NB_GALERKIN_LINEAR || jb.formalism == NB_GALERKIN_LINEAR_CONST;
+	const bool ls = jb.formalism == NB_GALERKIN_LINEAR || jb.formalism == NB_GALERKIN_CONST_LINEAR;
+	const bool nodal = lt || ls;
 	const bool ck = jb.ki != 0.0 && jb.family == NB_HELMHOLTZ;
-	const int64_t rows = nodal ? test->n_nodes : test->n, cols = nodal ? trial->n_nodes : trial->n;
+	const int64_t rows = lt ? test->n_nodes : test->n, cols = ls ? trial->n_nodes : trial->n;
 	const int eb = dtype_bytes(dtype);
 	auto zero = [&]() -> int {
 		launch_zero(out, ld, rows, cols, eb, st);
@@ -906,12 +951,15 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 		return NB_OK;
 	};
 	int s;
+	if ((!lt && nodal && (s = ensure_edof(test))) || (!ls && nodal && (s = ensure_edof(trial)))) return s;
+	const int *t_map = (const int *)(lt ? test->d_tri.get() : test->d_edof.get());
+	const int *s_map = (const int *)(ls ? trial->d_tri.get() : trial->d_edof.get());
 	if (jb.family == NB_IDENTITY) {
 		if (!accumulate && (s = zero())) return s;
 		// local operator: nothing on different meshes (nihu/core/assembly.hpp:141-142)
 		if (jb.same)
 			hipLaunchKernelGGL(nbk::galerkin_identity_kernel, dim3((test->n + 255) / 256), dim3(256), 0, st, out,
-				(long long)ld, test->n, (const double *)test->d_J.get(), (const int *)test->d_tri.get(), nodal ? 1 : 0,
+				(long long)ld, test->n, (const double *)test->d_J.get(), t_map, s_map, nodal ? 1 : 0,
 				dtype, jb.alpha.re, jb.alpha.im);
 		NB_HIP(hipGetLastError());
 		return NB_OK;
@@ -924,7 +972,8 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	policy_degrees(jb.pol, jb.km, nodal ? 1 : 0, degs);
 	for (int d : degs)
 		if ((s = ensure_quad(trial, d)) || (s = ensure_quad(test, d))) return s;
-	if (nodal && ((s = ensure_gal_layout(test)) || (s = ensure_gal_layout(trial)))) return s;
+	const int kt = lt ? 0 : 1, ks = ls ? 0 : 1;   // layout kinds: node or element dofs
+	if (nodal && ((s = ensure_gal_layout(test, kt)) || (s = ensure_gal_layout(trial, ks)))) return s;
 	if (jb.same && (s = ensure_sing_pairs(trial))) return s;
 	if (nodal && !accumulate && (s = zero())) return s;
 
@@ -934,14 +983,14 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	std::memset(&a, 0, sizeof(a));
 	a.ref = &rules->ref[0][0];     // device address (no host access)
 	if (nodal) {
-		a.n_tiles = test->g_tiles;
-		a.chunk = trial->g_chunk;
-		a.t_elem = (const int *)test->g_elem.get();
-		a.t_contrib = (const uint2 *)test->g_contrib.get();
-		a.t_rows = (const int *)test->g_rows.get();
-		a.s_order = (const int *)trial->g_order.get();
-		a.s_col = (const int *)trial->g_col.get();
-		a.s_fin = (const int *)trial->g_fin.get();
+		a.n_tiles = test->glay[kt].tiles;
+		a.chunk = trial->glay[ks].chunk;
+		a.t_elem = (const int *)test->glay[kt].elem.get();
+		a.t_contrib = (const uint2 *)test->glay[kt].contrib.get();
+		a.t_rows = (const int *)test->glay[kt].rows.get();
+		a.s_order = (const int *)trial->glay[ks].order.get();
+		a.s_col = (const int *)trial->glay[ks].col.get();
+		a.s_fin = (const int *)trial->glay[ks].fin.get();
 	} else {
 		a.n_tiles = (test->n + nbk::GAL_TILE - 1) / nbk::GAL_TILE;
 		a.chunk = gal_chunk(a.n_tiles, trial->n);
@@ -1020,7 +1069,8 @@ int run_galerkin(const Job &jb, void *out, int64_t ld, int dtype, int accumulate
 	g.X = (const double *)trial->d_X.get();
 	g.J = (const double *)trial->d_J.get();
 	g.nu = (const double *)trial->d_nu64.get();
-	g.tri = (const int *)trial->d_tri.get();
+	g.t_dof = t_map;
+	g.s_dof = s_map;
 	for (int m = 0; m < 3; ++m) { g.rule[m] = (const double *)sr->r[m].get(); g.rule_n[m] = sr->n[m]; }
 	g.k = jb.k;
 	g.ki = jb.ki;
@@ -2204,7 +2254,8 @@ namespace {
 int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, int formalism, nb_kernel kernel,
 	nb_scalar alpha, const nb_policy *policy, const nb_out *out, hipStream_t st)
 {
-	const bool nodal = formalism == NB_GALERKIN_LINEAR;
+	const bool lt = formalism == NB_GALERKIN_LINEAR || formalism == NB_GALERKIN_LINEAR_CONST;
+	const bool ls = formalism == NB_GALERKIN_LINEAR || formalism == NB_GALERKIN_CONST_LINEAR;
 	const bool ident = kernel.family == NB_IDENTITY;
 	int s;
 	if (!((test->npe == 3 && trial->npe == 3) || (test->npe == 4 && trial->npe == 4)))
@@ -2220,7 +2271,7 @@ int assemble_galerkin(const nb_mesh *test, const nb_mesh *trial, int same_mesh, 
 		if ((s = check_policy(policy))) return s;
 	}
 	const bool cplx = (!ident && kernel.family == NB_HELMHOLTZ) || alpha.im != 0.0;
-	if ((s = check_out(out, nodal ? test->n_nodes : test->n, nodal ? trial->n_nodes : trial->n, cplx))) return s;
+	if ((s = check_out(out, lt ? test->n_nodes : test->n, ls ? trial->n_nodes : trial->n, cplx))) return s;
 	nb_mesh *tr = const_cast<nb_mesh *>(trial);
 	nb_mesh *te = const_cast<nb_mesh *>(test);
 	// the test mesh's tables and layout are built on first use too
@@ -2475,7 +2526,8 @@ int nb_assemble(const nb_mesh *test, const nb_mesh *trial, int32_t same_mesh, in
 	if (!test || !trial) return fail(NB_ERR_ARG, "null mesh");
 	if (same_mesh && test != trial) return fail(NB_ERR_ARG, "same_mesh requires test == trial");
 	if (formalism != NB_COLLOCATION && formalism != NB_GALERKIN && formalism != NB_GALERKIN_LINEAR &&
-		formalism != NB_COLLOCATION_GAUSS && formalism != NB_COLLOCATION_GAUSS_TRIAL)
+		formalism != NB_COLLOCATION_GAUSS && formalism != NB_COLLOCATION_GAUSS_TRIAL &&
+		formalism != NB_GALERKIN_CONST_LINEAR && formalism != NB_GALERKIN_LINEAR_CONST)
 		return fail(NB_ERR_ARG, "unknown formalism");
 	int s;
 	if ((s = current_device_matches(test)) || (s = current_device_matches(trial))) return s;

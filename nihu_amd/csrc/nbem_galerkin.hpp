@@ -504,7 +504,8 @@ struct GalSingArgs {
 	const double *X;             // vertices [9][n]
 	const double *J;             // Jacobians
 	const double *nu;            // unit normals [3][n]
-	const int *tri;              // nodes [3][n]
+	const int *t_dof, *s_dof;    // row / column dof of each element corner [3][n] (nodes,
+	                             // or the element at every corner for a constant field)
 	const double *rule[3];       // canonical rules by match dimension: [points][5]
 	int rule_n[3];
 	double k;
@@ -573,7 +574,7 @@ __global__ void __launch_bounds__(64) galerkin_singular_kernel(GalSingArgs a)
 			if (NDOF == 1) {
 				store_entry64(a.out, (long long)f * a.ld + e, a.dtype, 1, re, im);
 			} else {
-				const int row = a.tri[ct[x3] * n + e], col = a.tri[cs[y3] * n + f];
+				const int row = a.t_dof[ct[x3] * n + e], col = a.s_dof[cs[y3] * n + f];
 				atomic_add_entry64(a.out, (long long)col * a.ld + row, a.dtype, re, im);
 			}
 		}
@@ -581,9 +582,12 @@ __global__ void __launch_bounds__(64) galerkin_singular_kernel(GalSingArgs a)
 
 // identity (mass matrix): constant fields J/2 on the diagonal; nodal fields
 // J/12 (a = b) and J/24 (a != b) per element, the values of the degree-2
-// Dunavant rule of nihu/core/single_integral.hpp:65-140
-__global__ void galerkin_identity_kernel(void *out, long long ld, int n, const double *J, const int *tri, int nodal,
-	int dtype, double ar, double ai)
+// Dunavant rule of nihu/core/single_integral.hpp:65-140; through dof maps
+// that name the element at every corner of a constant side, the sums are
+// J/6, the integral of a linear shape function (exact, as NiHu's degree-1
+// rule for a constant test and a linear trial)
+__global__ void galerkin_identity_kernel(void *out, long long ld, int n, const double *J, const int *t_map,
+	const int *s_map, int nodal, int dtype, double ar, double ai)
 {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= n) return;
@@ -595,7 +599,7 @@ __global__ void galerkin_identity_kernel(void *out, long long ld, int n, const d
 	for (int x = 0; x < 3; ++x)
 		for (int y = 0; y < 3; ++y) {
 			const double m = x == y ? j / 12.0 : j / 24.0;
-			atomic_add_entry64(out, (long long)tri[y * n + e] * ld + tri[x * n + e], dtype, ar * m, ai * m);
+			atomic_add_entry64(out, (long long)s_map[y * n + e] * ld + t_map[x * n + e], dtype, ar * m, ai * m);
 		}
 }
 

@@ -357,7 +357,8 @@ struct GalQSingArgs {
 	const int *pe, *pf, *pinfo;  // test / trial element, match | ind1 << 4 | ind2 << 8
 	int n;                       // elements (SoA stride)
 	const double *X;             // corners [12][n]
-	const int *quad;             // nodes [4][n]
+	const int *t_dof, *s_dof;    // row / column dof of each element corner [4][n] (nodes,
+	                             // or the element at every corner for a constant field)
 	const double *rule[3];       // canonical rules by match dimension: [points][5]
 	int rule_n[3];
 	double k;
@@ -440,7 +441,7 @@ __global__ void __launch_bounds__(64) galq_singular_kernel(GalQSingArgs a)
 			if (ND == 1) {
 				store_entry64(a.out, (long long)f * a.ld + e, a.dtype, 1, re, im);
 			} else {
-				const int row = a.quad[ct[x4] * n + e], col = a.quad[cs[y4] * n + f];
+				const int row = a.t_dof[ct[x4] * n + e], col = a.s_dof[cs[y4] * n + f];
 				atomic_add_entry64(a.out, (long long)col * a.ld + row, a.dtype, re, im);
 			}
 		}
@@ -449,9 +450,12 @@ __global__ void __launch_bounds__(64) galq_singular_kernel(GalQSingArgs a)
 // identity (mass matrix) on quadrilaterals: the Gauss rule of degree
 // shape orders + jacobian order 1 (nihu/core/single_integral.hpp:65-140):
 // constant fields 4 J(0, 0) on the diagonal; nodal fields
-// sum over the 2 x 2 points of N_a N_b J w per element
-__global__ void galq_identity_kernel(void *out, long long ld, int n, const double *X, const int *quad, int nodal,
-	int dtype, double ar, double ai)
+// sum over the 2 x 2 points of N_a N_b J w per element, through the dof maps
+// (a constant side names the element at every corner: the sums over its
+// corners are the 2 x 2 rule of N_b J, NiHu's degree 2 for a constant test
+// and a bilinear trial)
+__global__ void galq_identity_kernel(void *out, long long ld, int n, const double *X, const int *t_map,
+	const int *s_map, int nodal, int dtype, double ar, double ai)
 {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= n) return;
@@ -479,7 +483,7 @@ __global__ void galq_identity_kernel(void *out, long long ld, int n, const doubl
 	}
 	for (int x = 0; x < 4; ++x)
 		for (int y = 0; y < 4; ++y)
-			atomic_add_entry64(out, (long long)quad[y * n + e] * ld + quad[x * n + e], dtype, ar * M[x][y], ai * M[x][y]);
+			atomic_add_entry64(out, (long long)s_map[y * n + e] * ld + t_map[x * n + e], dtype, ar * M[x][y], ai * M[x][y]);
 }
 
 } // namespace nbk

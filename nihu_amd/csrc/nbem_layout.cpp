@@ -68,16 +68,28 @@ Tiles make_tiles(const std::vector<int> &order, const int32_t *tri, int npe, int
 			for (int a = 0; a < npe; ++a) {
 				const int v = tri[npe * e + a];
 				auto it = std::find(rows.begin(), rows.end(), v);
-				if (it == rows.end()) { l3[a] = -1; ++extra; continue; }
+				if (it == rows.end()) {
+					l3[a] = -1;
+					bool repeat = false;
+					for (int b = 0; b < a; ++b) repeat = repeat || tri[npe * e + b] == v;
+					if (!repeat) ++extra;
+					continue;
+				}
 				l3[a] = (int)(it - rows.begin());
 				if ((int)con[l3[a]].size() >= KMAX) full = true;
 			}
 			if (full || (int)rows.size() + extra > rpl * RMAX) break;
 			for (int a = 0; a < npe; ++a) {
 				if (l3[a] < 0) {
-					l3[a] = (int)rows.size();
-					rows.push_back(tri[npe * e + a]);
-					con.emplace_back();
+					// a dof repeated within the element (a constant field's
+					// dof map names the element at every corner) takes its row once
+					auto it = std::find(rows.begin(), rows.end(), tri[npe * e + a]);
+					if (it != rows.end()) l3[a] = (int)(it - rows.begin());
+					else {
+						l3[a] = (int)rows.size();
+						rows.push_back(tri[npe * e + a]);
+						con.emplace_back();
+					}
 				}
 				con[l3[a]].push_back(cnt * 4 + a);
 			}
@@ -121,6 +133,10 @@ int64_t schedule(const std::vector<int> &order, const int32_t *tri, int npe, int
 				for (int s = 0; s < CS; ++s)
 					if (node[s] == tri[npe * e + b]) { slot[b] = s; last[s] = i; }
 			for (int b = 0; b < npe; ++b) {
+				if (slot[b] >= 0) continue;
+				// a dof repeated within the element shares the slot just given to it
+				for (int s = 0; s < CS; ++s)
+					if (last[s] == i && node[s] == tri[npe * e + b]) slot[b] = s;
 				if (slot[b] >= 0) continue;
 				int best = -1;
 				for (int s = 0; s < CS; ++s) {

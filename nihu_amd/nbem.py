@@ -24,13 +24,15 @@ NB_PREC_MIXED, NB_PREC_FP64 = 0, 1
 NB_C64, NB_C128, NB_F32, NB_F64 = 0, 1, 2, 3
 NB_HOST, NB_DEVICE = 0, 1
 NB_COLLOCATION, NB_GALERKIN, NB_GALERKIN_LINEAR = 0, 1, 2
+NB_GALERKIN_CONST_LINEAR, NB_GALERKIN_LINEAR_CONST = 5, 6
 # collocation with the quadrilateral gauss field (4 dofs per element at the
 # 2 x 2 Gauss points, NiHu's quad_1_gauss_field): gauss test field, or the
 # constant test field of another mesh (field points)
 NB_COLLOCATION_GAUSS, NB_COLLOCATION_GAUSS_TRIAL = 3, 4
 NB_TRIA_1_ID, NB_QUAD_1_ID = 32303, 32404   # element type ids of NiHu's tria_1 and quad_1
 FORMALISMS = {"collocation": NB_COLLOCATION, "galerkin": NB_GALERKIN, "galerkin_linear": NB_GALERKIN_LINEAR,
-              "collocation_gauss": NB_COLLOCATION_GAUSS, "collocation_gauss_trial": NB_COLLOCATION_GAUSS_TRIAL}
+              "collocation_gauss": NB_COLLOCATION_GAUSS, "collocation_gauss_trial": NB_COLLOCATION_GAUSS_TRIAL,
+              "galerkin_const_linear": NB_GALERKIN_CONST_LINEAR, "galerkin_linear_const": NB_GALERKIN_LINEAR_CONST}
 
 # symbols declared in include/nbem.h
 EXPORTS = ["nb_last_error", "nb_version", "nb_device_info", "nb_mesh_create", "nb_mesh_create_quads",
@@ -222,12 +224,13 @@ class Mesh:
     def n_nodes(self):
         return len(self.nodes)
 
-    def dofs(self, formalism="collocation"):
-        """rows / columns of this mesh's side of a matrix: nodes for
-        isoparametric Galerkin, 4 per element for the gauss field, elements
-        otherwise"""
+    def dofs(self, formalism="collocation", side="test"):
+        """rows (side "test") or columns (side "trial") of this mesh's side of
+        a matrix: nodes for an isoparametric Galerkin field, 4 per element for
+        the gauss field, elements otherwise"""
         f = FORMALISMS[formalism]
-        if f == NB_GALERKIN_LINEAR:
+        if f == NB_GALERKIN_LINEAR or (f == NB_GALERKIN_CONST_LINEAR and side == "trial") or \
+                (f == NB_GALERKIN_LINEAR_CONST and side == "test"):
             return self.n_nodes
         return 4 * self.n if f == NB_COLLOCATION_GAUSS else self.n
 
@@ -289,8 +292,10 @@ def colmajor(rows, cols, dtype="complex64", device="cuda"):
 def assemble(test: Mesh, trial: Mesh, out, family=NB_HELMHOLTZ, nx=0, ny=0, k=0.0, alpha=1.0, pol=None,
              same_mesh=None, accumulate=True, stream=None, formalism="collocation"):
     """out (+)= alpha * operator; formalism "collocation" (dirac(constant) x
-    constant), "galerkin" (constant x constant) or "galerkin_linear"
-    (isoparametric x isoparametric, rows / columns = mesh nodes).  k may be
+    constant), "galerkin" (constant x constant), "galerkin_linear"
+    (isoparametric x isoparametric, rows / columns = mesh nodes),
+    "galerkin_const_linear" (constant test x isoparametric trial) or
+    "galerkin_linear_const" (the reverse).  k may be
     complex (collocation on every mesh type, Galerkin on meshes of triangles
     or of quadrilaterals)."""
     pol = pol or policy(**NIHU_POLICY)

@@ -272,3 +272,46 @@ def test_quad_wave_tiles_and_slot_schedule(raw):
                 if v >= 0:
                     flushed[v] = flushed.get(v, 0) + 1
         assert set(flushed) == set(range(len(nodes)))
+
+
+@pytest.mark.parametrize("npe", [3, 4])
+def test_element_dof_map_layout(raw, npe):
+    """The dof map of a constant field in a pair with a nodal one names the
+    element at every corner: each local row is one element with its npe
+    corners as contributors, and every corner of a trial element shares one
+    column slot (one eviction per element)."""
+    if npe == 3:
+        nodes, el = _mesh(3)
+    else:
+        nodes, el = _qmesh(6)
+    n = len(el)
+    dm = np.ascontiguousarray(np.repeat(np.arange(n, dtype=np.int32)[:, None], npe, axis=1))
+    c = np.ascontiguousarray(nodes[el].mean(1).T)
+    cs, rpl, chunk = (8, 1, 128) if npe == 3 else (QCS, 2, 64)
+    tiles = raw.nbg_galerkin_layout_n(n, _i(dm), npe, cs, c.ctypes.data_as(D), chunk, None, None, None, None,
+                                      None, None)
+    order = np.zeros(n, np.int32)
+    elem = np.zeros(tiles * TILE, np.int32)
+    rows = np.zeros(tiles * TILE * rpl, np.int32)
+    contrib = np.zeros(tiles * TILE * rpl * 2, np.uint32)
+    col = np.zeros((npe + 1) * n, np.int32)
+    fin = np.zeros(((n + chunk - 1) // chunk) * cs, np.int32)
+    assert raw.nbg_galerkin_layout_n(n, _i(dm), npe, cs, c.ctypes.data_as(D), chunk, _i(order), _i(elem),
+                                     contrib.ctypes.data_as(I32), _i(rows), _i(col), _i(fin)) == tiles
+    contrib = contrib.reshape(-1, 2)
+    col = col.reshape(n, npe + 1)
+    assert tiles == (n + (TILE - (npe == 4)) - 1) // (TILE - (npe == 4))
+    for t in range(tiles):
+        for l in range(TILE):
+            e = elem[t * TILE + l]
+            if e < 0:
+                continue
+            # the lane's element is one local row with its npe corners
+            r = [j for j in range(rpl * TILE) if rows[t * TILE * rpl + j] == e]
+            assert len(r) == 1
+            bytes8 = [(int(contrib[t * TILE * rpl + r[0]][q >> 2]) >> (8 * (q & 3))) & 255 for q in range(8)]
+            assert sorted(b for b in bytes8 if b != 255) == [l * 4 + a for a in range(npe)]
+    for i in range(n):
+        sl = [(col[i, npe] >> (5 * b)) & 31 for b in range(npe)]
+        assert len(set(sl)) == 1
+        assert (col[i, 1:npe] == -1).all()

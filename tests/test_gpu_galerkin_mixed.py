@@ -1,0 +1,89 @@
+"""Galerkin pairs with mixed field types through the C ABI, against the FP64
+oracle: constant test x isoparametric trial ("galerkin_const_linear", rows =
+elements, columns = nodes) and the reverse ("galerkin_linear_const"), on
+triangle and quadrilateral meshes.  The engine runs the nodal kernels with the
+constant side's dof map naming the element at every corner; the bar is that of
+the other Galerkin matrices: rows within 2e-6 of their largest entry, 1e-5
+relative Frobenius, the mass matrix to 1e-14.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROW_TOL = 2e-6
+FRO_TOL = 1e-5
+CODES = {"galerkin_const_linear": 2, "galerkin_linear_const": 3}
+NIHU = (dict(rule="nihu", near="mean_edge", limit=1.5), dict(rule=0, fixed_degree=0, near_mode=0, near_limit=1.5))
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    from nihu_amd import nbem
+    return nbem
+
+
+@pytest.fixture(scope="module", params=["tri", "quad"])
+def mesh(request):
+    from nihu_amd.mesh import jittered_icosphere, jittered_cubed_sphere
+    if request.param == "tri":
+        return request.param, jittered_icosphere(2, seed=4)
+    return request.param, jittered_cubed_sphere(4, seed=4)
+
+
+def rowscaled(got, ref):
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    scale[scale == 0] = 1.0
+    return (np.abs(got - ref) / scale).max()
+
+
+def relfro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _assemble(nb, m, form, fam, nx, ny, k, alpha, gp, dtype="complex128"):
+    import torch
+    out = nb.colmajor(m.dofs(form, "test"), m.dofs(form, "trial"), dtype)
+    nb.assemble(m, m, out, fam, nx, ny, k, alpha=alpha, pol=nb.policy(**gp), accumulate=False, formalism=form)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("form", ["galerkin_const_linear", "galerkin_linear_const"])
+@pytest.mark.parametrize("fam,k", [(0, 0.0), (1, 3.0), (1, complex(3.0, -0.5))])
+@pytest.mark.parametrize("nxy", [(0, 0), (0, 1), (1, 0)])
+def test_mixed_operator(nb, orc, mesh, form, fam, k, nxy):
+    kind, (nodes, el) = mesh
+    m = nb.Mesh(nodes, el)
+    print("case", kind, form, fam, k, nxy, flush=True)
+    got = _assemble(nb, m, form, fam, nxy[0], nxy[1], k, 0.8 + 0.25j, NIHU[0])
+    ref = orc.assemble_galerkin(nodes, el, fam, nxy[0], nxy[1], k=k, alpha=0.8 + 0.25j, pol=orc.policy(**NIHU[1]),
+                                linear=CODES[form])
+    assert got.shape == ref.shape
+    rs, rf = rowscaled(got, ref), relfro(got, ref)
+    print("row-scaled %.2e  rel fro %.2e" % (rs, rf), flush=True)
+    assert rs <= ROW_TOL and rf <= FRO_TOL
+
+
+@pytest.mark.parametrize("form", ["galerkin_const_linear", "galerkin_linear_const"])
+def test_mixed_mass_matrix(nb, orc, mesh, form):
+    kind, (nodes, el) = mesh
+    m = nb.Mesh(nodes, el)
+    got = _assemble(nb, m, form, nb.NB_IDENTITY, 0, 0, 0.0, -0.5, NIHU[0])
+    ref = orc.assemble_galerkin(nodes, el, 2, 0, 0, alpha=-0.5, linear=CODES[form])
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_mixed_then_plain_layouts(nb, orc, mesh):
+    """The node-dof and element-dof layouts of one mesh live side by side:
+    a mixed assembly, then the plain nodal one, then the mixed one again, give
+    the oracle's matrices each time."""
+    kind, (nodes, el) = mesh
+    m = nb.Mesh(nodes, el)
+    for form, code in (("galerkin_const_linear", 2), ("galerkin_linear", 1), ("galerkin_const_linear", 2)):
+        got = _assemble(nb, m, form, 1, 0, 0, 2.0, 1.0, NIHU[0])
+        ref = orc.assemble_galerkin(nodes, el, 1, 0, 0, k=2.0, pol=orc.policy(**NIHU[1]), linear=code)
+        assert rowscaled(got, ref) <= ROW_TOL and relfro(got, ref) <= FRO_TOL
