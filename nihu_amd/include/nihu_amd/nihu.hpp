@@ -18,9 +18,9 @@
 // std::complex<double>, std::complex<float>, double or float, on the host, or
 // a nihu_amd::device_matrix on the GPU.  Implemented formalisms: collocation
 // (dirac(constant_view) test, constant_view trial; the quadrilateral gauss
-// field) and Galerkin (constant_view x constant_view, isoparametric_view x
-// isoparametric_view, dofs = elements or mesh nodes, on meshes of triangles or
-// of quadrilaterals); other test / trial combinations throw std::logic_error.  Errors map to NiHu's exception types:
+// field) and Galerkin (constant_view or isoparametric_view on either side,
+// dofs = elements or mesh nodes, on meshes of triangles or of
+// quadrilaterals); other test / trial combinations throw std::logic_error.  Errors map to NiHu's exception types:
 // std::out_of_range (quadrature degree), std::invalid_argument,
 // std::logic_error, std::bad_alloc and std::runtime_error.
 
@@ -415,6 +415,8 @@ inline int formalism_of(int test_kind, int trial_kind)
 	if (test_kind == dirac_constant_kind && trial_kind == constant_kind) return NB_COLLOCATION;
 	if (test_kind == constant_kind && trial_kind == constant_kind) return NB_GALERKIN;
 	if (test_kind == isoparametric_kind && trial_kind == isoparametric_kind) return NB_GALERKIN_LINEAR;
+	if (test_kind == constant_kind && trial_kind == isoparametric_kind) return NB_GALERKIN_CONST_LINEAR;
+	if (test_kind == isoparametric_kind && trial_kind == constant_kind) return NB_GALERKIN_LINEAR_CONST;
 	if (test_kind == dirac_gauss_kind && trial_kind == gauss_kind) return NB_COLLOCATION_GAUSS;
 	if (test_kind == dirac_constant_kind && trial_kind == gauss_kind) return NB_COLLOCATION_GAUSS_TRIAL;
 	return -1;
@@ -430,8 +432,8 @@ public:
 	weighted_residual(space_ref const &test, integral_transform const &tr)
 	{
 		if (formalism_of(test.kind, tr.trial().kind) < 0)
-			throw std::logic_error("test and trial spaces: dirac(constant) x constant, constant x constant, "
-				"isoparametric x isoparametric, dirac(gauss) x gauss or dirac(constant) x gauss");
+			throw std::logic_error("test and trial spaces: dirac(constant) x constant, constant or isoparametric "
+				"x constant or isoparametric, dirac(gauss) x gauss or dirac(constant) x gauss");
 		m_terms.push_back(term{tr, test});
 	}
 	std::vector<term> const &terms() const { return m_terms; }

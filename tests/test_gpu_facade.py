@@ -41,5 +41,5 @@ def test_cpp_facade_known_answers(tmp_path):
                  "Pulsating sphere relative error (quadrilateral gauss field)",
                  "Gauss-field Burton-Miller composition", "gauss dof numbering: invalid_argument as expected",
                  "Mixed-mesh collocation transparent error", "Mixed-mesh Burton-Miller composition",
-                 "mixed-mesh Galerkin: logic_error as expected"):
+                 "mixed-mesh Galerkin: logic_error as expected", "Mixed-field Galerkin: Gauss identity"):
         assert line in r.stdout
