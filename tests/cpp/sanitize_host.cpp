@@ -31,6 +31,11 @@ int nbg_singular_rule(int match, double *out, int cap);
 int64_t nbg_singular_pairs(int n_nodes, int n, const int32_t *tri, int32_t *pe, int32_t *pf, int32_t *info);
 int nbg_galerkin_layout(int n, const int32_t *tri, const double *c, int chunk, int32_t *order_out, int32_t *elem,
 	uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin);
+int nbg_quad_singular_rule(int match, double *out, int cap);
+int64_t nbg_singular_pairs_n(int n_nodes, int n, const int32_t *el, int npe, int32_t *pe, int32_t *pf, int32_t *info);
+int nbg_galerkin_layout_n(int n, const int32_t *tri, int npe, int cs, const double *c, int chunk,
+	int32_t *order_out, int32_t *elem, uint32_t *contrib, int32_t *rows, int32_t *col, int32_t *fin);
+int nbg_quad_gal_table(int N, int n, const double *c, const double *X, float *out);
 #ifdef WITH_ORACLE
 typedef struct { int rule; int fixed_degree; int near_mode; double near_limit; } orc_policy;
 int orc_assemble_bm(const double *xyz, int nn, const int *tri, int nt, double k, double alpha_re, double alpha_im,
@@ -155,6 +160,36 @@ std::vector<unsigned char> run_host(const std::vector<double> &xyz, const std::v
 	std::vector<double> du(3 * 64);
 	const int nd = nbg_quad_duffy(4, 0.1, -0.2, du.data());
 	put(du.data(), (size_t)3 * nd * 8);
+	// Galerkin on quadrilaterals: rules, node-sharing pairs, point rows, and
+	// the layouts of the node dofs and of the element dofs (every corner the
+	// element itself)
+	for (int m = 0; m < 3; ++m) {
+		std::vector<double> r(5 * 2048);
+		const int k = nbg_quad_singular_rule(m, r.data(), 2048);
+		put(r.data(), (size_t)5 * k * 8);
+	}
+	const int64_t nqp = nbg_singular_pairs_n(nqn, nq, quad.data(), 4, nullptr, nullptr, nullptr);
+	std::vector<int32_t> qpe(nqp), qpf(nqp), qinfo(nqp);
+	nbg_singular_pairs_n(nqn, nq, quad.data(), 4, qpe.data(), qpf.data(), qinfo.data());
+	put(qinfo.data(), nqp * 4);
+	std::vector<float> gq((size_t)nq * 4 * 16);
+	nbg_quad_gal_table(2, nq, qc.data(), qX.data(), gq.data());
+	put(gq.data(), gq.size() * 4);
+	std::vector<int32_t> edof(4 * (size_t)nq);
+	for (int e = 0; e < nq; ++e)
+		for (int a = 0; a < 4; ++a) edof[4 * e + a] = e;
+	const std::vector<int32_t> *maps[2] = {&quad, &edof};
+	for (const std::vector<int32_t> *dm : maps) {
+		const int qt_tiles = nbg_galerkin_layout_n(nq, dm->data(), 4, 6, qc.data(), 2, nullptr, nullptr, nullptr,
+			nullptr, nullptr, nullptr);
+		std::vector<int32_t> qo(nq), qe((size_t)qt_tiles * 64), qr((size_t)qt_tiles * 128), qcol(5 * (size_t)nq),
+			qfin((size_t)((nq + 1) / 2) * 6);
+		std::vector<uint32_t> qcon((size_t)qt_tiles * 256);
+		nbg_galerkin_layout_n(nq, dm->data(), 4, 6, qc.data(), 2, qo.data(), qe.data(), qcon.data(), qr.data(),
+			qcol.data(), qfin.data());
+		put(qr.data(), qr.size() * 4);
+		put(qcol.data(), qcol.size() * 4);
+	}
 	return out;
 }
 
