@@ -87,3 +87,25 @@ def test_mixed_then_plain_layouts(nb, orc, mesh):
         got = _assemble(nb, m, form, 1, 0, 0, 2.0, 1.0, NIHU[0])
         ref = orc.assemble_galerkin(nodes, el, 1, 0, 0, k=2.0, pol=orc.policy(**NIHU[1]), linear=code)
         assert rowscaled(got, ref) <= ROW_TOL and relfro(got, ref) <= FRO_TOL
+
+
+@pytest.mark.parametrize("form", ["galerkin_const_linear", "galerkin_linear_const"])
+@pytest.mark.parametrize("nxy", [(0, 0), (1, 1)])
+def test_mixed_other_mesh(nb, orc, mesh, form, nxy):
+    """Different test and trial meshes of one element type: no singular
+    check, so the hypersingular kernel assembles too; each mesh uses the
+    layout of its own side's dof kind."""
+    import torch
+    from nihu_amd.mesh import icosphere, cubed_sphere
+    kind, (nodes, el) = mesh
+    fn, fe = icosphere(1) if kind == "tri" else cubed_sphere(2)
+    fn = fn * 1.7 + np.array([0.2, -0.1, 0.3])
+    mt, ms = nb.Mesh(nodes, el), nb.Mesh(fn, fe)
+    out = nb.colmajor(mt.dofs(form, "test"), ms.dofs(form, "trial"), "complex128")
+    nb.assemble(mt, ms, out, 1, nxy[0], nxy[1], 2.5, pol=nb.policy(**NIHU[0]), accumulate=False, formalism=form)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ref = orc.assemble_galerkin(nodes, el, 1, nxy[0], nxy[1], k=2.5, pol=orc.policy(**NIHU[1]), trial=(fn, fe),
+                                linear=CODES[form])
+    assert got.shape == ref.shape
+    assert rowscaled(got, ref) <= ROW_TOL and relfro(got, ref) <= FRO_TOL
